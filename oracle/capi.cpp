@@ -1,0 +1,278 @@
+// capi.cpp -- TEST INFRASTRUCTURE ONLY (see oracle.h).
+// C entry points used by tests/ (ctypes) and by bench.py's CPU baseline leg,
+// plus the multilevel pipeline of partitioner.cpp:163-204 built from the
+// restated pieces.
+#include "oracle.h"
+#include "../mt-kahypar_amd/host/initial_partitioning.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+using namespace orc;
+
+namespace {
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+mt_kahypar_amd::HostHypergraph to_host(const Hypergraph& hg) {
+  mt_kahypar_amd::HostHypergraph h;
+  h.n = hg.n; h.m = hg.m; h.noff = hg.noff; h.pins = hg.pins; h.voff = hg.voff; h.inc = hg.inc;
+  h.vw = hg.vw; h.nw = hg.nw;
+  return h;
+}
+}  // namespace
+
+extern "C" {
+
+void* orc_hg_new(uint32_t n, uint32_t m, const uint64_t* noff, const uint32_t* pins, const int32_t* vw,
+                 const int32_t* nw) {
+  return new Hypergraph(build_hypergraph(n, m, noff, pins, vw, nw));
+}
+void orc_hg_free(void* h) { delete static_cast<Hypergraph*>(h); }
+void orc_hg_set_comm(void* h, const int32_t* comm) {
+  auto* hg = static_cast<Hypergraph*>(h);
+  std::copy(comm, comm + hg->n, hg->comm.begin());
+}
+void orc_hg_sizes(void* h, uint32_t* n, uint32_t* m, uint64_t* p, uint32_t* max_esize, int64_t* total_w) {
+  auto* hg = static_cast<Hypergraph*>(h);
+  *n = hg->n; *m = hg->m; *p = hg->P(); *max_esize = hg->max_esize; *total_w = hg->total_w;
+}
+void orc_hg_get(void* h, uint64_t* noff, uint32_t* pins, uint64_t* voff, uint32_t* inc, int32_t* vw,
+                int32_t* nw, int32_t* comm) {
+  auto* hg = static_cast<Hypergraph*>(h);
+  std::copy(hg->noff.begin(), hg->noff.end(), noff);
+  std::copy(hg->pins.begin(), hg->pins.end(), pins);
+  std::copy(hg->voff.begin(), hg->voff.end(), voff);
+  std::copy(hg->inc.begin(), hg->inc.end(), inc);
+  std::copy(hg->vw.begin(), hg->vw.end(), vw);
+  std::copy(hg->nw.begin(), hg->nw.end(), nw);
+  std::copy(hg->comm.begin(), hg->comm.end(), comm);
+}
+
+// ---------------------------------------------------------- partition --
+int64_t orc_phg_init(void* h, int k, const int32_t* part, int64_t* part_w, uint32_t* pc, uint64_t* conn) {
+  auto* hg = static_cast<Hypergraph*>(h);
+  Partition p;
+  phg_init(p, *hg, k, part);
+  if (part_w) std::copy(p.part_w.begin(), p.part_w.end(), part_w);
+  if (pc) std::copy(p.pc.begin(), p.pc.end(), pc);
+  if (conn) std::copy(p.conn.begin(), p.conn.end(), conn);
+  return km1(p);
+}
+int64_t orc_cut(void* h, int k, const int32_t* part) {
+  Partition p;
+  phg_init(p, *static_cast<Hypergraph*>(h), k, part);
+  return cut(p);
+}
+void orc_border(void* h, int k, const int32_t* part, uint8_t* out) {
+  auto* hg = static_cast<Hypergraph*>(h);
+  Partition p;
+  phg_init(p, *hg, k, part);
+  for (uint32_t u = 0; u < hg->n; ++u) out[u] = is_border(p, u);
+}
+void orc_max_gain_move(void* h, int k, const int32_t* part, uint32_t u, const int64_t* max_w, int32_t* to,
+                       int32_t* gain) {
+  Partition p;
+  phg_init(p, *static_cast<Hypergraph*>(h), k, part);
+  km1_max_gain_move(p, u, std::vector<int64_t>(max_w, max_w + k), to, gain);
+}
+// stats: delta, moves, reverts, visits, rounds
+int64_t orc_lp_refine(void* h, int k, int32_t* part, const int64_t* max_w, const int64_t* perfect_w,
+                      uint64_t seed, int max_rounds, int sub_rounds, int sync, int threads, int disable_rand,
+                      int64_t* stats) {
+  auto* hg = static_cast<Hypergraph*>(h);
+  Partition p;
+  phg_init(p, *hg, k, part);
+  LpParams prm;
+  prm.k = k;
+  prm.max_w.assign(max_w, max_w + k);
+  prm.perfect_w.assign(perfect_w, perfect_w + k);
+  prm.seed = seed;
+  prm.max_rounds = max_rounds;
+  prm.sub_rounds = sub_rounds;
+  prm.threads = threads;
+  prm.disable_randomization = disable_rand != 0;
+  LpStats st = sync ? lp_refine_sync(p, prm) : lp_refine_ref(p, prm);
+  std::copy(p.part.begin(), p.part.end(), part);
+  if (stats) { stats[0] = st.delta; stats[1] = st.moves; stats[2] = st.reverts; stats[3] = st.visits; stats[4] = st.rounds; }
+  return km1(p);
+}
+
+// ------------------------------------------------------------- graph --
+void* orc_graph_new(void* h, int wt) {
+  return new Graph(build_graph(*static_cast<Hypergraph*>(h), EdgeWeight(wt)));
+}
+void orc_graph_free(void* g) { delete static_cast<Graph*>(g); }
+void orc_graph_sizes(void* gp, uint32_t* n, uint64_t* arcs, double* tv, uint64_t* max_degree) {
+  auto* g = static_cast<Graph*>(gp);
+  *n = g->n; *arcs = g->idx[g->n]; *tv = g->tv; *max_degree = g->max_degree;
+}
+void orc_graph_get(void* gp, uint64_t* idx, uint32_t* head, double* w, double* vol) {
+  auto* g = static_cast<Graph*>(gp);
+  std::copy(g->idx.begin(), g->idx.end(), idx);
+  std::copy(g->head.begin(), g->head.end(), head);
+  std::copy(g->w.begin(), g->w.end(), w);
+  std::copy(g->vol.begin(), g->vol.end(), vol);
+}
+void* orc_graph_contract(void* gp, int32_t* C) {
+  auto* g = static_cast<Graph*>(gp);
+  std::vector<int32_t> c(C, C + g->n);
+  Graph* cg = new Graph(contract_graph(*g, c));
+  std::copy(c.begin(), c.end(), C);
+  return cg;
+}
+double orc_modularity(void* gp, const int32_t* C) {
+  auto* g = static_cast<Graph*>(gp);
+  return modularity(*g, std::vector<int32_t>(C, C + g->n));
+}
+// cluster volumes are initialised from C as in initializeClusterVolumes
+int32_t orc_louvain_best_cluster(void* gp, const int32_t* C, uint32_t u, uint64_t sampling, double* best_gain) {
+  auto* g = static_cast<Graph*>(gp);
+  std::vector<int32_t> c(C, C + g->n);
+  std::vector<double> cvol(g->n, 0.0);
+  for (uint32_t x = 0; x < g->n; ++x) cvol[c[x]] += g->vol[x];
+  return louvain_best_cluster(*g, c, cvol, u, sampling, best_gain);
+}
+// stats: arcs_swept, rounds, levels, moves
+void orc_louvain(void* gp, int sync, uint64_t seed, int max_passes, double min_frac, uint64_t sampling,
+                 int sub_rounds, int threads, int disable_rand, int32_t* C_out, int64_t* stats) {
+  auto* g = static_cast<Graph*>(gp);
+  LouvainParams prm;
+  prm.seed = seed; prm.max_passes = max_passes; prm.min_move_fraction = min_frac; prm.sampling = sampling;
+  prm.sub_rounds = sub_rounds; prm.threads = threads; prm.disable_randomization = disable_rand != 0;
+  LouvainStats st;
+  std::vector<int32_t> C = sync ? louvain_sync(*g, prm, &st) : louvain_ref(*g, prm, &st);
+  std::copy(C.begin(), C.end(), C_out);
+  if (stats) { stats[0] = st.arcs_swept; stats[1] = st.rounds; stats[2] = st.levels; stats[3] = st.moves; }
+}
+int orc_louvain_level_sync(void* gp, uint64_t seed, int max_passes, double min_frac, uint64_t sampling,
+                           int sub_rounds, int32_t* C_out, int64_t* stats) {
+  auto* g = static_cast<Graph*>(gp);
+  LouvainParams prm;
+  prm.seed = seed; prm.max_passes = max_passes; prm.min_move_fraction = min_frac; prm.sampling = sampling;
+  prm.sub_rounds = sub_rounds;
+  LouvainStats st;
+  std::vector<int32_t> C;
+  const bool changed = louvain_local_moving_sync(*g, C, prm, &st);
+  std::copy(C.begin(), C.end(), C_out);
+  if (stats) { stats[0] = st.arcs_swept; stats[1] = st.rounds; stats[2] = st.levels; stats[3] = st.moves; }
+  return changed ? 1 : 0;
+}
+
+// --------------------------------------------------------- coarsening --
+static CoarsenParams coarsen_params(uint32_t contraction_limit, int32_t max_node_weight, double min_shrink,
+                                    double max_shrink, uint32_t ignore, uint64_t sampling, uint64_t seed,
+                                    int sub_rounds, int disable_rand, int prefer_unmatched, int threads) {
+  CoarsenParams p;
+  p.contraction_limit = contraction_limit; p.max_node_weight = max_node_weight; p.min_shrink = min_shrink;
+  p.max_shrink = max_shrink; p.ignore_he_size = ignore; p.sampling = sampling; p.seed = seed;
+  p.sub_rounds = sub_rounds; p.disable_randomization = disable_rand != 0;
+  p.prefer_unmatched = prefer_unmatched != 0; p.threads = threads;
+  return p;
+}
+uint32_t orc_cluster_pass(void* h, int sync, int pass, uint32_t contraction_limit, int32_t max_node_weight,
+                          double max_shrink, uint32_t ignore, uint64_t sampling, uint64_t seed, int sub_rounds,
+                          int disable_rand, int prefer_unmatched, uint32_t* cluster_out, int64_t* pin_visits) {
+  auto* hg = static_cast<Hypergraph*>(h);
+  CoarsenParams p = coarsen_params(contraction_limit, max_node_weight, 1.01, max_shrink, ignore, sampling, seed,
+                                   sub_rounds, disable_rand, prefer_unmatched, 1);
+  uint32_t after = 0;
+  std::vector<uint32_t> c = sync ? cluster_pass_sync(*hg, p, pass, &after, pin_visits)
+                                 : cluster_pass_ref(*hg, p, pass, &after);
+  std::copy(c.begin(), c.end(), cluster_out);
+  return after;
+}
+void* orc_hg_contract(void* h, const uint32_t* cluster, uint32_t* cmap) {
+  auto* hg = static_cast<Hypergraph*>(h);
+  std::vector<uint32_t> cl(cluster, cluster + hg->n), cm;
+  Hypergraph* c = new Hypergraph(contract_hypergraph(*hg, cl, cm));
+  std::copy(cm.begin(), cm.end(), cmap);
+  return c;
+}
+// Runs the coarsening driver and returns the number of levels; the coarsest
+// hypergraph is returned as a new handle in *coarsest (or null).
+int orc_coarsen(void* h, int sync, uint32_t contraction_limit, int32_t max_node_weight, double min_shrink,
+                double max_shrink, uint32_t ignore, uint64_t sampling, uint64_t seed, int sub_rounds,
+                int disable_rand, int prefer_unmatched, void** coarsest) {
+  auto* hg = static_cast<Hypergraph*>(h);
+  CoarsenParams p = coarsen_params(contraction_limit, max_node_weight, min_shrink, max_shrink, ignore, sampling,
+                                   seed, sub_rounds, disable_rand, prefer_unmatched, 1);
+  std::vector<Level> lv = coarsen(*hg, p, sync != 0, nullptr);
+  *coarsest = lv.empty() ? nullptr : new Hypergraph(lv.back().hg);
+  return int(lv.size());
+}
+
+// ----------------------------------------------------------- pipeline --
+// partition (partitioner.cpp:163-204) with the fast preset: community
+// detection, multilevel coarsening, initial partitioning on the coarsest
+// hypergraph, uncoarsening with LP at every level (FM disabled).
+// times: louvain, coarsening, initial partitioning, refinement, total
+// counters: louvain arcs swept, lp incidences visited, levels, rating pin visits
+int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int threads, int32_t* part_out,
+                      double* times, int64_t* counters) {
+  const double t0 = now();
+  Hypergraph hg = *static_cast<Hypergraph*>(h);
+  // setupPartWeights / setupContractionLimit / setupMaximumAllowedNodeWeight
+  // (context.cpp:213-284)
+  const int64_t W = hg.total_w;
+  const int64_t perfect = int64_t(std::ceil(double(W) / double(k)));
+  const int64_t maxw = int64_t(int32_t((1.0 + eps) * double(perfect)));
+  const uint32_t climit = 160u * uint32_t(k);
+  int32_t max_node_w = int32_t(std::ceil(1.0 / double(climit) * double(W)));
+  max_node_w = int32_t(std::min<int64_t>(max_node_w, maxw));
+  // preprocess (partitioner.cpp:62-72,106-126)
+  const double density = double(hg.m) / double(hg.n);
+  const EdgeWeight wt = density < 0.75 ? EdgeWeight::degree : EdgeWeight::uniform;
+  LouvainParams lp;
+  lp.seed = seed; lp.threads = threads;
+  LouvainStats lst;
+  {
+    Graph g = build_graph(hg, wt);
+    std::vector<int32_t> C = sync ? louvain_sync(g, lp, &lst) : louvain_ref(g, lp, &lst);
+    for (uint32_t u = 0; u < hg.n; ++u) hg.comm[u] = C[u];
+  }
+  const double t1 = now();
+  CoarsenParams cp;
+  cp.contraction_limit = climit; cp.max_node_weight = max_node_w; cp.seed = seed; cp.threads = threads;
+  int64_t rating_visits = 0;
+  std::vector<Level> levels = coarsen(hg, cp, sync != 0, &rating_visits);
+  const double t2 = now();
+  const Hypergraph& coarsest = levels.empty() ? hg : levels.back().hg;
+  mt_kahypar_amd::InitialPartitioningParams ip;
+  ip.k = k; ip.max_w.assign(k, maxw); ip.seed = seed;
+  std::vector<int32_t> part = mt_kahypar_amd::initial_partition(to_host(coarsest), ip, nullptr);
+  const double t3 = now();
+  LpParams prm;
+  prm.k = k; prm.max_w.assign(k, maxw); prm.perfect_w.assign(k, perfect); prm.seed = seed; prm.threads = threads;
+  int64_t lp_visits = 0;
+  auto refine = [&](const Hypergraph& g, std::vector<int32_t>& pt, int level) {
+    Partition p;
+    phg_init(p, g, k, pt.data());
+    prm.seed = seed + uint64_t(level) * 1000003ull;
+    LpStats st = sync ? lp_refine_sync(p, prm) : lp_refine_ref(p, prm);
+    lp_visits += st.visits;
+    pt = p.part;
+  };
+  refine(coarsest, part, int(levels.size()));
+  for (int i = int(levels.size()) - 1; i >= 0; --i) {
+    const Hypergraph& fine = (i == 0) ? hg : levels[i - 1].hg;
+    std::vector<int32_t> fp(fine.n);
+    for (uint32_t u = 0; u < fine.n; ++u) fp[u] = part[levels[i].cmap[u]];
+    part.swap(fp);
+    refine(fine, part, i);
+  }
+  const double t4 = now();
+  Partition fin;
+  phg_init(fin, hg, k, part.data());
+  std::copy(part.begin(), part.end(), part_out);
+  if (times) { times[0] = t1 - t0; times[1] = t2 - t1; times[2] = t3 - t2; times[3] = t4 - t3; times[4] = t4 - t0; }
+  if (counters) {
+    counters[0] = lst.arcs_swept; counters[1] = lp_visits; counters[2] = int64_t(levels.size());
+    counters[3] = rating_visits;
+  }
+  return km1(fin);
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+// oracle.h -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of Mt-KaHyPar's data-parallel core (Louvain local moving on
+// the star expansion, community-restricted heavy-edge clustering and
+// contraction, km1 label propagation over the partitioned hypergraph).  It is
+// the checker for the HIP implementation under mt-kahypar_amd/ and the timed
+// CPU baseline; nothing in the product links or calls it.
+//
+// Two semantics are restated:
+//   * reference semantics ("ref"): the asynchronous algorithms as written in
+//     the reference (moves visible immediately, atomics, per-net spin locks,
+//     insertion-ordered sparse maps, coin flips from per-thread generators).
+//     With one thread and randomization disabled this is the deterministic
+//     Gauss-Seidel order the reference's own tests pin (karate club, gain
+//     vectors, coarsener chain).
+//   * synchronous semantics ("sync"): every round/sub-round computes on the
+//     state at its start and then applies; ties are broken by a stateless
+//     hash of (seed, round, vertex, candidate).  The device kernels implement
+//     exactly this, so integer state is compared bit for bit.
+//
+// Reference files are cited as path:line relative to the reference root.
+#pragma once
+#include <cstdint>
+#include <cstddef>
+#include <vector>
+#include <atomic>
+
+namespace orc {
+
+// ---------------------------------------------------------------- hashing --
+// SplitMix64 finalizer (Steele, Lea, Flood: "Fast splittable pseudorandom
+// number generators", OOPSLA 2014).  The same function is the tie-breaking
+// specification of the synchronous semantics.
+inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline uint64_t hash4(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
+  uint64_t h = mix64(seed ^ (a * 0xD6E8FEB86659FD93ull));
+  h = mix64(h ^ (b * 0xA0761D6478BD642Full));
+  return mix64(h ^ (c * 0xE7037ED1A0B428DBull));
+}
+
+// ------------------------------------------------------------- hypergraph --
+// Static CSR hypergraph (static_hypergraph.h:44); incident nets ascending,
+// which is the order static_hypergraph_factory.cpp:124-156 produces when
+// stable construction is requested.
+struct Hypergraph {
+  uint32_t n = 0, m = 0;
+  std::vector<uint64_t> noff;   // m+1
+  std::vector<uint32_t> pins;   // P
+  std::vector<uint64_t> voff;   // n+1
+  std::vector<uint32_t> inc;    // P
+  std::vector<int32_t> vw;      // n
+  std::vector<int32_t> nw;      // m
+  std::vector<int32_t> comm;    // n (community ids, 0 if none)
+  uint32_t max_esize = 0;
+  int64_t total_w = 0;
+  uint64_t P() const { return pins.size(); }
+  uint32_t esize(uint32_t e) const { return uint32_t(noff[e + 1] - noff[e]); }
+  uint32_t degree(uint32_t u) const { return uint32_t(voff[u + 1] - voff[u]); }
+};
+Hypergraph build_hypergraph(uint32_t n, uint32_t m, const uint64_t* noff, const uint32_t* pins,
+                            const int32_t* vw, const int32_t* nw);
+void build_incidence(Hypergraph& hg);
+
+// ------------------------------------------------------ partition state --
+// PartitionedHypergraph (partitioned_hypergraph.h:46).  Pin counts are kept
+// unpacked (values are what is compared; the packing of pin_count_in_part.h
+// is a storage detail).
+struct Partition {
+  const Hypergraph* hg = nullptr;
+  int k = 0;
+  std::vector<int32_t> part;
+  std::vector<int64_t> part_w;     // atomically updated in ref mode
+  std::vector<uint32_t> pc;        // m*k
+  std::vector<uint64_t> conn;      // m*cw
+  int cw = 1;
+  uint32_t pcount(uint32_t e, int b) const { return pc[size_t(e) * k + b]; }
+  bool has(uint32_t e, int b) const { return (conn[size_t(e) * cw + b / 64] >> (b % 64)) & 1ull; }
+  int lambda(uint32_t e) const;
+};
+void phg_init(Partition& p, const Hypergraph& hg, int k, const int32_t* part);
+int64_t km1(const Partition& p);
+int64_t cut(const Partition& p);
+bool is_border(const Partition& p, uint32_t u);
+
+struct LpParams {
+  int k = 2;
+  std::vector<int64_t> max_w, perfect_w;
+  int max_rounds = 5;                 // fast_preset.ini:58
+  uint32_t activation_threshold = 100;// fast_preset.ini:60
+  bool rebalancing = true;            // fast_preset.ini:59
+  uint64_t seed = 0;
+  int sub_rounds = 4;                 // sync only
+  bool disable_randomization = false; // ref only
+  int threads = 1;                    // ref only
+};
+struct LpStats {
+  int64_t delta = 0;       // km1 change
+  int64_t moves = 0;       // kept moves
+  int64_t reverts = 0;     // moves reverted by the real-delta check
+  int64_t visits = 0;      // incidences swept by the gain computation
+  int rounds = 0;
+};
+// Km1Policy::computeMaxGainMoveImpl (gain_policy.h:108-167) with
+// randomization disabled: returns {to, gain}.
+void km1_max_gain_move(const Partition& p, uint32_t u, const std::vector<int64_t>& max_w,
+                       int32_t* to, int32_t* gain);
+LpStats lp_refine_ref(Partition& p, const LpParams& prm);
+LpStats lp_refine_sync(Partition& p, const LpParams& prm);
+
+// ------------------------------------------------------------------ graph --
+// Graph (graph.h:41): star expansion of a hypergraph, or the hypergraph
+// itself when every net has two pins.
+enum class EdgeWeight { uniform = 0, non_uniform = 1, degree = 2 };
+struct Graph {
+  uint32_t n = 0;
+  std::vector<uint64_t> idx;
+  std::vector<uint32_t> head;
+  std::vector<double> w;
+  std::vector<double> vol;
+  double tv = 0;
+  uint64_t max_degree = 0;
+};
+Graph build_graph(const Hypergraph& hg, EdgeWeight wt);
+Graph contract_graph(const Graph& g, std::vector<int32_t>& C);
+double modularity(const Graph& g, const std::vector<int32_t>& C);
+
+struct LouvainParams {
+  int max_passes = 5;                  // fast_preset.ini:12
+  double min_move_fraction = 0.01;     // fast_preset.ini:13
+  uint64_t sampling = 200000;          // fast_preset.ini:14
+  uint64_t seed = 0;
+  bool disable_randomization = false;  // ref only
+  int threads = 1;                     // ref only
+  int sub_rounds = 8;                  // sync only
+};
+struct LouvainStats { int64_t arcs_swept = 0; int rounds = 0; int levels = 0; int64_t moves = 0; };
+// computeMaxGainCluster (local_moving_modularity.h:87-129), cluster volumes
+// given (initializeClusterVolumes, local_moving_modularity.cpp:211-218).
+int32_t louvain_best_cluster(const Graph& g, const std::vector<int32_t>& C,
+                             const std::vector<double>& cvol, uint32_t u, uint64_t sampling,
+                             double* best_gain);
+std::vector<int32_t> louvain_ref(Graph& g, const LouvainParams& prm, LouvainStats* st);
+std::vector<int32_t> louvain_sync(Graph& g, const LouvainParams& prm, LouvainStats* st);
+// One synchronous local-moving level (no contraction); returns #moved per round.
+bool louvain_local_moving_sync(const Graph& g, std::vector<int32_t>& C, const LouvainParams& prm,
+                               LouvainStats* st);
+
+// -------------------------------------------------------------- coarsening --
+struct CoarsenParams {
+  uint32_t contraction_limit = 320;   // 160*k (context.cpp:256-269)
+  int32_t max_node_weight = 1;        // context.cpp:271-284
+  double min_shrink = 1.01, max_shrink = 2.5;  // fast_preset.ini:18-19
+  uint32_t ignore_he_size = 1000;     // fast_preset.ini:3
+  uint64_t sampling = 200000;         // fast_preset.ini:22
+  uint64_t seed = 0;
+  int sub_rounds = 16;                // sync only
+  bool disable_randomization = false; // ref only
+  bool prefer_unmatched = true;       // best_prefer_unmatched vs best_without_tie_breaking
+  int threads = 1;                    // ref only
+};
+// One clustering pass; returns cluster representative per vertex
+// (cluster_ids, multilevel_coarsener.h:161-239) and the node count after it.
+std::vector<uint32_t> cluster_pass_ref(const Hypergraph& hg, const CoarsenParams& prm, int pass,
+                                       uint32_t* nodes_after);
+std::vector<uint32_t> cluster_pass_sync(const Hypergraph& hg, const CoarsenParams& prm, int pass,
+                                        uint32_t* nodes_after, int64_t* pin_visits);
+// StaticHypergraph::contract (static_hypergraph.cpp:58-513), canonical form:
+// pins ascending per net, incident nets ascending, survivor of parallel nets
+// is the smallest original net id.  cmap receives fine->coarse vertex ids.
+Hypergraph contract_hypergraph(const Hypergraph& hg, const std::vector<uint32_t>& cluster,
+                               std::vector<uint32_t>& cmap);
+
+struct Level { Hypergraph hg; std::vector<uint32_t> cmap; };
+// MultilevelCoarsener::coarsenImpl (multilevel_coarsener.h:143-315)
+std::vector<Level> coarsen(const Hypergraph& hg, const CoarsenParams& prm, bool sync,
+                           int64_t* pin_visits);
+
+}  // namespace orc

@@ -1,0 +1,170 @@
+"""Benchmark of the Louvain + km1 label propagation hot path on one MI355X.
+
+Workload (default): seeded synthetic hypergraph with 1,000,000 vertices and
+2,000,000 nets, net sizes ~ s^-2.2 on [2, 10000], 80% of the pins inside a
+4096-wide vertex-id window, unit weights, k = 8, eps = 0.03.  The hypergraph
+is resident in HBM before the timed region.
+
+One step = Louvain community detection on the star expansion (all levels,
+fast-preset limits) + re-initialisation of the partition state from the
+resident starting partition (pin counts, connectivity, block weights) + km1
+label propagation (5 rounds).  value = (Louvain pins swept + LP incidences
+visited) / wall time of the timed steps.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mt-kahypar_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import numpy as np  # noqa: E402
+
+import mtk_amd  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "powerlaw_1M_2M_k8": dict(num_vertices=1_000_000, num_nets=2_000_000, exponent=2.2, min_size=2,
+                              max_size=10_000, window=4096, local_fraction=0.8, k=8, eps=0.03, seed=42),
+    "powerlaw_16k_16k_k2": dict(num_vertices=16_384, num_nets=16_384, exponent=2.5, min_size=2, max_size=64,
+                                window=1024, local_fraction=0.8, k=2, eps=0.03, seed=42),
+}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_workload(name):
+    w = WORKLOADS[name]
+    h = mtk_amd.HostHypergraph.powerlaw(w["num_vertices"], w["num_nets"], w["exponent"], w["min_size"],
+                                        w["max_size"], window=w["window"], local_fraction=w["local_fraction"],
+                                        seed=w["seed"])
+    return w, h
+
+
+def start_partition(n, k):
+    # contiguous vertex-id blocks: with id-local pins this is a projected-like
+    # starting partition whose block borders carry the LP work
+    return (np.arange(n, dtype=np.int64) * k // n).astype(np.int32)
+
+
+def cpu_baseline(w, arrays, part, maxw, perfect, threads):
+    """The oracle's reference-semantics LP (asynchronous, OpenMP) on the same
+    hypergraph and starting partition."""
+    import oracle as orc
+    k = w["k"]
+    oh = orc.Hypergraph(arrays["n"], arrays["noff"], arrays["pins"], arrays["vw"], arrays["nw"])
+    t0 = time.perf_counter()
+    _, km1, st = oh.lp_refine(k, part, [maxw] * k, [perfect] * k, seed=w["seed"], sync=False, threads=threads)
+    lp_s = time.perf_counter() - t0
+    return dict(lp_s=lp_s, lp_visits=st["visits"], km1=km1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="powerlaw_1M_2M_k8", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (requested --gpus %d)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    w, host = make_workload(args.workload)
+    arrays = host.arrays()
+    n, k = arrays["n"], w["k"]
+    P = len(arrays["pins"])
+    log("workload %s: n=%d m=%d pins=%d" % (args.workload, n, len(arrays["noff"]) - 1, P))
+    ctx = mtk_amd.Context(0)
+    dh = mtk_amd.Hypergraph(ctx, host=host)
+    W = int(arrays["vw"].sum())
+    perfect = -(-W // k)
+    maxw = int((1.0 + w["eps"]) * perfect)
+    part0 = start_partition(n, k)
+    base = mtk_amd.Partition(dh, k)
+    base.set_parts(part0)
+    km1_start = base.km1()
+    work = mtk_amd.Partition(dh, k)
+
+    def step():
+        work.copy_parts_from(base)
+        st = work.lp_refine([maxw] * k, [perfect] * k, seed=w["seed"])
+        return st["visits"], st
+
+    for i in range(args.warmup):
+        step()
+        log("warmup step %d done" % i)
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    ctx.reset_profile()
+    work_units = 0
+    last = None
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        u, last = step()
+        work_units += u
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    prof = ctx.profile()
+    ctx.set_profiling(False)
+    km1_end = work.km1()
+    log("timed %d steps in %.3f s; km1 %d -> %d" % (args.steps, wall, km1_start, km1_end))
+    for f, v in prof.items():
+        if v["launches"]:
+            log("  %-18s %9.3f ms %6d launches %10.3f GB/s" % (f, v["ms"], v["launches"],
+                                                              v["bytes"] / (v["ms"] * 1e6) if v["ms"] else 0))
+
+    # dominant kernel family with an algorithmic byte count
+    fams = {f: v for f, v in prof.items() if v["launches"] and v["bytes"] > 0}
+    dom = max(fams, key=lambda f: fams[f]["ms"])
+    d = fams[dom]
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+    roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_launch_ms=round(d["ms"] / d["launches"], 4),
+                    algorithmic_bytes_per_launch=round(d["bytes"] / d["launches"]))
+
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        log("cpu baseline: oracle reference-semantics LP, %d threads" % threads)
+        c = cpu_baseline(w, arrays, part0, maxw, perfect, threads)
+        cpu = dict(value=round(c["lp_visits"] / c["lp_s"], 1), unit="pins/s", cores=threads, kind="port",
+                   sample="one full LP refinement (5 rounds) of the same hypergraph and starting partition; "
+                          "%.2f s, km1 %d" % (c["lp_s"], c["km1"]))
+
+    value = work_units / wall
+    out = {
+        "metric": "LP+Louvain pins/s",
+        "value": round(value, 1),
+        "unit": "pins/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "vertices": n, "nets": len(arrays["noff"]) - 1, "pins": P,
+                   "k": k, "epsilon": w["eps"], "km1_start": km1_start, "km1_end": km1_end,
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,162 @@
+/*
+ * mtkahypar_amd.h -- C ABI of the MI355X (gfx950) implementation of
+ * Mt-KaHyPar's data-parallel core: Louvain local moving on the star
+ * expansion, community-restricted heavy-edge clustering and contraction, and
+ * km1 label propagation over the partitioned hypergraph.
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success and a
+ * nonzero status otherwise (1 = invalid argument, 2 = HIP runtime error,
+ * 3 = other); mkp_last_error() returns the message of the last failure of the
+ * calling thread.  Nothing here calls exit().  Objects created on one context
+ * are used from one host thread at a time; distinct contexts are independent.
+ *
+ * "host" pointers are ordinary host memory; the library copies them to the
+ * device.  Device state stays resident between calls.
+ */
+#ifndef MTKAHYPAR_AMD_H
+#define MTKAHYPAR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MKP_API __attribute__((visibility("default")))
+
+typedef struct mkp_context mkp_context;
+typedef struct mkp_hypergraph mkp_hypergraph;
+typedef struct mkp_partition mkp_partition;
+typedef struct mkp_host_hypergraph mkp_host_hypergraph;
+
+/* ------------------------------------------------------------- errors -- */
+MKP_API const char* mkp_last_error(void);
+
+/* ------------------------------------------------------------ context -- */
+/* One device plus one stream.  Replaces the TBB arena set up by
+ * mt_kahypar_initialize_thread_pool (lib/libkahypar.cc:62-83). */
+MKP_API int mkp_context_create(int device, mkp_context** out);
+MKP_API int mkp_context_destroy(mkp_context* ctx);
+MKP_API int mkp_context_synchronize(mkp_context* ctx);
+
+/* Kernel-family timing with HIP events on the context stream (the
+ * counterpart of the reference's utils::Timer phases, utils/timer.h:35-282).
+ * Families: 0 pin_count_init, 1 lp_gain, 2 lp_apply, 3 louvain_move,
+ * 4 louvain_contract, 5 rating, 6 contraction.  Bytes are the algorithmic
+ * (compulsory) HBM bytes of the launches. */
+MKP_API int mkp_context_set_profiling(mkp_context* ctx, int enable);
+MKP_API int mkp_context_reset_profile(mkp_context* ctx);
+MKP_API int mkp_context_get_profile(mkp_context* ctx, int family, double* device_ms, double* algorithmic_bytes,
+                                    int64_t* launches);
+MKP_API const char* mkp_profile_family_name(int family);
+
+/* --------------------------------------------------------- hypergraph -- */
+typedef struct {
+  uint32_t num_vertices;
+  uint32_t num_nets;
+  uint64_t num_pins;
+  uint32_t max_net_size;
+  int64_t total_weight;
+} mkp_hypergraph_info;
+
+/* Device-resident static CSR hypergraph; incident nets are built on the
+ * device in ascending net order.  Replaces StaticHypergraphFactory::construct
+ * (datastructures/static_hypergraph_factory.cpp:32-166).  net_offsets has
+ * num_nets+1 entries; NULL weights mean unit weights. */
+MKP_API int mkp_hypergraph_create(mkp_context* ctx, uint32_t num_vertices, uint32_t num_nets,
+                                  const uint64_t* net_offsets, const uint32_t* pins,
+                                  const int32_t* vertex_weights, const int32_t* net_weights,
+                                  mkp_hypergraph** out);
+MKP_API int mkp_hypergraph_destroy(mkp_hypergraph* hg);
+MKP_API int mkp_hypergraph_get_info(const mkp_hypergraph* hg, mkp_hypergraph_info* info);
+/* StaticHypergraph::setCommunityIDs (datastructures/static_hypergraph.h:790-793) */
+MKP_API int mkp_hypergraph_set_communities(mkp_hypergraph* hg, const int32_t* communities);
+/* Copies the CSR back (any pointer may be NULL). */
+MKP_API int mkp_hypergraph_copy_to_host(const mkp_hypergraph* hg, uint64_t* net_offsets, uint32_t* pins,
+                                        uint64_t* vertex_offsets, uint32_t* incident_nets,
+                                        int32_t* vertex_weights, int32_t* net_weights, int32_t* communities);
+
+/* ---------------------------------------------------- partition state -- */
+/* PartitionedHypergraph (datastructures/partitioned_hypergraph.h:95-126):
+ * block ids, block weights, packed pin counts per (net, block)
+ * (datastructures/pin_count_in_part.h:45-225) and connectivity bitsets
+ * (datastructures/connectivity_set.h:53-221). */
+MKP_API int mkp_partition_create(mkp_hypergraph* hg, int32_t k, mkp_partition** out);
+MKP_API int mkp_partition_destroy(mkp_partition* p);
+/* setOnlyNodePart for every vertex followed by initializePartition
+ * (partitioned_hypergraph.h:522-526, 693-698, 1067-1110). */
+MKP_API int mkp_partition_set_parts(mkp_partition* p, const int32_t* parts);
+/* initializePartition on the block ids already resident on the device. */
+MKP_API int mkp_partition_initialize(mkp_partition* p);
+/* Device-to-device copy of the block ids of src (same hypergraph) followed by
+ * initializePartition. */
+MKP_API int mkp_partition_copy_parts(mkp_partition* dst, const mkp_partition* src);
+MKP_API int mkp_partition_get_parts(const mkp_partition* p, int32_t* parts);
+MKP_API int mkp_partition_get_part_weights(const mkp_partition* p, int64_t* weights);
+/* Unpacked pin counts, num_nets*k values (pinCountInPart, partitioned_hypergraph.h:640-645). */
+MKP_API int mkp_partition_get_pin_counts(const mkp_partition* p, uint32_t* counts);
+/* Connectivity bitsets, num_nets*ceil(k/64) words (connectivity_set.h:80-84,108-116). */
+MKP_API int mkp_partition_get_connectivity(const mkp_partition* p, uint64_t* words);
+/* isBorderNode for every vertex (partitioned_hypergraph.h:605-620). */
+MKP_API int mkp_partition_get_border(const mkp_partition* p, uint8_t* border);
+/* metrics::km1 (partition/metrics.cpp:49-65). */
+MKP_API int mkp_partition_km1(mkp_partition* p, int64_t* km1);
+
+/* ---------------------------------------------------------------- LP --- */
+typedef struct {
+  int32_t max_rounds;              /* r-lp-maximum-iterations (fast preset: 5) */
+  int32_t sub_rounds;              /* buckets per round of the synchronous form */
+  uint32_t activation_threshold;   /* r-lp-he-size-activation-threshold (100) */
+  int32_t rebalancing;             /* r-lp-rebalancing (1) */
+  uint64_t seed;
+  const int64_t* max_part_weights;     /* k values, host */
+  const int64_t* perfect_part_weights; /* k values, host */
+} mkp_lp_params;
+typedef struct {
+  int64_t delta;    /* km1 change */
+  int64_t moves;    /* kept moves */
+  int64_t reverts;  /* moves undone by the real-delta check */
+  int64_t visits;   /* incidences swept by the gain kernels */
+  int64_t rounds;
+  double device_ms; /* stream time of the refinement */
+} mkp_lp_stats;
+/* LabelPropagationRefiner<Km1Policy>::refineImpl
+ * (refinement/label_propagation/label_propagation_refiner.cpp:33-164), in
+ * synchronous sub-rounds. */
+MKP_API int mkp_lp_refine(mkp_partition* p, const mkp_lp_params* params, mkp_lp_stats* stats);
+
+/* ---------------------------------------------------------- generator -- */
+typedef struct {
+  uint32_t num_vertices;
+  uint32_t num_nets;
+  double exponent;          /* net sizes ~ s^-exponent on [min_size, max_size] */
+  uint32_t min_size, max_size;
+  uint32_t window;          /* locality window width in vertex ids, 0 = none */
+  double local_fraction;    /* share of pins drawn inside the window */
+  int32_t vertex_weight_min, vertex_weight_max;
+  uint64_t seed;
+} mkp_powerlaw_params;
+typedef struct {
+  uint32_t num_rows;        /* rows become vertices */
+  uint32_t num_cols;        /* columns become nets */
+  double mean_nnz;          /* nnz per row ~ power law with this mean on [1, max_nnz] */
+  uint32_t max_nnz;
+  uint32_t band;            /* band width in column ids */
+  double band_fraction;
+  uint64_t seed;
+} mkp_rownet_params;
+/* Seeded synthetic inputs (SplitMix64 stream). */
+MKP_API int mkp_generate_powerlaw(const mkp_powerlaw_params* params, mkp_host_hypergraph** out);
+MKP_API int mkp_generate_rownet(const mkp_rownet_params* params, mkp_host_hypergraph** out);
+MKP_API int mkp_host_hypergraph_info(const mkp_host_hypergraph* h, mkp_hypergraph_info* info);
+MKP_API int mkp_host_hypergraph_copy(const mkp_host_hypergraph* h, uint64_t* net_offsets, uint32_t* pins,
+                                     int32_t* vertex_weights, int32_t* net_weights);
+MKP_API int mkp_host_hypergraph_destroy(mkp_host_hypergraph* h);
+MKP_API int mkp_hypergraph_create_from_host(mkp_context* ctx, const mkp_host_hypergraph* h,
+                                            mkp_hypergraph** out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,66 @@
+// Internal object layouts behind the opaque C handles, and the device-wide
+// primitives (rocPRIM radix sort and scans) shared by the kernel files.
+#pragma once
+#include "common.h"
+#include "../../include/mtkahypar_amd.h"
+
+struct mkp_context {
+  mkp::Context c;
+};
+
+// Static CSR hypergraph in HBM.  32-bit offsets (checked at creation),
+// incident nets ascending per vertex.
+struct mkp_hypergraph {
+  mkp_context* ctx = nullptr;
+  uint32_t n = 0, m = 0;
+  uint64_t p = 0;
+  uint32_t max_esize = 0;
+  int64_t total_w = 0;
+  mkp::DevBuf<uint32_t> noff, pins, voff, inc;
+  mkp::DevBuf<int32_t> vw, nw, comm;
+};
+
+namespace mkp {
+// Packed pin counts: b bits per (net, block), epw entries per 64-bit word,
+// vpc words per net (pin_count_in_part.h:198-215).  Fields never carry into
+// each other because a count stays within [0, |e|] <= 2^b - 1, so a 64-bit
+// atomic add of +-(1 << shift) is an exact per-field update.
+struct PcLayout {
+  int b = 1, epw = 1, vpc = 1;
+  uint64_t mask = 1;
+};
+PcLayout pc_layout(int k, uint32_t max_esize);
+}  // namespace mkp
+
+namespace mkp { struct LpWork; void lp_work_free(LpWork* w); }
+
+struct mkp_partition {
+  ~mkp_partition() { mkp::lp_work_free(lp); }
+  mkp_hypergraph* hg = nullptr;
+  mkp::LpWork* lp = nullptr;  // label propagation workspace, kept across calls
+  int k = 0;
+  int vcs = 1;  // connectivity words per net
+  mkp::PcLayout L;
+  mkp::DevBuf<int32_t> part;
+  mkp::DevBuf<int64_t> part_w;
+  mkp::DevBuf<uint64_t> pc, conn;
+  mkp::DevBuf<int64_t> km1_acc;
+};
+
+namespace mkp {
+// rocPRIM wrappers on the context stream (temporary storage from ctx.tmp).
+void sort_keys_u64(Context& c, const uint64_t* in, uint64_t* out, size_t n, int begin_bit, int end_bit);
+void sort_pairs_u64_u32(Context& c, const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout,
+                        size_t n, int begin_bit, int end_bit);
+// out[i] = sum(in[0..i-1]); out has n+1 entries, out[n] = total
+void exclusive_scan_u32(Context& c, const uint32_t* in, uint32_t* out, size_t n);
+void exclusive_scan_u64(Context& c, const uint64_t* in, uint64_t* out, size_t n);
+void inclusive_scan_i64(Context& c, const int64_t* in, int64_t* out, size_t n);
+
+// Builds voff/inc of hg from noff/pins on the device.
+void build_incidence(mkp_hypergraph* hg);
+// Partition initialisation kernels (block weights, pin counts,
+// connectivity, km1) on the device-resident part ids.
+void partition_initialize(mkp_partition* p);
+int64_t partition_km1(mkp_partition* p);
+}  // namespace mkp

@@ -1,0 +1,635 @@
+// km1 label propagation in synchronous sub-rounds.
+//
+// Reference: LabelPropagationRefiner (label_propagation_refiner.h:75-143,
+// label_propagation_refiner.cpp:33-164), Km1Policy::computeMaxGainMoveImpl
+// (gain_policy.h:108-167), changeNodePart (partitioned_hypergraph.h:539-563)
+// and km1Delta (hypergraph_common.h:133-140).
+//
+// One sub-round (r, s):
+//   gain      every active vertex of bucket s computes its Km1Policy move on
+//             the state at the start of the sub-round (coin flips replaced by
+//             hash bits of (seed, r, s, u, block));
+//   admit     candidates into a block whose capacity would be exceeded are
+//             admitted in (gain, hash, id) order up to the capacity;
+//   apply     block ids, block weights, packed pin counts (64-bit atomics on
+//             the packed words) and connectivity bits (atomic xor on 0<->1
+//             transitions, as connectivity_set.h:143-150 does);
+//   verify    each applied move is kept iff its km1 delta on the post-move
+//             state is <= 0 (label_propagation_refiner.h:100-137), otherwise
+//             moved back; a sub-round with positive total delta is undone;
+//   activate  kept moves activate their own vertex and the pins of incident
+//             nets with at most 100 pins for the next round.
+#include <algorithm>
+
+#include "internal.h"
+
+namespace mkp {
+
+constexpr uint32_t kHighDegree = 100000;   // partitioned_hypergraph.h:67
+constexpr uint32_t kThreadDegreeLimit = 512;
+constexpr int kGainThreads = 128;
+
+struct LpWork {
+  uint32_t n = 0;
+  int k = 0;
+  DevBuf<uint32_t> active, active_next, flag, pos, cand_flag, cand_u, heavy;
+  DevBuf<int32_t> cand_to_a, cand_gain_a, cand_from, cand_to, cand_gain;
+  DevBuf<uint8_t> admit, keep;
+  DevBuf<uint64_t> keys, keys_sorted;
+  DevBuf<uint32_t> vals, vals_sorted;
+  DevBuf<int64_t> wsort, wscan, seg_base, in_w, max_w, perfect_w, counters;
+  DevBuf<uint8_t> overflow;
+  void ensure(uint32_t nn, int kk) {
+    if (nn <= n && kk <= k) return;
+    n = std::max(nn, n);
+    k = std::max(kk, k);
+    const size_t N = std::max<uint32_t>(n, 1);
+    active.alloc(N); active_next.alloc(N); flag.alloc(N); pos.alloc(N + 1); cand_flag.alloc(N);
+    cand_u.alloc(N); heavy.alloc(N);
+    cand_to_a.alloc(N); cand_gain_a.alloc(N); cand_from.alloc(N); cand_to.alloc(N); cand_gain.alloc(N);
+    admit.alloc(N); keep.alloc(N);
+    keys.alloc(N); keys_sorted.alloc(N); vals.alloc(N); vals_sorted.alloc(N);
+    wsort.alloc(N); wscan.alloc(N);
+    seg_base.alloc(k); in_w.alloc(k); max_w.alloc(k); perfect_w.alloc(k); overflow.alloc(k);
+    counters.alloc(8);
+  }
+};
+
+// counters: 0 = delta, 1 = visits, 2 = reverts, 3 = heavy count, 4 = moves,
+//           5 = vertices swept by the gain kernels
+struct LpArgs {
+  const uint32_t* voff;
+  const uint32_t* inc;
+  const uint32_t* noff;
+  const uint32_t* pins;
+  const int32_t* vw;
+  const int32_t* nw;
+  int32_t* part;
+  int64_t* part_w;
+  uint64_t* pc;
+  uint64_t* conn;
+  const int64_t* max_w;
+  const int64_t* perfect_w;
+  PcLayout L;
+  int vcs;
+  int k;
+  int rebalancing;
+  uint64_t seed;
+  uint64_t rs;       // (round << 8) | sub-round
+  uint32_t round;
+  int buckets;
+  int bucket;
+  int64_t* counters;
+};
+
+__device__ inline uint32_t pc_field(const uint64_t* pc, const PcLayout& L, uint32_t e, int wi, int sh) {
+  return uint32_t((pc[uint64_t(e) * L.vpc + wi] >> sh) & L.mask);
+}
+
+__device__ inline bool in_bucket(const LpArgs& a, uint32_t u) {
+  return a.buckets <= 1 || hash4(a.seed, kLpBucketTag | uint64_t(a.round), u, 0) % uint64_t(a.buckets) ==
+                               uint64_t(a.bucket);
+}
+
+__device__ inline bool lp_coin(const LpArgs& a, uint32_t u, int b) {
+  return hash4(a.seed, kLpCoinTag | a.rs, u, uint64_t(b)) & 1ull;
+}
+
+// perform condition of moveVertex (label_propagation_refiner.h:84-95)
+__device__ inline bool lp_perform(const LpArgs& a, int from, int to, int32_t gain) {
+  if (to == from) return false;
+  if (gain < 0) return true;
+  return a.rebalancing && gain == 0 && a.part_w[from] - 1 > a.part_w[to] + 1 && a.part_w[to] < a.perfect_w[to];
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline int wave_min_i32(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__device__ inline int wave_max_i32(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// ------------------------------------------------------------ gain (A) --
+// One thread per vertex, K <= 32 scores in registers (fully unrolled), for
+// vertices of degree <= kThreadDegreeLimit; heavier vertices are appended to
+// the heavy list for the wave kernel.
+template <int K>
+__global__ void __launch_bounds__(kGainThreads)
+lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na, uint32_t* __restrict__ cand_flag,
+                      int32_t* __restrict__ cand_to, int32_t* __restrict__ cand_gain, uint32_t* __restrict__ heavy) {
+  int64_t visits = 0, swept = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+    const uint32_t u = active[i];
+    uint32_t flag = 0;
+    if (in_bucket(a, u)) {
+      const uint32_t jb = a.voff[u], je = a.voff[u + 1];
+      const uint32_t deg = je - jb;
+      if (deg > kThreadDegreeLimit) {
+        if (deg <= kHighDegree) {
+          const uint32_t h = atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[3]), 1ull);
+          heavy[h] = i;
+        }
+      } else {
+        const int from = a.part[u];
+        const int fwi = from / a.L.epw;
+        const int fsh = (from - fwi * a.L.epw) * a.L.b;
+        int32_t score[K];
+#pragma unroll
+        for (int b = 0; b < K; ++b) score[b] = 0;
+        int32_t internal = 0;
+        bool border = false;
+        for (uint32_t j = jb; j < je; ++j) {
+          const uint32_t e = a.inc[j];
+          const int32_t w = a.nw[e];
+          const uint64_t c = a.conn[e];
+          if (pc_field(a.pc, a.L, e, fwi, fsh) > 1) internal += w;
+          border |= (c & (c - 1)) != 0;
+#pragma unroll
+          for (int b = 0; b < K; ++b) score[b] -= ((c >> b) & 1ull) ? w : 0;
+        }
+        visits += deg;
+        swept += 1;
+        if (border) {
+          const int64_t wu = a.vw[u];
+          int to = from;
+          int32_t gain = 0;
+#pragma unroll
+          for (int b = 0; b < K; ++b) {
+            if (b < a.k && b != from) {
+              const int32_t sc = score[b] + internal;
+              const bool nb = sc < gain || (sc == gain && lp_coin(a, u, b));
+              if (nb && a.part_w[b] + wu <= a.max_w[b]) { to = b; gain = sc; }
+            }
+          }
+          if (lp_perform(a, from, to, gain)) {
+            flag = 1;
+            cand_to[i] = to;
+            cand_gain[i] = gain;
+          }
+        }
+      }
+    }
+    cand_flag[i] = flag;
+  }
+  visits = wave_sum(visits);
+  swept = wave_sum(swept);
+  if ((threadIdx.x & 63) == 0 && visits) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[1]), static_cast<unsigned long long>(visits));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[5]), static_cast<unsigned long long>(swept));
+  }
+}
+
+// ------------------------------------------------------------ gain (B) --
+// One wave per vertex, any k: lanes split the incident nets and accumulate
+// negative scores into a per-wave LDS array with LDS atomics; the sequential
+// coin scan of Km1Policy is evaluated as wave reductions: with m the minimum
+// eligible score, the result is the last coin-winning block with score m after
+// the first block reaching m (m < 0), or the last coin-winning block with
+// score 0 (otherwise).
+__global__ void __launch_bounds__(64)
+lp_gain_wave_kernel(LpArgs a, const uint32_t* __restrict__ active, const uint32_t* __restrict__ list,
+                    const int64_t* __restrict__ list_count, uint32_t count_val, uint32_t* __restrict__ cand_flag,
+                    int32_t* __restrict__ cand_to, int32_t* __restrict__ cand_gain) {
+  extern __shared__ int32_t sc[];
+  const int lane = threadIdx.x;
+  const uint32_t count = list_count ? uint32_t(*list_count) : count_val;
+  int64_t visits = 0, swept = 0;
+  for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
+    const uint32_t i = list ? list[item] : item;
+    const uint32_t u = active[i];
+    if (!list && !in_bucket(a, u)) {
+      if (lane == 0) cand_flag[i] = 0;
+      continue;
+    }
+    const uint32_t jb = a.voff[u], je = a.voff[u + 1];
+    if (je - jb > kHighDegree) {
+      if (lane == 0) cand_flag[i] = 0;
+      continue;
+    }
+    for (int b = lane; b < a.k; b += 64) sc[b] = 0;
+    __syncthreads();
+    const int from = a.part[u];
+    const int fwi = from / a.L.epw;
+    const int fsh = (from - fwi * a.L.epw) * a.L.b;
+    int32_t internal = 0;
+    int border = 0;
+    for (uint32_t j = jb + lane; j < je; j += 64) {
+      const uint32_t e = a.inc[j];
+      const int32_t w = a.nw[e];
+      if (pc_field(a.pc, a.L, e, fwi, fsh) > 1) internal += w;
+      int lambda = 0;
+      for (int ci = 0; ci < a.vcs; ++ci) {
+        uint64_t bits = a.conn[uint64_t(e) * a.vcs + ci];
+        lambda += __popcll(bits);
+        while (bits) {
+          const int b = ci * 64 + __ffsll(static_cast<long long>(bits)) - 1;
+          bits &= bits - 1;
+          atomicAdd(&sc[b], -w);
+        }
+      }
+      border |= lambda > 1;
+    }
+    visits += (lane == 0) ? int64_t(je - jb) : 0;
+    swept += (lane == 0) ? 1 : 0;
+    internal = wave_sum(internal);
+    border = wave_max_i32(border);
+    __syncthreads();
+    int flag = 0, to = from;
+    int32_t gain = 0;
+    if (border) {
+      const int64_t wu = a.vw[u];
+      const int INF = 0x7fffffff;
+      int m = INF;
+      for (int b = lane; b < a.k; b += 64)
+        if (b != from && a.part_w[b] + wu <= a.max_w[b]) m = min(m, sc[b] + internal);
+      m = wave_min_i32(m);
+      if (m != INF && m < 0) {
+        int i0 = INF;
+        for (int b = lane; b < a.k; b += 64)
+          if (b != from && a.part_w[b] + wu <= a.max_w[b] && sc[b] + internal == m) i0 = min(i0, b);
+        i0 = wave_min_i32(i0);
+        int i1 = -1;
+        for (int b = lane; b < a.k; b += 64)
+          if (b > i0 && b != from && a.part_w[b] + wu <= a.max_w[b] && sc[b] + internal == m && lp_coin(a, u, b))
+            i1 = max(i1, b);
+        i1 = wave_max_i32(i1);
+        to = i1 >= 0 ? i1 : i0;
+        gain = m;
+      } else if (m == 0) {
+        int i1 = -1;
+        for (int b = lane; b < a.k; b += 64)
+          if (b != from && a.part_w[b] + wu <= a.max_w[b] && sc[b] + internal == 0 && lp_coin(a, u, b))
+            i1 = max(i1, b);
+        i1 = wave_max_i32(i1);
+        if (i1 >= 0) to = i1;
+        gain = 0;
+      }
+      flag = lp_perform(a, from, to, gain) ? 1 : 0;
+    }
+    if (lane == 0) {
+      cand_flag[i] = uint32_t(flag);
+      if (flag) { cand_to[i] = to; cand_gain[i] = gain; }
+    }
+    __syncthreads();
+  }
+  if (lane == 0 && visits) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[1]), static_cast<unsigned long long>(visits));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[5]), static_cast<unsigned long long>(swept));
+  }
+}
+
+// ------------------------------------------------------------- compact --
+__global__ void lp_scatter_candidates_kernel(const uint32_t* __restrict__ active, uint32_t na,
+                                             const uint32_t* __restrict__ cand_flag, const uint32_t* __restrict__ pos,
+                                             const int32_t* __restrict__ cto_a, const int32_t* __restrict__ cgain_a,
+                                             const int32_t* __restrict__ part, const int32_t* __restrict__ vw,
+                                             uint32_t* __restrict__ cu, int32_t* __restrict__ cfrom,
+                                             int32_t* __restrict__ cto, int32_t* __restrict__ cgain,
+                                             int64_t* __restrict__ in_w) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+    if (!cand_flag[i]) continue;
+    const uint32_t j = pos[i];
+    const uint32_t u = active[i];
+    cu[j] = u;
+    cfrom[j] = part[u];
+    cto[j] = cto_a[i];
+    cgain[j] = cgain_a[i];
+    atomicAdd(reinterpret_cast<unsigned long long*>(&in_w[cto_a[i]]), static_cast<unsigned long long>(vw[u]));
+  }
+}
+
+// admission sort key: target block, gain clamped to 24 bits, 30 hash bits
+__global__ void lp_admission_keys_kernel(LpArgs a, const uint32_t* __restrict__ cu, const int32_t* __restrict__ cto,
+                                         const int32_t* __restrict__ cgain, uint32_t nc, uint64_t* __restrict__ keys,
+                                         uint32_t* __restrict__ vals) {
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+    const int64_t g = min<int64_t>(max<int64_t>(int64_t(cgain[j]) + (1 << 23), 0), (1 << 24) - 1);
+    const uint64_t prio = hash4(a.seed, kLpPrioTag | a.rs, cu[j], 0) >> 34;
+    keys[j] = (uint64_t(cto[j]) << 54) | (uint64_t(g) << 30) | prio;
+    vals[j] = j;
+  }
+}
+
+__global__ void lp_sorted_weights_kernel(const uint32_t* __restrict__ vals, const uint32_t* __restrict__ cu,
+                                         const int32_t* __restrict__ vw, uint32_t nc, int64_t* __restrict__ w) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x)
+    w[i] = vw[cu[vals[i]]];
+}
+
+__global__ void lp_segment_base_kernel(const uint64_t* __restrict__ keys, const int64_t* __restrict__ w,
+                                       const int64_t* __restrict__ scan, uint32_t nc, int64_t* __restrict__ base) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+    const uint32_t to = uint32_t(keys[i] >> 54);
+    if (i == 0 || uint32_t(keys[i - 1] >> 54) != to) base[to] = scan[i] - w[i];
+  }
+}
+
+__global__ void lp_admit_kernel(LpArgs a, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                const int64_t* __restrict__ scan, const int64_t* __restrict__ base,
+                                const uint8_t* __restrict__ overflow, uint32_t nc, uint8_t* __restrict__ admit) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+    const uint32_t to = uint32_t(keys[i] >> 54);
+    const bool ok = !overflow[to] || a.part_w[to] + (scan[i] - base[to]) <= a.max_w[to];
+    admit[vals[i]] = ok ? 1 : 0;
+  }
+}
+
+// --------------------------------------------------------------- apply --
+// mode 0: apply admitted moves; mode 1: undo admitted && !keep;
+// mode 2: undo admitted && keep (sub-round rollback)
+__global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const int32_t* __restrict__ cfrom,
+                                const int32_t* __restrict__ cto, const uint8_t* __restrict__ admit,
+                                const uint8_t* __restrict__ keep, uint32_t nc, int mode) {
+  int64_t d = 0;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+    if (admit && !admit[j]) continue;
+    if (mode == 1 && keep[j]) continue;
+    if (mode == 2 && !keep[j]) continue;
+    const uint32_t u = cu[j];
+    const int from = mode == 0 ? cfrom[j] : cto[j];
+    const int to = mode == 0 ? cto[j] : cfrom[j];
+    const int64_t wu = a.vw[u];
+    a.part[u] = to;
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a.part_w[to]), static_cast<unsigned long long>(wu));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a.part_w[from]), static_cast<unsigned long long>(-wu));
+    const int fwi = from / a.L.epw, fsh = (from - fwi * a.L.epw) * a.L.b;
+    const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
+    const unsigned long long dec = static_cast<unsigned long long>(-(int64_t(1) << fsh));
+    const unsigned long long incv = 1ull << tsh;
+    for (uint32_t jj = a.voff[u]; jj < a.voff[u + 1]; ++jj) {
+      const uint32_t e = a.inc[jj];
+      const int32_t w = a.nw[e];
+      unsigned long long* base = reinterpret_cast<unsigned long long*>(a.pc + uint64_t(e) * a.L.vpc);
+      const uint64_t of = atomicAdd(base + fwi, dec);
+      if (((of >> fsh) & a.L.mask) == 1) {
+        atomicXor(reinterpret_cast<unsigned long long*>(&a.conn[uint64_t(e) * a.vcs + from / 64]), 1ull << (from % 64));
+        d -= w;
+      }
+      const uint64_t ot = atomicAdd(base + twi, incv);
+      if (((ot >> tsh) & a.L.mask) == 0) {
+        atomicXor(reinterpret_cast<unsigned long long*>(&a.conn[uint64_t(e) * a.vcs + to / 64]), 1ull << (to % 64));
+        d += w;
+      }
+    }
+  }
+  d = wave_sum(d);
+  if ((threadIdx.x & 63) == 0 && d)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), static_cast<unsigned long long>(d));
+}
+
+// real km1 delta of each applied move on the post-move state
+__global__ void lp_verify_kernel(LpArgs a, const uint32_t* __restrict__ cu, const int32_t* __restrict__ cfrom,
+                                 const int32_t* __restrict__ cto, const int32_t* __restrict__ cgain,
+                                 const uint8_t* __restrict__ admit, uint32_t nc, uint8_t* __restrict__ keep) {
+  int64_t reverts = 0;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+    if (admit && !admit[j]) continue;
+    const uint32_t u = cu[j];
+    const int from = cfrom[j], to = cto[j];
+    const int fwi = from / a.L.epw, fsh = (from - fwi * a.L.epw) * a.L.b;
+    const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
+    int64_t real = 0;
+    for (uint32_t jj = a.voff[u]; jj < a.voff[u + 1]; ++jj) {
+      const uint32_t e = a.inc[jj];
+      const int32_t w = a.nw[e];
+      if (pc_field(a.pc, a.L, e, twi, tsh) == 1) real += w;
+      if (pc_field(a.pc, a.L, e, fwi, fsh) == 0) real -= w;
+    }
+    const bool k = real == int64_t(cgain[j]) || real <= 0;
+    keep[j] = k ? 1 : 0;
+    reverts += k ? 0 : 1;
+  }
+  reverts = wave_sum(reverts);
+  if ((threadIdx.x & 63) == 0 && reverts)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[2]), static_cast<unsigned long long>(reverts));
+}
+
+__global__ void lp_activate_kernel(LpArgs a, const uint32_t* __restrict__ cu, const uint8_t* __restrict__ admit,
+                                   const uint8_t* __restrict__ keep, uint32_t nc, uint32_t threshold,
+                                   uint32_t* __restrict__ flag) {
+  int64_t moves = 0;
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+    if ((admit && !admit[j]) || !keep[j]) continue;
+    ++moves;
+    const uint32_t u = cu[j];
+    flag[u] = 1;
+    for (uint32_t jj = a.voff[u]; jj < a.voff[u + 1]; ++jj) {
+      const uint32_t e = a.inc[jj];
+      const uint32_t pb = a.noff[e], pe = a.noff[e + 1];
+      if (pe - pb <= threshold)
+        for (uint32_t q = pb; q < pe; ++q) flag[a.pins[q]] = 1;
+    }
+  }
+  moves = wave_sum(moves);
+  if ((threadIdx.x & 63) == 0 && moves)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[4]), static_cast<unsigned long long>(moves));
+}
+
+__global__ void lp_compact_active_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                         uint32_t n, uint32_t* __restrict__ out) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+    if (flag[u]) out[pos[u]] = u;
+}
+
+__global__ void iota_kernel(uint32_t* out, uint32_t n) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) out[u] = u;
+}
+
+// --------------------------------------------------------------- driver --
+static void launch_gain(const LpArgs& a, LpWork& w, uint32_t na, hipStream_t s, int num_cus) {
+  const unsigned g = grid_for(na, kGainThreads, 65536);
+  if (a.k <= 32) {
+    if (a.k <= 2)
+      lp_gain_thread_kernel<2><<<g, kGainThreads, 0, s>>>(a, w.active.p, na, w.cand_flag.p, w.cand_to_a.p,
+                                                          w.cand_gain_a.p, w.heavy.p);
+    else if (a.k <= 4)
+      lp_gain_thread_kernel<4><<<g, kGainThreads, 0, s>>>(a, w.active.p, na, w.cand_flag.p, w.cand_to_a.p,
+                                                          w.cand_gain_a.p, w.heavy.p);
+    else if (a.k <= 8)
+      lp_gain_thread_kernel<8><<<g, kGainThreads, 0, s>>>(a, w.active.p, na, w.cand_flag.p, w.cand_to_a.p,
+                                                          w.cand_gain_a.p, w.heavy.p);
+    else if (a.k <= 16)
+      lp_gain_thread_kernel<16><<<g, kGainThreads, 0, s>>>(a, w.active.p, na, w.cand_flag.p, w.cand_to_a.p,
+                                                           w.cand_gain_a.p, w.heavy.p);
+    else
+      lp_gain_thread_kernel<32><<<g, kGainThreads, 0, s>>>(a, w.active.p, na, w.cand_flag.p, w.cand_to_a.p,
+                                                           w.cand_gain_a.p, w.heavy.p);
+    MKP_LAUNCH_CHECK();
+    // heavy vertices: persistent grid over the device-side list
+    lp_gain_wave_kernel<<<unsigned(num_cus) * 8, 64, a.k * sizeof(int32_t), s>>>(
+        a, w.active.p, w.heavy.p, a.counters + 3, 0, w.cand_flag.p, w.cand_to_a.p, w.cand_gain_a.p);
+    MKP_LAUNCH_CHECK();
+  } else {
+    lp_gain_wave_kernel<<<grid_for(na, 1, unsigned(num_cus) * 64), 64, a.k * sizeof(int32_t), s>>>(
+        a, w.active.p, nullptr, nullptr, na, w.cand_flag.p, w.cand_to_a.p, w.cand_gain_a.p);
+    MKP_LAUNCH_CHECK();
+  }
+}
+
+void lp_work_free(LpWork* w) { delete w; }
+
+static LpWork& work_of(mkp_partition* p) {
+  if (!p->lp) p->lp = new LpWork();
+  return *p->lp;
+}
+
+void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats) {
+  mkp_hypergraph* hg = p->hg;
+  Context& c = hg->ctx->c;
+  hipStream_t s = c.stream;
+  const uint32_t n = hg->n;
+  LpWork& w = work_of(p);
+  w.ensure(n, p->k);
+  MKP_HIP(hipMemcpyAsync(w.max_w.p, prm.max_part_weights, size_t(p->k) * 8, hipMemcpyHostToDevice, s));
+  MKP_HIP(hipMemcpyAsync(w.perfect_w.p, prm.perfect_part_weights, size_t(p->k) * 8, hipMemcpyHostToDevice, s));
+  fill_async<int64_t>(w.counters.p, 0, 8, s);
+  MKP_HIP(hipEventRecord(c.ev_a, s));
+
+  LpArgs a;
+  a.voff = hg->voff.p; a.inc = hg->inc.p; a.noff = hg->noff.p; a.pins = hg->pins.p;
+  a.vw = hg->vw.p; a.nw = hg->nw.p; a.part = p->part.p; a.part_w = p->part_w.p;
+  a.pc = p->pc.p; a.conn = p->conn.p; a.max_w = w.max_w.p; a.perfect_w = w.perfect_w.p;
+  a.L = p->L; a.vcs = p->vcs; a.k = p->k; a.rebalancing = prm.rebalancing; a.seed = prm.seed;
+  a.counters = w.counters.p;
+  const int B = std::max(1, prm.sub_rounds);
+  a.buckets = B;
+
+  // initializeActiveNodes with rebalancing: all vertices (the non-border ones
+  // drop out in the gain kernel)
+  uint32_t na = n;
+  iota_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(w.active.p, n);
+  MKP_LAUNCH_CHECK();
+  int64_t* host = c.pinned;
+  int64_t delta_total = 0;
+  int rounds = 0;
+  bool stop = false;
+  for (int r = 0; r < prm.max_rounds && na > 0 && !stop; ++r) {
+    a.round = uint32_t(r);
+    fill_async<uint32_t>(w.flag.p, 0u, n, s);
+    for (int sr = 0; sr < B && !stop; ++sr) {
+      a.bucket = sr;
+      a.rs = (uint64_t(r) << 8) | uint64_t(sr);
+      fill_async<int64_t>(w.counters.p + 3, 0, 1, s);
+      {
+        TimedRegion tr(c.prof, s, kFamLpGain, 0.0);
+        launch_gain(a, w, na, s, c.num_cus);
+      }
+      exclusive_scan_u32(c, w.cand_flag.p, w.pos.p, na);
+      fill_async<int64_t>(w.in_w.p, 0, p->k, s);
+      lp_scatter_candidates_kernel<<<grid_for(na, 256, 65536), 256, 0, s>>>(
+          w.active.p, na, w.cand_flag.p, w.pos.p, w.cand_to_a.p, w.cand_gain_a.p, p->part.p, hg->vw.p, w.cand_u.p,
+          w.cand_from.p, w.cand_to.p, w.cand_gain.p, w.in_w.p);
+      MKP_LAUNCH_CHECK();
+      // read back: number of candidates, arrivals per block, block weights
+      uint32_t nc32 = 0;
+      MKP_HIP(hipMemcpyAsync(&nc32, w.pos.p + na, 4, hipMemcpyDeviceToHost, s));
+      MKP_HIP(hipMemcpyAsync(host, w.in_w.p, size_t(p->k) * 8, hipMemcpyDeviceToHost, s));
+      std::vector<int64_t> pw(p->k);
+      MKP_HIP(hipMemcpyAsync(pw.data(), p->part_w.p, size_t(p->k) * 8, hipMemcpyDeviceToHost, s));
+      c.sync();
+      const uint32_t nc = nc32;
+      if (nc == 0) continue;
+      std::vector<uint8_t> over(p->k, 0);
+      bool any_over = false;
+      for (int b = 0; b < p->k; ++b) {
+        over[b] = pw[b] + host[b] > prm.max_part_weights[b];
+        any_over |= over[b] != 0;
+      }
+      const uint8_t* admit = nullptr;
+      if (any_over) {
+        MKP_HIP(hipMemcpyAsync(w.overflow.p, over.data(), p->k, hipMemcpyHostToDevice, s));
+        lp_admission_keys_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(a, w.cand_u.p, w.cand_to.p, w.cand_gain.p,
+                                                                          nc, w.keys.p, w.vals.p);
+        MKP_LAUNCH_CHECK();
+        int kb = 1;
+        while ((1 << kb) < p->k) ++kb;
+        sort_pairs_u64_u32(c, w.keys.p, w.keys_sorted.p, w.vals.p, w.vals_sorted.p, nc, 0, 54 + kb);
+        lp_sorted_weights_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(w.vals_sorted.p, w.cand_u.p, hg->vw.p, nc,
+                                                                          w.wsort.p);
+        MKP_LAUNCH_CHECK();
+        inclusive_scan_i64(c, w.wsort.p, w.wscan.p, nc);
+        lp_segment_base_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(w.keys_sorted.p, w.wsort.p, w.wscan.p, nc,
+                                                                        w.seg_base.p);
+        MKP_LAUNCH_CHECK();
+        lp_admit_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(a, w.keys_sorted.p, w.vals_sorted.p, w.wscan.p,
+                                                                 w.seg_base.p, w.overflow.p, nc, w.admit.p);
+        MKP_LAUNCH_CHECK();
+        admit = w.admit.p;
+      }
+      fill_async<int64_t>(w.counters.p, 0, 1, s);
+      const unsigned gc = grid_for(nc, 256, 65536);
+      TimedRegion tr_apply(c.prof, s, kFamLpApply, 0.0);
+      lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, nullptr, nc, 0);
+      MKP_LAUNCH_CHECK();
+      lp_verify_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, w.cand_gain.p, admit, nc,
+                                          w.keep.p);
+      MKP_LAUNCH_CHECK();
+      lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, w.keep.p, nc, 1);
+      MKP_LAUNCH_CHECK();
+      MKP_HIP(hipMemcpyAsync(host, w.counters.p, 8, hipMemcpyDeviceToHost, s));
+      c.sync();
+      int64_t D = host[0];
+      if (D > 0) {
+        lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, w.keep.p, nc, 2);
+        MKP_LAUNCH_CHECK();
+        MKP_HIP(hipMemcpyAsync(host, w.counters.p, 8, hipMemcpyDeviceToHost, s));
+        c.sync();
+        D = host[0];
+        MKP_CHECK(D == 0, "label propagation rollback did not restore the sub-round state");
+        stop = true;
+        continue;
+      }
+      delta_total += D;
+      lp_activate_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, admit, w.keep.p, nc, prm.activation_threshold, w.flag.p);
+      MKP_LAUNCH_CHECK();
+    }
+    ++rounds;
+    // next active set = flagged vertices in ascending order
+    exclusive_scan_u32(c, w.flag.p, w.pos.p, n);
+    lp_compact_active_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(w.flag.p, w.pos.p, n, w.active.p);
+    MKP_LAUNCH_CHECK();
+    uint32_t na32 = 0;
+    MKP_HIP(hipMemcpyAsync(&na32, w.pos.p + n, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    na = na32;
+  }
+  MKP_HIP(hipEventRecord(c.ev_b, s));
+  MKP_HIP(hipMemcpyAsync(host, w.counters.p, 8 * 8, hipMemcpyDeviceToHost, s));
+  c.sync();
+  if (c.prof.enabled) {
+    // gain sweep: 16 B per swept vertex (offsets, block id, weight) and per
+    // incidence the net id, net weight, one packed counter word and the
+    // connectivity words
+    c.prof.bytes[kFamLpGain] += 16.0 * double(host[5]) + (16.0 + 8.0 * p->vcs) * double(host[1]);
+  }
+  if (stats) {
+    stats->delta = delta_total;
+    stats->moves = host[4];
+    stats->reverts = host[2];
+    stats->visits = host[1];
+    stats->rounds = rounds;
+    float ms = 0;
+    MKP_HIP(hipEventElapsedTime(&ms, c.ev_a, c.ev_b));
+    stats->device_ms = ms;
+  }
+}
+
+}  // namespace mkp
+
+using namespace mkp;
+
+extern "C" int mkp_lp_refine(mkp_partition* p, const mkp_lp_params* prm, mkp_lp_stats* stats) {
+  return guarded([&] {
+    MKP_CHECK(p && prm && prm->max_part_weights && prm->perfect_part_weights, "null argument");
+    MKP_CHECK(prm->max_rounds >= 0 && prm->sub_rounds >= 1 && prm->sub_rounds <= 255, "bad round parameters");
+    MKP_HIP(hipSetDevice(p->hg->ctx->c.device));
+    lp_refine(p, *prm, stats);
+  });
+}

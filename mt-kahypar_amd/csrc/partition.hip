@@ -1,0 +1,312 @@
+// Partition state: block weights, packed pin counts, connectivity sets, km1.
+//
+// initializePinCountInPart (partitioned_hypergraph.h:1086-1110) is the pure
+// bandwidth kernel of the path: one coalesced sweep over the pins of a tile of
+// nets, a gather of the pin's block id, and LDS 64-bit atomics into the tile's
+// packed counters; the tile is then written out once, together with the
+// connectivity words and the tile's km1 contribution.
+#include "internal.h"
+
+namespace mkp {
+
+constexpr int kInitThreads = 256;
+
+__device__ inline uint32_t find_net(const uint32_t* off, int count, uint32_t pos) {
+  // largest i in [0, count) with off[i] <= pos
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= pos) lo = mid; else hi = mid;
+  }
+  return uint32_t(lo);
+}
+
+__global__ void __launch_bounds__(kInitThreads)
+pin_count_init_kernel(const uint32_t* __restrict__ noff, const uint32_t* __restrict__ pins,
+                      const int32_t* __restrict__ part, const int32_t* __restrict__ nw, uint32_t m, int k,
+                      PcLayout L, int vcs, int nets_per_block, uint64_t* __restrict__ pc,
+                      uint64_t* __restrict__ conn, int64_t* __restrict__ km1_acc) {
+  extern __shared__ uint64_t smem[];
+  uint64_t* words = smem;                                                   // nets_per_block * vpc
+  uint32_t* off = reinterpret_cast<uint32_t*>(words + size_t(nets_per_block) * L.vpc);  // nets_per_block + 1
+  __shared__ int64_t red[kInitThreads / 64];
+  const uint32_t e0 = blockIdx.x * uint32_t(nets_per_block);
+  const int ne = int(min<uint32_t>(uint32_t(nets_per_block), m - e0));
+  const int nwords = ne * L.vpc;
+  for (int i = threadIdx.x; i < nwords; i += blockDim.x) words[i] = 0;
+  for (int i = threadIdx.x; i <= ne; i += blockDim.x) off[i] = noff[e0 + i];
+  __syncthreads();
+  const uint32_t pbeg = off[0], pend = off[ne];
+  for (uint32_t q = pbeg + threadIdx.x; q < pend; q += blockDim.x) {
+    const uint32_t le = find_net(off, ne, q);
+    const int blk = part[pins[q]];
+    const int wi = blk / L.epw;
+    const int sh = (blk - wi * L.epw) * L.b;
+    atomicAdd(reinterpret_cast<unsigned long long*>(&words[le * L.vpc + wi]), 1ull << sh);
+  }
+  __syncthreads();
+  uint64_t* out = pc + uint64_t(e0) * L.vpc;
+  for (int i = threadIdx.x; i < nwords; i += blockDim.x) out[i] = words[i];
+  int64_t local = 0;
+  for (int le = threadIdx.x; le < ne; le += blockDim.x) {
+    int lambda = 0;
+    for (int ci = 0; ci < vcs; ++ci) {
+      uint64_t bits = 0;
+      const int bbeg = ci * 64, bend = min(k, bbeg + 64);
+      for (int blk = bbeg; blk < bend; ++blk) {
+        const int wi = blk / L.epw;
+        const int sh = (blk - wi * L.epw) * L.b;
+        if ((words[le * L.vpc + wi] >> sh) & L.mask) bits |= 1ull << (blk - bbeg);
+      }
+      conn[uint64_t(e0 + le) * vcs + ci] = bits;
+      lambda += __popcll(bits);
+    }
+    local += int64_t(max(lambda - 1, 0)) * nw[e0 + le];
+  }
+  for (int o = 32; o > 0; o >>= 1) local += __shfl_down(local, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t s = 0;
+    for (int w = 0; w < int(blockDim.x >> 6); ++w) s += red[w];
+    if (s) atomicAdd(reinterpret_cast<unsigned long long*>(km1_acc), static_cast<unsigned long long>(s));
+  }
+}
+
+__global__ void block_weight_kernel(const int32_t* __restrict__ part, const int32_t* __restrict__ vw, uint32_t n,
+                                    int k, int64_t* __restrict__ part_w) {
+  extern __shared__ int64_t bins[];
+  for (int i = threadIdx.x; i < k; i += blockDim.x) bins[i] = 0;
+  __syncthreads();
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&bins[part[u]]), static_cast<unsigned long long>(vw[u]));
+  __syncthreads();
+  for (int i = threadIdx.x; i < k; i += blockDim.x)
+    if (bins[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&part_w[i]), static_cast<unsigned long long>(bins[i]));
+}
+
+__global__ void km1_kernel(const uint64_t* __restrict__ conn, const int32_t* __restrict__ nw, uint32_t m, int vcs,
+                           int64_t* __restrict__ acc) {
+  int64_t local = 0;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+    int lambda = 0;
+    for (int ci = 0; ci < vcs; ++ci) lambda += __popcll(conn[uint64_t(e) * vcs + ci]);
+    local += int64_t(max(lambda - 1, 0)) * nw[e];
+  }
+  for (int o = 32; o > 0; o >>= 1) local += __shfl_down(local, o, 64);
+  if ((threadIdx.x & 63) == 0 && local)
+    atomicAdd(reinterpret_cast<unsigned long long*>(acc), static_cast<unsigned long long>(local));
+}
+
+__global__ void unpack_pc_kernel(const uint64_t* __restrict__ pc, uint32_t m, int k, PcLayout L,
+                                 uint32_t* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < uint64_t(m) * k;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t e = i / k;
+    const int blk = int(i - e * k);
+    const int wi = blk / L.epw;
+    const int sh = (blk - wi * L.epw) * L.b;
+    out[i] = uint32_t((pc[e * L.vpc + wi] >> sh) & L.mask);
+  }
+}
+
+__global__ void border_kernel(const uint32_t* __restrict__ voff, const uint32_t* __restrict__ inc,
+                              const uint64_t* __restrict__ conn, uint32_t n, int vcs, uint8_t* __restrict__ out) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const uint32_t b = voff[u], e = voff[u + 1];
+    uint8_t border = 0;
+    if (e - b <= 100000u) {
+      for (uint32_t j = b; j < e && !border; ++j) {
+        int lambda = 0;
+        for (int ci = 0; ci < vcs; ++ci) lambda += __popcll(conn[uint64_t(inc[j]) * vcs + ci]);
+        border = lambda > 1;
+      }
+    }
+    out[u] = border;
+  }
+}
+
+static int nets_per_block(const PcLayout& L) {
+  int npb = 256;
+  while (npb > 16 && size_t(npb) * L.vpc * 8 + (npb + 1) * 4 > 64 * 1024) npb >>= 1;
+  return npb;
+}
+
+void partition_initialize(mkp_partition* p) {
+  mkp_hypergraph* hg = p->hg;
+  Context& c = hg->ctx->c;
+  fill_async<int64_t>(p->part_w.p, 0, p->k, c.stream);
+  fill_async<int64_t>(p->km1_acc.p, 0, 1, c.stream);
+  // algorithmic bytes: net offsets, pin id + block id per pin, packed
+  // counters and connectivity written once, block id + weight per vertex
+  const double bytes = 4.0 * (hg->m + 1) + 8.0 * double(hg->p) + 8.0 * hg->m * (p->L.vpc + p->vcs) +
+                       8.0 * hg->n + 4.0 * p->k;
+  TimedRegion tr(c.prof, c.stream, kFamPinCountInit, bytes);
+  if (hg->n) {
+    block_weight_kernel<<<grid_for(hg->n, 256, 2048), 256, p->k * sizeof(int64_t), c.stream>>>(
+        p->part.p, hg->vw.p, hg->n, p->k, p->part_w.p);
+    MKP_LAUNCH_CHECK();
+  }
+  if (hg->m) {
+    const int npb = nets_per_block(p->L);
+    const size_t lds = size_t(npb) * p->L.vpc * 8 + size_t(npb + 1) * 4;
+    pin_count_init_kernel<<<(hg->m + npb - 1) / npb, kInitThreads, lds, c.stream>>>(
+        hg->noff.p, hg->pins.p, p->part.p, hg->nw.p, hg->m, p->k, p->L, p->vcs, npb, p->pc.p, p->conn.p,
+        p->km1_acc.p);
+    MKP_LAUNCH_CHECK();
+  }
+}
+
+int64_t partition_km1(mkp_partition* p) {
+  Context& c = p->hg->ctx->c;
+  fill_async<int64_t>(p->km1_acc.p, 0, 1, c.stream);
+  if (p->hg->m) {
+    km1_kernel<<<grid_for(p->hg->m, 256, 4096), 256, 0, c.stream>>>(p->conn.p, p->hg->nw.p, p->hg->m, p->vcs,
+                                                                    p->km1_acc.p);
+    MKP_LAUNCH_CHECK();
+  }
+  int64_t v = 0;
+  MKP_HIP(hipMemcpyAsync(&v, p->km1_acc.p, 8, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  return v;
+}
+
+}  // namespace mkp
+
+using namespace mkp;
+
+extern "C" {
+
+int mkp_partition_create(mkp_hypergraph* hg, int32_t k, mkp_partition** out) {
+  return guarded([&] {
+    MKP_CHECK(hg && out, "null argument");
+    MKP_CHECK(k >= 2 && k <= 1024, "k must be in [2, 1024]");
+    MKP_HIP(hipSetDevice(hg->ctx->c.device));
+    auto* p = new mkp_partition();
+    try {
+      p->hg = hg;
+      p->k = k;
+      p->vcs = (k + 63) / 64;
+      p->L = pc_layout(k, std::max<uint32_t>(hg->max_esize, 1));
+      p->part.alloc(std::max<uint32_t>(hg->n, 1));
+      p->part_w.alloc(k);
+      p->pc.alloc(std::max<uint64_t>(uint64_t(hg->m) * p->L.vpc, 1));
+      p->conn.alloc(std::max<uint64_t>(uint64_t(hg->m) * p->vcs, 1));
+      p->km1_acc.alloc(1);
+      fill_async<int32_t>(p->part.p, 0, hg->n, hg->ctx->c.stream);
+      partition_initialize(p);
+      hg->ctx->c.sync();
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+  });
+}
+
+int mkp_partition_destroy(mkp_partition* p) {
+  return guarded([&] {
+    if (!p) return;
+    MKP_HIP(hipSetDevice(p->hg->ctx->c.device));
+    (void)hipStreamSynchronize(p->hg->ctx->c.stream);
+    delete p;
+  });
+}
+
+int mkp_partition_set_parts(mkp_partition* p, const int32_t* parts) {
+  return guarded([&] {
+    MKP_CHECK(p && parts, "null argument");
+    for (uint32_t u = 0; u < p->hg->n; ++u) MKP_CHECK(parts[u] >= 0 && parts[u] < p->k, "block id out of range");
+    Context& c = p->hg->ctx->c;
+    MKP_HIP(hipSetDevice(c.device));
+    MKP_HIP(hipMemcpyAsync(p->part.p, parts, size_t(p->hg->n) * 4, hipMemcpyHostToDevice, c.stream));
+    partition_initialize(p);
+    c.sync();
+  });
+}
+
+int mkp_partition_initialize(mkp_partition* p) {
+  return guarded([&] {
+    MKP_CHECK(p, "null argument");
+    MKP_HIP(hipSetDevice(p->hg->ctx->c.device));
+    partition_initialize(p);
+    p->hg->ctx->c.sync();
+  });
+}
+
+int mkp_partition_copy_parts(mkp_partition* dst, const mkp_partition* src) {
+  return guarded([&] {
+    MKP_CHECK(dst && src, "null argument");
+    MKP_CHECK(dst->hg == src->hg && dst->k == src->k, "partitions of different hypergraphs or k");
+    Context& c = dst->hg->ctx->c;
+    MKP_HIP(hipSetDevice(c.device));
+    MKP_HIP(hipMemcpyAsync(dst->part.p, src->part.p, size_t(dst->hg->n) * 4, hipMemcpyDeviceToDevice, c.stream));
+    partition_initialize(dst);
+    c.sync();
+  });
+}
+
+int mkp_partition_get_parts(const mkp_partition* p, int32_t* parts) {
+  return guarded([&] {
+    MKP_CHECK(p && parts, "null argument");
+    Context& c = p->hg->ctx->c;
+    MKP_HIP(hipMemcpyAsync(parts, p->part.p, size_t(p->hg->n) * 4, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  });
+}
+
+int mkp_partition_get_part_weights(const mkp_partition* p, int64_t* w) {
+  return guarded([&] {
+    MKP_CHECK(p && w, "null argument");
+    Context& c = p->hg->ctx->c;
+    MKP_HIP(hipMemcpyAsync(w, p->part_w.p, size_t(p->k) * 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  });
+}
+
+int mkp_partition_get_pin_counts(const mkp_partition* p, uint32_t* counts) {
+  return guarded([&] {
+    MKP_CHECK(p && counts, "null argument");
+    Context& c = p->hg->ctx->c;
+    const uint64_t total = uint64_t(p->hg->m) * p->k;
+    if (!total) return;
+    DevBuf<uint32_t> tmp(total);
+    unpack_pc_kernel<<<grid_for(total, 256, 65536), 256, 0, c.stream>>>(p->pc.p, p->hg->m, p->k, p->L, tmp.p);
+    MKP_LAUNCH_CHECK();
+    MKP_HIP(hipMemcpyAsync(counts, tmp.p, total * 4, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  });
+}
+
+int mkp_partition_get_connectivity(const mkp_partition* p, uint64_t* words) {
+  return guarded([&] {
+    MKP_CHECK(p && words, "null argument");
+    Context& c = p->hg->ctx->c;
+    if (p->hg->m)
+      MKP_HIP(hipMemcpyAsync(words, p->conn.p, uint64_t(p->hg->m) * p->vcs * 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  });
+}
+
+int mkp_partition_get_border(const mkp_partition* p, uint8_t* border) {
+  return guarded([&] {
+    MKP_CHECK(p && border, "null argument");
+    Context& c = p->hg->ctx->c;
+    if (!p->hg->n) return;
+    DevBuf<uint8_t> tmp(p->hg->n);
+    border_kernel<<<grid_for(p->hg->n, 256, 65536), 256, 0, c.stream>>>(p->hg->voff.p, p->hg->inc.p, p->conn.p,
+                                                                        p->hg->n, p->vcs, tmp.p);
+    MKP_LAUNCH_CHECK();
+    MKP_HIP(hipMemcpyAsync(border, tmp.p, p->hg->n, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  });
+}
+
+int mkp_partition_km1(mkp_partition* p, int64_t* km1) {
+  return guarded([&] {
+    MKP_CHECK(p && km1, "null argument");
+    *km1 = partition_km1(p);
+  });
+}
+
+}  // extern "C"

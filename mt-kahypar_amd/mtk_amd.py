@@ -1,0 +1,289 @@
+"""ctypes bindings of libmtkahypar_amd.so (the C ABI in include/mtkahypar_amd.h).
+
+Raises at import-use time if the library has not been built; there is no
+fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmtkahypar_amd.so")
+_lib = None
+
+vp = C.c_void_p
+P = C.POINTER
+
+
+class HypergraphInfo(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("num_nets", C.c_uint32), ("num_pins", C.c_uint64),
+                ("max_net_size", C.c_uint32), ("total_weight", C.c_int64)]
+
+
+class LpParams(C.Structure):
+    _fields_ = [("max_rounds", C.c_int32), ("sub_rounds", C.c_int32), ("activation_threshold", C.c_uint32),
+                ("rebalancing", C.c_int32), ("seed", C.c_uint64), ("max_part_weights", P(C.c_int64)),
+                ("perfect_part_weights", P(C.c_int64))]
+
+
+class LpStats(C.Structure):
+    _fields_ = [("delta", C.c_int64), ("moves", C.c_int64), ("reverts", C.c_int64), ("visits", C.c_int64),
+                ("rounds", C.c_int64), ("device_ms", C.c_double)]
+
+
+class PowerlawParams(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("num_nets", C.c_uint32), ("exponent", C.c_double),
+                ("min_size", C.c_uint32), ("max_size", C.c_uint32), ("window", C.c_uint32),
+                ("local_fraction", C.c_double), ("vertex_weight_min", C.c_int32),
+                ("vertex_weight_max", C.c_int32), ("seed", C.c_uint64)]
+
+
+class RownetParams(C.Structure):
+    _fields_ = [("num_rows", C.c_uint32), ("num_cols", C.c_uint32), ("mean_nnz", C.c_double),
+                ("max_nnz", C.c_uint32), ("band", C.c_uint32), ("band_fraction", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+# name -> (restype, argtypes); every function returns int status unless noted
+SIGNATURES = {
+    "mkp_last_error": (C.c_char_p, []),
+    "mkp_context_create": (C.c_int, [C.c_int, P(vp)]),
+    "mkp_context_destroy": (C.c_int, [vp]),
+    "mkp_context_synchronize": (C.c_int, [vp]),
+    "mkp_context_set_profiling": (C.c_int, [vp, C.c_int]),
+    "mkp_context_reset_profile": (C.c_int, [vp]),
+    "mkp_context_get_profile": (C.c_int, [vp, C.c_int, P(C.c_double), P(C.c_double), P(C.c_int64)]),
+    "mkp_profile_family_name": (C.c_char_p, [C.c_int]),
+    "mkp_partition_initialize": (C.c_int, [vp]),
+    "mkp_partition_copy_parts": (C.c_int, [vp, vp]),
+    "mkp_hypergraph_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, P(vp)]),
+    "mkp_hypergraph_destroy": (C.c_int, [vp]),
+    "mkp_hypergraph_get_info": (C.c_int, [vp, P(HypergraphInfo)]),
+    "mkp_hypergraph_set_communities": (C.c_int, [vp, vp]),
+    "mkp_hypergraph_copy_to_host": (C.c_int, [vp, vp, vp, vp, vp, vp, vp, vp]),
+    "mkp_partition_create": (C.c_int, [vp, C.c_int32, P(vp)]),
+    "mkp_partition_destroy": (C.c_int, [vp]),
+    "mkp_partition_set_parts": (C.c_int, [vp, vp]),
+    "mkp_partition_get_parts": (C.c_int, [vp, vp]),
+    "mkp_partition_get_part_weights": (C.c_int, [vp, vp]),
+    "mkp_partition_get_pin_counts": (C.c_int, [vp, vp]),
+    "mkp_partition_get_connectivity": (C.c_int, [vp, vp]),
+    "mkp_partition_get_border": (C.c_int, [vp, vp]),
+    "mkp_partition_km1": (C.c_int, [vp, P(C.c_int64)]),
+    "mkp_lp_refine": (C.c_int, [vp, P(LpParams), P(LpStats)]),
+    "mkp_generate_powerlaw": (C.c_int, [P(PowerlawParams), P(vp)]),
+    "mkp_generate_rownet": (C.c_int, [P(RownetParams), P(vp)]),
+    "mkp_host_hypergraph_info": (C.c_int, [vp, P(HypergraphInfo)]),
+    "mkp_host_hypergraph_copy": (C.c_int, [vp, vp, vp, vp, vp]),
+    "mkp_host_hypergraph_destroy": (C.c_int, [vp]),
+    "mkp_hypergraph_create_from_host": (C.c_int, [vp, vp, P(vp)]),
+}
+
+
+class MkpError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MkpError("libmtkahypar_amd.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != 0:
+        raise MkpError("status %d: %s" % (status, lib().mkp_last_error().decode()))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(vp)
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = vp()
+        check(lib().mkp_context_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            check(lib().mkp_context_destroy(self.h))
+            self.h = vp()
+
+    def synchronize(self):
+        check(lib().mkp_context_synchronize(self.h))
+
+    FAMILIES = 7
+
+    def set_profiling(self, on):
+        check(lib().mkp_context_set_profiling(self.h, int(on)))
+
+    def reset_profile(self):
+        check(lib().mkp_context_reset_profile(self.h))
+
+    def profile(self):
+        out = {}
+        for f in range(self.FAMILIES):
+            ms, by, n = C.c_double(), C.c_double(), C.c_int64()
+            check(lib().mkp_context_get_profile(self.h, f, C.byref(ms), C.byref(by), C.byref(n)))
+            out[lib().mkp_profile_family_name(f).decode()] = dict(ms=ms.value, bytes=by.value, launches=n.value)
+        return out
+
+
+class HostHypergraph:
+    """Synthetic input generated on the host."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @staticmethod
+    def powerlaw(num_vertices, num_nets, exponent, min_size, max_size, window=0, local_fraction=0.0,
+                 vertex_weight_min=1, vertex_weight_max=1, seed=42):
+        prm = PowerlawParams(num_vertices, num_nets, exponent, min_size, max_size, window, local_fraction,
+                             vertex_weight_min, vertex_weight_max, seed)
+        h = vp()
+        check(lib().mkp_generate_powerlaw(C.byref(prm), C.byref(h)))
+        return HostHypergraph(h)
+
+    @staticmethod
+    def rownet(num_rows, num_cols, mean_nnz, max_nnz, band, band_fraction, seed=42):
+        prm = RownetParams(num_rows, num_cols, mean_nnz, max_nnz, band, band_fraction, seed)
+        h = vp()
+        check(lib().mkp_generate_rownet(C.byref(prm), C.byref(h)))
+        return HostHypergraph(h)
+
+    def info(self):
+        i = HypergraphInfo()
+        check(lib().mkp_host_hypergraph_info(self.h, C.byref(i)))
+        return i
+
+    def arrays(self):
+        i = self.info()
+        noff = np.zeros(i.num_nets + 1, np.uint64)
+        pins = np.zeros(max(i.num_pins, 1), np.uint32)
+        vw = np.zeros(max(i.num_vertices, 1), np.int32)
+        nw = np.zeros(max(i.num_nets, 1), np.int32)
+        check(lib().mkp_host_hypergraph_copy(self.h, _p(noff), _p(pins), _p(vw), _p(nw)))
+        return dict(n=i.num_vertices, noff=noff, pins=pins[:i.num_pins], vw=vw[:i.num_vertices],
+                    nw=nw[:i.num_nets])
+
+    def close(self):
+        if self.h:
+            check(lib().mkp_host_hypergraph_destroy(self.h))
+            self.h = vp()
+
+
+class Hypergraph:
+    def __init__(self, ctx, n=None, noff=None, pins=None, vw=None, nw=None, host=None, handle=None):
+        self.ctx = ctx
+        self.h = vp()
+        if handle is not None:
+            self.h = handle
+        elif host is not None:
+            check(lib().mkp_hypergraph_create_from_host(ctx.h, host.h, C.byref(self.h)))
+        else:
+            self._noff = np.ascontiguousarray(noff, dtype=np.uint64)
+            self._pins = np.ascontiguousarray(pins, dtype=np.uint32)
+            self._vw = None if vw is None else np.ascontiguousarray(vw, dtype=np.int32)
+            self._nw = None if nw is None else np.ascontiguousarray(nw, dtype=np.int32)
+            check(lib().mkp_hypergraph_create(ctx.h, n, len(self._noff) - 1, _p(self._noff), _p(self._pins),
+                                              _p(self._vw), _p(self._nw), C.byref(self.h)))
+
+    def info(self):
+        i = HypergraphInfo()
+        check(lib().mkp_hypergraph_get_info(self.h, C.byref(i)))
+        return i
+
+    def set_communities(self, comm):
+        c = np.ascontiguousarray(comm, dtype=np.int32)
+        check(lib().mkp_hypergraph_set_communities(self.h, _p(c)))
+
+    def arrays(self):
+        i = self.info()
+        noff = np.zeros(i.num_nets + 1, np.uint64); pins = np.zeros(max(i.num_pins, 1), np.uint32)
+        voff = np.zeros(i.num_vertices + 1, np.uint64); inc = np.zeros(max(i.num_pins, 1), np.uint32)
+        vw = np.zeros(max(i.num_vertices, 1), np.int32); nw = np.zeros(max(i.num_nets, 1), np.int32)
+        comm = np.zeros(max(i.num_vertices, 1), np.int32)
+        check(lib().mkp_hypergraph_copy_to_host(self.h, _p(noff), _p(pins), _p(voff), _p(inc), _p(vw), _p(nw),
+                                                _p(comm)))
+        p, n, m = i.num_pins, i.num_vertices, i.num_nets
+        return dict(noff=noff, pins=pins[:p], voff=voff, inc=inc[:p], vw=vw[:n], nw=nw[:m], comm=comm[:n])
+
+    def close(self):
+        if self.h:
+            check(lib().mkp_hypergraph_destroy(self.h))
+            self.h = vp()
+
+
+class Partition:
+    def __init__(self, hg, k):
+        self.hg, self.k = hg, k
+        self.h = vp()
+        check(lib().mkp_partition_create(hg.h, k, C.byref(self.h)))
+
+    def set_parts(self, parts):
+        a = np.ascontiguousarray(parts, dtype=np.int32)
+        check(lib().mkp_partition_set_parts(self.h, _p(a)))
+
+    def initialize(self):
+        check(lib().mkp_partition_initialize(self.h))
+
+    def copy_parts_from(self, src):
+        check(lib().mkp_partition_copy_parts(self.h, src.h))
+
+    def parts(self):
+        out = np.zeros(self.hg.info().num_vertices, np.int32)
+        check(lib().mkp_partition_get_parts(self.h, _p(out)))
+        return out
+
+    def part_weights(self):
+        out = np.zeros(self.k, np.int64)
+        check(lib().mkp_partition_get_part_weights(self.h, _p(out)))
+        return out
+
+    def pin_counts(self):
+        m = self.hg.info().num_nets
+        out = np.zeros(max(m * self.k, 1), np.uint32)
+        check(lib().mkp_partition_get_pin_counts(self.h, _p(out)))
+        return out[:m * self.k].reshape(m, self.k)
+
+    def connectivity(self):
+        m = self.hg.info().num_nets
+        cw = (self.k + 63) // 64
+        out = np.zeros(max(m * cw, 1), np.uint64)
+        check(lib().mkp_partition_get_connectivity(self.h, _p(out)))
+        return out[:m * cw].reshape(m, cw)
+
+    def border(self):
+        out = np.zeros(max(self.hg.info().num_vertices, 1), np.uint8)
+        check(lib().mkp_partition_get_border(self.h, _p(out)))
+        return out[:self.hg.info().num_vertices]
+
+    def km1(self):
+        v = C.c_int64()
+        check(lib().mkp_partition_km1(self.h, C.byref(v)))
+        return v.value
+
+    def lp_refine(self, max_w, perfect_w, seed=0, max_rounds=5, sub_rounds=4, activation_threshold=100,
+                  rebalancing=True):
+        mw = np.ascontiguousarray(max_w, dtype=np.int64)
+        pw = np.ascontiguousarray(perfect_w, dtype=np.int64)
+        prm = LpParams(max_rounds, sub_rounds, activation_threshold, int(rebalancing), seed,
+                       mw.ctypes.data_as(P(C.c_int64)), pw.ctypes.data_as(P(C.c_int64)))
+        st = LpStats()
+        check(lib().mkp_lp_refine(self.h, C.byref(prm), C.byref(st)))
+        return dict(delta=st.delta, moves=st.moves, reverts=st.reverts, visits=st.visits, rounds=st.rounds,
+                    device_ms=st.device_ms)
+
+    def close(self):
+        if self.h:
+            check(lib().mkp_partition_destroy(self.h))
+            self.h = vp()

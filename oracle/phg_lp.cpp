@@ -231,8 +231,8 @@ struct RefLp {
 
   // label_propagation_refiner.h:75-143
   bool move_vertex(uint32_t u, int t, std::vector<int64_t>& score) {
+    if (hg.degree(u) <= 100000) __atomic_fetch_add(&visits, int64_t(hg.degree(u)), __ATOMIC_RELAXED);
     if (!border(u)) return false;
-    ++visits;  // approximate counter (racy, informational only)
     int to;
     int64_t gain;
     best_move(u, t, score, to, gain);
@@ -359,8 +359,9 @@ LpStats lp_refine_sync(Partition& p, const LpParams& prm) {
       const uint64_t rs = (uint64_t(r) << 8) | uint64_t(s);
       for (uint32_t u : active) {
         if (B > 1 && hash4(prm.seed, kLpBucketTag | uint64_t(r), u, 0) % B != uint64_t(s)) continue;
+        if (hg.degree(u) > 100000) continue;
+        st.visits += hg.degree(u);  // the gain sweep reads every incidence once
         if (!is_border(p, u)) continue;
-        st.visits += hg.degree(u);
         const int from = p.part[u];
         int64_t internal = 0;
         for (uint64_t i = hg.voff[u]; i < hg.voff[u + 1]; ++i) {
@@ -394,10 +395,12 @@ LpStats lp_refine_sync(Partition& p, const LpParams& prm) {
         if (p.part_w[b] + in_w[b] <= prm.max_w[b]) continue;
         std::vector<size_t> idx;
         for (size_t i = 0; i < cands.size(); ++i) if (cands[i].to == b) idx.push_back(i);
+        // order: gain clamped to 24 bits, then the top 30 hash bits, then id
+        auto clampg = [](int32_t g) { return std::min<int64_t>(std::max<int64_t>(int64_t(g) + (1 << 23), 0), (1 << 24) - 1); };
         std::sort(idx.begin(), idx.end(), [&](size_t x, size_t y) {
           const Cand& a = cands[x]; const Cand& c = cands[y];
-          if (a.gain != c.gain) return a.gain < c.gain;
-          if (a.prio != c.prio) return a.prio < c.prio;
+          if (clampg(a.gain) != clampg(c.gain)) return clampg(a.gain) < clampg(c.gain);
+          if ((a.prio >> 34) != (c.prio >> 34)) return (a.prio >> 34) < (c.prio >> 34);
           return a.u < c.u;
         });
         int64_t cum = p.part_w[b];

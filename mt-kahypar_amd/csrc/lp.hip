@@ -39,6 +39,17 @@ struct LpWork {
   DevBuf<uint32_t> vals, vals_sorted;
   DevBuf<int64_t> wsort, wscan, seg_base, in_w, max_w, perfect_w, counters;
   DevBuf<uint8_t> overflow;
+  DevBuf<uint32_t> net_stamp;  // last round stamp that expanded each net
+  DevBuf<uint64_t> bkeys, bkeys_sorted;
+  DevBuf<int64_t> bcount;
+  uint32_t m = 0, stamp = 0;
+  void ensure_nets(uint32_t mm, hipStream_t s) {
+    if (mm <= m && net_stamp.p) return;
+    m = std::max(mm, m);
+    net_stamp.alloc(std::max<uint32_t>(m, 1));
+    fill_async<uint32_t>(net_stamp.p, 0u, std::max<uint32_t>(m, 1), s);
+    stamp = 0;
+  }
   void ensure(uint32_t nn, int kk) {
     if (nn <= n && kk <= k) return;
     n = std::max(nn, n);
@@ -49,7 +60,7 @@ struct LpWork {
     cand_to_a.alloc(N); cand_gain_a.alloc(N); cand_from.alloc(N); cand_to.alloc(N); cand_gain.alloc(N);
     admit.alloc(N); keep.alloc(N);
     keys.alloc(N); keys_sorted.alloc(N); vals.alloc(N); vals_sorted.alloc(N);
-    wsort.alloc(N); wscan.alloc(N);
+    wsort.alloc(N); wscan.alloc(N); bkeys.alloc(N); bkeys_sorted.alloc(N); bcount.alloc(256);
     seg_base.alloc(k); in_w.alloc(k); max_w.alloc(k); perfect_w.alloc(k); overflow.alloc(k);
     counters.alloc(8);
   }
@@ -411,24 +422,28 @@ __global__ void lp_verify_kernel(LpArgs a, const uint32_t* __restrict__ cu, cons
     atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[2]), static_cast<unsigned long long>(reverts));
 }
 
-__global__ void lp_activate_kernel(LpArgs a, const uint32_t* __restrict__ cu, const uint8_t* __restrict__ admit,
-                                   const uint8_t* __restrict__ keep, uint32_t nc, uint32_t threshold,
-                                   uint32_t* __restrict__ flag) {
+// One wave per kept move; lanes take the incident nets.  A net of at most
+// `threshold` pins is expanded by the first move that stamps it in this round
+// (the visited-net flags of label_propagation_refiner.h:113-125).
+__global__ void __launch_bounds__(64)
+lp_activate_kernel(LpArgs a, const uint32_t* __restrict__ cu, const uint8_t* __restrict__ admit,
+                   const uint8_t* __restrict__ keep, uint32_t nc, uint32_t threshold, uint32_t stamp,
+                   uint32_t* __restrict__ net_stamp, uint32_t* __restrict__ flag) {
   int64_t moves = 0;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+  const int lane = threadIdx.x;
+  for (uint32_t j = blockIdx.x; j < nc; j += gridDim.x) {
     if ((admit && !admit[j]) || !keep[j]) continue;
-    ++moves;
     const uint32_t u = cu[j];
-    flag[u] = 1;
-    for (uint32_t jj = a.voff[u]; jj < a.voff[u + 1]; ++jj) {
+    if (lane == 0) { flag[u] = 1; ++moves; }
+    const uint32_t jb = a.voff[u], je = a.voff[u + 1];
+    for (uint32_t jj = jb + lane; jj < je; jj += 64) {
       const uint32_t e = a.inc[jj];
       const uint32_t pb = a.noff[e], pe = a.noff[e + 1];
-      if (pe - pb <= threshold)
+      if (pe - pb <= threshold && atomicExch(&net_stamp[e], stamp) != stamp)
         for (uint32_t q = pb; q < pe; ++q) flag[a.pins[q]] = 1;
     }
   }
-  moves = wave_sum(moves);
-  if ((threadIdx.x & 63) == 0 && moves)
+  if (lane == 0 && moves)
     atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[4]), static_cast<unsigned long long>(moves));
 }
 
@@ -443,32 +458,37 @@ __global__ void iota_kernel(uint32_t* out, uint32_t n) {
 }
 
 // --------------------------------------------------------------- driver --
-static void launch_gain(const LpArgs& a, LpWork& w, uint32_t na, hipStream_t s, int num_cus) {
-  const unsigned g = grid_for(na, kGainThreads, 65536);
+__global__ void lp_bucket_keys_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na,
+                                      uint64_t* __restrict__ keys, int64_t* __restrict__ count) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+    const uint64_t b = hash4(a.seed, kLpBucketTag | uint64_t(a.round), active[i], 0) % uint64_t(a.buckets);
+    keys[i] = b;
+    atomicAdd(reinterpret_cast<unsigned long long*>(&count[b]), 1ull);
+  }
+}
+
+// --------------------------------------------------------------- driver --
+template <int K>
+static void launch_thread_gain(const LpArgs& a, LpWork& w, const uint32_t* act, uint32_t na, hipStream_t s) {
+  lp_gain_thread_kernel<K><<<grid_for(na, kGainThreads, 65536), kGainThreads, 0, s>>>(
+      a, act, na, w.cand_flag.p, w.cand_to_a.p, w.cand_gain_a.p, w.heavy.p);
+}
+
+static void launch_gain(const LpArgs& a, LpWork& w, const uint32_t* act, uint32_t na, hipStream_t s, int num_cus) {
   if (a.k <= 32) {
-    if (a.k <= 2)
-      lp_gain_thread_kernel<2><<<g, kGainThreads, 0, s>>>(a, w.active.p, na, w.cand_flag.p, w.cand_to_a.p,
-                                                          w.cand_gain_a.p, w.heavy.p);
-    else if (a.k <= 4)
-      lp_gain_thread_kernel<4><<<g, kGainThreads, 0, s>>>(a, w.active.p, na, w.cand_flag.p, w.cand_to_a.p,
-                                                          w.cand_gain_a.p, w.heavy.p);
-    else if (a.k <= 8)
-      lp_gain_thread_kernel<8><<<g, kGainThreads, 0, s>>>(a, w.active.p, na, w.cand_flag.p, w.cand_to_a.p,
-                                                          w.cand_gain_a.p, w.heavy.p);
-    else if (a.k <= 16)
-      lp_gain_thread_kernel<16><<<g, kGainThreads, 0, s>>>(a, w.active.p, na, w.cand_flag.p, w.cand_to_a.p,
-                                                           w.cand_gain_a.p, w.heavy.p);
-    else
-      lp_gain_thread_kernel<32><<<g, kGainThreads, 0, s>>>(a, w.active.p, na, w.cand_flag.p, w.cand_to_a.p,
-                                                           w.cand_gain_a.p, w.heavy.p);
+    if (a.k <= 2) launch_thread_gain<2>(a, w, act, na, s);
+    else if (a.k <= 4) launch_thread_gain<4>(a, w, act, na, s);
+    else if (a.k <= 8) launch_thread_gain<8>(a, w, act, na, s);
+    else if (a.k <= 16) launch_thread_gain<16>(a, w, act, na, s);
+    else launch_thread_gain<32>(a, w, act, na, s);
     MKP_LAUNCH_CHECK();
     // heavy vertices: persistent grid over the device-side list
     lp_gain_wave_kernel<<<unsigned(num_cus) * 8, 64, a.k * sizeof(int32_t), s>>>(
-        a, w.active.p, w.heavy.p, a.counters + 3, 0, w.cand_flag.p, w.cand_to_a.p, w.cand_gain_a.p);
+        a, act, w.heavy.p, a.counters + 3, 0, w.cand_flag.p, w.cand_to_a.p, w.cand_gain_a.p);
     MKP_LAUNCH_CHECK();
   } else {
     lp_gain_wave_kernel<<<grid_for(na, 1, unsigned(num_cus) * 64), 64, a.k * sizeof(int32_t), s>>>(
-        a, w.active.p, nullptr, nullptr, na, w.cand_flag.p, w.cand_to_a.p, w.cand_gain_a.p);
+        a, act, nullptr, nullptr, na, w.cand_flag.p, w.cand_to_a.p, w.cand_gain_a.p);
     MKP_LAUNCH_CHECK();
   }
 }
@@ -487,6 +507,7 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats) 
   const uint32_t n = hg->n;
   LpWork& w = work_of(p);
   w.ensure(n, p->k);
+  w.ensure_nets(hg->m, s);
   MKP_HIP(hipMemcpyAsync(w.max_w.p, prm.max_part_weights, size_t(p->k) * 8, hipMemcpyHostToDevice, s));
   MKP_HIP(hipMemcpyAsync(w.perfect_w.p, prm.perfect_part_weights, size_t(p->k) * 8, hipMemcpyHostToDevice, s));
   fill_async<int64_t>(w.counters.p, 0, 8, s);
@@ -512,25 +533,54 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats) 
   bool stop = false;
   for (int r = 0; r < prm.max_rounds && na > 0 && !stop; ++r) {
     a.round = uint32_t(r);
+    if (++w.stamp == 0) {  // stamp wrapped: clear the net stamps once
+      fill_async<uint32_t>(w.net_stamp.p, 0u, std::max<uint32_t>(w.m, 1), s);
+      w.stamp = 1;
+    }
+    const uint32_t round_stamp = w.stamp;
     fill_async<uint32_t>(w.flag.p, 0u, n, s);
+    // stable partition of the active list by bucket: each sub-round sweeps a
+    // contiguous ascending slice
+    const uint32_t* base = w.active.p;
+    std::vector<uint32_t> boff(size_t(B) + 1, 0);
+    boff[B] = na;
+    a.buckets = B;
+    if (B > 1) {
+      fill_async<int64_t>(w.bcount.p, 0, B, s);
+      lp_bucket_keys_kernel<<<grid_for(na, 256, 65536), 256, 0, s>>>(a, w.active.p, na, w.bkeys.p, w.bcount.p);
+      MKP_LAUNCH_CHECK();
+      int bb = 1;
+      while ((1 << bb) < B) ++bb;
+      sort_pairs_u64_u32(c, w.bkeys.p, w.bkeys_sorted.p, w.active.p, w.active_next.p, na, 0, bb);
+      std::vector<int64_t> cnt(B);
+      MKP_HIP(hipMemcpyAsync(cnt.data(), w.bcount.p, size_t(B) * 8, hipMemcpyDeviceToHost, s));
+      c.sync();
+      for (int b = 0; b < B; ++b) boff[b + 1] = boff[b] + uint32_t(cnt[b]);
+      base = w.active_next.p;
+    }
+    a.buckets = 1;  // slices are pre-partitioned
     for (int sr = 0; sr < B && !stop; ++sr) {
       a.bucket = sr;
       a.rs = (uint64_t(r) << 8) | uint64_t(sr);
+      const uint32_t* act = base + boff[sr];
+      const uint32_t ns = boff[sr + 1] - boff[sr];
+      if (ns == 0) continue;
       fill_async<int64_t>(w.counters.p + 3, 0, 1, s);
       {
         TimedRegion tr(c.prof, s, kFamLpGain, 0.0);
-        launch_gain(a, w, na, s, c.num_cus);
+        launch_gain(a, w, act, ns, s, c.num_cus);
       }
-      exclusive_scan_u32(c, w.cand_flag.p, w.pos.p, na);
+      exclusive_scan_u32(c, w.cand_flag.p, w.pos.p, ns);
       fill_async<int64_t>(w.in_w.p, 0, p->k, s);
-      lp_scatter_candidates_kernel<<<grid_for(na, 256, 65536), 256, 0, s>>>(
-          w.active.p, na, w.cand_flag.p, w.pos.p, w.cand_to_a.p, w.cand_gain_a.p, p->part.p, hg->vw.p, w.cand_u.p,
+      lp_scatter_candidates_kernel<<<grid_for(ns, 256, 65536), 256, 0, s>>>(
+          act, ns, w.cand_flag.p, w.pos.p, w.cand_to_a.p, w.cand_gain_a.p, p->part.p, hg->vw.p, w.cand_u.p,
           w.cand_from.p, w.cand_to.p, w.cand_gain.p, w.in_w.p);
       MKP_LAUNCH_CHECK();
       // read back: number of candidates, arrivals per block, block weights
       uint32_t nc32 = 0;
-      MKP_HIP(hipMemcpyAsync(&nc32, w.pos.p + na, 4, hipMemcpyDeviceToHost, s));
-      MKP_HIP(hipMemcpyAsync(host, w.in_w.p, size_t(p->k) * 8, hipMemcpyDeviceToHost, s));
+      MKP_HIP(hipMemcpyAsync(&nc32, w.pos.p + ns, 4, hipMemcpyDeviceToHost, s));
+      std::vector<int64_t> inw(p->k);
+      MKP_HIP(hipMemcpyAsync(inw.data(), w.in_w.p, size_t(p->k) * 8, hipMemcpyDeviceToHost, s));
       std::vector<int64_t> pw(p->k);
       MKP_HIP(hipMemcpyAsync(pw.data(), p->part_w.p, size_t(p->k) * 8, hipMemcpyDeviceToHost, s));
       c.sync();
@@ -539,7 +589,7 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats) 
       std::vector<uint8_t> over(p->k, 0);
       bool any_over = false;
       for (int b = 0; b < p->k; ++b) {
-        over[b] = pw[b] + host[b] > prm.max_part_weights[b];
+        over[b] = pw[b] + inw[b] > prm.max_part_weights[b];
         any_over |= over[b] != 0;
       }
       const uint8_t* admit = nullptr;
@@ -565,14 +615,16 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats) 
       }
       fill_async<int64_t>(w.counters.p, 0, 1, s);
       const unsigned gc = grid_for(nc, 256, 65536);
-      TimedRegion tr_apply(c.prof, s, kFamLpApply, 0.0);
-      lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, nullptr, nc, 0);
-      MKP_LAUNCH_CHECK();
-      lp_verify_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, w.cand_gain.p, admit, nc,
-                                          w.keep.p);
-      MKP_LAUNCH_CHECK();
-      lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, w.keep.p, nc, 1);
-      MKP_LAUNCH_CHECK();
+      {
+        TimedRegion tr_apply(c.prof, s, kFamLpApply, 0.0);
+        lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, nullptr, nc, 0);
+        MKP_LAUNCH_CHECK();
+        lp_verify_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, w.cand_gain.p, admit, nc,
+                                            w.keep.p);
+        MKP_LAUNCH_CHECK();
+        lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, w.keep.p, nc, 1);
+        MKP_LAUNCH_CHECK();
+      }
       MKP_HIP(hipMemcpyAsync(host, w.counters.p, 8, hipMemcpyDeviceToHost, s));
       c.sync();
       int64_t D = host[0];
@@ -587,7 +639,8 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats) 
         continue;
       }
       delta_total += D;
-      lp_activate_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, admit, w.keep.p, nc, prm.activation_threshold, w.flag.p);
+      lp_activate_kernel<<<grid_for(nc, 1, unsigned(c.num_cus) * 16), 64, 0, s>>>(
+          a, w.cand_u.p, admit, w.keep.p, nc, prm.activation_threshold, round_stamp, w.net_stamp.p, w.flag.p);
       MKP_LAUNCH_CHECK();
     }
     ++rounds;

@@ -56,15 +56,18 @@ def start_partition(n, k):
 
 
 def cpu_baseline(w, arrays, part, maxw, perfect, threads):
-    """The oracle's reference-semantics LP (asynchronous, OpenMP) on the same
-    hypergraph and starting partition."""
+    """One step of the oracle's reference-semantics path (asynchronous Louvain
+    and LP, OpenMP) on the same hypergraph and starting partition."""
     import oracle as orc
     k = w["k"]
     oh = orc.Hypergraph(arrays["n"], arrays["noff"], arrays["pins"], arrays["vw"], arrays["nw"])
+    g = orc.Graph(oh, orc.Graph.UNIFORM)
     t0 = time.perf_counter()
+    _, lst = g.louvain(sync=False, seed=w["seed"], threads=threads)
+    t1 = time.perf_counter()
     _, km1, st = oh.lp_refine(k, part, [maxw] * k, [perfect] * k, seed=w["seed"], sync=False, threads=threads)
-    lp_s = time.perf_counter() - t0
-    return dict(lp_s=lp_s, lp_visits=st["visits"], km1=km1)
+    t2 = time.perf_counter()
+    return dict(louvain_s=t1 - t0, lp_s=t2 - t1, pins=lst["arcs"] // 2 + st["visits"], km1=km1)
 
 
 def main():
@@ -97,9 +100,11 @@ def main():
     work = mtk_amd.Partition(dh, k)
 
     def step():
+        _, lst = dh.louvain(seed=w["seed"], want_communities=False)
         work.copy_parts_from(base)
         st = work.lp_refine([maxw] * k, [perfect] * k, seed=w["seed"])
-        return st["visits"], st
+        # Louvain work in pins: two arcs of the star expansion per pin
+        return lst["arcs_swept"] // 2 + st["visits"], (lst, st)
 
     for i in range(args.warmup):
         step()
@@ -137,11 +142,12 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
-        log("cpu baseline: oracle reference-semantics LP, %d threads" % threads)
+        log("cpu baseline: oracle reference-semantics Louvain + LP, %d threads" % threads)
         c = cpu_baseline(w, arrays, part0, maxw, perfect, threads)
-        cpu = dict(value=round(c["lp_visits"] / c["lp_s"], 1), unit="pins/s", cores=threads, kind="port",
-                   sample="one full LP refinement (5 rounds) of the same hypergraph and starting partition; "
-                          "%.2f s, km1 %d" % (c["lp_s"], c["km1"]))
+        cpu = dict(value=round(c["pins"] / (c["louvain_s"] + c["lp_s"]), 1), unit="pins/s", cores=threads,
+                   kind="port",
+                   sample="one step on the same hypergraph and starting partition: Louvain %.2f s + LP %.2f s "
+                          "(asynchronous restatement, OpenMP), LP km1 %d" % (c["louvain_s"], c["lp_s"], c["km1"]))
 
     value = work_units / wall
     out = {

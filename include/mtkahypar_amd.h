@@ -126,6 +126,30 @@ typedef struct {
  * synchronous sub-rounds. */
 MKP_API int mkp_lp_refine(mkp_partition* p, const mkp_lp_params* params, mkp_lp_stats* stats);
 
+/* ----------------------------------------------------------- Louvain --- */
+typedef struct {
+  int32_t edge_weight;       /* -1 hybrid (density rule of partitioner.cpp:62-72),
+                                0 uniform, 1 non_uniform, 2 degree (graph.cpp:46-73) */
+  int32_t max_passes;        /* p-max-louvain-pass-iterations (fast preset: 5) */
+  double min_move_fraction;  /* p-louvain-min-vertex-move-fraction (0.01) */
+  uint64_t sampling;         /* p-vertex-degree-sampling-threshold (200000) */
+  uint64_t seed;
+  int32_t sub_rounds;        /* buckets per round of the synchronous form */
+} mkp_louvain_params;
+typedef struct {
+  int64_t arcs_swept;  /* arcs of every local-moving round, all levels */
+  int64_t rounds;
+  int64_t levels;
+  int64_t moves;
+  double device_ms;
+} mkp_louvain_stats;
+/* run_parallel_louvain on the star expansion (or the graph itself when every
+ * net has two pins), restricted to the hypernodes and stored as the
+ * hypergraph's community ids: preprocess() of partitioner.cpp:106-126 with
+ * parallel_louvain.cpp:62-66.  communities (host, num_vertices) may be NULL. */
+MKP_API int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* params, int32_t* communities,
+                        mkp_louvain_stats* stats);
+
 /* ---------------------------------------------------------- generator -- */
 typedef struct {
   uint32_t num_vertices;

@@ -1,0 +1,772 @@
+// Louvain modularity local moving on the star expansion, graph contraction
+// and prolongation.
+//
+// Reference: ParallelLocalMovingModularity::localMoving and
+// computeMaxGainCluster (local_moving_modularity.cpp:61-130,
+// local_moving_modularity.h:87-137), Graph construction and contraction
+// (graph.cpp:36-78,128-268,290-446), local_moving_contract_recurse
+// (parallel_louvain.cpp:27-66).
+//
+// Level 0 is never materialised: the star expansion's arcs are read straight
+// from the hypergraph CSR (hypernode u -> net nodes N+e of its incident nets,
+// net node N+e -> its pins).  Coarser levels are explicit CSR graphs built by
+// a sort of (cluster(u), cluster(v)) keys.
+//
+// A round r is split into sub-rounds by hash(seed, level, r, u); within a
+// sub-round every node of the bucket picks the cluster of maximum modularity
+// gain on the state at the sub-round start (ties: first occurrence in the arc
+// order, i.e. the insertion order of the reference's sparse map), then all
+// moves are applied.  A singleton moves into another singleton only towards
+// the smaller cluster id.  Gains are FP64 with the reference's operation
+// order (w - (recip*volU)*vol), built without contraction into FMA.
+#include <cfloat>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce_by_key.hpp>
+
+#include "internal.h"
+
+namespace mkp {
+
+// ------------------------------------------------------------ graph views --
+struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
+  uint32_t N, M;
+  uint64_t P;
+  const uint32_t* voff;
+  const uint32_t* inc;
+  const uint32_t* noff;
+  const uint32_t* pins;
+  const int32_t* nw;
+  int wt;  // 0 uniform, 1 non_uniform, 2 degree
+  __device__ uint64_t num_nodes() const { return uint64_t(N) + M; }
+  __device__ uint64_t offset(uint64_t u) const { return u < N ? voff[u] : P + noff[u - N]; }
+  __device__ uint32_t degree(uint32_t u) const {
+    return u < N ? voff[u + 1] - voff[u] : noff[u - N + 1] - noff[u - N];
+  }
+  __device__ double weight(int32_t w, uint32_t size, uint32_t deg) const {
+    if (wt == 0) return double(w);
+    if (wt == 1) return double(w) / double(size);
+    return double(w) * (double(deg) / double(size));
+  }
+  __device__ void arc(uint32_t u, uint32_t i, uint32_t& head, double& w) const {
+    if (u < N) {
+      const uint32_t e = inc[voff[u] + i];
+      head = N + e;
+      w = wt == 0 ? double(nw[e]) : weight(nw[e], noff[e + 1] - noff[e], voff[u + 1] - voff[u]);
+    } else {
+      const uint32_t e = u - N;
+      const uint32_t v = pins[noff[e] + i];
+      head = v;
+      w = wt == 0 ? double(nw[e]) : weight(nw[e], noff[e + 1] - noff[e], voff[v + 1] - voff[v]);
+    }
+  }
+};
+
+struct PlainView {  // all nets have two pins (graph.cpp:401-446)
+  uint32_t N;
+  const uint32_t* voff;
+  const uint32_t* inc;
+  const uint32_t* noff;
+  const uint32_t* pins;
+  const int32_t* nw;
+  int wt;
+  __device__ uint64_t num_nodes() const { return N; }
+  __device__ uint64_t offset(uint64_t u) const { return voff[u]; }
+  __device__ uint32_t degree(uint32_t u) const { return voff[u + 1] - voff[u]; }
+  __device__ void arc(uint32_t u, uint32_t i, uint32_t& head, double& w) const {
+    const uint32_t e = inc[voff[u] + i];
+    const uint32_t p0 = pins[noff[e]], p1 = pins[noff[e] + 1];
+    head = p0 != u ? p0 : p1;
+    const double deg = double(voff[u + 1] - voff[u]);
+    w = wt == 0 ? double(nw[e]) : (wt == 1 ? double(nw[e]) / 2.0 : double(nw[e]) * (deg / 2.0));
+  }
+};
+
+struct ExplicitView {  // contracted levels
+  uint32_t n;
+  const uint64_t* idx;
+  const uint32_t* head;
+  const double* w;
+  __device__ uint64_t num_nodes() const { return n; }
+  __device__ uint64_t offset(uint64_t u) const { return idx[u]; }
+  __device__ uint32_t degree(uint32_t u) const { return uint32_t(idx[u + 1] - idx[u]); }
+  __device__ void arc(uint32_t u, uint32_t i, uint32_t& h, double& ww) const {
+    const uint64_t a = idx[u] + i;
+    h = head[a];
+    ww = w[a];
+  }
+};
+
+struct ExplicitGraph {
+  uint32_t n = 0;
+  uint64_t arcs = 0;
+  DevBuf<uint64_t> idx;
+  DevBuf<uint32_t> head;
+  DevBuf<double> w;
+  DevBuf<double> vol;
+  ExplicitView view() const { return ExplicitView{n, idx.p, head.p, w.p}; }
+};
+
+struct LvArgs {
+  uint64_t seed;
+  uint32_t level, round;
+  int buckets, bucket;
+  uint32_t sampling;
+  double recip;
+  const int32_t* C;
+  const double* cvol;
+  const int32_t* csize;
+  const double* nvol;
+  uint32_t* move_u;
+  int32_t* move_to;
+  uint32_t* move_count;
+};
+
+__device__ inline bool lv_in_bucket(const LvArgs& a, uint32_t u) {
+  return a.buckets <= 1 ||
+         hash4(a.seed, kLvBucketTag | (uint64_t(a.level) << 8) | uint64_t(a.round), u, 0) % uint64_t(a.buckets) ==
+             uint64_t(a.bucket);
+}
+
+__device__ inline uint32_t slot_hash(int32_t key, int bits) {
+  return (uint32_t(key) * 0x9E3779B1u) >> (32 - bits);
+}
+
+// best (gain, first) pair: larger gain, then smaller first occurrence
+__device__ inline bool better(double g, uint32_t f, double bg, uint32_t bf) {
+  return g > bg || (g == bg && f < bf);
+}
+
+__device__ inline void lv_decide(const LvArgs& a, uint32_t u, double wfrom, double best_gain, int32_t best_key) {
+  const int32_t from = a.C[u];
+  const double volU = a.nvol[u];
+  const double mult = a.recip * volU;
+  const double bg = wfrom - mult * (a.cvol[from] - volU);
+  if (best_key >= 0 && best_gain > bg) {
+    if (a.csize[from] == 1 && a.csize[best_key] == 1 && best_key > from) return;
+    const uint32_t slot = atomicAdd(a.move_count, 1u);
+    a.move_u[slot] = u;
+    a.move_to[slot] = best_key;
+  }
+}
+
+// ---------------------------------------------------------- move kernels --
+// G lanes per node, T-slot open-addressing map per group in LDS.
+template <class View, int G, int TBITS, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count) {
+  constexpr int T = 1 << TBITS;
+  constexpr int GPB = BLOCK / G;
+  __shared__ int32_t keys[GPB * T];
+  __shared__ double wsum[GPB * T];
+  __shared__ uint32_t first[GPB * T];
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  int32_t* K = keys + grp * T;
+  double* Wt = wsum + grp * T;
+  uint32_t* F = first + grp * T;
+  for (uint32_t base = blockIdx.x * GPB; base < count; base += gridDim.x * GPB) {
+    const uint32_t idx = base + grp;
+    const bool in = idx < count;  // lists hold exactly the nodes of this sub-round
+    const uint32_t u = in ? list[idx] : 0;
+    for (int t = lane; t < T; t += G) { K[t] = -1; Wt[t] = 0.0; F[t] = 0xffffffffu; }
+    __syncthreads();
+    if (in) {
+      const uint32_t deg = min(g.degree(u), a.sampling);
+      for (uint32_t i = lane; i < deg; i += G) {
+        uint32_t v;
+        double w;
+        g.arc(u, i, v, w);
+        const int32_t c = a.C[v];
+        uint32_t h = slot_hash(c, TBITS);
+        while (true) {
+          const int32_t old = atomicCAS(&K[h], -1, c);
+          if (old == -1 || old == c) break;
+          h = (h + 1) & (T - 1);
+        }
+        atomicAdd(&Wt[h], w);
+        atomicMin(&F[h], i);
+      }
+    }
+    __syncthreads();
+    if (in) {
+      const int32_t from = a.C[u];
+      const double mult = a.recip * a.nvol[u];
+      double wfrom = 0.0, bgain = -DBL_MAX;
+      uint32_t bfirst = 0xffffffffu;
+      int32_t bkey = -1;
+      for (int t = lane; t < T; t += G) {
+        const int32_t k = K[t];
+        if (k < 0) continue;
+        if (k == from) { wfrom = Wt[t]; continue; }
+        const double gain = Wt[t] - mult * a.cvol[k];
+        if (better(gain, F[t], bgain, bfirst)) { bgain = gain; bfirst = F[t]; bkey = k; }
+      }
+      for (int o = G / 2; o > 0; o >>= 1) {
+        const double og = __shfl_xor(bgain, o, G);
+        const uint32_t of = __shfl_xor(bfirst, o, G);
+        const int32_t ok = __shfl_xor(bkey, o, G);
+        wfrom += __shfl_xor(wfrom, o, G);
+        if (ok >= 0 && (bkey < 0 || better(og, of, bgain, bfirst))) { bgain = og; bfirst = of; bkey = ok; }
+      }
+      if (lane == 0) lv_decide(a, u, wfrom, bgain, bkey);
+    }
+    __syncthreads();
+  }
+}
+
+// One block per node, map in global scratch (table of `tsize` slots per block).
+template <class View>
+__global__ void __launch_bounds__(256)
+louvain_move_global_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count, int max_tbits,
+                           int32_t* __restrict__ gkeys, double* __restrict__ gw, uint32_t* __restrict__ gf) {
+  const uint64_t cap = uint64_t(1) << max_tbits;
+  int32_t* K = gkeys + uint64_t(blockIdx.x) * cap;
+  double* Wt = gw + uint64_t(blockIdx.x) * cap;
+  uint32_t* F = gf + uint64_t(blockIdx.x) * cap;
+  __shared__ double s_g[4];
+  __shared__ uint32_t s_f[4];
+  __shared__ int32_t s_k[4];
+  __shared__ double s_w[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
+    const uint32_t u = list[item];
+    const uint32_t deg = min(g.degree(u), a.sampling);
+    int tbits = 1;  // table of at least twice the (sampled) degree
+    while ((1u << tbits) < 2 * deg) ++tbits;
+    const uint32_t T = 1u << tbits;
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) { K[t] = -1; Wt[t] = 0.0; F[t] = 0xffffffffu; }
+    __threadfence_block();
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < deg; i += blockDim.x) {
+      uint32_t v;
+      double w;
+      g.arc(u, i, v, w);
+      const int32_t c = a.C[v];
+      uint32_t h = slot_hash(c, tbits);
+      while (true) {
+        const int32_t old = atomicCAS(&K[h], -1, c);
+        if (old == -1 || old == c) break;
+        h = (h + 1) & (T - 1);
+      }
+      atomicAdd(&Wt[h], w);
+      atomicMin(&F[h], i);
+    }
+    __threadfence_block();
+    __syncthreads();
+    const int32_t from = a.C[u];
+    const double mult = a.recip * a.nvol[u];
+    double wfrom = 0.0, bgain = -DBL_MAX;
+    uint32_t bfirst = 0xffffffffu;
+    int32_t bkey = -1;
+    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
+      const int32_t k = K[t];
+      if (k < 0) continue;
+      if (k == from) { wfrom = Wt[t]; continue; }
+      const double gain = Wt[t] - mult * a.cvol[k];
+      if (better(gain, F[t], bgain, bfirst)) { bgain = gain; bfirst = F[t]; bkey = k; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double og = __shfl_xor(bgain, o, 64);
+      const uint32_t of = __shfl_xor(bfirst, o, 64);
+      const int32_t ok = __shfl_xor(bkey, o, 64);
+      wfrom += __shfl_xor(wfrom, o, 64);
+      if (ok >= 0 && (bkey < 0 || better(og, of, bgain, bfirst))) { bgain = og; bfirst = of; bkey = ok; }
+    }
+    if (lane == 0) { s_g[wid] = bgain; s_f[wid] = bfirst; s_k[wid] = bkey; s_w[wid] = wfrom; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double wf = 0.0;
+      for (int x = 0; x < int(blockDim.x >> 6); ++x) {
+        wf += s_w[x];
+        if (s_k[x] >= 0 && (bkey < 0 || better(s_g[x], s_f[x], bgain, bfirst))) {
+          bgain = s_g[x]; bfirst = s_f[x]; bkey = s_k[x];
+        }
+      }
+      lv_decide(a, u, wf, bgain, bkey);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void louvain_apply_kernel(const uint32_t* __restrict__ mu, const int32_t* __restrict__ mto,
+                                     const uint32_t* __restrict__ mcount, const double* __restrict__ nvol,
+                                     int32_t* __restrict__ C, double* __restrict__ cvol, int32_t* __restrict__ csize,
+                                     uint32_t* __restrict__ moved_total) {
+  const uint32_t cnt = *mcount;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(moved_total, cnt);
+  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
+    const uint32_t u = mu[j];
+    const int32_t to = mto[j];
+    const int32_t from = C[u];
+    C[u] = to;
+    atomicAdd(&cvol[to], nvol[u]);
+    atomicAdd(&cvol[from], -nvol[u]);
+    atomicAdd(&csize[to], 1);
+    atomicAdd(&csize[from], -1);
+  }
+}
+
+// node volumes (graph.cpp:319-327), sequential sum in arc order per node
+template <class View>
+__global__ void node_volume_kernel(View g, uint32_t n, double* __restrict__ vol, uint32_t* __restrict__ deg_max) {
+  uint32_t dm = 0;
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const uint32_t d = g.degree(u);
+    double s = 0.0;
+    for (uint32_t i = 0; i < d; ++i) {
+      uint32_t v;
+      double w;
+      g.arc(u, i, v, w);
+      s += w;
+    }
+    vol[u] = s;
+    dm = max(dm, d);
+  }
+  for (int o = 32; o > 0; o >>= 1) dm = max(dm, __shfl_xor(dm, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(deg_max, dm);
+}
+
+template <class View>
+__global__ void max_degree_kernel(View g, uint32_t n, uint32_t* __restrict__ deg_max) {
+  uint32_t dm = 0;
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+    dm = max(dm, g.degree(u));
+  for (int o = 32; o > 0; o >>= 1) dm = max(dm, __shfl_xor(dm, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(deg_max, dm);
+}
+
+// degree classes: 0: <= 16, 1: <= 128, 2: <= 1024, 3: heavier; the work list
+// of a round is grouped by (class, bucket) so that every sub-round launch
+// covers exactly its own nodes.
+template <class View>
+__device__ inline int lv_slot(const View& g, const LvArgs& a, uint32_t u) {
+  const uint32_t d = min(g.degree(u), a.sampling);
+  const int c = d <= 16 ? 0 : (d <= 128 ? 1 : (d <= 1024 ? 2 : 3));
+  const int b = a.buckets <= 1 ? 0
+                               : int(hash4(a.seed, kLvBucketTag | (uint64_t(a.level) << 8) | uint64_t(a.round), u, 0) %
+                                     uint64_t(a.buckets));
+  return c * a.buckets + b;
+}
+
+// one node per thread; per-block LDS histograms keep global atomics to one
+// per (block, slot)
+template <class View>
+__global__ void __launch_bounds__(256)
+lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t h[4 * 256];
+  const int slots = 4 * a.buckets;
+  for (int i = threadIdx.x; i < slots; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u < n) atomicAdd(&h[lv_slot(g, a, u)], 1u);
+  __syncthreads();
+  for (int i = threadIdx.x; i < slots; i += blockDim.x)
+    if (h[i]) atomicAdd(&counts[i], h[i]);
+}
+
+template <class View>
+__global__ void __launch_bounds__(256)
+lv_scatter_kernel(View g, LvArgs a, uint32_t n, const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
+                  uint32_t* __restrict__ list) {
+  __shared__ uint32_t h[4 * 256];
+  __shared__ uint32_t base[4 * 256];
+  const int slots = 4 * a.buckets;
+  for (int i = threadIdx.x; i < slots; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  int slot = 0;
+  uint32_t local = 0;
+  if (u < n) {
+    slot = lv_slot(g, a, u);
+    local = atomicAdd(&h[slot], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < slots; i += blockDim.x)
+    if (h[i]) base[i] = offsets[i] + atomicAdd(&cursor[i], h[i]);
+  __syncthreads();
+  if (u < n) list[base[slot] + local] = u;
+}
+
+__global__ void lv_init_kernel(uint32_t n, const double* __restrict__ nvol, int32_t* __restrict__ C,
+                               double* __restrict__ cvol, int32_t* __restrict__ csize) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    C[u] = int32_t(u);
+    cvol[u] = nvol[u];
+    csize[u] = 1;
+  }
+}
+
+// ----------------------------------------------------------- contraction --
+__global__ void mark_used_kernel(const int32_t* __restrict__ C, uint32_t n, uint32_t* __restrict__ used) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) used[C[u]] = 1;
+}
+
+__global__ void remap_kernel(int32_t* __restrict__ C, uint32_t n, const uint32_t* __restrict__ map,
+                             const double* __restrict__ nvol, double* __restrict__ cvol) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const int32_t c = int32_t(map[C[u]]);
+    C[u] = c;
+    atomicAdd(&cvol[c], nvol[u]);
+  }
+}
+
+template <class View>
+__device__ inline uint32_t arc_source(const View& g, uint64_t a) {
+  uint64_t lo = 0, hi = g.num_nodes();  // largest u with offset(u) <= a
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (g.offset(mid) <= a) lo = mid; else hi = mid;
+  }
+  return uint32_t(lo);
+}
+
+template <class View>
+__global__ void emit_arcs_kernel(View g, uint64_t num_arcs, const int32_t* __restrict__ C, int bits,
+                                 uint64_t* __restrict__ keys, double* __restrict__ vals) {
+  const uint64_t sentinel = uint64_t(1) << (2 * bits);
+  for (uint64_t a = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; a < num_arcs;
+       a += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t u = arc_source(g, a);
+    uint32_t v;
+    double w;
+    g.arc(u, uint32_t(a - g.offset(u)), v, w);
+    const uint64_t cu = uint32_t(C[u]), cv = uint32_t(C[v]);
+    keys[a] = cu == cv ? sentinel : ((cu << bits) | cv);
+    vals[a] = w;
+  }
+}
+
+__global__ void coarse_csr_kernel(const uint64_t* __restrict__ ukeys, uint64_t ucount, int bits,
+                                  uint32_t* __restrict__ head, uint32_t* __restrict__ deg) {
+  const uint64_t mask = (uint64_t(1) << bits) - 1;
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < ucount;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    head[i] = uint32_t(ukeys[i] & mask);
+    atomicAdd(&deg[ukeys[i] >> bits], 1u);
+  }
+}
+
+__global__ void u32_to_u64_kernel(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, uint64_t n) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    out[i] = in[i];
+}
+
+__global__ void prolong_kernel(int32_t* __restrict__ C, uint32_t n, const int32_t* __restrict__ coarse) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) C[u] = coarse[C[u]];
+}
+
+__global__ void all_size_two_kernel(const uint32_t* __restrict__ noff, uint32_t m, uint32_t* __restrict__ flag) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x)
+    if (noff[e + 1] - noff[e] != 2) *flag = 0;
+}
+
+__global__ void sum_volume_kernel(const double* __restrict__ vol, uint32_t n, double* __restrict__ out) {
+  // fixed-shape reduction; exact for integer-valued volumes
+  __shared__ double part[256];
+  double s = 0.0;
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) s += vol[u];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = part[0];
+}
+
+// --------------------------------------------------------------- driver --
+struct LouvainRun {
+  Context& c;
+  const mkp_louvain_params& prm;
+  mkp_louvain_stats st{};
+  explicit LouvainRun(Context& ctx, const mkp_louvain_params& p) : c(ctx), prm(p) {}
+
+  double total_volume(const double* vol, uint32_t n) {
+    DevBuf<double> partial(64);
+    sum_volume_kernel<<<64, 256, 0, c.stream>>>(vol, n, partial.p);
+    MKP_LAUNCH_CHECK();
+    double h[64];
+    MKP_HIP(hipMemcpyAsync(h, partial.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    double s = 0.0;
+    for (double x : h) s += x;
+    return s;
+  }
+
+  // launches the four degree classes of sub-round `sr`; lists are grouped by
+  // slot = class * B + bucket with offsets off[] and sizes cnt[]
+  template <class View>
+  void launch_move(const View& g, const LvArgs& a, const uint32_t* list, const std::vector<uint32_t>& off,
+                   const std::vector<uint32_t>& cnt, int sr, int tbits_global, DevBuf<int32_t>& gk,
+                   DevBuf<double>& gw, DevBuf<uint32_t>& gf) {
+    const hipStream_t s = c.stream;
+    const int B = a.buckets;
+    const int s0 = 0 * B + sr, s1 = 1 * B + sr, s2 = 2 * B + sr, s3 = 3 * B + sr;
+    if (cnt[s0])
+      louvain_move_group_kernel<View, 8, 5, 256><<<grid_for(cnt[s0], 32, 65536), 256, 0, s>>>(g, a, list + off[s0],
+                                                                                             cnt[s0]);
+    if (cnt[s1])
+      louvain_move_group_kernel<View, 32, 8, 256><<<grid_for(cnt[s1], 8, 65536), 256, 0, s>>>(g, a, list + off[s1],
+                                                                                              cnt[s1]);
+    if (cnt[s2])
+      louvain_move_group_kernel<View, 64, 11, 64><<<grid_for(cnt[s2], 1, 65536), 64, 0, s>>>(g, a, list + off[s2],
+                                                                                             cnt[s2]);
+    if (cnt[s3])
+      louvain_move_global_kernel<View><<<unsigned(std::min<uint32_t>(cnt[s3], 128)), 256, 0, s>>>(
+          g, a, list + off[s3], cnt[s3], tbits_global, gk.p, gw.p, gf.p);
+    MKP_LAUNCH_CHECK();
+  }
+
+  // one level of local moving; returns whether any node moved
+  template <class View>
+  bool local_moving(const View& g, uint32_t n, uint64_t num_arcs, const double* nvol, double tv, uint32_t level,
+                    int32_t* C) {
+    const hipStream_t s = c.stream;
+    const int B = std::max(1, prm.sub_rounds);
+    const int slots = 4 * B;
+    DevBuf<double> cvol(std::max<uint32_t>(n, 1));
+    DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
+    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), counts(slots), cursor(slots), offsets(slots), mcount(1),
+        mu(std::max<uint32_t>(n, 1)), moved_total(1);
+    DevBuf<int32_t> mto(std::max<uint32_t>(n, 1));
+    lv_init_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, nvol, C, cvol.p, csize.p);
+    MKP_LAUNCH_CHECK();
+    if (num_arcs == 0) {
+      c.sync();
+      return false;
+    }
+    // global maps for the heaviest class, sized by the largest (sampled) degree
+    int tbits = 1;
+    DevBuf<int32_t> gk;
+    DevBuf<double> gw;
+    DevBuf<uint32_t> gf;
+    const uint64_t cap = std::min<uint64_t>(max_degree_, prm.sampling);
+    if (cap > 1024) {
+      while ((uint64_t(1) << tbits) < 2 * cap) ++tbits;
+      gk.alloc(uint64_t(128) << tbits);
+      gw.alloc(uint64_t(128) << tbits);
+      gf.alloc(uint64_t(128) << tbits);
+    }
+    LvArgs a;
+    a.seed = prm.seed; a.level = level; a.sampling = uint32_t(std::min<uint64_t>(prm.sampling, 0xffffffffu));
+    a.recip = 1.0 / tv; a.C = C; a.cvol = cvol.p; a.csize = csize.p; a.nvol = nvol;
+    a.move_u = mu.p; a.move_to = mto.p; a.move_count = mcount.p;
+    a.buckets = B;
+    bool changed = false;
+    uint64_t moved = n;
+    std::vector<uint32_t> cnt(slots), off(slots);
+    for (int r = 0; double(moved) >= prm.min_move_fraction * double(n) && r < prm.max_passes; ++r) {
+      a.round = uint32_t(r);
+      // group this round's nodes by (degree class, sub-round bucket)
+      fill_async<uint32_t>(counts.p, 0u, slots, s);
+      fill_async<uint32_t>(cursor.p, 0u, slots, s);
+      fill_async<uint32_t>(moved_total.p, 0u, 1, s);
+      lv_count_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, counts.p);
+      MKP_LAUNCH_CHECK();
+      MKP_HIP(hipMemcpyAsync(cnt.data(), counts.p, size_t(slots) * 4, hipMemcpyDeviceToHost, s));
+      c.sync();
+      uint32_t run = 0;
+      for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
+      MKP_HIP(hipMemcpyAsync(offsets.p, off.data(), size_t(slots) * 4, hipMemcpyHostToDevice, s));
+      lv_scatter_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, offsets.p, cursor.p, list.p);
+      MKP_LAUNCH_CHECK();
+      for (int sr = 0; sr < B; ++sr) {
+        a.bucket = sr;
+        fill_async<uint32_t>(mcount.p, 0u, 1, s);
+        TimedRegion tr(c.prof, s, kFamLouvainMove, 0.0);
+        launch_move(g, a, list.p, off, cnt, sr, tbits, gk, gw, gf);
+        louvain_apply_kernel<<<grid_for(n, 256, 4096), 256, 0, s>>>(mu.p, mto.p, mcount.p, nvol, C, cvol.p,
+                                                                   csize.p, moved_total.p);
+        MKP_LAUNCH_CHECK();
+      }
+      uint32_t mt = 0;
+      MKP_HIP(hipMemcpyAsync(&mt, moved_total.p, 4, hipMemcpyDeviceToHost, s));
+      c.sync();
+      moved = mt;
+      st.arcs_swept += int64_t(num_arcs);
+      st.rounds += 1;
+      st.moves += int64_t(moved);
+      if (c.prof.enabled) {
+        // level 0: 20P + 28N + 32E (per pin: two head ids, two cluster
+        // gathers, a net weight; per node: offsets, volume, cluster, cluster
+        // volume, write); coarse levels: 16 per arc + 28 per node
+        c.prof.bytes[kFamLouvainMove] += level == 0 ? 10.0 * double(num_arcs) + 28.0 * double(n)
+                                                    : 16.0 * double(num_arcs) + 28.0 * double(n);
+      }
+      changed |= moved > 0;
+    }
+    return changed;
+  }
+
+  uint32_t max_degree_ = 0;
+
+  // level 0: node volumes from the arcs; every level: maximum degree
+  template <class View>
+  void prepare(const View& g, uint32_t n, double* nvol) {
+    DevBuf<uint32_t> dmax(1);
+    fill_async<uint32_t>(dmax.p, 0u, 1, c.stream);
+    if (nvol)
+      node_volume_kernel<View><<<grid_for(n, 256, 65536), 256, 0, c.stream>>>(g, n, nvol, dmax.p);
+    else
+      max_degree_kernel<View><<<grid_for(n, 256, 65536), 256, 0, c.stream>>>(g, n, dmax.p);
+    MKP_LAUNCH_CHECK();
+    MKP_HIP(hipMemcpyAsync(&max_degree_, dmax.p, 4, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  }
+
+  // Graph::contract; C is remapped to coarse ids in place
+  template <class View>
+  ExplicitGraph contract(const View& g, uint32_t n, uint64_t num_arcs, const double* nvol, int32_t* C) {
+    const hipStream_t s = c.stream;
+    TimedRegion tr(c.prof, s, kFamLouvainContract, (12.0 + 4.0 + 12.0) * double(num_arcs) + 20.0 * n);
+    ExplicitGraph cg;
+    DevBuf<uint32_t> used(std::max<uint32_t>(n, 1)), map(size_t(n) + 1);
+    fill_async<uint32_t>(used.p, 0u, n, s);
+    mark_used_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(C, n, used.p);
+    MKP_LAUNCH_CHECK();
+    exclusive_scan_u32(c, used.p, map.p, n);
+    uint32_t nc = 0;
+    MKP_HIP(hipMemcpyAsync(&nc, map.p + n, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    cg.n = nc;
+    cg.vol.alloc(std::max<uint32_t>(nc, 1));
+    fill_async<double>(cg.vol.p, 0.0, nc, s);
+    remap_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(C, n, map.p, nvol, cg.vol.p);
+    MKP_LAUNCH_CHECK();
+    int bits = 1;
+    while ((uint64_t(1) << bits) < nc) ++bits;
+    MKP_CHECK(2 * bits + 1 <= 64, "graph too large for 64-bit arc keys");
+    cg.idx.alloc(size_t(nc) + 1);
+    if (num_arcs == 0 || nc <= 1) {
+      fill_async<uint64_t>(cg.idx.p, 0, size_t(nc) + 1, s);
+      cg.head.alloc(1);
+      cg.w.alloc(1);
+      cg.arcs = 0;
+      c.sync();
+      return cg;
+    }
+    DevBuf<uint64_t> keys(num_arcs), skeys(num_arcs), ukeys(num_arcs);
+    DevBuf<double> vals(num_arcs), svals(num_arcs), uvals(num_arcs);
+    emit_arcs_kernel<View><<<grid_for(num_arcs, 256, 65536), 256, 0, s>>>(g, num_arcs, C, bits, keys.p, vals.p);
+    MKP_LAUNCH_CHECK();
+    size_t bytes = 0;
+    MKP_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys.p, skeys.p, vals.p, svals.p, num_arcs, 0u,
+                                      unsigned(2 * bits + 1), s));
+    MKP_HIP(rocprim::radix_sort_pairs(c.temp(bytes), bytes, keys.p, skeys.p, vals.p, svals.p, num_arcs, 0u,
+                                      unsigned(2 * bits + 1), s));
+    DevBuf<uint64_t> ucount(1);
+    bytes = 0;
+    MKP_HIP(rocprim::reduce_by_key(nullptr, bytes, skeys.p, svals.p, num_arcs, ukeys.p, uvals.p, ucount.p,
+                                   rocprim::plus<double>(), rocprim::equal_to<uint64_t>(), s));
+    MKP_HIP(rocprim::reduce_by_key(c.temp(bytes), bytes, skeys.p, svals.p, num_arcs, ukeys.p, uvals.p, ucount.p,
+                                   rocprim::plus<double>(), rocprim::equal_to<uint64_t>(), s));
+    uint64_t U = 0, last = 0;
+    MKP_HIP(hipMemcpyAsync(&U, ucount.p, 8, hipMemcpyDeviceToHost, s));
+    c.sync();
+    if (U) {
+      MKP_HIP(hipMemcpyAsync(&last, ukeys.p + (U - 1), 8, hipMemcpyDeviceToHost, s));
+      c.sync();
+      if (last == (uint64_t(1) << (2 * bits))) --U;  // self-loop sentinel segment
+    }
+    cg.arcs = U;
+    cg.head.alloc(std::max<uint64_t>(U, 1));
+    cg.w.alloc(std::max<uint64_t>(U, 1));
+    DevBuf<uint32_t> deg(std::max<uint32_t>(nc, 1));
+    DevBuf<uint64_t> deg64(std::max<uint32_t>(nc, 1));
+    fill_async<uint32_t>(deg.p, 0u, nc, s);
+    if (U) {
+      coarse_csr_kernel<<<grid_for(U, 256, 65536), 256, 0, s>>>(ukeys.p, U, bits, cg.head.p, deg.p);
+      MKP_LAUNCH_CHECK();
+      MKP_HIP(hipMemcpyAsync(cg.w.p, uvals.p, U * 8, hipMemcpyDeviceToDevice, s));
+    }
+    u32_to_u64_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(deg.p, deg64.p, nc);
+    MKP_LAUNCH_CHECK();
+    exclusive_scan_u64(c, deg64.p, cg.idx.p, nc);
+    c.sync();
+    return cg;
+  }
+
+  // local_moving_contract_recurse (parallel_louvain.cpp:27-60), iterative
+  template <class View>
+  void run_from(const View& g0, uint32_t n0, uint64_t arcs0, int32_t* C0) {
+    const hipStream_t s = c.stream;
+    DevBuf<double> vol0(std::max<uint32_t>(n0, 1));
+    prepare(g0, n0, vol0.p);
+    const double tv = total_volume(vol0.p, n0);
+    std::vector<ExplicitGraph> levels;
+    std::vector<DevBuf<int32_t>> cl;  // clustering of each contracted level
+    bool changed = local_moving(g0, n0, arcs0, vol0.p, tv, 0, C0);
+    st.levels = 1;
+    if (changed) {
+      levels.push_back(contract(g0, n0, arcs0, vol0.p, C0));
+      while (true) {
+        ExplicitGraph& g = levels.back();
+        cl.emplace_back(std::max<uint32_t>(g.n, 1));
+        int32_t* C = cl.back().p;
+        prepare(g.view(), g.n, nullptr);  // volumes are the summed fine volumes of the contraction
+        bool ch = local_moving(g.view(), g.n, g.arcs, g.vol.p, tv, uint32_t(levels.size()), C);
+        st.levels += 1;
+        if (!ch) break;
+        levels.push_back(contract(g.view(), g.n, g.arcs, g.vol.p, C));
+      }
+      // prolong from the deepest clustering back to level 0
+      for (int i = int(cl.size()) - 1; i >= 1; --i)
+        prolong_kernel<<<grid_for(levels[i - 1].n, 256, 65536), 256, 0, s>>>(cl[i - 1].p, levels[i - 1].n, cl[i].p);
+      prolong_kernel<<<grid_for(n0, 256, 65536), 256, 0, s>>>(C0, n0, cl[0].p);
+      MKP_LAUNCH_CHECK();
+    }
+    c.sync();
+  }
+};
+
+}  // namespace mkp
+
+using namespace mkp;
+
+extern "C" int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* prm, int32_t* communities,
+                           mkp_louvain_stats* stats) {
+  return guarded([&] {
+    MKP_CHECK(hg && prm, "null argument");
+    MKP_CHECK(prm->max_passes >= 0 && prm->sub_rounds >= 1 && prm->sub_rounds <= 255, "bad round parameters");
+    Context& c = hg->ctx->c;
+    MKP_HIP(hipSetDevice(c.device));
+    MKP_HIP(hipEventRecord(c.ev_a, c.stream));
+    // configurePreprocessing (partitioner.cpp:62-72): hybrid -> degree if
+    // |E|/|V| < 0.75, else uniform
+    int wt = prm->edge_weight;
+    if (wt < 0) wt = (double(hg->m) / double(std::max<uint32_t>(hg->n, 1)) < 0.75) ? 2 : 0;
+    LouvainRun run(c, *prm);
+    DevBuf<uint32_t> flag(1);
+    fill_async<uint32_t>(flag.p, 1u, 1, c.stream);
+    if (hg->m) {
+      all_size_two_kernel<<<grid_for(hg->m, 256, 65536), 256, 0, c.stream>>>(hg->noff.p, hg->m, flag.p);
+      MKP_LAUNCH_CHECK();
+    }
+    uint32_t is_graph = 0;
+    MKP_HIP(hipMemcpyAsync(&is_graph, flag.p, 4, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    is_graph = is_graph && hg->m > 0;
+    const uint32_t nodes = is_graph ? hg->n : hg->n + hg->m;
+    DevBuf<int32_t> C(std::max<uint32_t>(nodes, 1));
+    if (is_graph) {
+      PlainView g{hg->n, hg->voff.p, hg->inc.p, hg->noff.p, hg->pins.p, hg->nw.p, wt};
+      run.run_from(g, hg->n, hg->p, C.p);
+    } else {
+      StarView g{hg->n, hg->m, hg->p, hg->voff.p, hg->inc.p, hg->noff.p, hg->pins.p, hg->nw.p, wt};
+      run.run_from(g, nodes, 2 * hg->p, C.p);
+    }
+    // restrictClusteringToHypernodes + setCommunityIDs (graph.h:123-125,
+    // partitioner.cpp:116-117)
+    MKP_HIP(hipMemcpyAsync(hg->comm.p, C.p, size_t(hg->n) * 4, hipMemcpyDeviceToDevice, c.stream));
+    if (communities)
+      MKP_HIP(hipMemcpyAsync(communities, C.p, size_t(hg->n) * 4, hipMemcpyDeviceToHost, c.stream));
+    MKP_HIP(hipEventRecord(c.ev_b, c.stream));
+    c.sync();
+    if (stats) {
+      *stats = run.st;
+      float ms = 0;
+      MKP_HIP(hipEventElapsedTime(&ms, c.ev_a, c.ev_b));
+      stats->device_ms = ms;
+    }
+  });
+}

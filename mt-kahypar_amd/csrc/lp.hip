@@ -458,13 +458,22 @@ __global__ void iota_kernel(uint32_t* out, uint32_t n) {
 }
 
 // --------------------------------------------------------------- driver --
-__global__ void lp_bucket_keys_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na,
-                                      uint64_t* __restrict__ keys, int64_t* __restrict__ count) {
+// bucket key per active entry; per-block LDS histograms, one global atomic per
+// (block, bucket)
+__global__ void __launch_bounds__(256)
+lp_bucket_keys_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na, uint64_t* __restrict__ keys,
+                      int64_t* __restrict__ count) {
+  __shared__ uint32_t h[256];
+  for (int i = threadIdx.x; i < a.buckets; i += blockDim.x) h[i] = 0;
+  __syncthreads();
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
     const uint64_t b = hash4(a.seed, kLpBucketTag | uint64_t(a.round), active[i], 0) % uint64_t(a.buckets);
     keys[i] = b;
-    atomicAdd(reinterpret_cast<unsigned long long*>(&count[b]), 1ull);
+    atomicAdd(&h[b], 1u);
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < a.buckets; i += blockDim.x)
+    if (h[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&count[i]), static_cast<unsigned long long>(h[i]));
 }
 
 // --------------------------------------------------------------- driver --

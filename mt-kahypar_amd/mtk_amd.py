@@ -32,6 +32,16 @@ class LpStats(C.Structure):
                 ("rounds", C.c_int64), ("device_ms", C.c_double)]
 
 
+class LouvainParams(C.Structure):
+    _fields_ = [("edge_weight", C.c_int32), ("max_passes", C.c_int32), ("min_move_fraction", C.c_double),
+                ("sampling", C.c_uint64), ("seed", C.c_uint64), ("sub_rounds", C.c_int32)]
+
+
+class LouvainStats(C.Structure):
+    _fields_ = [("arcs_swept", C.c_int64), ("rounds", C.c_int64), ("levels", C.c_int64), ("moves", C.c_int64),
+                ("device_ms", C.c_double)]
+
+
 class PowerlawParams(C.Structure):
     _fields_ = [("num_vertices", C.c_uint32), ("num_nets", C.c_uint32), ("exponent", C.c_double),
                 ("min_size", C.c_uint32), ("max_size", C.c_uint32), ("window", C.c_uint32),
@@ -72,6 +82,7 @@ SIGNATURES = {
     "mkp_partition_get_border": (C.c_int, [vp, vp]),
     "mkp_partition_km1": (C.c_int, [vp, P(C.c_int64)]),
     "mkp_lp_refine": (C.c_int, [vp, P(LpParams), P(LpStats)]),
+    "mkp_louvain": (C.c_int, [vp, P(LouvainParams), vp, P(LouvainStats)]),
     "mkp_generate_powerlaw": (C.c_int, [P(PowerlawParams), P(vp)]),
     "mkp_generate_rownet": (C.c_int, [P(RownetParams), P(vp)]),
     "mkp_host_hypergraph_info": (C.c_int, [vp, P(HypergraphInfo)]),
@@ -205,6 +216,18 @@ class Hypergraph:
     def set_communities(self, comm):
         c = np.ascontiguousarray(comm, dtype=np.int32)
         check(lib().mkp_hypergraph_set_communities(self.h, _p(c)))
+
+    UNIFORM, NON_UNIFORM, DEGREE, HYBRID = 0, 1, 2, -1
+
+    def louvain(self, edge_weight=-1, max_passes=5, min_move_fraction=0.01, sampling=200000, seed=0,
+                sub_rounds=8, want_communities=True):
+        prm = LouvainParams(edge_weight, max_passes, min_move_fraction, sampling, seed, sub_rounds)
+        st = LouvainStats()
+        out = np.zeros(max(self.info().num_vertices, 1), np.int32) if want_communities else None
+        check(lib().mkp_louvain(self.h, C.byref(prm), _p(out), C.byref(st)))
+        stats = dict(arcs_swept=st.arcs_swept, rounds=st.rounds, levels=st.levels, moves=st.moves,
+                     device_ms=st.device_ms)
+        return (out[:self.info().num_vertices] if out is not None else None), stats
 
     def arrays(self):
         i = self.info()

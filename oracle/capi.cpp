@@ -16,6 +16,12 @@ namespace {
 double now() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
+// clusterings handed in from Python index per-node arrays: reject ids out of range
+bool ids_in_range(const int32_t* C, uint32_t n) {
+  for (uint32_t u = 0; u < n; ++u)
+    if (C[u] < 0 || uint32_t(C[u]) >= n) return false;
+  return true;
+}
 mt_kahypar_amd::HostHypergraph to_host(const Hypergraph& hg) {
   mt_kahypar_amd::HostHypergraph h;
   h.n = hg.n; h.m = hg.m; h.noff = hg.noff; h.pins = hg.pins; h.voff = hg.voff; h.inc = hg.inc;
@@ -118,6 +124,7 @@ void orc_graph_get(void* gp, uint64_t* idx, uint32_t* head, double* w, double* v
 }
 void* orc_graph_contract(void* gp, int32_t* C) {
   auto* g = static_cast<Graph*>(gp);
+  if (!ids_in_range(C, g->n)) return nullptr;
   std::vector<int32_t> c(C, C + g->n);
   Graph* cg = new Graph(contract_graph(*g, c));
   std::copy(c.begin(), c.end(), C);
@@ -125,11 +132,13 @@ void* orc_graph_contract(void* gp, int32_t* C) {
 }
 double orc_modularity(void* gp, const int32_t* C) {
   auto* g = static_cast<Graph*>(gp);
+  if (!ids_in_range(C, g->n)) return std::nan("");
   return modularity(*g, std::vector<int32_t>(C, C + g->n));
 }
 // cluster volumes are initialised from C as in initializeClusterVolumes
 int32_t orc_louvain_best_cluster(void* gp, const int32_t* C, uint32_t u, uint64_t sampling, double* best_gain) {
   auto* g = static_cast<Graph*>(gp);
+  if (!ids_in_range(C, g->n) || u >= g->n) return -1;
   std::vector<int32_t> c(C, C + g->n);
   std::vector<double> cvol(g->n, 0.0);
   for (uint32_t x = 0; x < g->n; ++x) cvol[c[x]] += g->vol[x];

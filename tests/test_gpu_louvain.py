@@ -1,0 +1,108 @@
+"""Parity of the HIP Louvain (star expansion, all levels) with the oracle's
+synchronous restatement.  With integer-valued arc weights every FP64 sum is
+exact, so community ids and round statistics are compared bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+import mtk_amd
+import oracle as orc
+from helpers import GOLDEN, csr, read_hgr, vectors
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = mtk_amd.Context(0)
+    yield c
+    c.close()
+
+
+def powerlaw(n, m, max_size, seed, window=512, exponent=2.3, heavy_nets=0):
+    h = mtk_amd.HostHypergraph.powerlaw(n, m, exponent, 2, max_size, window=window, local_fraction=0.8, seed=seed)
+    a = h.arrays()
+    h.close()
+    nets = [a["pins"][a["noff"][e]:a["noff"][e + 1]].tolist() for e in range(len(a["noff"]) - 1)]
+    rng = np.random.default_rng(seed)
+    for _ in range(heavy_nets):  # net nodes of degree > 1024 take the global-map kernel
+        nets.append(sorted(rng.choice(n, size=1500, replace=False).tolist()))
+    noff, pins = csr(nets)
+    return n, noff, pins
+
+
+def compare(ctx, n, noff, pins, nw=None, weight=0, seed=3, sub_rounds=8, sampling=200000, max_passes=5,
+            min_frac=0.01):
+    dh = mtk_amd.Hypergraph(ctx, n, noff, pins, None, nw)
+    comm, st = dh.louvain(edge_weight=weight, seed=seed, sub_rounds=sub_rounds, sampling=sampling,
+                          max_passes=max_passes, min_move_fraction=min_frac)
+    oh = orc.Hypergraph(n, noff, pins, None, nw)
+    g = orc.Graph(oh, weight)
+    C, ost = g.louvain(sync=True, seed=seed, sub_rounds=sub_rounds, sampling=sampling, max_passes=max_passes,
+                       min_frac=min_frac)
+    assert np.array_equal(dh.arrays()["comm"], comm)
+    dh.close()
+    return comm, C[:n], st, ost
+
+
+def test_fixture_star_expansion(ctx):
+    fx = vectors()["fixture_hypergraph"]
+    noff, pins = csr(fx["nets"])
+    comm, ref, st, ost = compare(ctx, fx["n"], noff, pins, sub_rounds=1)
+    assert comm.tolist() == ref.tolist()
+    assert (st["arcs_swept"], st["rounds"], st["levels"], st["moves"]) == \
+        (ost["arcs"], ost["rounds"], ost["levels"], ost["moves"])
+
+
+def test_karate_club_plain_graph_path(ctx):
+    n, noff, pins, vw, nw = read_hgr(os.path.join(GOLDEN, vectors()["karate_club"]["instance"]))
+    for sub_rounds in (1, 4):
+        comm, ref, st, ost = compare(ctx, n, noff, pins, nw=nw, sub_rounds=sub_rounds, max_passes=100,
+                                     min_frac=0.0001)
+        assert comm.tolist() == ref.tolist()
+        assert st["levels"] == ost["levels"] and st["moves"] == ost["moves"]
+
+
+@pytest.mark.parametrize("seed,sub_rounds", [(1, 1), (2, 4), (3, 8), (4, 16)])
+def test_powerlaw_bit_exact(ctx, seed, sub_rounds):
+    n, noff, pins = powerlaw(6000, 9000, 400, seed, heavy_nets=3)
+    comm, ref, st, ost = compare(ctx, n, noff, pins, seed=seed, sub_rounds=sub_rounds)
+    assert comm.tolist() == ref.tolist()
+    assert (st["arcs_swept"], st["rounds"], st["levels"], st["moves"]) == \
+        (ost["arcs"], ost["rounds"], ost["levels"], ost["moves"])
+    assert st["levels"] >= 2
+
+
+def test_sampling_threshold(ctx):
+    n, noff, pins = powerlaw(3000, 4000, 300, 7, heavy_nets=2)
+    comm, ref, st, ost = compare(ctx, n, noff, pins, sampling=12)
+    assert comm.tolist() == ref.tolist()
+
+
+def test_weighted_nets_uniform_weight_function(ctx):
+    n, noff, pins = powerlaw(4000, 5000, 100, 9)
+    nw = np.random.default_rng(9).integers(1, 5, len(noff) - 1).astype(np.int32)
+    comm, ref, st, ost = compare(ctx, n, noff, pins, nw=nw)
+    assert comm.tolist() == ref.tolist()
+
+
+@pytest.mark.parametrize("weight", [1, 2])
+def test_fractional_weights_modularity(ctx, weight):
+    """non_uniform / degree arc weights are fractional: sums depend on the
+    order of atomic additions, so only the quality is compared."""
+    n, noff, pins = powerlaw(4000, 2500, 60, 11)
+    comm, ref, st, ost = compare(ctx, n, noff, pins, weight=weight)
+    oh = orc.Hypergraph(n, noff, pins)
+    g = orc.Graph(oh, weight)
+    nn = g.sizes()[0]
+    # evaluate both hypernode clusterings on the star expansion with every
+    # net node a singleton under its own node id (ids >= n, unused by the
+    # hypernode communities, which are < n)
+    def q(c):
+        assert c.max() < n
+        full = np.concatenate([c, np.arange(n, nn, dtype=np.int32)]) if nn > n else c
+        return g.modularity(full.astype(np.int32))
+    assert abs(q(comm) - q(ref)) < 0.02
+    nc, nref = len(set(comm.tolist())), len(set(ref.tolist()))
+    assert abs(nc - nref) <= 0.2 * nref + 2

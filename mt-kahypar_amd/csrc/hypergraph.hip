@@ -14,7 +14,7 @@ void set_last_error(const std::string& s) { g_last_error = s; }
 // pin position in the net offsets.
 __global__ void incidence_keys_kernel(const uint32_t* __restrict__ noff, const uint32_t* __restrict__ pins,
                                       uint32_t m, uint64_t p, uint64_t* __restrict__ keys,
-                                      uint32_t* __restrict__ degree) {
+                                      uint32_t* __restrict__ degree, uint32_t* __restrict__ pin_net) {
   for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < p; i += uint64_t(gridDim.x) * blockDim.x) {
     uint32_t lo = 0, hi = m;  // largest e with noff[e] <= i
     while (hi - lo > 1) {
@@ -23,37 +23,55 @@ __global__ void incidence_keys_kernel(const uint32_t* __restrict__ noff, const u
     }
     const uint32_t v = pins[i];
     keys[i] = (uint64_t(v) << 32) | lo;
+    pin_net[i] = lo;
     atomicAdd(&degree[v], 1u);
   }
 }
 
-__global__ void low_word_kernel(const uint64_t* __restrict__ keys, uint32_t* __restrict__ out, uint64_t p) {
-  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < p; i += uint64_t(gridDim.x) * blockDim.x)
-    out[i] = uint32_t(keys[i]);
+__global__ void split_words_kernel(const uint64_t* __restrict__ keys, uint32_t* __restrict__ lo_out,
+                                   uint32_t* __restrict__ hi_out, uint64_t p) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < p; i += uint64_t(gridDim.x) * blockDim.x) {
+    lo_out[i] = uint32_t(keys[i]);
+    hi_out[i] = uint32_t(keys[i] >> 32);
+  }
+}
+
+__global__ void unit_weight_kernel(const int32_t* __restrict__ nw, uint32_t m, uint32_t* __restrict__ flag) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x)
+    if (nw[e] != 1) *flag = 0;
 }
 
 void build_incidence(mkp_hypergraph* hg) {
   Context& c = hg->ctx->c;
   hg->voff.alloc(size_t(hg->n) + 1);
   hg->inc.alloc(std::max<uint64_t>(hg->p, 1));
-  uint32_t* degree = c.scratch_as<uint32_t>(0, size_t(hg->n) + 1);
-  fill_async<uint32_t>(degree, 0u, hg->n, c.stream);
+  hg->inc_vtx.alloc(std::max<uint64_t>(hg->p, 1));
+  hg->pin_net.alloc(std::max<uint64_t>(hg->p, 1));
+  DevBuf<uint32_t> degree(size_t(hg->n) + 1), flag(1);
+  fill_async<uint32_t>(degree.p, 0u, hg->n, c.stream);
+  fill_async<uint32_t>(flag.p, 1u, 1, c.stream);
+  if (hg->m) {
+    unit_weight_kernel<<<grid_for(hg->m, 256, 65536), 256, 0, c.stream>>>(hg->nw.p, hg->m, flag.p);
+    MKP_LAUNCH_CHECK();
+  }
   if (hg->p > 0) {
     DevBuf<uint64_t> keys(hg->p), sorted(hg->p);
     incidence_keys_kernel<<<grid_for(hg->p, 256, 65536), 256, 0, c.stream>>>(hg->noff.p, hg->pins.p, hg->m, hg->p,
-                                                                            keys.p, degree);
+                                                                            keys.p, degree.p, hg->pin_net.p);
     MKP_LAUNCH_CHECK();
     int pin_bits = 1;
     while ((uint64_t(1) << pin_bits) < hg->n) ++pin_bits;
     sort_keys_u64(c, keys.p, sorted.p, hg->p, 0, 32 + pin_bits);
-    low_word_kernel<<<grid_for(hg->p, 256, 65536), 256, 0, c.stream>>>(sorted.p, hg->inc.p, hg->p);
+    split_words_kernel<<<grid_for(hg->p, 256, 65536), 256, 0, c.stream>>>(sorted.p, hg->inc.p, hg->inc_vtx.p, hg->p);
     MKP_LAUNCH_CHECK();
-    exclusive_scan_u32(c, degree, hg->voff.p, hg->n);
-    c.sync();
+    exclusive_scan_u32(c, degree.p, hg->voff.p, hg->n);
   } else {
     fill_async<uint32_t>(hg->voff.p, 0u, size_t(hg->n) + 1, c.stream);
-    c.sync();
   }
+  uint32_t unit = 0;
+  MKP_HIP(hipMemcpyAsync(&unit, flag.p, 4, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  hg->unit_net_weights = unit != 0;
 }
 
 PcLayout pc_layout(int k, uint32_t max_esize) {

@@ -17,7 +17,10 @@ struct mkp_hypergraph {
   uint32_t max_esize = 0;
   int64_t total_w = 0;
   mkp::DevBuf<uint32_t> noff, pins, voff, inc;
+  mkp::DevBuf<uint32_t> pin_net;  // net of every pin position
+  mkp::DevBuf<uint32_t> inc_vtx;  // vertex of every incidence position
   mkp::DevBuf<int32_t> vw, nw, comm;
+  bool unit_net_weights = false;
 };
 
 namespace mkp {
@@ -56,6 +59,7 @@ void sort_pairs_u64_u32(Context& c, const uint64_t* kin, uint64_t* kout, const u
 void exclusive_scan_u32(Context& c, const uint32_t* in, uint32_t* out, size_t n);
 void exclusive_scan_u64(Context& c, const uint64_t* in, uint64_t* out, size_t n);
 void inclusive_scan_i64(Context& c, const int64_t* in, int64_t* out, size_t n);
+void inclusive_max_scan_u64(Context& c, const uint64_t* in, uint64_t* out, size_t n);
 
 // Builds voff/inc of hg from noff/pins on the device.
 void build_incidence(mkp_hypergraph* hg);

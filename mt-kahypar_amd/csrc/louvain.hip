@@ -29,6 +29,9 @@
 namespace mkp {
 
 // ------------------------------------------------------------ graph views --
+// Every view exposes degree(u), arc(u, i) and source(a) (the tail of arc a in
+// the global arc order).  `unit` marks hypergraphs whose net weights are all
+// one, so the uniform arc weight needs no gather.
 struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
   uint32_t N, M;
   uint64_t P;
@@ -36,13 +39,17 @@ struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
   const uint32_t* inc;
   const uint32_t* noff;
   const uint32_t* pins;
+  const uint32_t* inc_vtx;
+  const uint32_t* pin_net;
   const int32_t* nw;
   int wt;  // 0 uniform, 1 non_uniform, 2 degree
+  bool unit;
   __device__ uint64_t num_nodes() const { return uint64_t(N) + M; }
   __device__ uint64_t offset(uint64_t u) const { return u < N ? voff[u] : P + noff[u - N]; }
   __device__ uint32_t degree(uint32_t u) const {
     return u < N ? voff[u + 1] - voff[u] : noff[u - N + 1] - noff[u - N];
   }
+  __device__ uint32_t source(uint64_t a) const { return a < P ? inc_vtx[a] : N + pin_net[a - P]; }
   __device__ double weight(int32_t w, uint32_t size, uint32_t deg) const {
     if (wt == 0) return double(w);
     if (wt == 1) return double(w) / double(size);
@@ -52,12 +59,12 @@ struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
     if (u < N) {
       const uint32_t e = inc[voff[u] + i];
       head = N + e;
-      w = wt == 0 ? double(nw[e]) : weight(nw[e], noff[e + 1] - noff[e], voff[u + 1] - voff[u]);
+      w = wt == 0 ? (unit ? 1.0 : double(nw[e])) : weight(nw[e], noff[e + 1] - noff[e], voff[u + 1] - voff[u]);
     } else {
       const uint32_t e = u - N;
       const uint32_t v = pins[noff[e] + i];
       head = v;
-      w = wt == 0 ? double(nw[e]) : weight(nw[e], noff[e + 1] - noff[e], voff[v + 1] - voff[v]);
+      w = wt == 0 ? (unit ? 1.0 : double(nw[e])) : weight(nw[e], noff[e + 1] - noff[e], voff[v + 1] - voff[v]);
     }
   }
 };
@@ -68,17 +75,21 @@ struct PlainView {  // all nets have two pins (graph.cpp:401-446)
   const uint32_t* inc;
   const uint32_t* noff;
   const uint32_t* pins;
+  const uint32_t* inc_vtx;
   const int32_t* nw;
   int wt;
+  bool unit;
   __device__ uint64_t num_nodes() const { return N; }
   __device__ uint64_t offset(uint64_t u) const { return voff[u]; }
   __device__ uint32_t degree(uint32_t u) const { return voff[u + 1] - voff[u]; }
+  __device__ uint32_t source(uint64_t a) const { return inc_vtx[a]; }
   __device__ void arc(uint32_t u, uint32_t i, uint32_t& head, double& w) const {
     const uint32_t e = inc[voff[u] + i];
     const uint32_t p0 = pins[noff[e]], p1 = pins[noff[e] + 1];
     head = p0 != u ? p0 : p1;
     const double deg = double(voff[u + 1] - voff[u]);
-    w = wt == 0 ? double(nw[e]) : (wt == 1 ? double(nw[e]) / 2.0 : double(nw[e]) * (deg / 2.0));
+    const double we = unit ? 1.0 : double(nw[e]);
+    w = wt == 0 ? we : (wt == 1 ? we / 2.0 : we * (deg / 2.0));
   }
 };
 
@@ -86,10 +97,12 @@ struct ExplicitView {  // contracted levels
   uint32_t n;
   const uint64_t* idx;
   const uint32_t* head;
+  const uint32_t* src;
   const double* w;
   __device__ uint64_t num_nodes() const { return n; }
   __device__ uint64_t offset(uint64_t u) const { return idx[u]; }
   __device__ uint32_t degree(uint32_t u) const { return uint32_t(idx[u + 1] - idx[u]); }
+  __device__ uint32_t source(uint64_t a) const { return src[a]; }
   __device__ void arc(uint32_t u, uint32_t i, uint32_t& h, double& ww) const {
     const uint64_t a = idx[u] + i;
     h = head[a];
@@ -101,16 +114,17 @@ struct ExplicitGraph {
   uint32_t n = 0;
   uint64_t arcs = 0;
   DevBuf<uint64_t> idx;
-  DevBuf<uint32_t> head;
+  DevBuf<uint32_t> head, src;
   DevBuf<double> w;
   DevBuf<double> vol;
-  ExplicitView view() const { return ExplicitView{n, idx.p, head.p, w.p}; }
+  ExplicitView view() const { return ExplicitView{n, idx.p, head.p, src.p, w.p}; }
 };
 
 struct LvArgs {
   uint64_t seed;
   uint32_t level, round;
   int buckets, bucket;
+  uint32_t bucket_salt;
   uint32_t sampling;
   double recip;
   const int32_t* C;
@@ -121,12 +135,6 @@ struct LvArgs {
   int32_t* move_to;
   uint32_t* move_count;
 };
-
-__device__ inline bool lv_in_bucket(const LvArgs& a, uint32_t u) {
-  return a.buckets <= 1 ||
-         hash4(a.seed, kLvBucketTag | (uint64_t(a.level) << 8) | uint64_t(a.round), u, 0) % uint64_t(a.buckets) ==
-             uint64_t(a.bucket);
-}
 
 __device__ inline uint32_t slot_hash(int32_t key, int bits) {
   return (uint32_t(key) * 0x9E3779B1u) >> (32 - bits);
@@ -151,6 +159,58 @@ __device__ inline void lv_decide(const LvArgs& a, uint32_t u, double wfrom, doub
 }
 
 // ---------------------------------------------------------- move kernels --
+// Degree <= 16: one thread per node, no map at all.  The (cluster, weight)
+// pairs of the arcs stay in registers; a cluster's weight is summed in arc
+// order starting at its first occurrence, and the first occurrence with a
+// strictly larger gain wins: the insertion-ordered scan of
+// computeMaxGainCluster, summation order included.
+template <class View>
+__global__ void __launch_bounds__(256)
+louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count) {
+  constexpr int D = 16;
+  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
+    const uint32_t u = list[idx];
+    const uint32_t deg = min(g.degree(u), a.sampling);
+    int32_t key[D];
+    double w[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      key[i] = -1;
+      w[i] = 0.0;
+      if (uint32_t(i) < deg) {
+        uint32_t v;
+        double ww;
+        g.arc(u, uint32_t(i), v, ww);
+        key[i] = a.C[v];
+        w[i] = ww;
+      }
+    }
+    const int32_t from = a.C[u];
+    const double mult = a.recip * a.nvol[u];
+    double wfrom = 0.0, bgain = -DBL_MAX;
+    int32_t bkey = -1;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      bool first = key[i] >= 0;
+#pragma unroll
+      for (int j = 0; j < i; ++j) first = first && key[j] != key[i];
+      if (first) {
+        double sum = w[i];
+#pragma unroll
+        for (int j = i + 1; j < D; ++j)
+          if (key[j] == key[i]) sum += w[j];
+        if (key[i] == from) {
+          wfrom = sum;
+        } else {
+          const double gain = sum - mult * a.cvol[key[i]];
+          if (gain > bgain) { bgain = gain; bkey = key[i]; }
+        }
+      }
+    }
+    lv_decide(a, u, wfrom, bgain, bkey);
+  }
+}
+
 // G lanes per node, T-slot open-addressing map per group in LDS.
 template <class View, int G, int TBITS, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
@@ -308,16 +368,29 @@ __global__ void louvain_apply_kernel(const uint32_t* __restrict__ mu, const int3
 
 // node volumes (graph.cpp:319-327), sequential sum in arc order per node
 template <class View>
+__device__ inline bool constant_arc_weights(const View&, uint32_t) { return false; }
+__device__ inline bool constant_arc_weights(const StarView& g, uint32_t u) { return g.wt == 0 && u >= g.N; }
+
+template <class View>
 __global__ void node_volume_kernel(View g, uint32_t n, double* __restrict__ vol, uint32_t* __restrict__ deg_max) {
   uint32_t dm = 0;
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
     const uint32_t d = g.degree(u);
     double s = 0.0;
-    for (uint32_t i = 0; i < d; ++i) {
+    if (constant_arc_weights(g, u) && d > 0) {
+      // a net node under uniform weights: |e| arcs of weight w(e); the
+      // sequential sum of equal integer-valued terms is exactly |e| * w(e)
       uint32_t v;
       double w;
-      g.arc(u, i, v, w);
-      s += w;
+      g.arc(u, 0, v, w);
+      s = double(d) * w;
+    } else {
+      for (uint32_t i = 0; i < d; ++i) {
+        uint32_t v;
+        double w;
+        g.arc(u, i, v, w);
+        s += w;
+      }
     }
     vol[u] = s;
     dm = max(dm, d);
@@ -342,9 +415,7 @@ template <class View>
 __device__ inline int lv_slot(const View& g, const LvArgs& a, uint32_t u) {
   const uint32_t d = min(g.degree(u), a.sampling);
   const int c = d <= 16 ? 0 : (d <= 128 ? 1 : (d <= 1024 ? 2 : 3));
-  const int b = a.buckets <= 1 ? 0
-                               : int(hash4(a.seed, kLvBucketTag | (uint64_t(a.level) << 8) | uint64_t(a.round), u, 0) %
-                                     uint64_t(a.buckets));
+  const int b = a.buckets <= 1 ? 0 : int(bucket_of(a.bucket_salt, u, uint32_t(a.buckets)));
   return c * a.buckets + b;
 }
 
@@ -401,13 +472,14 @@ __global__ void mark_used_kernel(const int32_t* __restrict__ C, uint32_t n, uint
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) used[C[u]] = 1;
 }
 
-__global__ void remap_kernel(int32_t* __restrict__ C, uint32_t n, const uint32_t* __restrict__ map,
-                             const double* __restrict__ nvol, double* __restrict__ cvol) {
-  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
-    const int32_t c = int32_t(map[C[u]]);
-    C[u] = c;
-    atomicAdd(&cvol[c], nvol[u]);
-  }
+__global__ void remap_kernel(int32_t* __restrict__ C, uint32_t n, const uint32_t* __restrict__ map) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) C[u] = int32_t(map[C[u]]);
+}
+
+__global__ void coarse_volume_kernel(const uint32_t* __restrict__ used, const uint32_t* __restrict__ map,
+                                     const double* __restrict__ cvol, uint32_t n, double* __restrict__ out) {
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+    if (used[c]) out[map[c]] = cvol[c];
 }
 
 template <class View>
@@ -426,7 +498,7 @@ __global__ void emit_arcs_kernel(View g, uint64_t num_arcs, const int32_t* __res
   const uint64_t sentinel = uint64_t(1) << (2 * bits);
   for (uint64_t a = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; a < num_arcs;
        a += uint64_t(gridDim.x) * blockDim.x) {
-    const uint32_t u = arc_source(g, a);
+    const uint32_t u = g.source(a);
     uint32_t v;
     double w;
     g.arc(u, uint32_t(a - g.offset(u)), v, w);
@@ -437,12 +509,16 @@ __global__ void emit_arcs_kernel(View g, uint64_t num_arcs, const int32_t* __res
 }
 
 __global__ void coarse_csr_kernel(const uint64_t* __restrict__ ukeys, uint64_t ucount, int bits,
-                                  uint32_t* __restrict__ head, uint32_t* __restrict__ deg) {
+                                  uint32_t* __restrict__ head, uint32_t* __restrict__ src,
+                                  uint64_t* __restrict__ ends) {
   const uint64_t mask = (uint64_t(1) << bits) - 1;
   for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < ucount;
        i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t cu = uint32_t(ukeys[i] >> bits);
     head[i] = uint32_t(ukeys[i] & mask);
-    atomicAdd(&deg[ukeys[i] >> bits], 1u);
+    src[i] = cu;
+    // keys are sorted by source: the last arc of each source writes its degree
+    if (i + 1 == ucount || uint32_t(ukeys[i + 1] >> bits) != cu) ends[cu] = i + 1;
   }
 }
 
@@ -503,8 +579,7 @@ struct LouvainRun {
     const int B = a.buckets;
     const int s0 = 0 * B + sr, s1 = 1 * B + sr, s2 = 2 * B + sr, s3 = 3 * B + sr;
     if (cnt[s0])
-      louvain_move_group_kernel<View, 8, 5, 256><<<grid_for(cnt[s0], 32, 65536), 256, 0, s>>>(g, a, list + off[s0],
-                                                                                             cnt[s0]);
+      louvain_move_small_kernel<View><<<grid_for(cnt[s0], 256, 65536), 256, 0, s>>>(g, a, list + off[s0], cnt[s0]);
     if (cnt[s1])
       louvain_move_group_kernel<View, 32, 8, 256><<<grid_for(cnt[s1], 8, 65536), 256, 0, s>>>(g, a, list + off[s1],
                                                                                               cnt[s1]);
@@ -512,7 +587,7 @@ struct LouvainRun {
       louvain_move_group_kernel<View, 64, 11, 64><<<grid_for(cnt[s2], 1, 65536), 64, 0, s>>>(g, a, list + off[s2],
                                                                                              cnt[s2]);
     if (cnt[s3])
-      louvain_move_global_kernel<View><<<unsigned(std::min<uint32_t>(cnt[s3], 128)), 256, 0, s>>>(
+      louvain_move_global_kernel<View><<<unsigned(std::min<uint32_t>(cnt[s3], global_blocks_)), 256, 0, s>>>(
           g, a, list + off[s3], cnt[s3], tbits_global, gk.p, gw.p, gf.p);
     MKP_LAUNCH_CHECK();
   }
@@ -524,7 +599,8 @@ struct LouvainRun {
     const hipStream_t s = c.stream;
     const int B = std::max(1, prm.sub_rounds);
     const int slots = 4 * B;
-    DevBuf<double> cvol(std::max<uint32_t>(n, 1));
+    cvol_.alloc(std::max<uint32_t>(n, 1));
+    DevBuf<double>& cvol = cvol_;
     DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
     DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), counts(slots), cursor(slots), offsets(slots), mcount(1),
         mu(std::max<uint32_t>(n, 1)), moved_total(1);
@@ -543,9 +619,11 @@ struct LouvainRun {
     const uint64_t cap = std::min<uint64_t>(max_degree_, prm.sampling);
     if (cap > 1024) {
       while ((uint64_t(1) << tbits) < 2 * cap) ++tbits;
-      gk.alloc(uint64_t(128) << tbits);
-      gw.alloc(uint64_t(128) << tbits);
-      gf.alloc(uint64_t(128) << tbits);
+      // up to 512 concurrent maps, at most ~1 GiB of map slots
+      global_blocks_ = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(512, (uint64_t(1) << 30) / (16ull << tbits))));
+      gk.alloc(uint64_t(global_blocks_) << tbits);
+      gw.alloc(uint64_t(global_blocks_) << tbits);
+      gf.alloc(uint64_t(global_blocks_) << tbits);
     }
     LvArgs a;
     a.seed = prm.seed; a.level = level; a.sampling = uint32_t(std::min<uint64_t>(prm.sampling, 0xffffffffu));
@@ -557,6 +635,7 @@ struct LouvainRun {
     std::vector<uint32_t> cnt(slots), off(slots);
     for (int r = 0; double(moved) >= prm.min_move_fraction * double(n) && r < prm.max_passes; ++r) {
       a.round = uint32_t(r);
+      a.bucket_salt = bucket_salt(a.seed, kLvBucketTag | (uint64_t(level) << 8) | uint64_t(r));
       // group this round's nodes by (degree class, sub-round bucket)
       fill_async<uint32_t>(counts.p, 0u, slots, s);
       fill_async<uint32_t>(cursor.p, 0u, slots, s);
@@ -599,6 +678,8 @@ struct LouvainRun {
   }
 
   uint32_t max_degree_ = 0;
+  DevBuf<double> cvol_;  // cluster volumes left by the last local-moving level
+  uint32_t global_blocks_ = 1;
 
   // level 0: node volumes from the arcs; every level: maximum degree
   template <class View>
@@ -630,8 +711,9 @@ struct LouvainRun {
     c.sync();
     cg.n = nc;
     cg.vol.alloc(std::max<uint32_t>(nc, 1));
-    fill_async<double>(cg.vol.p, 0.0, nc, s);
-    remap_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(C, n, map.p, nvol, cg.vol.p);
+    // coarse volume = tracked volume of the cluster (the sum of its members)
+    coarse_volume_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(used.p, map.p, cvol_.p, n, cg.vol.p);
+    remap_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(C, n, map.p);
     MKP_LAUNCH_CHECK();
     int bits = 1;
     while ((uint64_t(1) << bits) < nc) ++bits;
@@ -640,6 +722,7 @@ struct LouvainRun {
     if (num_arcs == 0 || nc <= 1) {
       fill_async<uint64_t>(cg.idx.p, 0, size_t(nc) + 1, s);
       cg.head.alloc(1);
+      cg.src.alloc(1);
       cg.w.alloc(1);
       cg.arcs = 0;
       c.sync();
@@ -670,18 +753,19 @@ struct LouvainRun {
     }
     cg.arcs = U;
     cg.head.alloc(std::max<uint64_t>(U, 1));
+    cg.src.alloc(std::max<uint64_t>(U, 1));
     cg.w.alloc(std::max<uint64_t>(U, 1));
-    DevBuf<uint32_t> deg(std::max<uint32_t>(nc, 1));
-    DevBuf<uint64_t> deg64(std::max<uint32_t>(nc, 1));
-    fill_async<uint32_t>(deg.p, 0u, nc, s);
+    // idx[c + 1] = end of the arcs of source c; sources without arcs carry the
+    // previous end forward (inclusive max scan)
+    DevBuf<uint64_t> ends(std::max<uint32_t>(nc, 1));
+    fill_async<uint64_t>(ends.p, 0, nc, s);
     if (U) {
-      coarse_csr_kernel<<<grid_for(U, 256, 65536), 256, 0, s>>>(ukeys.p, U, bits, cg.head.p, deg.p);
+      coarse_csr_kernel<<<grid_for(U, 256, 65536), 256, 0, s>>>(ukeys.p, U, bits, cg.head.p, cg.src.p, ends.p);
       MKP_LAUNCH_CHECK();
       MKP_HIP(hipMemcpyAsync(cg.w.p, uvals.p, U * 8, hipMemcpyDeviceToDevice, s));
     }
-    u32_to_u64_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(deg.p, deg64.p, nc);
-    MKP_LAUNCH_CHECK();
-    exclusive_scan_u64(c, deg64.p, cg.idx.p, nc);
+    fill_async<uint64_t>(cg.idx.p, 0, 1, s);
+    inclusive_max_scan_u64(c, ends.p, cg.idx.p + 1, nc);
     c.sync();
     return cg;
   }
@@ -749,10 +833,12 @@ extern "C" int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* prm, in
     const uint32_t nodes = is_graph ? hg->n : hg->n + hg->m;
     DevBuf<int32_t> C(std::max<uint32_t>(nodes, 1));
     if (is_graph) {
-      PlainView g{hg->n, hg->voff.p, hg->inc.p, hg->noff.p, hg->pins.p, hg->nw.p, wt};
+      PlainView g{hg->n, hg->voff.p, hg->inc.p, hg->noff.p, hg->pins.p, hg->inc_vtx.p, hg->nw.p, wt,
+                  hg->unit_net_weights};
       run.run_from(g, hg->n, hg->p, C.p);
     } else {
-      StarView g{hg->n, hg->m, hg->p, hg->voff.p, hg->inc.p, hg->noff.p, hg->pins.p, hg->nw.p, wt};
+      StarView g{hg->n,          hg->m,          hg->p,     hg->voff.p, hg->inc.p, hg->noff.p, hg->pins.p,
+                 hg->inc_vtx.p,  hg->pin_net.p,  hg->nw.p,  wt,         hg->unit_net_weights};
       run.run_from(g, nodes, 2 * hg->p, C.p);
     }
     // restrictClusteringToHypernodes + setCommunityIDs (graph.h:123-125,

@@ -90,6 +90,7 @@ struct LpArgs {
   uint32_t round;
   int buckets;
   int bucket;
+  uint32_t bucket_salt;
   int64_t* counters;
 };
 
@@ -98,8 +99,7 @@ __device__ inline uint32_t pc_field(const uint64_t* pc, const PcLayout& L, uint3
 }
 
 __device__ inline bool in_bucket(const LpArgs& a, uint32_t u) {
-  return a.buckets <= 1 || hash4(a.seed, kLpBucketTag | uint64_t(a.round), u, 0) % uint64_t(a.buckets) ==
-                               uint64_t(a.bucket);
+  return a.buckets <= 1 || bucket_of(a.bucket_salt, u, uint32_t(a.buckets)) == uint32_t(a.bucket);
 }
 
 __device__ inline bool lp_coin(const LpArgs& a, uint32_t u, int b) {
@@ -467,7 +467,7 @@ lp_bucket_keys_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
   for (int i = threadIdx.x; i < a.buckets; i += blockDim.x) h[i] = 0;
   __syncthreads();
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const uint64_t b = hash4(a.seed, kLpBucketTag | uint64_t(a.round), active[i], 0) % uint64_t(a.buckets);
+    const uint32_t b = bucket_of(a.bucket_salt, active[i], uint32_t(a.buckets));
     keys[i] = b;
     atomicAdd(&h[b], 1u);
   }
@@ -554,6 +554,7 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats) 
     std::vector<uint32_t> boff(size_t(B) + 1, 0);
     boff[B] = na;
     a.buckets = B;
+    a.bucket_salt = bucket_salt(a.seed, kLpBucketTag | uint64_t(r));
     if (B > 1) {
       fill_async<int64_t>(w.bcount.p, 0, B, s);
       lp_bucket_keys_kernel<<<grid_for(na, 256, 65536), 256, 0, s>>>(a, w.active.p, na, w.bkeys.p, w.bcount.p);

@@ -43,6 +43,14 @@ static void excl_scan(Context& c, const T* in, T* out, size_t n) {
 void exclusive_scan_u32(Context& c, const uint32_t* in, uint32_t* out, size_t n) { excl_scan(c, in, out, n); }
 void exclusive_scan_u64(Context& c, const uint64_t* in, uint64_t* out, size_t n) { excl_scan(c, in, out, n); }
 
+void inclusive_max_scan_u64(Context& c, const uint64_t* in, uint64_t* out, size_t n) {
+  if (n == 0) return;
+  size_t bytes = 0;
+  MKP_HIP(rocprim::inclusive_scan(nullptr, bytes, in, out, n, rocprim::maximum<uint64_t>(), c.stream));
+  void* t = c.temp(bytes);
+  MKP_HIP(rocprim::inclusive_scan(t, bytes, in, out, n, rocprim::maximum<uint64_t>(), c.stream));
+}
+
 void inclusive_scan_i64(Context& c, const int64_t* in, int64_t* out, size_t n) {
   if (n == 0) return;
   size_t bytes = 0;

@@ -209,9 +209,9 @@ std::vector<uint32_t> cluster_pass_sync(const Hypergraph& hg, const CoarsenParam
     if (current <= limit) break;
     const uint64_t tag = uint64_t(pass) << 8 | uint64_t(s);
     std::vector<uint32_t> proposers;
+    const uint32_t salt = bucket_salt(prm.seed, kCoBucketTag | uint64_t(pass));
     for (uint32_t u = 0; u < n; ++u)
-      if (!matched[u] && (S == 1 || hash4(prm.seed, kCoBucketTag | uint64_t(pass), u, 0) % S == uint64_t(s)))
-        proposers.push_back(u);
+      if (!matched[u] && (S == 1 || bucket_of(salt, u, uint32_t(S)) == uint32_t(s))) proposers.push_back(u);
     for (uint32_t u : proposers) {
       target[u] = UINT32_MAX;
       uint64_t accesses = 0;

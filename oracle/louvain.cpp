@@ -286,12 +286,11 @@ bool local_moving_sync_level(const Graph& g, std::vector<int32_t>& C, const Louv
   std::vector<std::pair<uint32_t, int32_t>> moves;
   for (int r = 0; double(moved) >= prm.min_move_fraction * g.n && r < prm.max_passes; ++r) {
     moved = 0;
+    const uint32_t salt = bucket_salt(prm.seed, kLvBucketTag | (uint64_t(level) << 8) | uint64_t(r));
     for (int s = 0; s < B; ++s) {
       moves.clear();
       for (uint32_t u = 0; u < g.n; ++u) {
-        if (B > 1 &&
-            hash4(prm.seed, kLvBucketTag | (uint64_t(level) << 8) | uint64_t(r), u, 0) % B != uint64_t(s))
-          continue;
+        if (B > 1 && bucket_of(salt, u, uint32_t(B)) != uint32_t(s)) continue;
         const int32_t from = C[u];
         const int32_t best = best_cluster_with(g, C.data(), cvol.data(), u, prm.sampling, recip, map, nullptr);
         if (best == from) continue;

@@ -42,6 +42,16 @@ inline uint64_t hash4(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
   h = mix64(h ^ (b * 0xA0761D6478BD642Full));
   return mix64(h ^ (c * 0xE7037ED1A0B428DBull));
 }
+// Sub-round bucket of a vertex: MurmurHash3's 32-bit finalizer over the id
+// and a per-(seed, round) salt.
+inline uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16; h *= 0x85EBCA6Bu; h ^= h >> 13; h *= 0xC2B2AE35u; h ^= h >> 16;
+  return h;
+}
+inline uint32_t bucket_salt(uint64_t seed, uint64_t tag) { return uint32_t(mix64(seed ^ tag)); }
+inline uint32_t bucket_of(uint32_t salt, uint32_t u, uint32_t buckets) {
+  return fmix32(u * 0x9E3779B1u + salt) % buckets;
+}
 
 // ------------------------------------------------------------- hypergraph --
 // Static CSR hypergraph (static_hypergraph.h:44); incident nets ascending,

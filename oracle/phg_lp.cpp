@@ -358,7 +358,8 @@ LpStats lp_refine_sync(Partition& p, const LpParams& prm) {
       std::vector<Cand> cands;
       const uint64_t rs = (uint64_t(r) << 8) | uint64_t(s);
       for (uint32_t u : active) {
-        if (B > 1 && hash4(prm.seed, kLpBucketTag | uint64_t(r), u, 0) % B != uint64_t(s)) continue;
+        if (B > 1 && bucket_of(bucket_salt(prm.seed, kLpBucketTag | uint64_t(r)), u, uint32_t(B)) != uint32_t(s))
+          continue;
         if (hg.degree(u) > 100000) continue;
         st.visits += hg.degree(u);  // the gain sweep reads every incidence once
         if (!is_border(p, u)) continue;

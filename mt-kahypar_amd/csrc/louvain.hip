@@ -140,6 +140,47 @@ __device__ inline uint32_t slot_hash(int32_t key, int bits) {
   return (uint32_t(key) * 0x9E3779B1u) >> (32 - bits);
 }
 
+// Inserts (key, w, pos) of every lane of a WIDTH-lane segment into an
+// open-addressing map.  Up to `passes` times, the lanes holding the key of the
+// lowest pending lane are combined (weight sum, minimum position) and inserted
+// once by that lane, so arcs of a heavy node that fall into one cluster cost
+// one atomic per pass instead of one each; the remaining lanes insert alone.
+template <int WIDTH>
+__device__ inline void map_insert_aggregated(int32_t* K, double* Wt, uint32_t* F, int tbits, bool valid,
+                                             int32_t key, double w, uint32_t pos, int passes) {
+  const int lane = threadIdx.x & 63;
+  const int seg_lo = lane & ~(WIDTH - 1);
+  const uint64_t seg_mask = (WIDTH == 64) ? ~0ull : (((1ull << WIDTH) - 1) << seg_lo);
+  bool pending = valid;
+  auto insert = [&](int32_t k, double ww, uint32_t pp) {
+    const uint32_t T = 1u << tbits;
+    uint32_t h = slot_hash(k, tbits);
+    while (true) {
+      const int32_t old = atomicCAS(&K[h], -1, k);
+      if (old == -1 || old == k) break;
+      h = (h + 1) & (T - 1);
+    }
+    atomicAdd(&Wt[h], ww);
+    atomicMin(&F[h], pp);
+  };
+  // every lane of the wave runs every pass (the shuffles need the whole wave)
+  for (int it = 0; it < passes; ++it) {
+    const uint64_t act = __ballot(pending) & seg_mask;
+    const int leader = act ? __ffsll(static_cast<long long>(act)) - 1 : seg_lo;
+    const int32_t lk = __shfl(key, leader, 64);
+    const bool mine = act != 0 && pending && key == lk;
+    double sum = mine ? w : 0.0;
+    uint32_t mn = mine ? pos : 0xffffffffu;
+    for (int o = WIDTH / 2; o > 0; o >>= 1) {
+      sum += __shfl_xor(sum, o, WIDTH);
+      mn = min(mn, __shfl_xor(mn, o, WIDTH));
+    }
+    if (act != 0 && lane == leader) insert(lk, sum, mn);
+    if (mine) pending = false;
+  }
+  if (pending) insert(key, w, pos);
+}
+
 // best (gain, first) pair: larger gain, then smaller first occurrence
 __device__ inline bool better(double g, uint32_t f, double bg, uint32_t bf) {
   return g > bg || (g == bg && f < bf);
@@ -230,21 +271,22 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
     const uint32_t u = in ? list[idx] : 0;
     for (int t = lane; t < T; t += G) { K[t] = -1; Wt[t] = 0.0; F[t] = 0xffffffffu; }
     __syncthreads();
-    if (in) {
-      const uint32_t deg = min(g.degree(u), a.sampling);
-      for (uint32_t i = lane; i < deg; i += G) {
-        uint32_t v;
-        double w;
-        g.arc(u, i, v, w);
-        const int32_t c = a.C[v];
-        uint32_t h = slot_hash(c, TBITS);
-        while (true) {
-          const int32_t old = atomicCAS(&K[h], -1, c);
-          if (old == -1 || old == c) break;
-          h = (h + 1) & (T - 1);
+    {
+      // trip count uniform over the wave: the aggregation shuffles need it
+      const uint32_t deg = in ? min(g.degree(u), a.sampling) : 0;
+      uint32_t trips = (deg + G - 1) / G;
+      for (int o = 32; o > 0; o >>= 1) trips = max(trips, __shfl_xor(trips, o, 64));
+      for (uint32_t t = 0; t < trips; ++t) {
+        const uint32_t i = t * G + lane;
+        const bool valid = i < deg;
+        int32_t c = -1;
+        double w = 0.0;
+        if (valid) {
+          uint32_t v;
+          g.arc(u, i, v, w);
+          c = a.C[v];
         }
-        atomicAdd(&Wt[h], w);
-        atomicMin(&F[h], i);
+        map_insert_aggregated<G>(K, Wt, F, TBITS, valid, c, w, i, 2);
       }
     }
     __syncthreads();
@@ -297,19 +339,17 @@ louvain_move_global_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, 
     for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) { K[t] = -1; Wt[t] = 0.0; F[t] = 0xffffffffu; }
     __threadfence_block();
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < deg; i += blockDim.x) {
-      uint32_t v;
-      double w;
-      g.arc(u, i, v, w);
-      const int32_t c = a.C[v];
-      uint32_t h = slot_hash(c, tbits);
-      while (true) {
-        const int32_t old = atomicCAS(&K[h], -1, c);
-        if (old == -1 || old == c) break;
-        h = (h + 1) & (T - 1);
+    for (uint32_t base = 0; base < deg; base += blockDim.x) {  // uniform over the block
+      const uint32_t i = base + threadIdx.x;
+      const bool valid = i < deg;
+      int32_t c = -1;
+      double w = 0.0;
+      if (valid) {
+        uint32_t v;
+        g.arc(u, i, v, w);
+        c = a.C[v];
       }
-      atomicAdd(&Wt[h], w);
-      atomicMin(&F[h], i);
+      map_insert_aggregated<64>(K, Wt, F, tbits, valid, c, w, i, 4);
     }
     __threadfence_block();
     __syncthreads();
@@ -654,8 +694,10 @@ struct LouvainRun {
         fill_async<uint32_t>(mcount.p, 0u, 1, s);
         TimedRegion tr(c.prof, s, kFamLouvainMove, 0.0);
         launch_move(g, a, list.p, off, cnt, sr, tbits, gk, gw, gf);
-        louvain_apply_kernel<<<grid_for(n, 256, 4096), 256, 0, s>>>(mu.p, mto.p, mcount.p, nvol, C, cvol.p,
-                                                                   csize.p, moved_total.p);
+        // at most one move per node of this sub-round
+        const uint32_t sub_nodes = cnt[sr] + cnt[B + sr] + cnt[2 * B + sr] + cnt[3 * B + sr];
+        louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(mu.p, mto.p, mcount.p, nvol, C,
+                                                                           cvol.p, csize.p, moved_total.p);
         MKP_LAUNCH_CHECK();
       }
       uint32_t mt = 0;

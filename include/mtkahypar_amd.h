@@ -150,6 +150,66 @@ typedef struct {
 MKP_API int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* params, int32_t* communities,
                         mkp_louvain_stats* stats);
 
+/* -------------------------------------------------------- coarsening --- */
+typedef struct mkp_hierarchy mkp_hierarchy;
+typedef struct {
+  uint32_t contraction_limit;   /* coarsening.contraction_limit (160 * k, context.cpp:256-269) */
+  int32_t max_node_weight;      /* context.cpp:271-284 */
+  double min_shrink_factor;     /* c-min-shrink-factor (1.01) */
+  double max_shrink_factor;     /* c-max-shrink-factor (2.5) */
+  uint32_t ignore_net_size;     /* maxnet-ignore (1000) */
+  uint64_t sampling;            /* c-vertex-degree-sampling-threshold (200000) */
+  uint64_t seed;
+  int32_t sub_rounds;           /* buckets per clustering pass of the synchronous form */
+} mkp_coarsen_params;
+/* One clustering pass (the parallel_for of multilevel_coarsener.h:197-239 with
+ * MultilevelVertexPairRater::rate and matchVertices); clusters (host,
+ * num_vertices) receives each vertex's representative. */
+MKP_API int mkp_cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params* params, int32_t pass, uint32_t* clusters,
+                             uint32_t* nodes_after, int64_t* pin_visits);
+/* StaticHypergraph::contract (datastructures/static_hypergraph.cpp:58-513):
+ * clusters (host) maps every vertex to its representative; cmap (host, may be
+ * NULL) receives the coarse vertex of every vertex. */
+MKP_API int mkp_hypergraph_contract(mkp_hypergraph* hg, const uint32_t* clusters, mkp_hypergraph** coarse,
+                                    uint32_t* cmap);
+/* MultilevelCoarsener::coarsenImpl (multilevel_coarsener.h:143-315): the
+ * level hierarchy stays on the device; level 0 is hg itself. */
+MKP_API int mkp_coarsen(mkp_hypergraph* hg, const mkp_coarsen_params* params, mkp_hierarchy** out);
+MKP_API int mkp_hierarchy_num_levels(const mkp_hierarchy* h, int32_t* levels);
+/* Borrowed handle of level i (0 = input, num_levels-1 = coarsest). */
+MKP_API int mkp_hierarchy_level(mkp_hierarchy* h, int32_t level, mkp_hypergraph** hg);
+MKP_API int mkp_hierarchy_destroy(mkp_hierarchy* h);
+/* Projection of doUncoarsen (partition/coarsening/uncoarsening.cpp:111-122):
+ * fine (on level i) takes the block of its coarse vertex in coarse (on level
+ * i+1), then initializePartition. */
+MKP_API int mkp_partition_project(mkp_partition* fine, const mkp_partition* coarse, const mkp_hierarchy* h,
+                                  int32_t level);
+
+/* ----------------------------------------------------------- pipeline -- */
+typedef struct {
+  int32_t k;
+  double epsilon;
+  uint64_t seed;
+  int32_t use_community_detection; /* p-enable-community-detection (1) */
+  int32_t lp_sub_rounds;
+  int32_t louvain_sub_rounds;
+  int32_t coarsening_sub_rounds;
+  int32_t ip_runs;                 /* i-runs (20) */
+} mkp_partition_params;
+typedef struct {
+  int64_t km1;
+  double imbalance;
+  int32_t levels;
+  double louvain_ms, coarsening_ms, initial_ms, refinement_ms, total_ms;
+  int64_t louvain_arcs, rating_visits, lp_visits;
+} mkp_partition_stats;
+/* partition() of partitioner.cpp:163-204 with the fast preset (community
+ * detection, multilevel coarsening, initial partitioning of the coarsest
+ * hypergraph, LP at every level; FM and rebalancing not run).  parts (host,
+ * num_vertices) receives the block ids. */
+MKP_API int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* params, int32_t* parts,
+                                     mkp_partition_stats* stats);
+
 /* ---------------------------------------------------------- generator -- */
 typedef struct {
   uint32_t num_vertices;

@@ -1,0 +1,731 @@
+// Community-restricted heavy-edge clustering and hypergraph contraction.
+//
+// Reference: MultilevelCoarsener::coarsenImpl / matchVertices
+// (multilevel_coarsener.h:143-438), MultilevelVertexPairRater::rate /
+// fillRatingMap / adaptiveEdgeSize (multilevel_vertex_pair_rater.h:109-279),
+// HeavyEdgeScore, NoWeightPenalty, BestRatingPreferringUnmatched
+// (policies/rating_*.h), StaticHypergraph::contract
+// (static_hypergraph.cpp:58-513).
+//
+// Clustering pass, synchronous form: vertices are split into sub-rounds by a
+// hash bucket; in a sub-round every unmatched vertex of the bucket rates the
+// clusters of its neighbours on the state at the sub-round start:
+//   size'(e) = max(#distinct clusters among the pins of e, 2), nets of at
+//   least 1000 pins ignored, score(e) = w(e) / (size'(e) - 1) in 36.28 fixed
+//   point (exact, associative sums), every distinct cluster of e gains
+//   score(e) once;
+// the target is the best cluster by (rating, not yet matched, hash) among
+// clusters of the same community whose weight fits.  A vertex whose target
+// does not move in the sub-round joins it (joiners of a target are admitted
+// in (rating desc, hash, id) order up to the node weight limit); two vertices
+// that chose each other pair up under the smaller id.
+//
+// Contraction, canonical form: coarse ids are the ranks of the cluster
+// representatives, pins are sorted and unique per net, single-pin nets
+// vanish, parallel nets (equal pin sets) merge into the smallest original net
+// id with summed weight, coarse nets keep the original order, incident nets
+// are ascending.
+#include <algorithm>
+#include <cstring>
+#include <map>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "internal.h"
+
+namespace mkp {
+
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr int kFixShift = 28;
+
+struct CoArgs {
+  uint32_t n;
+  const uint32_t* voff;
+  const uint32_t* inc;
+  const uint32_t* noff;
+  const uint32_t* pins;
+  const int32_t* nw;
+  const int32_t* comm;
+  const uint64_t* ub;  // rating upper bound per vertex (sizes the global map)
+  const uint32_t* cid;
+  const int64_t* cw;
+  const uint8_t* matched;
+  uint32_t* target;
+  uint64_t* rating;
+  uint32_t ignore;
+  uint64_t sampling;
+  int64_t max_w;
+  uint64_t seed;
+  uint64_t tag;  // (pass << 8) | sub-round
+  int prefer_unmatched;
+  int64_t* visits;
+};
+
+__device__ inline uint32_t co_slot(uint32_t key, int bits) { return (key * 0x9E3779B1u) >> (32 - bits); }
+
+// (value, unmatched, hash) order of the target choice
+struct Choice {
+  uint64_t val;
+  int un;
+  uint32_t t;
+};
+__device__ inline bool choice_better(const CoArgs& a, uint32_t u, const Choice& x, const Choice& y) {
+  if (x.t == kNone) return false;
+  if (y.t == kNone) return true;
+  if (x.val != y.val) return x.val > y.val;
+  if (x.un != y.un) return x.un > y.un;
+  return hash4(a.seed, kCoTieTag | a.tag, u, x.t) > hash4(a.seed, kCoTieTag | a.tag, u, y.t);
+}
+
+// Rating of one proposer per block.  The net set (distinct clusters of the
+// current net) lives in LDS; the rating map lives in LDS (MAP_BITS > 0) or in
+// per-block global scratch of 2^gbits slots.
+template <int BLOCK, int MAP_BITS>
+__global__ void __launch_bounds__(BLOCK)
+co_rate_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, int gbits,
+               uint32_t* __restrict__ gkeys, uint64_t* __restrict__ gvals) {
+  constexpr int NS_BITS = 11;  // net set: 2048 slots (nets < 1000 pins)
+  constexpr int NS = 1 << NS_BITS;
+  __shared__ uint32_t nset[NS];
+  __shared__ uint32_t lkeys[MAP_BITS > 0 ? (1 << MAP_BITS) : 1];
+  __shared__ uint64_t lvals[MAP_BITS > 0 ? (1 << MAP_BITS) : 1];
+  __shared__ uint32_t s_cnt;
+  __shared__ uint64_t s_val[BLOCK / 64];
+  __shared__ int s_un[BLOCK / 64];
+  __shared__ uint32_t s_t[BLOCK / 64];
+  uint32_t* mkeys = MAP_BITS > 0 ? lkeys : gkeys + (uint64_t(blockIdx.x) << gbits);
+  uint64_t* mvals = MAP_BITS > 0 ? lvals : gvals + (uint64_t(blockIdx.x) << gbits);
+  const uint32_t count = *list_count;
+  int64_t visits = 0;
+  for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
+    const uint32_t u = list[item];
+    // map of at least twice the number of distinct keys the sweep can insert
+    int mbits = MAP_BITS;
+    if (MAP_BITS == 0) {
+      const uint64_t keys = min(a.ub[u], a.sampling);
+      mbits = 1;
+      while ((uint64_t(1) << mbits) < 2 * keys) ++mbits;
+      mbits = min(mbits, gbits);
+    }
+    const uint32_t MT = 1u << mbits;
+    for (uint32_t t = threadIdx.x; t < MT; t += BLOCK) { mkeys[t] = kNone; mvals[t] = 0; }
+    __syncthreads();
+    uint64_t accesses = 0;
+    for (uint32_t j = a.voff[u]; j < a.voff[u + 1]; ++j) {  // uniform over the block
+      const uint32_t e = a.inc[j];
+      const uint32_t pb = a.noff[e], s = a.noff[e + 1] - pb;
+      if (s >= a.ignore) continue;
+      int nbits = 1;
+      while ((1u << nbits) < 2 * s) ++nbits;
+      const uint32_t NT = 1u << nbits;
+      for (uint32_t t = threadIdx.x; t < NT; t += BLOCK) nset[t] = kNone;
+      if (threadIdx.x == 0) s_cnt = 0;
+      __syncthreads();
+      uint32_t fresh = 0;
+      for (uint32_t q = threadIdx.x; q < s; q += BLOCK) {
+        const uint32_t rep = a.cid[a.pins[pb + q]];
+        uint32_t h = co_slot(rep, nbits);
+        while (true) {
+          const uint32_t old = atomicCAS(&nset[h], kNone, rep);
+          if (old == kNone) { ++fresh; break; }
+          if (old == rep) break;
+          h = (h + 1) & (NT - 1);
+        }
+      }
+      if (fresh) atomicAdd(&s_cnt, fresh);
+      visits += threadIdx.x == 0 ? 2 * int64_t(s) : 0;
+      __syncthreads();
+      const uint32_t distinct = s_cnt;
+      const uint32_t sz = max(distinct, 2u);
+      if (accesses + sz > a.sampling) break;  // uniform: s_cnt is block-wide
+      const uint64_t score = (uint64_t(uint32_t(a.nw[e])) << kFixShift) / uint64_t(sz - 1);
+      for (uint32_t t = threadIdx.x; t < NT; t += BLOCK) {
+        const uint32_t rep = nset[t];
+        if (rep == kNone) continue;
+        uint32_t h = co_slot(rep, mbits);
+        while (true) {
+          const uint32_t old = atomicCAS(&mkeys[h], kNone, rep);
+          if (old == kNone || old == rep) break;
+          h = (h + 1) & (MT - 1);
+        }
+        atomicAdd(reinterpret_cast<unsigned long long*>(&mvals[h]), static_cast<unsigned long long>(score));
+      }
+      accesses += distinct;
+      if (MAP_BITS == 0) __threadfence_block();
+      __syncthreads();
+    }
+    // target choice over the map
+    const int64_t cwu = a.cw[u];
+    const int32_t cu = a.comm[u];
+    Choice best{0, 0, kNone};
+    for (uint32_t t = threadIdx.x; t < MT; t += BLOCK) {
+      const uint32_t rep = mkeys[t];
+      if (rep == kNone || rep == u) continue;
+      if (cwu + a.cw[rep] > a.max_w || a.comm[rep] != cu) continue;
+      Choice c{mvals[t], a.prefer_unmatched ? (a.matched[rep] ? 0 : 1) : 0, rep};
+      if (choice_better(a, u, c, best)) best = c;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      Choice other;
+      other.val = __shfl_xor(best.val, o, 64);
+      other.un = __shfl_xor(best.un, o, 64);
+      other.t = __shfl_xor(best.t, o, 64);
+      if (choice_better(a, u, other, best)) best = other;
+    }
+    if ((threadIdx.x & 63) == 0) {
+      s_val[threadIdx.x >> 6] = best.val;
+      s_un[threadIdx.x >> 6] = best.un;
+      s_t[threadIdx.x >> 6] = best.t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < BLOCK / 64; ++w) {
+        Choice other{s_val[w], s_un[w], s_t[w]};
+        if (choice_better(a, u, other, best)) best = other;
+      }
+      a.target[u] = best.t;
+      a.rating[u] = best.val;
+    }
+    __syncthreads();
+  }
+  if (visits) atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(visits));
+}
+
+// rating upper bound: sum of the sizes of the nets the rating sweeps
+__global__ void co_ub_kernel(const uint32_t* __restrict__ voff, const uint32_t* __restrict__ inc,
+                             const uint32_t* __restrict__ noff, uint32_t n, uint32_t ignore,
+                             uint64_t* __restrict__ ub) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    uint64_t s = 0;
+    for (uint32_t j = voff[u]; j < voff[u + 1]; ++j) {
+      const uint32_t e = inc[j];
+      const uint32_t sz = noff[e + 1] - noff[e];
+      s += sz < ignore ? sz : 0;
+    }
+    ub[u] = s;
+  }
+}
+
+// proposers of the sub-round by tier (rating map in LDS if ub <= 1024)
+__global__ void __launch_bounds__(256)
+co_select_kernel(uint32_t n, const uint8_t* __restrict__ matched, const uint64_t* __restrict__ ub, uint32_t salt,
+                 uint32_t buckets, uint32_t bucket, uint32_t* __restrict__ target, uint32_t* __restrict__ list_a,
+                 uint32_t* __restrict__ list_b, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t ca, cb, ba, bb;
+  if (threadIdx.x == 0) { ca = 0; cb = 0; }
+  __syncthreads();
+  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  bool pa = false, pb = false;
+  uint32_t la = 0, lb = 0;
+  if (u < n) {
+    target[u] = kNone;
+    if (!matched[u] && (buckets <= 1 || bucket_of(salt, u, buckets) == bucket)) {
+      if (ub[u] <= 1024) { pa = true; la = atomicAdd(&ca, 1u); } else { pb = true; lb = atomicAdd(&cb, 1u); }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ba = ca ? atomicAdd(&counts[0], ca) : 0;
+    bb = cb ? atomicAdd(&counts[1], cb) : 0;
+  }
+  __syncthreads();
+  if (pa) list_a[ba + la] = u;
+  if (pb) list_b[bb + lb] = u;
+}
+
+// joiners and pairs (counts: 2 = joiners, 3 = pairs, 4 = overflowing targets)
+__global__ void co_resolve_kernel(uint32_t n, const uint32_t* __restrict__ target, const int32_t* __restrict__ vw,
+                                  int64_t* __restrict__ jw, uint8_t* __restrict__ role) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const uint32_t t = target[u];
+    uint8_t r = 0;
+    if (t != kNone) {
+      if (target[t] == kNone) {
+        r = 1;  // joiner
+        atomicAdd(reinterpret_cast<unsigned long long*>(&jw[t]), static_cast<unsigned long long>(int64_t(vw[u])));
+      } else if (target[t] == u && u < t) {
+        r = 2;  // pair representative
+      }
+    }
+    role[u] = r;
+  }
+}
+
+__global__ void co_overflow_kernel(uint32_t n, const uint32_t* __restrict__ target, const uint8_t* __restrict__ role,
+                                   const int64_t* __restrict__ cw, const int64_t* __restrict__ jw, int64_t max_w,
+                                   uint32_t* __restrict__ olist, uint32_t* __restrict__ ocount) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    if (role[u] != 1) continue;
+    const uint32_t t = target[u];
+    if (cw[t] + jw[t] > max_w) olist[atomicAdd(ocount, 1u)] = u;
+  }
+}
+
+// admit: role 1 joiners join unless listed as rejected (role 3)
+__global__ void co_apply_kernel(uint32_t n, const uint32_t* __restrict__ target, const uint8_t* __restrict__ role,
+                                const int32_t* __restrict__ vw, uint32_t* __restrict__ cid, int64_t* __restrict__ cw,
+                                uint8_t* __restrict__ matched, uint32_t* __restrict__ contracted) {
+  uint32_t local = 0;
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const uint8_t r = role[u];
+    if (r == 1) {
+      const uint32_t t = target[u];
+      cid[u] = t;
+      atomicAdd(reinterpret_cast<unsigned long long*>(&cw[t]), static_cast<unsigned long long>(int64_t(vw[u])));
+      matched[u] = 1;
+      matched[t] = 1;
+      ++local;
+    } else if (r == 2) {
+      const uint32_t t = target[u];  // u < t, u represents the pair
+      cid[t] = u;
+      atomicAdd(reinterpret_cast<unsigned long long*>(&cw[u]), static_cast<unsigned long long>(int64_t(vw[t])));
+      matched[u] = 1;
+      matched[t] = 1;
+      ++local;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
+  if ((threadIdx.x & 63) == 0 && local) atomicAdd(contracted, local);
+}
+
+__global__ void co_init_kernel(uint32_t n, const int32_t* __restrict__ vw, uint32_t* __restrict__ cid,
+                               int64_t* __restrict__ cw, uint8_t* __restrict__ matched) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    cid[u] = u;
+    cw[u] = vw[u];
+    matched[u] = 0;
+  }
+}
+
+// ---------------------------------------------------------- contraction --
+__global__ void co_mark_kernel(const uint32_t* __restrict__ cid, uint32_t n, uint32_t* __restrict__ used) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) used[cid[u]] = 1;
+}
+
+__global__ void co_cmap_kernel(const uint32_t* __restrict__ cid, const uint32_t* __restrict__ map, uint32_t n,
+                               const int32_t* __restrict__ vw, uint32_t* __restrict__ cmap, int32_t* __restrict__ cvw,
+                               uint32_t* __restrict__ last) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const uint32_t c = map[cid[u]];
+    cmap[u] = c;
+    atomicAdd(&cvw[c], vw[u]);
+    atomicMax(&last[c], u);
+  }
+}
+
+// A coarse vertex takes the community of its highest-numbered member (all
+// members share one community when clustering respects communities).
+__global__ void co_comm_kernel(uint32_t nc, const uint32_t* __restrict__ last, const int32_t* __restrict__ comm,
+                               int32_t* __restrict__ ccomm) {
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+    ccomm[c] = comm[last[c]];
+}
+
+__global__ void co_pin_keys_kernel(const uint32_t* __restrict__ pins, const uint32_t* __restrict__ pin_net,
+                                   const uint32_t* __restrict__ cmap, uint64_t p, int bits,
+                                   uint64_t* __restrict__ keys) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < p; i += uint64_t(gridDim.x) * blockDim.x)
+    keys[i] = (uint64_t(pin_net[i]) << bits) | cmap[pins[i]];
+}
+
+// unique flags on the sorted (net, coarse pin) keys; per net distinct count
+// and an order-free 64-bit fingerprint of the pin set (sum of mixed pin ids;
+// it plays the role of the footprint of static_hypergraph.cpp:152,193-196)
+__global__ void co_unique_kernel(const uint64_t* __restrict__ keys, uint64_t p, int bits, uint32_t* __restrict__ flag,
+                                 uint32_t* __restrict__ size, uint64_t* __restrict__ foot) {
+  const uint64_t mask = (uint64_t(1) << bits) - 1;
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < p; i += uint64_t(gridDim.x) * blockDim.x) {
+    const bool first = i == 0 || keys[i - 1] != keys[i];
+    flag[i] = first ? 1u : 0u;
+    if (first) {
+      const uint32_t e = uint32_t(keys[i] >> bits);
+      const uint32_t v = uint32_t(keys[i] & mask);
+      atomicAdd(&size[e], 1u);
+      atomicAdd(reinterpret_cast<unsigned long long*>(&foot[e]), static_cast<unsigned long long>(mix64(v)));
+    }
+  }
+}
+
+__global__ void co_net_keys_kernel(uint32_t m, const uint32_t* __restrict__ size, const uint64_t* __restrict__ foot,
+                                   uint64_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                   uint32_t* __restrict__ nvalid) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+    const bool valid = size[e] > 1;
+    // invalid nets sort last: valid keys keep the top bit clear
+    keys[e] = valid ? ((foot[e] ^ uint64_t(size[e])) >> 1) : ~0ull;
+    vals[e] = e;
+    if (valid) atomicAdd(nvalid, 1u);
+  }
+}
+
+// start index of every run of equal fingerprints (max-scanned afterwards)
+__global__ void co_run_start_kernel(const uint64_t* __restrict__ skeys, uint32_t nvalid, uint64_t* __restrict__ rs) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nvalid; i += gridDim.x * blockDim.x)
+    rs[i] = (i == 0 || skeys[i - 1] != skeys[i]) ? i : 0;
+}
+
+// For every valid net in (fingerprint, id) order: the first earlier net of
+// the same run with identical pins is its representative (smallest id).
+__global__ void co_parallel_kernel(const uint32_t* __restrict__ sids, uint32_t nvalid,
+                                   const uint64_t* __restrict__ run_start, const uint32_t* __restrict__ size,
+                                   const uint32_t* __restrict__ ustart, const uint64_t* __restrict__ pk, int bits,
+                                   uint32_t* __restrict__ rep) {
+  const uint64_t mask = (uint64_t(1) << bits) - 1;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nvalid; i += gridDim.x * blockDim.x) {
+    const uint32_t e = sids[i];
+    uint32_t r = e;
+    const uint32_t sz = size[e];
+    for (uint32_t j = uint32_t(run_start[i]); j < i; ++j) {
+      const uint32_t f = sids[j];
+      if (size[f] != sz) continue;
+      bool same = true;
+      for (uint32_t q = 0; q < sz && same; ++q) same = (pk[ustart[e] + q] & mask) == (pk[ustart[f] + q] & mask);
+      if (same) { r = f; break; }
+    }
+    rep[e] = r;
+  }
+}
+
+__global__ void co_net_weight_kernel(uint32_t m, const uint32_t* __restrict__ size, const uint32_t* __restrict__ rep,
+                                     const int32_t* __restrict__ nw, int64_t* __restrict__ acc,
+                                     uint32_t* __restrict__ survivor) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+    const bool valid = size[e] > 1;
+    const bool surv = valid && rep[e] == e;
+    survivor[e] = surv ? 1u : 0u;
+    if (valid)
+      atomicAdd(reinterpret_cast<unsigned long long*>(&acc[rep[e]]), static_cast<unsigned long long>(int64_t(nw[e])));
+  }
+}
+
+__global__ void co_net_offsets_kernel(uint32_t m, const uint32_t* __restrict__ survivor,
+                                      const uint32_t* __restrict__ newid, const uint32_t* __restrict__ size,
+                                      const int64_t* __restrict__ acc, uint32_t* __restrict__ csize,
+                                      int32_t* __restrict__ cnw) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+    if (!survivor[e]) continue;
+    csize[newid[e]] = size[e];
+    cnw[newid[e]] = int32_t(acc[e]);
+  }
+}
+
+__global__ void co_copy_pins_kernel(uint32_t m, const uint32_t* __restrict__ survivor,
+                                    const uint32_t* __restrict__ newid, const uint32_t* __restrict__ ustart,
+                                    const uint32_t* __restrict__ size, const uint64_t* __restrict__ pk, int bits,
+                                    const uint32_t* __restrict__ coff, uint32_t* __restrict__ cpins) {
+  const uint64_t mask = (uint64_t(1) << bits) - 1;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+    if (!survivor[e]) continue;
+    const uint32_t c = newid[e], sz = size[e], src = ustart[e], dst = coff[c];
+    for (uint32_t q = 0; q < sz; ++q) cpins[dst + q] = uint32_t(pk[src + q] & mask);
+  }
+}
+
+__global__ void co_compact_keys_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ flag,
+                                       const uint32_t* __restrict__ pos, uint64_t p, uint64_t* __restrict__ out,
+                                       uint32_t* __restrict__ ustart, int bits) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < p; i += uint64_t(gridDim.x) * blockDim.x) {
+    if (!flag[i]) continue;
+    const uint32_t j = pos[i];
+    out[j] = keys[i];
+    const uint32_t e = uint32_t(keys[i] >> bits);
+    if (i == 0 || uint32_t(keys[i - 1] >> bits) != e) ustart[e] = j;
+  }
+}
+
+}  // namespace mkp
+
+// ------------------------------------------------------------ host side --
+namespace mkp {
+
+// One clustering pass on hg; cid (device, n) receives the representatives.
+ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, int pass, uint32_t* cid_out) {
+  Context& c = hg->ctx->c;
+  const hipStream_t s = c.stream;
+  const uint32_t n = hg->n;
+  ClusterResult res;
+  res.nodes_after = n;
+  // the per-net cluster set of the rating kernel holds 2048 slots
+  MKP_CHECK(prm.ignore_net_size >= 1 && prm.ignore_net_size <= 1025, "ignore_net_size must be in [1, 1025]");
+  if (n == 0) return res;
+  DevBuf<int64_t> cw(n), jw(n), visits(1);
+  DevBuf<uint8_t> matched(n), role(n);
+  DevBuf<uint32_t> target(n), list_a(n), list_b(n), counts(8), olist(n);
+  DevBuf<uint64_t> rating(n), ub(n);
+  co_init_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, hg->vw.p, cid_out, cw.p, matched.p);
+  co_ub_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(hg->voff.p, hg->inc.p, hg->noff.p, n, prm.ignore_net_size,
+                                                       ub.p);
+  fill_async<int64_t>(visits.p, 0, 1, s);
+  MKP_LAUNCH_CHECK();
+  // global rating maps for the heavy tier: sized by the largest bound
+  uint64_t max_ub = 0;
+  {
+    std::vector<uint64_t> h(n);
+    MKP_HIP(hipMemcpyAsync(h.data(), ub.p, size_t(n) * 8, hipMemcpyDeviceToHost, s));
+    c.sync();
+    for (uint64_t x : h) max_ub = std::max(max_ub, x);
+  }
+  const uint64_t cap = std::min<uint64_t>(max_ub, prm.sampling + 1000);
+  int gbits = 1;
+  while ((uint64_t(1) << gbits) < 2 * cap) ++gbits;
+  const uint32_t gblocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(256, (uint64_t(1) << 30) / (12ull << gbits))));
+  DevBuf<uint32_t> gkeys;
+  DevBuf<uint64_t> gvals;
+  if (max_ub > 1024) {
+    gkeys.alloc(uint64_t(gblocks) << gbits);
+    gvals.alloc(uint64_t(gblocks) << gbits);
+  }
+  const uint32_t limit = std::max<uint32_t>(uint32_t(double(n) / prm.max_shrink_factor), prm.contraction_limit);
+  uint32_t current = n;
+  const int S = std::max(1, prm.sub_rounds);
+  const uint32_t salt = bucket_salt(prm.seed, kCoBucketTag | uint64_t(pass));
+  CoArgs a;
+  a.n = n; a.voff = hg->voff.p; a.inc = hg->inc.p; a.noff = hg->noff.p; a.pins = hg->pins.p; a.nw = hg->nw.p;
+  a.comm = hg->comm.p; a.ub = ub.p; a.cid = cid_out; a.cw = cw.p; a.matched = matched.p; a.target = target.p;
+  a.rating = rating.p; a.ignore = prm.ignore_net_size; a.sampling = prm.sampling; a.max_w = prm.max_node_weight;
+  a.seed = prm.seed; a.prefer_unmatched = 1; a.visits = visits.p;
+  for (int sr = 0; sr < S; ++sr) {
+    if (current <= limit) break;
+    a.tag = (uint64_t(pass) << 8) | uint64_t(sr);
+    fill_async<uint32_t>(counts.p, 0u, 8, s);
+    co_select_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, matched.p, ub.p, salt, uint32_t(S), uint32_t(sr), target.p,
+                                                     list_a.p, list_b.p, counts.p);
+    MKP_LAUNCH_CHECK();
+    {
+      TimedRegion tr(c.prof, s, kFamRating, 0.0);
+      co_rate_kernel<64, 11><<<unsigned(c.num_cus) * 16, 64, 0, s>>>(a, list_a.p, counts.p + 0, 0, nullptr, nullptr);
+      MKP_LAUNCH_CHECK();
+      if (max_ub > 1024) {
+        co_rate_kernel<256, 0><<<gblocks, 256, 0, s>>>(a, list_b.p, counts.p + 1, gbits, gkeys.p, gvals.p);
+        MKP_LAUNCH_CHECK();
+      }
+    }
+    fill_async<int64_t>(jw.p, 0, n, s);
+    co_resolve_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, target.p, hg->vw.p, jw.p, role.p);
+    co_overflow_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, target.p, role.p, cw.p, jw.p,
+                                                               prm.max_node_weight, olist.p, counts.p + 4);
+    MKP_LAUNCH_CHECK();
+    uint32_t oc = 0;
+    MKP_HIP(hipMemcpyAsync(&oc, counts.p + 4, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    if (oc) {
+      // joiners of targets whose capacity would be exceeded: admit in
+      // (rating desc, hash, id) order up to the limit (host side; rare)
+      std::vector<uint32_t> ol(oc);
+      MKP_HIP(hipMemcpyAsync(ol.data(), olist.p, size_t(oc) * 4, hipMemcpyDeviceToHost, s));
+      std::vector<uint32_t> tg(n);
+      std::vector<uint64_t> rt(n);
+      std::vector<int64_t> cwh(n);
+      std::vector<int32_t> vwh(n);
+      MKP_HIP(hipMemcpyAsync(tg.data(), target.p, size_t(n) * 4, hipMemcpyDeviceToHost, s));
+      MKP_HIP(hipMemcpyAsync(rt.data(), rating.p, size_t(n) * 8, hipMemcpyDeviceToHost, s));
+      MKP_HIP(hipMemcpyAsync(cwh.data(), cw.p, size_t(n) * 8, hipMemcpyDeviceToHost, s));
+      MKP_HIP(hipMemcpyAsync(vwh.data(), hg->vw.p, size_t(n) * 4, hipMemcpyDeviceToHost, s));
+      c.sync();
+      std::map<uint32_t, std::vector<uint32_t>> groups;
+      for (uint32_t u : ol) groups[tg[u]].push_back(u);
+      std::vector<uint32_t> rejected;
+      for (auto& kv : groups) {
+        std::vector<uint32_t>& J = kv.second;
+        std::sort(J.begin(), J.end(), [&](uint32_t x, uint32_t y) {
+          if (rt[x] != rt[y]) return rt[x] > rt[y];
+          const uint64_t hx = hash4(prm.seed, kCoPrioTag | a.tag, x, 0), hy = hash4(prm.seed, kCoPrioTag | a.tag, y, 0);
+          if (hx != hy) return hx < hy;
+          return x < y;
+        });
+        int64_t cum = cwh[kv.first];
+        bool full = false;
+        for (uint32_t u : J) {
+          if (!full && cum + vwh[u] <= prm.max_node_weight) cum += vwh[u];
+          else { full = true; rejected.push_back(u); }
+        }
+      }
+      if (!rejected.empty()) {
+        DevBuf<uint32_t> rej(rejected.size());
+        MKP_HIP(hipMemcpyAsync(rej.p, rejected.data(), rejected.size() * 4, hipMemcpyHostToDevice, s));
+        // role 3 = rejected joiner
+        std::vector<uint8_t> roles(n);
+        MKP_HIP(hipMemcpyAsync(roles.data(), role.p, n, hipMemcpyDeviceToHost, s));
+        c.sync();
+        for (uint32_t u : rejected) roles[u] = 3;
+        MKP_HIP(hipMemcpyAsync(role.p, roles.data(), n, hipMemcpyHostToDevice, s));
+      }
+    }
+    fill_async<uint32_t>(counts.p + 5, 0u, 1, s);
+    co_apply_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, target.p, role.p, hg->vw.p, cid_out, cw.p, matched.p,
+                                                            counts.p + 5);
+    MKP_LAUNCH_CHECK();
+    uint32_t contracted = 0;
+    MKP_HIP(hipMemcpyAsync(&contracted, counts.p + 5, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    current -= contracted;
+  }
+  MKP_HIP(hipMemcpyAsync(&res.visits, visits.p, 8, hipMemcpyDeviceToHost, s));
+  c.sync();
+  res.nodes_after = current;
+  if (c.prof.enabled) {
+    // pin id + cluster gather for the adaptive-size sweep and the scoring
+    // sweep, net id/weight/size per incidence, offsets and weight per vertex
+    c.prof.bytes[kFamRating] += 8.0 * double(res.visits) + 12.0 * double(hg->p) + 12.0 * n;
+  }
+  return res;
+}
+
+// StaticHypergraph::contract in canonical form; returns the coarse
+// hypergraph and writes the fine -> coarse map into cmap (device, n).
+mkp_hypergraph* contract(mkp_hypergraph* hg, const uint32_t* cid, uint32_t* cmap) {
+  Context& c = hg->ctx->c;
+  const hipStream_t s = c.stream;
+  const uint32_t n = hg->n, m = hg->m;
+  const uint64_t p = hg->p;
+  TimedRegion tr(c.prof, s, kFamContraction, 12.0 * double(p) + 8.0 * double(p) + 16.0 * m + 16.0 * n);
+  auto* cg = new mkp_hypergraph();
+  try {
+    cg->ctx = hg->ctx;
+    DevBuf<uint32_t> used(std::max<uint32_t>(n, 1)), map(size_t(n) + 1), last(std::max<uint32_t>(n, 1));
+    fill_async<uint32_t>(used.p, 0u, n, s);
+    co_mark_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(cid, n, used.p);
+    exclusive_scan_u32(c, used.p, map.p, n);
+    uint32_t nc = 0;
+    MKP_HIP(hipMemcpyAsync(&nc, map.p + n, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    cg->n = nc;
+    cg->vw.alloc(std::max<uint32_t>(nc, 1));
+    cg->comm.alloc(std::max<uint32_t>(nc, 1));
+    fill_async<int32_t>(cg->vw.p, 0, nc, s);
+    fill_async<uint32_t>(last.p, 0u, nc, s);
+    co_cmap_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(cid, map.p, n, hg->vw.p, cmap, cg->vw.p, last.p);
+    if (nc) co_comm_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(nc, last.p, hg->comm.p, cg->comm.p);
+    MKP_LAUNCH_CHECK();
+    cg->total_w = hg->total_w;
+    int bits = 1;
+    while ((uint64_t(1) << bits) < nc) ++bits;
+    int nbits = 1;
+    while ((uint64_t(1) << nbits) < m) ++nbits;
+    MKP_CHECK(bits + nbits <= 64, "hypergraph too large for 64-bit pin keys");
+    // stage 2: sorted unique coarse pins per net
+    DevBuf<uint64_t> keys(std::max<uint64_t>(p, 1)), skeys(std::max<uint64_t>(p, 1)), ukeys(std::max<uint64_t>(p, 1));
+    DevBuf<uint32_t> flag(std::max<uint64_t>(p, 1)), pos(p + 1), size(std::max<uint32_t>(m, 1)),
+        ustart(std::max<uint32_t>(m, 1));
+    DevBuf<uint64_t> foot(std::max<uint32_t>(m, 1));
+    fill_async<uint32_t>(size.p, 0u, m, s);
+    fill_async<uint64_t>(foot.p, 0ull, m, s);
+    uint32_t U = 0;
+    if (p) {
+      co_pin_keys_kernel<<<grid_for(p, 256, 65536), 256, 0, s>>>(hg->pins.p, hg->pin_net.p, cmap, p, bits, keys.p);
+      MKP_LAUNCH_CHECK();
+      sort_keys_u64(c, keys.p, skeys.p, p, 0, bits + nbits);
+      co_unique_kernel<<<grid_for(p, 256, 65536), 256, 0, s>>>(skeys.p, p, bits, flag.p, size.p, foot.p);
+      MKP_LAUNCH_CHECK();
+      exclusive_scan_u32(c, flag.p, pos.p, p);
+      co_compact_keys_kernel<<<grid_for(p, 256, 65536), 256, 0, s>>>(skeys.p, flag.p, pos.p, p, ukeys.p, ustart.p,
+                                                                     bits);
+      MKP_LAUNCH_CHECK();
+      MKP_HIP(hipMemcpyAsync(&U, pos.p + p, 4, hipMemcpyDeviceToHost, s));
+    }
+    // stage 3: parallel nets
+    DevBuf<uint64_t> nkeys(std::max<uint32_t>(m, 1)), snkeys(std::max<uint32_t>(m, 1));
+    DevBuf<uint32_t> nvals(std::max<uint32_t>(m, 1)), snvals(std::max<uint32_t>(m, 1)), rep(std::max<uint32_t>(m, 1)),
+        survivor(std::max<uint32_t>(m, 1)), newid(size_t(m) + 1), nvalid(1);
+    DevBuf<int64_t> acc(std::max<uint32_t>(m, 1));
+    fill_async<uint32_t>(nvalid.p, 0u, 1, s);
+    fill_async<int64_t>(acc.p, 0, m, s);
+    uint32_t nv = 0;
+    if (m) {
+      co_net_keys_kernel<<<grid_for(m, 256, 65536), 256, 0, s>>>(m, size.p, foot.p, nkeys.p, nvals.p, nvalid.p);
+      MKP_LAUNCH_CHECK();
+      sort_pairs_u64_u32(c, nkeys.p, snkeys.p, nvals.p, snvals.p, m, 0, 64);
+      MKP_HIP(hipMemcpyAsync(&nv, nvalid.p, 4, hipMemcpyDeviceToHost, s));
+      c.sync();
+      if (nv) {
+        DevBuf<uint64_t> rs(nv), rsm(nv);
+        co_run_start_kernel<<<grid_for(nv, 256, 65536), 256, 0, s>>>(snkeys.p, nv, rs.p);
+        MKP_LAUNCH_CHECK();
+        inclusive_max_scan_u64(c, rs.p, rsm.p, nv);
+        co_parallel_kernel<<<grid_for(nv, 256, 65536), 256, 0, s>>>(snvals.p, nv, rsm.p, size.p, ustart.p, ukeys.p,
+                                                                    bits, rep.p);
+      }
+      co_net_weight_kernel<<<grid_for(m, 256, 65536), 256, 0, s>>>(m, size.p, rep.p, hg->nw.p, acc.p, survivor.p);
+      MKP_LAUNCH_CHECK();
+      exclusive_scan_u32(c, survivor.p, newid.p, m);
+    } else {
+      fill_async<uint32_t>(newid.p, 0u, 1, s);
+    }
+    uint32_t mc = 0;
+    MKP_HIP(hipMemcpyAsync(&mc, newid.p + m, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    // stage 4: coarse CSR in original net order
+    cg->m = mc;
+    cg->nw.alloc(std::max<uint32_t>(mc, 1));
+    DevBuf<uint32_t> csize(std::max<uint32_t>(mc, 1));
+    cg->noff.alloc(size_t(mc) + 1);
+    if (m) {
+      co_net_offsets_kernel<<<grid_for(m, 256, 65536), 256, 0, s>>>(m, survivor.p, newid.p, size.p, acc.p, csize.p,
+                                                                    cg->nw.p);
+      MKP_LAUNCH_CHECK();
+    }
+    exclusive_scan_u32(c, csize.p, cg->noff.p, mc);
+    uint32_t cp = 0;
+    MKP_HIP(hipMemcpyAsync(&cp, cg->noff.p + mc, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    cg->p = cp;
+    cg->pins.alloc(std::max<uint32_t>(cp, 1));
+    if (m && cp) {
+      co_copy_pins_kernel<<<grid_for(m, 256, 65536), 256, 0, s>>>(m, survivor.p, newid.p, ustart.p, size.p, ukeys.p,
+                                                                  bits, cg->noff.p, cg->pins.p);
+      MKP_LAUNCH_CHECK();
+    }
+    // max net size on the host from the offsets
+    {
+      std::vector<uint32_t> off(size_t(mc) + 1);
+      MKP_HIP(hipMemcpyAsync(off.data(), cg->noff.p, off.size() * 4, hipMemcpyDeviceToHost, s));
+      c.sync();
+      uint32_t me = 0;
+      for (uint32_t e = 0; e < mc; ++e) me = std::max(me, off[e + 1] - off[e]);
+      cg->max_esize = me;
+    }
+    build_incidence(cg);
+  } catch (...) {
+    delete cg;
+    throw;
+  }
+  return cg;
+}
+
+}  // namespace mkp
+
+using namespace mkp;
+
+extern "C" {
+
+int mkp_cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params* prm, int32_t pass, uint32_t* clusters,
+                     uint32_t* nodes_after, int64_t* pin_visits) {
+  return guarded([&] {
+    MKP_CHECK(hg && prm && clusters, "null argument");
+    MKP_CHECK(prm->sub_rounds >= 1 && prm->max_shrink_factor > 0, "bad parameters");
+    Context& c = hg->ctx->c;
+    MKP_HIP(hipSetDevice(c.device));
+    DevBuf<uint32_t> cid(std::max<uint32_t>(hg->n, 1));
+    ClusterResult r = cluster_pass(hg, *prm, pass, cid.p);
+    MKP_HIP(hipMemcpyAsync(clusters, cid.p, size_t(hg->n) * 4, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    if (nodes_after) *nodes_after = r.nodes_after;
+    if (pin_visits) *pin_visits = r.visits;
+  });
+}
+
+int mkp_hypergraph_contract(mkp_hypergraph* hg, const uint32_t* clusters, mkp_hypergraph** coarse, uint32_t* cmap) {
+  return guarded([&] {
+    MKP_CHECK(hg && clusters && coarse, "null argument");
+    for (uint32_t u = 0; u < hg->n; ++u) MKP_CHECK(clusters[u] < hg->n, "cluster id out of range");
+    Context& c = hg->ctx->c;
+    MKP_HIP(hipSetDevice(c.device));
+    DevBuf<uint32_t> cid(std::max<uint32_t>(hg->n, 1)), cm(std::max<uint32_t>(hg->n, 1));
+    MKP_HIP(hipMemcpyAsync(cid.p, clusters, size_t(hg->n) * 4, hipMemcpyHostToDevice, c.stream));
+    *coarse = contract(hg, cid.p, cm.p);
+    if (cmap) MKP_HIP(hipMemcpyAsync(cmap, cm.p, size_t(hg->n) * 4, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+  });
+}
+
+}  // extern "C"

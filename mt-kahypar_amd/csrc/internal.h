@@ -67,4 +67,13 @@ void build_incidence(mkp_hypergraph* hg);
 // connectivity, km1) on the device-resident part ids.
 void partition_initialize(mkp_partition* p);
 int64_t partition_km1(mkp_partition* p);
+
+// clustering pass / contraction / LP entry points shared by the drivers
+struct ClusterResult {
+  uint32_t nodes_after = 0;
+  int64_t visits = 0;
+};
+ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, int pass, uint32_t* cid_out);
+mkp_hypergraph* contract(mkp_hypergraph* hg, const uint32_t* cid, uint32_t* cmap);
+void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats);
 }  // namespace mkp

@@ -1,0 +1,214 @@
+// Level hierarchy, projection and the multilevel partitioner.
+//
+// Reference: MultilevelCoarsener::coarsenImpl (multilevel_coarsener.h:143-315),
+// MultilevelCoarsenerBase::doUncoarsen / refine (uncoarsening.cpp:94-188,
+// 404-457), partition() (partitioner.cpp:163-204) and the context set-up of
+// context.cpp:213-284.
+#include <chrono>
+#include <cmath>
+#include <memory>
+
+#include "../host/initial_partitioning.h"
+#include "internal.h"
+
+struct mkp_hierarchy {
+  mkp_hypergraph* top = nullptr;             // borrowed input
+  std::vector<mkp_hypergraph*> levels;       // owned coarse levels 1..L
+  std::vector<mkp::DevBuf<uint32_t>> cmaps;  // cmaps[i]: level i -> level i+1
+  int64_t rating_visits = 0;
+  ~mkp_hierarchy() {
+    for (auto* h : levels) delete h;
+  }
+  mkp_hypergraph* level(int i) const { return i == 0 ? top : levels[size_t(i) - 1]; }
+  int num_levels() const { return int(levels.size()) + 1; }
+};
+
+namespace mkp {
+
+__global__ void project_kernel(const int32_t* __restrict__ coarse_part, const uint32_t* __restrict__ cmap, uint32_t n,
+                               int32_t* __restrict__ part) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+    part[u] = coarse_part[cmap[u]];
+}
+
+mkp_hierarchy* coarsen(mkp_hypergraph* hg, const mkp_coarsen_params& prm) {
+  auto* h = new mkp_hierarchy();
+  try {
+    h->top = hg;
+    int pass = 0;
+    mkp_hypergraph* cur = hg;
+    while (cur->n > prm.contraction_limit) {
+      DevBuf<uint32_t> cid(std::max<uint32_t>(cur->n, 1));
+      ClusterResult r = cluster_pass(cur, prm, pass, cid.p);
+      h->rating_visits += r.visits;
+      // stop if the pass shrinks by at most the minimum factor
+      // (multilevel_coarsener.h:275-280)
+      if (r.nodes_after == 0 || double(cur->n) / double(r.nodes_after) <= prm.min_shrink_factor) break;
+      DevBuf<uint32_t> cmap(std::max<uint32_t>(cur->n, 1));
+      mkp_hypergraph* coarse = contract(cur, cid.p, cmap.p);
+      h->levels.push_back(coarse);
+      h->cmaps.push_back(std::move(cmap));
+      cur = coarse;
+      ++pass;
+    }
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  return h;
+}
+
+void project(mkp_partition* fine, const mkp_partition* coarse, const mkp_hierarchy* h, int level) {
+  Context& c = fine->hg->ctx->c;
+  project_kernel<<<grid_for(fine->hg->n, 256, 65536), 256, 0, c.stream>>>(coarse->part.p, h->cmaps[level].p,
+                                                                          fine->hg->n, fine->part.p);
+  MKP_LAUNCH_CHECK();
+  partition_initialize(fine);
+}
+
+mt_kahypar_amd::HostHypergraph download(mkp_hypergraph* hg) {
+  Context& c = hg->ctx->c;
+  mt_kahypar_amd::HostHypergraph h;
+  h.n = hg->n;
+  h.m = hg->m;
+  std::vector<uint32_t> off(size_t(hg->m) + 1);
+  h.pins.resize(hg->p);
+  h.vw.resize(hg->n);
+  h.nw.resize(hg->m);
+  MKP_HIP(hipMemcpyAsync(off.data(), hg->noff.p, off.size() * 4, hipMemcpyDeviceToHost, c.stream));
+  if (hg->p) MKP_HIP(hipMemcpyAsync(h.pins.data(), hg->pins.p, hg->p * 4, hipMemcpyDeviceToHost, c.stream));
+  if (hg->n) MKP_HIP(hipMemcpyAsync(h.vw.data(), hg->vw.p, size_t(hg->n) * 4, hipMemcpyDeviceToHost, c.stream));
+  if (hg->m) MKP_HIP(hipMemcpyAsync(h.nw.data(), hg->nw.p, size_t(hg->m) * 4, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  h.noff.assign(off.begin(), off.end());
+  mt_kahypar_amd::build_host_incidence(h);
+  return h;
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace mkp
+
+using namespace mkp;
+
+extern "C" {
+
+int mkp_coarsen(mkp_hypergraph* hg, const mkp_coarsen_params* prm, mkp_hierarchy** out) {
+  return guarded([&] {
+    MKP_CHECK(hg && prm && out, "null argument");
+    MKP_HIP(hipSetDevice(hg->ctx->c.device));
+    *out = coarsen(hg, *prm);
+  });
+}
+
+int mkp_hierarchy_num_levels(const mkp_hierarchy* h, int32_t* levels) {
+  return guarded([&] {
+    MKP_CHECK(h && levels, "null argument");
+    *levels = h->num_levels();
+  });
+}
+
+int mkp_hierarchy_level(mkp_hierarchy* h, int32_t level, mkp_hypergraph** hg) {
+  return guarded([&] {
+    MKP_CHECK(h && hg, "null argument");
+    MKP_CHECK(level >= 0 && level < h->num_levels(), "level out of range");
+    *hg = h->level(level);
+  });
+}
+
+int mkp_hierarchy_destroy(mkp_hierarchy* h) {
+  return guarded([&] { delete h; });
+}
+
+int mkp_partition_project(mkp_partition* fine, const mkp_partition* coarse, const mkp_hierarchy* h, int32_t level) {
+  return guarded([&] {
+    MKP_CHECK(fine && coarse && h, "null argument");
+    MKP_CHECK(level >= 0 && level + 1 < h->num_levels(), "level out of range");
+    MKP_CHECK(fine->hg == h->level(level) && coarse->hg == h->level(level + 1), "partitions not on levels i, i+1");
+    MKP_CHECK(fine->k == coarse->k, "different k");
+    project(fine, coarse, h, level);
+    fine->hg->ctx->c.sync();
+  });
+}
+
+int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm, int32_t* parts,
+                             mkp_partition_stats* stats) {
+  return guarded([&] {
+    MKP_CHECK(hg && prm && parts, "null argument");
+    MKP_CHECK(prm->k >= 2 && prm->epsilon >= 0, "bad k or epsilon");
+    Context& c = hg->ctx->c;
+    MKP_HIP(hipSetDevice(c.device));
+    const double t0 = now_ms();
+    const int k = prm->k;
+    // setupPartWeights / setupContractionLimit / setupMaximumAllowedNodeWeight
+    const int64_t W = hg->total_w;
+    const int64_t perfect = int64_t(std::ceil(double(W) / double(k)));
+    const int64_t maxw = int64_t(int32_t((1.0 + prm->epsilon) * double(perfect)));
+    const uint32_t climit = 160u * uint32_t(k);
+    int64_t max_node_w = int64_t(std::ceil(1.0 / double(climit) * double(W)));
+    max_node_w = std::min(max_node_w, maxw);
+    mkp_partition_stats st{};
+    // preprocess: community detection (partitioner.cpp:106-126)
+    if (prm->use_community_detection) {
+      mkp_louvain_params lp{-1, 5, 0.01, 200000, prm->seed, prm->louvain_sub_rounds > 0 ? prm->louvain_sub_rounds : 8};
+      mkp_louvain_stats ls{};
+      MKP_CHECK(mkp_louvain(hg, &lp, nullptr, &ls) == 0, mkp_last_error());
+      st.louvain_arcs = ls.arcs_swept;
+    } else {
+      fill_async<int32_t>(hg->comm.p, 0, hg->n, c.stream);
+    }
+    const double t1 = now_ms();
+    mkp_coarsen_params cp{climit, int32_t(max_node_w), 1.01, 2.5, 1000, 200000, prm->seed,
+                          prm->coarsening_sub_rounds > 0 ? prm->coarsening_sub_rounds : 16};
+    mkp_hierarchy* h = coarsen(hg, cp);
+    std::unique_ptr<mkp_hierarchy> hold(h);
+    st.rating_visits = h->rating_visits;
+    const double t2 = now_ms();
+    const int L = h->num_levels();
+    mkp_hypergraph* coarsest = h->level(L - 1);
+    mt_kahypar_amd::InitialPartitioningParams ip;
+    ip.k = k;
+    ip.max_w.assign(k, maxw);
+    ip.seed = prm->seed;
+    if (prm->ip_runs > 0) ip.runs = prm->ip_runs;
+    std::vector<int32_t> part = mt_kahypar_amd::initial_partition(download(coarsest), ip, nullptr);
+    const double t3 = now_ms();
+    std::vector<int64_t> maxv(k, maxw), perv(k, perfect);
+    mkp_lp_params lpp{5, prm->lp_sub_rounds > 0 ? prm->lp_sub_rounds : 4, 100, 1, 0, maxv.data(), perv.data()};
+    std::vector<std::unique_ptr<mkp_partition>> phg(L);
+    for (int i = L - 1; i >= 0; --i) {
+      mkp_partition* p = nullptr;
+      MKP_CHECK(mkp_partition_create(h->level(i), k, &p) == 0, mkp_last_error());
+      phg[i].reset(p);
+      if (i == L - 1) {
+        MKP_HIP(hipMemcpyAsync(p->part.p, part.data(), part.size() * 4, hipMemcpyHostToDevice, c.stream));
+        partition_initialize(p);
+      } else {
+        project(p, phg[i + 1].get(), h, i);
+        phg[i + 1].reset();
+      }
+      lpp.seed = prm->seed + uint64_t(i) * 1000003ull;
+      mkp_lp_stats ls{};
+      lp_refine(p, lpp, &ls);
+      st.lp_visits += ls.visits;
+    }
+    mkp_partition* top = phg[0].get();
+    MKP_HIP(hipMemcpyAsync(parts, top->part.p, size_t(hg->n) * 4, hipMemcpyDeviceToHost, c.stream));
+    std::vector<int64_t> pw(k);
+    MKP_HIP(hipMemcpyAsync(pw.data(), top->part_w.p, size_t(k) * 8, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    st.km1 = partition_km1(top);
+    double imb = 0;
+    for (int b = 0; b < k; ++b) imb = std::max(imb, double(pw[b]) / double(perfect));
+    st.imbalance = imb - 1.0;
+    st.levels = L;
+    const double t4 = now_ms();
+    st.louvain_ms = t1 - t0; st.coarsening_ms = t2 - t1; st.initial_ms = t3 - t2; st.refinement_ms = t4 - t3;
+    st.total_ms = t4 - t0;
+    if (stats) *stats = st;
+  });
+}
+
+}  // extern "C"

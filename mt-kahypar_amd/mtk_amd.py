@@ -42,6 +42,25 @@ class LouvainStats(C.Structure):
                 ("device_ms", C.c_double)]
 
 
+class CoarsenParams(C.Structure):
+    _fields_ = [("contraction_limit", C.c_uint32), ("max_node_weight", C.c_int32), ("min_shrink_factor", C.c_double),
+                ("max_shrink_factor", C.c_double), ("ignore_net_size", C.c_uint32), ("sampling", C.c_uint64),
+                ("seed", C.c_uint64), ("sub_rounds", C.c_int32)]
+
+
+class PartitionParams(C.Structure):
+    _fields_ = [("k", C.c_int32), ("epsilon", C.c_double), ("seed", C.c_uint64),
+                ("use_community_detection", C.c_int32), ("lp_sub_rounds", C.c_int32),
+                ("louvain_sub_rounds", C.c_int32), ("coarsening_sub_rounds", C.c_int32), ("ip_runs", C.c_int32)]
+
+
+class PartitionStats(C.Structure):
+    _fields_ = [("km1", C.c_int64), ("imbalance", C.c_double), ("levels", C.c_int32), ("louvain_ms", C.c_double),
+                ("coarsening_ms", C.c_double), ("initial_ms", C.c_double), ("refinement_ms", C.c_double),
+                ("total_ms", C.c_double), ("louvain_arcs", C.c_int64), ("rating_visits", C.c_int64),
+                ("lp_visits", C.c_int64)]
+
+
 class PowerlawParams(C.Structure):
     _fields_ = [("num_vertices", C.c_uint32), ("num_nets", C.c_uint32), ("exponent", C.c_double),
                 ("min_size", C.c_uint32), ("max_size", C.c_uint32), ("window", C.c_uint32),
@@ -83,6 +102,14 @@ SIGNATURES = {
     "mkp_partition_km1": (C.c_int, [vp, P(C.c_int64)]),
     "mkp_lp_refine": (C.c_int, [vp, P(LpParams), P(LpStats)]),
     "mkp_louvain": (C.c_int, [vp, P(LouvainParams), vp, P(LouvainStats)]),
+    "mkp_cluster_pass": (C.c_int, [vp, P(CoarsenParams), C.c_int32, vp, P(C.c_uint32), P(C.c_int64)]),
+    "mkp_hypergraph_contract": (C.c_int, [vp, vp, P(vp), vp]),
+    "mkp_coarsen": (C.c_int, [vp, P(CoarsenParams), P(vp)]),
+    "mkp_hierarchy_num_levels": (C.c_int, [vp, P(C.c_int32)]),
+    "mkp_hierarchy_level": (C.c_int, [vp, C.c_int32, P(vp)]),
+    "mkp_hierarchy_destroy": (C.c_int, [vp]),
+    "mkp_partition_project": (C.c_int, [vp, vp, vp, C.c_int32]),
+    "mkp_partition_hypergraph": (C.c_int, [vp, P(PartitionParams), vp, P(PartitionStats)]),
     "mkp_generate_powerlaw": (C.c_int, [P(PowerlawParams), P(vp)]),
     "mkp_generate_rownet": (C.c_int, [P(RownetParams), P(vp)]),
     "mkp_host_hypergraph_info": (C.c_int, [vp, P(HypergraphInfo)]),
@@ -229,6 +256,44 @@ class Hypergraph:
                      device_ms=st.device_ms)
         return (out[:self.info().num_vertices] if out is not None else None), stats
 
+    @staticmethod
+    def coarsen_params(contraction_limit, max_node_weight, min_shrink=1.01, max_shrink=2.5, ignore=1000,
+                       sampling=200000, seed=0, sub_rounds=16):
+        return CoarsenParams(contraction_limit, max_node_weight, min_shrink, max_shrink, ignore, sampling, seed,
+                             sub_rounds)
+
+    def cluster_pass(self, pass_nr, **kw):
+        prm = self.coarsen_params(**kw)
+        n = self.info().num_vertices
+        out = np.zeros(max(n, 1), np.uint32)
+        after, visits = C.c_uint32(), C.c_int64()
+        check(lib().mkp_cluster_pass(self.h, C.byref(prm), pass_nr, _p(out), C.byref(after), C.byref(visits)))
+        return out[:n], after.value, visits.value
+
+    def contract(self, clusters):
+        cl = np.ascontiguousarray(clusters, dtype=np.uint32)
+        n = self.info().num_vertices
+        cmap = np.zeros(max(n, 1), np.uint32)
+        h = vp()
+        check(lib().mkp_hypergraph_contract(self.h, _p(cl), C.byref(h), _p(cmap)))
+        return Hypergraph(self.ctx, handle=h), cmap[:n]
+
+    def coarsen(self, **kw):
+        prm = self.coarsen_params(**kw)
+        h = vp()
+        check(lib().mkp_coarsen(self.h, C.byref(prm), C.byref(h)))
+        return Hierarchy(self.ctx, h, self)
+
+    def partition(self, k, epsilon, seed, use_community_detection=True, lp_sub_rounds=4, louvain_sub_rounds=8,
+                  coarsening_sub_rounds=16, ip_runs=20):
+        prm = PartitionParams(k, epsilon, seed, int(use_community_detection), lp_sub_rounds, louvain_sub_rounds,
+                              coarsening_sub_rounds, ip_runs)
+        st = PartitionStats()
+        out = np.zeros(max(self.info().num_vertices, 1), np.int32)
+        check(lib().mkp_partition_hypergraph(self.h, C.byref(prm), _p(out), C.byref(st)))
+        stats = {f: getattr(st, f) for f, _ in PartitionStats._fields_}
+        return out[:self.info().num_vertices], stats
+
     def arrays(self):
         i = self.info()
         noff = np.zeros(i.num_nets + 1, np.uint64); pins = np.zeros(max(i.num_pins, 1), np.uint32)
@@ -243,6 +308,33 @@ class Hypergraph:
     def close(self):
         if self.h:
             check(lib().mkp_hypergraph_destroy(self.h))
+            self.h = vp()
+
+
+class Hierarchy:
+    """Level hierarchy of a coarsening run (level 0 is the input)."""
+
+    def __init__(self, ctx, handle, top):
+        self.ctx, self.h, self.top = ctx, handle, top
+
+    def num_levels(self):
+        v = C.c_int32()
+        check(lib().mkp_hierarchy_num_levels(self.h, C.byref(v)))
+        return v.value
+
+    def level(self, i):
+        h = vp()
+        check(lib().mkp_hierarchy_level(self.h, i, C.byref(h)))
+        hg = Hypergraph(self.ctx, handle=h)
+        hg.borrowed = True
+        return hg
+
+    def project(self, fine, coarse, level):
+        check(lib().mkp_partition_project(fine.h, coarse.h, self.h, level))
+
+    def close(self):
+        if self.h:
+            check(lib().mkp_hierarchy_destroy(self.h))
             self.h = vp()
 
 

@@ -140,14 +140,15 @@ class Hypergraph:
                                rounds=int(st[4]))
 
     def cluster_pass(self, sync, pass_nr, contraction_limit, max_node_weight, max_shrink=2.5, ignore=1000,
-                     sampling=200000, seed=0, sub_rounds=16, disable_rand=False, prefer_unmatched=True):
+                     sampling=200000, seed=0, sub_rounds=16, disable_rand=False, prefer_unmatched=True,
+                     with_visits=False):
         n = self.sizes()[0]
         out = np.zeros(n, np.uint32)
         visits = np.zeros(1, np.int64)
         after = lib().orc_cluster_pass(self.h, int(sync), pass_nr, contraction_limit, max_node_weight, max_shrink,
                                        ignore, sampling, seed, sub_rounds, int(disable_rand), int(prefer_unmatched),
                                        out, visits)
-        return out, after
+        return (out, after, int(visits[0])) if with_visits else (out, after)
 
     def contract(self, cluster):
         n = self.sizes()[0]

@@ -125,6 +125,12 @@ typedef struct {
  * (refinement/label_propagation/label_propagation_refiner.cpp:33-164), in
  * synchronous sub-rounds. */
 MKP_API int mkp_lp_refine(mkp_partition* p, const mkp_lp_params* params, mkp_lp_stats* stats);
+/* The same with refinement_nodes (IRefiner::refine, i_refiner.h:49-54): the
+ * given vertices (duplicates dropped) form the first active set; an empty
+ * list means all vertices, as in initializeActiveNodes
+ * (label_propagation_refiner.cpp:194-233). */
+MKP_API int mkp_lp_refine_nodes(mkp_partition* p, const mkp_lp_params* params, const uint32_t* nodes,
+                                uint32_t num_nodes, mkp_lp_stats* stats);
 
 /* ----------------------------------------------------------- Louvain --- */
 typedef struct {
@@ -195,7 +201,31 @@ typedef struct {
   int32_t louvain_sub_rounds;
   int32_t coarsening_sub_rounds;
   int32_t ip_runs;                 /* i-runs (20) */
+  /* preset values (config/fast_preset.ini); mkp_partition_params_default fills them */
+  int32_t louvain_edge_weight;     /* p-louvain-edge-weight-function: -1 hybrid, 0 uniform, 1 non_uniform, 2 degree */
+  int32_t louvain_max_passes;      /* p-max-louvain-pass-iterations (5) */
+  double louvain_min_move_fraction;/* p-louvain-min-vertex-move-fraction (0.01) */
+  uint64_t louvain_sampling;       /* p-vertex-degree-sampling-threshold (200000) */
+  double min_shrink_factor;        /* c-min-shrink-factor (1.01) */
+  double max_shrink_factor;        /* c-max-shrink-factor (2.5) */
+  double max_allowed_weight_multiplier; /* c-s (1) */
+  uint32_t contraction_limit_multiplier;/* c-t (160) */
+  uint32_t ignore_net_size;        /* maxnet-ignore (1000) */
+  uint64_t rating_sampling;        /* c-vertex-degree-sampling-threshold (200000) */
+  int32_t lp_enabled;              /* r-lp-type: label_propagation_km1 (1) or do_nothing (0) */
+  int32_t lp_max_rounds;           /* r-lp-maximum-iterations (5) */
+  uint32_t lp_activation_threshold;/* r-lp-he-size-activation-threshold (100) */
+  int32_t lp_rebalancing;          /* r-lp-rebalancing (1) */
 } mkp_partition_params;
+/* Initial partitioning of p's hypergraph (the coarsest level) on the host:
+ * a pool of greedy-growing and random starts, each improved by k-way km1
+ * hill climbing, best balanced result (simplified stand-in for
+ * recursive_bisection_initial_partitioner.cpp:475-492).  Block weights are
+ * bounded by (1 + epsilon) * ceil(W / k) (context.cpp:237-250); p is
+ * initialised with the result. */
+MKP_API int mkp_initial_partition(mkp_partition* p, double epsilon, uint64_t seed, int32_t runs, int64_t* km1);
+/* Fast-preset defaults with the given k, epsilon and seed. */
+MKP_API int mkp_partition_params_default(mkp_partition_params* params, int32_t k, double epsilon, uint64_t seed);
 typedef struct {
   int64_t km1;
   double imbalance;
@@ -239,6 +269,14 @@ MKP_API int mkp_host_hypergraph_copy(const mkp_host_hypergraph* h, uint64_t* net
 MKP_API int mkp_host_hypergraph_destroy(mkp_host_hypergraph* h);
 MKP_API int mkp_hypergraph_create_from_host(mkp_context* ctx, const mkp_host_hypergraph* h,
                                             mkp_hypergraph** out);
+
+/* ---------------------------------------------------------------- io -- */
+/* hMetis reader (io/hypergraph_io.cpp:95-177, readHypergraphFile): nets with
+ * fewer than two pins are dropped and counted in *removed_single_pin_nets. */
+MKP_API int mkp_read_hgr(const char* path, mkp_host_hypergraph** out, uint64_t* removed_single_pin_nets);
+MKP_API int mkp_write_hgr(const mkp_host_hypergraph* h, const char* path);
+/* writePartitionFile (io/hypergraph_io.cpp:319-333): one block id per line. */
+MKP_API int mkp_write_partition_file(const int32_t* parts, uint32_t num_vertices, const char* path);
 
 #ifdef __cplusplus
 }

@@ -48,7 +48,27 @@ def build(jobs=None):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    build_programs()
     return LIB
+
+
+def build_programs():
+    """Example and test programs that link the library through its public
+    headers only (C++ plugin interfaces, the C library interface in C)."""
+    src = os.path.join(ROOT, "tests", "cpp")
+    out = os.path.join(src, "_build")
+    os.makedirs(out, exist_ok=True)
+    rpath = "-Wl,-rpath,$ORIGIN/../../../mt-kahypar_amd"
+    jobs = [["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+             os.path.join(src, "plugin_pipeline.cpp"), "-L" + HERE, "-lmtkahypar_amd", rpath,
+             "-o", os.path.join(out, "plugin_pipeline")],
+            ["gcc", "-O2", "-std=c11", "-Wall", "-I" + os.path.join(ROOT, "include"),
+             os.path.join(src, "capi_partition.c"), "-L" + HERE, "-lmtkahypar_amd", rpath,
+             "-o", os.path.join(out, "capi_partition")]]
+    for cmd in jobs:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("program build failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
 
 
 if __name__ == "__main__":

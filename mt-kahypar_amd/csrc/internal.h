@@ -75,5 +75,7 @@ struct ClusterResult {
 };
 ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, int pass, uint32_t* cid_out);
 mkp_hypergraph* contract(mkp_hypergraph* hg, const uint32_t* cid, uint32_t* cmap);
-void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats);
+// nodes (host, distinct, ascending) replaces the initial active set if given
+void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, const uint32_t* nodes = nullptr,
+               uint32_t num_nodes = 0);
 }  // namespace mkp

@@ -453,6 +453,22 @@ __global__ void lp_compact_active_kernel(const uint32_t* __restrict__ flag, cons
     if (flag[u]) out[pos[u]] = u;
 }
 
+// isBorderNode as a 0/1 word per vertex (partitioned_hypergraph.h:605-620)
+__global__ void lp_border_flag_kernel(LpArgs a, uint32_t n, uint32_t* __restrict__ flag) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const uint32_t b = a.voff[u], e = a.voff[u + 1];
+    uint32_t border = 0;
+    if (e - b <= kHighDegree) {
+      for (uint32_t j = b; j < e && !border; ++j) {
+        int lambda = 0;
+        for (int ci = 0; ci < a.vcs; ++ci) lambda += __popcll(a.conn[uint64_t(a.inc[j]) * a.vcs + ci]);
+        border = lambda > 1;
+      }
+    }
+    flag[u] = border;
+  }
+}
+
 __global__ void iota_kernel(uint32_t* out, uint32_t n) {
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) out[u] = u;
 }
@@ -509,7 +525,8 @@ static LpWork& work_of(mkp_partition* p) {
   return *p->lp;
 }
 
-void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats) {
+void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, const uint32_t* nodes,
+               uint32_t num_nodes) {
   mkp_hypergraph* hg = p->hg;
   Context& c = hg->ctx->c;
   hipStream_t s = c.stream;
@@ -531,11 +548,27 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats) 
   const int B = std::max(1, prm.sub_rounds);
   a.buckets = B;
 
-  // initializeActiveNodes with rebalancing: all vertices (the non-border ones
-  // drop out in the gain kernel)
+  // initializeActiveNodes (label_propagation_refiner.cpp:194-233): the given
+  // refinement nodes (distinct, ascending), else all vertices with
+  // rebalancing (the non-border ones drop out in the gain kernel), else the
+  // border vertices
   uint32_t na = n;
-  iota_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(w.active.p, n);
-  MKP_LAUNCH_CHECK();
+  if (nodes) {
+    na = num_nodes;
+    if (na) MKP_HIP(hipMemcpyAsync(w.active.p, nodes, size_t(na) * 4, hipMemcpyHostToDevice, s));
+  } else if (prm.rebalancing) {
+    iota_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(w.active.p, n);
+    MKP_LAUNCH_CHECK();
+  } else {
+    lp_border_flag_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(a, n, w.flag.p);
+    exclusive_scan_u32(c, w.flag.p, w.pos.p, n);
+    lp_compact_active_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(w.flag.p, w.pos.p, n, w.active.p);
+    MKP_LAUNCH_CHECK();
+    uint32_t na32 = 0;
+    MKP_HIP(hipMemcpyAsync(&na32, w.pos.p + n, 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    na = na32;
+  }
   int64_t* host = c.pinned;
   int64_t delta_total = 0;
   int rounds = 0;
@@ -693,6 +726,25 @@ extern "C" int mkp_lp_refine(mkp_partition* p, const mkp_lp_params* prm, mkp_lp_
     MKP_CHECK(p && prm && prm->max_part_weights && prm->perfect_part_weights, "null argument");
     MKP_CHECK(prm->max_rounds >= 0 && prm->sub_rounds >= 1 && prm->sub_rounds <= 255, "bad round parameters");
     MKP_HIP(hipSetDevice(p->hg->ctx->c.device));
-    lp_refine(p, *prm, stats);
+    lp_refine(p, *prm, stats, nullptr, 0);
+  });
+}
+
+extern "C" int mkp_lp_refine_nodes(mkp_partition* p, const mkp_lp_params* prm, const uint32_t* nodes,
+                                   uint32_t num_nodes, mkp_lp_stats* stats) {
+  return guarded([&] {
+    MKP_CHECK(p && prm && prm->max_part_weights && prm->perfect_part_weights, "null argument");
+    MKP_CHECK(nodes || num_nodes == 0, "null refinement nodes");
+    MKP_CHECK(prm->max_rounds >= 0 && prm->sub_rounds >= 1 && prm->sub_rounds <= 255, "bad round parameters");
+    MKP_HIP(hipSetDevice(p->hg->ctx->c.device));
+    if (num_nodes == 0) {  // empty list: all vertices, as in the reference
+      lp_refine(p, *prm, stats, nullptr, 0);
+      return;
+    }
+    std::vector<uint32_t> v(nodes, nodes + num_nodes);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    MKP_CHECK(v.back() < p->hg->n, "refinement node out of range");
+    lp_refine(p, *prm, stats, v.data(), uint32_t(v.size()));
   });
 }

@@ -133,11 +133,66 @@ int mkp_partition_project(mkp_partition* fine, const mkp_partition* coarse, cons
   });
 }
 
+int mkp_initial_partition(mkp_partition* p, double epsilon, uint64_t seed, int32_t runs, int64_t* km1) {
+  return guarded([&] {
+    MKP_CHECK(p, "null argument");
+    MKP_CHECK(epsilon >= 0 && runs >= 1, "bad epsilon or runs");
+    Context& c = p->hg->ctx->c;
+    MKP_HIP(hipSetDevice(c.device));
+    const int k = p->k;
+    const int64_t W = p->hg->total_w;
+    const int64_t perfect = int64_t(std::ceil(double(W) / double(k)));
+    mt_kahypar_amd::InitialPartitioningParams ip;
+    ip.k = k;
+    ip.max_w.assign(k, int64_t(int32_t((1.0 + epsilon) * double(perfect))));
+    ip.seed = seed;
+    ip.runs = runs;
+    std::vector<int32_t> part = mt_kahypar_amd::initial_partition(download(p->hg), ip, nullptr);
+    if (!part.empty())
+      MKP_HIP(hipMemcpyAsync(p->part.p, part.data(), part.size() * 4, hipMemcpyHostToDevice, c.stream));
+    partition_initialize(p);
+    c.sync();
+    if (km1) *km1 = partition_km1(p);
+  });
+}
+
+int mkp_partition_params_default(mkp_partition_params* p, int32_t k, double epsilon, uint64_t seed) {
+  return guarded([&] {
+    MKP_CHECK(p, "null argument");
+    *p = mkp_partition_params{};
+    p->k = k;
+    p->epsilon = epsilon;
+    p->seed = seed;
+    p->use_community_detection = 1;
+    p->lp_sub_rounds = 4;
+    p->louvain_sub_rounds = 8;
+    p->coarsening_sub_rounds = 16;
+    p->ip_runs = 20;
+    p->louvain_edge_weight = -1;
+    p->louvain_max_passes = 5;
+    p->louvain_min_move_fraction = 0.01;
+    p->louvain_sampling = 200000;
+    p->min_shrink_factor = 1.01;
+    p->max_shrink_factor = 2.5;
+    p->max_allowed_weight_multiplier = 1.0;
+    p->contraction_limit_multiplier = 160;
+    p->ignore_net_size = 1000;
+    p->rating_sampling = 200000;
+    p->lp_enabled = 1;
+    p->lp_max_rounds = 5;
+    p->lp_activation_threshold = 100;
+    p->lp_rebalancing = 1;
+  });
+}
+
 int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm, int32_t* parts,
                              mkp_partition_stats* stats) {
   return guarded([&] {
     MKP_CHECK(hg && prm && parts, "null argument");
     MKP_CHECK(prm->k >= 2 && prm->epsilon >= 0, "bad k or epsilon");
+    MKP_CHECK(prm->contraction_limit_multiplier >= 1 && prm->max_shrink_factor > 1.0 &&
+                  prm->max_allowed_weight_multiplier > 0 && prm->lp_max_rounds >= 0,
+              "bad preset parameters");
     Context& c = hg->ctx->c;
     MKP_HIP(hipSetDevice(c.device));
     const double t0 = now_ms();
@@ -146,13 +201,15 @@ int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm
     const int64_t W = hg->total_w;
     const int64_t perfect = int64_t(std::ceil(double(W) / double(k)));
     const int64_t maxw = int64_t(int32_t((1.0 + prm->epsilon) * double(perfect)));
-    const uint32_t climit = 160u * uint32_t(k);
-    int64_t max_node_w = int64_t(std::ceil(1.0 / double(climit) * double(W)));
+    const uint32_t climit = prm->contraction_limit_multiplier * uint32_t(k);
+    int64_t max_node_w = int64_t(std::ceil(prm->max_allowed_weight_multiplier / double(climit) * double(W)));
     max_node_w = std::min(max_node_w, maxw);
     mkp_partition_stats st{};
     // preprocess: community detection (partitioner.cpp:106-126)
     if (prm->use_community_detection) {
-      mkp_louvain_params lp{-1, 5, 0.01, 200000, prm->seed, prm->louvain_sub_rounds > 0 ? prm->louvain_sub_rounds : 8};
+      mkp_louvain_params lp{prm->louvain_edge_weight, prm->louvain_max_passes, prm->louvain_min_move_fraction,
+                            prm->louvain_sampling, prm->seed,
+                            prm->louvain_sub_rounds > 0 ? prm->louvain_sub_rounds : 8};
       mkp_louvain_stats ls{};
       MKP_CHECK(mkp_louvain(hg, &lp, nullptr, &ls) == 0, mkp_last_error());
       st.louvain_arcs = ls.arcs_swept;
@@ -160,7 +217,8 @@ int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm
       fill_async<int32_t>(hg->comm.p, 0, hg->n, c.stream);
     }
     const double t1 = now_ms();
-    mkp_coarsen_params cp{climit, int32_t(max_node_w), 1.01, 2.5, 1000, 200000, prm->seed,
+    mkp_coarsen_params cp{climit, int32_t(max_node_w), prm->min_shrink_factor, prm->max_shrink_factor,
+                          prm->ignore_net_size, prm->rating_sampling, prm->seed,
                           prm->coarsening_sub_rounds > 0 ? prm->coarsening_sub_rounds : 16};
     mkp_hierarchy* h = coarsen(hg, cp);
     std::unique_ptr<mkp_hierarchy> hold(h);
@@ -176,7 +234,8 @@ int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm
     std::vector<int32_t> part = mt_kahypar_amd::initial_partition(download(coarsest), ip, nullptr);
     const double t3 = now_ms();
     std::vector<int64_t> maxv(k, maxw), perv(k, perfect);
-    mkp_lp_params lpp{5, prm->lp_sub_rounds > 0 ? prm->lp_sub_rounds : 4, 100, 1, 0, maxv.data(), perv.data()};
+    mkp_lp_params lpp{prm->lp_max_rounds, prm->lp_sub_rounds > 0 ? prm->lp_sub_rounds : 4,
+                      prm->lp_activation_threshold, prm->lp_rebalancing, 0, maxv.data(), perv.data()};
     std::vector<std::unique_ptr<mkp_partition>> phg(L);
     for (int i = L - 1; i >= 0; --i) {
       mkp_partition* p = nullptr;
@@ -189,10 +248,12 @@ int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm
         project(p, phg[i + 1].get(), h, i);
         phg[i + 1].reset();
       }
-      lpp.seed = prm->seed + uint64_t(i) * 1000003ull;
-      mkp_lp_stats ls{};
-      lp_refine(p, lpp, &ls);
-      st.lp_visits += ls.visits;
+      if (prm->lp_enabled && prm->lp_max_rounds > 0) {
+        lpp.seed = prm->seed + uint64_t(i) * 1000003ull;
+        mkp_lp_stats ls{};
+        lp_refine(p, lpp, &ls);
+        st.lp_visits += ls.visits;
+      }
     }
     mkp_partition* top = phg[0].get();
     MKP_HIP(hipMemcpyAsync(parts, top->part.p, size_t(hg->n) * 4, hipMemcpyDeviceToHost, c.stream));

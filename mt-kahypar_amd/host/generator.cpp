@@ -7,11 +7,7 @@
 
 #include "../../include/mtkahypar_amd.h"
 #include "../csrc/common.h"
-#include "initial_partitioning.h"
-
-struct mkp_host_hypergraph {
-  mt_kahypar_amd::HostHypergraph h;
-};
+#include "host_hypergraph.h"
 
 namespace {
 

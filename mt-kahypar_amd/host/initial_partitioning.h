@@ -11,16 +11,9 @@
 #include <cstdint>
 #include <vector>
 
-namespace mt_kahypar_amd {
+#include "host_hypergraph.h"
 
-struct HostHypergraph {
-  uint32_t n = 0, m = 0;
-  std::vector<uint64_t> noff;
-  std::vector<uint32_t> pins;
-  std::vector<uint64_t> voff;
-  std::vector<uint32_t> inc;
-  std::vector<int32_t> vw, nw;
-};
+namespace mt_kahypar_amd {
 
 struct InitialPartitioningParams {
   int k = 2;
@@ -29,9 +22,6 @@ struct InitialPartitioningParams {
   int runs = 20;                // fast_preset.ini:29 (i-runs)
   int refinement_passes = 8;
 };
-
-// Builds the incident-net arrays of a host hypergraph from its pin lists.
-void build_host_incidence(HostHypergraph& hg);
 
 // Returns a block id per vertex; *km1_out receives its km1.
 std::vector<int32_t> initial_partition(const HostHypergraph& hg, const InitialPartitioningParams& prm,

@@ -51,7 +51,13 @@ class CoarsenParams(C.Structure):
 class PartitionParams(C.Structure):
     _fields_ = [("k", C.c_int32), ("epsilon", C.c_double), ("seed", C.c_uint64),
                 ("use_community_detection", C.c_int32), ("lp_sub_rounds", C.c_int32),
-                ("louvain_sub_rounds", C.c_int32), ("coarsening_sub_rounds", C.c_int32), ("ip_runs", C.c_int32)]
+                ("louvain_sub_rounds", C.c_int32), ("coarsening_sub_rounds", C.c_int32), ("ip_runs", C.c_int32),
+                ("louvain_edge_weight", C.c_int32), ("louvain_max_passes", C.c_int32),
+                ("louvain_min_move_fraction", C.c_double), ("louvain_sampling", C.c_uint64),
+                ("min_shrink_factor", C.c_double), ("max_shrink_factor", C.c_double),
+                ("max_allowed_weight_multiplier", C.c_double), ("contraction_limit_multiplier", C.c_uint32),
+                ("ignore_net_size", C.c_uint32), ("rating_sampling", C.c_uint64), ("lp_enabled", C.c_int32),
+                ("lp_max_rounds", C.c_int32), ("lp_activation_threshold", C.c_uint32), ("lp_rebalancing", C.c_int32)]
 
 
 class PartitionStats(C.Structure):
@@ -101,6 +107,10 @@ SIGNATURES = {
     "mkp_partition_get_border": (C.c_int, [vp, vp]),
     "mkp_partition_km1": (C.c_int, [vp, P(C.c_int64)]),
     "mkp_lp_refine": (C.c_int, [vp, P(LpParams), P(LpStats)]),
+    "mkp_lp_refine_nodes": (C.c_int, [vp, P(LpParams), vp, C.c_uint32, P(LpStats)]),
+    "mkp_read_hgr": (C.c_int, [C.c_char_p, P(vp), P(C.c_uint64)]),
+    "mkp_write_hgr": (C.c_int, [vp, C.c_char_p]),
+    "mkp_write_partition_file": (C.c_int, [vp, C.c_uint32, C.c_char_p]),
     "mkp_louvain": (C.c_int, [vp, P(LouvainParams), vp, P(LouvainStats)]),
     "mkp_cluster_pass": (C.c_int, [vp, P(CoarsenParams), C.c_int32, vp, P(C.c_uint32), P(C.c_int64)]),
     "mkp_hypergraph_contract": (C.c_int, [vp, vp, P(vp), vp]),
@@ -110,6 +120,8 @@ SIGNATURES = {
     "mkp_hierarchy_destroy": (C.c_int, [vp]),
     "mkp_partition_project": (C.c_int, [vp, vp, vp, C.c_int32]),
     "mkp_partition_hypergraph": (C.c_int, [vp, P(PartitionParams), vp, P(PartitionStats)]),
+    "mkp_partition_params_default": (C.c_int, [P(PartitionParams), C.c_int32, C.c_double, C.c_uint64]),
+    "mkp_initial_partition": (C.c_int, [vp, C.c_double, C.c_uint64, C.c_int32, P(C.c_int64)]),
     "mkp_generate_powerlaw": (C.c_int, [P(PowerlawParams), P(vp)]),
     "mkp_generate_rownet": (C.c_int, [P(RownetParams), P(vp)]),
     "mkp_host_hypergraph_info": (C.c_int, [vp, P(HypergraphInfo)]),
@@ -198,6 +210,16 @@ class HostHypergraph:
         check(lib().mkp_generate_rownet(C.byref(prm), C.byref(h)))
         return HostHypergraph(h)
 
+    @staticmethod
+    def read_hgr(path):
+        """hMetis file; returns (hypergraph, number of dropped single-pin nets)."""
+        h, removed = vp(), C.c_uint64()
+        check(lib().mkp_read_hgr(str(path).encode(), C.byref(h), C.byref(removed)))
+        return HostHypergraph(h), removed.value
+
+    def write_hgr(self, path):
+        check(lib().mkp_write_hgr(self.h, str(path).encode()))
+
     def info(self):
         i = HypergraphInfo()
         check(lib().mkp_host_hypergraph_info(self.h, C.byref(i)))
@@ -284,10 +306,15 @@ class Hypergraph:
         check(lib().mkp_coarsen(self.h, C.byref(prm), C.byref(h)))
         return Hierarchy(self.ctx, h, self)
 
-    def partition(self, k, epsilon, seed, use_community_detection=True, lp_sub_rounds=4, louvain_sub_rounds=8,
-                  coarsening_sub_rounds=16, ip_runs=20):
-        prm = PartitionParams(k, epsilon, seed, int(use_community_detection), lp_sub_rounds, louvain_sub_rounds,
-                              coarsening_sub_rounds, ip_runs)
+    def partition(self, k, epsilon, seed, **overrides):
+        """Multilevel partition with the fast preset; keyword arguments
+        override fields of the parameter struct (e.g. lp_sub_rounds=8)."""
+        prm = PartitionParams()
+        check(lib().mkp_partition_params_default(C.byref(prm), k, epsilon, seed))
+        for key, val in overrides.items():
+            if key not in dict(PartitionParams._fields_):
+                raise KeyError(key)
+            setattr(prm, key, int(val) if isinstance(val, bool) else val)
         st = PartitionStats()
         out = np.zeros(max(self.info().num_vertices, 1), np.int32)
         check(lib().mkp_partition_hypergraph(self.h, C.byref(prm), _p(out), C.byref(st)))
@@ -388,13 +415,17 @@ class Partition:
         return v.value
 
     def lp_refine(self, max_w, perfect_w, seed=0, max_rounds=5, sub_rounds=4, activation_threshold=100,
-                  rebalancing=True):
+                  rebalancing=True, nodes=None):
         mw = np.ascontiguousarray(max_w, dtype=np.int64)
         pw = np.ascontiguousarray(perfect_w, dtype=np.int64)
         prm = LpParams(max_rounds, sub_rounds, activation_threshold, int(rebalancing), seed,
                        mw.ctypes.data_as(P(C.c_int64)), pw.ctypes.data_as(P(C.c_int64)))
         st = LpStats()
-        check(lib().mkp_lp_refine(self.h, C.byref(prm), C.byref(st)))
+        if nodes is None:
+            check(lib().mkp_lp_refine(self.h, C.byref(prm), C.byref(st)))
+        else:
+            nd = np.ascontiguousarray(nodes, dtype=np.uint32)
+            check(lib().mkp_lp_refine_nodes(self.h, C.byref(prm), _p(nd), len(nd), C.byref(st)))
         return dict(delta=st.delta, moves=st.moves, reverts=st.reverts, visits=st.visits, rounds=st.rounds,
                     device_ms=st.device_ms)
 

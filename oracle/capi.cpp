@@ -87,7 +87,7 @@ void orc_max_gain_move(void* h, int k, const int32_t* part, uint32_t u, const in
 // stats: delta, moves, reverts, visits, rounds
 int64_t orc_lp_refine(void* h, int k, int32_t* part, const int64_t* max_w, const int64_t* perfect_w,
                       uint64_t seed, int max_rounds, int sub_rounds, int sync, int threads, int disable_rand,
-                      int64_t* stats) {
+                      int rebalancing, const uint32_t* nodes, uint32_t num_nodes, int64_t* stats) {
   auto* hg = static_cast<Hypergraph*>(h);
   Partition p;
   phg_init(p, *hg, k, part);
@@ -100,6 +100,13 @@ int64_t orc_lp_refine(void* h, int k, int32_t* part, const int64_t* max_w, const
   prm.sub_rounds = sub_rounds;
   prm.threads = threads;
   prm.disable_randomization = disable_rand != 0;
+  prm.rebalancing = rebalancing != 0;
+  if (nodes && num_nodes) {
+    prm.refinement_nodes.assign(nodes, nodes + num_nodes);
+    std::sort(prm.refinement_nodes.begin(), prm.refinement_nodes.end());
+    prm.refinement_nodes.erase(std::unique(prm.refinement_nodes.begin(), prm.refinement_nodes.end()),
+                               prm.refinement_nodes.end());
+  }
   LpStats st = sync ? lp_refine_sync(p, prm) : lp_refine_ref(p, prm);
   std::copy(p.part.begin(), p.part.end(), part);
   if (stats) { stats[0] = st.delta; stats[1] = st.moves; stats[2] = st.reverts; stats[3] = st.visits; stats[4] = st.rounds; }

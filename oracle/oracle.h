@@ -107,6 +107,9 @@ struct LpParams {
   int sub_rounds = 4;                 // sync only
   bool disable_randomization = false; // ref only
   int threads = 1;                    // ref only
+  // initial active set (distinct, ascending); empty = all / border vertices
+  // (initializeActiveNodes, label_propagation_refiner.cpp:194-233)
+  std::vector<uint32_t> refinement_nodes;
 };
 struct LpStats {
   int64_t delta = 0;       // km1 change

@@ -41,7 +41,7 @@ def lib():
             "orc_max_gain_move": (None, [vp, C.c_int, i32p, C.c_uint32, i64p, C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32)]),
             "orc_lp_refine": (C.c_int64, [vp, C.c_int, i32p, i64p, i64p, C.c_uint64, C.c_int, C.c_int, C.c_int,
-                                          C.c_int, C.c_int, i64p]),
+                                          C.c_int, C.c_int, C.c_int, u32p, C.c_uint32, i64p]),
             "orc_graph_new": (vp, [vp, C.c_int]),
             "orc_graph_free": (None, [vp]),
             "orc_graph_sizes": (None, [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_double),
@@ -130,12 +130,14 @@ class Hypergraph:
         return to.value, gain.value
 
     def lp_refine(self, k, part, max_w, perfect_w, seed=0, max_rounds=5, sub_rounds=4, sync=True, threads=1,
-                  disable_rand=False):
+                  disable_rand=False, rebalancing=True, nodes=None):
         part = np.array(part, dtype=np.int32)
         st = np.zeros(5, np.int64)
+        nd = np.zeros(1, np.uint32) if nodes is None else np.ascontiguousarray(nodes, dtype=np.uint32)
         km1 = lib().orc_lp_refine(self.h, k, part, np.ascontiguousarray(max_w, dtype=np.int64),
                                   np.ascontiguousarray(perfect_w, dtype=np.int64), seed, max_rounds, sub_rounds,
-                                  int(sync), threads, int(disable_rand), st)
+                                  int(sync), threads, int(disable_rand), int(rebalancing), nd,
+                                  0 if nodes is None else len(nd), st)
         return part, km1, dict(delta=int(st[0]), moves=int(st[1]), reverts=int(st[2]), visits=int(st[3]),
                                rounds=int(st[4]))
 

@@ -347,8 +347,12 @@ LpStats lp_refine_sync(Partition& p, const LpParams& prm) {
   const int B = std::max(1, prm.sub_rounds);
   LpStats st;
   std::vector<uint32_t> active;
-  for (uint32_t u = 0; u < hg.n; ++u)
-    if (prm.rebalancing || is_border(p, u)) active.push_back(u);
+  if (!prm.refinement_nodes.empty()) {
+    active = prm.refinement_nodes;
+  } else {
+    for (uint32_t u = 0; u < hg.n; ++u)
+      if (prm.rebalancing || is_border(p, u)) active.push_back(u);
+  }
   std::vector<uint8_t> next_flag(hg.n, 0);
   std::vector<int64_t> score(k, 0);
   bool stop = false;

@@ -1,0 +1,64 @@
+// Runs the multilevel pipeline through the C++ plugin interfaces
+// (run_parallel_louvain, MultilevelCoarsener / ICoarsener, initial
+// partitioning, LabelPropagationRefiner / IRefiner) and checks that it gives
+// the same block ids and km1 as the one-call pipeline mkp_partition_hypergraph.
+//
+// usage: plugin_pipeline <file.hgr> <k> <seed>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <vector>
+
+#include "mtkahypar_amd_plugins.h"
+
+using namespace mt_kahypar_amd;
+
+int main(int argc, char** argv) {
+  if (argc != 4) {
+    std::fprintf(stderr, "usage: %s <file.hgr> <k> <seed>\n", argv[0]);
+    return 2;
+  }
+  const int k = std::atoi(argv[2]);
+  const uint64_t seed = std::strtoull(argv[3], nullptr, 10);
+  try {
+    mkp_host_hypergraph* host = nullptr;
+    check(mkp_read_hgr(argv[1], &host, nullptr));
+    mkp_hypergraph_info info{};
+    check(mkp_host_hypergraph_info(host, &info));
+    std::vector<uint64_t> noff(info.num_nets + 1);
+    std::vector<uint32_t> pins(info.num_pins);
+    std::vector<int32_t> vw(info.num_vertices), nw(info.num_nets);
+    check(mkp_host_hypergraph_copy(host, noff.data(), pins.data(), vw.data(), nw.data()));
+    check(mkp_host_hypergraph_destroy(host));
+
+    Context ctx(0);
+    PartitionContext pc(k, 0.03, seed);
+
+    // plugin path
+    Hypergraph hg(ctx, info.num_vertices, info.num_nets, noff, pins, vw.data(), nw.data());
+    run_parallel_louvain(hg, pc);
+    MultilevelCoarsener coarsener(hg, pc, true);
+    coarsener.coarsen();
+    initial_partition(coarsener.coarsestPartitionedHypergraph(), pc);
+    std::unique_ptr<IRefiner> lp(new LabelPropagationRefiner(pc));
+    std::unique_ptr<IRefiner> fm;
+    PartitionedHypergraph phg = coarsener.uncoarsen(lp, fm);
+    const std::vector<int32_t> parts = phg.partIDs();
+    const int64_t km1 = phg.km1();
+
+    // one-call path on a fresh copy
+    Hypergraph hg2(ctx, info.num_vertices, info.num_nets, noff, pins, vw.data(), nw.data());
+    std::vector<int32_t> parts2(info.num_vertices);
+    mkp_partition_stats st{};
+    check(mkp_partition_hypergraph(hg2.handle(), &pc.p, parts2.data(), &st));
+
+    const bool same = parts == parts2 && km1 == st.km1;
+    std::printf("levels=%d km1_plugins=%lld km1_pipeline=%lld imbalance=%.4f %s\n", coarsener.numLevels(),
+                static_cast<long long>(km1), static_cast<long long>(st.km1), st.imbalance,
+                same ? "SAME" : "DIFFERENT");
+    return same ? 0 : 1;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 3;
+  }
+}

@@ -28,6 +28,8 @@ import numpy as np  # noqa: E402
 import mtk_amd  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30  # calibration / ceiling copy: 1 GiB read + 1 GiB written
+PROFILES = os.path.join(ROOT, "profiles")
 
 WORKLOADS = {
     "powerlaw_1M_2M_k8": dict(num_vertices=1_000_000, num_nets=2_000_000, exponent=2.2, min_size=2,
@@ -70,6 +72,70 @@ def cpu_baseline(w, arrays, part, maxw, perfect, threads):
     return dict(louvain_s=t1 - t0, lp_s=t2 - t1, pins=lst["arcs"] // 2 + st["visits"], km1=km1)
 
 
+def traffic_per_launch(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled by the factor the
+    same counter shows for the calibration copy), or None without one."""
+    path = os.path.join(PROFILES, "hbm_traffic_%s.json" % workload)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return round(k["fetch_bytes"] / t["fetch_factor"] + k["write_bytes"] / t["write_factor"])
+
+
+def normalize_kernel_name(name):
+    """'void mkp::f<mkp::StarView, 32>(args)' -> 'f<StarView, 32>'."""
+    name = name.strip().strip('"')
+    if name.startswith("void "):
+        name = name[5:]
+    depth, cut = 0, len(name)
+    for i, ch in enumerate(name):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = i
+            break
+    return name[:cut].replace("mkp::", "").strip()
+
+
+def pmc_summary(fetch_dir, write_dir, workload):
+    """Per-kernel mean FETCH_SIZE / WRITE_SIZE per dispatch (bytes) from two
+    rocprofv3 --pmc runs of this script, and the factors the calibration copy
+    (COPY_BYTES read and written) gives; written to profiles/."""
+    import csv
+    import glob
+
+    def load(d, counter):
+        acc = {}
+        for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(fn) as f:
+                for row in csv.DictReader(f):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    k = normalize_kernel_name(row["Kernel_Name"])
+                    a = acc.setdefault(k, [0.0, 0])
+                    a[0] += float(row["Counter_Value"]) * 1024.0  # KiB per dispatch
+                    a[1] += 1
+        return {k: v[0] / v[1] for k, v in acc.items() if v[1]}
+
+    fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    copy = "stream_copy_kernel"
+    out = dict(workload=workload, fetch_factor=fetch[copy] / COPY_BYTES, write_factor=write[copy] / COPY_BYTES,
+               calibration="stream_copy_kernel: %d bytes read and %d written per dispatch" % (COPY_BYTES, COPY_BYTES),
+               kernels={k: dict(fetch_bytes=fetch.get(k, 0.0), write_bytes=write.get(k, 0.0))
+                        for k in sorted(set(fetch) | set(write))})
+    path = os.path.join(PROFILES, "hbm_traffic_%s.json" % workload)
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(path)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,7 +143,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="powerlaw_1M_2M_k8", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarise two rocprofv3 --pmc output folders of this script into profiles/ and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.workload)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (requested --gpus %d)"
               % args.gpus, file=sys.stderr)
@@ -129,15 +200,26 @@ def main():
             log("  %-18s %9.3f ms %6d launches %10.3f GB/s" % (f, v["ms"], v["launches"],
                                                               v["bytes"] / (v["ms"] * 1e6) if v["ms"] else 0))
 
-    # dominant kernel family with an algorithmic byte count
-    fams = {f: v for f, v in prof.items() if v["launches"] and v["bytes"] > 0}
-    dom = max(fams, key=lambda f: fams[f]["ms"])
-    d = fams[dom]
-    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+    # dominant kernel: the single kernel (instantiation) with the most device
+    # time among those with an algorithmic byte count; every launch of it was
+    # timed on its own with an event pair on the launch stream
+    kern = {k: v for k, v in ctx.kernel_profile().items() if v["launches"] and v["bytes"] > 0}
+    for name, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"]):
+        log("  kernel %-52s %8.3f ms %5d launches %8.1f us/launch %9.1f GB/s" % (
+            name, v["ms"], v["launches"], 1e3 * v["ms"] / v["launches"], v["bytes"] / (v["ms"] * 1e6)))
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    d = kern[dom]
+    avg_ms = d["ms"] / d["launches"]
+    bytes_per_launch = d["bytes"] / d["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = traffic_per_launch(args.workload, dom)
+    copy_ms = ctx.stream_copy(COPY_BYTES, 20)
     roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    avg_launch_ms=round(d["ms"] / d["launches"], 4),
-                    algorithmic_bytes_per_launch=round(d["bytes"] / d["launches"]))
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                    avg_launch_us=round(avg_ms * 1e3, 2), launches=d["launches"],
+                    algorithmic_bytes_per_launch=round(bytes_per_launch),
+                    share_of_step_device_time=round(d["ms"] / sum(v["ms"] for v in prof.values()), 3),
+                    measured_copy_GBs=round(2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9, 1))
 
     cpu = None
     if not args.no_cpu:

@@ -50,6 +50,17 @@ MKP_API int mkp_context_reset_profile(mkp_context* ctx);
 MKP_API int mkp_context_get_profile(mkp_context* ctx, int family, double* device_ms, double* algorithmic_bytes,
                                     int64_t* launches);
 MKP_API const char* mkp_profile_family_name(int family);
+/* Per-kernel records: launches timed one at a time (an event pair around a
+ * single kernel), keyed by kernel name and template variant, with their
+ * summed device time and algorithmic bytes.  *name stays valid until the
+ * next reset. */
+MKP_API int mkp_context_profile_kernel_count(mkp_context* ctx, int32_t* count);
+MKP_API int mkp_context_profile_kernel(mkp_context* ctx, int32_t index, const char** name, double* device_ms,
+                                       double* algorithmic_bytes, int64_t* launches);
+/* Copies `bytes` bytes device to device `reps` times with 16-byte vector
+ * loads and stores and returns the mean time of one copy: a measured copy
+ * ceiling, and the calibration kernel for HBM traffic counters. */
+MKP_API int mkp_stream_copy(mkp_context* ctx, uint64_t bytes, int32_t reps, double* ms_per_copy);
 
 /* --------------------------------------------------------- hypergraph -- */
 typedef struct {

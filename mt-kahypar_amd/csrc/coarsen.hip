@@ -704,7 +704,7 @@ int mkp_cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params* prm, int32_t 
     MKP_CHECK(hg && prm && clusters, "null argument");
     MKP_CHECK(prm->sub_rounds >= 1 && prm->max_shrink_factor > 0, "bad parameters");
     Context& c = hg->ctx->c;
-    MKP_HIP(hipSetDevice(c.device));
+    Bind bind_(c);
     DevBuf<uint32_t> cid(std::max<uint32_t>(hg->n, 1));
     ClusterResult r = cluster_pass(hg, *prm, pass, cid.p);
     MKP_HIP(hipMemcpyAsync(clusters, cid.p, size_t(hg->n) * 4, hipMemcpyDeviceToHost, c.stream));
@@ -719,7 +719,7 @@ int mkp_hypergraph_contract(mkp_hypergraph* hg, const uint32_t* clusters, mkp_hy
     MKP_CHECK(hg && clusters && coarse, "null argument");
     for (uint32_t u = 0; u < hg->n; ++u) MKP_CHECK(clusters[u] < hg->n, "cluster id out of range");
     Context& c = hg->ctx->c;
-    MKP_HIP(hipSetDevice(c.device));
+    Bind bind_(c);
     DevBuf<uint32_t> cid(std::max<uint32_t>(hg->n, 1)), cm(std::max<uint32_t>(hg->n, 1));
     MKP_HIP(hipMemcpyAsync(cid.p, clusters, size_t(hg->n) * 4, hipMemcpyHostToDevice, c.stream));
     *coarse = contract(hg, cid.p, cm.p);

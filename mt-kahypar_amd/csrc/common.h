@@ -6,6 +6,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -84,30 +87,117 @@ constexpr uint64_t kCoTieTag = 0x434F540000000000ull;
 constexpr uint64_t kCoPrioTag = 0x434F500000000000ull;
 
 // ------------------------------------------------------------ buffers --
+// Cache of device allocations of one context.  Blocks come in power-of-two
+// size classes and go back to the cache when their buffer is released, so the
+// steady state of repeated calls allocates nothing and, unlike hipFree, a
+// release never waits for the device.  A block is reused only by work of the
+// same context, i.e. on the same stream, where stream order makes reuse safe.
+struct DevicePool {
+  std::mutex mu;
+  std::multimap<size_t, void*> blocks;  // size class -> cached block
+  size_t cached = 0;
+  bool closed = false;
+  static constexpr size_t kMaxCached = size_t(64) << 30;
+  static size_t size_class(size_t bytes) {
+    size_t c = 256;
+    while (c < bytes) c <<= 1;
+    return c;
+  }
+  void* get(size_t cls) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      auto it = blocks.find(cls);
+      if (it != blocks.end()) {
+        void* p = it->second;
+        blocks.erase(it);
+        cached -= cls;
+        return p;
+      }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, cls);
+    if (e != hipSuccess) {  // return the cache to the device and retry once
+      trim();
+      MKP_HIP(hipMalloc(&p, cls));
+    }
+    return p;
+  }
+  void put(void* p, size_t cls) {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      if (!closed && cached + cls <= kMaxCached) {
+        blocks.emplace(cls, p);
+        cached += cls;
+        return;
+      }
+    }
+    (void)hipFree(p);
+  }
+  void trim() {
+    std::multimap<size_t, void*> b;
+    {
+      std::lock_guard<std::mutex> g(mu);
+      b.swap(blocks);
+      cached = 0;
+    }
+    for (auto& kv : b) (void)hipFree(kv.second);
+  }
+  void close() {
+    {
+      std::lock_guard<std::mutex> g(mu);
+      closed = true;
+    }
+    trim();
+  }
+};
+
+// Pool of the context whose entry point runs on this thread (set by Bind).
+inline thread_local std::shared_ptr<DevicePool> t_pool;
+
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  size_t cls = 0;                     // pooled size class, 0 = plain allocation
+  std::shared_ptr<DevicePool> pool;   // owner of a pooled block
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), cls(o.cls), pool(std::move(o.pool)) {
+    o.p = nullptr; o.n = 0; o.cls = 0;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    if (this != &o) {
+      release();
+      p = o.p; n = o.n; cls = o.cls; pool = std::move(o.pool);
+      o.p = nullptr; o.n = 0; o.cls = 0;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
   void alloc(size_t count) {
     release();
     n = count;
-    if (count) MKP_HIP(hipMalloc(&p, count * sizeof(T)));
+    if (!count) return;
+    if (t_pool) {
+      pool = t_pool;
+      cls = DevicePool::size_class(count * sizeof(T));
+      p = static_cast<T*>(pool->get(cls));
+    } else {
+      MKP_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
   }
   void ensure(size_t count) { if (count > n) alloc(count); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (pool) pool->put(p, cls);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    cls = 0;
+    pool.reset();
   }
   T* get() const { return p; }
 };
@@ -128,12 +218,21 @@ enum Family : int {
 
 struct Profiler {
   bool enabled = false;
-  struct Rec { hipEvent_t a, b; int fam; double bytes; };
+  // kname: single-kernel record (nullptr for a family-only region)
+  struct Rec { hipEvent_t a, b; int fam; double bytes; std::string kname; };
+  struct KernelStat { std::string name; double ms = 0, bytes = 0; int64_t launches = 0; };
   std::vector<Rec> open;
   std::vector<hipEvent_t> pool;
   double ms[kNumFamilies] = {};
   double bytes[kNumFamilies] = {};
   int64_t launches[kNumFamilies] = {};
+  std::vector<KernelStat> kernels;  // per kernel (instantiation) name
+  KernelStat& kernel(const std::string& name) {
+    for (auto& k : kernels)
+      if (k.name == name) return k;
+    kernels.push_back(KernelStat{name});
+    return kernels.back();
+  }
   hipEvent_t take() {
     if (pool.empty()) {
       hipEvent_t e;
@@ -152,6 +251,12 @@ struct Profiler {
         ms[r.fam] += t;
         bytes[r.fam] += r.bytes;
         launches[r.fam] += 1;
+        if (!r.kname.empty()) {
+          KernelStat& k = kernel(r.kname);
+          k.ms += t;
+          k.bytes += r.bytes;
+          k.launches += 1;
+        }
       }
       pool.push_back(r.a);
       pool.push_back(r.b);
@@ -160,24 +265,30 @@ struct Profiler {
   }
   void reset() {
     for (int i = 0; i < kNumFamilies; ++i) { ms[i] = 0; bytes[i] = 0; launches[i] = 0; }
+    kernels.clear();
   }
 };
 
-// Scoped event pair around the launches of one kernel family.
+// Scoped event pair around the launches of one kernel family; with a kernel
+// name it times exactly one launch and is also recorded per kernel.
+// Kernel names follow the demangled form without namespaces and arguments,
+// e.g. "louvain_move_group_kernel<StarView, 32, 8, 256>".
 struct TimedRegion {
   Profiler& p;
   hipStream_t s;
   int fam;
   double bytes;
+  std::string kname;
   hipEvent_t a = nullptr;
-  TimedRegion(Profiler& prof, hipStream_t st, int f, double b) : p(prof), s(st), fam(f), bytes(b) {
+  TimedRegion(Profiler& prof, hipStream_t st, int f, double b, std::string kernel_name = std::string())
+      : p(prof), s(st), fam(f), bytes(b), kname(std::move(kernel_name)) {
     if (p.enabled) { a = p.take(); (void)hipEventRecord(a, s); }
   }
   ~TimedRegion() {
     if (a) {
       hipEvent_t b = p.take();
       (void)hipEventRecord(b, s);
-      p.open.push_back({a, b, fam, bytes});
+      p.open.push_back({a, b, fam, bytes, std::move(kname)});
     }
   }
 };
@@ -197,8 +308,20 @@ struct Context {
     if (!prof.open.empty()) prof.collect();
   }
   void* temp(size_t bytes) { tmp.ensure(bytes ? bytes : 1); return tmp.p; }
+  std::shared_ptr<DevicePool> pool = std::make_shared<DevicePool>();
   template <typename T>
   T* scratch_as(int i, size_t count) { scratch[i].ensure(count * sizeof(T)); return reinterpret_cast<T*>(scratch[i].p); }
+};
+
+// Entry-point scope on a context: its device is current and device buffers
+// allocated on this thread come from its pool.
+struct Bind {
+  std::shared_ptr<DevicePool> prev;
+  explicit Bind(Context& c) : prev(t_pool) {
+    MKP_HIP(hipSetDevice(c.device));
+    t_pool = c.pool;
+  }
+  ~Bind() { t_pool = prev; }
 };
 
 template <typename T>

@@ -86,6 +86,13 @@ PcLayout pc_layout(int k, uint32_t max_esize) {
   return L;
 }
 
+// Plain device copy, 16 bytes per lane per iteration.
+__global__ void __launch_bounds__(256) stream_copy_kernel(const uint4* __restrict__ in, uint4* __restrict__ out,
+                                                          uint64_t n) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    out[i] = in[i];
+}
+
 }  // namespace mkp
 
 using namespace mkp;
@@ -117,10 +124,13 @@ int mkp_context_create(int device, mkp_context** out) {
 int mkp_context_destroy(mkp_context* ctx) {
   return guarded([&] {
     if (!ctx) return;
-    MKP_HIP(hipSetDevice(ctx->c.device));
+    Bind bind_(ctx->c);
     (void)hipStreamSynchronize(ctx->c.stream);
     ctx->c.tmp.release();
     for (auto& s : ctx->c.scratch) s.release();
+    // cached blocks go back to the device; buffers still alive (objects
+    // destroyed after their context) are freed directly when released
+    ctx->c.pool->close();
     if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
     if (ctx->c.ev_a) (void)hipEventDestroy(ctx->c.ev_a);
     if (ctx->c.ev_b) (void)hipEventDestroy(ctx->c.ev_b);
@@ -160,6 +170,51 @@ int mkp_context_get_profile(mkp_context* ctx, int family, double* ms, double* by
   });
 }
 
+int mkp_context_profile_kernel_count(mkp_context* ctx, int32_t* count) {
+  return guarded([&] {
+    MKP_CHECK(ctx && count, "null argument");
+    ctx->c.sync();
+    *count = int32_t(ctx->c.prof.kernels.size());
+  });
+}
+
+int mkp_context_profile_kernel(mkp_context* ctx, int32_t index, const char** name, double* ms, double* bytes,
+                               int64_t* launches) {
+  return guarded([&] {
+    MKP_CHECK(ctx, "null argument");
+    ctx->c.sync();
+    MKP_CHECK(index >= 0 && size_t(index) < ctx->c.prof.kernels.size(), "kernel record index out of range");
+    const auto& k = ctx->c.prof.kernels[size_t(index)];
+    if (name) *name = k.name.c_str();
+    if (ms) *ms = k.ms;
+    if (bytes) *bytes = k.bytes;
+    if (launches) *launches = k.launches;
+  });
+}
+
+int mkp_stream_copy(mkp_context* ctx, uint64_t bytes, int32_t reps, double* ms_per_copy) {
+  return guarded([&] {
+    MKP_CHECK(ctx && ms_per_copy, "null argument");
+    MKP_CHECK(bytes >= 16 && bytes % 16 == 0 && reps >= 1, "bytes must be a positive multiple of 16");
+    Context& c = ctx->c;
+    Bind bind_(c);
+    const uint64_t n = bytes / 16;
+    DevBuf<uint4> a(n), b(n);
+    fill_async<uint32_t>(reinterpret_cast<uint32_t*>(a.p), 1u, n * 4, c.stream);
+    const unsigned grid = unsigned(std::min<uint64_t>((n + 255) / 256, uint64_t(c.num_cus) * 32));
+    stream_copy_kernel<<<grid, 256, 0, c.stream>>>(a.p, b.p, n);  // warm-up
+    MKP_LAUNCH_CHECK();
+    MKP_HIP(hipEventRecord(c.ev_a, c.stream));
+    for (int32_t r = 0; r < reps; ++r) stream_copy_kernel<<<grid, 256, 0, c.stream>>>(a.p, b.p, n);
+    MKP_LAUNCH_CHECK();
+    MKP_HIP(hipEventRecord(c.ev_b, c.stream));
+    c.sync();
+    float ms = 0;
+    MKP_HIP(hipEventElapsedTime(&ms, c.ev_a, c.ev_b));
+    *ms_per_copy = double(ms) / reps;
+  });
+}
+
 const char* mkp_profile_family_name(int family) {
   static const char* names[kNumFamilies] = {"pin_count_init", "lp_gain", "lp_apply", "louvain_move",
                                             "louvain_contract", "rating", "contraction"};
@@ -173,7 +228,7 @@ int mkp_hypergraph_create(mkp_context* ctx, uint32_t n, uint32_t m, const uint64
     const uint64_t p = noff[m];
     MKP_CHECK(p < (uint64_t(1) << 32), "more than 2^32-1 pins are not supported");
     MKP_CHECK(p == 0 || pins, "pins is null");
-    MKP_HIP(hipSetDevice(ctx->c.device));
+    Bind bind_(ctx->c);
     std::vector<uint32_t> off32(size_t(m) + 1);
     uint32_t max_esize = 0;
     for (uint32_t e = 0; e <= m; ++e) {
@@ -220,7 +275,7 @@ int mkp_hypergraph_create(mkp_context* ctx, uint32_t n, uint32_t m, const uint64
 int mkp_hypergraph_destroy(mkp_hypergraph* hg) {
   return guarded([&] {
     if (!hg) return;
-    MKP_HIP(hipSetDevice(hg->ctx->c.device));
+    Bind bind_(hg->ctx->c);
     (void)hipStreamSynchronize(hg->ctx->c.stream);
     delete hg;
   });
@@ -237,7 +292,7 @@ int mkp_hypergraph_get_info(const mkp_hypergraph* hg, mkp_hypergraph_info* info)
 int mkp_hypergraph_set_communities(mkp_hypergraph* hg, const int32_t* comm) {
   return guarded([&] {
     MKP_CHECK(hg && comm, "null argument");
-    MKP_HIP(hipSetDevice(hg->ctx->c.device));
+    Bind bind_(hg->ctx->c);
     MKP_HIP(hipMemcpyAsync(hg->comm.p, comm, size_t(hg->n) * 4, hipMemcpyHostToDevice, hg->ctx->c.stream));
     hg->ctx->c.sync();
   });
@@ -247,7 +302,7 @@ int mkp_hypergraph_copy_to_host(const mkp_hypergraph* hg, uint64_t* noff, uint32
                                 uint32_t* inc, int32_t* vw, int32_t* nw, int32_t* comm) {
   return guarded([&] {
     MKP_CHECK(hg, "null hypergraph");
-    MKP_HIP(hipSetDevice(hg->ctx->c.device));
+    Bind bind_(hg->ctx->c);
     hipStream_t s = hg->ctx->c.stream;
     std::vector<uint32_t> t;
     if (noff) {

@@ -33,6 +33,7 @@ namespace mkp {
 // the global arc order).  `unit` marks hypergraphs whose net weights are all
 // one, so the uniform arc weight needs no gather.
 struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
+  static constexpr const char* kName = "StarView";
   uint32_t N, M;
   uint64_t P;
   const uint32_t* voff;
@@ -50,6 +51,12 @@ struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
     return u < N ? voff[u + 1] - voff[u] : noff[u - N + 1] - noff[u - N];
   }
   __device__ uint32_t source(uint64_t a) const { return a < P ? inc_vtx[a] : N + pin_net[a - P]; }
+  // algorithmic bytes of one arc of u: head id + cluster gather, plus the
+  // weight inputs the weighting reads per arc
+  __device__ uint32_t arc_bytes(uint32_t u) const {
+    if (u < N) return 8u + (wt == 0 ? (unit ? 0u : 4u) : 12u);
+    return 8u + (wt == 0 ? 0u : 8u);
+  }
   __device__ double weight(int32_t w, uint32_t size, uint32_t deg) const {
     if (wt == 0) return double(w);
     if (wt == 1) return double(w) / double(size);
@@ -70,6 +77,7 @@ struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
 };
 
 struct PlainView {  // all nets have two pins (graph.cpp:401-446)
+  static constexpr const char* kName = "PlainView";
   uint32_t N;
   const uint32_t* voff;
   const uint32_t* inc;
@@ -83,6 +91,8 @@ struct PlainView {  // all nets have two pins (graph.cpp:401-446)
   __device__ uint64_t offset(uint64_t u) const { return voff[u]; }
   __device__ uint32_t degree(uint32_t u) const { return voff[u + 1] - voff[u]; }
   __device__ uint32_t source(uint64_t a) const { return inc_vtx[a]; }
+  // net id, net offset, both pins, cluster gather (+ net weight)
+  __device__ uint32_t arc_bytes(uint32_t) const { return 20u + (unit ? 0u : 4u); }
   __device__ void arc(uint32_t u, uint32_t i, uint32_t& head, double& w) const {
     const uint32_t e = inc[voff[u] + i];
     const uint32_t p0 = pins[noff[e]], p1 = pins[noff[e] + 1];
@@ -94,6 +104,7 @@ struct PlainView {  // all nets have two pins (graph.cpp:401-446)
 };
 
 struct ExplicitView {  // contracted levels
+  static constexpr const char* kName = "ExplicitView";
   uint32_t n;
   const uint64_t* idx;
   const uint32_t* head;
@@ -103,6 +114,8 @@ struct ExplicitView {  // contracted levels
   __device__ uint64_t offset(uint64_t u) const { return idx[u]; }
   __device__ uint32_t degree(uint32_t u) const { return uint32_t(idx[u + 1] - idx[u]); }
   __device__ uint32_t source(uint64_t a) const { return src[a]; }
+  // head id, f64 weight, cluster gather
+  __device__ uint32_t arc_bytes(uint32_t) const { return 16u; }
   __device__ void arc(uint32_t u, uint32_t i, uint32_t& h, double& ww) const {
     const uint64_t a = idx[u] + i;
     h = head[a];
@@ -463,16 +476,26 @@ __device__ inline int lv_slot(const View& g, const LvArgs& a, uint32_t u) {
 // per (block, slot)
 template <class View>
 __global__ void __launch_bounds__(256)
-lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts) {
+lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uint64_t* __restrict__ sbytes) {
   __shared__ uint32_t h[4 * 256];
+  __shared__ unsigned long long hb[4 * 256];
   const int slots = 4 * a.buckets;
-  for (int i = threadIdx.x; i < slots; i += blockDim.x) h[i] = 0;
+  for (int i = threadIdx.x; i < slots; i += blockDim.x) { h[i] = 0; hb[i] = 0; }
   __syncthreads();
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u < n) atomicAdd(&h[lv_slot(g, a, u)], 1u);
+  if (u < n) {
+    const int slot = lv_slot(g, a, u);
+    atomicAdd(&h[slot], 1u);
+    if (sbytes) {  // algorithmic bytes of the node's move: 28 per node + its (sampled) arcs
+      const uint32_t d = min(g.degree(u), a.sampling);
+      atomicAdd(&hb[slot], 28ull + uint64_t(d) * g.arc_bytes(u));
+    }
+  }
   __syncthreads();
-  for (int i = threadIdx.x; i < slots; i += blockDim.x)
+  for (int i = threadIdx.x; i < slots; i += blockDim.x) {
     if (h[i]) atomicAdd(&counts[i], h[i]);
+    if (sbytes && hb[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&sbytes[i]), hb[i]);
+  }
 }
 
 template <class View>
@@ -611,24 +634,38 @@ struct LouvainRun {
 
   // launches the four degree classes of sub-round `sr`; lists are grouped by
   // slot = class * B + bucket with offsets off[] and sizes cnt[]
+  // (sb: algorithmic bytes per slot, profiling only); each launch is timed
+  // on its own so the per-kernel averages match a kernel trace
   template <class View>
   void launch_move(const View& g, const LvArgs& a, const uint32_t* list, const std::vector<uint32_t>& off,
-                   const std::vector<uint32_t>& cnt, int sr, int tbits_global, DevBuf<int32_t>& gk,
-                   DevBuf<double>& gw, DevBuf<uint32_t>& gf) {
+                   const std::vector<uint32_t>& cnt, const std::vector<uint64_t>& sb, int sr, int tbits_global,
+                   DevBuf<int32_t>& gk, DevBuf<double>& gw, DevBuf<uint32_t>& gf) {
     const hipStream_t s = c.stream;
     const int B = a.buckets;
     const int s0 = 0 * B + sr, s1 = 1 * B + sr, s2 = 2 * B + sr, s3 = 3 * B + sr;
-    if (cnt[s0])
+    if (cnt[s0]) {
+      TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s0]),
+                     std::string("louvain_move_small_kernel<") + View::kName + ">");
       louvain_move_small_kernel<View><<<grid_for(cnt[s0], 256, 65536), 256, 0, s>>>(g, a, list + off[s0], cnt[s0]);
-    if (cnt[s1])
+    }
+    if (cnt[s1]) {
+      TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s1]),
+                     std::string("louvain_move_group_kernel<") + View::kName + ", 32, 8, 256>");
       louvain_move_group_kernel<View, 32, 8, 256><<<grid_for(cnt[s1], 8, 65536), 256, 0, s>>>(g, a, list + off[s1],
                                                                                               cnt[s1]);
-    if (cnt[s2])
+    }
+    if (cnt[s2]) {
+      TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s2]),
+                     std::string("louvain_move_group_kernel<") + View::kName + ", 64, 11, 64>");
       louvain_move_group_kernel<View, 64, 11, 64><<<grid_for(cnt[s2], 1, 65536), 64, 0, s>>>(g, a, list + off[s2],
                                                                                              cnt[s2]);
-    if (cnt[s3])
+    }
+    if (cnt[s3]) {
+      TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s3]),
+                     std::string("louvain_move_global_kernel<") + View::kName + ">");
       louvain_move_global_kernel<View><<<unsigned(std::min<uint32_t>(cnt[s3], global_blocks_)), 256, 0, s>>>(
           g, a, list + off[s3], cnt[s3], tbits_global, gk.p, gw.p, gf.p);
+    }
     MKP_LAUNCH_CHECK();
   }
 
@@ -644,6 +681,8 @@ struct LouvainRun {
     DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
     DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), counts(slots), cursor(slots), offsets(slots), mcount(1),
         mu(std::max<uint32_t>(n, 1)), moved_total(1);
+    DevBuf<uint64_t> sbytes(slots);
+    std::vector<uint64_t> sb(slots, 0);
     DevBuf<int32_t> mto(std::max<uint32_t>(n, 1));
     lv_init_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, nvol, C, cvol.p, csize.p);
     MKP_LAUNCH_CHECK();
@@ -680,9 +719,12 @@ struct LouvainRun {
       fill_async<uint32_t>(counts.p, 0u, slots, s);
       fill_async<uint32_t>(cursor.p, 0u, slots, s);
       fill_async<uint32_t>(moved_total.p, 0u, 1, s);
-      lv_count_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, counts.p);
+      const bool prof = c.prof.enabled;
+      if (prof) fill_async<uint64_t>(sbytes.p, 0ull, slots, s);
+      lv_count_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, counts.p, prof ? sbytes.p : nullptr);
       MKP_LAUNCH_CHECK();
       MKP_HIP(hipMemcpyAsync(cnt.data(), counts.p, size_t(slots) * 4, hipMemcpyDeviceToHost, s));
+      if (prof) MKP_HIP(hipMemcpyAsync(sb.data(), sbytes.p, size_t(slots) * 8, hipMemcpyDeviceToHost, s));
       c.sync();
       uint32_t run = 0;
       for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
@@ -692,8 +734,7 @@ struct LouvainRun {
       for (int sr = 0; sr < B; ++sr) {
         a.bucket = sr;
         fill_async<uint32_t>(mcount.p, 0u, 1, s);
-        TimedRegion tr(c.prof, s, kFamLouvainMove, 0.0);
-        launch_move(g, a, list.p, off, cnt, sr, tbits, gk, gw, gf);
+        launch_move(g, a, list.p, off, cnt, sb, sr, tbits, gk, gw, gf);
         // at most one move per node of this sub-round
         const uint32_t sub_nodes = cnt[sr] + cnt[B + sr] + cnt[2 * B + sr] + cnt[3 * B + sr];
         louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(mu.p, mto.p, mcount.p, nvol, C,
@@ -707,13 +748,6 @@ struct LouvainRun {
       st.arcs_swept += int64_t(num_arcs);
       st.rounds += 1;
       st.moves += int64_t(moved);
-      if (c.prof.enabled) {
-        // level 0: 20P + 28N + 32E (per pin: two head ids, two cluster
-        // gathers, a net weight; per node: offsets, volume, cluster, cluster
-        // volume, write); coarse levels: 16 per arc + 28 per node
-        c.prof.bytes[kFamLouvainMove] += level == 0 ? 10.0 * double(num_arcs) + 28.0 * double(n)
-                                                    : 16.0 * double(num_arcs) + 28.0 * double(n);
-      }
       changed |= moved > 0;
     }
     return changed;
@@ -855,7 +889,7 @@ extern "C" int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* prm, in
     MKP_CHECK(hg && prm, "null argument");
     MKP_CHECK(prm->max_passes >= 0 && prm->sub_rounds >= 1 && prm->sub_rounds <= 255, "bad round parameters");
     Context& c = hg->ctx->c;
-    MKP_HIP(hipSetDevice(c.device));
+    Bind bind_(c);
     MKP_HIP(hipEventRecord(c.ev_a, c.stream));
     // configurePreprocessing (partitioner.cpp:62-72): hybrid -> degree if
     // |E|/|V| < 0.75, else uniform

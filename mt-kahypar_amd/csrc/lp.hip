@@ -725,7 +725,7 @@ extern "C" int mkp_lp_refine(mkp_partition* p, const mkp_lp_params* prm, mkp_lp_
   return guarded([&] {
     MKP_CHECK(p && prm && prm->max_part_weights && prm->perfect_part_weights, "null argument");
     MKP_CHECK(prm->max_rounds >= 0 && prm->sub_rounds >= 1 && prm->sub_rounds <= 255, "bad round parameters");
-    MKP_HIP(hipSetDevice(p->hg->ctx->c.device));
+    Bind bind_(p->hg->ctx->c);
     lp_refine(p, *prm, stats, nullptr, 0);
   });
 }
@@ -736,7 +736,7 @@ extern "C" int mkp_lp_refine_nodes(mkp_partition* p, const mkp_lp_params* prm, c
     MKP_CHECK(p && prm && prm->max_part_weights && prm->perfect_part_weights, "null argument");
     MKP_CHECK(nodes || num_nodes == 0, "null refinement nodes");
     MKP_CHECK(prm->max_rounds >= 0 && prm->sub_rounds >= 1 && prm->sub_rounds <= 255, "bad round parameters");
-    MKP_HIP(hipSetDevice(p->hg->ctx->c.device));
+    Bind bind_(p->hg->ctx->c);
     if (num_nodes == 0) {  // empty list: all vertices, as in the reference
       lp_refine(p, *prm, stats, nullptr, 0);
       return;

@@ -98,7 +98,7 @@ extern "C" {
 int mkp_coarsen(mkp_hypergraph* hg, const mkp_coarsen_params* prm, mkp_hierarchy** out) {
   return guarded([&] {
     MKP_CHECK(hg && prm && out, "null argument");
-    MKP_HIP(hipSetDevice(hg->ctx->c.device));
+    Bind bind_(hg->ctx->c);
     *out = coarsen(hg, *prm);
   });
 }
@@ -128,6 +128,7 @@ int mkp_partition_project(mkp_partition* fine, const mkp_partition* coarse, cons
     MKP_CHECK(level >= 0 && level + 1 < h->num_levels(), "level out of range");
     MKP_CHECK(fine->hg == h->level(level) && coarse->hg == h->level(level + 1), "partitions not on levels i, i+1");
     MKP_CHECK(fine->k == coarse->k, "different k");
+    Bind bind_(fine->hg->ctx->c);
     project(fine, coarse, h, level);
     fine->hg->ctx->c.sync();
   });
@@ -138,7 +139,7 @@ int mkp_initial_partition(mkp_partition* p, double epsilon, uint64_t seed, int32
     MKP_CHECK(p, "null argument");
     MKP_CHECK(epsilon >= 0 && runs >= 1, "bad epsilon or runs");
     Context& c = p->hg->ctx->c;
-    MKP_HIP(hipSetDevice(c.device));
+    Bind bind_(c);
     const int k = p->k;
     const int64_t W = p->hg->total_w;
     const int64_t perfect = int64_t(std::ceil(double(W) / double(k)));
@@ -194,7 +195,7 @@ int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm
                   prm->max_allowed_weight_multiplier > 0 && prm->lp_max_rounds >= 0,
               "bad preset parameters");
     Context& c = hg->ctx->c;
-    MKP_HIP(hipSetDevice(c.device));
+    Bind bind_(c);
     const double t0 = now_ms();
     const int k = prm->k;
     // setupPartWeights / setupContractionLimit / setupMaximumAllowedNodeWeight

@@ -137,17 +137,19 @@ void partition_initialize(mkp_partition* p) {
   Context& c = hg->ctx->c;
   fill_async<int64_t>(p->part_w.p, 0, p->k, c.stream);
   fill_async<int64_t>(p->km1_acc.p, 0, 1, c.stream);
-  // algorithmic bytes: net offsets, pin id + block id per pin, packed
-  // counters and connectivity written once, block id + weight per vertex
-  const double bytes = 4.0 * (hg->m + 1) + 8.0 * double(hg->p) + 8.0 * hg->m * (p->L.vpc + p->vcs) +
-                       8.0 * hg->n + 4.0 * p->k;
-  TimedRegion tr(c.prof, c.stream, kFamPinCountInit, bytes);
+  // algorithmic bytes: block id + weight per vertex and k weights (block
+  // weights); net offsets, pin id + block id per pin, packed counters and
+  // connectivity written once (pin counts)
   if (hg->n) {
+    TimedRegion tr(c.prof, c.stream, kFamPinCountInit, 8.0 * hg->n + 4.0 * p->k, "block_weight_kernel");
     block_weight_kernel<<<grid_for(hg->n, 256, 2048), 256, p->k * sizeof(int64_t), c.stream>>>(
         p->part.p, hg->vw.p, hg->n, p->k, p->part_w.p);
     MKP_LAUNCH_CHECK();
   }
   if (hg->m) {
+    TimedRegion tr(c.prof, c.stream, kFamPinCountInit,
+                   4.0 * (hg->m + 1) + 8.0 * double(hg->p) + 8.0 * hg->m * (p->L.vpc + p->vcs),
+                   "pin_count_init_kernel");
     const int npb = nets_per_block(p->L);
     const size_t lds = size_t(npb) * p->L.vpc * 8 + size_t(npb + 1) * 4;
     pin_count_init_kernel<<<(hg->m + npb - 1) / npb, kInitThreads, lds, c.stream>>>(
@@ -181,7 +183,7 @@ int mkp_partition_create(mkp_hypergraph* hg, int32_t k, mkp_partition** out) {
   return guarded([&] {
     MKP_CHECK(hg && out, "null argument");
     MKP_CHECK(k >= 2 && k <= 1024, "k must be in [2, 1024]");
-    MKP_HIP(hipSetDevice(hg->ctx->c.device));
+    Bind bind_(hg->ctx->c);
     auto* p = new mkp_partition();
     try {
       p->hg = hg;
@@ -207,7 +209,7 @@ int mkp_partition_create(mkp_hypergraph* hg, int32_t k, mkp_partition** out) {
 int mkp_partition_destroy(mkp_partition* p) {
   return guarded([&] {
     if (!p) return;
-    MKP_HIP(hipSetDevice(p->hg->ctx->c.device));
+    Bind bind_(p->hg->ctx->c);
     (void)hipStreamSynchronize(p->hg->ctx->c.stream);
     delete p;
   });
@@ -218,7 +220,7 @@ int mkp_partition_set_parts(mkp_partition* p, const int32_t* parts) {
     MKP_CHECK(p && parts, "null argument");
     for (uint32_t u = 0; u < p->hg->n; ++u) MKP_CHECK(parts[u] >= 0 && parts[u] < p->k, "block id out of range");
     Context& c = p->hg->ctx->c;
-    MKP_HIP(hipSetDevice(c.device));
+    Bind bind_(c);
     MKP_HIP(hipMemcpyAsync(p->part.p, parts, size_t(p->hg->n) * 4, hipMemcpyHostToDevice, c.stream));
     partition_initialize(p);
     c.sync();
@@ -228,7 +230,7 @@ int mkp_partition_set_parts(mkp_partition* p, const int32_t* parts) {
 int mkp_partition_initialize(mkp_partition* p) {
   return guarded([&] {
     MKP_CHECK(p, "null argument");
-    MKP_HIP(hipSetDevice(p->hg->ctx->c.device));
+    Bind bind_(p->hg->ctx->c);
     partition_initialize(p);
     p->hg->ctx->c.sync();
   });
@@ -239,7 +241,7 @@ int mkp_partition_copy_parts(mkp_partition* dst, const mkp_partition* src) {
     MKP_CHECK(dst && src, "null argument");
     MKP_CHECK(dst->hg == src->hg && dst->k == src->k, "partitions of different hypergraphs or k");
     Context& c = dst->hg->ctx->c;
-    MKP_HIP(hipSetDevice(c.device));
+    Bind bind_(c);
     MKP_HIP(hipMemcpyAsync(dst->part.p, src->part.p, size_t(dst->hg->n) * 4, hipMemcpyDeviceToDevice, c.stream));
     partition_initialize(dst);
     c.sync();
@@ -249,6 +251,7 @@ int mkp_partition_copy_parts(mkp_partition* dst, const mkp_partition* src) {
 int mkp_partition_get_parts(const mkp_partition* p, int32_t* parts) {
   return guarded([&] {
     MKP_CHECK(p && parts, "null argument");
+    Bind bind_(p->hg->ctx->c);
     Context& c = p->hg->ctx->c;
     MKP_HIP(hipMemcpyAsync(parts, p->part.p, size_t(p->hg->n) * 4, hipMemcpyDeviceToHost, c.stream));
     c.sync();
@@ -258,6 +261,7 @@ int mkp_partition_get_parts(const mkp_partition* p, int32_t* parts) {
 int mkp_partition_get_part_weights(const mkp_partition* p, int64_t* w) {
   return guarded([&] {
     MKP_CHECK(p && w, "null argument");
+    Bind bind_(p->hg->ctx->c);
     Context& c = p->hg->ctx->c;
     MKP_HIP(hipMemcpyAsync(w, p->part_w.p, size_t(p->k) * 8, hipMemcpyDeviceToHost, c.stream));
     c.sync();
@@ -267,6 +271,7 @@ int mkp_partition_get_part_weights(const mkp_partition* p, int64_t* w) {
 int mkp_partition_get_pin_counts(const mkp_partition* p, uint32_t* counts) {
   return guarded([&] {
     MKP_CHECK(p && counts, "null argument");
+    Bind bind_(p->hg->ctx->c);
     Context& c = p->hg->ctx->c;
     const uint64_t total = uint64_t(p->hg->m) * p->k;
     if (!total) return;
@@ -281,6 +286,7 @@ int mkp_partition_get_pin_counts(const mkp_partition* p, uint32_t* counts) {
 int mkp_partition_get_connectivity(const mkp_partition* p, uint64_t* words) {
   return guarded([&] {
     MKP_CHECK(p && words, "null argument");
+    Bind bind_(p->hg->ctx->c);
     Context& c = p->hg->ctx->c;
     if (p->hg->m)
       MKP_HIP(hipMemcpyAsync(words, p->conn.p, uint64_t(p->hg->m) * p->vcs * 8, hipMemcpyDeviceToHost, c.stream));
@@ -291,6 +297,7 @@ int mkp_partition_get_connectivity(const mkp_partition* p, uint64_t* words) {
 int mkp_partition_get_border(const mkp_partition* p, uint8_t* border) {
   return guarded([&] {
     MKP_CHECK(p && border, "null argument");
+    Bind bind_(p->hg->ctx->c);
     Context& c = p->hg->ctx->c;
     if (!p->hg->n) return;
     DevBuf<uint8_t> tmp(p->hg->n);
@@ -305,6 +312,7 @@ int mkp_partition_get_border(const mkp_partition* p, uint8_t* border) {
 int mkp_partition_km1(mkp_partition* p, int64_t* km1) {
   return guarded([&] {
     MKP_CHECK(p && km1, "null argument");
+    Bind bind_(p->hg->ctx->c);
     *km1 = partition_km1(p);
   });
 }

@@ -90,6 +90,10 @@ SIGNATURES = {
     "mkp_context_reset_profile": (C.c_int, [vp]),
     "mkp_context_get_profile": (C.c_int, [vp, C.c_int, P(C.c_double), P(C.c_double), P(C.c_int64)]),
     "mkp_profile_family_name": (C.c_char_p, [C.c_int]),
+    "mkp_context_profile_kernel_count": (C.c_int, [vp, P(C.c_int32)]),
+    "mkp_context_profile_kernel": (C.c_int, [vp, C.c_int32, P(C.c_char_p), P(C.c_double), P(C.c_double),
+                                             P(C.c_int64)]),
+    "mkp_stream_copy": (C.c_int, [vp, C.c_uint64, C.c_int32, P(C.c_double)]),
     "mkp_partition_initialize": (C.c_int, [vp]),
     "mkp_partition_copy_parts": (C.c_int, [vp, vp]),
     "mkp_hypergraph_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, vp, vp, vp, vp, P(vp)]),
@@ -186,6 +190,23 @@ class Context:
             check(lib().mkp_context_get_profile(self.h, f, C.byref(ms), C.byref(by), C.byref(n)))
             out[lib().mkp_profile_family_name(f).decode()] = dict(ms=ms.value, bytes=by.value, launches=n.value)
         return out
+
+    def kernel_profile(self):
+        """Per-kernel records of single-launch timings: name -> ms, bytes, launches."""
+        cnt = C.c_int32()
+        check(lib().mkp_context_profile_kernel_count(self.h, C.byref(cnt)))
+        out = {}
+        for i in range(cnt.value):
+            name, ms, by, n = C.c_char_p(), C.c_double(), C.c_double(), C.c_int64()
+            check(lib().mkp_context_profile_kernel(self.h, i, C.byref(name), C.byref(ms), C.byref(by), C.byref(n)))
+            out[name.value.decode()] = dict(ms=ms.value, bytes=by.value, launches=n.value)
+        return out
+
+    def stream_copy(self, nbytes, reps):
+        """Mean milliseconds of one device copy of nbytes."""
+        ms = C.c_double()
+        check(lib().mkp_stream_copy(self.h, nbytes, reps, C.byref(ms)))
+        return ms.value
 
 
 class HostHypergraph:

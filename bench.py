@@ -63,7 +63,7 @@ def cpu_baseline(w, arrays, part, maxw, perfect, threads):
     import oracle as orc
     k = w["k"]
     oh = orc.Hypergraph(arrays["n"], arrays["noff"], arrays["pins"], arrays["vw"], arrays["nw"])
-    g = orc.Graph(oh, orc.Graph.UNIFORM)
+    g = orc.Graph(oh, orc.Graph.UNIFORM, ranked=True)
     t0 = time.perf_counter()
     _, lst = g.louvain(sync=False, seed=w["seed"], threads=threads)
     t1 = time.perf_counter()

@@ -301,8 +301,27 @@ struct Context {
   DevBuf<uint8_t> tmp;          // rocPRIM temporary storage
   DevBuf<uint8_t> scratch[4];   // general scratch arenas
   int64_t* pinned = nullptr;    // small host readback area (64 words)
+  // pinned staging for per-round transfers (copies from and to it are truly
+  // asynchronous); grows by reallocation, which waits for the stream
+  uint8_t* stage = nullptr;
+  size_t stage_bytes = 0;
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   int num_cus = 256;
+  template <typename T>
+  T* staging(size_t count) {
+    const size_t bytes = count * sizeof(T);
+    if (bytes > stage_bytes) {
+      MKP_HIP(hipStreamSynchronize(stream));
+      if (stage) (void)hipHostFree(stage);
+      stage = nullptr;
+      stage_bytes = 0;
+      size_t b = 4096;
+      while (b < bytes) b <<= 1;
+      MKP_HIP(hipHostMalloc(reinterpret_cast<void**>(&stage), b, hipHostMallocDefault));
+      stage_bytes = b;
+    }
+    return reinterpret_cast<T*>(stage);
+  }
   void sync() {
     MKP_HIP(hipStreamSynchronize(stream));
     if (!prof.open.empty()) prof.collect();

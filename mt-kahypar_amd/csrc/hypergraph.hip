@@ -132,6 +132,7 @@ int mkp_context_destroy(mkp_context* ctx) {
     // destroyed after their context) are freed directly when released
     ctx->c.pool->close();
     if (ctx->c.pinned) (void)hipHostFree(ctx->c.pinned);
+    if (ctx->c.stage) (void)hipHostFree(ctx->c.stage);
     if (ctx->c.ev_a) (void)hipEventDestroy(ctx->c.ev_a);
     if (ctx->c.ev_b) (void)hipEventDestroy(ctx->c.ev_b);
     (void)hipStreamDestroy(ctx->c.stream);

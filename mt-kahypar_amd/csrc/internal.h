@@ -21,6 +21,11 @@ struct mkp_hypergraph {
   mkp::DevBuf<uint32_t> inc_vtx;  // vertex of every incidence position
   mkp::DevBuf<int32_t> vw, nw, comm;
   bool unit_net_weights = false;
+  // locality ranks of the nets (the star expansion numbers net e as N +
+  // rank(e)); built on first use: order[r] = e, rank[e] = r, inc_rank[j] =
+  // rank(inc[j])
+  mkp::DevBuf<uint32_t> net_order, net_rank, inc_rank;
+  bool ranked = false;
 };
 
 namespace mkp {

@@ -32,6 +32,10 @@ namespace mkp {
 // Every view exposes degree(u), arc(u, i) and source(a) (the tail of arc a in
 // the global arc order).  `unit` marks hypergraphs whose net weights are all
 // one, so the uniform arc weight needs no gather.
+// Net nodes are numbered N + rank(e), ranks ordering the nets by a locality
+// key (lower median of their first nine pins, then id): net nodes close in id
+// then have pins close in id, and the incident nets of a vertex get nearby
+// node ids, so the cluster gathers of neighbouring nodes share cache lines.
 struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
   static constexpr const char* kName = "StarView";
   uint32_t N, M;
@@ -45,16 +49,21 @@ struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
   const int32_t* nw;
   int wt;  // 0 uniform, 1 non_uniform, 2 degree
   bool unit;
+  const uint32_t* order;     // rank -> net
+  const uint32_t* rank;      // net -> rank
+  const uint32_t* inc_rank;  // per incidence: rank of the net
   __device__ uint64_t num_nodes() const { return uint64_t(N) + M; }
-  __device__ uint64_t offset(uint64_t u) const { return u < N ? voff[u] : P + noff[u - N]; }
+  __device__ uint64_t offset(uint64_t u) const { return u < N ? voff[u] : P + noff[order[u - N]]; }
   __device__ uint32_t degree(uint32_t u) const {
-    return u < N ? voff[u + 1] - voff[u] : noff[u - N + 1] - noff[u - N];
+    if (u < N) return voff[u + 1] - voff[u];
+    const uint32_t e = order[u - N];
+    return noff[e + 1] - noff[e];
   }
-  __device__ uint32_t source(uint64_t a) const { return a < P ? inc_vtx[a] : N + pin_net[a - P]; }
+  __device__ uint32_t source(uint64_t a) const { return a < P ? inc_vtx[a] : N + rank[pin_net[a - P]]; }
   // algorithmic bytes of one arc of u: head id + cluster gather, plus the
   // weight inputs the weighting reads per arc
   __device__ uint32_t arc_bytes(uint32_t u) const {
-    if (u < N) return 8u + (wt == 0 ? (unit ? 0u : 4u) : 12u);
+    if (u < N) return 8u + (wt == 0 ? (unit ? 0u : 8u) : 16u);
     return 8u + (wt == 0 ? 0u : 8u);
   }
   __device__ double weight(int32_t w, uint32_t size, uint32_t deg) const {
@@ -64,11 +73,16 @@ struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
   }
   __device__ void arc(uint32_t u, uint32_t i, uint32_t& head, double& w) const {
     if (u < N) {
-      const uint32_t e = inc[voff[u] + i];
-      head = N + e;
-      w = wt == 0 ? (unit ? 1.0 : double(nw[e])) : weight(nw[e], noff[e + 1] - noff[e], voff[u + 1] - voff[u]);
+      const uint32_t j = voff[u] + i;
+      head = N + inc_rank[j];
+      if (wt == 0 && unit) {
+        w = 1.0;
+      } else {
+        const uint32_t e = inc[j];
+        w = wt == 0 ? double(nw[e]) : weight(nw[e], noff[e + 1] - noff[e], voff[u + 1] - voff[u]);
+      }
     } else {
-      const uint32_t e = u - N;
+      const uint32_t e = order[u - N];
       const uint32_t v = pins[noff[e] + i];
       head = v;
       w = wt == 0 ? (unit ? 1.0 : double(nw[e])) : weight(nw[e], noff[e + 1] - noff[e], voff[v + 1] - voff[v]);
@@ -329,54 +343,173 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
   }
 }
 
-// One block per node, map in global scratch (table of `tsize` slots per block).
+// ------------------------------------------------------- heavy nodes --
+// Nodes with more than 1024 (sampled) arcs.  Their arcs are spread over the
+// whole grid instead of one block per node.  Every heavy node of a sub-round
+// owns a table of T = pow2 >= 2d slots in one arena; a slot key carries the
+// sub-round epoch in its high word, so the tables are never cleared: a slot
+// of an older epoch is empty.
+//   hv_prep     d and T per node            (then exclusive scans: arc and
+//   hv_insert   per arc: find or claim the    table offsets)
+//               slot of its cluster; a claimer resets weight and first
+//               position; the slot is remembered per arc
+//   hv_accum    per arc: weight sum, minimum arc position (wave-aggregated)
+//   hv_decide   per node: best cluster over its table, then the move rule
+struct HeavyArgs {
+  const uint32_t* list;  // heavy nodes of the sub-round
+  uint32_t count;
+  const uint64_t* aoff;  // count + 1 arc offsets
+  const uint64_t* toff;  // count + 1 table offsets
+  uint64_t* keys;        // (epoch << 32) | cluster
+  double* wsum;
+  uint32_t* first;
+  uint32_t* arc_slot;    // per arc: absolute table slot
+  uint32_t epoch;
+};
+
 template <class View>
-__global__ void __launch_bounds__(256)
-louvain_move_global_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count, int max_tbits,
-                           int32_t* __restrict__ gkeys, double* __restrict__ gw, uint32_t* __restrict__ gf) {
-  const uint64_t cap = uint64_t(1) << max_tbits;
-  int32_t* K = gkeys + uint64_t(blockIdx.x) * cap;
-  double* Wt = gw + uint64_t(blockIdx.x) * cap;
-  uint32_t* F = gf + uint64_t(blockIdx.x) * cap;
-  __shared__ double s_g[4];
-  __shared__ uint32_t s_f[4];
-  __shared__ int32_t s_k[4];
-  __shared__ double s_w[4];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
-    const uint32_t u = list[item];
-    const uint32_t deg = min(g.degree(u), a.sampling);
-    int tbits = 1;  // table of at least twice the (sampled) degree
-    while ((1u << tbits) < 2 * deg) ++tbits;
-    const uint32_t T = 1u << tbits;
-    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) { K[t] = -1; Wt[t] = 0.0; F[t] = 0xffffffffu; }
-    __threadfence_block();
-    __syncthreads();
-    for (uint32_t base = 0; base < deg; base += blockDim.x) {  // uniform over the block
-      const uint32_t i = base + threadIdx.x;
-      const bool valid = i < deg;
-      int32_t c = -1;
-      double w = 0.0;
-      if (valid) {
-        uint32_t v;
-        g.arc(u, i, v, w);
-        c = a.C[v];
+__global__ void hv_prep_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count,
+                               uint64_t* __restrict__ dsz, uint64_t* __restrict__ tsz) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    const uint32_t d = min(g.degree(list[i]), a.sampling);
+    uint64_t T = 2;
+    while (T < 2ull * d) T <<= 1;
+    dsz[i] = d;
+    tsz[i] = T;
+  }
+}
+
+// largest i with aoff[i] <= j
+__device__ inline uint32_t hv_node_of(const uint64_t* aoff, uint32_t count, uint64_t j) {
+  uint32_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (aoff[mid] <= j) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// aoff / toff hold offsets of the whole round's heavy nodes; a sub-round's
+// arcs and table slots are counted from its first node's offsets.
+template <class View>
+__global__ void __launch_bounds__(256) hv_insert_kernel(View g, LvArgs a, HeavyArgs h) {
+  const uint64_t a0 = h.aoff[0], t0 = h.toff[0];
+  const uint64_t total = h.aoff[h.count] - a0;
+  const uint64_t ep = uint64_t(h.epoch) << 32;
+  for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < total; j += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t i = hv_node_of(h.aoff, h.count, a0 + j);
+    const uint32_t u = h.list[i];
+    uint32_t v;
+    double w;
+    g.arc(u, uint32_t(a0 + j - h.aoff[i]), v, w);
+    const uint32_t c = uint32_t(a.C[v]);
+    const uint64_t base = h.toff[i] - t0;
+    const uint32_t T = uint32_t(h.toff[i + 1] - h.toff[i]);
+    const int tbits = __builtin_ctz(T);
+    const uint64_t key = ep | c;
+    uint32_t s = slot_hash(int32_t(c), tbits);
+    // a key of the current epoch never changes, so a plain read that shows
+    // one is exact; anything else is settled by the CAS
+    uint64_t cur = h.keys[base + s];
+    while (true) {
+      if ((cur & 0xffffffff00000000ull) == ep) {
+        if (uint32_t(cur) == c) break;
+        s = (s + 1) & (T - 1);
+        cur = h.keys[base + s];
+        continue;
       }
-      map_insert_aggregated<64>(K, Wt, F, tbits, valid, c, w, i, 4);
+      const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(&h.keys[base + s]),
+                                     static_cast<unsigned long long>(cur), static_cast<unsigned long long>(key));
+      if (old == cur) {
+        h.wsum[base + s] = 0.0;
+        h.first[base + s] = 0xffffffffu;
+        break;
+      }
+      cur = old;
     }
-    __threadfence_block();
-    __syncthreads();
+    h.arc_slot[j] = uint32_t(base + s);
+  }
+}
+
+template <class View>
+__global__ void __launch_bounds__(256) hv_accum_kernel(View g, LvArgs a, HeavyArgs h) {
+  const uint64_t a0 = h.aoff[0];
+  const uint64_t total = h.aoff[h.count] - a0;
+  const int lane = threadIdx.x & 63;
+  // whole waves iterate together (the aggregation shuffles need every lane)
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t j0 = blockIdx.x * uint64_t(blockDim.x) + (threadIdx.x & ~63u); j0 < total; j0 += stride) {
+    const uint64_t j = j0 + lane;
+    bool pending = j < total;
+    uint32_t slot = 0xffffffffu, pos = 0;
+    double w = 0.0;
+    if (pending) {
+      const uint32_t i = hv_node_of(h.aoff, h.count, a0 + j);
+      pos = uint32_t(a0 + j - h.aoff[i]);
+      uint32_t v;
+      g.arc(h.list[i], pos, v, w);
+      slot = h.arc_slot[j];
+    }
+    for (int it = 0; it < 2; ++it) {  // combine lanes of the lowest pending slot
+      const uint64_t act = __ballot(pending);
+      if (!act) break;
+      const int leader = __ffsll(static_cast<long long>(act)) - 1;
+      const uint32_t ls = __shfl(slot, leader, 64);
+      const bool mine = pending && slot == ls;
+      double sum = mine ? w : 0.0;
+      uint32_t mn = mine ? pos : 0xffffffffu;
+      for (int o = 32; o > 0; o >>= 1) {
+        sum += __shfl_xor(sum, o, 64);
+        mn = min(mn, __shfl_xor(mn, o, 64));
+      }
+      if (lane == leader) {
+        atomicAdd(&h.wsum[ls], sum);
+        atomicMin(&h.first[ls], mn);
+      }
+      if (mine) pending = false;
+    }
+    if (pending) {
+      atomicAdd(&h.wsum[slot], w);
+      atomicMin(&h.first[slot], pos);
+    }
+  }
+}
+
+// Best candidate of one chunk of kHvChunk table slots (chunks never straddle
+// two tables: T is a power of two >= 2048 for heavy nodes, and tables start
+// at multiples of their own size's divisor kHvChunk).
+constexpr uint32_t kHvChunk = 1024;
+struct HvBest {
+  double gain;
+  double wfrom;
+  uint32_t first;
+  int32_t key;
+};
+
+template <class View>
+__global__ void __launch_bounds__(256) hv_scan_kernel(View g, LvArgs a, HeavyArgs h, HvBest* __restrict__ out) {
+  __shared__ HvBest s_b[4];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const uint64_t ep = uint64_t(h.epoch) << 32;
+  const uint64_t t0 = h.toff[0];
+  const uint64_t chunks = (h.toff[h.count] - t0) / kHvChunk;
+  for (uint64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+    const uint64_t s0 = ch * kHvChunk;
+    const uint32_t i = hv_node_of(h.toff, h.count, t0 + s0);
+    const uint32_t u = h.list[i];
     const int32_t from = a.C[u];
     const double mult = a.recip * a.nvol[u];
     double wfrom = 0.0, bgain = -DBL_MAX;
     uint32_t bfirst = 0xffffffffu;
     int32_t bkey = -1;
-    for (uint32_t t = threadIdx.x; t < T; t += blockDim.x) {
-      const int32_t k = K[t];
-      if (k < 0) continue;
-      if (k == from) { wfrom = Wt[t]; continue; }
-      const double gain = Wt[t] - mult * a.cvol[k];
-      if (better(gain, F[t], bgain, bfirst)) { bgain = gain; bfirst = F[t]; bkey = k; }
+    for (uint32_t t = threadIdx.x; t < kHvChunk; t += blockDim.x) {
+      const uint64_t kk = h.keys[s0 + t];
+      if ((kk & 0xffffffff00000000ull) != ep) continue;
+      const int32_t k = int32_t(uint32_t(kk));
+      if (k == from) { wfrom = h.wsum[s0 + t]; continue; }
+      const double gain = h.wsum[s0 + t] - mult * a.cvol[k];
+      const uint32_t f = h.first[s0 + t];
+      if (better(gain, f, bgain, bfirst)) { bgain = gain; bfirst = f; bkey = k; }
     }
     for (int o = 32; o > 0; o >>= 1) {
       const double og = __shfl_xor(bgain, o, 64);
@@ -385,28 +518,50 @@ louvain_move_global_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, 
       wfrom += __shfl_xor(wfrom, o, 64);
       if (ok >= 0 && (bkey < 0 || better(og, of, bgain, bfirst))) { bgain = og; bfirst = of; bkey = ok; }
     }
-    if (lane == 0) { s_g[wid] = bgain; s_f[wid] = bfirst; s_k[wid] = bkey; s_w[wid] = wfrom; }
+    if (lane == 0) s_b[wid] = HvBest{bgain, wfrom, bfirst, bkey};
     __syncthreads();
     if (threadIdx.x == 0) {
-      double wf = 0.0;
+      HvBest r{-DBL_MAX, 0.0, 0xffffffffu, -1};
       for (int x = 0; x < int(blockDim.x >> 6); ++x) {
-        wf += s_w[x];
-        if (s_k[x] >= 0 && (bkey < 0 || better(s_g[x], s_f[x], bgain, bfirst))) {
-          bgain = s_g[x]; bfirst = s_f[x]; bkey = s_k[x];
+        r.wfrom += s_b[x].wfrom;
+        if (s_b[x].key >= 0 && (r.key < 0 || better(s_b[x].gain, s_b[x].first, r.gain, r.first))) {
+          r.gain = s_b[x].gain; r.first = s_b[x].first; r.key = s_b[x].key;
         }
       }
-      lv_decide(a, u, wf, bgain, bkey);
+      out[ch] = r;
     }
     __syncthreads();
   }
 }
 
+// per heavy node: combine its chunks, then the move rule
+__global__ void hv_decide_kernel(LvArgs a, HeavyArgs h, const HvBest* __restrict__ best) {
+  const uint64_t t0 = h.toff[0];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < h.count; i += gridDim.x * blockDim.x) {
+    const uint64_t c0 = (h.toff[i] - t0) / kHvChunk, c1 = (h.toff[i + 1] - t0) / kHvChunk;
+    HvBest r{-DBL_MAX, 0.0, 0xffffffffu, -1};
+    for (uint64_t ch = c0; ch < c1; ++ch) {
+      const HvBest b = best[ch];
+      r.wfrom += b.wfrom;
+      if (b.key >= 0 && (r.key < 0 || better(b.gain, b.first, r.gain, r.first))) {
+        r.gain = b.gain; r.first = b.first; r.key = b.key;
+      }
+    }
+    lv_decide(a, h.list[i], r.wfrom, r.gain, r.key);
+  }
+}
+
+// Applies a sub-round's moves.  The move counter alternates between two
+// words: this launch clears the other one, which the next sub-round fills.
 __global__ void louvain_apply_kernel(const uint32_t* __restrict__ mu, const int32_t* __restrict__ mto,
                                      const uint32_t* __restrict__ mcount, const double* __restrict__ nvol,
                                      int32_t* __restrict__ C, double* __restrict__ cvol, int32_t* __restrict__ csize,
-                                     uint32_t* __restrict__ moved_total) {
+                                     uint32_t* __restrict__ moved_total, uint32_t* __restrict__ next_count) {
   const uint32_t cnt = *mcount;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(moved_total, cnt);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicAdd(moved_total, cnt);
+    *next_count = 0;
+  }
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
     const uint32_t u = mu[j];
     const int32_t to = mto[j];
@@ -476,24 +631,27 @@ __device__ inline int lv_slot(const View& g, const LvArgs& a, uint32_t u) {
 // per (block, slot)
 template <class View>
 __global__ void __launch_bounds__(256)
-lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uint64_t* __restrict__ sbytes) {
+lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uint64_t* __restrict__ sarcs,
+                uint64_t* __restrict__ sbytes) {
   __shared__ uint32_t h[4 * 256];
+  __shared__ unsigned long long ha[4 * 256];
   __shared__ unsigned long long hb[4 * 256];
   const int slots = 4 * a.buckets;
-  for (int i = threadIdx.x; i < slots; i += blockDim.x) { h[i] = 0; hb[i] = 0; }
+  for (int i = threadIdx.x; i < slots; i += blockDim.x) { h[i] = 0; ha[i] = 0; hb[i] = 0; }
   __syncthreads();
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u < n) {
     const int slot = lv_slot(g, a, u);
+    const uint32_t d = min(g.degree(u), a.sampling);
     atomicAdd(&h[slot], 1u);
-    if (sbytes) {  // algorithmic bytes of the node's move: 28 per node + its (sampled) arcs
-      const uint32_t d = min(g.degree(u), a.sampling);
-      atomicAdd(&hb[slot], 28ull + uint64_t(d) * g.arc_bytes(u));
-    }
+    atomicAdd(&ha[slot], static_cast<unsigned long long>(d));
+    // algorithmic bytes of the node's move: 28 per node + its (sampled) arcs
+    if (sbytes) atomicAdd(&hb[slot], 28ull + uint64_t(d) * g.arc_bytes(u));
   }
   __syncthreads();
   for (int i = threadIdx.x; i < slots; i += blockDim.x) {
     if (h[i]) atomicAdd(&counts[i], h[i]);
+    if (ha[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&sarcs[i]), ha[i]);
     if (sbytes && hb[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&sbytes[i]), hb[i]);
   }
 }
@@ -636,10 +794,77 @@ struct LouvainRun {
   // slot = class * B + bucket with offsets off[] and sizes cnt[]
   // (sb: algorithmic bytes per slot, profiling only); each launch is timed
   // on its own so the per-kernel averages match a kernel trace
+  // heavy-node arena and per-sub-round arrays (see hv_* kernels)
+  DevBuf<uint64_t> hkeys_, hdsz_, htsz_, haoff_, htoff_;
+  DevBuf<double> hw_;
+  DevBuf<uint32_t> hf_, hslot_;
+  DevBuf<HvBest> hbest_;
+  uint32_t epoch_ = 0;
+
+  // reserves the arena for tables of up to 4 * arcs slots (T < 4d) and the
+  // per-arc slot array; new key memory starts at epoch 0, which is never used
+  void reserve_heavy(uint64_t arcs, uint32_t nodes) {
+    const hipStream_t s = c.stream;
+    const uint64_t slots = std::max<uint64_t>(4 * arcs, 2);
+    MKP_CHECK(slots < (uint64_t(1) << 32), "heavy-node tables exceed 2^32 slots");
+    if (slots > hkeys_.n) {
+      hkeys_.alloc(slots);
+      hw_.alloc(slots);
+      hf_.alloc(slots);
+      fill_async<uint64_t>(hkeys_.p, 0ull, slots, s);
+    }
+    hslot_.ensure(std::max<uint64_t>(arcs, 1));
+    hbest_.ensure(slots / kHvChunk + 1);
+    hdsz_.ensure(nodes + 1);
+    htsz_.ensure(nodes + 1);
+    haoff_.ensure(nodes + 1);
+    htoff_.ensure(nodes + 1);
+  }
+
+  // arc and table offsets of all heavy nodes of a round (their list slots
+  // are contiguous: class 3 of every bucket)
+  template <class View>
+  void prepare_heavy(const View& g, const LvArgs& a, const uint32_t* list, uint32_t count) {
+    const hipStream_t s = c.stream;
+    hv_prep_kernel<View><<<grid_for(count, 256, 4096), 256, 0, s>>>(g, a, list, count, hdsz_.p, htsz_.p);
+    MKP_LAUNCH_CHECK();
+    exclusive_scan_u64(c, hdsz_.p, haoff_.p, count);
+    exclusive_scan_u64(c, htsz_.p, htoff_.p, count);
+  }
+
+  // first: index of the sub-round's first heavy node in the round's list
+  template <class View>
+  void launch_heavy(const View& g, const LvArgs& a, const uint32_t* list, uint32_t first, uint32_t count,
+                    uint64_t arcs, double bytes) {
+    const hipStream_t s = c.stream;
+    HeavyArgs h{list, count, haoff_.p + first, htoff_.p + first, hkeys_.p, hw_.p, hf_.p, hslot_.p, ++epoch_};
+    const unsigned grid = grid_for(arcs, 256, unsigned(c.num_cus) * 16);
+    {
+      // the compulsory bytes of the heavy moves (node data, head id and
+      // cluster gather per arc) are attributed to the gather pass
+      TimedRegion t1(c.prof, s, kFamLouvainMove, bytes, std::string("hv_insert_kernel<") + View::kName + ">");
+      hv_insert_kernel<View><<<grid, 256, 0, s>>>(g, a, h);
+      MKP_LAUNCH_CHECK();
+    }
+    {
+      TimedRegion t2(c.prof, s, kFamLouvainMove, 0.0, std::string("hv_accum_kernel<") + View::kName + ">");
+      hv_accum_kernel<View><<<grid, 256, 0, s>>>(g, a, h);
+      MKP_LAUNCH_CHECK();
+    }
+    {
+      TimedRegion t3(c.prof, s, kFamLouvainMove, 0.0, std::string("hv_scan_kernel<") + View::kName + ">");
+      const unsigned chunks = unsigned(std::min<uint64_t>(4 * arcs / kHvChunk + 1, uint64_t(c.num_cus) * 16));
+      hv_scan_kernel<View><<<chunks, 256, 0, s>>>(g, a, h, hbest_.p);
+      MKP_LAUNCH_CHECK();
+    }
+    hv_decide_kernel<<<grid_for(count, 256, 4096), 256, 0, s>>>(a, h, hbest_.p);
+    MKP_LAUNCH_CHECK();
+  }
+
   template <class View>
   void launch_move(const View& g, const LvArgs& a, const uint32_t* list, const std::vector<uint32_t>& off,
-                   const std::vector<uint32_t>& cnt, const std::vector<uint64_t>& sb, int sr, int tbits_global,
-                   DevBuf<int32_t>& gk, DevBuf<double>& gw, DevBuf<uint32_t>& gf) {
+                   const std::vector<uint32_t>& cnt, const std::vector<uint64_t>& sa,
+                   const std::vector<uint64_t>& sb, int sr) {
     const hipStream_t s = c.stream;
     const int B = a.buckets;
     const int s0 = 0 * B + sr, s1 = 1 * B + sr, s2 = 2 * B + sr, s3 = 3 * B + sr;
@@ -660,12 +885,7 @@ struct LouvainRun {
       louvain_move_group_kernel<View, 64, 11, 64><<<grid_for(cnt[s2], 1, 65536), 64, 0, s>>>(g, a, list + off[s2],
                                                                                              cnt[s2]);
     }
-    if (cnt[s3]) {
-      TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s3]),
-                     std::string("louvain_move_global_kernel<") + View::kName + ">");
-      louvain_move_global_kernel<View><<<unsigned(std::min<uint32_t>(cnt[s3], global_blocks_)), 256, 0, s>>>(
-          g, a, list + off[s3], cnt[s3], tbits_global, gk.p, gw.p, gf.p);
-    }
+    if (cnt[s3]) launch_heavy(g, a, list + off[s3], off[s3] - off[3 * B], cnt[s3], sa[s3], double(sb[s3]));
     MKP_LAUNCH_CHECK();
   }
 
@@ -679,30 +899,16 @@ struct LouvainRun {
     cvol_.alloc(std::max<uint32_t>(n, 1));
     DevBuf<double>& cvol = cvol_;
     DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
-    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), counts(slots), cursor(slots), offsets(slots), mcount(1),
+    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), counts(slots), cursor(slots), offsets(slots), mcount(2),
         mu(std::max<uint32_t>(n, 1)), moved_total(1);
-    DevBuf<uint64_t> sbytes(slots);
-    std::vector<uint64_t> sb(slots, 0);
+    DevBuf<uint64_t> sbytes(slots), sarcs(slots);
+    std::vector<uint64_t> sb(slots, 0), sa(slots, 0);
     DevBuf<int32_t> mto(std::max<uint32_t>(n, 1));
     lv_init_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, nvol, C, cvol.p, csize.p);
     MKP_LAUNCH_CHECK();
     if (num_arcs == 0) {
       c.sync();
       return false;
-    }
-    // global maps for the heaviest class, sized by the largest (sampled) degree
-    int tbits = 1;
-    DevBuf<int32_t> gk;
-    DevBuf<double> gw;
-    DevBuf<uint32_t> gf;
-    const uint64_t cap = std::min<uint64_t>(max_degree_, prm.sampling);
-    if (cap > 1024) {
-      while ((uint64_t(1) << tbits) < 2 * cap) ++tbits;
-      // up to 512 concurrent maps, at most ~1 GiB of map slots
-      global_blocks_ = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(512, (uint64_t(1) << 30) / (16ull << tbits))));
-      gk.alloc(uint64_t(global_blocks_) << tbits);
-      gw.alloc(uint64_t(global_blocks_) << tbits);
-      gf.alloc(uint64_t(global_blocks_) << tbits);
     }
     LvArgs a;
     a.seed = prm.seed; a.level = level; a.sampling = uint32_t(std::min<uint64_t>(prm.sampling, 0xffffffffu));
@@ -719,32 +925,50 @@ struct LouvainRun {
       fill_async<uint32_t>(counts.p, 0u, slots, s);
       fill_async<uint32_t>(cursor.p, 0u, slots, s);
       fill_async<uint32_t>(moved_total.p, 0u, 1, s);
+      fill_async<uint64_t>(sarcs.p, 0ull, slots, s);
       const bool prof = c.prof.enabled;
       if (prof) fill_async<uint64_t>(sbytes.p, 0ull, slots, s);
-      lv_count_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, counts.p, prof ? sbytes.p : nullptr);
+      lv_count_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, counts.p, sarcs.p, prof ? sbytes.p : nullptr);
       MKP_LAUNCH_CHECK();
-      MKP_HIP(hipMemcpyAsync(cnt.data(), counts.p, size_t(slots) * 4, hipMemcpyDeviceToHost, s));
-      if (prof) MKP_HIP(hipMemcpyAsync(sb.data(), sbytes.p, size_t(slots) * 8, hipMemcpyDeviceToHost, s));
+      // pinned staging: [sarcs u64][sbytes u64][counts u32][offsets u32]
+      uint64_t* stg = c.staging<uint64_t>(4 * size_t(slots));
+      uint32_t* stg_cnt = reinterpret_cast<uint32_t*>(stg + 2 * slots);
+      uint32_t* stg_off = reinterpret_cast<uint32_t*>(stg + 3 * slots);
+      MKP_HIP(hipMemcpyAsync(stg_cnt, counts.p, size_t(slots) * 4, hipMemcpyDeviceToHost, s));
+      MKP_HIP(hipMemcpyAsync(stg, sarcs.p, size_t(slots) * 8, hipMemcpyDeviceToHost, s));
+      if (prof) MKP_HIP(hipMemcpyAsync(stg + slots, sbytes.p, size_t(slots) * 8, hipMemcpyDeviceToHost, s));
       c.sync();
+      std::copy(stg_cnt, stg_cnt + slots, cnt.begin());
+      std::copy(stg, stg + slots, sa.begin());
+      if (prof) std::copy(stg + slots, stg + 2 * slots, sb.begin());
+      uint64_t harcs = 0;
+      uint32_t hnodes = 0;
+      for (int sr = 0; sr < B; ++sr) {
+        harcs = std::max(harcs, sa[3 * B + sr]);
+        hnodes += cnt[3 * B + sr];
+      }
+      if (hnodes) reserve_heavy(harcs, hnodes);
       uint32_t run = 0;
       for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
-      MKP_HIP(hipMemcpyAsync(offsets.p, off.data(), size_t(slots) * 4, hipMemcpyHostToDevice, s));
+      std::copy(off.begin(), off.end(), stg_off);
+      MKP_HIP(hipMemcpyAsync(offsets.p, stg_off, size_t(slots) * 4, hipMemcpyHostToDevice, s));
       lv_scatter_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, offsets.p, cursor.p, list.p);
       MKP_LAUNCH_CHECK();
+      if (hnodes) prepare_heavy(g, a, list.p + off[3 * B], hnodes);
+      fill_async<uint32_t>(mcount.p, 0u, 2, s);
       for (int sr = 0; sr < B; ++sr) {
         a.bucket = sr;
-        fill_async<uint32_t>(mcount.p, 0u, 1, s);
-        launch_move(g, a, list.p, off, cnt, sb, sr, tbits, gk, gw, gf);
+        a.move_count = mcount.p + (sr & 1);
+        launch_move(g, a, list.p, off, cnt, sa, sb, sr);
         // at most one move per node of this sub-round
         const uint32_t sub_nodes = cnt[sr] + cnt[B + sr] + cnt[2 * B + sr] + cnt[3 * B + sr];
-        louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(mu.p, mto.p, mcount.p, nvol, C,
-                                                                           cvol.p, csize.p, moved_total.p);
+        louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(
+            mu.p, mto.p, a.move_count, nvol, C, cvol.p, csize.p, moved_total.p, mcount.p + ((sr + 1) & 1));
         MKP_LAUNCH_CHECK();
       }
-      uint32_t mt = 0;
-      MKP_HIP(hipMemcpyAsync(&mt, moved_total.p, 4, hipMemcpyDeviceToHost, s));
+      MKP_HIP(hipMemcpyAsync(c.pinned, moved_total.p, 4, hipMemcpyDeviceToHost, s));
       c.sync();
-      moved = mt;
+      moved = *reinterpret_cast<const uint32_t*>(c.pinned);
       st.arcs_swept += int64_t(num_arcs);
       st.rounds += 1;
       st.moves += int64_t(moved);
@@ -755,7 +979,6 @@ struct LouvainRun {
 
   uint32_t max_degree_ = 0;
   DevBuf<double> cvol_;  // cluster volumes left by the last local-moving level
-  uint32_t global_blocks_ = 1;
 
   // level 0: node volumes from the arcs; every level: maximum degree
   template <class View>
@@ -879,6 +1102,70 @@ struct LouvainRun {
   }
 };
 
+// Locality key of every net: the lower median of its first nine pins, packed
+// above the net id so a radix sort yields the (key, id) order.
+__global__ void net_key_kernel(const uint32_t* __restrict__ noff, const uint32_t* __restrict__ pins, uint32_t m,
+                               uint64_t* __restrict__ keys) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+    const uint32_t b = noff[e];
+    const uint32_t s = min(noff[e + 1] - b, 9u);
+    uint32_t p[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) p[i] = i < int(s) ? pins[b + i] : 0xffffffffu;
+#pragma unroll
+    for (int i = 1; i < 9; ++i)  // insertion sort of nine registers
+#pragma unroll
+      for (int j = i; j > 0; --j)
+        if (p[j] < p[j - 1]) { const uint32_t t = p[j]; p[j] = p[j - 1]; p[j - 1] = t; }
+    uint32_t key = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      if (s > 0 && uint32_t(i) == (s - 1) / 2) key = p[i];
+    keys[e] = (uint64_t(key) << 32) | e;
+  }
+}
+
+__global__ void net_rank_kernel(const uint64_t* __restrict__ sorted, uint32_t m, uint32_t* __restrict__ order,
+                                uint32_t* __restrict__ rank) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x) {
+    const uint32_t e = uint32_t(sorted[r]);
+    order[r] = e;
+    rank[e] = r;
+  }
+}
+
+__global__ void inc_rank_kernel(const uint32_t* __restrict__ inc, const uint32_t* __restrict__ rank, uint64_t p,
+                                uint32_t* __restrict__ out) {
+  for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < p; j += uint64_t(gridDim.x) * blockDim.x)
+    out[j] = rank[inc[j]];
+}
+
+// Builds the net locality ranks of hg once (the hypergraph is immutable).
+void ensure_net_ranks(mkp_hypergraph* hg) {
+  if (hg->ranked) return;
+  Context& c = hg->ctx->c;
+  const hipStream_t s = c.stream;
+  const uint32_t m = hg->m;
+  hg->net_order.alloc(std::max<uint32_t>(m, 1));
+  hg->net_rank.alloc(std::max<uint32_t>(m, 1));
+  hg->inc_rank.alloc(std::max<uint64_t>(hg->p, 1));
+  if (m) {
+    DevBuf<uint64_t> keys(m), sorted(m);
+    net_key_kernel<<<grid_for(m, 256, 65536), 256, 0, s>>>(hg->noff.p, hg->pins.p, m, keys.p);
+    MKP_LAUNCH_CHECK();
+    int bits = 1;  // key bits above the 32-bit net id
+    while ((uint64_t(1) << bits) < hg->n) ++bits;
+    sort_keys_u64(c, keys.p, sorted.p, m, 0, 32 + bits);
+    net_rank_kernel<<<grid_for(m, 256, 65536), 256, 0, s>>>(sorted.p, m, hg->net_order.p, hg->net_rank.p);
+    MKP_LAUNCH_CHECK();
+  }
+  if (hg->p) {
+    inc_rank_kernel<<<grid_for(hg->p, 256, 65536), 256, 0, s>>>(hg->inc.p, hg->net_rank.p, hg->p, hg->inc_rank.p);
+    MKP_LAUNCH_CHECK();
+  }
+  hg->ranked = true;
+}
+
 }  // namespace mkp
 
 using namespace mkp;
@@ -913,8 +1200,10 @@ extern "C" int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* prm, in
                   hg->unit_net_weights};
       run.run_from(g, hg->n, hg->p, C.p);
     } else {
-      StarView g{hg->n,          hg->m,          hg->p,     hg->voff.p, hg->inc.p, hg->noff.p, hg->pins.p,
-                 hg->inc_vtx.p,  hg->pin_net.p,  hg->nw.p,  wt,         hg->unit_net_weights};
+      ensure_net_ranks(hg);
+      StarView g{hg->n,         hg->m,         hg->p,    hg->voff.p, hg->inc.p,          hg->noff.p,
+                 hg->pins.p,    hg->inc_vtx.p, hg->pin_net.p, hg->nw.p, wt, hg->unit_net_weights,
+                 hg->net_order.p, hg->net_rank.p, hg->inc_rank.p};
       run.run_from(g, nodes, 2 * hg->p, C.p);
     }
     // restrictClusteringToHypernodes + setCommunityIDs (graph.h:123-125,

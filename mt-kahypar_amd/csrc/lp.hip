@@ -135,6 +135,7 @@ template <int K>
 __global__ void __launch_bounds__(kGainThreads)
 lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na, uint32_t* __restrict__ cand_flag,
                       int32_t* __restrict__ cand_to, int32_t* __restrict__ cand_gain, uint32_t* __restrict__ heavy) {
+  if (a.counters[6]) return;  // a rolled-back sub-round stopped the refinement
   int64_t visits = 0, swept = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
     const uint32_t u = active[i];
@@ -208,6 +209,7 @@ __global__ void __launch_bounds__(64)
 lp_gain_wave_kernel(LpArgs a, const uint32_t* __restrict__ active, const uint32_t* __restrict__ list,
                     const int64_t* __restrict__ list_count, uint32_t count_val, uint32_t* __restrict__ cand_flag,
                     int32_t* __restrict__ cand_to, int32_t* __restrict__ cand_gain) {
+  if (a.counters[6]) return;  // a rolled-back sub-round stopped the refinement
   extern __shared__ int32_t sc[];
   const int lane = threadIdx.x;
   const uint32_t count = list_count ? uint32_t(*list_count) : count_val;
@@ -358,6 +360,7 @@ __global__ void lp_admit_kernel(LpArgs a, const uint64_t* __restrict__ keys, con
 __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const int32_t* __restrict__ cfrom,
                                 const int32_t* __restrict__ cto, const uint8_t* __restrict__ admit,
                                 const uint8_t* __restrict__ keep, uint32_t nc, int mode) {
+  if (mode == 2 && !a.counters[6]) return;  // roll back only a sub-round that worsened km1
   int64_t d = 0;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
     if (admit && !admit[j]) continue;
@@ -429,6 +432,7 @@ __global__ void __launch_bounds__(64)
 lp_activate_kernel(LpArgs a, const uint32_t* __restrict__ cu, const uint8_t* __restrict__ admit,
                    const uint8_t* __restrict__ keep, uint32_t nc, uint32_t threshold, uint32_t stamp,
                    uint32_t* __restrict__ net_stamp, uint32_t* __restrict__ flag) {
+  if (a.counters[6]) return;  // the sub-round was rolled back
   int64_t moves = 0;
   const int lane = threadIdx.x;
   for (uint32_t j = blockIdx.x; j < nc; j += gridDim.x) {
@@ -466,6 +470,17 @@ __global__ void lp_border_flag_kernel(LpArgs a, uint32_t n, uint32_t* __restrict
       }
     }
     flag[u] = border;
+  }
+}
+
+// After a sub-round's moves and reverts: if its total real delta is positive
+// the sub-round is rolled back and the refinement stops (flag counters[6]),
+// else the delta joins the total (counters[7]).
+__global__ void lp_check_kernel(int64_t* __restrict__ counters) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const int64_t d = counters[0];
+    if (d > 0) counters[6] = 1;
+    else counters[7] += d;
   }
 }
 
@@ -570,7 +585,6 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
     na = na32;
   }
   int64_t* host = c.pinned;
-  int64_t delta_total = 0;
   int rounds = 0;
   bool stop = false;
   for (int r = 0; r < prm.max_rounds && na > 0 && !stop; ++r) {
@@ -619,25 +633,32 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
           act, ns, w.cand_flag.p, w.pos.p, w.cand_to_a.p, w.cand_gain_a.p, p->part.p, hg->vw.p, w.cand_u.p,
           w.cand_from.p, w.cand_to.p, w.cand_gain.p, w.in_w.p);
       MKP_LAUNCH_CHECK();
-      // read back: number of candidates, arrivals per block, block weights
-      uint32_t nc32 = 0;
-      MKP_HIP(hipMemcpyAsync(&nc32, w.pos.p + ns, 4, hipMemcpyDeviceToHost, s));
-      std::vector<int64_t> inw(p->k);
-      MKP_HIP(hipMemcpyAsync(inw.data(), w.in_w.p, size_t(p->k) * 8, hipMemcpyDeviceToHost, s));
-      std::vector<int64_t> pw(p->k);
-      MKP_HIP(hipMemcpyAsync(pw.data(), p->part_w.p, size_t(p->k) * 8, hipMemcpyDeviceToHost, s));
+      // read back (pinned staging): stop flag of the previous sub-round,
+      // number of candidates, arrivals per block, block weights
+      const int K = p->k;
+      int64_t* stg = c.staging<int64_t>(2 * size_t(K) + 2 + (size_t(K) + 7) / 8);
+      uint8_t* stg_over = reinterpret_cast<uint8_t*>(stg + 2 * K + 2);
+      MKP_HIP(hipMemcpyAsync(stg, w.counters.p + 6, 8, hipMemcpyDeviceToHost, s));
+      MKP_HIP(hipMemcpyAsync(stg + 1, w.pos.p + ns, 4, hipMemcpyDeviceToHost, s));
+      MKP_HIP(hipMemcpyAsync(stg + 2, w.in_w.p, size_t(K) * 8, hipMemcpyDeviceToHost, s));
+      MKP_HIP(hipMemcpyAsync(stg + 2 + K, p->part_w.p, size_t(K) * 8, hipMemcpyDeviceToHost, s));
       c.sync();
-      const uint32_t nc = nc32;
+      if (stg[0]) {  // the previous sub-round was rolled back on the device
+        stop = true;
+        break;
+      }
+      const uint32_t nc = *reinterpret_cast<const uint32_t*>(stg + 1);
       if (nc == 0) continue;
-      std::vector<uint8_t> over(p->k, 0);
+      const int64_t* inw = stg + 2;
+      const int64_t* pw = stg + 2 + K;
       bool any_over = false;
-      for (int b = 0; b < p->k; ++b) {
-        over[b] = pw[b] + inw[b] > prm.max_part_weights[b];
-        any_over |= over[b] != 0;
+      for (int b = 0; b < K; ++b) {
+        stg_over[b] = pw[b] + inw[b] > prm.max_part_weights[b];
+        any_over |= stg_over[b] != 0;
       }
       const uint8_t* admit = nullptr;
       if (any_over) {
-        MKP_HIP(hipMemcpyAsync(w.overflow.p, over.data(), p->k, hipMemcpyHostToDevice, s));
+        MKP_HIP(hipMemcpyAsync(w.overflow.p, stg_over, K, hipMemcpyHostToDevice, s));
         lp_admission_keys_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(a, w.cand_u.p, w.cand_to.p, w.cand_gain.p,
                                                                           nc, w.keys.p, w.vals.p);
         MKP_LAUNCH_CHECK();
@@ -668,20 +689,11 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
         lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, w.keep.p, nc, 1);
         MKP_LAUNCH_CHECK();
       }
-      MKP_HIP(hipMemcpyAsync(host, w.counters.p, 8, hipMemcpyDeviceToHost, s));
-      c.sync();
-      int64_t D = host[0];
-      if (D > 0) {
-        lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, w.keep.p, nc, 2);
-        MKP_LAUNCH_CHECK();
-        MKP_HIP(hipMemcpyAsync(host, w.counters.p, 8, hipMemcpyDeviceToHost, s));
-        c.sync();
-        D = host[0];
-        MKP_CHECK(D == 0, "label propagation rollback did not restore the sub-round state");
-        stop = true;
-        continue;
-      }
-      delta_total += D;
+      // decided on the device: keep the sub-round (delta into the total) or
+      // roll it back and stop; the host sees the flag at its next read-back
+      lp_check_kernel<<<1, 64, 0, s>>>(w.counters.p);
+      lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, w.keep.p, nc, 2);
+      MKP_LAUNCH_CHECK();
       lp_activate_kernel<<<grid_for(nc, 1, unsigned(c.num_cus) * 16), 64, 0, s>>>(
           a, w.cand_u.p, admit, w.keep.p, nc, prm.activation_threshold, round_stamp, w.net_stamp.p, w.flag.p);
       MKP_LAUNCH_CHECK();
@@ -691,10 +703,11 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
     exclusive_scan_u32(c, w.flag.p, w.pos.p, n);
     lp_compact_active_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(w.flag.p, w.pos.p, n, w.active.p);
     MKP_LAUNCH_CHECK();
-    uint32_t na32 = 0;
-    MKP_HIP(hipMemcpyAsync(&na32, w.pos.p + n, 4, hipMemcpyDeviceToHost, s));
+    MKP_HIP(hipMemcpyAsync(host, w.pos.p + n, 4, hipMemcpyDeviceToHost, s));
+    MKP_HIP(hipMemcpyAsync(host + 1, w.counters.p + 6, 8, hipMemcpyDeviceToHost, s));
     c.sync();
-    na = na32;
+    na = *reinterpret_cast<const uint32_t*>(host);
+    if (host[1]) stop = true;  // the round's last sub-round was rolled back
   }
   MKP_HIP(hipEventRecord(c.ev_b, s));
   MKP_HIP(hipMemcpyAsync(host, w.counters.p, 8 * 8, hipMemcpyDeviceToHost, s));
@@ -706,7 +719,7 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
     c.prof.bytes[kFamLpGain] += 16.0 * double(host[5]) + (16.0 + 8.0 * p->vcs) * double(host[1]);
   }
   if (stats) {
-    stats->delta = delta_total;
+    stats->delta = host[7];
     stats->moves = host[4];
     stats->reverts = host[2];
     stats->visits = host[1];

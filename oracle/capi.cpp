@@ -114,8 +114,8 @@ int64_t orc_lp_refine(void* h, int k, int32_t* part, const int64_t* max_w, const
 }
 
 // ------------------------------------------------------------- graph --
-void* orc_graph_new(void* h, int wt) {
-  return new Graph(build_graph(*static_cast<Hypergraph*>(h), EdgeWeight(wt)));
+void* orc_graph_new(void* h, int wt, int ranked) {
+  return new Graph(build_graph(*static_cast<Hypergraph*>(h), EdgeWeight(wt), ranked != 0));
 }
 void orc_graph_free(void* g) { delete static_cast<Graph*>(g); }
 void orc_graph_sizes(void* gp, uint32_t* n, uint64_t* arcs, double* tv, uint64_t* max_degree) {
@@ -245,7 +245,7 @@ int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int t
   lp.seed = seed; lp.threads = threads;
   LouvainStats lst;
   {
-    Graph g = build_graph(hg, wt);
+    Graph g = build_graph(hg, wt, true);
     std::vector<int32_t> C = sync ? louvain_sync(g, lp, &lst) : louvain_ref(g, lp, &lst);
     for (uint32_t u = 0; u < hg.n; ++u) hg.comm[u] = C[u];
   }

@@ -13,7 +13,21 @@ namespace orc {
 
 // Graph::Graph / construct / constructBipartiteGraph / constructGraph
 // (graph.cpp:36-78, 290-446)
-Graph build_graph(const Hypergraph& hg, EdgeWeight wt) {
+std::vector<uint32_t> net_locality_ranks(const Hypergraph& hg) {
+  std::vector<std::pair<uint32_t, uint32_t>> key(hg.m);
+  for (uint32_t e = 0; e < hg.m; ++e) {
+    const uint32_t s = std::min<uint32_t>(hg.esize(e), 9);
+    std::vector<uint32_t> p(hg.pins.begin() + int64_t(hg.noff[e]), hg.pins.begin() + int64_t(hg.noff[e] + s));
+    std::sort(p.begin(), p.end());
+    key[e] = {s ? p[(s - 1) / 2] : 0u, e};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<uint32_t> rank(hg.m);
+  for (uint32_t r = 0; r < hg.m; ++r) rank[key[r].second] = r;
+  return rank;
+}
+
+Graph build_graph(const Hypergraph& hg, EdgeWeight wt, bool ranked) {
   auto f = [&](int32_t w, uint32_t size, uint32_t deg) -> double {
     switch (wt) {
       case EdgeWeight::uniform: return double(w);
@@ -44,10 +58,13 @@ Graph build_graph(const Hypergraph& hg, EdgeWeight wt) {
     }
   } else {
     const uint32_t N = hg.n;
+    std::vector<uint32_t> rank(hg.m);
+    if (ranked) rank = net_locality_ranks(hg);
+    else for (uint32_t e = 0; e < hg.m; ++e) rank[e] = e;
     g.n = hg.n + hg.m;
     g.idx.assign(size_t(g.n) + 1, 0);
     for (uint32_t u = 0; u < N; ++u) g.idx[u + 1] = hg.degree(u);
-    for (uint32_t e = 0; e < hg.m; ++e) g.idx[N + e + 1] = hg.esize(e);
+    for (uint32_t e = 0; e < hg.m; ++e) g.idx[N + rank[e] + 1] = hg.esize(e);
     for (uint32_t x = 0; x < g.n; ++x) g.idx[x + 1] += g.idx[x];
     g.head.resize(g.idx[g.n]);
     g.w.resize(g.idx[g.n]);
@@ -56,12 +73,12 @@ Graph build_graph(const Hypergraph& hg, EdgeWeight wt) {
       g.max_degree = std::max<uint64_t>(g.max_degree, hg.degree(u));
       for (uint64_t i = hg.voff[u]; i < hg.voff[u + 1]; ++i) {
         const uint32_t e = hg.inc[i];
-        g.head[pos] = N + e;
+        g.head[pos] = N + rank[e];
         g.w[pos++] = f(hg.nw[e], hg.esize(e), hg.degree(u));
       }
     }
     for (uint32_t e = 0; e < hg.m; ++e) {
-      uint64_t pos = g.idx[N + e];
+      uint64_t pos = g.idx[N + rank[e]];
       g.max_degree = std::max<uint64_t>(g.max_degree, hg.esize(e));
       for (uint64_t j = hg.noff[e]; j < hg.noff[e + 1]; ++j) {
         const uint32_t v = hg.pins[j];

@@ -138,7 +138,12 @@ struct Graph {
   double tv = 0;
   uint64_t max_degree = 0;
 };
-Graph build_graph(const Hypergraph& hg, EdgeWeight wt);
+// Star expansion (or the graph itself when every net has two pins).  With
+// `ranked`, net node ids are N + rank(e), ranks ordering the nets by
+// (locality key, id) with the key the lower median of the first nine pins
+// (net_locality_ranks); otherwise N + e as in graph.cpp:331-398.
+Graph build_graph(const Hypergraph& hg, EdgeWeight wt, bool ranked = false);
+std::vector<uint32_t> net_locality_ranks(const Hypergraph& hg);
 Graph contract_graph(const Graph& g, std::vector<int32_t>& C);
 double modularity(const Graph& g, const std::vector<int32_t>& C);
 

@@ -42,7 +42,7 @@ def lib():
                                          C.POINTER(C.c_int32)]),
             "orc_lp_refine": (C.c_int64, [vp, C.c_int, i32p, i64p, i64p, C.c_uint64, C.c_int, C.c_int, C.c_int,
                                           C.c_int, C.c_int, C.c_int, u32p, C.c_uint32, i64p]),
-            "orc_graph_new": (vp, [vp, C.c_int]),
+            "orc_graph_new": (vp, [vp, C.c_int, C.c_int]),
             "orc_graph_free": (None, [vp]),
             "orc_graph_sizes": (None, [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_double),
                                        C.POINTER(C.c_uint64)]),
@@ -180,8 +180,9 @@ class Hypergraph:
 class Graph:
     UNIFORM, NON_UNIFORM, DEGREE = 0, 1, 2
 
-    def __init__(self, hg=None, weight=0, handle=None):
-        self.h = handle if handle is not None else lib().orc_graph_new(hg.h, weight)
+    def __init__(self, hg=None, weight=0, handle=None, ranked=False):
+        """ranked: net nodes numbered by locality rank, as the device path does."""
+        self.h = handle if handle is not None else lib().orc_graph_new(hg.h, weight, int(ranked))
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

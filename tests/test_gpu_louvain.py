@@ -38,7 +38,7 @@ def compare(ctx, n, noff, pins, nw=None, weight=0, seed=3, sub_rounds=8, samplin
     comm, st = dh.louvain(edge_weight=weight, seed=seed, sub_rounds=sub_rounds, sampling=sampling,
                           max_passes=max_passes, min_move_fraction=min_frac)
     oh = orc.Hypergraph(n, noff, pins, None, nw)
-    g = orc.Graph(oh, weight)
+    g = orc.Graph(oh, weight, ranked=True)
     C, ost = g.louvain(sync=True, seed=seed, sub_rounds=sub_rounds, sampling=sampling, max_passes=max_passes,
                        min_frac=min_frac)
     assert np.array_equal(dh.arrays()["comm"], comm)
@@ -94,7 +94,7 @@ def test_fractional_weights_modularity(ctx, weight):
     n, noff, pins = powerlaw(4000, 2500, 60, 11)
     comm, ref, st, ost = compare(ctx, n, noff, pins, weight=weight)
     oh = orc.Hypergraph(n, noff, pins)
-    g = orc.Graph(oh, weight)
+    g = orc.Graph(oh, weight, ranked=True)
     nn = g.sizes()[0]
     # evaluate both hypernode clusterings on the star expansion with every
     # net node a singleton under its own node id (ids >= n, unused by the

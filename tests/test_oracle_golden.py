@@ -333,3 +333,33 @@ def test_louvain_sync_improves_modularity():
     assert g.modularity(C) > g.modularity(np.arange(n, dtype=np.int32)) + 0.1
     C2, _ = g.louvain(sync=True, seed=5)
     assert C.tolist() == C2.tolist()
+
+
+def test_ranked_star_expansion_is_a_relabelling():
+    """The locality-ranked star expansion is the star expansion with the net
+    nodes permuted, in non-decreasing order of the lower median of each net's
+    first nine pins (ties by net id)."""
+    rng = np.random.default_rng(21)
+    hg, nets, vw, nw = random_hg(rng, 300, 500, 14, weighted=True)
+    n, m = 300, len(nets)
+    for kind in (orc.Graph.UNIFORM, orc.Graph.DEGREE):
+        a = orc.Graph(hg, kind).arrays()
+        b = orc.Graph(hg, kind, ranked=True).arrays()
+        # the permutation read off the hypernode arcs
+        perm = {}
+        for u in range(n):
+            ha = a["head"][a["idx"][u]:a["idx"][u + 1]]
+            hb = b["head"][b["idx"][u]:b["idx"][u + 1]]
+            assert np.array_equal(a["w"][a["idx"][u]:a["idx"][u + 1]], b["w"][b["idx"][u]:b["idx"][u + 1]])
+            for x, y in zip(ha.tolist(), hb.tolist()):
+                assert perm.setdefault(x, y) == y
+        assert sorted(perm) == list(range(n, n + m)) and sorted(perm.values()) == list(range(n, n + m))
+        for x, y in perm.items():
+            assert a["vol"][x] == b["vol"][y]
+            assert np.array_equal(a["head"][a["idx"][x]:a["idx"][x + 1]], b["head"][b["idx"][y]:b["idx"][y + 1]])
+        keys = []
+        for e in range(m):
+            p = sorted(nets[e][:9])
+            keys.append((p[(len(p) - 1) // 2], e))
+        order = [e for _, e in sorted(keys)]
+        assert [perm[n + e] - n for e in order] == list(range(m))

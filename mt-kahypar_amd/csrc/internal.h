@@ -25,6 +25,9 @@ struct mkp_hypergraph {
   // rank(e)); built on first use: order[r] = e, rank[e] = r, inc_rank[j] =
   // rank(inc[j])
   mkp::DevBuf<uint32_t> net_order, net_rank, inc_rank;
+  // pins stored in rank order: net of rank r holds rpins[rpoff[r] ..
+  // rpoff[r+1]); rpin_rank[j] = rank of the net of ranked pin slot j
+  mkp::DevBuf<uint32_t> rpoff, rpins, rpin_rank;
   bool ranked = false;
 };
 

@@ -49,17 +49,17 @@ struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
   const int32_t* nw;
   int wt;  // 0 uniform, 1 non_uniform, 2 degree
   bool unit;
-  const uint32_t* order;     // rank -> net
-  const uint32_t* rank;      // net -> rank
-  const uint32_t* inc_rank;  // per incidence: rank of the net
+  const uint32_t* order;      // rank -> net
+  const uint32_t* inc_rank;   // per incidence: rank of the net
+  const uint32_t* rpoff;      // rank-ordered pin lists
+  const uint32_t* rpins;
+  const uint32_t* rpin_rank;  // per rank-ordered pin slot: rank of its net
   __device__ uint64_t num_nodes() const { return uint64_t(N) + M; }
-  __device__ uint64_t offset(uint64_t u) const { return u < N ? voff[u] : P + noff[order[u - N]]; }
+  __device__ uint64_t offset(uint64_t u) const { return u < N ? voff[u] : P + rpoff[u - N]; }
   __device__ uint32_t degree(uint32_t u) const {
-    if (u < N) return voff[u + 1] - voff[u];
-    const uint32_t e = order[u - N];
-    return noff[e + 1] - noff[e];
+    return u < N ? voff[u + 1] - voff[u] : rpoff[u - N + 1] - rpoff[u - N];
   }
-  __device__ uint32_t source(uint64_t a) const { return a < P ? inc_vtx[a] : N + rank[pin_net[a - P]]; }
+  __device__ uint32_t source(uint64_t a) const { return a < P ? inc_vtx[a] : N + rpin_rank[a - P]; }
   // algorithmic bytes of one arc of u: head id + cluster gather, plus the
   // weight inputs the weighting reads per arc
   __device__ uint32_t arc_bytes(uint32_t u) const {
@@ -82,9 +82,14 @@ struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
         w = wt == 0 ? double(nw[e]) : weight(nw[e], noff[e + 1] - noff[e], voff[u + 1] - voff[u]);
       }
     } else {
-      const uint32_t e = order[u - N];
-      const uint32_t v = pins[noff[e] + i];
+      const uint32_t r = u - N;
+      const uint32_t v = rpins[rpoff[r] + i];
       head = v;
+      if (wt == 0 && unit) {
+        w = 1.0;
+        return;
+      }
+      const uint32_t e = order[r];
       w = wt == 0 ? (unit ? 1.0 : double(nw[e])) : weight(nw[e], noff[e + 1] - noff[e], voff[v + 1] - voff[v]);
     }
   }
@@ -231,7 +236,9 @@ __device__ inline void lv_decide(const LvArgs& a, uint32_t u, double wfrom, doub
 // pairs of the arcs stay in registers; a cluster's weight is summed in arc
 // order starting at its first occurrence, and the first occurrence with a
 // strictly larger gain wins: the insertion-ordered scan of
-// computeMaxGainCluster, summation order included.
+// computeMaxGainCluster, summation order included.  The loads are issued in
+// three independent batches (arcs, clusters of the heads, cluster volumes) so
+// a node waits for three memory round trips, not one per arc.
 template <class View>
 __global__ void __launch_bounds__(256)
 louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count) {
@@ -239,21 +246,24 @@ louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
   for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
     const uint32_t u = list[idx];
     const uint32_t deg = min(g.degree(u), a.sampling);
-    int32_t key[D];
+    uint32_t head[D];
     double w[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-      key[i] = -1;
+      head[i] = u;  // a valid index for the unconditional gathers below
       w[i] = 0.0;
-      if (uint32_t(i) < deg) {
-        uint32_t v;
-        double ww;
-        g.arc(u, uint32_t(i), v, ww);
-        key[i] = a.C[v];
-        w[i] = ww;
-      }
+      if (uint32_t(i) < deg) g.arc(u, uint32_t(i), head[i], w[i]);
     }
     const int32_t from = a.C[u];
+    int32_t key[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const int32_t c = a.C[head[i]];
+      key[i] = uint32_t(i) < deg ? c : -1;
+    }
+    double cv[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) cv[i] = a.cvol[key[i] >= 0 ? key[i] : from];
     const double mult = a.recip * a.nvol[u];
     double wfrom = 0.0, bgain = -DBL_MAX;
     int32_t bkey = -1;
@@ -270,7 +280,7 @@ louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
         if (key[i] == from) {
           wfrom = sum;
         } else {
-          const double gain = sum - mult * a.cvol[key[i]];
+          const double gain = sum - mult * cv[i];
           if (gain > bgain) { bgain = gain; bkey = key[i]; }
         }
       }
@@ -296,11 +306,15 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
     const uint32_t idx = base + grp;
     const bool in = idx < count;  // lists hold exactly the nodes of this sub-round
     const uint32_t u = in ? list[idx] : 0;
-    for (int t = lane; t < T; t += G) { K[t] = -1; Wt[t] = 0.0; F[t] = 0xffffffffu; }
+    const uint32_t deg = in ? min(g.degree(u), a.sampling) : 0;
+    // the group's map uses the first Tn <= T slots, Tn >= 2 deg
+    int tb = 1;
+    while (tb < TBITS && (1u << tb) < 2 * deg) ++tb;
+    const int Tn = 1 << tb;
+    for (int t = lane; t < Tn; t += G) { K[t] = -1; Wt[t] = 0.0; F[t] = 0xffffffffu; }
     __syncthreads();
     {
       // trip count uniform over the wave: the aggregation shuffles need it
-      const uint32_t deg = in ? min(g.degree(u), a.sampling) : 0;
       uint32_t trips = (deg + G - 1) / G;
       for (int o = 32; o > 0; o >>= 1) trips = max(trips, __shfl_xor(trips, o, 64));
       for (uint32_t t = 0; t < trips; ++t) {
@@ -313,7 +327,7 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
           g.arc(u, i, v, w);
           c = a.C[v];
         }
-        map_insert_aggregated<G>(K, Wt, F, TBITS, valid, c, w, i, 2);
+        map_insert_aggregated<G>(K, Wt, F, tb, valid, c, w, i, 2);
       }
     }
     __syncthreads();
@@ -323,7 +337,7 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
       double wfrom = 0.0, bgain = -DBL_MAX;
       uint32_t bfirst = 0xffffffffu;
       int32_t bkey = -1;
-      for (int t = lane; t < T; t += G) {
+      for (int t = lane; t < Tn; t += G) {
         const int32_t k = K[t];
         if (k < 0) continue;
         if (k == from) { wfrom = Wt[t]; continue; }
@@ -1140,6 +1154,26 @@ __global__ void inc_rank_kernel(const uint32_t* __restrict__ inc, const uint32_t
     out[j] = rank[inc[j]];
 }
 
+__global__ void ranked_size_kernel(const uint32_t* __restrict__ noff, const uint32_t* __restrict__ order, uint32_t m,
+                                   uint32_t* __restrict__ size) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x)
+    size[r] = noff[order[r] + 1] - noff[order[r]];
+}
+
+// pin slot j of net e moves to slot rpoff[rank(e)] + (j - noff[e])
+__global__ void ranked_pins_kernel(const uint32_t* __restrict__ noff, const uint32_t* __restrict__ pins,
+                                   const uint32_t* __restrict__ pin_net, const uint32_t* __restrict__ rank,
+                                   const uint32_t* __restrict__ rpoff, uint64_t p, uint32_t* __restrict__ rpins,
+                                   uint32_t* __restrict__ rpin_rank) {
+  for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < p; j += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t e = pin_net[j];
+    const uint32_t r = rank[e];
+    const uint32_t dst = rpoff[r] + uint32_t(j - noff[e]);
+    rpins[dst] = pins[j];
+    rpin_rank[dst] = r;
+  }
+}
+
 // Builds the net locality ranks of hg once (the hypergraph is immutable).
 void ensure_net_ranks(mkp_hypergraph* hg) {
   if (hg->ranked) return;
@@ -1159,8 +1193,22 @@ void ensure_net_ranks(mkp_hypergraph* hg) {
     net_rank_kernel<<<grid_for(m, 256, 65536), 256, 0, s>>>(sorted.p, m, hg->net_order.p, hg->net_rank.p);
     MKP_LAUNCH_CHECK();
   }
+  hg->rpoff.alloc(size_t(m) + 1);
+  hg->rpins.alloc(std::max<uint64_t>(hg->p, 1));
+  hg->rpin_rank.alloc(std::max<uint64_t>(hg->p, 1));
+  {
+    DevBuf<uint32_t> size(std::max<uint32_t>(m, 1));
+    if (m) {
+      ranked_size_kernel<<<grid_for(m, 256, 65536), 256, 0, s>>>(hg->noff.p, hg->net_order.p, m, size.p);
+      MKP_LAUNCH_CHECK();
+    }
+    exclusive_scan_u32(c, size.p, hg->rpoff.p, m);
+  }
   if (hg->p) {
     inc_rank_kernel<<<grid_for(hg->p, 256, 65536), 256, 0, s>>>(hg->inc.p, hg->net_rank.p, hg->p, hg->inc_rank.p);
+    ranked_pins_kernel<<<grid_for(hg->p, 256, 65536), 256, 0, s>>>(hg->noff.p, hg->pins.p, hg->pin_net.p,
+                                                                   hg->net_rank.p, hg->rpoff.p, hg->p, hg->rpins.p,
+                                                                   hg->rpin_rank.p);
     MKP_LAUNCH_CHECK();
   }
   hg->ranked = true;
@@ -1203,7 +1251,7 @@ extern "C" int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* prm, in
       ensure_net_ranks(hg);
       StarView g{hg->n,         hg->m,         hg->p,    hg->voff.p, hg->inc.p,          hg->noff.p,
                  hg->pins.p,    hg->inc_vtx.p, hg->pin_net.p, hg->nw.p, wt, hg->unit_net_weights,
-                 hg->net_order.p, hg->net_rank.p, hg->inc_rank.p};
+                 hg->net_order.p, hg->inc_rank.p, hg->rpoff.p, hg->rpins.p, hg->rpin_rank.p};
       run.run_from(g, nodes, 2 * hg->p, C.p);
     }
     // restrictClusteringToHypernodes + setCommunityIDs (graph.h:123-125,

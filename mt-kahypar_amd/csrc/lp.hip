@@ -157,14 +157,31 @@ lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
         for (int b = 0; b < K; ++b) score[b] = 0;
         int32_t internal = 0;
         bool border = false;
-        for (uint32_t j = jb; j < je; ++j) {
-          const uint32_t e = a.inc[j];
-          const int32_t w = a.nw[e];
-          const uint64_t c = a.conn[e];
-          if (pc_field(a.pc, a.L, e, fwi, fsh) > 1) internal += w;
-          border |= (c & (c - 1)) != 0;
+        // batches of 8 incident nets: the net ids, then their weight,
+        // connectivity and pin-count words, are each fetched together
+        constexpr int NB = 8;
+        for (uint32_t j0 = jb; j0 < je; j0 += NB) {
+          uint32_t e[NB];
 #pragma unroll
-          for (int b = 0; b < K; ++b) score[b] -= ((c >> b) & 1ull) ? w : 0;
+          for (int t = 0; t < NB; ++t) e[t] = a.inc[j0 + t < je ? j0 + t : jb];
+          int32_t w[NB];
+          uint64_t cw[NB];
+          uint32_t pf[NB];
+#pragma unroll
+          for (int t = 0; t < NB; ++t) {
+            w[t] = a.nw[e[t]];
+            cw[t] = a.conn[e[t]];
+            pf[t] = pc_field(a.pc, a.L, e[t], fwi, fsh);
+          }
+#pragma unroll
+          for (int t = 0; t < NB; ++t) {
+            if (j0 + t >= je) break;
+            const uint64_t c = cw[t];
+            if (pf[t] > 1) internal += w[t];
+            border |= (c & (c - 1)) != 0;
+#pragma unroll
+            for (int b = 0; b < K; ++b) score[b] -= ((c >> b) & 1ull) ? w[t] : 0;
+          }
         }
         visits += deg;
         swept += 1;
@@ -377,19 +394,37 @@ __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const
     const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
     const unsigned long long dec = static_cast<unsigned long long>(-(int64_t(1) << fsh));
     const unsigned long long incv = 1ull << tsh;
-    for (uint32_t jj = a.voff[u]; jj < a.voff[u + 1]; ++jj) {
-      const uint32_t e = a.inc[jj];
-      const int32_t w = a.nw[e];
-      unsigned long long* base = reinterpret_cast<unsigned long long*>(a.pc + uint64_t(e) * a.L.vpc);
-      const uint64_t of = atomicAdd(base + fwi, dec);
-      if (((of >> fsh) & a.L.mask) == 1) {
-        atomicXor(reinterpret_cast<unsigned long long*>(&a.conn[uint64_t(e) * a.vcs + from / 64]), 1ull << (from % 64));
-        d -= w;
+    // batches of 8 incident nets: both counter updates of all 8 nets are in
+    // flight together; the 0<->1 transitions then toggle connectivity
+    constexpr int NB = 8;
+    const uint32_t jb = a.voff[u], je = a.voff[u + 1];
+    for (uint32_t j0 = jb; j0 < je; j0 += NB) {
+      uint32_t e[NB];
+      uint64_t of[NB], ot[NB];
+#pragma unroll
+      for (int t = 0; t < NB; ++t) e[t] = j0 + t < je ? a.inc[j0 + t] : 0u;
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        if (j0 + t < je) {
+          unsigned long long* base = reinterpret_cast<unsigned long long*>(a.pc + uint64_t(e[t]) * a.L.vpc);
+          of[t] = atomicAdd(base + fwi, dec);
+          ot[t] = atomicAdd(base + twi, incv);
+        }
       }
-      const uint64_t ot = atomicAdd(base + twi, incv);
-      if (((ot >> tsh) & a.L.mask) == 0) {
-        atomicXor(reinterpret_cast<unsigned long long*>(&a.conn[uint64_t(e) * a.vcs + to / 64]), 1ull << (to % 64));
-        d += w;
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        if (j0 + t >= je) break;
+        const int32_t w = a.nw[e[t]];
+        if (((of[t] >> fsh) & a.L.mask) == 1) {
+          atomicXor(reinterpret_cast<unsigned long long*>(&a.conn[uint64_t(e[t]) * a.vcs + from / 64]),
+                    1ull << (from % 64));
+          d -= w;
+        }
+        if (((ot[t] >> tsh) & a.L.mask) == 0) {
+          atomicXor(reinterpret_cast<unsigned long long*>(&a.conn[uint64_t(e[t]) * a.vcs + to / 64]),
+                    1ull << (to % 64));
+          d += w;
+        }
       }
     }
   }
@@ -410,11 +445,26 @@ __global__ void lp_verify_kernel(LpArgs a, const uint32_t* __restrict__ cu, cons
     const int fwi = from / a.L.epw, fsh = (from - fwi * a.L.epw) * a.L.b;
     const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
     int64_t real = 0;
-    for (uint32_t jj = a.voff[u]; jj < a.voff[u + 1]; ++jj) {
-      const uint32_t e = a.inc[jj];
-      const int32_t w = a.nw[e];
-      if (pc_field(a.pc, a.L, e, twi, tsh) == 1) real += w;
-      if (pc_field(a.pc, a.L, e, fwi, fsh) == 0) real -= w;
+    constexpr int NB = 8;  // batches of 8 incident nets, loads issued together
+    const uint32_t jb = a.voff[u], je = a.voff[u + 1];
+    for (uint32_t j0 = jb; j0 < je; j0 += NB) {
+      uint32_t e[NB];
+#pragma unroll
+      for (int t = 0; t < NB; ++t) e[t] = a.inc[j0 + t < je ? j0 + t : jb];
+      int32_t w[NB];
+      uint32_t pt[NB], pf[NB];
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        w[t] = a.nw[e[t]];
+        pt[t] = pc_field(a.pc, a.L, e[t], twi, tsh);
+        pf[t] = pc_field(a.pc, a.L, e[t], fwi, fsh);
+      }
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        if (j0 + t >= je) break;
+        if (pt[t] == 1) real += w[t];
+        if (pf[t] == 0) real -= w[t];
+      }
     }
     const bool k = real == int64_t(cgain[j]) || real <= 0;
     keep[j] = k ? 1 : 0;

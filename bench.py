@@ -36,6 +36,9 @@ WORKLOADS = {
                               max_size=10_000, window=4096, local_fraction=0.8, k=8, eps=0.03, seed=42),
     "powerlaw_16k_16k_k2": dict(num_vertices=16_384, num_nets=16_384, exponent=2.5, min_size=2, max_size=64,
                                 window=1024, local_fraction=0.8, k=2, eps=0.03, seed=42),
+    # row-net model: rows are vertices (weight = row nnz), columns are nets
+    "rownet_4M_4M_k64": dict(rows=4_000_000, cols=4_000_000, mean_nnz=8.0, max_nnz=5000, band=10_000,
+                             band_fraction=0.7, k=64, eps=0.03, seed=42),
 }
 
 
@@ -45,6 +48,9 @@ def log(msg):
 
 def make_workload(name):
     w = WORKLOADS[name]
+    if "rows" in w:
+        return w, mtk_amd.HostHypergraph.rownet(w["rows"], w["cols"], w["mean_nnz"], w["max_nnz"], w["band"],
+                                                w["band_fraction"], seed=w["seed"])
     h = mtk_amd.HostHypergraph.powerlaw(w["num_vertices"], w["num_nets"], w["exponent"], w["min_size"],
                                         w["max_size"], window=w["window"], local_fraction=w["local_fraction"],
                                         seed=w["seed"])

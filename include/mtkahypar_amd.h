@@ -289,6 +289,44 @@ MKP_API int mkp_write_hgr(const mkp_host_hypergraph* h, const char* path);
 /* writePartitionFile (io/hypergraph_io.cpp:319-333): one block id per line. */
 MKP_API int mkp_write_partition_file(const int32_t* parts, uint32_t num_vertices, const char* path);
 
+/* ----------------------------------------------------------- batches --- */
+/* The batched form (no counterpart in the reference, which partitions one
+ * hypergraph per call): many independent member hypergraphs live in HBM as
+ * one disjoint union and every round of every member runs in the same kernel
+ * launches.  Members keep local vertex and block ids for all hashing and
+ * tie-breaking, their own seeds, block weights and stop state, so each member
+ * gets exactly the result it would get alone.  Vertices of member g are
+ * numbered consecutively after those of members 0..g-1 in every array. */
+typedef struct mkp_batch mkp_batch;
+typedef struct mkp_batch_partition mkp_batch_partition;
+typedef struct {
+  double epsilon;                /* balance of every member (context.cpp:237-250) */
+  int32_t max_rounds;            /* r-lp-maximum-iterations (5) */
+  int32_t sub_rounds;
+  uint32_t activation_threshold; /* r-lp-he-size-activation-threshold (100) */
+  int32_t rebalancing;           /* r-lp-rebalancing (1) */
+} mkp_batch_lp_params;
+MKP_API int mkp_batch_create(mkp_context* ctx, uint32_t count, const mkp_host_hypergraph* const* members,
+                             mkp_batch** out);
+MKP_API int mkp_batch_destroy(mkp_batch* b);
+MKP_API int mkp_batch_get_info(const mkp_batch* b, uint32_t* members, uint32_t* vertices, uint32_t* nets,
+                               uint64_t* pins);
+/* Borrowed handle of the union hypergraph. */
+MKP_API int mkp_batch_hypergraph(mkp_batch* b, mkp_hypergraph** hg);
+/* k (one value per member); all vertices start in block 0. */
+MKP_API int mkp_batch_partition_create(mkp_batch* b, const int32_t* k, mkp_batch_partition** out);
+MKP_API int mkp_batch_partition_destroy(mkp_batch_partition* bp);
+/* parts: local block ids of all vertices (host, union order), then
+ * initializePartition of every member. */
+MKP_API int mkp_batch_partition_set_parts(mkp_batch_partition* bp, const int32_t* parts);
+MKP_API int mkp_batch_partition_get_parts(const mkp_batch_partition* bp, int32_t* parts);
+/* km1 of every member (host, one value per member). */
+MKP_API int mkp_batch_partition_km1(mkp_batch_partition* bp, int64_t* km1);
+/* LabelPropagationRefiner on every member at once; seeds and stats have one
+ * entry per member. */
+MKP_API int mkp_batch_lp_refine(mkp_batch_partition* bp, const mkp_batch_lp_params* params, const uint64_t* seeds,
+                                mkp_lp_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

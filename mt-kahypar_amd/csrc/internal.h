@@ -75,6 +75,9 @@ void build_incidence(mkp_hypergraph* hg);
 // connectivity, km1) on the device-resident part ids.
 void partition_initialize(mkp_partition* p);
 int64_t partition_km1(mkp_partition* p);
+// batch forms: block weights per (member, block), km1 per member (host out)
+void partition_initialize_batch(mkp_partition* p, const uint32_t* vgraph, const uint32_t* gkbase, uint32_t ktot);
+void partition_km1_batch(mkp_partition* p, const uint32_t* ngraph, uint32_t members, int64_t* out);
 
 // clustering pass / contraction / LP entry points shared by the drivers
 struct ClusterResult {
@@ -83,7 +86,22 @@ struct ClusterResult {
 };
 ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, int pass, uint32_t* cid_out);
 mkp_hypergraph* contract(mkp_hypergraph* hg, const uint32_t* cid, uint32_t* cmap);
-// nodes (host, distinct, ascending) replaces the initial active set if given
+// A batch of independent member hypergraphs stored as one disjoint union
+// (device arrays): vertex -> member, member -> first vertex, first block in
+// the batch's block-weight array, k, seed.
+struct LpBatch {
+  uint32_t members = 0;
+  uint32_t ktot = 0;  // blocks of all members
+  int kmax = 0;
+  const uint32_t* vgraph = nullptr;
+  const uint32_t* gvbase = nullptr;
+  const uint32_t* gkbase = nullptr;
+  const int32_t* gk = nullptr;
+  const uint64_t* gseed = nullptr;
+};
+// nodes (host, distinct, ascending) replaces the initial active set if given.
+// With a batch, prm's weight arrays hold ktot entries, stats holds one entry
+// per member and every member refines exactly as it would alone.
 void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, const uint32_t* nodes = nullptr,
-               uint32_t num_nodes = 0);
+               uint32_t num_nodes = 0, const LpBatch* batch = nullptr);
 }  // namespace mkp

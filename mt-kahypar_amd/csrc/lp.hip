@@ -50,7 +50,13 @@ struct LpWork {
     fill_async<uint32_t>(net_stamp.p, 0u, std::max<uint32_t>(m, 1), s);
     stamp = 0;
   }
-  void ensure(uint32_t nn, int kk) {
+  uint32_t members = 0;
+  // nn vertices, kk blocks (all members together), mm member hypergraphs
+  void ensure(uint32_t nn, int kk, uint32_t mm = 1) {
+    if (mm > members) {
+      members = mm;
+      counters.alloc(size_t(kGcStrideWords) * members);
+    }
     if (nn <= n && kk <= k) return;
     n = std::max(nn, n);
     k = std::max(kk, k);
@@ -62,12 +68,17 @@ struct LpWork {
     keys.alloc(N); keys_sorted.alloc(N); vals.alloc(N); vals_sorted.alloc(N);
     wsort.alloc(N); wscan.alloc(N); bkeys.alloc(N); bkeys_sorted.alloc(N); bcount.alloc(256);
     seg_base.alloc(k); in_w.alloc(k); max_w.alloc(k); perfect_w.alloc(k); overflow.alloc(k);
-    counters.alloc(8);
   }
+  static constexpr int kGcStrideWords = 16;
 };
 
-// counters: 0 = delta, 1 = visits, 2 = reverts, 3 = heavy count, 4 = moves,
-//           5 = vertices swept by the gain kernels
+// counters, kGcStride words per member hypergraph (one member unless batched):
+//   0 = delta of the current sub-round, 1 = visits, 2 = reverts,
+//   3 = heavy-list length (member 0's word, shared), 4 = moves,
+//   5 = vertices swept by the gain kernels, 6 = stopped (rolled back),
+//   7 = total delta, 8 = rounds, 9 = active in the current round
+constexpr int kGcStride = LpWork::kGcStrideWords;
+
 struct LpArgs {
   const uint32_t* voff;
   const uint32_t* inc;
@@ -92,7 +103,31 @@ struct LpArgs {
   int bucket;
   uint32_t bucket_salt;
   int64_t* counters;
+  // batch of independent hypergraphs stored as one disjoint union: block ids
+  // stay local to a member, its block weights live at gkbase[g] + b, and every
+  // hash uses the member's seed and local vertex id, so each member evolves
+  // exactly as it would alone (vgraph == nullptr: a single hypergraph)
+  const uint32_t* vgraph;
+  const uint32_t* gvbase;
+  const uint32_t* gkbase;
+  const int32_t* gk;
+  const uint64_t* gseed;
 };
+
+struct LpMember {
+  uint32_t g, lu, kb;
+  int k;
+  uint64_t seed;
+};
+__device__ inline LpMember member_of(const LpArgs& a, uint32_t u) {
+  if (!a.vgraph) return LpMember{0, u, 0, a.k, a.seed};
+  const uint32_t g = a.vgraph[u];
+  return LpMember{g, u - a.gvbase[g], a.gkbase[g], a.gk[g], a.gseed[g]};
+}
+__device__ inline int64_t* member_counters(const LpArgs& a, uint32_t g) { return a.counters + kGcStride * g; }
+__device__ inline void counter_add(int64_t* c, int64_t v) {
+  if (v) atomicAdd(reinterpret_cast<unsigned long long*>(c), static_cast<unsigned long long>(v));
+}
 
 __device__ inline uint32_t pc_field(const uint64_t* pc, const PcLayout& L, uint32_t e, int wi, int sh) {
   return uint32_t((pc[uint64_t(e) * L.vpc + wi] >> sh) & L.mask);
@@ -102,15 +137,16 @@ __device__ inline bool in_bucket(const LpArgs& a, uint32_t u) {
   return a.buckets <= 1 || bucket_of(a.bucket_salt, u, uint32_t(a.buckets)) == uint32_t(a.bucket);
 }
 
-__device__ inline bool lp_coin(const LpArgs& a, uint32_t u, int b) {
-  return hash4(a.seed, kLpCoinTag | a.rs, u, uint64_t(b)) & 1ull;
+__device__ inline bool lp_coin(const LpArgs& a, const LpMember& m, int b) {
+  return hash4(m.seed, kLpCoinTag | a.rs, m.lu, uint64_t(b)) & 1ull;
 }
 
 // perform condition of moveVertex (label_propagation_refiner.h:84-95)
-__device__ inline bool lp_perform(const LpArgs& a, int from, int to, int32_t gain) {
+__device__ inline bool lp_perform(const LpArgs& a, const LpMember& m, int from, int to, int32_t gain) {
   if (to == from) return false;
   if (gain < 0) return true;
-  return a.rebalancing && gain == 0 && a.part_w[from] - 1 > a.part_w[to] + 1 && a.part_w[to] < a.perfect_w[to];
+  const int64_t* pw = a.part_w + m.kb;
+  return a.rebalancing && gain == 0 && pw[from] - 1 > pw[to] + 1 && pw[to] < a.perfect_w[m.kb + to];
 }
 
 template <typename T>
@@ -135,12 +171,13 @@ template <int K>
 __global__ void __launch_bounds__(kGainThreads)
 lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na, uint32_t* __restrict__ cand_flag,
                       int32_t* __restrict__ cand_to, int32_t* __restrict__ cand_gain, uint32_t* __restrict__ heavy) {
-  if (a.counters[6]) return;  // a rolled-back sub-round stopped the refinement
+  if (!a.vgraph && a.counters[6]) return;  // a rolled-back sub-round stopped the refinement
   int64_t visits = 0, swept = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
     const uint32_t u = active[i];
+    const LpMember mb = member_of(a, u);
     uint32_t flag = 0;
-    if (in_bucket(a, u)) {
+    if (in_bucket(a, u) && !(a.vgraph && member_counters(a, mb.g)[6])) {
       const uint32_t jb = a.voff[u], je = a.voff[u + 1];
       const uint32_t deg = je - jb;
       if (deg > kThreadDegreeLimit) {
@@ -183,21 +220,28 @@ lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
             for (int b = 0; b < K; ++b) score[b] -= ((c >> b) & 1ull) ? w[t] : 0;
           }
         }
-        visits += deg;
-        swept += 1;
+        if (a.vgraph) {
+          counter_add(member_counters(a, mb.g) + 1, deg);
+          counter_add(member_counters(a, mb.g) + 5, 1);
+        } else {
+          visits += deg;
+          swept += 1;
+        }
         if (border) {
           const int64_t wu = a.vw[u];
+          const int64_t* pw = a.part_w + mb.kb;
+          const int64_t* mw = a.max_w + mb.kb;
           int to = from;
           int32_t gain = 0;
 #pragma unroll
           for (int b = 0; b < K; ++b) {
-            if (b < a.k && b != from) {
+            if (b < mb.k && b != from) {
               const int32_t sc = score[b] + internal;
-              const bool nb = sc < gain || (sc == gain && lp_coin(a, u, b));
-              if (nb && a.part_w[b] + wu <= a.max_w[b]) { to = b; gain = sc; }
+              const bool nb = sc < gain || (sc == gain && lp_coin(a, mb, b));
+              if (nb && pw[b] + wu <= mw[b]) { to = b; gain = sc; }
             }
           }
-          if (lp_perform(a, from, to, gain)) {
+          if (lp_perform(a, mb, from, to, gain)) {
             flag = 1;
             cand_to[i] = to;
             cand_gain[i] = gain;
@@ -207,6 +251,7 @@ lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
     }
     cand_flag[i] = flag;
   }
+  if (a.vgraph) return;  // batched: counted per member above
   visits = wave_sum(visits);
   swept = wave_sum(swept);
   if ((threadIdx.x & 63) == 0 && visits) {
@@ -226,7 +271,7 @@ __global__ void __launch_bounds__(64)
 lp_gain_wave_kernel(LpArgs a, const uint32_t* __restrict__ active, const uint32_t* __restrict__ list,
                     const int64_t* __restrict__ list_count, uint32_t count_val, uint32_t* __restrict__ cand_flag,
                     int32_t* __restrict__ cand_to, int32_t* __restrict__ cand_gain) {
-  if (a.counters[6]) return;  // a rolled-back sub-round stopped the refinement
+  if (!a.vgraph && a.counters[6]) return;  // a rolled-back sub-round stopped the refinement
   extern __shared__ int32_t sc[];
   const int lane = threadIdx.x;
   const uint32_t count = list_count ? uint32_t(*list_count) : count_val;
@@ -234,7 +279,8 @@ lp_gain_wave_kernel(LpArgs a, const uint32_t* __restrict__ active, const uint32_
   for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
     const uint32_t i = list ? list[item] : item;
     const uint32_t u = active[i];
-    if (!list && !in_bucket(a, u)) {
+    const LpMember mb = member_of(a, u);
+    if ((!list && !in_bucket(a, u)) || (a.vgraph && member_counters(a, mb.g)[6])) {
       if (lane == 0) cand_flag[i] = 0;
       continue;
     }
@@ -243,7 +289,9 @@ lp_gain_wave_kernel(LpArgs a, const uint32_t* __restrict__ active, const uint32_
       if (lane == 0) cand_flag[i] = 0;
       continue;
     }
-    for (int b = lane; b < a.k; b += 64) sc[b] = 0;
+    const int64_t* pw = a.part_w + mb.kb;
+    const int64_t* mw = a.max_w + mb.kb;
+    for (int b = lane; b < mb.k; b += 64) sc[b] = 0;
     __syncthreads();
     const int from = a.part[u];
     const int fwi = from / a.L.epw;
@@ -266,8 +314,15 @@ lp_gain_wave_kernel(LpArgs a, const uint32_t* __restrict__ active, const uint32_
       }
       border |= lambda > 1;
     }
-    visits += (lane == 0) ? int64_t(je - jb) : 0;
-    swept += (lane == 0) ? 1 : 0;
+    if (a.vgraph) {
+      if (lane == 0) {
+        counter_add(member_counters(a, mb.g) + 1, int64_t(je - jb));
+        counter_add(member_counters(a, mb.g) + 5, 1);
+      }
+    } else {
+      visits += (lane == 0) ? int64_t(je - jb) : 0;
+      swept += (lane == 0) ? 1 : 0;
+    }
     internal = wave_sum(internal);
     border = wave_max_i32(border);
     __syncthreads();
@@ -277,31 +332,31 @@ lp_gain_wave_kernel(LpArgs a, const uint32_t* __restrict__ active, const uint32_
       const int64_t wu = a.vw[u];
       const int INF = 0x7fffffff;
       int m = INF;
-      for (int b = lane; b < a.k; b += 64)
-        if (b != from && a.part_w[b] + wu <= a.max_w[b]) m = min(m, sc[b] + internal);
+      for (int b = lane; b < mb.k; b += 64)
+        if (b != from && pw[b] + wu <= mw[b]) m = min(m, sc[b] + internal);
       m = wave_min_i32(m);
       if (m != INF && m < 0) {
         int i0 = INF;
-        for (int b = lane; b < a.k; b += 64)
-          if (b != from && a.part_w[b] + wu <= a.max_w[b] && sc[b] + internal == m) i0 = min(i0, b);
+        for (int b = lane; b < mb.k; b += 64)
+          if (b != from && pw[b] + wu <= mw[b] && sc[b] + internal == m) i0 = min(i0, b);
         i0 = wave_min_i32(i0);
         int i1 = -1;
-        for (int b = lane; b < a.k; b += 64)
-          if (b > i0 && b != from && a.part_w[b] + wu <= a.max_w[b] && sc[b] + internal == m && lp_coin(a, u, b))
+        for (int b = lane; b < mb.k; b += 64)
+          if (b > i0 && b != from && pw[b] + wu <= mw[b] && sc[b] + internal == m && lp_coin(a, mb, b))
             i1 = max(i1, b);
         i1 = wave_max_i32(i1);
         to = i1 >= 0 ? i1 : i0;
         gain = m;
       } else if (m == 0) {
         int i1 = -1;
-        for (int b = lane; b < a.k; b += 64)
-          if (b != from && a.part_w[b] + wu <= a.max_w[b] && sc[b] + internal == 0 && lp_coin(a, u, b))
+        for (int b = lane; b < mb.k; b += 64)
+          if (b != from && pw[b] + wu <= mw[b] && sc[b] + internal == 0 && lp_coin(a, mb, b))
             i1 = max(i1, b);
         i1 = wave_max_i32(i1);
         if (i1 >= 0) to = i1;
         gain = 0;
       }
-      flag = lp_perform(a, from, to, gain) ? 1 : 0;
+      flag = lp_perform(a, mb, from, to, gain) ? 1 : 0;
     }
     if (lane == 0) {
       cand_flag[i] = uint32_t(flag);
@@ -322,7 +377,8 @@ __global__ void lp_scatter_candidates_kernel(const uint32_t* __restrict__ active
                                              const int32_t* __restrict__ part, const int32_t* __restrict__ vw,
                                              uint32_t* __restrict__ cu, int32_t* __restrict__ cfrom,
                                              int32_t* __restrict__ cto, int32_t* __restrict__ cgain,
-                                             int64_t* __restrict__ in_w) {
+                                             int64_t* __restrict__ in_w, const uint32_t* __restrict__ vgraph,
+                                             const uint32_t* __restrict__ gkbase) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
     if (!cand_flag[i]) continue;
     const uint32_t j = pos[i];
@@ -331,19 +387,35 @@ __global__ void lp_scatter_candidates_kernel(const uint32_t* __restrict__ active
     cfrom[j] = part[u];
     cto[j] = cto_a[i];
     cgain[j] = cgain_a[i];
-    atomicAdd(reinterpret_cast<unsigned long long*>(&in_w[cto_a[i]]), static_cast<unsigned long long>(vw[u]));
+    const uint32_t kb = vgraph ? gkbase[vgraph[u]] : 0u;  // arrivals per (member, block)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&in_w[kb + cto_a[i]]), static_cast<unsigned long long>(vw[u]));
   }
 }
 
-// admission sort key: target block, gain clamped to 24 bits, 30 hash bits
+// admission sort key: target block, gain clamped to 24 bits, 30 hash bits.
+// The block is the member's block index in the batch (its own block for one
+// hypergraph).  With more than 1024 blocks (`split`), keys hold only gain and
+// hash, and the block goes to `blocks` for a second, stable sort.
 __global__ void lp_admission_keys_kernel(LpArgs a, const uint32_t* __restrict__ cu, const int32_t* __restrict__ cto,
                                          const int32_t* __restrict__ cgain, uint32_t nc, uint64_t* __restrict__ keys,
-                                         uint32_t* __restrict__ vals) {
+                                         uint32_t* __restrict__ vals, int split) {
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+    const LpMember mb = member_of(a, cu[j]);
     const int64_t g = min<int64_t>(max<int64_t>(int64_t(cgain[j]) + (1 << 23), 0), (1 << 24) - 1);
-    const uint64_t prio = hash4(a.seed, kLpPrioTag | a.rs, cu[j], 0) >> 34;
-    keys[j] = (uint64_t(cto[j]) << 54) | (uint64_t(g) << 30) | prio;
+    const uint64_t prio = hash4(mb.seed, kLpPrioTag | a.rs, mb.lu, 0) >> 34;
+    const uint64_t low = (uint64_t(g) << 30) | prio;
+    keys[j] = split ? low : ((uint64_t(mb.kb + cto[j]) << 54) | low);
     vals[j] = j;
+  }
+}
+
+// second pass of a split admission sort: block of every candidate in the
+// (gain, hash) order
+__global__ void lp_sorted_blocks_kernel(LpArgs a, const uint32_t* __restrict__ vals, const uint32_t* __restrict__ cu,
+                                        const int32_t* __restrict__ cto, uint32_t nc, uint64_t* __restrict__ blocks) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+    const uint32_t j = vals[i];
+    blocks[i] = member_of(a, cu[j]).kb + uint32_t(cto[j]);
   }
 }
 
@@ -353,19 +425,25 @@ __global__ void lp_sorted_weights_kernel(const uint32_t* __restrict__ vals, cons
     w[i] = vw[cu[vals[i]]];
 }
 
-__global__ void lp_segment_base_kernel(const uint64_t* __restrict__ keys, const int64_t* __restrict__ w,
-                                       const int64_t* __restrict__ scan, uint32_t nc, int64_t* __restrict__ base) {
+__device__ inline uint32_t sorted_block(const uint64_t* keys, const uint64_t* blocks, uint32_t i) {
+  return blocks ? uint32_t(blocks[i]) : uint32_t(keys[i] >> 54);
+}
+
+__global__ void lp_segment_base_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ blocks,
+                                       const int64_t* __restrict__ w, const int64_t* __restrict__ scan, uint32_t nc,
+                                       int64_t* __restrict__ base) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
-    const uint32_t to = uint32_t(keys[i] >> 54);
-    if (i == 0 || uint32_t(keys[i - 1] >> 54) != to) base[to] = scan[i] - w[i];
+    const uint32_t to = sorted_block(keys, blocks, i);
+    if (i == 0 || sorted_block(keys, blocks, i - 1) != to) base[to] = scan[i] - w[i];
   }
 }
 
-__global__ void lp_admit_kernel(LpArgs a, const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                const int64_t* __restrict__ scan, const int64_t* __restrict__ base,
-                                const uint8_t* __restrict__ overflow, uint32_t nc, uint8_t* __restrict__ admit) {
+__global__ void lp_admit_kernel(LpArgs a, const uint64_t* __restrict__ keys, const uint64_t* __restrict__ blocks,
+                                const uint32_t* __restrict__ vals, const int64_t* __restrict__ scan,
+                                const int64_t* __restrict__ base, const uint8_t* __restrict__ overflow, uint32_t nc,
+                                uint8_t* __restrict__ admit) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
-    const uint32_t to = uint32_t(keys[i] >> 54);
+    const uint32_t to = sorted_block(keys, blocks, i);
     const bool ok = !overflow[to] || a.part_w[to] + (scan[i] - base[to]) <= a.max_w[to];
     admit[vals[i]] = ok ? 1 : 0;
   }
@@ -377,19 +455,24 @@ __global__ void lp_admit_kernel(LpArgs a, const uint64_t* __restrict__ keys, con
 __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const int32_t* __restrict__ cfrom,
                                 const int32_t* __restrict__ cto, const uint8_t* __restrict__ admit,
                                 const uint8_t* __restrict__ keep, uint32_t nc, int mode) {
-  if (mode == 2 && !a.counters[6]) return;  // roll back only a sub-round that worsened km1
+  // roll back only a sub-round that worsened km1 (mode 2 adds nothing to the
+  // delta: the sub-round's delta was already set aside by the check)
+  if (mode == 2 && !a.vgraph && !a.counters[6]) return;
   int64_t d = 0;
   for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
     if (admit && !admit[j]) continue;
     if (mode == 1 && keep[j]) continue;
     if (mode == 2 && !keep[j]) continue;
     const uint32_t u = cu[j];
+    const LpMember mb = member_of(a, u);
+    if (mode == 2 && a.vgraph && !member_counters(a, mb.g)[6]) continue;
     const int from = mode == 0 ? cfrom[j] : cto[j];
     const int to = mode == 0 ? cto[j] : cfrom[j];
     const int64_t wu = a.vw[u];
     a.part[u] = to;
-    atomicAdd(reinterpret_cast<unsigned long long*>(&a.part_w[to]), static_cast<unsigned long long>(wu));
-    atomicAdd(reinterpret_cast<unsigned long long*>(&a.part_w[from]), static_cast<unsigned long long>(-wu));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a.part_w[mb.kb + to]), static_cast<unsigned long long>(wu));
+    atomicAdd(reinterpret_cast<unsigned long long*>(&a.part_w[mb.kb + from]), static_cast<unsigned long long>(-wu));
+    int64_t du = 0;
     const int fwi = from / a.L.epw, fsh = (from - fwi * a.L.epw) * a.L.b;
     const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
     const unsigned long long dec = static_cast<unsigned long long>(-(int64_t(1) << fsh));
@@ -418,16 +501,21 @@ __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const
         if (((of[t] >> fsh) & a.L.mask) == 1) {
           atomicXor(reinterpret_cast<unsigned long long*>(&a.conn[uint64_t(e[t]) * a.vcs + from / 64]),
                     1ull << (from % 64));
-          d -= w;
+          du -= w;
         }
         if (((ot[t] >> tsh) & a.L.mask) == 0) {
           atomicXor(reinterpret_cast<unsigned long long*>(&a.conn[uint64_t(e[t]) * a.vcs + to / 64]),
                     1ull << (to % 64));
-          d += w;
+          du += w;
         }
       }
     }
+    if (mode != 2) {
+      if (a.vgraph) counter_add(member_counters(a, mb.g), du);
+      else d += du;
+    }
   }
+  if (a.vgraph) return;
   d = wave_sum(d);
   if ((threadIdx.x & 63) == 0 && d)
     atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), static_cast<unsigned long long>(d));
@@ -468,8 +556,13 @@ __global__ void lp_verify_kernel(LpArgs a, const uint32_t* __restrict__ cu, cons
     }
     const bool k = real == int64_t(cgain[j]) || real <= 0;
     keep[j] = k ? 1 : 0;
-    reverts += k ? 0 : 1;
+    if (a.vgraph) {
+      if (!k) counter_add(member_counters(a, a.vgraph[u]) + 2, 1);
+    } else {
+      reverts += k ? 0 : 1;
+    }
   }
+  if (a.vgraph) return;
   reverts = wave_sum(reverts);
   if ((threadIdx.x & 63) == 0 && reverts)
     atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[2]), static_cast<unsigned long long>(reverts));
@@ -482,13 +575,20 @@ __global__ void __launch_bounds__(64)
 lp_activate_kernel(LpArgs a, const uint32_t* __restrict__ cu, const uint8_t* __restrict__ admit,
                    const uint8_t* __restrict__ keep, uint32_t nc, uint32_t threshold, uint32_t stamp,
                    uint32_t* __restrict__ net_stamp, uint32_t* __restrict__ flag) {
-  if (a.counters[6]) return;  // the sub-round was rolled back
+  if (!a.vgraph && a.counters[6]) return;  // the sub-round was rolled back
   int64_t moves = 0;
   const int lane = threadIdx.x;
   for (uint32_t j = blockIdx.x; j < nc; j += gridDim.x) {
     if ((admit && !admit[j]) || !keep[j]) continue;
     const uint32_t u = cu[j];
-    if (lane == 0) { flag[u] = 1; ++moves; }
+    if (a.vgraph) {
+      const uint32_t g = a.vgraph[u];
+      if (member_counters(a, g)[6]) continue;  // the member's sub-round was rolled back
+      if (lane == 0) { flag[u] = 1; counter_add(member_counters(a, g) + 4, 1); }
+    } else if (lane == 0) {
+      flag[u] = 1;
+      ++moves;
+    }
     const uint32_t jb = a.voff[u], je = a.voff[u + 1];
     for (uint32_t jj = jb + lane; jj < je; jj += 64) {
       const uint32_t e = a.inc[jj];
@@ -526,11 +626,29 @@ __global__ void lp_border_flag_kernel(LpArgs a, uint32_t n, uint32_t* __restrict
 // After a sub-round's moves and reverts: if its total real delta is positive
 // the sub-round is rolled back and the refinement stops (flag counters[6]),
 // else the delta joins the total (counters[7]).
-__global__ void lp_check_kernel(int64_t* __restrict__ counters) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    const int64_t d = counters[0];
-    if (d > 0) counters[6] = 1;
-    else counters[7] += d;
+__global__ void lp_check_kernel(int64_t* __restrict__ counters, uint32_t members) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < members; g += gridDim.x * blockDim.x) {
+    int64_t* c = counters + kGcStride * g;
+    const int64_t d = c[0];
+    c[0] = 0;
+    if (d > 0) c[6] = 1;
+    else c[7] += d;
+  }
+}
+
+// batched rounds: a member takes part in a round while it has active
+// vertices and has not stopped
+__global__ void lp_round_mark_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
+    int64_t* c = member_counters(a, a.vgraph[active[i]]);
+    if (!c[6] && !c[9]) c[9] = 1;
+  }
+}
+__global__ void lp_round_count_kernel(int64_t* __restrict__ counters, uint32_t members) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < members; g += gridDim.x * blockDim.x) {
+    int64_t* c = counters + kGcStride * g;
+    c[8] += c[9];
+    c[9] = 0;
   }
 }
 
@@ -548,7 +666,13 @@ lp_bucket_keys_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
   for (int i = threadIdx.x; i < a.buckets; i += blockDim.x) h[i] = 0;
   __syncthreads();
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const uint32_t b = bucket_of(a.bucket_salt, active[i], uint32_t(a.buckets));
+    uint32_t b;
+    if (a.vgraph) {  // the member's own (seed, round) salt and local id
+      const LpMember mb = member_of(a, active[i]);
+      b = bucket_of(bucket_salt(mb.seed, kLpBucketTag | uint64_t(a.round)), mb.lu, uint32_t(a.buckets));
+    } else {
+      b = bucket_of(a.bucket_salt, active[i], uint32_t(a.buckets));
+    }
     keys[i] = b;
     atomicAdd(&h[b], 1u);
   }
@@ -591,25 +715,33 @@ static LpWork& work_of(mkp_partition* p) {
 }
 
 void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, const uint32_t* nodes,
-               uint32_t num_nodes) {
+               uint32_t num_nodes, const LpBatch* batch) {
   mkp_hypergraph* hg = p->hg;
   Context& c = hg->ctx->c;
   hipStream_t s = c.stream;
   const uint32_t n = hg->n;
+  const uint32_t G = batch ? batch->members : 1;
+  const int K = batch ? int(batch->ktot) : p->k;  // entries of the block-weight arrays
   LpWork& w = work_of(p);
-  w.ensure(n, p->k);
+  w.ensure(n, K, G);
   w.ensure_nets(hg->m, s);
-  MKP_HIP(hipMemcpyAsync(w.max_w.p, prm.max_part_weights, size_t(p->k) * 8, hipMemcpyHostToDevice, s));
-  MKP_HIP(hipMemcpyAsync(w.perfect_w.p, prm.perfect_part_weights, size_t(p->k) * 8, hipMemcpyHostToDevice, s));
-  fill_async<int64_t>(w.counters.p, 0, 8, s);
+  MKP_HIP(hipMemcpyAsync(w.max_w.p, prm.max_part_weights, size_t(K) * 8, hipMemcpyHostToDevice, s));
+  MKP_HIP(hipMemcpyAsync(w.perfect_w.p, prm.perfect_part_weights, size_t(K) * 8, hipMemcpyHostToDevice, s));
+  fill_async<int64_t>(w.counters.p, 0, size_t(kGcStride) * G, s);
   MKP_HIP(hipEventRecord(c.ev_a, s));
 
   LpArgs a;
   a.voff = hg->voff.p; a.inc = hg->inc.p; a.noff = hg->noff.p; a.pins = hg->pins.p;
   a.vw = hg->vw.p; a.nw = hg->nw.p; a.part = p->part.p; a.part_w = p->part_w.p;
   a.pc = p->pc.p; a.conn = p->conn.p; a.max_w = w.max_w.p; a.perfect_w = w.perfect_w.p;
-  a.L = p->L; a.vcs = p->vcs; a.k = p->k; a.rebalancing = prm.rebalancing; a.seed = prm.seed;
+  a.L = p->L; a.vcs = p->vcs; a.k = batch ? batch->kmax : p->k; a.rebalancing = prm.rebalancing;
+  a.seed = prm.seed;
   a.counters = w.counters.p;
+  a.vgraph = batch ? batch->vgraph : nullptr;
+  a.gvbase = batch ? batch->gvbase : nullptr;
+  a.gkbase = batch ? batch->gkbase : nullptr;
+  a.gk = batch ? batch->gk : nullptr;
+  a.gseed = batch ? batch->gseed : nullptr;
   const int B = std::max(1, prm.sub_rounds);
   a.buckets = B;
 
@@ -634,6 +766,9 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
     c.sync();
     na = na32;
   }
+  int kbits = 1;  // bits of a block index in the admission keys
+  while ((1 << kbits) < K) ++kbits;
+  const bool split = kbits > 10;
   int64_t* host = c.pinned;
   int rounds = 0;
   bool stop = false;
@@ -645,6 +780,11 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
     }
     const uint32_t round_stamp = w.stamp;
     fill_async<uint32_t>(w.flag.p, 0u, n, s);
+    if (batch) {  // rounds per member
+      lp_round_mark_kernel<<<grid_for(na, 256, 65536), 256, 0, s>>>(a, w.active.p, na);
+      lp_round_count_kernel<<<grid_for(G, 256, 4096), 256, 0, s>>>(w.counters.p, G);
+      MKP_LAUNCH_CHECK();
+    }
     // stable partition of the active list by bucket: each sub-round sweeps a
     // contiguous ascending slice
     const uint32_t* base = w.active.p;
@@ -678,14 +818,13 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
         launch_gain(a, w, act, ns, s, c.num_cus);
       }
       exclusive_scan_u32(c, w.cand_flag.p, w.pos.p, ns);
-      fill_async<int64_t>(w.in_w.p, 0, p->k, s);
+      fill_async<int64_t>(w.in_w.p, 0, K, s);
       lp_scatter_candidates_kernel<<<grid_for(ns, 256, 65536), 256, 0, s>>>(
           act, ns, w.cand_flag.p, w.pos.p, w.cand_to_a.p, w.cand_gain_a.p, p->part.p, hg->vw.p, w.cand_u.p,
-          w.cand_from.p, w.cand_to.p, w.cand_gain.p, w.in_w.p);
+          w.cand_from.p, w.cand_to.p, w.cand_gain.p, w.in_w.p, a.vgraph, a.gkbase);
       MKP_LAUNCH_CHECK();
       // read back (pinned staging): stop flag of the previous sub-round,
       // number of candidates, arrivals per block, block weights
-      const int K = p->k;
       int64_t* stg = c.staging<int64_t>(2 * size_t(K) + 2 + (size_t(K) + 7) / 8);
       uint8_t* stg_over = reinterpret_cast<uint8_t*>(stg + 2 * K + 2);
       MKP_HIP(hipMemcpyAsync(stg, w.counters.p + 6, 8, hipMemcpyDeviceToHost, s));
@@ -693,7 +832,7 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
       MKP_HIP(hipMemcpyAsync(stg + 2, w.in_w.p, size_t(K) * 8, hipMemcpyDeviceToHost, s));
       MKP_HIP(hipMemcpyAsync(stg + 2 + K, p->part_w.p, size_t(K) * 8, hipMemcpyDeviceToHost, s));
       c.sync();
-      if (stg[0]) {  // the previous sub-round was rolled back on the device
+      if (!batch && stg[0]) {  // the previous sub-round was rolled back on the device
         stop = true;
         break;
       }
@@ -710,24 +849,34 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
       if (any_over) {
         MKP_HIP(hipMemcpyAsync(w.overflow.p, stg_over, K, hipMemcpyHostToDevice, s));
         lp_admission_keys_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(a, w.cand_u.p, w.cand_to.p, w.cand_gain.p,
-                                                                          nc, w.keys.p, w.vals.p);
+                                                                          nc, w.keys.p, w.vals.p, split ? 1 : 0);
         MKP_LAUNCH_CHECK();
-        int kb = 1;
-        while ((1 << kb) < p->k) ++kb;
-        sort_pairs_u64_u32(c, w.keys.p, w.keys_sorted.p, w.vals.p, w.vals_sorted.p, nc, 0, 54 + kb);
-        lp_sorted_weights_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(w.vals_sorted.p, w.cand_u.p, hg->vw.p, nc,
+        const uint32_t* order = w.vals_sorted.p;
+        const uint64_t* blocks = nullptr;
+        if (!split) {
+          sort_pairs_u64_u32(c, w.keys.p, w.keys_sorted.p, w.vals.p, w.vals_sorted.p, nc, 0, 54 + kbits);
+        } else {
+          // (gain, hash) order first, then a stable sort by block
+          sort_pairs_u64_u32(c, w.keys.p, w.keys_sorted.p, w.vals.p, w.vals_sorted.p, nc, 0, 54);
+          lp_sorted_blocks_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(a, w.vals_sorted.p, w.cand_u.p,
+                                                                           w.cand_to.p, nc, w.bkeys.p);
+          MKP_LAUNCH_CHECK();
+          sort_pairs_u64_u32(c, w.bkeys.p, w.bkeys_sorted.p, w.vals_sorted.p, w.vals.p, nc, 0, kbits);
+          order = w.vals.p;
+          blocks = w.bkeys_sorted.p;
+        }
+        lp_sorted_weights_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(order, w.cand_u.p, hg->vw.p, nc,
                                                                           w.wsort.p);
         MKP_LAUNCH_CHECK();
         inclusive_scan_i64(c, w.wsort.p, w.wscan.p, nc);
-        lp_segment_base_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(w.keys_sorted.p, w.wsort.p, w.wscan.p, nc,
-                                                                        w.seg_base.p);
+        lp_segment_base_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(w.keys_sorted.p, blocks, w.wsort.p,
+                                                                        w.wscan.p, nc, w.seg_base.p);
         MKP_LAUNCH_CHECK();
-        lp_admit_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(a, w.keys_sorted.p, w.vals_sorted.p, w.wscan.p,
+        lp_admit_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(a, w.keys_sorted.p, blocks, order, w.wscan.p,
                                                                  w.seg_base.p, w.overflow.p, nc, w.admit.p);
         MKP_LAUNCH_CHECK();
         admit = w.admit.p;
       }
-      fill_async<int64_t>(w.counters.p, 0, 1, s);
       const unsigned gc = grid_for(nc, 256, 65536);
       {
         TimedRegion tr_apply(c.prof, s, kFamLpApply, 0.0);
@@ -739,9 +888,9 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
         lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, w.keep.p, nc, 1);
         MKP_LAUNCH_CHECK();
       }
-      // decided on the device: keep the sub-round (delta into the total) or
-      // roll it back and stop; the host sees the flag at its next read-back
-      lp_check_kernel<<<1, 64, 0, s>>>(w.counters.p);
+      // decided on the device, per member: keep the sub-round (delta into the
+      // total) or roll it back and stop that member
+      lp_check_kernel<<<grid_for(G, 256, 4096), 256, 0, s>>>(w.counters.p, G);
       lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, w.keep.p, nc, 2);
       MKP_LAUNCH_CHECK();
       lp_activate_kernel<<<grid_for(nc, 1, unsigned(c.num_cus) * 16), 64, 0, s>>>(
@@ -757,26 +906,35 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
     MKP_HIP(hipMemcpyAsync(host + 1, w.counters.p + 6, 8, hipMemcpyDeviceToHost, s));
     c.sync();
     na = *reinterpret_cast<const uint32_t*>(host);
-    if (host[1]) stop = true;  // the round's last sub-round was rolled back
+    if (!batch && host[1]) stop = true;  // the round's last sub-round was rolled back
   }
   MKP_HIP(hipEventRecord(c.ev_b, s));
-  MKP_HIP(hipMemcpyAsync(host, w.counters.p, 8 * 8, hipMemcpyDeviceToHost, s));
+  std::vector<int64_t> cnt(size_t(kGcStride) * G);
+  MKP_HIP(hipMemcpyAsync(cnt.data(), w.counters.p, cnt.size() * 8, hipMemcpyDeviceToHost, s));
   c.sync();
+  int64_t all_visits = 0, all_swept = 0;
+  for (uint32_t g = 0; g < G; ++g) {
+    all_visits += cnt[kGcStride * g + 1];
+    all_swept += cnt[kGcStride * g + 5];
+  }
   if (c.prof.enabled) {
     // gain sweep: 16 B per swept vertex (offsets, block id, weight) and per
     // incidence the net id, net weight, one packed counter word and the
     // connectivity words
-    c.prof.bytes[kFamLpGain] += 16.0 * double(host[5]) + (16.0 + 8.0 * p->vcs) * double(host[1]);
+    c.prof.bytes[kFamLpGain] += 16.0 * double(all_swept) + (16.0 + 8.0 * p->vcs) * double(all_visits);
   }
   if (stats) {
-    stats->delta = host[7];
-    stats->moves = host[4];
-    stats->reverts = host[2];
-    stats->visits = host[1];
-    stats->rounds = rounds;
     float ms = 0;
     MKP_HIP(hipEventElapsedTime(&ms, c.ev_a, c.ev_b));
-    stats->device_ms = ms;
+    for (uint32_t g = 0; g < G; ++g) {
+      const int64_t* cg = cnt.data() + kGcStride * g;
+      stats[g].delta = cg[7];
+      stats[g].moves = cg[4];
+      stats[g].reverts = cg[2];
+      stats[g].visits = cg[1];
+      stats[g].rounds = batch ? cg[8] : rounds;
+      stats[g].device_ms = ms;
+    }
   }
 }
 

@@ -159,6 +159,61 @@ void partition_initialize(mkp_partition* p) {
   }
 }
 
+// ------------------------------------------------------------- batches --
+// Members of a batch are disjoint parts of one hypergraph with local block
+// ids; block weights go to the member's own range of the weight array.
+__global__ void block_weight_batch_kernel(const int32_t* __restrict__ part, const int32_t* __restrict__ vw, uint32_t n,
+                                          const uint32_t* __restrict__ vgraph, const uint32_t* __restrict__ gkbase,
+                                          int64_t* __restrict__ part_w) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&part_w[gkbase[vgraph[u]] + uint32_t(part[u])]),
+              static_cast<unsigned long long>(int64_t(vw[u])));
+}
+
+__global__ void km1_batch_kernel(const uint64_t* __restrict__ conn, const int32_t* __restrict__ nw, uint32_t m,
+                                 int vcs, const uint32_t* __restrict__ ngraph, int64_t* __restrict__ acc) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+    int lambda = 0;
+    for (int ci = 0; ci < vcs; ++ci) lambda += __popcll(conn[uint64_t(e) * vcs + ci]);
+    if (lambda > 1)
+      atomicAdd(reinterpret_cast<unsigned long long*>(&acc[ngraph[e]]),
+                static_cast<unsigned long long>(int64_t(lambda - 1) * nw[e]));
+  }
+}
+
+void partition_initialize_batch(mkp_partition* p, const uint32_t* vgraph, const uint32_t* gkbase, uint32_t ktot) {
+  mkp_hypergraph* hg = p->hg;
+  Context& c = hg->ctx->c;
+  fill_async<int64_t>(p->part_w.p, 0, ktot, c.stream);
+  fill_async<int64_t>(p->km1_acc.p, 0, 1, c.stream);
+  if (hg->n) {
+    block_weight_batch_kernel<<<grid_for(hg->n, 256, 65536), 256, 0, c.stream>>>(p->part.p, hg->vw.p, hg->n, vgraph,
+                                                                               gkbase, p->part_w.p);
+    MKP_LAUNCH_CHECK();
+  }
+  if (hg->m) {  // pin counts need no member information: block ids are local
+    const int npb = nets_per_block(p->L);
+    const size_t lds = size_t(npb) * p->L.vpc * 8 + size_t(npb + 1) * 4;
+    pin_count_init_kernel<<<(hg->m + npb - 1) / npb, kInitThreads, lds, c.stream>>>(
+        hg->noff.p, hg->pins.p, p->part.p, hg->nw.p, hg->m, p->k, p->L, p->vcs, npb, p->pc.p, p->conn.p,
+        p->km1_acc.p);
+    MKP_LAUNCH_CHECK();
+  }
+}
+
+void partition_km1_batch(mkp_partition* p, const uint32_t* ngraph, uint32_t members, int64_t* out) {
+  Context& c = p->hg->ctx->c;
+  DevBuf<int64_t> acc(std::max<uint32_t>(members, 1));
+  fill_async<int64_t>(acc.p, 0, members, c.stream);
+  if (p->hg->m) {
+    km1_batch_kernel<<<grid_for(p->hg->m, 256, 65536), 256, 0, c.stream>>>(p->conn.p, p->hg->nw.p, p->hg->m, p->vcs,
+                                                                         ngraph, acc.p);
+    MKP_LAUNCH_CHECK();
+  }
+  MKP_HIP(hipMemcpyAsync(out, acc.p, size_t(members) * 8, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+}
+
 int64_t partition_km1(mkp_partition* p) {
   Context& c = p->hg->ctx->c;
   fill_async<int64_t>(p->km1_acc.p, 0, 1, c.stream);

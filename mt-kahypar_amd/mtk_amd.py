@@ -67,6 +67,11 @@ class PartitionStats(C.Structure):
                 ("lp_visits", C.c_int64)]
 
 
+class BatchLpParams(C.Structure):
+    _fields_ = [("epsilon", C.c_double), ("max_rounds", C.c_int32), ("sub_rounds", C.c_int32),
+                ("activation_threshold", C.c_uint32), ("rebalancing", C.c_int32)]
+
+
 class PowerlawParams(C.Structure):
     _fields_ = [("num_vertices", C.c_uint32), ("num_nets", C.c_uint32), ("exponent", C.c_double),
                 ("min_size", C.c_uint32), ("max_size", C.c_uint32), ("window", C.c_uint32),
@@ -132,6 +137,16 @@ SIGNATURES = {
     "mkp_host_hypergraph_copy": (C.c_int, [vp, vp, vp, vp, vp]),
     "mkp_host_hypergraph_destroy": (C.c_int, [vp]),
     "mkp_hypergraph_create_from_host": (C.c_int, [vp, vp, P(vp)]),
+    "mkp_batch_create": (C.c_int, [vp, C.c_uint32, P(vp), P(vp)]),
+    "mkp_batch_destroy": (C.c_int, [vp]),
+    "mkp_batch_get_info": (C.c_int, [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), P(C.c_uint64)]),
+    "mkp_batch_hypergraph": (C.c_int, [vp, P(vp)]),
+    "mkp_batch_partition_create": (C.c_int, [vp, vp, P(vp)]),
+    "mkp_batch_partition_destroy": (C.c_int, [vp]),
+    "mkp_batch_partition_set_parts": (C.c_int, [vp, vp]),
+    "mkp_batch_partition_get_parts": (C.c_int, [vp, vp]),
+    "mkp_batch_partition_km1": (C.c_int, [vp, vp]),
+    "mkp_batch_lp_refine": (C.c_int, [vp, P(BatchLpParams), vp, vp]),
 }
 
 
@@ -453,4 +468,65 @@ class Partition:
     def close(self):
         if self.h:
             check(lib().mkp_partition_destroy(self.h))
+            self.h = vp()
+
+
+class Batch:
+    """Many hypergraphs resident as one disjoint union (the batched form)."""
+
+    def __init__(self, ctx, hosts):
+        self.ctx = ctx
+        arr = (vp * len(hosts))(*[h.h for h in hosts])
+        self.h = vp()
+        check(lib().mkp_batch_create(ctx.h, len(hosts), arr, C.byref(self.h)))
+
+    def info(self):
+        m, n, e, p = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint64()
+        check(lib().mkp_batch_get_info(self.h, C.byref(m), C.byref(n), C.byref(e), C.byref(p)))
+        return dict(members=m.value, vertices=n.value, nets=e.value, pins=p.value)
+
+    def hypergraph(self):
+        h = vp()
+        check(lib().mkp_batch_hypergraph(self.h, C.byref(h)))
+        hg = Hypergraph(self.ctx, handle=h)
+        hg.borrowed = True
+        return hg
+
+    def close(self):
+        if self.h:
+            check(lib().mkp_batch_destroy(self.h))
+            self.h = vp()
+
+
+class BatchPartition:
+    def __init__(self, batch, k):
+        self.batch = batch
+        self.k = np.ascontiguousarray(k, dtype=np.int32)
+        self.h = vp()
+        check(lib().mkp_batch_partition_create(batch.h, _p(self.k), C.byref(self.h)))
+
+    def set_parts(self, parts):
+        a = np.ascontiguousarray(parts, dtype=np.int32)
+        check(lib().mkp_batch_partition_set_parts(self.h, _p(a)))
+
+    def parts(self):
+        out = np.zeros(max(self.batch.info()["vertices"], 1), np.int32)
+        check(lib().mkp_batch_partition_get_parts(self.h, _p(out)))
+        return out[:self.batch.info()["vertices"]]
+
+    def km1(self):
+        out = np.zeros(len(self.k), np.int64)
+        check(lib().mkp_batch_partition_km1(self.h, _p(out)))
+        return out
+
+    def lp_refine(self, seeds, epsilon=0.03, max_rounds=5, sub_rounds=4, activation_threshold=100, rebalancing=True):
+        prm = BatchLpParams(epsilon, max_rounds, sub_rounds, activation_threshold, int(rebalancing))
+        sd = np.ascontiguousarray(seeds, dtype=np.uint64)
+        st = (LpStats * len(self.k))()
+        check(lib().mkp_batch_lp_refine(self.h, C.byref(prm), _p(sd), C.cast(st, vp)))
+        return [dict(delta=x.delta, moves=x.moves, reverts=x.reverts, visits=x.visits, rounds=x.rounds) for x in st]
+
+    def close(self):
+        if self.h:
+            check(lib().mkp_batch_partition_destroy(self.h))
             self.h = vp()

@@ -277,6 +277,11 @@ MKP_API int mkp_generate_rownet(const mkp_rownet_params* params, mkp_host_hyperg
 MKP_API int mkp_host_hypergraph_info(const mkp_host_hypergraph* h, mkp_hypergraph_info* info);
 MKP_API int mkp_host_hypergraph_copy(const mkp_host_hypergraph* h, uint64_t* net_offsets, uint32_t* pins,
                                      int32_t* vertex_weights, int32_t* net_weights);
+/* Host hypergraph from CSR arrays (copied; vw / nw may be NULL for unit
+ * weights), e.g. to assemble batch members. */
+MKP_API int mkp_host_hypergraph_create(uint32_t n, uint32_t m, const uint64_t* net_offsets, const uint32_t* pins,
+                                       const int32_t* vertex_weights, const int32_t* net_weights,
+                                       mkp_host_hypergraph** out);
 MKP_API int mkp_host_hypergraph_destroy(mkp_host_hypergraph* h);
 MKP_API int mkp_hypergraph_create_from_host(mkp_context* ctx, const mkp_host_hypergraph* h,
                                             mkp_hypergraph** out);
@@ -313,6 +318,14 @@ MKP_API int mkp_batch_get_info(const mkp_batch* b, uint32_t* members, uint32_t* 
                                uint64_t* pins);
 /* Borrowed handle of the union hypergraph. */
 MKP_API int mkp_batch_hypergraph(mkp_batch* b, mkp_hypergraph** hg);
+/* mkp_louvain on every member at once: each member runs all its levels and
+ * rounds as it would alone, with its own seed (seeds: one per member, or NULL
+ * for params->seed everywhere).  communities (host, all vertices, may be
+ * NULL) and the union hypergraph's community ids receive member-local ids;
+ * stats (may be NULL) has one entry per member.  The members must resolve to
+ * the same edge weighting and either all or none be plain graphs. */
+MKP_API int mkp_batch_louvain(mkp_batch* b, const mkp_louvain_params* params, const uint64_t* seeds,
+                              int32_t* communities, mkp_louvain_stats* stats);
 /* k (one value per member); all vertices start in block 0. */
 MKP_API int mkp_batch_partition_create(mkp_batch* b, const int32_t* k, mkp_batch_partition** out);
 MKP_API int mkp_batch_partition_destroy(mkp_batch_partition* bp);

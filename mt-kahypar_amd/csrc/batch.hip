@@ -14,8 +14,10 @@ struct mkp_batch {
   mkp_hypergraph* hg = nullptr;  // the disjoint union (owned)
   uint32_t members = 0;
   std::vector<uint32_t> vbase, nbase;  // members + 1 entries
+  std::vector<uint64_t> pbase;         // members + 1 entries
   std::vector<int64_t> weight;         // total vertex weight per member
-  mkp::DevBuf<uint32_t> vgraph, ngraph, gvbase;
+  std::vector<uint8_t> plain;          // per member: has nets and all of them have two pins
+  mkp::DevBuf<uint32_t> vgraph, ngraph, gvbase, gnbase;
   ~mkp_batch() { delete hg; }
 };
 
@@ -57,6 +59,7 @@ int mkp_batch_create(mkp_context* ctx, uint32_t count, const mkp_host_hypergraph
       b->members = count;
       b->vbase.assign(1, 0);
       b->nbase.assign(1, 0);
+      b->pbase.assign(1, 0);
       std::vector<uint64_t> noff(1, 0);
       std::vector<uint32_t> pins, vgraph, ngraph;
       std::vector<int32_t> vw, nw;
@@ -66,10 +69,14 @@ int mkp_batch_create(mkp_context* ctx, uint32_t count, const mkp_host_hypergraph
         const uint64_t vb = b->vbase.back();
         MKP_CHECK(vb + h.n < (uint64_t(1) << 32) && b->nbase.back() + uint64_t(h.m) < (uint64_t(1) << 32),
                   "batch exceeds 2^32 vertices or nets");
+        bool two = h.m > 0;
         for (uint32_t e = 0; e < h.m; ++e) {
           for (uint64_t j = h.noff[e]; j < h.noff[e + 1]; ++j) pins.push_back(uint32_t(vb) + h.pins[j]);
           noff.push_back(pins.size());
+          two = two && h.noff[e + 1] - h.noff[e] == 2;
         }
+        b->plain.push_back(two ? 1 : 0);
+        b->pbase.push_back(pins.size());
         int64_t w = 0;
         for (uint32_t u = 0; u < h.n; ++u) w += h.vw[u];
         vw.insert(vw.end(), h.vw.begin(), h.vw.end());
@@ -87,9 +94,11 @@ int mkp_batch_create(mkp_context* ctx, uint32_t count, const mkp_host_hypergraph
       b->vgraph.alloc(std::max<uint32_t>(N, 1));
       b->ngraph.alloc(std::max<uint32_t>(M, 1));
       b->gvbase.alloc(count + 1);
+      b->gnbase.alloc(count + 1);
       if (N) MKP_HIP(hipMemcpyAsync(b->vgraph.p, vgraph.data(), size_t(N) * 4, hipMemcpyHostToDevice, c.stream));
       if (M) MKP_HIP(hipMemcpyAsync(b->ngraph.p, ngraph.data(), size_t(M) * 4, hipMemcpyHostToDevice, c.stream));
       MKP_HIP(hipMemcpyAsync(b->gvbase.p, b->vbase.data(), size_t(count + 1) * 4, hipMemcpyHostToDevice, c.stream));
+      MKP_HIP(hipMemcpyAsync(b->gnbase.p, b->nbase.data(), size_t(count + 1) * 4, hipMemcpyHostToDevice, c.stream));
       c.sync();
     } catch (...) {
       delete b;
@@ -121,6 +130,50 @@ int mkp_batch_hypergraph(mkp_batch* b, mkp_hypergraph** hg) {
   return guarded([&] {
     MKP_CHECK(b && hg, "null argument");
     *hg = b->hg;
+  });
+}
+
+int mkp_batch_louvain(mkp_batch* b, const mkp_louvain_params* prm, const uint64_t* seeds, int32_t* communities,
+                      mkp_louvain_stats* stats) {
+  return guarded([&] {
+    MKP_CHECK(b && prm, "null argument");
+    MKP_CHECK(prm->max_passes >= 0 && prm->sub_rounds >= 1 && prm->sub_rounds <= 255, "bad round parameters");
+    mkp_hypergraph* hg = b->hg;
+    Context& c = hg->ctx->c;
+    Bind bind_(c);
+    const uint32_t G = b->members;
+    // every member resolves the edge weighting and the graph case as it
+    // would alone (partitioner.cpp:62-72, graph.cpp:36-78); one union holds
+    // one view, so these must agree across the members
+    int wt = prm->edge_weight;
+    bool plain = b->plain[0] != 0;
+    for (uint32_t g = 0; g < G; ++g) {
+      const uint32_t n = b->vbase[g + 1] - b->vbase[g], m = b->nbase[g + 1] - b->nbase[g];
+      const int w = prm->edge_weight >= 0 ? prm->edge_weight
+                                          : ((double(m) / double(std::max<uint32_t>(n, 1)) < 0.75) ? 2 : 0);
+      if (g == 0) wt = w;
+      MKP_CHECK(w == wt, "batch members resolve to different Louvain edge weightings; set edge_weight explicitly");
+      MKP_CHECK((b->plain[g] != 0) == plain,
+                "a batch cannot mix members whose nets all have two pins with members that have larger nets");
+    }
+    std::vector<uint64_t> sd(G, prm->seed);
+    if (seeds) std::copy(seeds, seeds + G, sd.begin());
+    LouvainBatch d;
+    d.members = G;
+    d.vbase = b->vbase.data();
+    d.nbase = b->nbase.data();
+    d.pbase = b->pbase.data();
+    d.dvbase = b->gvbase.p;
+    d.dnbase = b->gnbase.p;
+    d.vgraph = b->vgraph.p;
+    d.ngraph = b->ngraph.p;
+    d.seeds = sd.data();
+    d.plain = plain;
+    louvain_batch(hg, *prm, wt, d, stats);
+    if (communities && hg->n) {
+      MKP_HIP(hipMemcpyAsync(communities, hg->comm.p, size_t(hg->n) * 4, hipMemcpyDeviceToHost, c.stream));
+      c.sync();
+    }
   });
 }
 

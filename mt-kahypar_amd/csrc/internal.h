@@ -99,6 +99,26 @@ struct LpBatch {
   const int32_t* gk = nullptr;
   const uint64_t* gseed = nullptr;
 };
+// Builds the net locality ranks of hg once (the hypergraph is immutable).
+void ensure_net_ranks(mkp_hypergraph* hg);
+// Louvain over a batch stored as one disjoint union hg: every member runs
+// local_moving_contract_recurse as it would alone (its own seed and total
+// volume, its own round and level stops); hg->comm receives member-local
+// community ids, stats one entry per member.
+struct LouvainBatch {
+  uint32_t members = 0;
+  const uint32_t* vbase = nullptr;   // host, members + 1: first vertex
+  const uint32_t* nbase = nullptr;   // host, members + 1: first net
+  const uint64_t* pbase = nullptr;   // host, members + 1: first pin
+  const uint32_t* dvbase = nullptr;  // device copies of vbase / nbase
+  const uint32_t* dnbase = nullptr;
+  const uint32_t* vgraph = nullptr;  // device, vertex -> member
+  const uint32_t* ngraph = nullptr;  // device, net -> member
+  const uint64_t* seeds = nullptr;   // host, per member
+  bool plain = false;                // every member is a graph (all nets have two pins)
+};
+void louvain_batch(mkp_hypergraph* hg, const mkp_louvain_params& prm, int wt, const LouvainBatch& d,
+                   mkp_louvain_stats* stats);
 // nodes (host, distinct, ascending) replaces the initial active set if given.
 // With a batch, prm's weight arrays hold ktot entries, stats holds one entry
 // per member and every member refines exactly as it would alone.

@@ -166,7 +166,17 @@ struct LvArgs {
   uint32_t* move_u;
   int32_t* move_to;
   uint32_t* move_count;
+  // batch of member hypergraphs as one disjoint union (nullptr: one
+  // hypergraph): member and member-local id of every node, the member's
+  // 1 / total volume, seed and whether it takes part in the round
+  const uint32_t* nmember = nullptr;
+  const uint32_t* nlocal = nullptr;
+  const double* grecip = nullptr;
+  const uint8_t* gactive = nullptr;
+  const uint64_t* gseed = nullptr;
 };
+
+__device__ inline double lv_recip(const LvArgs& a, uint32_t u) { return a.nmember ? a.grecip[a.nmember[u]] : a.recip; }
 
 __device__ inline uint32_t slot_hash(int32_t key, int bits) {
   return (uint32_t(key) * 0x9E3779B1u) >> (32 - bits);
@@ -221,7 +231,7 @@ __device__ inline bool better(double g, uint32_t f, double bg, uint32_t bf) {
 __device__ inline void lv_decide(const LvArgs& a, uint32_t u, double wfrom, double best_gain, int32_t best_key) {
   const int32_t from = a.C[u];
   const double volU = a.nvol[u];
-  const double mult = a.recip * volU;
+  const double mult = lv_recip(a, u) * volU;
   const double bg = wfrom - mult * (a.cvol[from] - volU);
   if (best_key >= 0 && best_gain > bg) {
     if (a.csize[from] == 1 && a.csize[best_key] == 1 && best_key > from) return;
@@ -264,7 +274,7 @@ louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
     double cv[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) cv[i] = a.cvol[key[i] >= 0 ? key[i] : from];
-    const double mult = a.recip * a.nvol[u];
+    const double mult = lv_recip(a, u) * a.nvol[u];
     double wfrom = 0.0, bgain = -DBL_MAX;
     int32_t bkey = -1;
 #pragma unroll
@@ -333,7 +343,7 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
     __syncthreads();
     if (in) {
       const int32_t from = a.C[u];
-      const double mult = a.recip * a.nvol[u];
+      const double mult = lv_recip(a, u) * a.nvol[u];
       double wfrom = 0.0, bgain = -DBL_MAX;
       uint32_t bfirst = 0xffffffffu;
       int32_t bkey = -1;
@@ -512,7 +522,7 @@ __global__ void __launch_bounds__(256) hv_scan_kernel(View g, LvArgs a, HeavyArg
     const uint32_t i = hv_node_of(h.toff, h.count, t0 + s0);
     const uint32_t u = h.list[i];
     const int32_t from = a.C[u];
-    const double mult = a.recip * a.nvol[u];
+    const double mult = lv_recip(a, u) * a.nvol[u];
     double wfrom = 0.0, bgain = -DBL_MAX;
     uint32_t bfirst = 0xffffffffu;
     int32_t bkey = -1;
@@ -570,7 +580,8 @@ __global__ void hv_decide_kernel(LvArgs a, HeavyArgs h, const HvBest* __restrict
 __global__ void louvain_apply_kernel(const uint32_t* __restrict__ mu, const int32_t* __restrict__ mto,
                                      const uint32_t* __restrict__ mcount, const double* __restrict__ nvol,
                                      int32_t* __restrict__ C, double* __restrict__ cvol, int32_t* __restrict__ csize,
-                                     uint32_t* __restrict__ moved_total, uint32_t* __restrict__ next_count) {
+                                     uint32_t* __restrict__ moved_total, uint32_t* __restrict__ next_count,
+                                     const uint32_t* __restrict__ nmember, uint32_t* __restrict__ gmoved) {
   const uint32_t cnt = *mcount;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     atomicAdd(moved_total, cnt);
@@ -581,6 +592,7 @@ __global__ void louvain_apply_kernel(const uint32_t* __restrict__ mu, const int3
     const int32_t to = mto[j];
     const int32_t from = C[u];
     C[u] = to;
+    if (nmember) atomicAdd(&gmoved[nmember[u]], 1u);  // moves per member (batch)
     atomicAdd(&cvol[to], nvol[u]);
     atomicAdd(&cvol[from], -nvol[u]);
     atomicAdd(&csize[to], 1);
@@ -633,11 +645,23 @@ __global__ void max_degree_kernel(View g, uint32_t n, uint32_t* __restrict__ deg
 // degree classes: 0: <= 16, 1: <= 128, 2: <= 1024, 3: heavier; the work list
 // of a round is grouped by (class, bucket) so that every sub-round launch
 // covers exactly its own nodes.
+// In a batch, nodes of members that sit the round out get no slot (-1), and
+// the bucket hash takes the member's seed and local id.
 template <class View>
 __device__ inline int lv_slot(const View& g, const LvArgs& a, uint32_t u) {
   const uint32_t d = min(g.degree(u), a.sampling);
   const int c = d <= 16 ? 0 : (d <= 128 ? 1 : (d <= 1024 ? 2 : 3));
-  const int b = a.buckets <= 1 ? 0 : int(bucket_of(a.bucket_salt, u, uint32_t(a.buckets)));
+  int b = 0;
+  if (a.nmember) {
+    const uint32_t gm = a.nmember[u];
+    if (!a.gactive[gm]) return -1;
+    if (a.buckets > 1) {
+      const uint32_t salt = bucket_salt(a.gseed[gm], kLvBucketTag | (uint64_t(a.level) << 8) | uint64_t(a.round));
+      b = int(bucket_of(salt, a.nlocal[u], uint32_t(a.buckets)));
+    }
+  } else if (a.buckets > 1) {
+    b = int(bucket_of(a.bucket_salt, u, uint32_t(a.buckets)));
+  }
   return c * a.buckets + b;
 }
 
@@ -654,8 +678,8 @@ lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uin
   for (int i = threadIdx.x; i < slots; i += blockDim.x) { h[i] = 0; ha[i] = 0; hb[i] = 0; }
   __syncthreads();
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u < n) {
-    const int slot = lv_slot(g, a, u);
+  const int slot = u < n ? lv_slot(g, a, u) : -1;
+  if (slot >= 0) {
     const uint32_t d = min(g.degree(u), a.sampling);
     atomicAdd(&h[slot], 1u);
     atomicAdd(&ha[slot], static_cast<unsigned long long>(d));
@@ -680,17 +704,14 @@ lv_scatter_kernel(View g, LvArgs a, uint32_t n, const uint32_t* __restrict__ off
   for (int i = threadIdx.x; i < slots; i += blockDim.x) h[i] = 0;
   __syncthreads();
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-  int slot = 0;
+  const int slot = u < n ? lv_slot(g, a, u) : -1;
   uint32_t local = 0;
-  if (u < n) {
-    slot = lv_slot(g, a, u);
-    local = atomicAdd(&h[slot], 1u);
-  }
+  if (slot >= 0) local = atomicAdd(&h[slot], 1u);
   __syncthreads();
   for (int i = threadIdx.x; i < slots; i += blockDim.x)
     if (h[i]) base[i] = offsets[i] + atomicAdd(&cursor[i], h[i]);
   __syncthreads();
-  if (u < n) list[base[slot] + local] = u;
+  if (slot >= 0) list[base[slot] + local] = u;
 }
 
 __global__ void lv_init_kernel(uint32_t n, const double* __restrict__ nvol, int32_t* __restrict__ C,
@@ -729,11 +750,17 @@ __device__ inline uint32_t arc_source(const View& g, uint64_t a) {
 
 template <class View>
 __global__ void emit_arcs_kernel(View g, uint64_t num_arcs, const int32_t* __restrict__ C, int bits,
-                                 uint64_t* __restrict__ keys, double* __restrict__ vals) {
+                                 uint64_t* __restrict__ keys, double* __restrict__ vals,
+                                 const uint32_t* __restrict__ nmember, const uint8_t* __restrict__ gkeep) {
   const uint64_t sentinel = uint64_t(1) << (2 * bits);
   for (uint64_t a = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; a < num_arcs;
        a += uint64_t(gridDim.x) * blockDim.x) {
     const uint32_t u = g.source(a);
+    if (nmember && !gkeep[nmember[u]]) {  // batch: the member ends at this level
+      keys[a] = sentinel;
+      vals[a] = 0.0;
+      continue;
+    }
     uint32_t v;
     double w;
     g.arc(u, uint32_t(a - g.offset(u)), v, w);
@@ -977,7 +1004,8 @@ struct LouvainRun {
         // at most one move per node of this sub-round
         const uint32_t sub_nodes = cnt[sr] + cnt[B + sr] + cnt[2 * B + sr] + cnt[3 * B + sr];
         louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(
-            mu.p, mto.p, a.move_count, nvol, C, cvol.p, csize.p, moved_total.p, mcount.p + ((sr + 1) & 1));
+            mu.p, mto.p, a.move_count, nvol, C, cvol.p, csize.p, moved_total.p, mcount.p + ((sr + 1) & 1),
+            nullptr, nullptr);
         MKP_LAUNCH_CHECK();
       }
       MKP_HIP(hipMemcpyAsync(c.pinned, moved_total.p, 4, hipMemcpyDeviceToHost, s));
@@ -1028,6 +1056,18 @@ struct LouvainRun {
     coarse_volume_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(used.p, map.p, cvol_.p, n, cg.vol.p);
     remap_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(C, n, map.p);
     MKP_LAUNCH_CHECK();
+    coarse_arcs(g, num_arcs, C, cg);
+    return cg;
+  }
+
+  // Coarse CSR of a clustering whose coarse ids (cg.n of them) are already in
+  // C: arcs keyed (C[u], C[v]), self loops dropped, parallel arcs summed.  In
+  // a batch, the arcs of members that end at this level are dropped as well.
+  template <class View>
+  void coarse_arcs(const View& g, uint64_t num_arcs, const int32_t* C, ExplicitGraph& cg,
+                   const uint32_t* nmember = nullptr, const uint8_t* gkeep = nullptr) {
+    const hipStream_t s = c.stream;
+    const uint32_t nc = cg.n;
     int bits = 1;
     while ((uint64_t(1) << bits) < nc) ++bits;
     MKP_CHECK(2 * bits + 1 <= 64, "graph too large for 64-bit arc keys");
@@ -1039,11 +1079,12 @@ struct LouvainRun {
       cg.w.alloc(1);
       cg.arcs = 0;
       c.sync();
-      return cg;
+      return;
     }
     DevBuf<uint64_t> keys(num_arcs), skeys(num_arcs), ukeys(num_arcs);
     DevBuf<double> vals(num_arcs), svals(num_arcs), uvals(num_arcs);
-    emit_arcs_kernel<View><<<grid_for(num_arcs, 256, 65536), 256, 0, s>>>(g, num_arcs, C, bits, keys.p, vals.p);
+    emit_arcs_kernel<View><<<grid_for(num_arcs, 256, 65536), 256, 0, s>>>(g, num_arcs, C, bits, keys.p, vals.p,
+                                                                           nmember, gkeep);
     MKP_LAUNCH_CHECK();
     size_t bytes = 0;
     MKP_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys.p, skeys.p, vals.p, svals.p, num_arcs, 0u,
@@ -1080,7 +1121,6 @@ struct LouvainRun {
     fill_async<uint64_t>(cg.idx.p, 0, 1, s);
     inclusive_max_scan_u64(c, ends.p, cg.idx.p + 1, nc);
     c.sync();
-    return cg;
   }
 
   // local_moving_contract_recurse (parallel_louvain.cpp:27-60), iterative
@@ -1115,6 +1155,465 @@ struct LouvainRun {
     c.sync();
   }
 };
+
+// ------------------------------------------------------------ batch form --
+// Many member hypergraphs as one disjoint union.  Level 0 (the star
+// expansion of the union) numbers the vertices of all members first, then
+// the net nodes of all members in rank order; a member's nets keep their
+// relative ranks because its locality keys lie in its own vertex range.
+// Every node knows its member and its member-local id (the id it has when
+// the member runs alone); contraction numbers the coarse nodes member by
+// member in local order, so from level 1 on every member is one contiguous
+// id range and its coarse graph is the one it builds alone.
+__global__ void batch_layout0_kernel(uint32_t N, uint32_t n, const uint32_t* __restrict__ vgraph,
+                                     const uint32_t* __restrict__ ngraph, const uint32_t* __restrict__ vbase,
+                                     const uint32_t* __restrict__ nbase, uint32_t* __restrict__ nmember,
+                                     uint32_t* __restrict__ nlocal) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    if (u < N) {
+      const uint32_t g = vgraph[u];
+      nmember[u] = g;
+      nlocal[u] = u - vbase[g];
+    } else {
+      const uint32_t r = u - N;
+      const uint32_t g = ngraph[r];
+      nmember[u] = g;
+      nlocal[u] = (vbase[g + 1] - vbase[g]) + (r - nbase[g]);
+    }
+  }
+}
+
+// contiguous members: member of node u = largest g with cb[g] <= u
+__global__ void batch_layout_kernel(const uint32_t* __restrict__ cb, uint32_t G, uint32_t n,
+                                    uint32_t* __restrict__ nmember, uint32_t* __restrict__ nlocal) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = G;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (cb[mid] <= u) lo = mid; else hi = mid;
+    }
+    nmember[u] = lo;
+    nlocal[u] = u - cb[lo];
+  }
+}
+
+// member-major position of a node: mbase[member] + local id
+template <typename T>
+__global__ void batch_to_positions_kernel(const T* __restrict__ in, uint32_t n, const uint32_t* __restrict__ nmember,
+                                          const uint32_t* __restrict__ nlocal, const uint32_t* __restrict__ mbase,
+                                          T* __restrict__ out) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+    out[mbase[nmember[u]] + nlocal[u]] = in[u];
+}
+
+__global__ void batch_from_positions_kernel(const uint32_t* __restrict__ in, uint32_t n,
+                                            const uint32_t* __restrict__ nmember, const uint32_t* __restrict__ nlocal,
+                                            const uint32_t* __restrict__ mbase, uint32_t* __restrict__ out) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+    out[u] = in[mbase[nmember[u]] + nlocal[u]];
+}
+
+__global__ void batch_mark_used_kernel(const int32_t* __restrict__ C, uint32_t n, const uint32_t* __restrict__ nmember,
+                                       const uint8_t* __restrict__ keep, uint32_t* __restrict__ used) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+    if (keep[nmember[u]]) used[C[u]] = 1;
+}
+
+__global__ void batch_remap_kernel(int32_t* __restrict__ C, uint32_t n, const uint32_t* __restrict__ map,
+                                   const uint32_t* __restrict__ nmember, const uint8_t* __restrict__ keep) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+    if (keep[nmember[u]]) C[u] = int32_t(map[C[u]]);
+}
+
+// out[g] = in[idx[g]], g < count
+template <typename T>
+__global__ void gather_kernel(const T* __restrict__ in, const uint32_t* __restrict__ idx, uint32_t count,
+                              T* __restrict__ out) {
+  for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < count; g += gridDim.x * blockDim.x) out[g] = in[idx[g]];
+}
+
+// Total volume of every member with the shape of sum_volume_kernel over the
+// member's own nodes in local order (64 blocks of 256 threads, then the 64
+// partials summed in order on the host), so it is the member's value alone.
+__global__ void __launch_bounds__(256)
+batch_member_volume_kernel(const double* __restrict__ volp, const uint32_t* __restrict__ mbase, uint32_t G,
+                           double* __restrict__ out) {
+  __shared__ double part[256];
+  for (uint32_t g = blockIdx.y; g < G; g += gridDim.y) {
+    const uint32_t b = mbase[g], n = mbase[g + 1] - b;
+    double s = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += volp[b + i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (int(threadIdx.x) < o) part[threadIdx.x] += part[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[size_t(g) * gridDim.x + blockIdx.x] = part[0];
+    __syncthreads();
+  }
+}
+
+// final community of every node of a level: the member's local id at the
+// level where the member ended, else the community of its coarse node
+__global__ void batch_final_kernel(int32_t* __restrict__ C, uint32_t n, const uint32_t* __restrict__ nmember,
+                                   const uint32_t* __restrict__ nlocal, const int32_t* __restrict__ gend,
+                                   int32_t level, const int32_t* __restrict__ coarse) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
+    C[u] = gend[nmember[u]] == level ? int32_t(nlocal[u]) : coarse[C[u]];
+}
+
+struct LouvainBatchRun : LouvainRun {
+  const LouvainBatch& d;
+  const uint32_t G;
+  std::vector<mkp_louvain_stats> gst;
+  DevBuf<uint8_t> gflag_;  // per member: active in the round / kept for the next level
+  DevBuf<uint32_t> gmoved_;
+  DevBuf<double> grecip_;
+  DevBuf<uint64_t> gseed_;
+
+  struct Level {
+    uint32_t n = 0;
+    uint64_t arcs = 0;
+    bool member_major = false;  // node ids are member-major positions
+    DevBuf<uint32_t> nmember, nlocal, mbase;
+    std::vector<uint32_t> nodes;  // per member
+    std::vector<uint64_t> marcs;  // per member
+  };
+
+  LouvainBatchRun(Context& ctx, const mkp_louvain_params& p, const LouvainBatch& desc)
+      : LouvainRun(ctx, p), d(desc), G(desc.members), gst(desc.members) {
+    gflag_.alloc(G);
+    gmoved_.alloc(G);
+    grecip_.alloc(G);
+    gseed_.alloc(G);
+    uint64_t* stg = c.staging<uint64_t>(G);
+    std::copy(d.seeds, d.seeds + G, stg);
+    MKP_HIP(hipMemcpyAsync(gseed_.p, stg, size_t(G) * 8, hipMemcpyHostToDevice, c.stream));
+    c.sync();
+  }
+
+  // uploads one flag per member through the pinned staging
+  void upload_flags(const std::vector<uint8_t>& f) {
+    uint8_t* stg = c.staging<uint8_t>(G);
+    std::copy(f.begin(), f.end(), stg);
+    MKP_HIP(hipMemcpyAsync(gflag_.p, stg, G, hipMemcpyHostToDevice, c.stream));
+    c.sync();
+  }
+
+  template <class View>
+  void local_moving_batch(const View& g, const Level& L, const double* nvol, uint32_t level, int32_t* C,
+                          const std::vector<uint8_t>& alive, std::vector<uint8_t>& changed) {
+    const hipStream_t s = c.stream;
+    const uint32_t n = L.n;
+    const int B = std::max(1, prm.sub_rounds);
+    const int slots = 4 * B;
+    changed.assign(G, 0);
+    cvol_.alloc(std::max<uint32_t>(n, 1));
+    DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
+    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), counts(slots), cursor(slots), offsets(slots), mcount(2),
+        mu(std::max<uint32_t>(n, 1)), moved_total(1);
+    DevBuf<uint64_t> sbytes(slots), sarcs(slots);
+    DevBuf<int32_t> mto(std::max<uint32_t>(n, 1));
+    std::vector<uint64_t> sb(slots, 0), sa(slots, 0);
+    std::vector<uint32_t> cnt(slots), off(slots);
+    if (n == 0) return;
+    lv_init_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, nvol, C, cvol_.p, csize.p);
+    MKP_LAUNCH_CHECK();
+    LvArgs a;
+    a.seed = 0; a.level = level; a.sampling = uint32_t(std::min<uint64_t>(prm.sampling, 0xffffffffu));
+    a.recip = 0.0; a.C = C; a.cvol = cvol_.p; a.csize = csize.p; a.nvol = nvol;
+    a.move_u = mu.p; a.move_to = mto.p; a.move_count = mcount.p;
+    a.buckets = B; a.bucket_salt = 0;
+    a.nmember = L.nmember.p; a.nlocal = L.nlocal.p; a.grecip = grecip_.p; a.gactive = gflag_.p; a.gseed = gseed_.p;
+    std::vector<uint64_t> moved(L.nodes.begin(), L.nodes.end());
+    std::vector<uint8_t> act(G);
+    for (int r = 0;; ++r) {
+      // the loop condition of local moving, per member
+      bool any = false;
+      for (uint32_t gm = 0; gm < G; ++gm) {
+        act[gm] = alive[gm] && L.marcs[gm] > 0 && r < prm.max_passes &&
+                  double(moved[gm]) >= prm.min_move_fraction * double(L.nodes[gm]);
+        any |= act[gm] != 0;
+      }
+      if (!any) break;
+      upload_flags(act);
+      a.round = uint32_t(r);
+      fill_async<uint32_t>(counts.p, 0u, slots, s);
+      fill_async<uint32_t>(cursor.p, 0u, slots, s);
+      fill_async<uint32_t>(moved_total.p, 0u, 1, s);
+      fill_async<uint32_t>(gmoved_.p, 0u, G, s);
+      fill_async<uint64_t>(sarcs.p, 0ull, slots, s);
+      const bool prof = c.prof.enabled;
+      if (prof) fill_async<uint64_t>(sbytes.p, 0ull, slots, s);
+      lv_count_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, counts.p, sarcs.p, prof ? sbytes.p : nullptr);
+      MKP_LAUNCH_CHECK();
+      uint64_t* stg = c.staging<uint64_t>(4 * size_t(slots));
+      uint32_t* stg_cnt = reinterpret_cast<uint32_t*>(stg + 2 * slots);
+      uint32_t* stg_off = reinterpret_cast<uint32_t*>(stg + 3 * slots);
+      MKP_HIP(hipMemcpyAsync(stg_cnt, counts.p, size_t(slots) * 4, hipMemcpyDeviceToHost, s));
+      MKP_HIP(hipMemcpyAsync(stg, sarcs.p, size_t(slots) * 8, hipMemcpyDeviceToHost, s));
+      if (prof) MKP_HIP(hipMemcpyAsync(stg + slots, sbytes.p, size_t(slots) * 8, hipMemcpyDeviceToHost, s));
+      c.sync();
+      std::copy(stg_cnt, stg_cnt + slots, cnt.begin());
+      std::copy(stg, stg + slots, sa.begin());
+      if (prof) std::copy(stg + slots, stg + 2 * slots, sb.begin());
+      uint64_t harcs = 0;
+      uint32_t hnodes = 0;
+      for (int sr = 0; sr < B; ++sr) {
+        harcs = std::max(harcs, sa[3 * B + sr]);
+        hnodes += cnt[3 * B + sr];
+      }
+      if (hnodes) reserve_heavy(harcs, hnodes);
+      uint32_t run = 0;
+      for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
+      std::copy(off.begin(), off.end(), stg_off);
+      MKP_HIP(hipMemcpyAsync(offsets.p, stg_off, size_t(slots) * 4, hipMemcpyHostToDevice, s));
+      lv_scatter_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, offsets.p, cursor.p, list.p);
+      MKP_LAUNCH_CHECK();
+      if (hnodes) prepare_heavy(g, a, list.p + off[3 * B], hnodes);
+      fill_async<uint32_t>(mcount.p, 0u, 2, s);
+      for (int sr = 0; sr < B; ++sr) {
+        a.bucket = sr;
+        a.move_count = mcount.p + (sr & 1);
+        launch_move(g, a, list.p, off, cnt, sa, sb, sr);
+        const uint32_t sub_nodes = cnt[sr] + cnt[B + sr] + cnt[2 * B + sr] + cnt[3 * B + sr];
+        louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(
+            mu.p, mto.p, a.move_count, nvol, C, cvol_.p, csize.p, moved_total.p, mcount.p + ((sr + 1) & 1),
+            L.nmember.p, gmoved_.p);
+        MKP_LAUNCH_CHECK();
+      }
+      uint32_t* stg_moved = c.staging<uint32_t>(G);
+      MKP_HIP(hipMemcpyAsync(stg_moved, gmoved_.p, size_t(G) * 4, hipMemcpyDeviceToHost, s));
+      c.sync();
+      for (uint32_t gm = 0; gm < G; ++gm) {
+        if (!act[gm]) continue;
+        moved[gm] = stg_moved[gm];
+        gst[gm].arcs_swept += int64_t(L.marcs[gm]);
+        gst[gm].rounds += 1;
+        gst[gm].moves += int64_t(moved[gm]);
+        changed[gm] |= moved[gm] > 0;
+      }
+    }
+  }
+
+  // Graph::contract of the members in `keep` (the others end at this level);
+  // coarse nodes are numbered member by member in local order.
+  template <class View>
+  void contract_batch(const View& g, const Level& L, int32_t* C, const std::vector<uint8_t>& keep, Level& out,
+                      ExplicitGraph& cg) {
+    const hipStream_t s = c.stream;
+    const uint32_t n = L.n;
+    TimedRegion tr(c.prof, s, kFamLouvainContract, (12.0 + 4.0 + 12.0) * double(L.arcs) + 20.0 * n);
+    upload_flags(keep);
+    DevBuf<uint32_t> used(std::max<uint32_t>(n, 1)), scan(size_t(n) + 1), map;
+    fill_async<uint32_t>(used.p, 0u, n, s);
+    batch_mark_used_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(C, n, L.nmember.p, gflag_.p, used.p);
+    MKP_LAUNCH_CHECK();
+    const uint32_t* mp = scan.p;
+    if (L.member_major) {
+      exclusive_scan_u32(c, used.p, scan.p, n);
+    } else {
+      DevBuf<uint32_t> upos(std::max<uint32_t>(n, 1));
+      batch_to_positions_kernel<uint32_t><<<grid_for(n, 256, 65536), 256, 0, s>>>(used.p, n, L.nmember.p,
+                                                                                 L.nlocal.p, L.mbase.p, upos.p);
+      MKP_LAUNCH_CHECK();
+      exclusive_scan_u32(c, upos.p, scan.p, n);
+      map.alloc(std::max<uint32_t>(n, 1));
+      batch_from_positions_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(scan.p, n, L.nmember.p, L.nlocal.p,
+                                                                          L.mbase.p, map.p);
+      MKP_LAUNCH_CHECK();
+      mp = map.p;
+    }
+    // coarse member bases: the scan at every member's first position
+    out.mbase.alloc(size_t(G) + 1);
+    gather_kernel<uint32_t><<<grid_for(G + 1, 256, 4096), 256, 0, s>>>(scan.p, L.mbase.p, G + 1, out.mbase.p);
+    MKP_LAUNCH_CHECK();
+    uint32_t* stg = c.staging<uint32_t>(size_t(G) + 1);
+    MKP_HIP(hipMemcpyAsync(stg, out.mbase.p, (size_t(G) + 1) * 4, hipMemcpyDeviceToHost, s));
+    c.sync();
+    std::vector<uint32_t> cb(stg, stg + G + 1);
+    const uint32_t nc = cb[G];
+    out.n = nc;
+    out.member_major = true;
+    out.nodes.resize(G);
+    for (uint32_t gm = 0; gm < G; ++gm) out.nodes[gm] = cb[gm + 1] - cb[gm];
+    cg.n = nc;
+    cg.vol.alloc(std::max<uint32_t>(nc, 1));
+    coarse_volume_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(used.p, mp, cvol_.p, n, cg.vol.p);
+    batch_remap_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(C, n, mp, L.nmember.p, gflag_.p);
+    MKP_LAUNCH_CHECK();
+    out.nmember.alloc(std::max<uint32_t>(nc, 1));
+    out.nlocal.alloc(std::max<uint32_t>(nc, 1));
+    if (nc) {
+      batch_layout_kernel<<<grid_for(nc, 256, 65536), 256, 0, s>>>(out.mbase.p, G, nc, out.nmember.p, out.nlocal.p);
+      MKP_LAUNCH_CHECK();
+    }
+    coarse_arcs(g, L.arcs, C, cg, L.nmember.p, gflag_.p);
+    out.arcs = cg.arcs;
+    // arcs per member: idx at the member bases
+    DevBuf<uint64_t> ab(size_t(G) + 1);
+    gather_kernel<uint64_t><<<grid_for(G + 1, 256, 4096), 256, 0, s>>>(cg.idx.p, out.mbase.p, G + 1, ab.p);
+    MKP_LAUNCH_CHECK();
+    uint64_t* stg64 = c.staging<uint64_t>(size_t(G) + 1);
+    MKP_HIP(hipMemcpyAsync(stg64, ab.p, (size_t(G) + 1) * 8, hipMemcpyDeviceToHost, s));
+    c.sync();
+    out.marcs.resize(G);
+    for (uint32_t gm = 0; gm < G; ++gm) out.marcs[gm] = stg64[gm + 1] - stg64[gm];
+  }
+
+  // local_moving_contract_recurse of every member, all members level by
+  // level in the same launches; C0 receives member-local communities
+  template <class View>
+  void run_batch(const View& g0, Level& L0, int32_t* C0) {
+    const hipStream_t s = c.stream;
+    const uint32_t n0 = L0.n;
+    DevBuf<double> vol0(std::max<uint32_t>(n0, 1));
+    prepare(g0, n0, vol0.p);
+    {
+      // total volume per member in its local node order
+      const double* volp = vol0.p;
+      DevBuf<double> vpos;
+      if (!L0.member_major && n0) {
+        vpos.alloc(n0);
+        batch_to_positions_kernel<double><<<grid_for(n0, 256, 65536), 256, 0, s>>>(vol0.p, n0, L0.nmember.p,
+                                                                                  L0.nlocal.p, L0.mbase.p, vpos.p);
+        MKP_LAUNCH_CHECK();
+        volp = vpos.p;
+      }
+      DevBuf<double> part(size_t(G) * 64);
+      batch_member_volume_kernel<<<dim3(64, std::min<uint32_t>(G, 65535)), 256, 0, s>>>(volp, L0.mbase.p, G,
+                                                                                        part.p);
+      MKP_LAUNCH_CHECK();
+      double* stg = c.staging<double>(size_t(G) * 65);  // partials, then 1 / volume
+      double* rc = stg + size_t(G) * 64;
+      MKP_HIP(hipMemcpyAsync(stg, part.p, size_t(G) * 64 * 8, hipMemcpyDeviceToHost, s));
+      c.sync();
+      for (uint32_t gm = 0; gm < G; ++gm) {
+        double tv = 0.0;
+        for (int i = 0; i < 64; ++i) tv += stg[size_t(gm) * 64 + i];
+        rc[gm] = 1.0 / tv;
+      }
+      MKP_HIP(hipMemcpyAsync(grecip_.p, rc, size_t(G) * 8, hipMemcpyHostToDevice, s));
+      c.sync();
+    }
+    std::vector<uint8_t> alive(G, 1), changed;
+    std::vector<int32_t> gend(G, -1);
+    // levels[i] / graphs[i - 1] / cl[i]: layout, graph and clustering of level i
+    std::vector<std::unique_ptr<Level>> levels;
+    std::vector<std::unique_ptr<ExplicitGraph>> graphs;
+    std::vector<DevBuf<int32_t>> cl;
+    uint32_t level = 0;
+    local_moving_batch(g0, L0, vol0.p, 0, C0, alive, changed);
+    auto settle = [&](uint32_t lv) {  // members without a move end at this level
+      bool more = false;
+      for (uint32_t gm = 0; gm < G; ++gm) {
+        if (!alive[gm]) continue;
+        gst[gm].levels += 1;
+        if (!changed[gm]) {
+          alive[gm] = 0;
+          gend[gm] = int32_t(lv);
+        } else {
+          more = true;
+        }
+      }
+      return more;
+    };
+    bool more = settle(0);
+    while (more) {
+      levels.emplace_back(new Level());
+      graphs.emplace_back(new ExplicitGraph());
+      if (level == 0) {
+        contract_batch(g0, L0, C0, alive, *levels.back(), *graphs.back());
+      } else {
+        contract_batch(graphs[level - 1]->view(), *levels[level - 1], cl[level - 1].p, alive, *levels.back(),
+                       *graphs.back());
+      }
+      ++level;
+      ExplicitGraph& g = *graphs.back();
+      cl.emplace_back(std::max<uint32_t>(g.n, 1));
+      local_moving_batch(g.view(), *levels.back(), g.vol.p, level, cl.back().p, alive, changed);
+      more = settle(level);
+    }
+    // communities from the deepest level back to level 0
+    DevBuf<int32_t> dend(G);
+    {
+      int32_t* stg = c.staging<int32_t>(G);
+      std::copy(gend.begin(), gend.end(), stg);
+      MKP_HIP(hipMemcpyAsync(dend.p, stg, size_t(G) * 4, hipMemcpyHostToDevice, s));
+    }
+    for (int lv = int(level); lv >= 1; --lv) {
+      const Level& L = *levels[lv - 1];
+      if (L.n)
+        batch_final_kernel<<<grid_for(L.n, 256, 65536), 256, 0, s>>>(
+            cl[lv - 1].p, L.n, L.nmember.p, L.nlocal.p, dend.p, lv, lv == int(level) ? nullptr : cl[lv].p);
+    }
+    if (n0)
+      batch_final_kernel<<<grid_for(n0, 256, 65536), 256, 0, s>>>(C0, n0, L0.nmember.p, L0.nlocal.p, dend.p, 0,
+                                                                  level == 0 ? nullptr : cl[0].p);
+    MKP_LAUNCH_CHECK();
+    c.sync();
+  }
+};
+
+void louvain_batch(mkp_hypergraph* hg, const mkp_louvain_params& prm, int wt, const LouvainBatch& d,
+                   mkp_louvain_stats* stats) {
+  Context& c = hg->ctx->c;
+  const hipStream_t s = c.stream;
+  const uint32_t G = d.members;
+  MKP_HIP(hipEventRecord(c.ev_a, s));
+  LouvainBatchRun run(c, prm, d);
+  LouvainBatchRun::Level L0;
+  L0.nodes.resize(G);
+  L0.marcs.resize(G);
+  std::vector<uint32_t> mb(size_t(G) + 1);
+  for (uint32_t g = 0; g <= G; ++g) mb[g] = d.plain ? d.vbase[g] : d.vbase[g] + d.nbase[g];
+  for (uint32_t g = 0; g < G; ++g) {
+    const uint64_t pins = d.pbase[g + 1] - d.pbase[g];
+    L0.nodes[g] = mb[g + 1] - mb[g];
+    L0.marcs[g] = d.plain ? pins : 2 * pins;
+  }
+  L0.n = mb[G];
+  L0.arcs = d.plain ? hg->p : 2 * hg->p;
+  L0.member_major = d.plain;
+  L0.nmember.alloc(std::max<uint32_t>(L0.n, 1));
+  L0.nlocal.alloc(std::max<uint32_t>(L0.n, 1));
+  L0.mbase.alloc(size_t(G) + 1);
+  {
+    uint32_t* stg = c.staging<uint32_t>(size_t(G) + 1);
+    std::copy(mb.begin(), mb.end(), stg);
+    MKP_HIP(hipMemcpyAsync(L0.mbase.p, stg, (size_t(G) + 1) * 4, hipMemcpyHostToDevice, s));
+    c.sync();
+  }
+  if (L0.n) {
+    batch_layout0_kernel<<<grid_for(L0.n, 256, 65536), 256, 0, s>>>(d.plain ? L0.n : hg->n, L0.n, d.vgraph, d.ngraph,
+                                                                    d.dvbase, d.dnbase, L0.nmember.p, L0.nlocal.p);
+    MKP_LAUNCH_CHECK();
+  }
+  DevBuf<int32_t> C(std::max<uint32_t>(L0.n, 1));
+  if (d.plain) {
+    PlainView g{hg->n, hg->voff.p, hg->inc.p, hg->noff.p, hg->pins.p, hg->inc_vtx.p, hg->nw.p, wt,
+                hg->unit_net_weights};
+    run.run_batch(g, L0, C.p);
+  } else {
+    ensure_net_ranks(hg);
+    StarView g{hg->n,         hg->m,         hg->p,    hg->voff.p, hg->inc.p,          hg->noff.p,
+               hg->pins.p,    hg->inc_vtx.p, hg->pin_net.p, hg->nw.p, wt, hg->unit_net_weights,
+               hg->net_order.p, hg->inc_rank.p, hg->rpoff.p, hg->rpins.p, hg->rpin_rank.p};
+    run.run_batch(g, L0, C.p);
+  }
+  MKP_HIP(hipMemcpyAsync(hg->comm.p, C.p, size_t(hg->n) * 4, hipMemcpyDeviceToDevice, s));
+  MKP_HIP(hipEventRecord(c.ev_b, s));
+  c.sync();
+  if (stats) {
+    float ms = 0;
+    MKP_HIP(hipEventElapsedTime(&ms, c.ev_a, c.ev_b));
+    for (uint32_t g = 0; g < G; ++g) {
+      stats[g] = run.gst[g];
+      stats[g].device_ms = ms;
+    }
+  }
+}
 
 // Locality key of every net: the lower median of its first nine pins, packed
 // above the net id so a radix sort yields the (key, id) order.

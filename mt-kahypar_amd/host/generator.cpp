@@ -194,6 +194,30 @@ int mkp_host_hypergraph_copy(const mkp_host_hypergraph* hh, uint64_t* noff, uint
   });
 }
 
+int mkp_host_hypergraph_create(uint32_t n, uint32_t m, const uint64_t* noff, const uint32_t* pins, const int32_t* vw,
+                               const int32_t* nw, mkp_host_hypergraph** out) {
+  return guarded([&] {
+    MKP_CHECK(out && noff, "null argument");
+    MKP_CHECK(noff[0] == 0, "net offsets must start at 0");
+    const uint64_t p = noff[m];
+    MKP_CHECK(p == 0 || pins, "pins is null");
+    for (uint32_t e = 0; e < m; ++e) MKP_CHECK(noff[e + 1] >= noff[e], "net offsets are not monotone");
+    for (uint64_t i = 0; i < p; ++i) MKP_CHECK(pins[i] < n, "pin id out of range");
+    if (vw)
+      for (uint32_t u = 0; u < n; ++u) MKP_CHECK(vw[u] >= 0, "negative vertex weight");
+    auto* hh = new mkp_host_hypergraph();
+    auto& h = hh->h;
+    h.n = n;
+    h.m = m;
+    h.noff.assign(noff, noff + size_t(m) + 1);
+    h.pins.assign(pins, pins + p);
+    h.vw = vw ? std::vector<int32_t>(vw, vw + n) : std::vector<int32_t>(n, 1);
+    h.nw = nw ? std::vector<int32_t>(nw, nw + m) : std::vector<int32_t>(m, 1);
+    build_host_incidence(h);
+    *out = hh;
+  });
+}
+
 int mkp_host_hypergraph_destroy(mkp_host_hypergraph* hh) {
   return guarded([&] { delete hh; });
 }

@@ -141,6 +141,8 @@ SIGNATURES = {
     "mkp_batch_destroy": (C.c_int, [vp]),
     "mkp_batch_get_info": (C.c_int, [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), P(C.c_uint64)]),
     "mkp_batch_hypergraph": (C.c_int, [vp, P(vp)]),
+    "mkp_batch_louvain": (C.c_int, [vp, P(LouvainParams), vp, vp, vp]),
+    "mkp_host_hypergraph_create": (C.c_int, [C.c_uint32, C.c_uint32, vp, vp, vp, vp, P(vp)]),
     "mkp_batch_partition_create": (C.c_int, [vp, vp, P(vp)]),
     "mkp_batch_partition_destroy": (C.c_int, [vp]),
     "mkp_batch_partition_set_parts": (C.c_int, [vp, vp]),
@@ -244,6 +246,16 @@ class HostHypergraph:
         prm = RownetParams(num_rows, num_cols, mean_nnz, max_nnz, band, band_fraction, seed)
         h = vp()
         check(lib().mkp_generate_rownet(C.byref(prm), C.byref(h)))
+        return HostHypergraph(h)
+
+    @staticmethod
+    def from_arrays(n, noff, pins, vw=None, nw=None):
+        noff = np.ascontiguousarray(noff, dtype=np.uint64)
+        pins = np.ascontiguousarray(pins, dtype=np.uint32)
+        vw = None if vw is None else np.ascontiguousarray(vw, dtype=np.int32)
+        nw = None if nw is None else np.ascontiguousarray(nw, dtype=np.int32)
+        h = vp()
+        check(lib().mkp_host_hypergraph_create(n, len(noff) - 1, _p(noff), _p(pins), _p(vw), _p(nw), C.byref(h)))
         return HostHypergraph(h)
 
     @staticmethod
@@ -491,6 +503,21 @@ class Batch:
         hg = Hypergraph(self.ctx, handle=h)
         hg.borrowed = True
         return hg
+
+    def louvain(self, seeds=None, edge_weight=-1, max_passes=5, min_move_fraction=0.01, sampling=200000, seed=0,
+                sub_rounds=8, want_communities=True):
+        """Louvain on every member; returns (member-local communities of all
+        vertices or None, per-member stats)."""
+        prm = LouvainParams(edge_weight, max_passes, min_move_fraction, sampling, seed, sub_rounds)
+        info = self.info()
+        G = info["members"]
+        st = (LouvainStats * G)()
+        sd = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.uint64)
+        out = np.zeros(max(info["vertices"], 1), np.int32) if want_communities else None
+        check(lib().mkp_batch_louvain(self.h, C.byref(prm), _p(sd), _p(out), C.cast(st, vp)))
+        stats = [dict(arcs_swept=x.arcs_swept, rounds=x.rounds, levels=x.levels, moves=x.moves,
+                      device_ms=x.device_ms) for x in st]
+        return (out[:info["vertices"]] if out is not None else None), stats
 
     def close(self):
         if self.h:

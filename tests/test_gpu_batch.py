@@ -88,3 +88,78 @@ def test_batch_member_matches_oracle(ctx):
     assert np.array_equal(out[n0:n0 + n], opart) and km1[1] == okm1
     for h in hosts:
         h.close()
+
+
+def arrays_member(n, m, max_size, seed, heavy_nets=0, exponent=2.3):
+    h = mtk_amd.HostHypergraph.powerlaw(n, m, exponent, 2, max_size, window=256, local_fraction=0.8, seed=seed)
+    a = h.arrays()
+    h.close()
+    nets = [a["pins"][a["noff"][e]:a["noff"][e + 1]] for e in range(len(a["noff"]) - 1)]
+    rng = np.random.default_rng(seed)
+    for _ in range(heavy_nets):  # a net node of degree > 1024: the heavy-node kernels
+        nets.append(np.sort(rng.choice(n, size=1300, replace=False)).astype(np.uint32))
+    noff = np.zeros(len(nets) + 1, np.uint64)
+    noff[1:] = np.cumsum([len(x) for x in nets])
+    return mtk_amd.HostHypergraph.from_arrays(n, noff, np.concatenate(nets))
+
+
+def louvain_alone(ctx, host, seed, **kw):
+    dh = mtk_amd.Hypergraph(ctx, host=host)
+    comm, st = dh.louvain(seed=seed, **kw)
+    dh.close()
+    return comm, st
+
+
+@pytest.mark.parametrize("sub_rounds,count", [(8, 7), (1, 3), (4, 24)])
+def test_batch_louvain_equals_members_alone(ctx, sub_rounds, count):
+    rng = np.random.default_rng(sub_rounds * 100 + count)
+    hosts = []
+    for g in range(count):
+        n = int(rng.integers(200, 4000))
+        hosts.append(arrays_member(n, int(1.2 * n), int(rng.choice([16, 64, 300])), 500 + g,
+                                   heavy_nets=1 if g % 3 == 1 and n > 1300 else 0))
+    seeds = [int(s) for s in rng.integers(1, 1 << 40, count)]
+    b = mtk_amd.Batch(ctx, hosts)
+    comm, stats = b.louvain(seeds=seeds, sub_rounds=sub_rounds)
+    dh = b.hypergraph()
+    assert np.array_equal(dh.arrays()["comm"], comm)  # resident member-local ids
+    off = 0
+    for g, h in enumerate(hosts):
+        n = h.info().num_vertices
+        ac, ast = louvain_alone(ctx, h, seeds[g], sub_rounds=sub_rounds)
+        assert np.array_equal(comm[off:off + n], ac), g
+        for key in ("arcs_swept", "rounds", "levels", "moves"):
+            assert stats[g][key] == ast[key], (g, key)
+        off += n
+    assert max(s["levels"] for s in stats) >= 2
+    b.close()
+    for h in hosts:
+        h.close()
+
+
+def test_batch_louvain_member_matches_oracle(ctx):
+    hosts = [arrays_member(3000, 3600, 300, 31, heavy_nets=1), arrays_member(1000, 1200, 40, 32)]
+    b = mtk_amd.Batch(ctx, hosts)
+    comm, stats = b.louvain(seeds=[9, 10])
+    a = hosts[0].arrays()
+    g = orc.Graph(orc.Hypergraph(3000, a["noff"], a["pins"]), 0, ranked=True)
+    C, ost = g.louvain(sync=True, seed=9, sub_rounds=8)
+    assert comm[:3000].tolist() == C[:3000].tolist()
+    assert (stats[0]["arcs_swept"], stats[0]["rounds"], stats[0]["levels"], stats[0]["moves"]) == \
+        (ost["arcs"], ost["rounds"], ost["levels"], ost["moves"])
+    b.close()
+    for h in hosts:
+        h.close()
+
+
+def test_batch_louvain_rejects_mixed_weighting(ctx):
+    # |E|/|V| < 0.75 resolves the hybrid weighting to "degree", >= 0.75 to "uniform"
+    hosts = [arrays_member(1000, 500, 16, 41), arrays_member(1000, 1200, 16, 42)]
+    b = mtk_amd.Batch(ctx, hosts)
+    with pytest.raises(RuntimeError, match="edge weighting"):
+        b.louvain(seeds=[1, 2])
+    comm, stats = b.louvain(seeds=[1, 2], edge_weight=0)  # explicit weighting is accepted
+    assert len(stats) == 2
+    b.close()
+    for h in hosts:
+        h.close()

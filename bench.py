@@ -42,6 +42,15 @@ WORKLOADS = {
 }
 
 
+# batched form: many independent hypergraphs refined in the same launches;
+# member vertex counts log-uniform in [n_min, n_max], nets = net_ratio * n
+BATCH_WORKLOADS = {
+    "batch_4096_k2-16": dict(members=4096, n_min=1000, n_max=64000, net_ratio=1.2, exponent=2.5, min_size=2,
+                             max_size=256, window=512, local_fraction=0.8, ks=(2, 4, 8, 16), eps=0.03, seed=42,
+                             loop_stride=32, cpu_stride=8),
+}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -142,12 +151,209 @@ def pmc_summary(fetch_dir, write_dir, workload):
     print(path)
 
 
+def roofline_of(ctx, prof, workload):
+    """Roofline entry of the dominant kernel: the single kernel (template
+    instantiation) with the most device time among those with an algorithmic
+    byte count; every launch of it was timed on its own with an event pair on
+    the launch stream."""
+    kern = {k: v for k, v in ctx.kernel_profile().items() if v["launches"] and v["bytes"] > 0}
+    for name, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"]):
+        log("  kernel %-52s %8.3f ms %5d launches %8.1f us/launch %9.1f GB/s" % (
+            name, v["ms"], v["launches"], 1e3 * v["ms"] / v["launches"], v["bytes"] / (v["ms"] * 1e6)))
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    d = kern[dom]
+    avg_ms = d["ms"] / d["launches"]
+    bytes_per_launch = d["bytes"] / d["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = traffic_per_launch(workload, dom)
+    copy_ms = ctx.stream_copy(COPY_BYTES, 20)
+    return dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                avg_launch_us=round(avg_ms * 1e3, 2), launches=d["launches"],
+                algorithmic_bytes_per_launch=round(bytes_per_launch),
+                share_of_step_device_time=round(d["ms"] / sum(v["ms"] for v in prof.values()), 3),
+                measured_copy_GBs=round(2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9, 1))
+
+
+def make_batch(w):
+    """Member hypergraphs of a batched workload (generated on host threads)."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(w["seed"])
+    G = w["members"]
+    ns = np.exp(rng.uniform(np.log(w["n_min"]), np.log(w["n_max"]), G)).astype(np.int64)
+    ks = [int(k) for k in rng.choice(w["ks"], G)]
+
+    def gen(g):
+        n = int(ns[g])
+        return mtk_amd.HostHypergraph.powerlaw(n, int(round(w["net_ratio"] * n)), w["exponent"], w["min_size"],
+                                               w["max_size"], window=w["window"], local_fraction=w["local_fraction"],
+                                               seed=w["seed"] + g)
+
+    mtk_amd.lib()  # load the library before the worker threads use it
+    with ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1))) as ex:
+        hosts = list(ex.map(gen, range(G)))
+    return hosts, ks
+
+
+def batch_cpu_baseline(w, hosts, ks, sample, threads):
+    """The oracle's reference-semantics path (asynchronous Louvain and LP,
+    OpenMP) on a sample of the members, one after another."""
+    import oracle as orc
+    pins, secs = 0, 0.0
+    for g in sample:
+        a = hosts[g].arrays()
+        n, k = a["n"], ks[g]
+        W = int(a["vw"].sum())
+        perfect = -(-W // k)
+        maxw = int((1.0 + w["eps"]) * perfect)
+        oh = orc.Hypergraph(n, a["noff"], a["pins"], a["vw"], a["nw"])
+        gr = orc.Graph(oh, orc.Graph.UNIFORM, ranked=True)
+        t0 = time.perf_counter()
+        _, lst = gr.louvain(sync=False, seed=w["seed"] + g, threads=threads)
+        _, _, st = oh.lp_refine(k, start_partition(n, k), [maxw] * k, [perfect] * k, seed=w["seed"] + g,
+                                sync=False, threads=threads)
+        secs += time.perf_counter() - t0
+        pins += lst["arcs"] // 2 + st["visits"]
+    return pins, secs
+
+
+def batch_main(args):
+    """One step = Louvain of every member (all levels) + re-initialisation of
+    every member's partition state from its resident starting partition + km1
+    LP of every member, all members in the same launches.  Beside it, the
+    plain one-hypergraph-per-call GPU path over every loop_stride-th member."""
+    w = BATCH_WORKLOADS[args.workload]
+    t_gen = time.perf_counter()
+    hosts, ks = make_batch(w)
+    infos = [h.info() for h in hosts]
+    ns = [i.num_vertices for i in infos]
+    G = len(hosts)
+    P = sum(i.num_pins for i in infos)
+    log("workload %s: %d members, n=%d m=%d pins=%d (generated in %.1f s)" % (
+        args.workload, G, sum(ns), sum(i.num_nets for i in infos), P, time.perf_counter() - t_gen))
+    ctx = mtk_amd.Context(0)
+    b = mtk_amd.Batch(ctx, hosts)
+    seeds = [w["seed"] + g for g in range(G)]
+    part0 = np.concatenate([start_partition(n, k) for n, k in zip(ns, ks)])
+    base = mtk_amd.BatchPartition(b, ks)
+    base.set_parts(part0)
+    km1_start = base.km1()
+    work = mtk_amd.BatchPartition(b, ks)
+
+    def step():
+        _, lst = b.louvain(seeds=seeds, want_communities=False)
+        work.copy_parts_from(base)
+        st = work.lp_refine(seeds, epsilon=w["eps"])
+        return sum(x["arcs_swept"] for x in lst) // 2 + sum(x["visits"] for x in st)
+
+    for i in range(args.warmup):
+        step()
+        log("warmup step %d done" % i)
+    ctx.synchronize()
+    ctx.set_profiling(True)
+    ctx.reset_profile()
+    units = 0
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        units += step()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    prof = ctx.profile()
+    ctx.set_profiling(False)
+    km1_end = work.km1()
+    log("timed %d steps in %.3f s; km1 (sum over members) %d -> %d" % (args.steps, wall, km1_start.sum(),
+                                                                      km1_end.sum()))
+    for f, v in prof.items():
+        if v["launches"]:
+            log("  %-18s %9.3f ms %6d launches %10.3f GB/s" % (f, v["ms"], v["launches"],
+                                                              v["bytes"] / (v["ms"] * 1e6) if v["ms"] else 0))
+    roofline = roofline_of(ctx, prof, args.workload)
+
+    # plain path: one hypergraph per call, same members, seeds and settings
+    sample = list(range(0, G, w["loop_stride"]))
+    dev = []
+    for g in sample:
+        n, k = ns[g], ks[g]
+        dh = mtk_amd.Hypergraph(ctx, host=hosts[g])
+        perfect = -(-int(infos[g].total_weight) // k)
+        maxw = int((1.0 + w["eps"]) * perfect)
+        bp = mtk_amd.Partition(dh, k)
+        bp.set_parts(start_partition(n, k))
+        dev.append((dh, bp, mtk_amd.Partition(dh, k), k, maxw, perfect, seeds[g]))
+
+    def loop_step():
+        u = 0
+        for dh, bp, wp, k, maxw, perfect, sd in dev:
+            _, lst = dh.louvain(seed=sd, want_communities=False)
+            wp.copy_parts_from(bp)
+            st = wp.lp_refine([maxw] * k, [perfect] * k, seed=sd)
+            u += lst["arcs_swept"] // 2 + st["visits"]
+        return u
+
+    loop_step()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    loop_units = loop_step()
+    ctx.synchronize()
+    loop_wall = time.perf_counter() - t1
+    same = all(int(d[2].km1()) == int(km1_end[g]) for d, g in zip(dev, sample))
+    log("per-hypergraph loop over %d members: %.3f s, km1 equal to the batch: %s" % (len(sample), loop_wall, same))
+    for d in dev:
+        d[2].close()
+        d[1].close()
+        d[0].close()
+
+    cpu = None
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        csample = list(range(0, G, w["cpu_stride"]))
+        log("cpu baseline: oracle reference-semantics Louvain + LP on %d members, %d threads" % (len(csample),
+                                                                                               threads))
+        cp, cs = batch_cpu_baseline(w, hosts, ks, csample, threads)
+        cpu = dict(value=round(cp / cs, 1), unit="pins/s", cores=threads, kind="port",
+                   sample="every %d-th member (%d members), one after another: %.2f s (asynchronous restatement, "
+                          "OpenMP)" % (w["cpu_stride"], len(csample), cs))
+
+    hg_per_s = G * args.steps / wall
+    loop_hg_per_s = len(sample) / loop_wall
+    out = {
+        "metric": "LP+Louvain pins/s",
+        "value": round(units / wall, 1),
+        "unit": "pins/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "members": G, "vertices": sum(ns),
+                   "nets": sum(i.num_nets for i in infos), "pins": P, "k": "per member from %s" % (w["ks"],),
+                   "epsilon": w["eps"], "km1_start_sum": int(km1_start.sum()), "km1_end_sum": int(km1_end.sum()),
+                   "parallelism": "single GPU"},
+        "batched": {"hypergraphs_per_s": round(hg_per_s, 1),
+                    "loop_hypergraphs_per_s": round(loop_hg_per_s, 1),
+                    "loop_pins_per_s": round(loop_units / loop_wall, 1),
+                    "speedup_vs_loop": round(hg_per_s / loop_hg_per_s, 2),
+                    "loop_sample": "every %d-th member (%d members)" % (w["loop_stride"], len(sample)),
+                    "km1_equal_to_loop": same},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    work.close()
+    base.close()
+    b.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="powerlaw_1M_2M_k8", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="powerlaw_1M_2M_k8", choices=sorted(WORKLOADS) + sorted(BATCH_WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc output folders of this script into profiles/ and exit")
@@ -159,6 +365,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path (requested --gpus %d)"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload in BATCH_WORKLOADS:
+        batch_main(args)
+        return
 
     w, host = make_workload(args.workload)
     arrays = host.arrays()
@@ -206,26 +415,7 @@ def main():
             log("  %-18s %9.3f ms %6d launches %10.3f GB/s" % (f, v["ms"], v["launches"],
                                                               v["bytes"] / (v["ms"] * 1e6) if v["ms"] else 0))
 
-    # dominant kernel: the single kernel (instantiation) with the most device
-    # time among those with an algorithmic byte count; every launch of it was
-    # timed on its own with an event pair on the launch stream
-    kern = {k: v for k, v in ctx.kernel_profile().items() if v["launches"] and v["bytes"] > 0}
-    for name, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"]):
-        log("  kernel %-52s %8.3f ms %5d launches %8.1f us/launch %9.1f GB/s" % (
-            name, v["ms"], v["launches"], 1e3 * v["ms"] / v["launches"], v["bytes"] / (v["ms"] * 1e6)))
-    dom = max(kern, key=lambda k: kern[k]["ms"])
-    d = kern[dom]
-    avg_ms = d["ms"] / d["launches"]
-    bytes_per_launch = d["bytes"] / d["launches"]
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = traffic_per_launch(args.workload, dom)
-    copy_ms = ctx.stream_copy(COPY_BYTES, 20)
-    roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    avg_launch_us=round(avg_ms * 1e3, 2), launches=d["launches"],
-                    algorithmic_bytes_per_launch=round(bytes_per_launch),
-                    share_of_step_device_time=round(d["ms"] / sum(v["ms"] for v in prof.values()), 3),
-                    measured_copy_GBs=round(2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9, 1))
+    roofline = roofline_of(ctx, prof, args.workload)
 
     cpu = None
     if not args.no_cpu:

@@ -333,6 +333,9 @@ MKP_API int mkp_batch_partition_destroy(mkp_batch_partition* bp);
  * initializePartition of every member. */
 MKP_API int mkp_batch_partition_set_parts(mkp_batch_partition* bp, const int32_t* parts);
 MKP_API int mkp_batch_partition_get_parts(const mkp_batch_partition* bp, int32_t* parts);
+/* Device-to-device copy of the block ids of src (same batch and k) followed
+ * by initializePartition of every member. */
+MKP_API int mkp_batch_partition_copy_parts(mkp_batch_partition* dst, const mkp_batch_partition* src);
 /* km1 of every member (host, one value per member). */
 MKP_API int mkp_batch_partition_km1(mkp_batch_partition* bp, int64_t* km1);
 /* LabelPropagationRefiner on every member at once; seeds and stats have one

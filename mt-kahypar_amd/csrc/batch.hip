@@ -235,6 +235,21 @@ int mkp_batch_partition_set_parts(mkp_batch_partition* bp, const int32_t* parts)
   });
 }
 
+int mkp_batch_partition_copy_parts(mkp_batch_partition* dst, const mkp_batch_partition* src) {
+  return guarded([&] {
+    MKP_CHECK(dst && src, "null argument");
+    MKP_CHECK(dst->b == src->b && dst->k == src->k, "partitions of different batches or block counts");
+    const mkp_batch* b = dst->b;
+    Context& c = b->hg->ctx->c;
+    Bind bind_(c);
+    if (b->hg->n)
+      MKP_HIP(hipMemcpyAsync(dst->p->part.p, src->p->part.p, size_t(b->hg->n) * 4, hipMemcpyDeviceToDevice,
+                             c.stream));
+    partition_initialize_batch(dst->p, b->vgraph.p, dst->gkbase.p, dst->kbase.back());
+    c.sync();
+  });
+}
+
 int mkp_batch_partition_get_parts(const mkp_batch_partition* bp, int32_t* parts) {
   return guarded([&] {
     MKP_CHECK(bp && parts, "null argument");

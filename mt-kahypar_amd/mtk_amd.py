@@ -142,6 +142,7 @@ SIGNATURES = {
     "mkp_batch_get_info": (C.c_int, [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), P(C.c_uint64)]),
     "mkp_batch_hypergraph": (C.c_int, [vp, P(vp)]),
     "mkp_batch_louvain": (C.c_int, [vp, P(LouvainParams), vp, vp, vp]),
+    "mkp_batch_partition_copy_parts": (C.c_int, [vp, vp]),
     "mkp_host_hypergraph_create": (C.c_int, [C.c_uint32, C.c_uint32, vp, vp, vp, vp, P(vp)]),
     "mkp_batch_partition_create": (C.c_int, [vp, vp, P(vp)]),
     "mkp_batch_partition_destroy": (C.c_int, [vp]),
@@ -535,6 +536,9 @@ class BatchPartition:
     def set_parts(self, parts):
         a = np.ascontiguousarray(parts, dtype=np.int32)
         check(lib().mkp_batch_partition_set_parts(self.h, _p(a)))
+
+    def copy_parts_from(self, src):
+        check(lib().mkp_batch_partition_copy_parts(self.h, src.h))
 
     def parts(self):
         out = np.zeros(max(self.batch.info()["vertices"], 1), np.int32)

@@ -343,6 +343,61 @@ MKP_API int mkp_batch_partition_km1(mkp_batch_partition* bp, int64_t* km1);
 MKP_API int mkp_batch_lp_refine(mkp_batch_partition* bp, const mkp_batch_lp_params* params, const uint64_t* seeds,
                                 mkp_lp_stats* stats);
 
+/* ------------------------------------------------------------ serving --- */
+/* The served form (no counterpart in the reference, which partitions one
+ * hypergraph per call): a request loop that takes partition requests of any
+ * size as they arrive.  `slots` requests run at once, each on a context of
+ * its own (own stream, own device allocation cache), so their kernels overlap
+ * on the device; a request is admitted, in arrival order, when a slot is free
+ * and its device byte estimate fits the HBM budget beside the estimates of
+ * the requests in flight and the allocation caches.  Its hypergraph and
+ * partition state stay on the device from admission to completion.  A
+ * request's result does not depend on what else is in flight: every request
+ * runs mkp_partition_hypergraph with its own parameters and seed, sharing no
+ * device state with the others. */
+typedef struct mkp_server mkp_server;
+typedef struct {
+  int32_t device;
+  int32_t slots;              /* requests in flight at once (1..64) */
+  uint64_t hbm_budget_bytes;  /* admission budget; 0 = device memory less 4 GiB */
+} mkp_server_params;
+typedef struct {
+  int64_t km1;
+  double imbalance;
+  int32_t levels;
+  int32_t slot;               /* slot that ran the request */
+  double queue_ms;            /* submission to admission */
+  double service_ms;          /* admission to completion */
+  double latency_ms;          /* submission to completion */
+  uint64_t estimate_bytes;    /* admission estimate */
+  uint64_t peak_bytes;        /* most device bytes its slot held while it ran */
+} mkp_request_result;
+typedef struct {
+  uint64_t submitted, completed, failed;
+  int32_t max_in_flight;       /* most requests running at once */
+  uint64_t max_reserved_bytes; /* most estimate bytes admitted at once */
+  uint64_t budget_bytes;
+  uint64_t trims;              /* allocation caches released for admission */
+} mkp_server_stats;
+/* Device bytes a request needs at most (hypergraph, community detection,
+ * coarsening hierarchy, partition state and refinement work, rounded to the
+ * allocation cache's power-of-two size classes). */
+MKP_API int mkp_request_estimate(uint32_t num_vertices, uint32_t num_nets, uint64_t num_pins, uint32_t max_net_size,
+                                 int32_t k, uint64_t* bytes);
+MKP_API int mkp_server_create(const mkp_server_params* params, mkp_server** out);
+/* Finishes the requests already submitted, then stops the slots. */
+MKP_API int mkp_server_destroy(mkp_server* s);
+/* Queues a request.  h stays owned by the caller and must stay alive until
+ * mkp_server_wait has returned for the ticket; params are copied.  A request
+ * whose estimate exceeds the whole budget is refused here. */
+MKP_API int mkp_server_submit(mkp_server* s, const mkp_host_hypergraph* h, const mkp_partition_params* params,
+                              uint64_t* ticket);
+/* Blocks until the request is done and releases it; parts (host,
+ * num_vertices) and result may be NULL.  A request that failed returns its
+ * error here. */
+MKP_API int mkp_server_wait(mkp_server* s, uint64_t ticket, int32_t* parts, mkp_request_result* result);
+MKP_API int mkp_server_get_stats(mkp_server* s, mkp_server_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -96,7 +97,26 @@ struct DevicePool {
   std::mutex mu;
   std::multimap<size_t, void*> blocks;  // size class -> cached block
   size_t cached = 0;
+  size_t in_use = 0;  // bytes of blocks handed out and not yet returned
+  size_t peak = 0;    // most bytes in use at once since reset_peak
   bool closed = false;
+  size_t held() {
+    std::lock_guard<std::mutex> g(mu);
+    return in_use + cached;
+  }
+  size_t cached_bytes() {
+    std::lock_guard<std::mutex> g(mu);
+    return cached;
+  }
+  size_t reset_peak() {
+    std::lock_guard<std::mutex> g(mu);
+    peak = in_use;
+    return peak;
+  }
+  size_t peak_bytes() {
+    std::lock_guard<std::mutex> g(mu);
+    return peak;
+  }
   static constexpr size_t kMaxCached = size_t(64) << 30;
   static size_t size_class(size_t bytes) {
     size_t c = 256;
@@ -111,6 +131,8 @@ struct DevicePool {
         void* p = it->second;
         blocks.erase(it);
         cached -= cls;
+        in_use += cls;
+        peak = std::max(peak, in_use);
         return p;
       }
     }
@@ -120,11 +142,15 @@ struct DevicePool {
       trim();
       MKP_HIP(hipMalloc(&p, cls));
     }
+    std::lock_guard<std::mutex> g(mu);
+    in_use += cls;
+    peak = std::max(peak, in_use);
     return p;
   }
   void put(void* p, size_t cls) {
     {
       std::lock_guard<std::mutex> g(mu);
+      in_use -= cls;
       if (!closed && cached + cls <= kMaxCached) {
         blocks.emplace(cls, p);
         cached += cls;

@@ -81,10 +81,14 @@ std::vector<int32_t> greedy_growing(const HostHypergraph& hg, int k, const std::
   std::vector<uint32_t> unassigned(hg.n);
   std::iota(unassigned.begin(), unassigned.end(), 0u);
   std::shuffle(unassigned.begin(), unassigned.end(), rng);
+  // part: block id, -1 free, -2 free but too heavy for the block being grown
+  // (free again for the next block)
   size_t next_seed = 0;
   auto pick_seed = [&]() -> int64_t {
     while (next_seed < unassigned.size() && part[unassigned[next_seed]] >= 0) ++next_seed;
-    return next_seed < unassigned.size() ? int64_t(unassigned[next_seed]) : -1;
+    for (size_t i = next_seed; i < unassigned.size(); ++i)
+      if (part[unassigned[i]] == -1) return int64_t(unassigned[i]);
+    return -1;
   };
   for (int b = 0; b + 1 < k; ++b) {
     std::priority_queue<std::pair<double, uint32_t>> pq;
@@ -94,7 +98,7 @@ std::vector<int32_t> greedy_growing(const HostHypergraph& hg, int k, const std::
       while (!pq.empty()) {
         const auto top = pq.top();
         pq.pop();
-        if (part[top.second] < 0 && top.first == score[top.second]) { v = top.second; break; }
+        if (part[top.second] == -1 && top.first == score[top.second]) { v = top.second; break; }
       }
       if (v < 0) v = pick_seed();
       if (v < 0) break;

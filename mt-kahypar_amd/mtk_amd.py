@@ -72,6 +72,22 @@ class BatchLpParams(C.Structure):
                 ("activation_threshold", C.c_uint32), ("rebalancing", C.c_int32)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("hbm_budget_bytes", C.c_uint64)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("km1", C.c_int64), ("imbalance", C.c_double), ("levels", C.c_int32), ("slot", C.c_int32),
+                ("queue_ms", C.c_double), ("service_ms", C.c_double), ("latency_ms", C.c_double),
+                ("estimate_bytes", C.c_uint64), ("peak_bytes", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("max_in_flight", C.c_int32), ("max_reserved_bytes", C.c_uint64), ("budget_bytes", C.c_uint64),
+                ("trims", C.c_uint64)]
+
+
 class PowerlawParams(C.Structure):
     _fields_ = [("num_vertices", C.c_uint32), ("num_nets", C.c_uint32), ("exponent", C.c_double),
                 ("min_size", C.c_uint32), ("max_size", C.c_uint32), ("window", C.c_uint32),
@@ -143,6 +159,13 @@ SIGNATURES = {
     "mkp_batch_hypergraph": (C.c_int, [vp, P(vp)]),
     "mkp_batch_louvain": (C.c_int, [vp, P(LouvainParams), vp, vp, vp]),
     "mkp_batch_partition_copy_parts": (C.c_int, [vp, vp]),
+    "mkp_request_estimate": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_int32,
+                                       P(C.c_uint64)]),
+    "mkp_server_create": (C.c_int, [P(ServerParams), P(vp)]),
+    "mkp_server_destroy": (C.c_int, [vp]),
+    "mkp_server_submit": (C.c_int, [vp, vp, P(PartitionParams), P(C.c_uint64)]),
+    "mkp_server_wait": (C.c_int, [vp, C.c_uint64, vp, P(RequestResult)]),
+    "mkp_server_get_stats": (C.c_int, [vp, P(ServerStats)]),
     "mkp_host_hypergraph_create": (C.c_int, [C.c_uint32, C.c_uint32, vp, vp, vp, vp, P(vp)]),
     "mkp_batch_partition_create": (C.c_int, [vp, vp, P(vp)]),
     "mkp_batch_partition_destroy": (C.c_int, [vp]),
@@ -560,4 +583,58 @@ class BatchPartition:
     def close(self):
         if self.h:
             check(lib().mkp_batch_partition_destroy(self.h))
+            self.h = vp()
+
+
+def partition_params(k, epsilon, seed, **overrides):
+    """Fast-preset parameter struct with keyword overrides."""
+    prm = PartitionParams()
+    check(lib().mkp_partition_params_default(C.byref(prm), k, epsilon, seed))
+    for key, val in overrides.items():
+        if key not in dict(PartitionParams._fields_):
+            raise KeyError(key)
+        setattr(prm, key, int(val) if isinstance(val, bool) else val)
+    return prm
+
+
+def request_estimate(num_vertices, num_nets, num_pins, max_net_size, k):
+    out = C.c_uint64()
+    check(lib().mkp_request_estimate(num_vertices, num_nets, num_pins, max_net_size, k, C.byref(out)))
+    return out.value
+
+
+class Server:
+    """Request loop of the served form: submit() queues a partition request,
+    wait() returns its block ids and timings."""
+
+    def __init__(self, device=0, slots=4, hbm_budget_bytes=0):
+        prm = ServerParams(device, slots, hbm_budget_bytes)
+        self.h = vp()
+        check(lib().mkp_server_create(C.byref(prm), C.byref(self.h)))
+        self._hosts = {}
+
+    def submit(self, host, k, epsilon, seed, **overrides):
+        prm = partition_params(k, epsilon, seed, **overrides)
+        t = C.c_uint64()
+        check(lib().mkp_server_submit(self.h, host.h, C.byref(prm), C.byref(t)))
+        self._hosts[t.value] = host  # alive until waited for
+        return t.value
+
+    def wait(self, ticket, want_parts=True):
+        host = self._hosts.pop(ticket)
+        n = host.info().num_vertices
+        out = np.zeros(max(n, 1), np.int32) if want_parts else None
+        r = RequestResult()
+        check(lib().mkp_server_wait(self.h, ticket, _p(out), C.byref(r)))
+        res = {f: getattr(r, f) for f, _ in RequestResult._fields_}
+        return (out[:n] if out is not None else None), res
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().mkp_server_get_stats(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            check(lib().mkp_server_destroy(self.h))
             self.h = vp()

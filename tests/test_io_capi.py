@@ -181,3 +181,12 @@ def test_header_is_valid_c_and_cxx(tmp_path):
         r = subprocess.run([compiler, "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"), "-c", str(src),
                             "-o", str(tmp_path / "h.o")], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_request_estimate_grows_with_size_and_k():
+    e = [mtk_amd.request_estimate(n, n, 6 * n, 2000, 8) for n in (1000, 10_000, 100_000)]
+    assert e[0] < e[1] < e[2]
+    assert mtk_amd.request_estimate(10_000, 10_000, 60_000, 2000, 128) >= e[1]
+    assert e[2] >= 2 * 28 * 600_000  # at least the resident CSR, rounded to size classes
+    with pytest.raises(RuntimeError):
+        mtk_amd.request_estimate(10, 10, 60, 5, 1)

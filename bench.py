@@ -51,6 +51,16 @@ BATCH_WORKLOADS = {
 }
 
 
+# served form: partition requests arriving as a Poisson stream; pins per
+# request log-uniform in [p_min, p_max], as many vertices as nets
+SERVE_WORKLOADS = {
+    "serve_poisson_10k-20M": dict(requests=160, p_min=10_000, p_max=20_000_000, exponent=2.3, min_size=2,
+                                  max_size=10_000, window=4096, local_fraction=0.8, vertex_weight_max=8,
+                                  ks=(2, 8, 32, 128), eps=0.03, seed=42, slots=8, loads=(0.5, 1.0, 2.0),
+                                  plain_stride=4),
+}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -348,12 +358,148 @@ def batch_main(args):
     b.close()
 
 
+def powerlaw_mean(exponent, lo, hi):
+    s = np.arange(lo, hi + 1, dtype=np.float64)
+    w = s ** -exponent
+    return float((s * w).sum() / w.sum())
+
+
+def make_requests(w):
+    """Request hypergraphs of a served workload (generated on host threads)."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(w["seed"])
+    R = w["requests"]
+    ps = np.exp(rng.uniform(np.log(w["p_min"]), np.log(w["p_max"]), R))
+    ks = [int(k) for k in rng.choice(w["ks"], R)]
+    mean = powerlaw_mean(w["exponent"], w["min_size"], w["max_size"])
+
+    def gen(i):
+        n = max(64, int(round(ps[i] / mean)))
+        return mtk_amd.HostHypergraph.powerlaw(n, n, w["exponent"], w["min_size"], w["max_size"], window=w["window"],
+                                               local_fraction=w["local_fraction"], vertex_weight_min=1,
+                                               vertex_weight_max=w["vertex_weight_max"], seed=w["seed"] + i)
+
+    mtk_amd.lib()
+    with ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1))) as ex:
+        hosts = list(ex.map(gen, range(R)))
+    return hosts, ks
+
+
+def serve_run(w, hosts, ks, arrivals):
+    """Submits request i at arrivals[i] seconds (None: all at once) and returns
+    (makespan s, per-request results, server stats)."""
+    srv = mtk_amd.Server(0, slots=w["slots"])
+    order = list(range(len(hosts)))
+    t0 = time.perf_counter()
+    tickets = []
+    for i in order:
+        if arrivals is not None:
+            d = t0 + arrivals[i] - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+        tickets.append(srv.submit(hosts[i], ks[i], w["eps"], w["seed"] + i))
+    res = [srv.wait(t, want_parts=False)[1] for t in tickets]
+    makespan = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return makespan, res, st
+
+
+def serve_main(args):
+    """Served form: saturation throughput (every request submitted at once),
+    then Poisson arrivals at fractions of it with p50 / p99 latency, beside the
+    plain path (one request at a time on one stream, upload included) over
+    every plain_stride-th request."""
+    w = SERVE_WORKLOADS[args.workload]
+    t_gen = time.perf_counter()
+    hosts, ks = make_requests(w)
+    infos = [h.info() for h in hosts]
+    R = len(hosts)
+    P = sum(i.num_pins for i in infos)
+    log("workload %s: %d requests, %d pins in total (mean %.0f), generated in %.1f s" % (
+        args.workload, R, P, P / R, time.perf_counter() - t_gen))
+
+    # plain path
+    ctx = mtk_amd.Context(0)
+    sample = list(range(0, R, w["plain_stride"]))
+    plain_km1 = {}
+    for i in sample[:2]:  # warm-up of the plain path
+        dh = mtk_amd.Hypergraph(ctx, host=hosts[i])
+        dh.partition(ks[i], w["eps"], w["seed"] + i)
+        dh.close()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in sample:
+        dh = mtk_amd.Hypergraph(ctx, host=hosts[i])
+        _, st = dh.partition(ks[i], w["eps"], w["seed"] + i)
+        plain_km1[i] = st["km1"]
+        dh.close()
+    ctx.synchronize()
+    plain_s = time.perf_counter() - t0
+    plain_rate = len(sample) / plain_s
+    plain_pins = sum(infos[i].num_pins for i in sample) / plain_s
+    ctx.close()
+    log("plain path: %d requests in %.2f s = %.2f req/s" % (len(sample), plain_s, plain_rate))
+
+    serve_run(w, hosts[:2 * w["slots"]], ks[:2 * w["slots"]], None)  # warm-up of the slots
+    makespan, res, st = serve_run(w, hosts, ks, None)
+    sat = R / makespan
+    same = all(res[i]["km1"] == plain_km1[i] for i in sample)
+    log("saturation: %d requests in %.2f s = %.2f req/s (max in flight %d); km1 equal to the plain path: %s" % (
+        R, makespan, sat, st["max_in_flight"], same))
+    peak_over = sum(1 for r in res if r["peak_bytes"] > r["estimate_bytes"])
+
+    loads = []
+    rng = np.random.default_rng(w["seed"] + 1)
+    for f in w["loads"]:
+        qps = f * sat
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, R))
+        mk, rr, _ = serve_run(w, hosts, ks, arrivals)
+        lat = np.array([r["latency_ms"] for r in rr])
+        loads.append(dict(offered_fraction_of_saturation=f, offered_req_per_s=round(qps, 2),
+                          achieved_req_per_s=round(R / mk, 2), p50_ms=round(float(np.percentile(lat, 50)), 2),
+                          p99_ms=round(float(np.percentile(lat, 99)), 2),
+                          mean_queue_ms=round(float(np.mean([r["queue_ms"] for r in rr])), 2)))
+        log("load %.1fx: offered %.2f req/s, achieved %.2f, p50 %.1f ms, p99 %.1f ms" % (
+            f, qps, R / mk, loads[-1]["p50_ms"], loads[-1]["p99_ms"]))
+
+    out = {
+        "metric": "served req/s",
+        "value": round(sat, 2),
+        "unit": "req/s",
+        "n_gpus": 1,
+        "steps": R,
+        "warmup": 2 * w["slots"],
+        "ms_per_step": round(makespan / R * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "requests": R, "pins_total": P, "pins_per_request":
+                   "log-uniform [%d, %d]" % (w["p_min"], w["p_max"]), "k": "per request from %s" % (w["ks"],),
+                   "epsilon": w["eps"], "slots": w["slots"], "parallelism": "single GPU"},
+        "served": {"saturation_req_per_s": round(sat, 2), "saturation_pins_per_s": round(P / makespan, 1),
+                   "plain_req_per_s": round(plain_rate, 2), "plain_pins_per_s": round(plain_pins, 1),
+                   "speedup_vs_plain": round(sat / plain_rate, 2),
+                   "plain_sample": "every %d-th request (%d requests)" % (w["plain_stride"], len(sample)),
+                   "km1_equal_to_plain": same, "max_in_flight": st["max_in_flight"],
+                   "max_reserved_bytes": st["max_reserved_bytes"], "budget_bytes": st["budget_bytes"],
+                   "requests_above_estimate": peak_over, "loads": loads},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    for h in hosts:
+        h.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="powerlaw_1M_2M_k8", choices=sorted(WORKLOADS) + sorted(BATCH_WORKLOADS))
+    ap.add_argument("--workload", default="powerlaw_1M_2M_k8", choices=sorted(WORKLOADS) + sorted(BATCH_WORKLOADS) + sorted(SERVE_WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc output folders of this script into profiles/ and exit")
@@ -367,6 +513,9 @@ def main():
         sys.exit(2)
     if args.workload in BATCH_WORKLOADS:
         batch_main(args)
+        return
+    if args.workload in SERVE_WORKLOADS:
+        serve_main(args)
         return
 
     w, host = make_workload(args.workload)

@@ -398,7 +398,11 @@ def serve_run(w, hosts, ks, arrivals):
             if d > 0:
                 time.sleep(d)
         tickets.append(srv.submit(hosts[i], ks[i], w["eps"], w["seed"] + i))
-    res = [srv.wait(t, want_parts=False)[1] for t in tickets]
+    res = []
+    for j, t in enumerate(tickets):
+        res.append(srv.wait(t, want_parts=False)[1])
+        if (j + 1) % 40 == 0:
+            log("  %d of %d requests done (%.1f s)" % (j + 1, len(tickets), time.perf_counter() - t0))
     makespan = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
@@ -429,11 +433,13 @@ def serve_main(args):
         dh.close()
     ctx.synchronize()
     t0 = time.perf_counter()
-    for i in sample:
+    for j, i in enumerate(sample):
         dh = mtk_amd.Hypergraph(ctx, host=hosts[i])
         _, st = dh.partition(ks[i], w["eps"], w["seed"] + i)
         plain_km1[i] = st["km1"]
         dh.close()
+        if (j + 1) % 10 == 0:
+            log("  plain path: %d of %d requests (%.1f s)" % (j + 1, len(sample), time.perf_counter() - t0))
     ctx.synchronize()
     plain_s = time.perf_counter() - t0
     plain_rate = len(sample) / plain_s

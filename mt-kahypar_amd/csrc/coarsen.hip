@@ -77,94 +77,285 @@ __device__ inline bool choice_better(const CoArgs& a, uint32_t u, const Choice& 
   return hash4(a.seed, kCoTieTag | a.tag, u, x.t) > hash4(a.seed, kCoTieTag | a.tag, u, y.t);
 }
 
-// Rating of one proposer per block.  The net set (distinct clusters of the
-// current net) lives in LDS; the rating map lives in LDS (MAP_BITS > 0) or in
-// per-block global scratch of 2^gbits slots.
-template <int BLOCK, int MAP_BITS>
+// ------------------------------------------------------------- rating --
+// A sub-round first marks the nets (fewer than `ignore` pins) of its
+// proposers, then lists the distinct clusters of every marked net once
+// (dclus[noff[e] ..], dcount[e]); score(e) follows from dcount.  A proposer
+// then sums score(e) into a map keyed by cluster for every distinct cluster
+// of its nets.  The fixed-point sums are exact and order-free, so any lane
+// order gives the same ratings.
+
+// The nets a proposer sweeps, in incidence order, stop before the first net
+// whose distinct clusters would push the accessed keys past `sampling`
+// (fillRatingMap's vertex degree sampling); visits counts 2|e| for every
+// swept net and for the net that stops the sweep.  When even all nets stay
+// within the threshold (ub + 2 <= sampling) the sweep takes every net.
+__device__ inline uint32_t co_sweep_end(const CoArgs& a, const uint32_t* __restrict__ dcount, uint32_t u,
+                                        int64_t& visits) {
+  const uint32_t jb = a.voff[u], je = a.voff[u + 1];
+  if (a.ub[u] + 2 <= a.sampling) {
+    visits += 2 * int64_t(a.ub[u]);
+    return je;
+  }
+  uint64_t acc = 0;
+  for (uint32_t j = jb; j < je; ++j) {
+    const uint32_t e = a.inc[j];
+    const uint32_t s = a.noff[e + 1] - a.noff[e];
+    if (s >= a.ignore) continue;
+    visits += 2 * int64_t(s);
+    const uint32_t d = dcount[e];
+    if (acc + max(d, 2u) > a.sampling) return j;
+    acc += d;
+  }
+  return je;
+}
+
+// nets of the proposers in `list`; the first marker of a net appends it to
+// mlist (each net once per sub-round); one wave per proposer
+__global__ void __launch_bounds__(256)
+co_mark_nets_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, uint32_t stamp,
+                    uint32_t* __restrict__ nmark, uint32_t* __restrict__ mlist, uint32_t* __restrict__ mcount) {
+  const uint32_t cnt = *count;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < cnt; i += waves) {
+    const uint32_t u = list[i];
+    for (uint32_t j = a.voff[u] + lane; j < a.voff[u + 1]; j += 64) {
+      const uint32_t e = a.inc[j];
+      if (a.noff[e + 1] - a.noff[e] >= a.ignore || nmark[e] == stamp) continue;
+      if (atomicExch(&nmark[e], stamp) != stamp) mlist[atomicAdd(mcount, 1u)] = e;
+    }
+  }
+}
+
+// distinct clusters of the marked nets with at most 16 pins: one net per
+// thread, pins in registers
+__global__ void __launch_bounds__(256)
+co_dedup_small_kernel(CoArgs a, const uint32_t* __restrict__ mlist, const uint32_t* __restrict__ mcount,
+                      uint32_t* __restrict__ dclus, uint32_t* __restrict__ dcount) {
+  constexpr int D = 16;
+  const uint32_t cnt = *mcount;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+    const uint32_t e = mlist[i];
+    const uint32_t pb = a.noff[e], s = a.noff[e + 1] - pb;
+    if (s > uint32_t(D)) continue;
+    uint32_t r[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) r[i] = uint32_t(i) < s ? a.pins[pb + i] : 0u;
+#pragma unroll
+    for (int i = 0; i < D; ++i) r[i] = uint32_t(i) < s ? a.cid[r[i]] : kNone;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      bool first = r[i] != kNone;
+#pragma unroll
+      for (int j = 0; j < i; ++j) first = first && r[j] != r[i];
+      if (first) dclus[pb + cnt++] = r[i];
+    }
+    dcount[e] = cnt;
+  }
+}
+
+// the larger marked nets (17 to ignore - 1 pins): one net per wave, an LDS
+// set of 2^ceil(log2(2|e|)) slots
+__global__ void __launch_bounds__(256)
+co_dedup_wave_kernel(CoArgs a, const uint32_t* __restrict__ mlist, const uint32_t* __restrict__ mcount,
+                     uint32_t* __restrict__ dclus, uint32_t* __restrict__ dcount) {
+  constexpr int NS = 2048;
+  __shared__ uint32_t sets[4 * NS];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t* set = sets + wid * NS;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  const uint32_t cnt = *mcount;
+  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + wid; i < cnt; i += waves) {
+    const uint32_t e = mlist[i];  // uniform over the wave
+    const uint32_t pb = a.noff[e], s = a.noff[e + 1] - pb;
+    if (s <= 16) continue;
+    int nbits = 1;
+    while ((1u << nbits) < 2 * s) ++nbits;
+    const uint32_t NT = 1u << nbits;
+    for (uint32_t t = lane; t < NT; t += 64) set[t] = kNone;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    for (uint32_t q = lane; q < s; q += 64) {
+      const uint32_t rep = a.cid[a.pins[pb + q]];
+      uint32_t h = co_slot(rep, nbits);
+      while (true) {
+        const uint32_t old = atomicCAS(&set[h], kNone, rep);
+        if (old == kNone || old == rep) break;
+        h = (h + 1) & (NT - 1);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    uint32_t base = 0;
+    for (uint32_t t0 = 0; t0 < NT; t0 += 64) {
+      const uint32_t t = t0 + lane;
+      const uint32_t key = t < NT ? set[t] : kNone;
+      const uint64_t bal = __ballot(key != kNone);
+      if (key != kNone) dclus[pb + base + __popcll(bal & ((1ull << lane) - 1))] = key;
+      base += __popcll(bal);
+    }
+    if (lane == 0) dcount[e] = base;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+__device__ inline void co_map_add(uint32_t* K, uint64_t* V, int bits, uint32_t rep, uint64_t score) {
+  const uint32_t T = 1u << bits;
+  uint32_t h = co_slot(rep, bits);
+  while (true) {
+    const uint32_t old = atomicCAS(&K[h], kNone, rep);
+    if (old == kNone || old == rep) break;
+    h = (h + 1) & (T - 1);
+  }
+  atomicAdd(reinterpret_cast<unsigned long long*>(&V[h]), static_cast<unsigned long long>(score));
+}
+
+// Sums score(e) of u's swept nets [jb, jend) into the map.  A group of G
+// lanes (G <= 64, inside one wave) takes the nets G at a time starting at
+// jb + first with stride `step`: an exclusive scan of the distinct counts
+// over the lanes lays the (net, cluster) items of the chunk out in a row,
+// and lane l inserts items l, l + G, ...; the owner net of an item is found
+// by a binary search over the lanes' running sums.
+template <int G>
+__device__ inline void co_fill_map(const CoArgs& a, const uint32_t* __restrict__ dclus,
+                                   const uint32_t* __restrict__ dcount, uint32_t jb, uint32_t jend, uint32_t first,
+                                   uint32_t step, uint32_t lane, uint32_t* K, uint64_t* V, int bits) {
+  for (uint32_t j0 = jb + first; j0 < jend; j0 += step) {
+    const uint32_t j = j0 + lane;
+    uint32_t d = 0, pb = 0;
+    uint64_t score = 0;
+    if (j < jend) {
+      const uint32_t e = a.inc[j];
+      pb = a.noff[e];
+      const uint32_t s = a.noff[e + 1] - pb;
+      if (s < a.ignore) {
+        d = dcount[e];
+        score = (uint64_t(uint32_t(a.nw[e])) << kFixShift) / uint64_t(max(d, 2u) - 1);
+      }
+    }
+    uint32_t incl = d;
+    for (int o = 1; o < G; o <<= 1) {
+      const uint32_t t = __shfl_up(incl, o, G);
+      if (int(lane) >= o) incl += t;
+    }
+    const uint32_t total = __shfl(incl, G - 1, G);
+    for (uint32_t i0 = 0; i0 < total; i0 += G) {
+      const uint32_t i = i0 + lane;
+      // owner: first lane whose running sum exceeds i
+      int lo = 0, hi = G - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const uint32_t x = __shfl(incl, mid, G);
+        if (x > i) hi = mid; else lo = mid + 1;
+      }
+      const uint32_t o_incl = __shfl(incl, lo, G);
+      const uint32_t o_d = __shfl(d, lo, G);
+      const uint32_t o_pb = __shfl(pb, lo, G);
+      const uint64_t o_score = __shfl(score, lo, G);
+      if (i < total) co_map_add(K, V, bits, dclus[o_pb + (i - (o_incl - o_d))], o_score);
+    }
+  }
+}
+
+// best target over map slots [t0, T) strided by `lanes`
+__device__ inline Choice co_scan_map(const CoArgs& a, uint32_t u, const uint32_t* K, const uint64_t* V, uint32_t T,
+                                     uint32_t lane, uint32_t lanes) {
+  const int64_t cwu = a.cw[u];
+  const int32_t cu = a.comm[u];
+  Choice best{0, 0, kNone};
+  for (uint32_t t = lane; t < T; t += lanes) {
+    const uint32_t rep = K[t];
+    if (rep == kNone || rep == u) continue;
+    if (cwu + a.cw[rep] > a.max_w || a.comm[rep] != cu) continue;
+    Choice c{V[t], a.prefer_unmatched ? (a.matched[rep] ? 0 : 1) : 0, rep};
+    if (choice_better(a, u, c, best)) best = c;
+  }
+  return best;
+}
+
+// G lanes per proposer, a 2^TBITS-slot map per group in LDS (proposers with
+// ub <= 2^(TBITS-1) distinct keys at most)
+template <int G, int TBITS, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-co_rate_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count, int gbits,
-               uint32_t* __restrict__ gkeys, uint64_t* __restrict__ gvals) {
-  constexpr int NS_BITS = 11;  // net set: 2048 slots (nets < 1000 pins)
-  constexpr int NS = 1 << NS_BITS;
-  __shared__ uint32_t nset[NS];
-  __shared__ uint32_t lkeys[MAP_BITS > 0 ? (1 << MAP_BITS) : 1];
-  __shared__ uint64_t lvals[MAP_BITS > 0 ? (1 << MAP_BITS) : 1];
-  __shared__ uint32_t s_cnt;
-  __shared__ uint64_t s_val[BLOCK / 64];
-  __shared__ int s_un[BLOCK / 64];
-  __shared__ uint32_t s_t[BLOCK / 64];
-  uint32_t* mkeys = MAP_BITS > 0 ? lkeys : gkeys + (uint64_t(blockIdx.x) << gbits);
-  uint64_t* mvals = MAP_BITS > 0 ? lvals : gvals + (uint64_t(blockIdx.x) << gbits);
+co_rate_group_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
+                     const uint32_t* __restrict__ dclus, const uint32_t* __restrict__ dcount) {
+  constexpr int T = 1 << TBITS;
+  constexpr int GPB = BLOCK / G;
+  __shared__ uint32_t keys[GPB * T];
+  __shared__ uint64_t vals[GPB * T];
+  __shared__ uint32_t s_end[GPB];
+  const int grp = threadIdx.x / G, lane = threadIdx.x % G;
+  uint32_t* K = keys + grp * T;
+  uint64_t* V = vals + grp * T;
+  const uint32_t count = *list_count;
+  int64_t visits = 0;
+  for (uint32_t base = blockIdx.x * GPB; base < count; base += gridDim.x * GPB) {
+    const uint32_t idx = base + grp;
+    const bool in = idx < count;
+    const uint32_t u = in ? list[idx] : 0;
+    int tb = 1;
+    const uint64_t ub = in ? a.ub[u] : 0;
+    while (tb < TBITS && (uint64_t(1) << tb) < 2 * ub) ++tb;
+    const uint32_t Tn = 1u << tb;
+    for (uint32_t t = lane; t < Tn; t += G) { K[t] = kNone; V[t] = 0; }
+    if (in && lane == 0) s_end[grp] = co_sweep_end(a, dcount, u, visits);
+    __syncthreads();
+    if (in) co_fill_map<G>(a, dclus, dcount, a.voff[u], s_end[grp], 0, G, lane, K, V, tb);
+    __syncthreads();
+    if (in) {
+      Choice best = co_scan_map(a, u, K, V, Tn, lane, G);
+      for (int o = G / 2; o > 0; o >>= 1) {
+        Choice other;
+        other.val = __shfl_xor(best.val, o, G);
+        other.un = __shfl_xor(best.un, o, G);
+        other.t = __shfl_xor(best.t, o, G);
+        if (choice_better(a, u, other, best)) best = other;
+      }
+      if (lane == 0) {
+        a.target[u] = best.t;
+        a.rating[u] = best.val;
+      }
+    }
+    __syncthreads();
+  }
+  for (int o = 32; o > 0; o >>= 1) visits += __shfl_xor(visits, o, 64);
+  if ((threadIdx.x & 63) == 0 && visits)
+    atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(visits));
+}
+
+// One proposer per block with an LDS map of 2^mbits <= 2^MAP_BITS slots
+// (mbits from ub).
+template <int MAP_BITS>
+__global__ void __launch_bounds__(256)
+co_rate_block_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
+                     const uint32_t* __restrict__ dclus, const uint32_t* __restrict__ dcount) {
+  __shared__ uint32_t s_end;
+  __shared__ uint64_t s_val[4];
+  __shared__ int s_un[4];
+  __shared__ uint32_t s_t[4];
+  __shared__ uint32_t K[1 << MAP_BITS];
+  __shared__ uint64_t V[1 << MAP_BITS];
+  const int gbits = MAP_BITS;
   const uint32_t count = *list_count;
   int64_t visits = 0;
   for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
     const uint32_t u = list[item];
-    // map of at least twice the number of distinct keys the sweep can insert
-    int mbits = MAP_BITS;
-    if (MAP_BITS == 0) {
-      const uint64_t keys = min(a.ub[u], a.sampling);
-      mbits = 1;
-      while ((uint64_t(1) << mbits) < 2 * keys) ++mbits;
-      mbits = min(mbits, gbits);
-    }
+    const uint64_t keys = min(a.ub[u], a.sampling);
+    int mbits = 1;
+    while ((uint64_t(1) << mbits) < 2 * keys) ++mbits;
+    mbits = min(mbits, gbits);
     const uint32_t MT = 1u << mbits;
-    for (uint32_t t = threadIdx.x; t < MT; t += BLOCK) { mkeys[t] = kNone; mvals[t] = 0; }
+    for (uint32_t t = threadIdx.x; t < MT; t += blockDim.x) { K[t] = kNone; V[t] = 0; }
+    if (threadIdx.x == 0) s_end = co_sweep_end(a, dcount, u, visits);
+    __threadfence_block();
     __syncthreads();
-    uint64_t accesses = 0;
-    for (uint32_t j = a.voff[u]; j < a.voff[u + 1]; ++j) {  // uniform over the block
-      const uint32_t e = a.inc[j];
-      const uint32_t pb = a.noff[e], s = a.noff[e + 1] - pb;
-      if (s >= a.ignore) continue;
-      int nbits = 1;
-      while ((1u << nbits) < 2 * s) ++nbits;
-      const uint32_t NT = 1u << nbits;
-      for (uint32_t t = threadIdx.x; t < NT; t += BLOCK) nset[t] = kNone;
-      if (threadIdx.x == 0) s_cnt = 0;
-      __syncthreads();
-      uint32_t fresh = 0;
-      for (uint32_t q = threadIdx.x; q < s; q += BLOCK) {
-        const uint32_t rep = a.cid[a.pins[pb + q]];
-        uint32_t h = co_slot(rep, nbits);
-        while (true) {
-          const uint32_t old = atomicCAS(&nset[h], kNone, rep);
-          if (old == kNone) { ++fresh; break; }
-          if (old == rep) break;
-          h = (h + 1) & (NT - 1);
-        }
-      }
-      if (fresh) atomicAdd(&s_cnt, fresh);
-      visits += threadIdx.x == 0 ? 2 * int64_t(s) : 0;
-      __syncthreads();
-      const uint32_t distinct = s_cnt;
-      const uint32_t sz = max(distinct, 2u);
-      if (accesses + sz > a.sampling) break;  // uniform: s_cnt is block-wide
-      const uint64_t score = (uint64_t(uint32_t(a.nw[e])) << kFixShift) / uint64_t(sz - 1);
-      for (uint32_t t = threadIdx.x; t < NT; t += BLOCK) {
-        const uint32_t rep = nset[t];
-        if (rep == kNone) continue;
-        uint32_t h = co_slot(rep, mbits);
-        while (true) {
-          const uint32_t old = atomicCAS(&mkeys[h], kNone, rep);
-          if (old == kNone || old == rep) break;
-          h = (h + 1) & (MT - 1);
-        }
-        atomicAdd(reinterpret_cast<unsigned long long*>(&mvals[h]), static_cast<unsigned long long>(score));
-      }
-      accesses += distinct;
-      if (MAP_BITS == 0) __threadfence_block();
-      __syncthreads();
-    }
-    // target choice over the map
-    const int64_t cwu = a.cw[u];
-    const int32_t cu = a.comm[u];
-    Choice best{0, 0, kNone};
-    for (uint32_t t = threadIdx.x; t < MT; t += BLOCK) {
-      const uint32_t rep = mkeys[t];
-      if (rep == kNone || rep == u) continue;
-      if (cwu + a.cw[rep] > a.max_w || a.comm[rep] != cu) continue;
-      Choice c{mvals[t], a.prefer_unmatched ? (a.matched[rep] ? 0 : 1) : 0, rep};
-      if (choice_better(a, u, c, best)) best = c;
-    }
+    co_fill_map<64>(a, dclus, dcount, a.voff[u], s_end, (threadIdx.x >> 6) * 64, blockDim.x, threadIdx.x & 63, K, V,
+                    mbits);
+    __threadfence_block();
+    __syncthreads();
+    Choice best = co_scan_map(a, u, K, V, MT, threadIdx.x, blockDim.x);
     for (int o = 32; o > 0; o >>= 1) {
       Choice other;
       other.val = __shfl_xor(best.val, o, 64);
@@ -179,7 +370,7 @@ co_rate_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t* __re
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      for (int w = 1; w < BLOCK / 64; ++w) {
+      for (int w = 1; w < int(blockDim.x >> 6); ++w) {
         Choice other{s_val[w], s_un[w], s_t[w]};
         if (choice_better(a, u, other, best)) best = other;
       }
@@ -188,7 +379,202 @@ co_rate_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t* __re
     }
     __syncthreads();
   }
-  if (visits) atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(visits));
+  if (threadIdx.x == 0 && visits)
+    atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(visits));
+}
+
+// ------------------------------------------------- heavy proposers --
+// Proposers with ub > 4096: their rating items are spread over the whole
+// grid.  Each owns a table of T = pow2 >= max(kHrChunk, 2 L) slots (L = its
+// distinct (net, cluster) items) in one arena; tables are multiples of
+// kHrChunk, so a chunk of the arena belongs to one proposer.
+//   hr_prep     per proposer: sweep end, visits, L, T, swept incidences
+//   (scans)     item, table and incidence offsets
+//   hr_inc      per swept incidence: net and item count
+//   (scan)      item offset of every incidence
+//   hr_clear    empty tables
+//   hr_insert   per item: score(e) into the owner's table
+//   hr_scan     best target per chunk
+//   hr_decide   per proposer: best over its chunks
+constexpr uint32_t kHrChunk = 1024;
+
+struct HrArgs {
+  const uint32_t* list;  // heavy proposers of the sub-round
+  uint32_t count;
+  uint64_t* tsize;       // T per proposer
+  uint64_t* isize;       // swept incidences per proposer
+  uint64_t* ioff;        // exclusive scan of isize (count + 1)
+  uint64_t* toff;        // exclusive scan of tsize (count + 1)
+  uint32_t* inc_net;     // per swept incidence: net (kNone: ignored net)
+  uint64_t* inc_cnt;     // per swept incidence: items
+  uint64_t* inc_off;     // exclusive scan of inc_cnt (incidences + 1)
+  uint32_t* keys;
+  uint64_t* vals;
+};
+
+// co_sweep_end by a whole wave: 64 nets at a time, a prefix scan of their
+// distinct counts, the first net that stops the sweep by ballot
+__device__ inline uint32_t co_sweep_end_wave(const CoArgs& a, const uint32_t* __restrict__ dcount, uint32_t u,
+                                             uint32_t lane, int64_t& visits) {
+  const uint32_t jb = a.voff[u], je = a.voff[u + 1];
+  if (a.ub[u] + 2 <= a.sampling) {
+    if (lane == 0) visits += 2 * int64_t(a.ub[u]);
+    return je;
+  }
+  uint64_t acc = 0;  // distinct keys of the nets before the chunk
+  for (uint32_t j0 = jb; j0 < je; j0 += 64) {
+    const uint32_t j = j0 + lane;
+    uint32_t s = 0, d = 0;
+    bool take = false;
+    if (j < je) {
+      const uint32_t e = a.inc[j];
+      s = a.noff[e + 1] - a.noff[e];
+      take = s < a.ignore;
+      if (take) d = dcount[e];
+    }
+    uint64_t incl = d;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint64_t t = __shfl_up(incl, o, 64);
+      if (int(lane) >= o) incl += t;
+    }
+    const bool stop = take && acc + (incl - d) + max(d, 2u) > a.sampling;
+    const uint64_t bal = __ballot(stop);
+    const int f = bal ? __ffsll(static_cast<long long>(bal)) - 1 : 64;
+    int64_t v = (take && int(lane) <= f) ? 2 * int64_t(s) : 0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) visits += v;
+    if (bal) return j0 + uint32_t(f);
+    acc += __shfl(incl, 63, 64);
+  }
+  return je;
+}
+
+__global__ void __launch_bounds__(256)
+hr_prep_kernel(CoArgs a, HrArgs h, const uint32_t* __restrict__ dcount) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  int64_t visits = 0;
+  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < h.count; i += waves) {
+    const uint32_t u = h.list[i];
+    const uint32_t jend = co_sweep_end_wave(a, dcount, u, lane, visits);
+    const uint32_t jb = a.voff[u];
+    uint64_t L = 0;
+    for (uint32_t j = jb + lane; j < jend; j += 64) {
+      const uint32_t e = a.inc[j];
+      if (a.noff[e + 1] - a.noff[e] < a.ignore) L += dcount[e];
+    }
+    for (int o = 32; o > 0; o >>= 1) L += __shfl_xor(L, o, 64);
+    if (lane == 0) {
+      uint64_t T = kHrChunk;
+      while (T < 2 * L) T <<= 1;
+      h.tsize[i] = T;
+      h.isize[i] = jend - jb;
+    }
+  }
+  if (lane == 0 && visits)
+    atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(visits));
+}
+
+// largest i with off[i] <= x
+__device__ inline uint32_t hr_owner(const uint64_t* off, uint32_t count, uint64_t x) {
+  uint32_t lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (off[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void hr_inc_kernel(CoArgs a, HrArgs h, uint64_t total, const uint32_t* __restrict__ dcount) {
+  for (uint64_t x = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; x < total;
+       x += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t i = hr_owner(h.ioff, h.count, x);
+    const uint32_t j = a.voff[h.list[i]] + uint32_t(x - h.ioff[i]);
+    const uint32_t e = a.inc[j];
+    const bool take = a.noff[e + 1] - a.noff[e] < a.ignore;
+    h.inc_net[x] = take ? e : kNone;
+    h.inc_cnt[x] = take ? dcount[e] : 0;
+  }
+}
+
+__global__ void hr_clear_kernel(HrArgs h, uint64_t slots) {
+  for (uint64_t x = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; x < slots;
+       x += uint64_t(gridDim.x) * blockDim.x) {
+    h.keys[x] = kNone;
+    h.vals[x] = 0;
+  }
+}
+
+// item t belongs to the largest incidence x with inc_off[x] <= t, which owns
+// items: a later incidence without items would start at the same offset
+__global__ void hr_insert_kernel(CoArgs a, HrArgs h, uint64_t total_items, uint64_t total_inc,
+                                 const uint32_t* __restrict__ dclus, const uint32_t* __restrict__ dcount) {
+  for (uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; t < total_items;
+       t += uint64_t(gridDim.x) * blockDim.x) {
+    uint64_t lo = 0, hi = total_inc;
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (h.inc_off[mid] <= t) lo = mid; else hi = mid;
+    }
+    const uint32_t e = h.inc_net[lo];
+    const uint32_t q = uint32_t(t - h.inc_off[lo]);
+    const uint32_t i = hr_owner(h.ioff, h.count, lo);
+    const uint32_t d = dcount[e];
+    const uint64_t score = (uint64_t(uint32_t(a.nw[e])) << kFixShift) / uint64_t(max(d, 2u) - 1);
+    const uint64_t base = h.toff[i];
+    const int bits = __builtin_ctzll(h.toff[i + 1] - base);
+    co_map_add(h.keys + base, h.vals + base, bits, dclus[a.noff[e] + q], score);
+  }
+}
+
+__global__ void __launch_bounds__(256) hr_scan_kernel(CoArgs a, HrArgs h, uint64_t chunks, uint64_t* __restrict__ cval,
+                                                      int* __restrict__ cun, uint32_t* __restrict__ ct) {
+  __shared__ uint64_t s_val[4];
+  __shared__ int s_un[4];
+  __shared__ uint32_t s_t[4];
+  for (uint64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+    const uint64_t s0 = ch * kHrChunk;
+    const uint32_t i = hr_owner(h.toff, h.count, s0);
+    const uint32_t u = h.list[i];
+    Choice best = co_scan_map(a, u, h.keys + s0, h.vals + s0, kHrChunk, threadIdx.x, blockDim.x);
+    for (int o = 32; o > 0; o >>= 1) {
+      Choice other;
+      other.val = __shfl_xor(best.val, o, 64);
+      other.un = __shfl_xor(best.un, o, 64);
+      other.t = __shfl_xor(best.t, o, 64);
+      if (choice_better(a, u, other, best)) best = other;
+    }
+    if ((threadIdx.x & 63) == 0) {
+      s_val[threadIdx.x >> 6] = best.val;
+      s_un[threadIdx.x >> 6] = best.un;
+      s_t[threadIdx.x >> 6] = best.t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < int(blockDim.x >> 6); ++w) {
+        Choice other{s_val[w], s_un[w], s_t[w]};
+        if (choice_better(a, u, other, best)) best = other;
+      }
+      cval[ch] = best.val;
+      cun[ch] = best.un;
+      ct[ch] = best.t;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void hr_decide_kernel(CoArgs a, HrArgs h, const uint64_t* __restrict__ cval, const int* __restrict__ cun,
+                                 const uint32_t* __restrict__ ct) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < h.count; i += gridDim.x * blockDim.x) {
+    const uint32_t u = h.list[i];
+    Choice best{0, 0, kNone};
+    for (uint64_t ch = h.toff[i] / kHrChunk; ch < h.toff[i + 1] / kHrChunk; ++ch) {
+      Choice c{cval[ch], cun[ch], ct[ch]};
+      if (choice_better(a, u, c, best)) best = c;
+    }
+    a.target[u] = best.t;
+    a.rating[u] = best.val;
+  }
 }
 
 // rating upper bound: sum of the sizes of the nets the rating sweeps
@@ -210,30 +596,41 @@ __global__ void co_ub_kernel(const uint32_t* __restrict__ voff, const uint32_t* 
 __global__ void __launch_bounds__(256)
 co_select_kernel(uint32_t n, const uint8_t* __restrict__ matched, const uint64_t* __restrict__ ub, uint32_t salt,
                  uint32_t buckets, uint32_t bucket, uint32_t* __restrict__ target, uint32_t* __restrict__ list_a,
-                 uint32_t* __restrict__ list_b, uint32_t* __restrict__ counts) {
-  __shared__ uint32_t ca, cb, ba, bb;
-  if (threadIdx.x == 0) { ca = 0; cb = 0; }
+                 uint32_t* __restrict__ list_b, uint32_t* __restrict__ list_c, uint32_t* __restrict__ list_d,
+                 uint32_t* __restrict__ counts) {
+  // tiers: c: ub <= 64 (counts[6]), a: ub <= 1024 (counts[0]), d: ub <= 4096
+  // (counts[2]), b: heavier (counts[1])
+  __shared__ uint32_t ca, cb, cc, cd, ba, bb, bc, bd;
+  if (threadIdx.x == 0) { ca = 0; cb = 0; cc = 0; cd = 0; }
   __syncthreads();
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-  bool pa = false, pb = false;
-  uint32_t la = 0, lb = 0;
+  int tier = -1;
+  uint32_t l = 0;
   if (u < n) {
     target[u] = kNone;
     if (!matched[u] && (buckets <= 1 || bucket_of(salt, u, buckets) == bucket)) {
-      if (ub[u] <= 1024) { pa = true; la = atomicAdd(&ca, 1u); } else { pb = true; lb = atomicAdd(&cb, 1u); }
+      const uint64_t b = ub[u];
+      if (b <= 64) { tier = 2; l = atomicAdd(&cc, 1u); }
+      else if (b <= 1024) { tier = 0; l = atomicAdd(&ca, 1u); }
+      else if (b <= 4096) { tier = 3; l = atomicAdd(&cd, 1u); }
+      else { tier = 1; l = atomicAdd(&cb, 1u); }
     }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     ba = ca ? atomicAdd(&counts[0], ca) : 0;
     bb = cb ? atomicAdd(&counts[1], cb) : 0;
+    bc = cc ? atomicAdd(&counts[6], cc) : 0;
+    bd = cd ? atomicAdd(&counts[2], cd) : 0;
   }
   __syncthreads();
-  if (pa) list_a[ba + la] = u;
-  if (pb) list_b[bb + lb] = u;
+  if (tier == 0) list_a[ba + l] = u;
+  if (tier == 1) list_b[bb + l] = u;
+  if (tier == 2) list_c[bc + l] = u;
+  if (tier == 3) list_d[bd + l] = u;
 }
 
-// joiners and pairs (counts: 2 = joiners, 3 = pairs, 4 = overflowing targets)
+// joiners and pairs (counts[4]: overflowing targets)
 __global__ void co_resolve_kernel(uint32_t n, const uint32_t* __restrict__ target, const int32_t* __restrict__ vw,
                                   int64_t* __restrict__ jw, uint8_t* __restrict__ role) {
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
@@ -438,6 +835,81 @@ __global__ void co_compact_keys_kernel(const uint64_t* __restrict__ keys, const 
 // ------------------------------------------------------------ host side --
 namespace mkp {
 
+// work arrays of the heavy-proposer rating, kept across sub-rounds
+struct HeavyRating {
+  DevBuf<uint64_t> tsize, isize, ioff, toff, inc_cnt, inc_off, cval;
+  DevBuf<uint32_t> inc_net, keys, ct;
+  DevBuf<uint64_t> vals;
+  DevBuf<int> cun;
+};
+
+// rating of the heavy proposers (list, *count on the device)
+void rate_heavy(Context& c, const CoArgs& a, const uint32_t* list, const uint32_t* count_dev, const uint32_t* dclus,
+                const uint32_t* dcount, HeavyRating& w) {
+  const hipStream_t s = c.stream;
+  uint32_t* pin = reinterpret_cast<uint32_t*>(c.pinned);
+  MKP_HIP(hipMemcpyAsync(pin, count_dev, 4, hipMemcpyDeviceToHost, s));
+  c.sync();
+  const uint32_t hb = *pin;
+  if (!hb) return;
+  w.tsize.ensure(hb);
+  w.isize.ensure(hb);
+  w.ioff.ensure(size_t(hb) + 1);
+  w.toff.ensure(size_t(hb) + 1);
+  HrArgs h{};
+  h.list = list; h.count = hb; h.tsize = w.tsize.p; h.isize = w.isize.p; h.ioff = w.ioff.p; h.toff = w.toff.p;
+  {
+    TimedRegion tr(c.prof, s, kFamRating, 0.0, "hr_prep_kernel");
+    hr_prep_kernel<<<(hb + 3) / 4, 256, 0, s>>>(a, h, dcount);
+    MKP_LAUNCH_CHECK();
+  }
+  exclusive_scan_u64(c, w.isize.p, w.ioff.p, hb);
+  exclusive_scan_u64(c, w.tsize.p, w.toff.p, hb);
+  uint64_t* pin64 = reinterpret_cast<uint64_t*>(c.pinned);
+  MKP_HIP(hipMemcpyAsync(pin64, w.ioff.p + hb, 8, hipMemcpyDeviceToHost, s));
+  MKP_HIP(hipMemcpyAsync(pin64 + 1, w.toff.p + hb, 8, hipMemcpyDeviceToHost, s));
+  c.sync();
+  const uint64_t total_inc = pin64[0], slots = pin64[1];
+  MKP_CHECK(slots < (uint64_t(1) << 32), "heavy rating tables exceed 2^32 slots");
+  w.inc_net.ensure(std::max<uint64_t>(total_inc, 1));
+  w.inc_cnt.ensure(std::max<uint64_t>(total_inc, 1));
+  w.inc_off.ensure(total_inc + 1);
+  h.inc_net = w.inc_net.p; h.inc_cnt = w.inc_cnt.p; h.inc_off = w.inc_off.p;
+  if (total_inc) {
+    TimedRegion tr(c.prof, s, kFamRating, 0.0, "hr_inc_kernel");
+    hr_inc_kernel<<<grid_for(total_inc, 256, 65536), 256, 0, s>>>(a, h, total_inc, dcount);
+    MKP_LAUNCH_CHECK();
+  }
+  exclusive_scan_u64(c, w.inc_cnt.p, w.inc_off.p, total_inc);
+  MKP_HIP(hipMemcpyAsync(pin64, w.inc_off.p + total_inc, 8, hipMemcpyDeviceToHost, s));
+  c.sync();
+  const uint64_t total_items = pin64[0];
+  w.keys.ensure(slots);
+  w.vals.ensure(slots);
+  h.keys = w.keys.p; h.vals = w.vals.p;
+  {
+    TimedRegion tr(c.prof, s, kFamRating, 0.0, "hr_clear_kernel");
+    hr_clear_kernel<<<grid_for(slots, 256, 65536), 256, 0, s>>>(h, slots);
+  }
+  if (total_items) {
+    TimedRegion tr(c.prof, s, kFamRating, 0.0, "hr_insert_kernel");
+    hr_insert_kernel<<<grid_for(total_items, 256, 65536), 256, 0, s>>>(a, h, total_items, total_inc, dclus, dcount);
+  }
+  MKP_LAUNCH_CHECK();
+  const uint64_t chunks = slots / kHrChunk;
+  w.cval.ensure(chunks);
+  w.cun.ensure(chunks);
+  w.ct.ensure(chunks);
+  {
+    TimedRegion tr(c.prof, s, kFamRating, 0.0, "hr_scan_kernel");
+    hr_scan_kernel<<<unsigned(std::min<uint64_t>(chunks, uint64_t(c.num_cus) * 16)), 256, 0, s>>>(a, h, chunks,
+                                                                                                   w.cval.p, w.cun.p,
+                                                                                                   w.ct.p);
+  }
+  hr_decide_kernel<<<grid_for(hb, 256, 4096), 256, 0, s>>>(a, h, w.cval.p, w.cun.p, w.ct.p);
+  MKP_LAUNCH_CHECK();
+}
+
 // One clustering pass on hg; cid (device, n) receives the representatives.
 ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, int pass, uint32_t* cid_out) {
   Context& c = hg->ctx->c;
@@ -450,7 +922,12 @@ ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, in
   if (n == 0) return res;
   DevBuf<int64_t> cw(n), jw(n), visits(1);
   DevBuf<uint8_t> matched(n), role(n);
-  DevBuf<uint32_t> target(n), list_a(n), list_b(n), counts(8), olist(n);
+  DevBuf<uint32_t> target(n), list_a(n), list_b(n), list_c(n), list_d(n), counts(8), olist(n);
+  // per sub-round distinct clusters of the proposers' nets
+  const uint32_t m = hg->m;
+  DevBuf<uint32_t> nmark(std::max<uint32_t>(m, 1)), mlist(std::max<uint32_t>(m, 1)), dcount(std::max<uint32_t>(m, 1)),
+      dclus(std::max<uint64_t>(hg->p, 1));
+  fill_async<uint32_t>(nmark.p, 0u, std::max<uint32_t>(m, 1), s);
   DevBuf<uint64_t> rating(n), ub(n);
   co_init_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, hg->vw.p, cid_out, cw.p, matched.p);
   co_ub_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(hg->voff.p, hg->inc.p, hg->noff.p, n, prm.ignore_net_size,
@@ -465,16 +942,7 @@ ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, in
     c.sync();
     for (uint64_t x : h) max_ub = std::max(max_ub, x);
   }
-  const uint64_t cap = std::min<uint64_t>(max_ub, prm.sampling + 1000);
-  int gbits = 1;
-  while ((uint64_t(1) << gbits) < 2 * cap) ++gbits;
-  const uint32_t gblocks = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(256, (uint64_t(1) << 30) / (12ull << gbits))));
-  DevBuf<uint32_t> gkeys;
-  DevBuf<uint64_t> gvals;
-  if (max_ub > 1024) {
-    gkeys.alloc(uint64_t(gblocks) << gbits);
-    gvals.alloc(uint64_t(gblocks) << gbits);
-  }
+  HeavyRating hw;
   const uint32_t limit = std::max<uint32_t>(uint32_t(double(n) / prm.max_shrink_factor), prm.contraction_limit);
   uint32_t current = n;
   const int S = std::max(1, prm.sub_rounds);
@@ -489,16 +957,44 @@ ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, in
     a.tag = (uint64_t(pass) << 8) | uint64_t(sr);
     fill_async<uint32_t>(counts.p, 0u, 8, s);
     co_select_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, matched.p, ub.p, salt, uint32_t(S), uint32_t(sr), target.p,
-                                                     list_a.p, list_b.p, counts.p);
+                                                     list_a.p, list_b.p, list_c.p, list_d.p, counts.p);
     MKP_LAUNCH_CHECK();
     {
-      TimedRegion tr(c.prof, s, kFamRating, 0.0);
-      co_rate_kernel<64, 11><<<unsigned(c.num_cus) * 16, 64, 0, s>>>(a, list_a.p, counts.p + 0, 0, nullptr, nullptr);
+      const uint32_t stamp = uint32_t(sr) + 1;  // nmark is cleared once per pass
+      const unsigned mg = unsigned(c.num_cus) * 8;
+      {
+        TimedRegion tr(c.prof, s, kFamRating, 0.0, "co_mark_nets_kernel");
+        co_mark_nets_kernel<<<mg, 256, 0, s>>>(a, list_c.p, counts.p + 6, stamp, nmark.p, mlist.p, counts.p + 7);
+        co_mark_nets_kernel<<<mg, 256, 0, s>>>(a, list_a.p, counts.p + 0, stamp, nmark.p, mlist.p, counts.p + 7);
+        co_mark_nets_kernel<<<mg, 256, 0, s>>>(a, list_b.p, counts.p + 1, stamp, nmark.p, mlist.p, counts.p + 7);
+        co_mark_nets_kernel<<<mg, 256, 0, s>>>(a, list_d.p, counts.p + 2, stamp, nmark.p, mlist.p, counts.p + 7);
+      }
+      {
+        TimedRegion tr(c.prof, s, kFamRating, 0.0, "co_dedup_small_kernel");
+        co_dedup_small_kernel<<<unsigned(c.num_cus) * 16, 256, 0, s>>>(a, mlist.p, counts.p + 7, dclus.p, dcount.p);
+      }
+      {
+        TimedRegion tr(c.prof, s, kFamRating, 0.0, "co_dedup_wave_kernel");
+        co_dedup_wave_kernel<<<unsigned(c.num_cus) * 8, 256, 0, s>>>(a, mlist.p, counts.p + 7, dclus.p, dcount.p);
+      }
+      MKP_LAUNCH_CHECK();
+      {
+        TimedRegion tr(c.prof, s, kFamRating, 0.0, "co_rate_group_kernel<16, 7, 256>");
+        co_rate_group_kernel<16, 7, 256><<<unsigned(c.num_cus) * 16, 256, 0, s>>>(a, list_c.p, counts.p + 6, dclus.p,
+                                                                                 dcount.p);
+      }
+      {
+        TimedRegion tr(c.prof, s, kFamRating, 0.0, "co_rate_group_kernel<64, 11, 64>");
+        co_rate_group_kernel<64, 11, 64><<<unsigned(c.num_cus) * 16, 64, 0, s>>>(a, list_a.p, counts.p + 0, dclus.p,
+                                                                                dcount.p);
+      }
       MKP_LAUNCH_CHECK();
       if (max_ub > 1024) {
-        co_rate_kernel<256, 0><<<gblocks, 256, 0, s>>>(a, list_b.p, counts.p + 1, gbits, gkeys.p, gvals.p);
+        TimedRegion tr(c.prof, s, kFamRating, 0.0, "co_rate_block_kernel<13>");
+        co_rate_block_kernel<13><<<unsigned(c.num_cus), 256, 0, s>>>(a, list_d.p, counts.p + 2, dclus.p, dcount.p);
         MKP_LAUNCH_CHECK();
       }
+      if (max_ub > 4096) rate_heavy(c, a, list_b.p, counts.p + 1, dclus.p, dcount.p, hw);
     }
     fill_async<int64_t>(jw.p, 0, n, s);
     co_resolve_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, target.p, hg->vw.p, jw.p, role.p);

@@ -280,7 +280,7 @@ def batch_main(args):
     roofline = roofline_of(ctx, prof, args.workload)
 
     # plain path: one hypergraph per call, same members, seeds and settings
-    sample = list(range(0, G, w["loop_stride"]))
+    sample = [] if args.no_loop else list(range(0, G, w["loop_stride"]))
     dev = []
     for g in sample:
         n, k = ns[g], ks[g]
@@ -300,14 +300,17 @@ def batch_main(args):
             u += lst["arcs_swept"] // 2 + st["visits"]
         return u
 
-    loop_step()
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    loop_units = loop_step()
-    ctx.synchronize()
-    loop_wall = time.perf_counter() - t1
-    same = all(int(d[2].km1()) == int(km1_end[g]) for d, g in zip(dev, sample))
-    log("per-hypergraph loop over %d members: %.3f s, km1 equal to the batch: %s" % (len(sample), loop_wall, same))
+    loop_wall = loop_units = same = None
+    if sample:
+        loop_step()
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        loop_units = loop_step()
+        ctx.synchronize()
+        loop_wall = time.perf_counter() - t1
+        same = all(int(d[2].km1()) == int(km1_end[g]) for d, g in zip(dev, sample))
+        log("per-hypergraph loop over %d members: %.3f s, km1 equal to the batch: %s" % (len(sample), loop_wall,
+                                                                                         same))
     for d in dev:
         d[2].close()
         d[1].close()
@@ -325,7 +328,7 @@ def batch_main(args):
                           "OpenMP)" % (w["cpu_stride"], len(csample), cs))
 
     hg_per_s = G * args.steps / wall
-    loop_hg_per_s = len(sample) / loop_wall
+    loop_hg_per_s = len(sample) / loop_wall if sample else None
     out = {
         "metric": "LP+Louvain pins/s",
         "value": round(units / wall, 1),
@@ -344,9 +347,9 @@ def batch_main(args):
                    "epsilon": w["eps"], "km1_start_sum": int(km1_start.sum()), "km1_end_sum": int(km1_end.sum()),
                    "parallelism": "single GPU"},
         "batched": {"hypergraphs_per_s": round(hg_per_s, 1),
-                    "loop_hypergraphs_per_s": round(loop_hg_per_s, 1),
-                    "loop_pins_per_s": round(loop_units / loop_wall, 1),
-                    "speedup_vs_loop": round(hg_per_s / loop_hg_per_s, 2),
+                    "loop_hypergraphs_per_s": round(loop_hg_per_s, 1) if sample else None,
+                    "loop_pins_per_s": round(loop_units / loop_wall, 1) if sample else None,
+                    "speedup_vs_loop": round(hg_per_s / loop_hg_per_s, 2) if sample else None,
                     "loop_sample": "every %d-th member (%d members)" % (w["loop_stride"], len(sample)),
                     "km1_equal_to_loop": same},
         "roofline": roofline,
@@ -507,6 +510,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="powerlaw_1M_2M_k8", choices=sorted(WORKLOADS) + sorted(BATCH_WORKLOADS) + sorted(SERVE_WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-loop", action="store_true",
+                    help="batched workloads: skip the per-hypergraph comparison leg (for per-kernel profiles)")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarise two rocprofv3 --pmc output folders of this script into profiles/ and exit")
     args = ap.parse_args()

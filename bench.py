@@ -161,16 +161,24 @@ def pmc_summary(fetch_dir, write_dir, workload):
     print(path)
 
 
-def roofline_of(ctx, prof, workload):
-    """Roofline entry of the dominant kernel: the single kernel (template
-    instantiation) with the most device time among those with an algorithmic
-    byte count; every launch of it was timed on its own with an event pair on
-    the launch stream."""
+def dominant_kernel(ctx):
+    """The single kernel (template instantiation) with the most device time
+    among those with an algorithmic byte count, from the per-launch event
+    timings of every kernel (logged as a table)."""
     kern = {k: v for k, v in ctx.kernel_profile().items() if v["launches"] and v["bytes"] > 0}
     for name, v in sorted(kern.items(), key=lambda kv: -kv[1]["ms"]):
         log("  kernel %-52s %8.3f ms %5d launches %8.1f us/launch %9.1f GB/s" % (
             name, v["ms"], v["launches"], 1e3 * v["ms"] / v["launches"], v["bytes"] / (v["ms"] * 1e6)))
-    dom = max(kern, key=lambda k: kern[k]["ms"])
+    return max(kern, key=lambda k: kern[k]["ms"])
+
+
+def roofline_of(ctx, prof, workload, dom=None):
+    """Roofline entry of the dominant kernel; every launch of it was timed on
+    its own with an event pair on the launch stream.  With `dom` given, the
+    profile holds that kernel's launches of the timed steps only."""
+    kern = {k: v for k, v in ctx.kernel_profile().items() if v["launches"] and v["bytes"] > 0}
+    if dom is None:
+        dom = dominant_kernel(ctx)
     d = kern[dom]
     avg_ms = d["ms"] / d["launches"]
     bytes_per_launch = d["bytes"] / d["launches"]
@@ -181,7 +189,8 @@ def roofline_of(ctx, prof, workload):
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 avg_launch_us=round(avg_ms * 1e3, 2), launches=d["launches"],
                 algorithmic_bytes_per_launch=round(bytes_per_launch),
-                share_of_step_device_time=round(d["ms"] / sum(v["ms"] for v in prof.values()), 3),
+                share_of_step_device_time=(round(d["ms"] / sum(v["ms"] for v in prof.values()), 3)
+                                           if prof else None),
                 measured_copy_GBs=round(2 * COPY_BYTES / (copy_ms * 1e-3) / 1e9, 1))
 
 
@@ -256,28 +265,39 @@ def batch_main(args):
         st = work.lp_refine(seeds, epsilon=w["eps"])
         return sum(x["arcs_swept"] for x in lst) // 2 + sum(x["visits"] for x in st)
 
+    ctx.set_profiling(True)
+    ctx.reset_profile()
     for i in range(args.warmup):
         step()
         log("warmup step %d done" % i)
     ctx.synchronize()
-    ctx.set_profiling(True)
-    ctx.reset_profile()
+    dom = share = None
+    if args.warmup:  # as in the single-hypergraph bench: only the dominant kernel is timed below
+        prof = ctx.profile()
+        for f, v in prof.items():
+            if v["launches"]:
+                log("  %-18s %9.3f ms %6d launches %10.3f GB/s (warm-up)" % (
+                    f, v["ms"], v["launches"], v["bytes"] / (v["ms"] * 1e6) if v["ms"] else 0))
+        dom = dominant_kernel(ctx)
+        share = ctx.kernel_profile()[dom]["ms"] / sum(v["ms"] for v in prof.values())
+        ctx.reset_profile()
+        ctx.set_profile_filter(dom)
     units = 0
     t0 = time.perf_counter()
     for i in range(args.steps):
         units += step()
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    prof = ctx.profile()
-    ctx.set_profiling(False)
+    ctx.set_profile_filter(None)
     km1_end = work.km1()
     log("timed %d steps in %.3f s; km1 (sum over members) %d -> %d" % (args.steps, wall, km1_start.sum(),
                                                                       km1_end.sum()))
-    for f, v in prof.items():
-        if v["launches"]:
-            log("  %-18s %9.3f ms %6d launches %10.3f GB/s" % (f, v["ms"], v["launches"],
-                                                              v["bytes"] / (v["ms"] * 1e6) if v["ms"] else 0))
-    roofline = roofline_of(ctx, prof, args.workload)
+    if dom is None:
+        roofline = roofline_of(ctx, ctx.profile(), args.workload)
+    else:
+        roofline = roofline_of(ctx, None, args.workload, dom)
+        roofline["share_of_step_device_time"] = round(share, 3)
+    ctx.set_profiling(False)
 
     # plain path: one hypergraph per call, same members, seeds and settings
     sample = [] if args.no_loop else list(range(0, G, w["loop_stride"]))
@@ -552,12 +572,27 @@ def main():
         # Louvain work in pins: two arcs of the star expansion per pin
         return lst["arcs_swept"] // 2 + st["visits"], (lst, st)
 
+    # warm-up steps with every kernel timed: the per-family table and the
+    # dominant kernel; the timed steps then carry event pairs around the
+    # launches of the dominant kernel only
+    # (without warm-up steps every kernel is timed inside the timed steps)
+    ctx.set_profiling(True)
+    ctx.reset_profile()
     for i in range(args.warmup):
         step()
         log("warmup step %d done" % i)
     ctx.synchronize()
-    ctx.set_profiling(True)
-    ctx.reset_profile()
+    dom = share = None
+    if args.warmup:
+        prof = ctx.profile()
+        for f, v in prof.items():
+            if v["launches"]:
+                log("  %-18s %9.3f ms %6d launches %10.3f GB/s (warm-up)" % (
+                    f, v["ms"], v["launches"], v["bytes"] / (v["ms"] * 1e6) if v["ms"] else 0))
+        dom = dominant_kernel(ctx)
+        share = ctx.kernel_profile()[dom]["ms"] / sum(v["ms"] for v in prof.values())
+        ctx.reset_profile()
+        ctx.set_profile_filter(dom)
     work_units = 0
     last = None
     t0 = time.perf_counter()
@@ -566,16 +601,16 @@ def main():
         work_units += u
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    prof = ctx.profile()
-    ctx.set_profiling(False)
+    ctx.set_profile_filter(None)
     km1_end = work.km1()
     log("timed %d steps in %.3f s; km1 %d -> %d" % (args.steps, wall, km1_start, km1_end))
-    for f, v in prof.items():
-        if v["launches"]:
-            log("  %-18s %9.3f ms %6d launches %10.3f GB/s" % (f, v["ms"], v["launches"],
-                                                              v["bytes"] / (v["ms"] * 1e6) if v["ms"] else 0))
-
-    roofline = roofline_of(ctx, prof, args.workload)
+    if dom is None:
+        prof = ctx.profile()
+        roofline = roofline_of(ctx, prof, args.workload)
+    else:
+        roofline = roofline_of(ctx, None, args.workload, dom)
+        roofline["share_of_step_device_time"] = round(share, 3)
+    ctx.set_profiling(False)
 
     cpu = None
     if not args.no_cpu:

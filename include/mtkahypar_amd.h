@@ -54,6 +54,10 @@ MKP_API const char* mkp_profile_family_name(int family);
  * single kernel), keyed by kernel name and template variant, with their
  * summed device time and algorithmic bytes.  *name stays valid until the
  * next reset. */
+/* Restricts timing to the launches of one kernel (name as reported by
+ * mkp_context_profile_kernel; NULL times everything), so that measuring one
+ * kernel inside a timed region adds one event pair per launch of it only. */
+MKP_API int mkp_context_set_profile_filter(mkp_context* ctx, const char* kernel_name);
 MKP_API int mkp_context_profile_kernel_count(mkp_context* ctx, int32_t* count);
 MKP_API int mkp_context_profile_kernel(mkp_context* ctx, int32_t index, const char** name, double* device_ms,
                                        double* algorithmic_bytes, int64_t* launches);

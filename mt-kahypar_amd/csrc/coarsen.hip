@@ -325,66 +325,8 @@ co_rate_group_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t
     atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(visits));
 }
 
-// One proposer per block with an LDS map of 2^mbits <= 2^MAP_BITS slots
-// (mbits from ub).
-template <int MAP_BITS>
-__global__ void __launch_bounds__(256)
-co_rate_block_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t* __restrict__ list_count,
-                     const uint32_t* __restrict__ dclus, const uint32_t* __restrict__ dcount) {
-  __shared__ uint32_t s_end;
-  __shared__ uint64_t s_val[4];
-  __shared__ int s_un[4];
-  __shared__ uint32_t s_t[4];
-  __shared__ uint32_t K[1 << MAP_BITS];
-  __shared__ uint64_t V[1 << MAP_BITS];
-  const int gbits = MAP_BITS;
-  const uint32_t count = *list_count;
-  int64_t visits = 0;
-  for (uint32_t item = blockIdx.x; item < count; item += gridDim.x) {
-    const uint32_t u = list[item];
-    const uint64_t keys = min(a.ub[u], a.sampling);
-    int mbits = 1;
-    while ((uint64_t(1) << mbits) < 2 * keys) ++mbits;
-    mbits = min(mbits, gbits);
-    const uint32_t MT = 1u << mbits;
-    for (uint32_t t = threadIdx.x; t < MT; t += blockDim.x) { K[t] = kNone; V[t] = 0; }
-    if (threadIdx.x == 0) s_end = co_sweep_end(a, dcount, u, visits);
-    __threadfence_block();
-    __syncthreads();
-    co_fill_map<64>(a, dclus, dcount, a.voff[u], s_end, (threadIdx.x >> 6) * 64, blockDim.x, threadIdx.x & 63, K, V,
-                    mbits);
-    __threadfence_block();
-    __syncthreads();
-    Choice best = co_scan_map(a, u, K, V, MT, threadIdx.x, blockDim.x);
-    for (int o = 32; o > 0; o >>= 1) {
-      Choice other;
-      other.val = __shfl_xor(best.val, o, 64);
-      other.un = __shfl_xor(best.un, o, 64);
-      other.t = __shfl_xor(best.t, o, 64);
-      if (choice_better(a, u, other, best)) best = other;
-    }
-    if ((threadIdx.x & 63) == 0) {
-      s_val[threadIdx.x >> 6] = best.val;
-      s_un[threadIdx.x >> 6] = best.un;
-      s_t[threadIdx.x >> 6] = best.t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int w = 1; w < int(blockDim.x >> 6); ++w) {
-        Choice other{s_val[w], s_un[w], s_t[w]};
-        if (choice_better(a, u, other, best)) best = other;
-      }
-      a.target[u] = best.t;
-      a.rating[u] = best.val;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && visits)
-    atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(visits));
-}
-
 // ------------------------------------------------- heavy proposers --
-// Proposers with ub > 4096: their rating items are spread over the whole
+// Proposers with ub > 1024: their rating items are spread over the whole
 // grid.  Each owns a table of T = pow2 >= max(kHrChunk, 2 L) slots (L = its
 // distinct (net, cluster) items) in one arena; tables are multiples of
 // kHrChunk, so a chunk of the arena belongs to one proposer.
@@ -580,7 +522,8 @@ __global__ void hr_decide_kernel(CoArgs a, HrArgs h, const uint64_t* __restrict_
 // rating upper bound: sum of the sizes of the nets the rating sweeps
 __global__ void co_ub_kernel(const uint32_t* __restrict__ voff, const uint32_t* __restrict__ inc,
                              const uint32_t* __restrict__ noff, uint32_t n, uint32_t ignore,
-                             uint64_t* __restrict__ ub) {
+                             uint64_t* __restrict__ ub, uint64_t* __restrict__ ub_max) {
+  uint64_t mx = 0;
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
     uint64_t s = 0;
     for (uint32_t j = voff[u]; j < voff[u + 1]; ++j) {
@@ -589,19 +532,21 @@ __global__ void co_ub_kernel(const uint32_t* __restrict__ voff, const uint32_t* 
       s += sz < ignore ? sz : 0;
     }
     ub[u] = s;
+    mx = max(mx, s);
   }
+  for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+  if ((threadIdx.x & 63) == 0 && mx) atomicMax(reinterpret_cast<unsigned long long*>(ub_max), static_cast<unsigned long long>(mx));
 }
 
 // proposers of the sub-round by tier (rating map in LDS if ub <= 1024)
 __global__ void __launch_bounds__(256)
 co_select_kernel(uint32_t n, const uint8_t* __restrict__ matched, const uint64_t* __restrict__ ub, uint32_t salt,
                  uint32_t buckets, uint32_t bucket, uint32_t* __restrict__ target, uint32_t* __restrict__ list_a,
-                 uint32_t* __restrict__ list_b, uint32_t* __restrict__ list_c, uint32_t* __restrict__ list_d,
-                 uint32_t* __restrict__ counts) {
-  // tiers: c: ub <= 64 (counts[6]), a: ub <= 1024 (counts[0]), d: ub <= 4096
-  // (counts[2]), b: heavier (counts[1])
-  __shared__ uint32_t ca, cb, cc, cd, ba, bb, bc, bd;
-  if (threadIdx.x == 0) { ca = 0; cb = 0; cc = 0; cd = 0; }
+                 uint32_t* __restrict__ list_b, uint32_t* __restrict__ list_c, uint32_t* __restrict__ counts) {
+  // tiers: c: ub <= 64 (counts[6]), a: ub <= 1024 (counts[0]), b: heavier
+  // (counts[1])
+  __shared__ uint32_t ca, cb, cc, ba, bb, bc;
+  if (threadIdx.x == 0) { ca = 0; cb = 0; cc = 0; }
   __syncthreads();
   const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
   int tier = -1;
@@ -612,7 +557,6 @@ co_select_kernel(uint32_t n, const uint8_t* __restrict__ matched, const uint64_t
       const uint64_t b = ub[u];
       if (b <= 64) { tier = 2; l = atomicAdd(&cc, 1u); }
       else if (b <= 1024) { tier = 0; l = atomicAdd(&ca, 1u); }
-      else if (b <= 4096) { tier = 3; l = atomicAdd(&cd, 1u); }
       else { tier = 1; l = atomicAdd(&cb, 1u); }
     }
   }
@@ -621,13 +565,11 @@ co_select_kernel(uint32_t n, const uint8_t* __restrict__ matched, const uint64_t
     ba = ca ? atomicAdd(&counts[0], ca) : 0;
     bb = cb ? atomicAdd(&counts[1], cb) : 0;
     bc = cc ? atomicAdd(&counts[6], cc) : 0;
-    bd = cd ? atomicAdd(&counts[2], cd) : 0;
   }
   __syncthreads();
   if (tier == 0) list_a[ba + l] = u;
   if (tier == 1) list_b[bb + l] = u;
   if (tier == 2) list_c[bc + l] = u;
-  if (tier == 3) list_d[bd + l] = u;
 }
 
 // joiners and pairs (counts[4]: overflowing targets)
@@ -922,7 +864,7 @@ ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, in
   if (n == 0) return res;
   DevBuf<int64_t> cw(n), jw(n), visits(1);
   DevBuf<uint8_t> matched(n), role(n);
-  DevBuf<uint32_t> target(n), list_a(n), list_b(n), list_c(n), list_d(n), counts(8), olist(n);
+  DevBuf<uint32_t> target(n), list_a(n), list_b(n), list_c(n), counts(8), olist(n);
   // per sub-round distinct clusters of the proposers' nets
   const uint32_t m = hg->m;
   DevBuf<uint32_t> nmark(std::max<uint32_t>(m, 1)), mlist(std::max<uint32_t>(m, 1)), dcount(std::max<uint32_t>(m, 1)),
@@ -930,18 +872,17 @@ ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, in
   fill_async<uint32_t>(nmark.p, 0u, std::max<uint32_t>(m, 1), s);
   DevBuf<uint64_t> rating(n), ub(n);
   co_init_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, hg->vw.p, cid_out, cw.p, matched.p);
+  DevBuf<uint64_t> ub_max(1);
+  fill_async<uint64_t>(ub_max.p, 0ull, 1, s);
   co_ub_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(hg->voff.p, hg->inc.p, hg->noff.p, n, prm.ignore_net_size,
-                                                       ub.p);
+                                                       ub.p, ub_max.p);
   fill_async<int64_t>(visits.p, 0, 1, s);
   MKP_LAUNCH_CHECK();
-  // global rating maps for the heavy tier: sized by the largest bound
+  // the heavy tier runs only when some proposer has more than 1024 rating keys
   uint64_t max_ub = 0;
-  {
-    std::vector<uint64_t> h(n);
-    MKP_HIP(hipMemcpyAsync(h.data(), ub.p, size_t(n) * 8, hipMemcpyDeviceToHost, s));
-    c.sync();
-    for (uint64_t x : h) max_ub = std::max(max_ub, x);
-  }
+  MKP_HIP(hipMemcpyAsync(c.pinned, ub_max.p, 8, hipMemcpyDeviceToHost, s));
+  c.sync();
+  max_ub = uint64_t(c.pinned[0]);
   HeavyRating hw;
   const uint32_t limit = std::max<uint32_t>(uint32_t(double(n) / prm.max_shrink_factor), prm.contraction_limit);
   uint32_t current = n;
@@ -957,7 +898,7 @@ ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, in
     a.tag = (uint64_t(pass) << 8) | uint64_t(sr);
     fill_async<uint32_t>(counts.p, 0u, 8, s);
     co_select_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, matched.p, ub.p, salt, uint32_t(S), uint32_t(sr), target.p,
-                                                     list_a.p, list_b.p, list_c.p, list_d.p, counts.p);
+                                                     list_a.p, list_b.p, list_c.p, counts.p);
     MKP_LAUNCH_CHECK();
     {
       const uint32_t stamp = uint32_t(sr) + 1;  // nmark is cleared once per pass
@@ -967,7 +908,6 @@ ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, in
         co_mark_nets_kernel<<<mg, 256, 0, s>>>(a, list_c.p, counts.p + 6, stamp, nmark.p, mlist.p, counts.p + 7);
         co_mark_nets_kernel<<<mg, 256, 0, s>>>(a, list_a.p, counts.p + 0, stamp, nmark.p, mlist.p, counts.p + 7);
         co_mark_nets_kernel<<<mg, 256, 0, s>>>(a, list_b.p, counts.p + 1, stamp, nmark.p, mlist.p, counts.p + 7);
-        co_mark_nets_kernel<<<mg, 256, 0, s>>>(a, list_d.p, counts.p + 2, stamp, nmark.p, mlist.p, counts.p + 7);
       }
       {
         TimedRegion tr(c.prof, s, kFamRating, 0.0, "co_dedup_small_kernel");
@@ -989,12 +929,7 @@ ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, in
                                                                                 dcount.p);
       }
       MKP_LAUNCH_CHECK();
-      if (max_ub > 1024) {
-        TimedRegion tr(c.prof, s, kFamRating, 0.0, "co_rate_block_kernel<13>");
-        co_rate_block_kernel<13><<<unsigned(c.num_cus), 256, 0, s>>>(a, list_d.p, counts.p + 2, dclus.p, dcount.p);
-        MKP_LAUNCH_CHECK();
-      }
-      if (max_ub > 4096) rate_heavy(c, a, list_b.p, counts.p + 1, dclus.p, dcount.p, hw);
+      if (max_ub > 1024) rate_heavy(c, a, list_b.p, counts.p + 1, dclus.p, dcount.p, hw);
     }
     fill_async<int64_t>(jw.p, 0, n, s);
     co_resolve_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, target.p, hg->vw.p, jw.p, role.p);

@@ -244,6 +244,7 @@ enum Family : int {
 
 struct Profiler {
   bool enabled = false;
+  std::string only;  // non-empty: time only the launches of this kernel
   // kname: single-kernel record (nullptr for a family-only region)
   struct Rec { hipEvent_t a, b; int fam; double bytes; std::string kname; };
   struct KernelStat { std::string name; double ms = 0, bytes = 0; int64_t launches = 0; };
@@ -308,7 +309,7 @@ struct TimedRegion {
   hipEvent_t a = nullptr;
   TimedRegion(Profiler& prof, hipStream_t st, int f, double b, std::string kernel_name = std::string())
       : p(prof), s(st), fam(f), bytes(b), kname(std::move(kernel_name)) {
-    if (p.enabled) { a = p.take(); (void)hipEventRecord(a, s); }
+    if (p.enabled && (p.only.empty() || kname == p.only)) { a = p.take(); (void)hipEventRecord(a, s); }
   }
   ~TimedRegion() {
     if (a) {

@@ -160,6 +160,14 @@ int mkp_context_reset_profile(mkp_context* ctx) {
   });
 }
 
+int mkp_context_set_profile_filter(mkp_context* ctx, const char* kernel_name) {
+  return guarded([&] {
+    MKP_CHECK(ctx, "ctx is null");
+    ctx->c.sync();
+    ctx->c.prof.only = kernel_name ? std::string(kernel_name) : std::string();
+  });
+}
+
 int mkp_context_get_profile(mkp_context* ctx, int family, double* ms, double* bytes, int64_t* launches) {
   return guarded([&] {
     MKP_CHECK(ctx, "ctx is null");

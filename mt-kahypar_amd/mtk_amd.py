@@ -158,6 +158,7 @@ SIGNATURES = {
     "mkp_batch_get_info": (C.c_int, [vp, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32), P(C.c_uint64)]),
     "mkp_batch_hypergraph": (C.c_int, [vp, P(vp)]),
     "mkp_batch_louvain": (C.c_int, [vp, P(LouvainParams), vp, vp, vp]),
+    "mkp_context_set_profile_filter": (C.c_int, [vp, C.c_char_p]),
     "mkp_batch_partition_copy_parts": (C.c_int, [vp, vp]),
     "mkp_request_estimate": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_int32,
                                        P(C.c_uint64)]),
@@ -223,6 +224,10 @@ class Context:
 
     def reset_profile(self):
         check(lib().mkp_context_reset_profile(self.h))
+
+    def set_profile_filter(self, kernel_name=None):
+        """Time only the launches of one kernel (None: every kernel)."""
+        check(lib().mkp_context_set_profile_filter(self.h, None if kernel_name is None else kernel_name.encode()))
 
     def profile(self):
         out = {}

@@ -160,6 +160,20 @@ int mkp_context_reset_profile(mkp_context* ctx) {
   });
 }
 
+int mkp_context_set_tuning(mkp_context* ctx, const char* key, int64_t value) {
+  return guarded([&] {
+    MKP_CHECK(ctx && key, "null argument");
+    const std::string k(key);
+    if (k == "louvain_group_lanes") {
+      MKP_CHECK(value == 0 || value == 16 || value == 32 || value == 64,
+                "louvain_group_lanes must be 0 (autotune), 16, 32 or 64");
+      ctx->c.tune.lv_group_lanes = int(value);
+    } else {
+      MKP_CHECK(false, "unknown tuning key");
+    }
+  });
+}
+
 int mkp_context_set_profile_filter(mkp_context* ctx, const char* kernel_name) {
   return guarded([&] {
     MKP_CHECK(ctx, "ctx is null");

@@ -192,7 +192,7 @@ __device__ inline void map_insert_aggregated(int32_t* K, double* Wt, uint32_t* F
                                              int32_t key, double w, uint32_t pos, int passes) {
   const int lane = threadIdx.x & 63;
   const int seg_lo = lane & ~(WIDTH - 1);
-  const uint64_t seg_mask = (WIDTH == 64) ? ~0ull : (((1ull << WIDTH) - 1) << seg_lo);
+  const uint64_t seg_mask = (WIDTH == 64) ? ~0ull : (((1ull << (WIDTH & 63)) - 1) << seg_lo);
   bool pending = valid;
   auto insert = [&](int32_t k, double ww, uint32_t pp) {
     const uint32_t T = 1u << tbits;
@@ -223,6 +223,14 @@ __device__ inline void map_insert_aggregated(int32_t* K, double* Wt, uint32_t* F
   if (pending) insert(key, w, pos);
 }
 
+// orders the LDS accesses of the lanes of one wave (in place of a block
+// barrier when every LDS region is private to one wave)
+__device__ inline void lds_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // best (gain, first) pair: larger gain, then smaller first occurrence
 __device__ inline bool better(double g, uint32_t f, double bg, uint32_t bf) {
   return g > bg || (g == bg && f < bf);
@@ -235,6 +243,24 @@ __device__ inline void lv_decide(const LvArgs& a, uint32_t u, double wfrom, doub
   const double bg = wfrom - mult * (a.cvol[from] - volU);
   if (best_key >= 0 && best_gain > bg) {
     if (a.csize[from] == 1 && a.csize[best_key] == 1 && best_key > from) return;
+    const uint32_t slot = atomicAdd(a.move_count, 1u);
+    a.move_u[slot] = u;
+    a.move_to[slot] = best_key;
+  }
+}
+
+// The node's own inputs of the decision, loaded ahead of the arc sweep so
+// their latency overlaps it (same values and operation order as lv_decide).
+struct LvNode {
+  int32_t from;
+  double volU, mult, cvol_from;
+  int32_t csize_from;
+};
+__device__ inline void lv_decide(const LvArgs& a, uint32_t u, const LvNode& nd, double wfrom, double best_gain,
+                                 int32_t best_key) {
+  const double bg = wfrom - nd.mult * (nd.cvol_from - nd.volU);
+  if (best_key >= 0 && best_gain > bg) {
+    if (nd.csize_from == 1 && a.csize[best_key] == 1 && best_key > nd.from) return;
     const uint32_t slot = atomicAdd(a.move_count, 1u);
     a.move_u[slot] = u;
     a.move_to[slot] = best_key;
@@ -264,17 +290,24 @@ louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
       w[i] = 0.0;
       if (uint32_t(i) < deg) g.arc(u, uint32_t(i), head[i], w[i]);
     }
-    const int32_t from = a.C[u];
+    LvNode nd;
+    nd.from = a.C[u];
+    nd.volU = a.nvol[u];
+    const double rc = lv_recip(a, u);
     int32_t key[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) {
       const int32_t c = a.C[head[i]];
       key[i] = uint32_t(i) < deg ? c : -1;
     }
+    const int32_t from = nd.from;
     double cv[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) cv[i] = a.cvol[key[i] >= 0 ? key[i] : from];
-    const double mult = lv_recip(a, u) * a.nvol[u];
+    nd.cvol_from = a.cvol[from];
+    nd.csize_from = a.csize[from];
+    nd.mult = rc * nd.volU;
+    const double mult = nd.mult;
     double wfrom = 0.0, bgain = -DBL_MAX;
     int32_t bkey = -1;
 #pragma unroll
@@ -295,7 +328,7 @@ louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
         }
       }
     }
-    lv_decide(a, u, wfrom, bgain, bkey);
+    lv_decide(a, u, nd, wfrom, bgain, bkey);
   }
 }
 
@@ -312,17 +345,29 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
   int32_t* K = keys + grp * T;
   double* Wt = wsum + grp * T;
   uint32_t* F = first + grp * T;
-  for (uint32_t base = blockIdx.x * GPB; base < count; base += gridDim.x * GPB) {
-    const uint32_t idx = base + grp;
+  // every wave walks its own nodes (64 / G groups per wave): a group's map is
+  // touched by its own lanes only, so waves never wait for each other
+  constexpr int GPW = 64 / G, WPB = BLOCK / 64;
+  const uint32_t wid = threadIdx.x >> 6, wgrp = (threadIdx.x & 63) / G;
+  for (uint32_t base = (blockIdx.x * WPB + wid) * GPW; base < count; base += gridDim.x * WPB * GPW) {
+    const uint32_t idx = base + wgrp;
     const bool in = idx < count;  // lists hold exactly the nodes of this sub-round
     const uint32_t u = in ? list[idx] : 0;
     const uint32_t deg = in ? min(g.degree(u), a.sampling) : 0;
+    // the node's own inputs of the decision, issued ahead of the sweep
+    LvNode nd{0, 0.0, 0.0, 0.0, 0};
+    double rc = 0.0;
+    if (in) {
+      nd.from = a.C[u];
+      nd.volU = a.nvol[u];
+      rc = lv_recip(a, u);
+    }
     // the group's map uses the first Tn <= T slots, Tn >= 2 deg
     int tb = 1;
     while (tb < TBITS && (1u << tb) < 2 * deg) ++tb;
     const int Tn = 1 << tb;
     for (int t = lane; t < Tn; t += G) { K[t] = -1; Wt[t] = 0.0; F[t] = 0xffffffffu; }
-    __syncthreads();
+    lds_wave_sync();
     {
       // trip count uniform over the wave: the aggregation shuffles need it
       uint32_t trips = (deg + G - 1) / G;
@@ -340,10 +385,15 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
         map_insert_aggregated<G>(K, Wt, F, tb, valid, c, w, i, 2);
       }
     }
-    __syncthreads();
+    lds_wave_sync();
     if (in) {
-      const int32_t from = a.C[u];
-      const double mult = lv_recip(a, u) * a.nvol[u];
+      const int32_t from = nd.from;
+      if (lane == 0) {  // needed by the decision only; in flight during the scan
+        nd.cvol_from = a.cvol[from];
+        nd.csize_from = a.csize[from];
+      }
+      nd.mult = rc * nd.volU;
+      const double mult = nd.mult;
       double wfrom = 0.0, bgain = -DBL_MAX;
       uint32_t bfirst = 0xffffffffu;
       int32_t bkey = -1;
@@ -361,9 +411,9 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
         wfrom += __shfl_xor(wfrom, o, G);
         if (ok >= 0 && (bkey < 0 || better(og, of, bgain, bfirst))) { bgain = og; bfirst = of; bkey = ok; }
       }
-      if (lane == 0) lv_decide(a, u, wfrom, bgain, bkey);
+      if (lane == 0) lv_decide(a, u, nd, wfrom, bgain, bkey);
     }
-    __syncthreads();
+    lds_wave_sync();
   }
 }
 
@@ -902,10 +952,73 @@ struct LouvainRun {
     MKP_LAUNCH_CHECK();
   }
 
+  // ----- autotuner of the lanes per node of the 17..128-arc class.  The
+  // three shapes compute the same result; in the first round of a level with
+  // an untuned key, sub-rounds 0..2 run them in turn with an event pair
+  // around the class launch, and the fastest per arc is cached in the context.
+  static constexpr int kLaneTrials[3] = {16, 32, 64};
+  hipEvent_t tev_[6] = {};
+  ~LouvainRun() {
+    for (auto& e : tev_)
+      if (e) (void)hipEventDestroy(e);
+  }
+  struct LaneChoice {
+    int lanes = 32;
+    bool trial = false;
+    uint64_t key = 0;
+  };
+  template <class View>
+  LaneChoice choose_lanes(const std::vector<uint32_t>& cnt, const std::vector<uint64_t>& sa, int B) {
+    LaneChoice ch;
+    if (c.tune.lv_group_lanes) {
+      ch.lanes = c.tune.lv_group_lanes;
+      return ch;
+    }
+    uint64_t nodes = 0, arcs = 0;
+    for (int sr = 0; sr < B; ++sr) { nodes += cnt[B + sr]; arcs += sa[B + sr]; }
+    int ln = 0, ld = 0;
+    while ((uint64_t(1) << (ln + 1)) <= nodes) ++ln;
+    while ((uint64_t(1) << (ld + 1)) <= arcs / std::max<uint64_t>(nodes, 1)) ++ld;
+    ch.key = (uint64_t(uint8_t(View::kName[0])) << 32) | (uint64_t(ln) << 8) | uint64_t(ld);
+    auto it = c.tune.lv_group_choice.find(ch.key);
+    if (it != c.tune.lv_group_choice.end()) {
+      ch.lanes = it->second;
+      return ch;
+    }
+    ch.trial = B >= 3 && cnt[B] && cnt[B + 1] && cnt[B + 2];
+    if (ch.trial)
+      for (auto& e : tev_)
+        if (!e) MKP_HIP(hipEventCreate(&e));
+    return ch;
+  }
+  // after the round's synchronisation: keep the fastest shape per arc
+  void finish_lanes(const LaneChoice& ch, const std::vector<uint64_t>& sa, int B) {
+    if (!ch.trial) return;
+    int best = 1;
+    double best_t = 0;
+    for (int i = 0; i < 3; ++i) {
+      float ms = 0;
+      MKP_HIP(hipEventElapsedTime(&ms, tev_[2 * i], tev_[2 * i + 1]));
+      const double t = double(ms) / double(std::max<uint64_t>(sa[B + i], 1));
+      if (i == 0 || t < best_t) { best = i; best_t = t; }
+    }
+    c.tune.lv_group_choice[ch.key] = kLaneTrials[best];
+  }
+  int lanes_for(const LaneChoice& ch, int sr, hipEvent_t* ta, hipEvent_t* tb) {
+    if (ch.trial && sr < 3) {
+      *ta = tev_[2 * sr];
+      *tb = tev_[2 * sr + 1];
+      return kLaneTrials[sr];
+    }
+    *ta = *tb = nullptr;
+    return ch.lanes;
+  }
+
   template <class View>
   void launch_move(const View& g, const LvArgs& a, const uint32_t* list, const std::vector<uint32_t>& off,
                    const std::vector<uint32_t>& cnt, const std::vector<uint64_t>& sa,
-                   const std::vector<uint64_t>& sb, int sr) {
+                   const std::vector<uint64_t>& sb, int sr, int lanes = 32, hipEvent_t ta = nullptr,
+                   hipEvent_t tb = nullptr) {
     const hipStream_t s = c.stream;
     const int B = a.buckets;
     const int s0 = 0 * B + sr, s1 = 1 * B + sr, s2 = 2 * B + sr, s3 = 3 * B + sr;
@@ -915,10 +1028,20 @@ struct LouvainRun {
       louvain_move_small_kernel<View><<<grid_for(cnt[s0], 256, 65536), 256, 0, s>>>(g, a, list + off[s0], cnt[s0]);
     }
     if (cnt[s1]) {
+      const int G = lanes;
+      if (ta) MKP_HIP(hipEventRecord(ta, s));
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s1]),
-                     std::string("louvain_move_group_kernel<") + View::kName + ", 32, 8, 256>");
-      louvain_move_group_kernel<View, 32, 8, 256><<<grid_for(cnt[s1], 8, 65536), 256, 0, s>>>(g, a, list + off[s1],
-                                                                                              cnt[s1]);
+                     std::string("louvain_move_group_kernel<") + View::kName + ", " + std::to_string(G) + ", 8, 256>");
+      if (G == 16)
+        louvain_move_group_kernel<View, 16, 8, 256><<<grid_for(cnt[s1], 16, 65536), 256, 0, s>>>(g, a, list + off[s1],
+                                                                                                 cnt[s1]);
+      else if (G == 64)
+        louvain_move_group_kernel<View, 64, 8, 256><<<grid_for(cnt[s1], 4, 65536), 256, 0, s>>>(g, a, list + off[s1],
+                                                                                                cnt[s1]);
+      else
+        louvain_move_group_kernel<View, 32, 8, 256><<<grid_for(cnt[s1], 8, 65536), 256, 0, s>>>(g, a, list + off[s1],
+                                                                                                cnt[s1]);
+      if (tb) MKP_HIP(hipEventRecord(tb, s));
     }
     if (cnt[s2]) {
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s2]),
@@ -997,10 +1120,13 @@ struct LouvainRun {
       MKP_LAUNCH_CHECK();
       if (hnodes) prepare_heavy(g, a, list.p + off[3 * B], hnodes);
       fill_async<uint32_t>(mcount.p, 0u, 2, s);
+      const LaneChoice lanes = choose_lanes<View>(cnt, sa, B);
       for (int sr = 0; sr < B; ++sr) {
         a.bucket = sr;
         a.move_count = mcount.p + (sr & 1);
-        launch_move(g, a, list.p, off, cnt, sa, sb, sr);
+        hipEvent_t ta, tb;
+        const int lanes_now = lanes_for(lanes, sr, &ta, &tb);
+        launch_move(g, a, list.p, off, cnt, sa, sb, sr, lanes_now, ta, tb);
         // at most one move per node of this sub-round
         const uint32_t sub_nodes = cnt[sr] + cnt[B + sr] + cnt[2 * B + sr] + cnt[3 * B + sr];
         louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(
@@ -1010,6 +1136,7 @@ struct LouvainRun {
       }
       MKP_HIP(hipMemcpyAsync(c.pinned, moved_total.p, 4, hipMemcpyDeviceToHost, s));
       c.sync();
+      finish_lanes(lanes, sa, B);
       moved = *reinterpret_cast<const uint32_t*>(c.pinned);
       st.arcs_swept += int64_t(num_arcs);
       st.rounds += 1;
@@ -1373,10 +1500,13 @@ struct LouvainBatchRun : LouvainRun {
       MKP_LAUNCH_CHECK();
       if (hnodes) prepare_heavy(g, a, list.p + off[3 * B], hnodes);
       fill_async<uint32_t>(mcount.p, 0u, 2, s);
+      const LaneChoice lanes = choose_lanes<View>(cnt, sa, B);
       for (int sr = 0; sr < B; ++sr) {
         a.bucket = sr;
         a.move_count = mcount.p + (sr & 1);
-        launch_move(g, a, list.p, off, cnt, sa, sb, sr);
+        hipEvent_t ta, tb;
+        const int lanes_now = lanes_for(lanes, sr, &ta, &tb);
+        launch_move(g, a, list.p, off, cnt, sa, sb, sr, lanes_now, ta, tb);
         const uint32_t sub_nodes = cnt[sr] + cnt[B + sr] + cnt[2 * B + sr] + cnt[3 * B + sr];
         louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(
             mu.p, mto.p, a.move_count, nvol, C, cvol_.p, csize.p, moved_total.p, mcount.p + ((sr + 1) & 1),
@@ -1386,6 +1516,7 @@ struct LouvainBatchRun : LouvainRun {
       uint32_t* stg_moved = c.staging<uint32_t>(G);
       MKP_HIP(hipMemcpyAsync(stg_moved, gmoved_.p, size_t(G) * 4, hipMemcpyDeviceToHost, s));
       c.sync();
+      finish_lanes(lanes, sa, B);
       for (uint32_t gm = 0; gm < G; ++gm) {
         if (!act[gm]) continue;
         moved[gm] = stg_moved[gm];

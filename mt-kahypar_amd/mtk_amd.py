@@ -159,6 +159,7 @@ SIGNATURES = {
     "mkp_batch_hypergraph": (C.c_int, [vp, P(vp)]),
     "mkp_batch_louvain": (C.c_int, [vp, P(LouvainParams), vp, vp, vp]),
     "mkp_context_set_profile_filter": (C.c_int, [vp, C.c_char_p]),
+    "mkp_context_set_tuning": (C.c_int, [vp, C.c_char_p, C.c_int64]),
     "mkp_batch_partition_copy_parts": (C.c_int, [vp, vp]),
     "mkp_request_estimate": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, C.c_int32,
                                        P(C.c_uint64)]),
@@ -224,6 +225,9 @@ class Context:
 
     def reset_profile(self):
         check(lib().mkp_context_reset_profile(self.h))
+
+    def set_tuning(self, key, value):
+        check(lib().mkp_context_set_tuning(self.h, key.encode(), int(value)))
 
     def set_profile_filter(self, kernel_name=None):
         """Time only the launches of one kernel (None: every kernel)."""

@@ -106,3 +106,29 @@ def test_fractional_weights_modularity(ctx, weight):
     assert abs(q(comm) - q(ref)) < 0.02
     nc, nref = len(set(comm.tolist())), len(set(ref.tolist()))
     assert abs(nc - nref) <= 0.2 * nref + 2
+
+
+def test_group_lane_shapes_and_autotuner_agree(ctx):
+    """The lanes per node of the 17..128-arc class (16, 32, 64, or chosen by
+    the autotuner) change only the speed: communities and statistics are
+    identical, and equal to the oracle's."""
+    n, noff, pins = powerlaw(8000, 12000, 200, 17, heavy_nets=1)
+    c2 = mtk_amd.Context(0)
+    try:
+        out = []
+        for lanes in (0, 16, 32, 64, 0):
+            c2.set_tuning("louvain_group_lanes", lanes)
+            dh = mtk_amd.Hypergraph(c2, n, noff, pins)
+            out.append(dh.louvain(seed=5))
+            dh.close()
+        for comm, st in out[1:]:
+            assert np.array_equal(comm, out[0][0])
+            assert (st["arcs_swept"], st["rounds"], st["levels"], st["moves"]) == \
+                (out[0][1]["arcs_swept"], out[0][1]["rounds"], out[0][1]["levels"], out[0][1]["moves"])
+        g = orc.Graph(orc.Hypergraph(n, noff, pins), 0, ranked=True)
+        C, _ = g.louvain(sync=True, seed=5)
+        assert out[0][0].tolist() == C[:n].tolist()
+        with pytest.raises(RuntimeError):
+            c2.set_tuning("louvain_group_lanes", 24)
+    finally:
+        c2.close()

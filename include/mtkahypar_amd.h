@@ -59,8 +59,8 @@ MKP_API const char* mkp_profile_family_name(int family);
  * kernel inside a timed region adds one event pair per launch of it only. */
 MKP_API int mkp_context_set_profile_filter(mkp_context* ctx, const char* kernel_name);
 /* Kernel shape choices that never change results, e.g. "louvain_group_lanes"
- * (lanes per node of the 17..128-arc Louvain class: 16, 32 or 64; 0, the
- * default, lets the autotuner time the three on the first sub-rounds of a
+ * (lanes per node of the 17..128-arc Louvain class: 8, 16, 32 or 64; 0, the
+ * default, lets the autotuner time the four on the first sub-rounds of a
  * level and keep the fastest per graph kind, class size and mean degree). */
 MKP_API int mkp_context_set_tuning(mkp_context* ctx, const char* key, int64_t value);
 MKP_API int mkp_context_profile_kernel_count(mkp_context* ctx, int32_t* count);

@@ -324,7 +324,7 @@ struct TimedRegion {
 // Kernel shape choices that do not change results (set by the autotuner or
 // mkp_context_set_tuning).
 struct Tuning {
-  // lanes per node of the Louvain 17..128-arc class (16, 32 or 64); 0 lets
+  // lanes per node of the Louvain 17..128-arc class (8, 16, 32 or 64); 0 lets
   // the autotuner choose per (graph kind, class size, mean degree)
   int lv_group_lanes = 0;
   std::map<uint64_t, int> lv_group_choice;  // autotuner cache

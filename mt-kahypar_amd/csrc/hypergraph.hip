@@ -165,8 +165,8 @@ int mkp_context_set_tuning(mkp_context* ctx, const char* key, int64_t value) {
     MKP_CHECK(ctx && key, "null argument");
     const std::string k(key);
     if (k == "louvain_group_lanes") {
-      MKP_CHECK(value == 0 || value == 16 || value == 32 || value == 64,
-                "louvain_group_lanes must be 0 (autotune), 16, 32 or 64");
+      MKP_CHECK(value == 0 || value == 8 || value == 16 || value == 32 || value == 64,
+                "louvain_group_lanes must be 0 (autotune), 8, 16, 32 or 64");
       ctx->c.tune.lv_group_lanes = int(value);
     } else {
       MKP_CHECK(false, "unknown tuning key");

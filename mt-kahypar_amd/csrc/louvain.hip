@@ -953,11 +953,12 @@ struct LouvainRun {
   }
 
   // ----- autotuner of the lanes per node of the 17..128-arc class.  The
-  // three shapes compute the same result; in the first round of a level with
-  // an untuned key, sub-rounds 0..2 run them in turn with an event pair
-  // around the class launch, and the fastest per arc is cached in the context.
-  static constexpr int kLaneTrials[3] = {16, 32, 64};
-  hipEvent_t tev_[6] = {};
+  // shapes compute the same result; in the first round of a level with an
+  // untuned key, sub-rounds 0..3 run them in turn with an event pair around
+  // the class launch, and the fastest per arc is cached in the context.
+  static constexpr int kLaneTrials[4] = {8, 16, 32, 64};
+  static constexpr int kTrials = 4;
+  hipEvent_t tev_[2 * kTrials] = {};
   ~LouvainRun() {
     for (auto& e : tev_)
       if (e) (void)hipEventDestroy(e);
@@ -985,7 +986,8 @@ struct LouvainRun {
       ch.lanes = it->second;
       return ch;
     }
-    ch.trial = B >= 3 && cnt[B] && cnt[B + 1] && cnt[B + 2];
+    ch.trial = B >= kTrials;
+    for (int i = 0; i < kTrials && ch.trial; ++i) ch.trial = cnt[B + i] != 0;
     if (ch.trial)
       for (auto& e : tev_)
         if (!e) MKP_HIP(hipEventCreate(&e));
@@ -996,7 +998,7 @@ struct LouvainRun {
     if (!ch.trial) return;
     int best = 1;
     double best_t = 0;
-    for (int i = 0; i < 3; ++i) {
+    for (int i = 0; i < kTrials; ++i) {
       float ms = 0;
       MKP_HIP(hipEventElapsedTime(&ms, tev_[2 * i], tev_[2 * i + 1]));
       const double t = double(ms) / double(std::max<uint64_t>(sa[B + i], 1));
@@ -1005,7 +1007,7 @@ struct LouvainRun {
     c.tune.lv_group_choice[ch.key] = kLaneTrials[best];
   }
   int lanes_for(const LaneChoice& ch, int sr, hipEvent_t* ta, hipEvent_t* tb) {
-    if (ch.trial && sr < 3) {
+    if (ch.trial && sr < kTrials) {
       *ta = tev_[2 * sr];
       *tb = tev_[2 * sr + 1];
       return kLaneTrials[sr];
@@ -1031,8 +1033,12 @@ struct LouvainRun {
       const int G = lanes;
       if (ta) MKP_HIP(hipEventRecord(ta, s));
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s1]),
-                     std::string("louvain_move_group_kernel<") + View::kName + ", " + std::to_string(G) + ", 8, 256>");
-      if (G == 16)
+                     std::string("louvain_move_group_kernel<") + View::kName + ", " + std::to_string(G) + ", 8, " +
+                         (G == 8 ? "128>" : "256>"));
+      if (G == 8)
+        louvain_move_group_kernel<View, 8, 8, 128><<<grid_for(cnt[s1], 16, 65536), 128, 0, s>>>(g, a, list + off[s1],
+                                                                                                cnt[s1]);
+      else if (G == 16)
         louvain_move_group_kernel<View, 16, 8, 256><<<grid_for(cnt[s1], 16, 65536), 256, 0, s>>>(g, a, list + off[s1],
                                                                                                  cnt[s1]);
       else if (G == 64)

@@ -109,14 +109,14 @@ def test_fractional_weights_modularity(ctx, weight):
 
 
 def test_group_lane_shapes_and_autotuner_agree(ctx):
-    """The lanes per node of the 17..128-arc class (16, 32, 64, or chosen by
+    """The lanes per node of the 17..128-arc class (8, 16, 32, 64, or chosen by
     the autotuner) change only the speed: communities and statistics are
     identical, and equal to the oracle's."""
     n, noff, pins = powerlaw(8000, 12000, 200, 17, heavy_nets=1)
     c2 = mtk_amd.Context(0)
     try:
         out = []
-        for lanes in (0, 16, 32, 64, 0):
+        for lanes in (0, 8, 16, 32, 64, 0):
             c2.set_tuning("louvain_group_lanes", lanes)
             dh = mtk_amd.Hypergraph(c2, n, noff, pins)
             out.append(dh.louvain(seed=5))

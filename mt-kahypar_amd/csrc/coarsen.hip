@@ -111,16 +111,17 @@ __device__ inline uint32_t co_sweep_end(const CoArgs& a, const uint32_t* __restr
 }
 
 // nets of the proposers in `list`; the first marker of a net appends it to
-// mlist (each net once per sub-round); one wave per proposer
+// mlist (each net once per sub-round); G lanes per proposer
+template <int G>
 __global__ void __launch_bounds__(256)
 co_mark_nets_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t* __restrict__ count, uint32_t stamp,
                     uint32_t* __restrict__ nmark, uint32_t* __restrict__ mlist, uint32_t* __restrict__ mcount) {
   const uint32_t cnt = *count;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < cnt; i += waves) {
+  const uint32_t lane = threadIdx.x % G;
+  const uint32_t groups = gridDim.x * (blockDim.x / G);
+  for (uint32_t i = blockIdx.x * (blockDim.x / G) + threadIdx.x / G; i < cnt; i += groups) {
     const uint32_t u = list[i];
-    for (uint32_t j = a.voff[u] + lane; j < a.voff[u + 1]; j += 64) {
+    for (uint32_t j = a.voff[u] + lane; j < a.voff[u + 1]; j += G) {
       const uint32_t e = a.inc[j];
       if (a.noff[e + 1] - a.noff[e] >= a.ignore || nmark[e] == stamp) continue;
       if (atomicExch(&nmark[e], stamp) != stamp) mlist[atomicAdd(mcount, 1u)] = e;
@@ -201,12 +202,15 @@ co_dedup_wave_kernel(CoArgs a, const uint32_t* __restrict__ mlist, const uint32_
   }
 }
 
+// A slot's key never changes once set within a sub-round, so a plain read
+// that shows a key is exact; only a slot that reads empty needs the CAS.
 __device__ inline void co_map_add(uint32_t* K, uint64_t* V, int bits, uint32_t rep, uint64_t score) {
   const uint32_t T = 1u << bits;
   uint32_t h = co_slot(rep, bits);
   while (true) {
-    const uint32_t old = atomicCAS(&K[h], kNone, rep);
-    if (old == kNone || old == rep) break;
+    uint32_t cur = __atomic_load_n(&K[h], __ATOMIC_RELAXED);
+    if (cur == kNone) cur = atomicCAS(&K[h], kNone, rep);
+    if (cur == kNone || cur == rep) break;
     h = (h + 1) & (T - 1);
   }
   atomicAdd(reinterpret_cast<unsigned long long*>(&V[h]), static_cast<unsigned long long>(score));
@@ -905,9 +909,9 @@ ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, in
       const unsigned mg = unsigned(c.num_cus) * 8;
       {
         TimedRegion tr(c.prof, s, kFamRating, 0.0, "co_mark_nets_kernel");
-        co_mark_nets_kernel<<<mg, 256, 0, s>>>(a, list_c.p, counts.p + 6, stamp, nmark.p, mlist.p, counts.p + 7);
-        co_mark_nets_kernel<<<mg, 256, 0, s>>>(a, list_a.p, counts.p + 0, stamp, nmark.p, mlist.p, counts.p + 7);
-        co_mark_nets_kernel<<<mg, 256, 0, s>>>(a, list_b.p, counts.p + 1, stamp, nmark.p, mlist.p, counts.p + 7);
+        co_mark_nets_kernel<8><<<mg, 256, 0, s>>>(a, list_c.p, counts.p + 6, stamp, nmark.p, mlist.p, counts.p + 7);
+        co_mark_nets_kernel<64><<<mg, 256, 0, s>>>(a, list_a.p, counts.p + 0, stamp, nmark.p, mlist.p, counts.p + 7);
+        co_mark_nets_kernel<64><<<mg, 256, 0, s>>>(a, list_b.p, counts.p + 1, stamp, nmark.p, mlist.p, counts.p + 7);
       }
       {
         TimedRegion tr(c.prof, s, kFamRating, 0.0, "co_dedup_small_kernel");

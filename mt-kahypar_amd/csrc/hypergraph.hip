@@ -83,6 +83,10 @@ PcLayout pc_layout(int k, uint32_t max_esize) {
   L.epw = std::min(64 / L.b, k);
   L.vpc = k / L.epw + (k % L.epw != 0);
   L.mask = (L.b >= 64) ? ~0ull : ((1ull << L.b) - 1);
+  L.vcs = (k + 63) / 64;
+  L.coff = L.vpc;
+  L.woff = L.vpc + L.vcs;
+  L.stride = L.vpc + L.vcs + 1;
   return L;
 }
 

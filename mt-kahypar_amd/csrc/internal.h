@@ -36,8 +36,13 @@ namespace mkp {
 // vpc words per net (pin_count_in_part.h:198-215).  Fields never carry into
 // each other because a count stays within [0, |e|] <= 2^b - 1, so a 64-bit
 // atomic add of +-(1 << shift) is an exact per-field update.
+// The partition state of a net is one record of `stride` words: its vpc
+// pin-count words, then its vcs connectivity words (at coff), then its net
+// weight (at woff), so a gain or apply step touches one record per incident
+// net instead of three arrays.
 struct PcLayout {
   int b = 1, epw = 1, vpc = 1;
+  int vcs = 1, coff = 1, woff = 2, stride = 3;
   uint64_t mask = 1;
 };
 PcLayout pc_layout(int k, uint32_t max_esize);
@@ -54,7 +59,7 @@ struct mkp_partition {
   mkp::PcLayout L;
   mkp::DevBuf<int32_t> part;
   mkp::DevBuf<int64_t> part_w;
-  mkp::DevBuf<uint64_t> pc, conn;
+  mkp::DevBuf<uint64_t> pc;  // net records (PcLayout): pin counts, connectivity, weight
   mkp::DevBuf<int64_t> km1_acc;
 };
 

@@ -89,7 +89,6 @@ struct LpArgs {
   int32_t* part;
   int64_t* part_w;
   uint64_t* pc;
-  uint64_t* conn;
   const int64_t* max_w;
   const int64_t* perfect_w;
   PcLayout L;
@@ -130,7 +129,7 @@ __device__ inline void counter_add(int64_t* c, int64_t v) {
 }
 
 __device__ inline uint32_t pc_field(const uint64_t* pc, const PcLayout& L, uint32_t e, int wi, int sh) {
-  return uint32_t((pc[uint64_t(e) * L.vpc + wi] >> sh) & L.mask);
+  return uint32_t((pc[uint64_t(e) * L.stride + wi] >> sh) & L.mask);
 }
 
 __device__ inline bool in_bucket(const LpArgs& a, uint32_t u) {
@@ -206,9 +205,10 @@ lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
           uint32_t pf[NB];
 #pragma unroll
           for (int t = 0; t < NB; ++t) {
-            w[t] = a.nw[e[t]];
-            cw[t] = a.conn[e[t]];
-            pf[t] = pc_field(a.pc, a.L, e[t], fwi, fsh);
+            const uint64_t* r = a.pc + uint64_t(e[t]) * a.L.stride;  // one record per net
+            w[t] = int32_t(r[a.L.woff]);
+            cw[t] = r[a.L.coff];
+            pf[t] = uint32_t((r[fwi] >> fsh) & a.L.mask);
           }
 #pragma unroll
           for (int t = 0; t < NB; ++t) {
@@ -300,11 +300,11 @@ lp_gain_wave_kernel(LpArgs a, const uint32_t* __restrict__ active, const uint32_
     int border = 0;
     for (uint32_t j = jb + lane; j < je; j += 64) {
       const uint32_t e = a.inc[j];
-      const int32_t w = a.nw[e];
+      const int32_t w = int32_t(a.pc[uint64_t(e) * a.L.stride + a.L.woff]);
       if (pc_field(a.pc, a.L, e, fwi, fsh) > 1) internal += w;
       int lambda = 0;
       for (int ci = 0; ci < a.vcs; ++ci) {
-        uint64_t bits = a.conn[uint64_t(e) * a.vcs + ci];
+        uint64_t bits = a.pc[uint64_t(e) * a.L.stride + a.L.coff + ci];
         lambda += __popcll(bits);
         while (bits) {
           const int b = ci * 64 + __ffsll(static_cast<long long>(bits)) - 1;
@@ -489,7 +489,7 @@ __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
         if (j0 + t < je) {
-          unsigned long long* base = reinterpret_cast<unsigned long long*>(a.pc + uint64_t(e[t]) * a.L.vpc);
+          unsigned long long* base = reinterpret_cast<unsigned long long*>(a.pc + uint64_t(e[t]) * a.L.stride);
           of[t] = atomicAdd(base + fwi, dec);
           ot[t] = atomicAdd(base + twi, incv);
         }
@@ -497,14 +497,14 @@ __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
         if (j0 + t >= je) break;
-        const int32_t w = a.nw[e[t]];
+        const int32_t w = int32_t(a.pc[uint64_t(e[t]) * a.L.stride + a.L.woff]);
         if (((of[t] >> fsh) & a.L.mask) == 1) {
-          atomicXor(reinterpret_cast<unsigned long long*>(&a.conn[uint64_t(e[t]) * a.vcs + from / 64]),
+          atomicXor(reinterpret_cast<unsigned long long*>(&a.pc[uint64_t(e[t]) * a.L.stride + a.L.coff + from / 64]),
                     1ull << (from % 64));
           du -= w;
         }
         if (((ot[t] >> tsh) & a.L.mask) == 0) {
-          atomicXor(reinterpret_cast<unsigned long long*>(&a.conn[uint64_t(e[t]) * a.vcs + to / 64]),
+          atomicXor(reinterpret_cast<unsigned long long*>(&a.pc[uint64_t(e[t]) * a.L.stride + a.L.coff + to / 64]),
                     1ull << (to % 64));
           du += w;
         }
@@ -543,9 +543,10 @@ __global__ void lp_verify_kernel(LpArgs a, const uint32_t* __restrict__ cu, cons
       uint32_t pt[NB], pf[NB];
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        w[t] = a.nw[e[t]];
-        pt[t] = pc_field(a.pc, a.L, e[t], twi, tsh);
-        pf[t] = pc_field(a.pc, a.L, e[t], fwi, fsh);
+        const uint64_t* r = a.pc + uint64_t(e[t]) * a.L.stride;  // one record per net
+        w[t] = int32_t(r[a.L.woff]);
+        pt[t] = uint32_t((r[twi] >> tsh) & a.L.mask);
+        pf[t] = uint32_t((r[fwi] >> fsh) & a.L.mask);
       }
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
@@ -615,7 +616,7 @@ __global__ void lp_border_flag_kernel(LpArgs a, uint32_t n, uint32_t* __restrict
     if (e - b <= kHighDegree) {
       for (uint32_t j = b; j < e && !border; ++j) {
         int lambda = 0;
-        for (int ci = 0; ci < a.vcs; ++ci) lambda += __popcll(a.conn[uint64_t(a.inc[j]) * a.vcs + ci]);
+        for (int ci = 0; ci < a.vcs; ++ci) lambda += __popcll(a.pc[uint64_t(a.inc[j]) * a.L.stride + a.L.coff + ci]);
         border = lambda > 1;
       }
     }
@@ -733,7 +734,7 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
   LpArgs a;
   a.voff = hg->voff.p; a.inc = hg->inc.p; a.noff = hg->noff.p; a.pins = hg->pins.p;
   a.vw = hg->vw.p; a.nw = hg->nw.p; a.part = p->part.p; a.part_w = p->part_w.p;
-  a.pc = p->pc.p; a.conn = p->conn.p; a.max_w = w.max_w.p; a.perfect_w = w.perfect_w.p;
+  a.pc = p->pc.p; a.max_w = w.max_w.p; a.perfect_w = w.perfect_w.p;
   a.L = p->L; a.vcs = p->vcs; a.k = batch ? batch->kmax : p->k; a.rebalancing = prm.rebalancing;
   a.seed = prm.seed;
   a.counters = w.counters.p;

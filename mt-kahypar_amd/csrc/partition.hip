@@ -24,8 +24,8 @@ __device__ inline uint32_t find_net(const uint32_t* off, int count, uint32_t pos
 __global__ void __launch_bounds__(kInitThreads)
 pin_count_init_kernel(const uint32_t* __restrict__ noff, const uint32_t* __restrict__ pins,
                       const int32_t* __restrict__ part, const int32_t* __restrict__ nw, uint32_t m, int k,
-                      PcLayout L, int vcs, int nets_per_block, uint64_t* __restrict__ pc,
-                      uint64_t* __restrict__ conn, int64_t* __restrict__ km1_acc) {
+                      PcLayout L, int vcs, int nets_per_block, uint64_t* __restrict__ rec,
+                      int64_t* __restrict__ km1_acc) {
   extern __shared__ uint64_t smem[];
   uint64_t* words = smem;                                                   // nets_per_block * vpc
   uint32_t* off = reinterpret_cast<uint32_t*>(words + size_t(nets_per_block) * L.vpc);  // nets_per_block + 1
@@ -45,8 +45,10 @@ pin_count_init_kernel(const uint32_t* __restrict__ noff, const uint32_t* __restr
     atomicAdd(reinterpret_cast<unsigned long long*>(&words[le * L.vpc + wi]), 1ull << sh);
   }
   __syncthreads();
-  uint64_t* out = pc + uint64_t(e0) * L.vpc;
-  for (int i = threadIdx.x; i < nwords; i += blockDim.x) out[i] = words[i];
+  for (int i = threadIdx.x; i < nwords; i += blockDim.x) {
+    const int le = i / L.vpc;
+    rec[uint64_t(e0 + le) * L.stride + (i - le * L.vpc)] = words[i];
+  }
   int64_t local = 0;
   for (int le = threadIdx.x; le < ne; le += blockDim.x) {
     int lambda = 0;
@@ -58,9 +60,10 @@ pin_count_init_kernel(const uint32_t* __restrict__ noff, const uint32_t* __restr
         const int sh = (blk - wi * L.epw) * L.b;
         if ((words[le * L.vpc + wi] >> sh) & L.mask) bits |= 1ull << (blk - bbeg);
       }
-      conn[uint64_t(e0 + le) * vcs + ci] = bits;
+      rec[uint64_t(e0 + le) * L.stride + L.coff + ci] = bits;
       lambda += __popcll(bits);
     }
+    rec[uint64_t(e0 + le) * L.stride + L.woff] = uint64_t(int64_t(nw[e0 + le]));
     local += int64_t(max(lambda - 1, 0)) * nw[e0 + le];
   }
   for (int o = 32; o > 0; o >>= 1) local += __shfl_down(local, o, 64);
@@ -85,12 +88,12 @@ __global__ void block_weight_kernel(const int32_t* __restrict__ part, const int3
     if (bins[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&part_w[i]), static_cast<unsigned long long>(bins[i]));
 }
 
-__global__ void km1_kernel(const uint64_t* __restrict__ conn, const int32_t* __restrict__ nw, uint32_t m, int vcs,
+__global__ void km1_kernel(const uint64_t* __restrict__ rec, const int32_t* __restrict__ nw, uint32_t m, PcLayout L,
                            int64_t* __restrict__ acc) {
   int64_t local = 0;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
     int lambda = 0;
-    for (int ci = 0; ci < vcs; ++ci) lambda += __popcll(conn[uint64_t(e) * vcs + ci]);
+    for (int ci = 0; ci < L.vcs; ++ci) lambda += __popcll(rec[uint64_t(e) * L.stride + L.coff + ci]);
     local += int64_t(max(lambda - 1, 0)) * nw[e];
   }
   for (int o = 32; o > 0; o >>= 1) local += __shfl_down(local, o, 64);
@@ -106,19 +109,29 @@ __global__ void unpack_pc_kernel(const uint64_t* __restrict__ pc, uint32_t m, in
     const int blk = int(i - e * k);
     const int wi = blk / L.epw;
     const int sh = (blk - wi * L.epw) * L.b;
-    out[i] = uint32_t((pc[e * L.vpc + wi] >> sh) & L.mask);
+    out[i] = uint32_t((pc[e * L.stride + wi] >> sh) & L.mask);
+  }
+}
+
+// connectivity words of every net, unpacked from the records
+__global__ void conn_words_kernel(const uint64_t* __restrict__ rec, uint32_t m, PcLayout L,
+                                  uint64_t* __restrict__ out) {
+  for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < uint64_t(m) * L.vcs;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t e = i / L.vcs;
+    out[i] = rec[e * L.stride + L.coff + (i - e * L.vcs)];
   }
 }
 
 __global__ void border_kernel(const uint32_t* __restrict__ voff, const uint32_t* __restrict__ inc,
-                              const uint64_t* __restrict__ conn, uint32_t n, int vcs, uint8_t* __restrict__ out) {
+                              const uint64_t* __restrict__ rec, uint32_t n, PcLayout L, uint8_t* __restrict__ out) {
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
     const uint32_t b = voff[u], e = voff[u + 1];
     uint8_t border = 0;
     if (e - b <= 100000u) {
       for (uint32_t j = b; j < e && !border; ++j) {
         int lambda = 0;
-        for (int ci = 0; ci < vcs; ++ci) lambda += __popcll(conn[uint64_t(inc[j]) * vcs + ci]);
+        for (int ci = 0; ci < L.vcs; ++ci) lambda += __popcll(rec[uint64_t(inc[j]) * L.stride + L.coff + ci]);
         border = lambda > 1;
       }
     }
@@ -148,13 +161,12 @@ void partition_initialize(mkp_partition* p) {
   }
   if (hg->m) {
     TimedRegion tr(c.prof, c.stream, kFamPinCountInit,
-                   4.0 * (hg->m + 1) + 8.0 * double(hg->p) + 8.0 * hg->m * (p->L.vpc + p->vcs),
+                   4.0 * (hg->m + 1) + 8.0 * double(hg->p) + 8.0 * hg->m * (p->L.vpc + p->vcs + 1),
                    "pin_count_init_kernel");
     const int npb = nets_per_block(p->L);
     const size_t lds = size_t(npb) * p->L.vpc * 8 + size_t(npb + 1) * 4;
     pin_count_init_kernel<<<(hg->m + npb - 1) / npb, kInitThreads, lds, c.stream>>>(
-        hg->noff.p, hg->pins.p, p->part.p, hg->nw.p, hg->m, p->k, p->L, p->vcs, npb, p->pc.p, p->conn.p,
-        p->km1_acc.p);
+        hg->noff.p, hg->pins.p, p->part.p, hg->nw.p, hg->m, p->k, p->L, p->vcs, npb, p->pc.p, p->km1_acc.p);
     MKP_LAUNCH_CHECK();
   }
 }
@@ -170,11 +182,11 @@ __global__ void block_weight_batch_kernel(const int32_t* __restrict__ part, cons
               static_cast<unsigned long long>(int64_t(vw[u])));
 }
 
-__global__ void km1_batch_kernel(const uint64_t* __restrict__ conn, const int32_t* __restrict__ nw, uint32_t m,
-                                 int vcs, const uint32_t* __restrict__ ngraph, int64_t* __restrict__ acc) {
+__global__ void km1_batch_kernel(const uint64_t* __restrict__ rec, const int32_t* __restrict__ nw, uint32_t m,
+                                 PcLayout L, const uint32_t* __restrict__ ngraph, int64_t* __restrict__ acc) {
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
     int lambda = 0;
-    for (int ci = 0; ci < vcs; ++ci) lambda += __popcll(conn[uint64_t(e) * vcs + ci]);
+    for (int ci = 0; ci < L.vcs; ++ci) lambda += __popcll(rec[uint64_t(e) * L.stride + L.coff + ci]);
     if (lambda > 1)
       atomicAdd(reinterpret_cast<unsigned long long*>(&acc[ngraph[e]]),
                 static_cast<unsigned long long>(int64_t(lambda - 1) * nw[e]));
@@ -195,8 +207,7 @@ void partition_initialize_batch(mkp_partition* p, const uint32_t* vgraph, const 
     const int npb = nets_per_block(p->L);
     const size_t lds = size_t(npb) * p->L.vpc * 8 + size_t(npb + 1) * 4;
     pin_count_init_kernel<<<(hg->m + npb - 1) / npb, kInitThreads, lds, c.stream>>>(
-        hg->noff.p, hg->pins.p, p->part.p, hg->nw.p, hg->m, p->k, p->L, p->vcs, npb, p->pc.p, p->conn.p,
-        p->km1_acc.p);
+        hg->noff.p, hg->pins.p, p->part.p, hg->nw.p, hg->m, p->k, p->L, p->vcs, npb, p->pc.p, p->km1_acc.p);
     MKP_LAUNCH_CHECK();
   }
 }
@@ -206,7 +217,7 @@ void partition_km1_batch(mkp_partition* p, const uint32_t* ngraph, uint32_t memb
   DevBuf<int64_t> acc(std::max<uint32_t>(members, 1));
   fill_async<int64_t>(acc.p, 0, members, c.stream);
   if (p->hg->m) {
-    km1_batch_kernel<<<grid_for(p->hg->m, 256, 65536), 256, 0, c.stream>>>(p->conn.p, p->hg->nw.p, p->hg->m, p->vcs,
+    km1_batch_kernel<<<grid_for(p->hg->m, 256, 65536), 256, 0, c.stream>>>(p->pc.p, p->hg->nw.p, p->hg->m, p->L,
                                                                          ngraph, acc.p);
     MKP_LAUNCH_CHECK();
   }
@@ -218,7 +229,7 @@ int64_t partition_km1(mkp_partition* p) {
   Context& c = p->hg->ctx->c;
   fill_async<int64_t>(p->km1_acc.p, 0, 1, c.stream);
   if (p->hg->m) {
-    km1_kernel<<<grid_for(p->hg->m, 256, 4096), 256, 0, c.stream>>>(p->conn.p, p->hg->nw.p, p->hg->m, p->vcs,
+    km1_kernel<<<grid_for(p->hg->m, 256, 4096), 256, 0, c.stream>>>(p->pc.p, p->hg->nw.p, p->hg->m, p->L,
                                                                     p->km1_acc.p);
     MKP_LAUNCH_CHECK();
   }
@@ -247,8 +258,7 @@ int mkp_partition_create(mkp_hypergraph* hg, int32_t k, mkp_partition** out) {
       p->L = pc_layout(k, std::max<uint32_t>(hg->max_esize, 1));
       p->part.alloc(std::max<uint32_t>(hg->n, 1));
       p->part_w.alloc(k);
-      p->pc.alloc(std::max<uint64_t>(uint64_t(hg->m) * p->L.vpc, 1));
-      p->conn.alloc(std::max<uint64_t>(uint64_t(hg->m) * p->vcs, 1));
+      p->pc.alloc(std::max<uint64_t>(uint64_t(hg->m) * p->L.stride, 1));
       p->km1_acc.alloc(1);
       fill_async<int32_t>(p->part.p, 0, hg->n, hg->ctx->c.stream);
       partition_initialize(p);
@@ -343,8 +353,13 @@ int mkp_partition_get_connectivity(const mkp_partition* p, uint64_t* words) {
     MKP_CHECK(p && words, "null argument");
     Bind bind_(p->hg->ctx->c);
     Context& c = p->hg->ctx->c;
-    if (p->hg->m)
-      MKP_HIP(hipMemcpyAsync(words, p->conn.p, uint64_t(p->hg->m) * p->vcs * 8, hipMemcpyDeviceToHost, c.stream));
+    const uint64_t total = uint64_t(p->hg->m) * p->vcs;
+    if (total) {
+      DevBuf<uint64_t> tmp(total);
+      conn_words_kernel<<<grid_for(total, 256, 65536), 256, 0, c.stream>>>(p->pc.p, p->hg->m, p->L, tmp.p);
+      MKP_LAUNCH_CHECK();
+      MKP_HIP(hipMemcpyAsync(words, tmp.p, total * 8, hipMemcpyDeviceToHost, c.stream));
+    }
     c.sync();
   });
 }
@@ -356,8 +371,8 @@ int mkp_partition_get_border(const mkp_partition* p, uint8_t* border) {
     Context& c = p->hg->ctx->c;
     if (!p->hg->n) return;
     DevBuf<uint8_t> tmp(p->hg->n);
-    border_kernel<<<grid_for(p->hg->n, 256, 65536), 256, 0, c.stream>>>(p->hg->voff.p, p->hg->inc.p, p->conn.p,
-                                                                        p->hg->n, p->vcs, tmp.p);
+    border_kernel<<<grid_for(p->hg->n, 256, 65536), 256, 0, c.stream>>>(p->hg->voff.p, p->hg->inc.p, p->pc.p,
+                                                                        p->hg->n, p->L, tmp.p);
     MKP_LAUNCH_CHECK();
     MKP_HIP(hipMemcpyAsync(border, tmp.p, p->hg->n, hipMemcpyDeviceToHost, c.stream));
     c.sync();

@@ -140,7 +140,6 @@ int mkp_request_estimate(uint32_t n, uint32_t m, uint64_t p, uint32_t max_net_si
     MKP_CHECK(k >= 2, "k must be at least 2");
     const double P = double(p), N = double(n), E = double(m), nodes = N + E, arcs = 2.0 * P;
     const PcLayout L = pc_layout(k, std::max<uint32_t>(max_net_size, 1));
-    const double vcs = double((k + 63) / 64);
     // resident hypergraph: CSR both ways, pin/incidence owners, net ranks and
     // rank-ordered pins, weights and communities
     const double hyper = 28.0 * P + 12.0 * N + 20.0 * E;
@@ -152,7 +151,7 @@ int mkp_request_estimate(uint32_t n, uint32_t m, uint64_t p, uint32_t max_net_si
     // total under the shrink bounds) and the rating and contraction work
     const double coarsen = hyper + 96.0 * P + 64.0 * N;
     // partition state and label propagation work at the finest level
-    const double partition = 8.0 * E * (double(L.vpc) + vcs) + 4.0 * N + 80.0 * N + 24.0 * P;
+    const double partition = 8.0 * E * double(L.stride) + 4.0 * N + 80.0 * N + 24.0 * P;
     const double need = hyper + std::max(louvain, coarsen + partition);
     // every buffer rounds up to a power of two in the allocation cache
     *bytes = uint64_t(2.0 * need) + (uint64_t(64) << 20);

@@ -50,10 +50,6 @@ MKP_API int mkp_context_reset_profile(mkp_context* ctx);
 MKP_API int mkp_context_get_profile(mkp_context* ctx, int family, double* device_ms, double* algorithmic_bytes,
                                     int64_t* launches);
 MKP_API const char* mkp_profile_family_name(int family);
-/* Per-kernel records: launches timed one at a time (an event pair around a
- * single kernel), keyed by kernel name and template variant, with their
- * summed device time and algorithmic bytes.  *name stays valid until the
- * next reset. */
 /* Restricts timing to the launches of one kernel (name as reported by
  * mkp_context_profile_kernel; NULL times everything), so that measuring one
  * kernel inside a timed region adds one event pair per launch of it only. */
@@ -63,6 +59,10 @@ MKP_API int mkp_context_set_profile_filter(mkp_context* ctx, const char* kernel_
  * default, lets the autotuner time the four on the first sub-rounds of a
  * level and keep the fastest per graph kind, class size and mean degree). */
 MKP_API int mkp_context_set_tuning(mkp_context* ctx, const char* key, int64_t value);
+/* Per-kernel records: launches timed one at a time (an event pair around a
+ * single kernel), keyed by kernel name and template variant, with their
+ * summed device time and algorithmic bytes.  *name stays valid until the
+ * next reset. */
 MKP_API int mkp_context_profile_kernel_count(mkp_context* ctx, int32_t* count);
 MKP_API int mkp_context_profile_kernel(mkp_context* ctx, int32_t index, const char** name, double* device_ms,
                                        double* algorithmic_bytes, int64_t* launches);

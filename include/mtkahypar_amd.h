@@ -54,10 +54,12 @@ MKP_API const char* mkp_profile_family_name(int family);
  * mkp_context_profile_kernel; NULL times everything), so that measuring one
  * kernel inside a timed region adds one event pair per launch of it only. */
 MKP_API int mkp_context_set_profile_filter(mkp_context* ctx, const char* kernel_name);
-/* Kernel shape choices that never change results, e.g. "louvain_group_lanes"
- * (lanes per node of the 17..128-arc Louvain class: 8, 16, 32 or 64; 0, the
- * default, lets the autotuner time the four on the first sub-rounds of a
- * level and keep the fastest per graph kind, class size and mean degree). */
+/* Choices that never change results: "louvain_group_lanes" (lanes per node
+ * of the 17..128-arc Louvain class: 8, 16, 32 or 64; 0, the default, lets the
+ * autotuner time the four on the first sub-rounds of a level and keep the
+ * fastest per graph kind, class size and mean degree) and "sleep_wait" (0,
+ * the default: host waits on the context's stream spin; 1: they poll for
+ * 20 us, then sleep between polls, leaving the core to other threads). */
 MKP_API int mkp_context_set_tuning(mkp_context* ctx, const char* key, int64_t value);
 /* Per-kernel records: launches timed one at a time (an event pair around a
  * single kernel), keyed by kernel name and template variant, with their
@@ -236,6 +238,9 @@ typedef struct {
   int32_t lp_max_rounds;           /* r-lp-maximum-iterations (5) */
   uint32_t lp_activation_threshold;/* r-lp-he-size-activation-threshold (100) */
   int32_t lp_rebalancing;          /* r-lp-rebalancing (1) */
+  int32_t ip_threads;              /* host threads of initial partitioning (4); the result does not depend on it */
+  double large_net_remove_factor;  /* maxnet-remove-factor (0.01): nets larger than max(N * it, 100, 50000) are
+                                      set aside during the multilevel phase */
 } mkp_partition_params;
 /* Initial partitioning of p's hypergraph (the coarsest level) on the host:
  * a pool of greedy-growing and random starts, each improved by k-way km1
@@ -252,13 +257,30 @@ typedef struct {
   int32_t levels;
   double louvain_ms, coarsening_ms, initial_ms, refinement_ms, total_ms;
   int64_t louvain_arcs, rating_visits, lp_visits;
+  int64_t removed_vertices, removed_nets;  /* degree-zero vertices and large nets set aside */
 } mkp_partition_stats;
 /* partition() of partitioner.cpp:163-204 with the fast preset (community
- * detection, multilevel coarsening, initial partitioning of the coarsest
- * hypergraph, LP at every level; FM and rebalancing not run).  parts (host,
- * num_vertices) receives the block ids. */
+ * detection, removal of degree-zero vertices and large nets, multilevel
+ * coarsening, initial partitioning of the coarsest hypergraph, LP at every
+ * level, restore of the removed vertices by bin packing; FM and rebalancing
+ * not run).  parts (host, num_vertices) receives the block ids. */
 MKP_API int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* params, int32_t* parts,
                                      mkp_partition_stats* stats);
+/* The removals of partition()'s sanitize step (partitioner.cpp:74-86):
+ * degree-zero vertices in id order while more than contraction_limit
+ * vertices remain (degree_zero_hn_remover.h:46-61) and nets with more than
+ * net_threshold pins (large_he_remover.h:47-58), as one compaction on the
+ * device.  *reduced receives the reduced hypergraph, owned by the caller
+ * (NULL when nothing was removed); vmap (host, num_vertices, may be NULL)
+ * the reduced id of every vertex, 0xffffffff for a removed one.  Kept
+ * vertices and nets keep their order and a net keeps its pin order;
+ * communities are carried along. */
+MKP_API int mkp_hypergraph_reduce(mkp_hypergraph* hg, uint32_t contraction_limit, uint32_t net_threshold,
+                                  mkp_hypergraph** reduced, uint32_t* vmap, uint32_t* removed_vertices,
+                                  uint32_t* removed_nets);
+/* max(max(num_vertices * factor, 100), 50000) (partitioner.cpp:37-38,
+ * large_he_remover.h:90-92): nets above it are removed. */
+MKP_API uint32_t mkp_large_net_threshold(uint32_t num_vertices, double factor);
 
 /* ---------------------------------------------------------- generator -- */
 typedef struct {
@@ -369,6 +391,8 @@ typedef struct {
   int32_t device;
   int32_t slots;              /* requests in flight at once (1..64) */
   uint64_t hbm_budget_bytes;  /* admission budget; 0 = device memory less 4 GiB */
+  int32_t spin_wait;          /* 0 (default): slot threads sleep while their stream runs; 1: they spin */
+  int32_t ip_threads;         /* initial-partitioning threads per request; 0 = the request's own (4) */
 } mkp_server_params;
 typedef struct {
   int64_t km1;

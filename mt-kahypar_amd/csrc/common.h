@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <thread>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -359,8 +361,29 @@ struct Context {
     }
     return reinterpret_cast<T*>(stage);
   }
+  // Host waits: spinning (hipStreamSynchronize) gives the lowest latency to
+  // one caller; with `sleep_wait` set, a wait polls the stream for 20 us and
+  // then sleeps between polls, so that threads waiting on several contexts
+  // leave the host cores to other work.
+  bool sleep_wait = false;
+  hipEvent_t ev_wait = nullptr;
+  void wait_stream() {
+    if (!sleep_wait || !ev_wait) {
+      MKP_HIP(hipStreamSynchronize(stream));
+      return;
+    }
+    MKP_HIP(hipEventRecord(ev_wait, stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+      const hipError_t q = hipEventQuery(ev_wait);
+      if (q == hipSuccess) return;
+      if (q != hipErrorNotReady) MKP_HIP(q);
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(20))
+        std::this_thread::sleep_for(std::chrono::microseconds(40));
+    }
+  }
   void sync() {
-    MKP_HIP(hipStreamSynchronize(stream));
+    wait_stream();
     if (!prof.open.empty()) prof.collect();
   }
   void* temp(size_t bytes) { tmp.ensure(bytes ? bytes : 1); return tmp.p; }

@@ -118,6 +118,7 @@ int mkp_context_create(int device, mkp_context** out) {
     MKP_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->c.pinned), 64 * sizeof(int64_t), hipHostMallocDefault));
     MKP_HIP(hipEventCreate(&ctx->c.ev_a));
     MKP_HIP(hipEventCreate(&ctx->c.ev_b));
+    MKP_HIP(hipEventCreateWithFlags(&ctx->c.ev_wait, hipEventDisableTiming));
     hipDeviceProp_t prop;
     MKP_HIP(hipGetDeviceProperties(&prop, device));
     ctx->c.num_cus = prop.multiProcessorCount;
@@ -139,6 +140,7 @@ int mkp_context_destroy(mkp_context* ctx) {
     if (ctx->c.stage) (void)hipHostFree(ctx->c.stage);
     if (ctx->c.ev_a) (void)hipEventDestroy(ctx->c.ev_a);
     if (ctx->c.ev_b) (void)hipEventDestroy(ctx->c.ev_b);
+    if (ctx->c.ev_wait) (void)hipEventDestroy(ctx->c.ev_wait);
     (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
   });
@@ -172,6 +174,9 @@ int mkp_context_set_tuning(mkp_context* ctx, const char* key, int64_t value) {
       MKP_CHECK(value == 0 || value == 8 || value == 16 || value == 32 || value == 64,
                 "louvain_group_lanes must be 0 (autotune), 8, 16, 32 or 64");
       ctx->c.tune.lv_group_lanes = int(value);
+    } else if (k == "sleep_wait") {
+      MKP_CHECK(value == 0 || value == 1, "sleep_wait must be 0 (spin) or 1 (poll, then sleep)");
+      ctx->c.sleep_wait = value != 0;
     } else {
       MKP_CHECK(false, "unknown tuning key");
     }

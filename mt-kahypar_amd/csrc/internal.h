@@ -1,6 +1,8 @@
 // Internal object layouts behind the opaque C handles, and the device-wide
 // primitives (rocPRIM radix sort and scans) shared by the kernel files.
 #pragma once
+#include <memory>
+
 #include "common.h"
 #include "../../include/mtkahypar_amd.h"
 
@@ -129,4 +131,19 @@ void louvain_batch(mkp_hypergraph* hg, const mkp_louvain_params& prm, int wt, co
 // per member and every member refines exactly as it would alone.
 void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, const uint32_t* nodes = nullptr,
                uint32_t num_nodes = 0, const LpBatch* batch = nullptr);
+
+// Degree-zero vertex and large-net removal (degree_zero_hn_remover.h:46-61,
+// large_he_remover.h:47-58) as one compaction: hg is null when nothing was
+// removed, vmap maps every vertex of the input to its reduced id
+// (0xffffffff: removed).
+struct Reduction {
+  std::unique_ptr<mkp_hypergraph> hg;
+  DevBuf<uint32_t> vmap;
+  uint32_t removed_vertices = 0, removed_nets = 0;
+};
+uint32_t large_net_threshold(uint32_t n, double factor);
+void reduce(mkp_hypergraph* hg, uint32_t contraction_limit, uint32_t net_threshold, Reduction& r);
+// Block ids of the input's vertices (host, n) from the reduced partition,
+// with the removed vertices bin-packed (degree_zero_hn_remover.h:64-83).
+void restore(mkp_hypergraph* hg, const Reduction& r, const mkp_partition* reduced, int32_t* parts);
 }  // namespace mkp

@@ -183,6 +183,8 @@ int mkp_partition_params_default(mkp_partition_params* p, int32_t k, double epsi
     p->lp_max_rounds = 5;
     p->lp_activation_threshold = 100;
     p->lp_rebalancing = 1;
+    p->ip_threads = 4;
+    p->large_net_remove_factor = 0.01;
   });
 }
 
@@ -217,11 +219,18 @@ int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm
     } else {
       fill_async<int32_t>(hg->comm.p, 0, hg->n, c.stream);
     }
+    // sanitize (partitioner.cpp:74-86): degree-zero vertices and large nets
+    // are set aside for the multilevel phase
+    Reduction red;
+    reduce(hg, climit, large_net_threshold(hg->n, prm->large_net_remove_factor), red);
+    st.removed_vertices = red.removed_vertices;
+    st.removed_nets = red.removed_nets;
+    mkp_hypergraph* work = red.hg ? red.hg.get() : hg;
     const double t1 = now_ms();
     mkp_coarsen_params cp{climit, int32_t(max_node_w), prm->min_shrink_factor, prm->max_shrink_factor,
                           prm->ignore_net_size, prm->rating_sampling, prm->seed,
                           prm->coarsening_sub_rounds > 0 ? prm->coarsening_sub_rounds : 16};
-    mkp_hierarchy* h = coarsen(hg, cp);
+    mkp_hierarchy* h = coarsen(work, cp);
     std::unique_ptr<mkp_hierarchy> hold(h);
     st.rating_visits = h->rating_visits;
     const double t2 = now_ms();
@@ -232,6 +241,7 @@ int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm
     ip.max_w.assign(k, maxw);
     ip.seed = prm->seed;
     if (prm->ip_runs > 0) ip.runs = prm->ip_runs;
+    if (prm->ip_threads > 0) ip.threads = prm->ip_threads;
     std::vector<int32_t> part = mt_kahypar_amd::initial_partition(download(coarsest), ip, nullptr);
     const double t3 = now_ms();
     std::vector<int64_t> maxv(k, maxw), perv(k, perfect);
@@ -257,7 +267,21 @@ int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm
       }
     }
     mkp_partition* top = phg[0].get();
-    MKP_HIP(hipMemcpyAsync(parts, top->part.p, size_t(hg->n) * 4, hipMemcpyDeviceToHost, c.stream));
+    std::unique_ptr<mkp_partition> full;
+    if (red.hg) {
+      // restore (partitioner.cpp:190-195): large nets return with their
+      // pins, the removed vertices are bin-packed into the lightest blocks
+      restore(hg, red, top, parts);
+      phg[0].reset();
+      mkp_partition* p = nullptr;
+      MKP_CHECK(mkp_partition_create(hg, k, &p) == 0, mkp_last_error());
+      full.reset(p);
+      MKP_HIP(hipMemcpyAsync(p->part.p, parts, size_t(hg->n) * 4, hipMemcpyHostToDevice, c.stream));
+      partition_initialize(p);
+      top = p;
+    } else {
+      MKP_HIP(hipMemcpyAsync(parts, top->part.p, size_t(hg->n) * 4, hipMemcpyDeviceToHost, c.stream));
+    }
     std::vector<int64_t> pw(k);
     MKP_HIP(hipMemcpyAsync(pw.data(), top->part_w.p, size_t(k) * 8, hipMemcpyDeviceToHost, c.stream));
     c.sync();

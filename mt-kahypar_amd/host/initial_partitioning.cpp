@@ -302,7 +302,7 @@ std::vector<uint8_t> bisect(const Sub& h, int64_t max0, int64_t max1, int64_t ta
   };
   // adaptive runs: batches of kBatch runs; another batch only while the last
   // one improved the best cut (batch bounds do not depend on the threads)
-  constexpr int kBatch = 5;
+  constexpr int kBatch = 3;
   int best = -1;
   for (int lo = 0; lo < runs; lo += kBatch) {
     const int hi = std::min(runs, lo + kBatch);

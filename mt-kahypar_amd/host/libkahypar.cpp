@@ -19,12 +19,12 @@
 struct mt_kahypar_context_s {
   mkp_partition_params prm{};  // fast preset; k, epsilon and seed are set per call
   int device = 0;
-  size_t host_threads = 0;
 };
 
 namespace {
 
 std::mutex g_lock;                 // partition() is serialised
+size_t g_host_threads = 0;         // mt_kahypar_initialize_thread_pool
 std::map<int, mkp_context*> g_ctx;  // one device context per GPU, kept until exit
 int g_return_on_error = 0;
 
@@ -82,6 +82,7 @@ void apply_option(mkp_partition_params& p, const std::string& key, const std::st
   else if (key == "c-t") p.contraction_limit_multiplier = uint32_t(std::stoul(v));
   else if (key == "c-vertex-degree-sampling-threshold") p.rating_sampling = std::stoull(v);
   else if (key == "i-runs") p.ip_runs = std::stoi(v);
+  else if (key == "maxnet-remove-factor") p.large_net_remove_factor = std::stod(v);
   else if (key == "r-lp-type") {
     if (v == "label_propagation_km1") p.lp_enabled = 1;
     else if (v == "do_nothing") p.lp_enabled = 0;
@@ -129,8 +130,10 @@ void mt_kahypar_configure_context_from_file(mt_kahypar_context_t* c, const char*
 }
 
 void mt_kahypar_initialize_thread_pool(const size_t num_threads, const bool) {
-  // The device path has no thread pool; the count is kept for the host phase.
-  (void)num_threads;
+  // The device path has no thread pool; the count bounds the host threads of
+  // initial partitioning.
+  std::lock_guard<std::mutex> guard(g_lock);
+  g_host_threads = num_threads;
 }
 
 void mt_kahypar_read_hypergraph_from_file(const char* file_name, mt_kahypar_hypernode_id_t* num_vertices,
@@ -197,6 +200,7 @@ void mt_kahypar_partition(const mt_kahypar_hypernode_id_t num_vertices, const mt
   prm.k = int32_t(num_blocks);
   prm.epsilon = epsilon;
   prm.seed = uint64_t(int64_t(seed));
+  if (g_host_threads > 0) prm.ip_threads = int32_t(std::min<size_t>(g_host_threads, 256));
   std::vector<int32_t> parts(std::max<uint32_t>(n, 1));
   mkp_partition_stats st{};
   mkp_hypergraph* hg = nullptr;

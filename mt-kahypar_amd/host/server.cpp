@@ -46,6 +46,7 @@ struct Request {
 
 struct mkp_server {
   uint64_t budget = 0;
+  int ip_threads = 0;
   std::vector<mkp_context*> ctx;
   std::vector<uint8_t> busy;
   std::vector<std::thread> workers;
@@ -152,7 +153,9 @@ int mkp_request_estimate(uint32_t n, uint32_t m, uint64_t p, uint32_t max_net_si
     const double coarsen = hyper + 96.0 * P + 64.0 * N;
     // partition state and label propagation work at the finest level
     const double partition = 8.0 * E * double(L.stride) + 4.0 * N + 80.0 * N + 24.0 * P;
-    const double need = hyper + std::max(louvain, coarsen + partition);
+    // with degree-zero vertices or large nets set aside, the multilevel phase
+    // runs on a reduced copy of the hypergraph
+    const double need = hyper + std::max(louvain, hyper + coarsen + partition);
     // every buffer rounds up to a power of two in the allocation cache
     *bytes = uint64_t(2.0 * need) + (uint64_t(64) << 20);
   });
@@ -168,8 +171,13 @@ int mkp_server_create(const mkp_server_params* prm, mkp_server** out) {
         mkp_context* c = nullptr;
         MKP_CHECK(mkp_context_create(prm->device, &c) == 0, mkp_last_error());
         s->ctx.push_back(c);
+        // the slots share the host cores with the initial partitioning of
+        // other requests: their waits sleep unless asked to spin
+        c->c.sleep_wait = prm->spin_wait == 0;
       }
       s->budget = prm->hbm_budget_bytes;
+      MKP_CHECK(prm->ip_threads >= 0 && prm->ip_threads <= 256, "ip_threads must be in [0, 256]");
+      s->ip_threads = prm->ip_threads;
       if (!s->budget) {
         size_t fr = 0, total = 0;
         MKP_HIP(hipSetDevice(prm->device));
@@ -211,6 +219,7 @@ int mkp_server_submit(mkp_server* s, const mkp_host_hypergraph* h, const mkp_par
     r->t_submit = Clock::now();
     r->h = h;
     r->prm = *prm;
+    if (s->ip_threads > 0) r->prm.ip_threads = s->ip_threads;
     uint32_t me = 0;
     for (uint32_t e = 0; e < h->h.m; ++e) me = std::max<uint32_t>(me, uint32_t(h->h.noff[e + 1] - h->h.noff[e]));
     MKP_CHECK(mkp_request_estimate(h->h.n, h->h.m, h->h.pins.size(), me, prm->k, &r->estimate) == 0,

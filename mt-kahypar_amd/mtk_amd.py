@@ -57,14 +57,15 @@ class PartitionParams(C.Structure):
                 ("min_shrink_factor", C.c_double), ("max_shrink_factor", C.c_double),
                 ("max_allowed_weight_multiplier", C.c_double), ("contraction_limit_multiplier", C.c_uint32),
                 ("ignore_net_size", C.c_uint32), ("rating_sampling", C.c_uint64), ("lp_enabled", C.c_int32),
-                ("lp_max_rounds", C.c_int32), ("lp_activation_threshold", C.c_uint32), ("lp_rebalancing", C.c_int32)]
+                ("lp_max_rounds", C.c_int32), ("lp_activation_threshold", C.c_uint32), ("lp_rebalancing", C.c_int32),
+                ("ip_threads", C.c_int32), ("large_net_remove_factor", C.c_double)]
 
 
 class PartitionStats(C.Structure):
     _fields_ = [("km1", C.c_int64), ("imbalance", C.c_double), ("levels", C.c_int32), ("louvain_ms", C.c_double),
                 ("coarsening_ms", C.c_double), ("initial_ms", C.c_double), ("refinement_ms", C.c_double),
                 ("total_ms", C.c_double), ("louvain_arcs", C.c_int64), ("rating_visits", C.c_int64),
-                ("lp_visits", C.c_int64)]
+                ("lp_visits", C.c_int64), ("removed_vertices", C.c_int64), ("removed_nets", C.c_int64)]
 
 
 class BatchLpParams(C.Structure):
@@ -73,7 +74,8 @@ class BatchLpParams(C.Structure):
 
 
 class ServerParams(C.Structure):
-    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("hbm_budget_bytes", C.c_uint64)]
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
+                ("spin_wait", C.c_int32), ("ip_threads", C.c_int32)]
 
 
 class RequestResult(C.Structure):
@@ -145,6 +147,8 @@ SIGNATURES = {
     "mkp_hierarchy_destroy": (C.c_int, [vp]),
     "mkp_partition_project": (C.c_int, [vp, vp, vp, C.c_int32]),
     "mkp_partition_hypergraph": (C.c_int, [vp, P(PartitionParams), vp, P(PartitionStats)]),
+    "mkp_hypergraph_reduce": (C.c_int, [vp, C.c_uint32, C.c_uint32, P(vp), vp, P(C.c_uint32), P(C.c_uint32)]),
+    "mkp_large_net_threshold": (C.c_uint32, [C.c_uint32, C.c_double]),
     "mkp_partition_params_default": (C.c_int, [P(PartitionParams), C.c_int32, C.c_double, C.c_uint64]),
     "mkp_initial_partition": (C.c_int, [vp, C.c_double, C.c_uint64, C.c_int32, P(C.c_int64)]),
     "mkp_generate_powerlaw": (C.c_int, [P(PowerlawParams), P(vp)]),
@@ -402,6 +406,19 @@ class Hypergraph:
         stats = {f: getattr(st, f) for f, _ in PartitionStats._fields_}
         return out[:self.info().num_vertices], stats
 
+    def reduce(self, contraction_limit, net_threshold):
+        """Degree-zero vertex and large-net removal: (reduced Hypergraph or
+        None when nothing was removed, vertex map with 0xffffffff for removed
+        vertices, removed vertices, removed nets)."""
+        n = self.info().num_vertices
+        out = vp()
+        vmap = np.zeros(max(n, 1), np.uint32)
+        rv, rn = C.c_uint32(), C.c_uint32()
+        check(lib().mkp_hypergraph_reduce(self.h, contraction_limit, net_threshold, C.byref(out), _p(vmap),
+                                          C.byref(rv), C.byref(rn)))
+        red = Hypergraph(self.ctx, handle=out) if out.value else None
+        return red, vmap[:n], rv.value, rn.value
+
     def arrays(self):
         i = self.info()
         noff = np.zeros(i.num_nets + 1, np.uint64); pins = np.zeros(max(i.num_pins, 1), np.uint32)
@@ -595,6 +612,10 @@ class BatchPartition:
             self.h = vp()
 
 
+def large_net_threshold(num_vertices, factor=0.01):
+    return int(lib().mkp_large_net_threshold(num_vertices, factor))
+
+
 def partition_params(k, epsilon, seed, **overrides):
     """Fast-preset parameter struct with keyword overrides."""
     prm = PartitionParams()
@@ -616,8 +637,8 @@ class Server:
     """Request loop of the served form: submit() queues a partition request,
     wait() returns its block ids and timings."""
 
-    def __init__(self, device=0, slots=4, hbm_budget_bytes=0):
-        prm = ServerParams(device, slots, hbm_budget_bytes)
+    def __init__(self, device=0, slots=4, hbm_budget_bytes=0, spin_wait=False, ip_threads=0):
+        prm = ServerParams(device, slots, hbm_budget_bytes, 1 if spin_wait else 0, ip_threads)
         self.h = vp()
         check(lib().mkp_server_create(C.byref(prm), C.byref(self.h)))
         self._hosts = {}

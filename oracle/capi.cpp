@@ -209,6 +209,27 @@ void* orc_hg_contract(void* h, const uint32_t* cluster, uint32_t* cmap) {
 }
 // Runs the coarsening driver and returns the number of levels; the coarsest
 // hypergraph is returned as a new handle in *coarsest (or null).
+// degree-zero vertex and large-net removal (sanitize.cpp)
+void* orc_reduce(void* h, uint32_t contraction_limit, uint32_t net_threshold, uint32_t* vmap, uint32_t* removed) {
+  const auto* hg = static_cast<const Hypergraph*>(h);
+  std::vector<uint32_t> vm;
+  Hypergraph r = reduce_hypergraph(*hg, contraction_limit, net_threshold, vm, removed[0], removed[1]);
+  std::copy(vm.begin(), vm.end(), vmap);
+  return new Hypergraph(std::move(r));
+}
+uint32_t orc_large_net_threshold(uint32_t n, double factor) { return large_net_threshold(n, factor); }
+// bin packing of removed vertices (ids in `removed`, weights indexed by id)
+void orc_restore_degree_zero(uint32_t nr, const uint32_t* removed, uint32_t n, const int32_t* weight, int k,
+                             int64_t* part_w, int32_t* part) {
+  std::vector<uint32_t> rm(removed, removed + nr);
+  std::vector<int32_t> w(weight, weight + n);
+  std::vector<int64_t> pw(part_w, part_w + k);
+  std::vector<int32_t> pt(part, part + n);
+  restore_degree_zero(rm, w, k, pw, pt);
+  std::copy(pw.begin(), pw.end(), part_w);
+  std::copy(pt.begin(), pt.end(), part);
+}
+
 int orc_coarsen(void* h, int sync, uint32_t contraction_limit, int32_t max_node_weight, double min_shrink,
                 double max_shrink, uint32_t ignore, uint64_t sampling, uint64_t seed, int sub_rounds,
                 int disable_rand, int prefer_unmatched, void** coarsest) {
@@ -222,10 +243,12 @@ int orc_coarsen(void* h, int sync, uint32_t contraction_limit, int32_t max_node_
 
 // ----------------------------------------------------------- pipeline --
 // partition (partitioner.cpp:163-204) with the fast preset: community
-// detection, multilevel coarsening, initial partitioning on the coarsest
-// hypergraph, uncoarsening with LP at every level (FM disabled).
+// detection, removal of degree-zero vertices and large nets, multilevel
+// coarsening, initial partitioning on the coarsest hypergraph, uncoarsening
+// with LP at every level (FM disabled), restore of the removed vertices.
 // times: louvain, coarsening, initial partitioning, refinement, total
-// counters: louvain arcs swept, lp incidences visited, levels, rating pin visits
+// counters: louvain arcs swept, lp incidences visited, levels, rating pin
+// visits, removed degree-zero vertices, removed large nets
 int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int threads, int32_t* part_out,
                       double* times, int64_t* counters) {
   const double t0 = now();
@@ -250,12 +273,19 @@ int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int t
     for (uint32_t u = 0; u < hg.n; ++u) hg.comm[u] = C[u];
   }
   const double t1 = now();
+  // sanitize (partitioner.cpp:74-86): degree-zero vertices and large nets
+  std::vector<uint32_t> vmap;
+  uint32_t removed_vertices = 0, removed_nets = 0;
+  Hypergraph red = reduce_hypergraph(hg, climit, large_net_threshold(hg.n, 0.01), vmap, removed_vertices,
+                                     removed_nets);
+  const bool reduced = removed_vertices > 0 || removed_nets > 0;
+  const Hypergraph& top = reduced ? red : hg;
   CoarsenParams cp;
   cp.contraction_limit = climit; cp.max_node_weight = max_node_w; cp.seed = seed; cp.threads = threads;
   int64_t rating_visits = 0;
-  std::vector<Level> levels = coarsen(hg, cp, sync != 0, &rating_visits);
+  std::vector<Level> levels = coarsen(top, cp, sync != 0, &rating_visits);
   const double t2 = now();
-  const Hypergraph& coarsest = levels.empty() ? hg : levels.back().hg;
+  const Hypergraph& coarsest = levels.empty() ? top : levels.back().hg;
   mt_kahypar_amd::InitialPartitioningParams ip;
   ip.k = k; ip.max_w.assign(k, maxw); ip.seed = seed;
   std::vector<int32_t> part = mt_kahypar_amd::initial_partition(to_host(coarsest), ip, nullptr);
@@ -273,11 +303,26 @@ int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int t
   };
   refine(coarsest, part, int(levels.size()));
   for (int i = int(levels.size()) - 1; i >= 0; --i) {
-    const Hypergraph& fine = (i == 0) ? hg : levels[i - 1].hg;
+    const Hypergraph& fine = (i == 0) ? top : levels[i - 1].hg;
     std::vector<int32_t> fp(fine.n);
     for (uint32_t u = 0; u < fine.n; ++u) fp[u] = part[levels[i].cmap[u]];
     part.swap(fp);
     refine(fine, part, i);
+  }
+  if (reduced) {
+    // restore (partitioner.cpp:190-195): large nets return with their pins,
+    // removed vertices are bin-packed
+    Partition p;
+    phg_init(p, top, k, part.data());
+    std::vector<int64_t> pw(p.part_w.begin(), p.part_w.end());
+    std::vector<int32_t> full(hg.n, -1);
+    std::vector<uint32_t> removed;
+    for (uint32_t u = 0; u < hg.n; ++u) {
+      if (vmap[u] != 0xffffffffu) full[u] = part[vmap[u]];
+      else removed.push_back(u);
+    }
+    restore_degree_zero(removed, hg.vw, k, pw, full);
+    part.swap(full);
   }
   const double t4 = now();
   Partition fin;
@@ -286,7 +331,7 @@ int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int t
   if (times) { times[0] = t1 - t0; times[1] = t2 - t1; times[2] = t3 - t2; times[3] = t4 - t3; times[4] = t4 - t0; }
   if (counters) {
     counters[0] = lst.arcs_swept; counters[1] = lp_visits; counters[2] = int64_t(levels.size());
-    counters[3] = rating_visits;
+    counters[3] = rating_visits; counters[4] = removed_vertices; counters[5] = removed_nets;
   }
   return km1(fin);
 }

@@ -198,4 +198,14 @@ struct Level { Hypergraph hg; std::vector<uint32_t> cmap; };
 std::vector<Level> coarsen(const Hypergraph& hg, const CoarsenParams& prm, bool sync,
                            int64_t* pin_visits);
 
+// ---------------------------------------------------- pre- and post-steps --
+// sanitize.cpp: degree-zero vertex and large-net removal as one compaction
+// (vmap: vertex -> reduced id, 0xffffffff if removed), and the bin-packing
+// restore of the removed vertices.
+uint32_t large_net_threshold(uint32_t n, double factor);
+Hypergraph reduce_hypergraph(const Hypergraph& hg, uint32_t contraction_limit, uint32_t net_threshold,
+                             std::vector<uint32_t>& vmap, uint32_t& removed_vertices, uint32_t& removed_nets);
+void restore_degree_zero(const std::vector<uint32_t>& removed, const std::vector<int32_t>& weight, int k,
+                         std::vector<int64_t>& part_w, std::vector<int32_t>& part);
+
 }  // namespace orc

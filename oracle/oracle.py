@@ -60,6 +60,9 @@ def lib():
             "orc_coarsen": (C.c_int, [vp, C.c_int, C.c_uint32, C.c_int32, C.c_double, C.c_double, C.c_uint32,
                                       C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
             "orc_partition": (C.c_int64, [vp, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int, i32p, f64p, i64p]),
+            "orc_reduce": (vp, [vp, C.c_uint32, C.c_uint32, u32p, u32p]),
+            "orc_large_net_threshold": (C.c_uint32, [C.c_uint32, C.c_double]),
+            "orc_restore_degree_zero": (None, [C.c_uint32, u32p, C.c_uint32, i32p, C.c_int, i64p, i32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -67,6 +70,20 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def large_net_threshold(n, factor=0.01):
+    return int(lib().orc_large_net_threshold(n, factor))
+
+
+def restore_degree_zero(removed, weight, k, part_w, part):
+    """Bin packing of the removed vertices; returns (part, part_w)."""
+    removed = np.ascontiguousarray(removed, np.uint32)
+    weight = np.ascontiguousarray(weight, np.int32)
+    pw = np.array(part_w, np.int64)
+    pt = np.array(part, np.int32)
+    lib().orc_restore_degree_zero(len(removed), removed, len(weight), weight, k, pw, pt)
+    return pt, pw
 
 
 def _ptr(a):
@@ -166,15 +183,26 @@ class Hypergraph:
                                    C.byref(out))
         return levels, (Hypergraph(0, None, None, handle=out.value) if out.value else None)
 
+    def reduce(self, contraction_limit, net_threshold):
+        """Degree-zero vertex and large-net removal: (reduced hypergraph,
+        vertex map with 0xffffffff for removed vertices, removed vertices,
+        removed nets)."""
+        n = self.sizes()[0]
+        vmap = np.zeros(n, np.uint32)
+        rem = np.zeros(2, np.uint32)
+        h = lib().orc_reduce(self.h, contraction_limit, net_threshold, vmap, rem)
+        return Hypergraph(0, None, None, handle=h), vmap, int(rem[0]), int(rem[1])
+
     def partition(self, k, eps, seed, sync, threads=1):
         n = self.sizes()[0]
         part = np.zeros(n, np.int32)
         times = np.zeros(5, np.float64)
-        cnt = np.zeros(4, np.int64)
+        cnt = np.zeros(6, np.int64)
         km1 = lib().orc_partition(self.h, k, eps, seed, int(sync), threads, part, times, cnt)
         return part, km1, dict(louvain_s=times[0], coarsen_s=times[1], ip_s=times[2], refine_s=times[3],
                                total_s=times[4], louvain_arcs=int(cnt[0]), lp_visits=int(cnt[1]),
-                               levels=int(cnt[2]), rating_visits=int(cnt[3]))
+                               levels=int(cnt[2]), rating_visits=int(cnt[3]), removed_vertices=int(cnt[4]),
+                               removed_nets=int(cnt[5]))
 
 
 class Graph:

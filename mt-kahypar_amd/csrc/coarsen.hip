@@ -352,6 +352,7 @@ struct HrArgs {
   uint64_t* ioff;        // exclusive scan of isize (count + 1)
   uint64_t* toff;        // exclusive scan of tsize (count + 1)
   uint32_t* inc_net;     // per swept incidence: net (kNone: ignored net)
+  uint32_t* inc_own;     // per swept incidence: index of its proposer in list
   uint64_t* inc_cnt;     // per swept incidence: items
   uint64_t* inc_off;     // exclusive scan of inc_cnt (incidences + 1)
   uint32_t* keys;
@@ -439,6 +440,7 @@ __global__ void hr_inc_kernel(CoArgs a, HrArgs h, uint64_t total, const uint32_t
     const uint32_t e = a.inc[j];
     const bool take = a.noff[e + 1] - a.noff[e] < a.ignore;
     h.inc_net[x] = take ? e : kNone;
+    h.inc_own[x] = i;
     h.inc_cnt[x] = take ? dcount[e] : 0;
   }
 }
@@ -451,25 +453,72 @@ __global__ void hr_clear_kernel(HrArgs h, uint64_t slots) {
   }
 }
 
-// item t belongs to the largest incidence x with inc_off[x] <= t, which owns
-// items: a later incidence without items would start at the same offset
-__global__ void hr_insert_kernel(CoArgs a, HrArgs h, uint64_t total_items, uint64_t total_inc,
-                                 const uint32_t* __restrict__ dclus, const uint32_t* __restrict__ dcount) {
-  for (uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; t < total_items;
-       t += uint64_t(gridDim.x) * blockDim.x) {
-    uint64_t lo = 0, hi = total_inc;
-    while (hi - lo > 1) {
-      const uint64_t mid = (lo + hi) >> 1;
-      if (h.inc_off[mid] <= t) lo = mid; else hi = mid;
+// Item t belongs to the largest incidence x with inc_off[x] <= t, which owns
+// items: a later incidence without items would start at the same offset.  A
+// block takes kHrItems consecutive items; one thread finds the first and
+// last incidence of the chunk in global memory, the offsets in between go to
+// LDS (relative to the chunk start), and every item finds its incidence
+// there, so an item costs one LDS search instead of a global one.
+constexpr uint32_t kHrItems = 2048;  // items per block and chunk
+constexpr uint32_t kHrSpan = 2048;   // incidences a chunk may span in LDS
+
+__device__ inline uint64_t hr_last_le(const uint64_t* off, uint64_t lo, uint64_t hi, uint64_t t) {
+  // largest x in [lo, hi) with off[x] <= t, given off[lo] <= t
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (off[mid] <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256)
+hr_insert_kernel(CoArgs a, HrArgs h, uint64_t total_items, uint64_t total_inc, const uint32_t* __restrict__ dclus,
+                 const uint32_t* __restrict__ dcount) {
+  __shared__ uint32_t s_off[kHrSpan];
+  __shared__ uint64_t s_x0, s_x1;
+  for (uint64_t t0 = blockIdx.x * uint64_t(kHrItems); t0 < total_items; t0 += uint64_t(gridDim.x) * kHrItems) {
+    const uint64_t t1 = min(t0 + kHrItems, total_items);
+    if (threadIdx.x == 0) {
+      const uint64_t x0 = hr_last_le(h.inc_off, 0, total_inc, t0);
+      s_x0 = x0;
+      s_x1 = hr_last_le(h.inc_off, x0, total_inc, t1 - 1);
     }
-    const uint32_t e = h.inc_net[lo];
-    const uint32_t q = uint32_t(t - h.inc_off[lo]);
-    const uint32_t i = hr_owner(h.ioff, h.count, lo);
-    const uint32_t d = dcount[e];
-    const uint64_t score = (uint64_t(uint32_t(a.nw[e])) << kFixShift) / uint64_t(max(d, 2u) - 1);
-    const uint64_t base = h.toff[i];
-    const int bits = __builtin_ctzll(h.toff[i + 1] - base);
-    co_map_add(h.keys + base, h.vals + base, bits, dclus[a.noff[e] + q], score);
+    __syncthreads();
+    const uint64_t x0 = s_x0, x1 = s_x1;
+    const uint64_t nx = x1 - x0 + 1;
+    const bool in_lds = nx <= kHrSpan;
+    if (in_lds) {
+      // x0 is the last incidence starting at or before t0: every later one
+      // starts after t0, so the relative offsets are exact except the first
+      for (uint32_t j = threadIdx.x; j < nx; j += blockDim.x) {
+        const uint64_t o = h.inc_off[x0 + j];
+        s_off[j] = o <= t0 ? 0u : uint32_t(o - t0);
+      }
+    }
+    __syncthreads();
+    for (uint64_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+      uint64_t x;
+      if (in_lds) {
+        const uint32_t rel = uint32_t(t - t0);
+        uint32_t lo = 0, hi = uint32_t(nx);
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_off[mid] <= rel) lo = mid; else hi = mid;
+        }
+        x = x0 + lo;
+      } else {
+        x = hr_last_le(h.inc_off, x0, x1 + 1, t);
+      }
+      const uint32_t e = h.inc_net[x];
+      const uint32_t q = uint32_t(t - h.inc_off[x]);
+      const uint32_t i = h.inc_own[x];
+      const uint32_t d = dcount[e];
+      const uint64_t score = (uint64_t(uint32_t(a.nw[e])) << kFixShift) / uint64_t(max(d, 2u) - 1);
+      const uint64_t base = h.toff[i];
+      const int bits = __builtin_ctzll(h.toff[i + 1] - base);
+      co_map_add(h.keys + base, h.vals + base, bits, dclus[a.noff[e] + q], score);
+    }
+    __syncthreads();
   }
 }
 
@@ -784,7 +833,7 @@ namespace mkp {
 // work arrays of the heavy-proposer rating, kept across sub-rounds
 struct HeavyRating {
   DevBuf<uint64_t> tsize, isize, ioff, toff, inc_cnt, inc_off, cval;
-  DevBuf<uint32_t> inc_net, keys, ct;
+  DevBuf<uint32_t> inc_net, inc_own, keys, ct;
   DevBuf<uint64_t> vals;
   DevBuf<int> cun;
 };
@@ -818,9 +867,10 @@ void rate_heavy(Context& c, const CoArgs& a, const uint32_t* list, const uint32_
   const uint64_t total_inc = pin64[0], slots = pin64[1];
   MKP_CHECK(slots < (uint64_t(1) << 32), "heavy rating tables exceed 2^32 slots");
   w.inc_net.ensure(std::max<uint64_t>(total_inc, 1));
+  w.inc_own.ensure(std::max<uint64_t>(total_inc, 1));
   w.inc_cnt.ensure(std::max<uint64_t>(total_inc, 1));
   w.inc_off.ensure(total_inc + 1);
-  h.inc_net = w.inc_net.p; h.inc_cnt = w.inc_cnt.p; h.inc_off = w.inc_off.p;
+  h.inc_net = w.inc_net.p; h.inc_own = w.inc_own.p; h.inc_cnt = w.inc_cnt.p; h.inc_off = w.inc_off.p;
   if (total_inc) {
     TimedRegion tr(c.prof, s, kFamRating, 0.0, "hr_inc_kernel");
     hr_inc_kernel<<<grid_for(total_inc, 256, 65536), 256, 0, s>>>(a, h, total_inc, dcount);
@@ -839,7 +889,8 @@ void rate_heavy(Context& c, const CoArgs& a, const uint32_t* list, const uint32_
   }
   if (total_items) {
     TimedRegion tr(c.prof, s, kFamRating, 0.0, "hr_insert_kernel");
-    hr_insert_kernel<<<grid_for(total_items, 256, 65536), 256, 0, s>>>(a, h, total_items, total_inc, dclus, dcount);
+    hr_insert_kernel<<<grid_for(total_items, kHrItems, 65536), 256, 0, s>>>(a, h, total_items, total_inc, dclus,
+                                                                             dcount);
   }
   MKP_LAUNCH_CHECK();
   const uint64_t chunks = slots / kHrChunk;

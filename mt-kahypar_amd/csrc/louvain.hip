@@ -625,8 +625,35 @@ __global__ void hv_decide_kernel(LvArgs a, HeavyArgs h, const HvBest* __restrict
   }
 }
 
+// Adds (dv, dn) to cvol[key] / csize[key] for every valid lane of the wave.
+// The lanes sharing the key of the first valid lane add their sum once (the
+// moves of neighbouring nodes often share a cluster); the others add alone.
+// Volumes are integer-valued under the integer arc weights of the star
+// expansion, so the order of the additions does not change them.
+__device__ inline void lv_volume_add(double* __restrict__ cvol, int32_t* __restrict__ csize, bool valid, int32_t key,
+                                     double dv, int dn) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t act = __ballot(valid);
+  const int leader = act ? __ffsll(static_cast<long long>(act)) - 1 : 0;
+  const int32_t lk = __shfl(key, leader, 64);
+  const bool mine = valid && key == lk;
+  double sum = mine ? dv : 0.0;
+  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+  const int cnt = __popcll(__ballot(mine));
+  if (act && lane == leader) {
+    atomicAdd(&cvol[lk], sum);
+    atomicAdd(&csize[lk], dn * cnt);
+  }
+  if (valid && !mine) {
+    atomicAdd(&cvol[key], dv);
+    atomicAdd(&csize[key], dn);
+  }
+}
+
 // Applies a sub-round's moves.  The move counter alternates between two
 // words: this launch clears the other one, which the next sub-round fills.
+// Whole waves step through the moves together (the volume and move-count
+// updates combine across lanes).
 __global__ void louvain_apply_kernel(const uint32_t* __restrict__ mu, const int32_t* __restrict__ mto,
                                      const uint32_t* __restrict__ mcount, const double* __restrict__ nvol,
                                      int32_t* __restrict__ C, double* __restrict__ cvol, int32_t* __restrict__ csize,
@@ -637,16 +664,33 @@ __global__ void louvain_apply_kernel(const uint32_t* __restrict__ mu, const int3
     atomicAdd(moved_total, cnt);
     *next_count = 0;
   }
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
-    const uint32_t u = mu[j];
-    const int32_t to = mto[j];
-    const int32_t from = C[u];
-    C[u] = to;
-    if (nmember) atomicAdd(&gmoved[nmember[u]], 1u);  // moves per member (batch)
-    atomicAdd(&cvol[to], nvol[u]);
-    atomicAdd(&cvol[from], -nvol[u]);
-    atomicAdd(&csize[to], 1);
-    atomicAdd(&csize[from], -1);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < cnt; base += stride) {
+    const uint32_t j = base + uint32_t(lane);
+    const bool valid = j < cnt;
+    uint32_t u = 0;
+    int32_t to = 0, from = 0;
+    double vu = 0.0;
+    if (valid) {
+      u = mu[j];
+      to = mto[j];
+      from = C[u];
+      C[u] = to;
+      vu = nvol[u];
+    }
+    if (nmember) {  // moves per member (batch): consecutive moves mostly share the member
+      const uint32_t g = valid ? nmember[u] : 0u;
+      const uint64_t act = __ballot(valid);
+      const int leader = act ? __ffsll(static_cast<long long>(act)) - 1 : 0;
+      const uint32_t lg = __shfl(g, leader, 64);
+      const bool mine = valid && g == lg;
+      const int c = __popcll(__ballot(mine));
+      if (act && lane == leader) atomicAdd(&gmoved[lg], uint32_t(c));
+      if (valid && !mine) atomicAdd(&gmoved[g], 1u);
+    }
+    lv_volume_add(cvol, csize, valid, to, vu, 1);
+    lv_volume_add(cvol, csize, valid, from, -vu, -1);
   }
 }
 
@@ -715,8 +759,11 @@ __device__ inline int lv_slot(const View& g, const LvArgs& a, uint32_t u) {
   return c * a.buckets + b;
 }
 
-// one node per thread; per-block LDS histograms keep global atomics to one
-// per (block, slot)
+// Per-block LDS histograms over a grid-stride range of nodes; a block flushes
+// once, so global atomics stay at one per (block, slot) for a grid of at most
+// kLvListBlocks blocks whatever the node count.
+constexpr unsigned kLvListBlocks = 2048;
+
 template <class View>
 __global__ void __launch_bounds__(256)
 lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uint64_t* __restrict__ sarcs,
@@ -727,14 +774,15 @@ lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uin
   const int slots = 4 * a.buckets;
   for (int i = threadIdx.x; i < slots; i += blockDim.x) { h[i] = 0; ha[i] = 0; hb[i] = 0; }
   __syncthreads();
-  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-  const int slot = u < n ? lv_slot(g, a, u) : -1;
-  if (slot >= 0) {
-    const uint32_t d = min(g.degree(u), a.sampling);
-    atomicAdd(&h[slot], 1u);
-    atomicAdd(&ha[slot], static_cast<unsigned long long>(d));
-    // algorithmic bytes of the node's move: 28 per node + its (sampled) arcs
-    if (sbytes) atomicAdd(&hb[slot], 28ull + uint64_t(d) * g.arc_bytes(u));
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const int slot = lv_slot(g, a, u);
+    if (slot >= 0) {
+      const uint32_t d = min(g.degree(u), a.sampling);
+      atomicAdd(&h[slot], 1u);
+      atomicAdd(&ha[slot], static_cast<unsigned long long>(d));
+      // algorithmic bytes of the node's move: 28 per node + its (sampled) arcs
+      if (sbytes) atomicAdd(&hb[slot], 28ull + uint64_t(d) * g.arc_bytes(u));
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < slots; i += blockDim.x) {
@@ -744,24 +792,29 @@ lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uin
   }
 }
 
+// Two passes over the block's nodes (the same grid-stride range): count per
+// slot, reserve the block's range of every slot with one global atomic, then
+// place the nodes in it.
 template <class View>
 __global__ void __launch_bounds__(256)
 lv_scatter_kernel(View g, LvArgs a, uint32_t n, const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
                   uint32_t* __restrict__ list) {
   __shared__ uint32_t h[4 * 256];
-  __shared__ uint32_t base[4 * 256];
   const int slots = 4 * a.buckets;
   for (int i = threadIdx.x; i < slots; i += blockDim.x) h[i] = 0;
   __syncthreads();
-  const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-  const int slot = u < n ? lv_slot(g, a, u) : -1;
-  uint32_t local = 0;
-  if (slot >= 0) local = atomicAdd(&h[slot], 1u);
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const int slot = lv_slot(g, a, u);
+    if (slot >= 0) atomicAdd(&h[slot], 1u);
+  }
   __syncthreads();
   for (int i = threadIdx.x; i < slots; i += blockDim.x)
-    if (h[i]) base[i] = offsets[i] + atomicAdd(&cursor[i], h[i]);
+    if (h[i]) h[i] = offsets[i] + atomicAdd(&cursor[i], h[i]);
   __syncthreads();
-  if (slot >= 0) list[base[slot] + local] = u;
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const int slot = lv_slot(g, a, u);
+    if (slot >= 0) list[atomicAdd(&h[slot], 1u)] = u;
+  }
 }
 
 __global__ void lv_init_kernel(uint32_t n, const double* __restrict__ nvol, int32_t* __restrict__ C,
@@ -1098,7 +1151,8 @@ struct LouvainRun {
       fill_async<uint64_t>(sarcs.p, 0ull, slots, s);
       const bool prof = c.prof.enabled;
       if (prof) fill_async<uint64_t>(sbytes.p, 0ull, slots, s);
-      lv_count_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, counts.p, sarcs.p, prof ? sbytes.p : nullptr);
+      lv_count_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, counts.p, sarcs.p,
+                                                                           prof ? sbytes.p : nullptr);
       MKP_LAUNCH_CHECK();
       // pinned staging: [sarcs u64][sbytes u64][counts u32][offsets u32]
       uint64_t* stg = c.staging<uint64_t>(4 * size_t(slots));
@@ -1122,7 +1176,7 @@ struct LouvainRun {
       for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
       std::copy(off.begin(), off.end(), stg_off);
       MKP_HIP(hipMemcpyAsync(offsets.p, stg_off, size_t(slots) * 4, hipMemcpyHostToDevice, s));
-      lv_scatter_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, offsets.p, cursor.p, list.p);
+      lv_scatter_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, offsets.p, cursor.p, list.p);
       MKP_LAUNCH_CHECK();
       if (hnodes) prepare_heavy(g, a, list.p + off[3 * B], hnodes);
       fill_async<uint32_t>(mcount.p, 0u, 2, s);
@@ -1479,7 +1533,8 @@ struct LouvainBatchRun : LouvainRun {
       fill_async<uint64_t>(sarcs.p, 0ull, slots, s);
       const bool prof = c.prof.enabled;
       if (prof) fill_async<uint64_t>(sbytes.p, 0ull, slots, s);
-      lv_count_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, counts.p, sarcs.p, prof ? sbytes.p : nullptr);
+      lv_count_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, counts.p, sarcs.p,
+                                                                           prof ? sbytes.p : nullptr);
       MKP_LAUNCH_CHECK();
       uint64_t* stg = c.staging<uint64_t>(4 * size_t(slots));
       uint32_t* stg_cnt = reinterpret_cast<uint32_t*>(stg + 2 * slots);
@@ -1502,7 +1557,7 @@ struct LouvainBatchRun : LouvainRun {
       for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
       std::copy(off.begin(), off.end(), stg_off);
       MKP_HIP(hipMemcpyAsync(offsets.p, stg_off, size_t(slots) * 4, hipMemcpyHostToDevice, s));
-      lv_scatter_kernel<View><<<(n + 255) / 256, 256, 0, s>>>(g, a, n, offsets.p, cursor.p, list.p);
+      lv_scatter_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, offsets.p, cursor.p, list.p);
       MKP_LAUNCH_CHECK();
       if (hnodes) prepare_heavy(g, a, list.p + off[3 * B], hnodes);
       fill_async<uint32_t>(mcount.p, 0u, 2, s);

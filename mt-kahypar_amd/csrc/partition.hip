@@ -174,22 +174,62 @@ void partition_initialize(mkp_partition* p) {
 // ------------------------------------------------------------- batches --
 // Members of a batch are disjoint parts of one hypergraph with local block
 // ids; block weights go to the member's own range of the weight array.
-__global__ void block_weight_batch_kernel(const int32_t* __restrict__ part, const int32_t* __restrict__ vw, uint32_t n,
-                                          const uint32_t* __restrict__ vgraph, const uint32_t* __restrict__ gkbase,
-                                          int64_t* __restrict__ part_w) {
-  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
-    atomicAdd(reinterpret_cast<unsigned long long*>(&part_w[gkbase[vgraph[u]] + uint32_t(part[u])]),
-              static_cast<unsigned long long>(int64_t(vw[u])));
+// A block takes kBwChunk consecutive vertices; they belong to few members
+// (members are contiguous vertex ranges), so the blocks they touch form a
+// short range of the weight array, summed in LDS and flushed once.
+constexpr uint32_t kBwChunk = 2048, kBwBins = 2048;
+
+__global__ void __launch_bounds__(256)
+block_weight_batch_kernel(const int32_t* __restrict__ part, const int32_t* __restrict__ vw, uint32_t n,
+                          const uint32_t* __restrict__ vgraph, const uint32_t* __restrict__ gkbase,
+                          int64_t* __restrict__ part_w) {
+  __shared__ int64_t bins[kBwBins];
+  for (uint32_t c0 = blockIdx.x * kBwChunk; c0 < n; c0 += gridDim.x * kBwChunk) {
+    const uint32_t c1 = min(c0 + kBwChunk, n);
+    const uint32_t lo = gkbase[vgraph[c0]], hi = gkbase[vgraph[c1 - 1] + 1];
+    const bool in_lds = hi - lo <= kBwBins;
+    if (in_lds)
+      for (uint32_t i = threadIdx.x; i < hi - lo; i += blockDim.x) bins[i] = 0;
+    __syncthreads();
+    for (uint32_t u = c0 + threadIdx.x; u < c1; u += blockDim.x) {
+      const uint32_t key = gkbase[vgraph[u]] + uint32_t(part[u]);
+      if (in_lds)
+        atomicAdd(reinterpret_cast<unsigned long long*>(&bins[key - lo]), static_cast<unsigned long long>(int64_t(vw[u])));
+      else
+        atomicAdd(reinterpret_cast<unsigned long long*>(&part_w[key]), static_cast<unsigned long long>(int64_t(vw[u])));
+    }
+    __syncthreads();
+    if (in_lds)
+      for (uint32_t i = threadIdx.x; i < hi - lo; i += blockDim.x)
+        if (bins[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&part_w[lo + i]), static_cast<unsigned long long>(bins[i]));
+    __syncthreads();
+  }
 }
 
+// km1 per member: nets of a member are contiguous, so the lanes of a wave
+// mostly share the member and add their sum with one atomic
 __global__ void km1_batch_kernel(const uint64_t* __restrict__ rec, const int32_t* __restrict__ nw, uint32_t m,
                                  PcLayout L, const uint32_t* __restrict__ ngraph, int64_t* __restrict__ acc) {
-  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
-    int lambda = 0;
-    for (int ci = 0; ci < L.vcs; ++ci) lambda += __popcll(rec[uint64_t(e) * L.stride + L.coff + ci]);
-    if (lambda > 1)
-      atomicAdd(reinterpret_cast<unsigned long long*>(&acc[ngraph[e]]),
-                static_cast<unsigned long long>(int64_t(lambda - 1) * nw[e]));
+  const int lane = threadIdx.x & 63;
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < m; base += gridDim.x * blockDim.x) {
+    const uint32_t e = base + uint32_t(lane);
+    const bool valid = e < m;
+    int64_t v = 0;
+    uint32_t g = 0;
+    if (valid) {
+      int lambda = 0;
+      for (int ci = 0; ci < L.vcs; ++ci) lambda += __popcll(rec[uint64_t(e) * L.stride + L.coff + ci]);
+      v = lambda > 1 ? int64_t(lambda - 1) * nw[e] : 0;
+      g = ngraph[e];
+    }
+    const uint64_t act = __ballot(valid);
+    const int leader = act ? __ffsll(static_cast<long long>(act)) - 1 : 0;
+    const uint32_t lg = __shfl(g, leader, 64);
+    const bool mine = valid && g == lg;
+    int64_t sum = mine ? v : 0;
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if (act && lane == leader && sum) atomicAdd(reinterpret_cast<unsigned long long*>(&acc[lg]), static_cast<unsigned long long>(sum));
+    if (valid && !mine && v) atomicAdd(reinterpret_cast<unsigned long long*>(&acc[g]), static_cast<unsigned long long>(v));
   }
 }
 
@@ -199,8 +239,8 @@ void partition_initialize_batch(mkp_partition* p, const uint32_t* vgraph, const 
   fill_async<int64_t>(p->part_w.p, 0, ktot, c.stream);
   fill_async<int64_t>(p->km1_acc.p, 0, 1, c.stream);
   if (hg->n) {
-    block_weight_batch_kernel<<<grid_for(hg->n, 256, 65536), 256, 0, c.stream>>>(p->part.p, hg->vw.p, hg->n, vgraph,
-                                                                               gkbase, p->part_w.p);
+    block_weight_batch_kernel<<<grid_for(hg->n, kBwChunk, 65536), 256, 0, c.stream>>>(p->part.p, hg->vw.p, hg->n,
+                                                                                     vgraph, gkbase, p->part_w.p);
     MKP_LAUNCH_CHECK();
   }
   if (hg->m) {  // pin counts need no member information: block ids are local

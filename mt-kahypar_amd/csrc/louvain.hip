@@ -759,9 +759,11 @@ __device__ inline int lv_slot(const View& g, const LvArgs& a, uint32_t u) {
   return c * a.buckets + b;
 }
 
-// Per-block LDS histograms over a grid-stride range of nodes; a block flushes
+// Per-block LDS histograms over a contiguous range of nodes; a block flushes
 // once, so global atomics stay at one per (block, slot) for a grid of at most
-// kLvListBlocks blocks whatever the node count.
+// kLvListBlocks blocks whatever the node count.  Contiguous ranges keep the
+// work lists in runs of neighbouring nodes, whose gathers share cache lines
+// in the move kernels.
 constexpr unsigned kLvListBlocks = 2048;
 
 template <class View>
@@ -774,7 +776,9 @@ lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uin
   const int slots = 4 * a.buckets;
   for (int i = threadIdx.x; i < slots; i += blockDim.x) { h[i] = 0; ha[i] = 0; hb[i] = 0; }
   __syncthreads();
-  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+  const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+  const uint32_t lo = min(blockIdx.x * per, n), hi = min(lo + per, n);
+  for (uint32_t u = lo + threadIdx.x; u < hi; u += blockDim.x) {
     const int slot = lv_slot(g, a, u);
     if (slot >= 0) {
       const uint32_t d = min(g.degree(u), a.sampling);
@@ -792,7 +796,7 @@ lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uin
   }
 }
 
-// Two passes over the block's nodes (the same grid-stride range): count per
+// Two passes over the block's nodes (the same contiguous range): count per
 // slot, reserve the block's range of every slot with one global atomic, then
 // place the nodes in it.
 template <class View>
@@ -803,7 +807,9 @@ lv_scatter_kernel(View g, LvArgs a, uint32_t n, const uint32_t* __restrict__ off
   const int slots = 4 * a.buckets;
   for (int i = threadIdx.x; i < slots; i += blockDim.x) h[i] = 0;
   __syncthreads();
-  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+  const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+  const uint32_t lo = min(blockIdx.x * per, n), hi = min(lo + per, n);
+  for (uint32_t u = lo + threadIdx.x; u < hi; u += blockDim.x) {
     const int slot = lv_slot(g, a, u);
     if (slot >= 0) atomicAdd(&h[slot], 1u);
   }
@@ -811,7 +817,7 @@ lv_scatter_kernel(View g, LvArgs a, uint32_t n, const uint32_t* __restrict__ off
   for (int i = threadIdx.x; i < slots; i += blockDim.x)
     if (h[i]) h[i] = offsets[i] + atomicAdd(&cursor[i], h[i]);
   __syncthreads();
-  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+  for (uint32_t u = lo + threadIdx.x; u < hi; u += blockDim.x) {
     const int slot = lv_slot(g, a, u);
     if (slot >= 0) list[atomicAdd(&h[slot], 1u)] = u;
   }
@@ -1007,10 +1013,14 @@ struct LouvainRun {
 
   // ----- autotuner of the lanes per node of the 17..128-arc class.  The
   // shapes compute the same result; in the first round of a level with an
-  // untuned key, sub-rounds 0..3 run them in turn with an event pair around
-  // the class launch, and the fastest per arc is cached in the context.
+  // untuned key, sub-rounds 0..7 run them in the order 8 16 32 64 64 32 16 8
+  // with an event pair around the class launch (mirrored, so that a drift of
+  // the per-arc cost over the round weighs on every shape alike), and the
+  // shape with the least time per arc over its two sub-rounds is cached in
+  // the context.  With fewer than 8 sub-rounds every shape runs once.
   static constexpr int kLaneTrials[4] = {8, 16, 32, 64};
-  static constexpr int kTrials = 4;
+  static constexpr int kTrials = 8;
+  static int trial_shape(int sr) { return sr < 4 ? sr : 7 - sr; }
   hipEvent_t tev_[2 * kTrials] = {};
   ~LouvainRun() {
     for (auto& e : tev_)
@@ -1019,6 +1029,7 @@ struct LouvainRun {
   struct LaneChoice {
     int lanes = 32;
     bool trial = false;
+    int trials = 0;  // sub-rounds that time a shape
     uint64_t key = 0;
   };
   template <class View>
@@ -1039,8 +1050,9 @@ struct LouvainRun {
       ch.lanes = it->second;
       return ch;
     }
-    ch.trial = B >= kTrials;
-    for (int i = 0; i < kTrials && ch.trial; ++i) ch.trial = cnt[B + i] != 0;
+    ch.trials = B >= kTrials ? kTrials : (B >= 4 ? 4 : 0);
+    ch.trial = ch.trials > 0;
+    for (int i = 0; i < ch.trials && ch.trial; ++i) ch.trial = cnt[B + i] != 0;
     if (ch.trial)
       for (auto& e : tev_)
         if (!e) MKP_HIP(hipEventCreate(&e));
@@ -1049,21 +1061,26 @@ struct LouvainRun {
   // after the round's synchronisation: keep the fastest shape per arc
   void finish_lanes(const LaneChoice& ch, const std::vector<uint64_t>& sa, int B) {
     if (!ch.trial) return;
-    int best = 1;
-    double best_t = 0;
-    for (int i = 0; i < kTrials; ++i) {
+    double ms_sum[4] = {0, 0, 0, 0}, arcs_sum[4] = {0, 0, 0, 0};
+    for (int i = 0; i < ch.trials; ++i) {
       float ms = 0;
       MKP_HIP(hipEventElapsedTime(&ms, tev_[2 * i], tev_[2 * i + 1]));
-      const double t = double(ms) / double(std::max<uint64_t>(sa[B + i], 1));
-      if (i == 0 || t < best_t) { best = i; best_t = t; }
+      ms_sum[trial_shape(i)] += double(ms);
+      arcs_sum[trial_shape(i)] += double(sa[B + i]);
+    }
+    int best = 1;
+    double best_t = 0;
+    for (int j = 0; j < 4; ++j) {
+      const double t = ms_sum[j] / std::max(arcs_sum[j], 1.0);
+      if (j == 0 || t < best_t) { best = j; best_t = t; }
     }
     c.tune.lv_group_choice[ch.key] = kLaneTrials[best];
   }
   int lanes_for(const LaneChoice& ch, int sr, hipEvent_t* ta, hipEvent_t* tb) {
-    if (ch.trial && sr < kTrials) {
+    if (ch.trial && sr < ch.trials) {
       *ta = tev_[2 * sr];
       *tb = tev_[2 * sr + 1];
-      return kLaneTrials[sr];
+      return kLaneTrials[trial_shape(sr)];
     }
     *ta = *tb = nullptr;
     return ch.lanes;

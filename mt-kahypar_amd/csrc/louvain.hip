@@ -736,15 +736,18 @@ __global__ void max_degree_kernel(View g, uint32_t n, uint32_t* __restrict__ deg
   if ((threadIdx.x & 63) == 0) atomicMax(deg_max, dm);
 }
 
-// degree classes: 0: <= 16, 1: <= 128, 2: <= 1024, 3: heavier; the work list
-// of a round is grouped by (class, bucket) so that every sub-round launch
-// covers exactly its own nodes.
+// degree classes: 0: <= 16 arcs, 1: <= 64, 2: <= 128, 3: <= 1024, 4: heavier;
+// the work list of a round is grouped by (class, bucket) so that every
+// sub-round launch covers exactly its own nodes.  Classes 1 and 2 run the
+// same group kernel with maps of 128 and 256 slots: the smaller maps of the
+// 17..64-arc nodes leave room in LDS for twice the waves.
+constexpr int kLvClasses = 5;
 // In a batch, nodes of members that sit the round out get no slot (-1), and
 // the bucket hash takes the member's seed and local id.
 template <class View>
 __device__ inline int lv_slot(const View& g, const LvArgs& a, uint32_t u) {
   const uint32_t d = min(g.degree(u), a.sampling);
-  const int c = d <= 16 ? 0 : (d <= 128 ? 1 : (d <= 1024 ? 2 : 3));
+  const int c = d <= 16 ? 0 : (d <= 64 ? 1 : (d <= 128 ? 2 : (d <= 1024 ? 3 : 4)));
   int b = 0;
   if (a.nmember) {
     const uint32_t gm = a.nmember[u];
@@ -770,10 +773,10 @@ template <class View>
 __global__ void __launch_bounds__(256)
 lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uint64_t* __restrict__ sarcs,
                 uint64_t* __restrict__ sbytes) {
-  __shared__ uint32_t h[4 * 256];
-  __shared__ unsigned long long ha[4 * 256];
-  __shared__ unsigned long long hb[4 * 256];
-  const int slots = 4 * a.buckets;
+  __shared__ uint32_t h[kLvClasses * 256];
+  __shared__ unsigned long long ha[kLvClasses * 256];
+  __shared__ unsigned long long hb[kLvClasses * 256];
+  const int slots = kLvClasses * a.buckets;
   for (int i = threadIdx.x; i < slots; i += blockDim.x) { h[i] = 0; ha[i] = 0; hb[i] = 0; }
   __syncthreads();
   const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
@@ -803,8 +806,8 @@ template <class View>
 __global__ void __launch_bounds__(256)
 lv_scatter_kernel(View g, LvArgs a, uint32_t n, const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
                   uint32_t* __restrict__ list) {
-  __shared__ uint32_t h[4 * 256];
-  const int slots = 4 * a.buckets;
+  __shared__ uint32_t h[kLvClasses * 256];
+  const int slots = kLvClasses * a.buckets;
   for (int i = threadIdx.x; i < slots; i += blockDim.x) h[i] = 0;
   __syncthreads();
   const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
@@ -1093,31 +1096,37 @@ struct LouvainRun {
                    hipEvent_t tb = nullptr) {
     const hipStream_t s = c.stream;
     const int B = a.buckets;
-    const int s0 = 0 * B + sr, s1 = 1 * B + sr, s2 = 2 * B + sr, s3 = 3 * B + sr;
+    const int s0 = 0 * B + sr, s1 = 1 * B + sr, s1w = 2 * B + sr, s2 = 3 * B + sr, s3 = 4 * B + sr;
     if (cnt[s0]) {
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s0]),
                      std::string("louvain_move_small_kernel<") + View::kName + ">");
       louvain_move_small_kernel<View><<<grid_for(cnt[s0], 256, 65536), 256, 0, s>>>(g, a, list + off[s0], cnt[s0]);
     }
-    if (cnt[s1]) {
+    if (cnt[s1]) {  // 17..64 arcs: 128-slot maps, lanes per node autotuned
       const int G = lanes;
       if (ta) MKP_HIP(hipEventRecord(ta, s));
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s1]),
-                     std::string("louvain_move_group_kernel<") + View::kName + ", " + std::to_string(G) + ", 8, " +
+                     std::string("louvain_move_group_kernel<") + View::kName + ", " + std::to_string(G) + ", 7, " +
                          (G == 8 ? "128>" : "256>"));
       if (G == 8)
-        louvain_move_group_kernel<View, 8, 8, 128><<<grid_for(cnt[s1], 16, 65536), 128, 0, s>>>(g, a, list + off[s1],
+        louvain_move_group_kernel<View, 8, 7, 128><<<grid_for(cnt[s1], 16, 65536), 128, 0, s>>>(g, a, list + off[s1],
                                                                                                 cnt[s1]);
       else if (G == 16)
-        louvain_move_group_kernel<View, 16, 8, 256><<<grid_for(cnt[s1], 16, 65536), 256, 0, s>>>(g, a, list + off[s1],
+        louvain_move_group_kernel<View, 16, 7, 256><<<grid_for(cnt[s1], 16, 65536), 256, 0, s>>>(g, a, list + off[s1],
                                                                                                  cnt[s1]);
       else if (G == 64)
-        louvain_move_group_kernel<View, 64, 8, 256><<<grid_for(cnt[s1], 4, 65536), 256, 0, s>>>(g, a, list + off[s1],
+        louvain_move_group_kernel<View, 64, 7, 256><<<grid_for(cnt[s1], 4, 65536), 256, 0, s>>>(g, a, list + off[s1],
                                                                                                 cnt[s1]);
       else
-        louvain_move_group_kernel<View, 32, 8, 256><<<grid_for(cnt[s1], 8, 65536), 256, 0, s>>>(g, a, list + off[s1],
+        louvain_move_group_kernel<View, 32, 7, 256><<<grid_for(cnt[s1], 8, 65536), 256, 0, s>>>(g, a, list + off[s1],
                                                                                                 cnt[s1]);
       if (tb) MKP_HIP(hipEventRecord(tb, s));
+    }
+    if (cnt[s1w]) {  // 65..128 arcs: 256-slot maps
+      TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s1w]),
+                     std::string("louvain_move_group_kernel<") + View::kName + ", 32, 8, 256>");
+      louvain_move_group_kernel<View, 32, 8, 256><<<grid_for(cnt[s1w], 8, 65536), 256, 0, s>>>(
+          g, a, list + off[s1w], cnt[s1w]);
     }
     if (cnt[s2]) {
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s2]),
@@ -1125,7 +1134,7 @@ struct LouvainRun {
       louvain_move_group_kernel<View, 64, 11, 64><<<grid_for(cnt[s2], 1, 65536), 64, 0, s>>>(g, a, list + off[s2],
                                                                                              cnt[s2]);
     }
-    if (cnt[s3]) launch_heavy(g, a, list + off[s3], off[s3] - off[3 * B], cnt[s3], sa[s3], double(sb[s3]));
+    if (cnt[s3]) launch_heavy(g, a, list + off[s3], off[s3] - off[4 * B], cnt[s3], sa[s3], double(sb[s3]));
     MKP_LAUNCH_CHECK();
   }
 
@@ -1135,7 +1144,7 @@ struct LouvainRun {
                     int32_t* C) {
     const hipStream_t s = c.stream;
     const int B = std::max(1, prm.sub_rounds);
-    const int slots = 4 * B;
+    const int slots = kLvClasses * B;
     cvol_.alloc(std::max<uint32_t>(n, 1));
     DevBuf<double>& cvol = cvol_;
     DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
@@ -1185,8 +1194,8 @@ struct LouvainRun {
       uint64_t harcs = 0;
       uint32_t hnodes = 0;
       for (int sr = 0; sr < B; ++sr) {
-        harcs = std::max(harcs, sa[3 * B + sr]);
-        hnodes += cnt[3 * B + sr];
+        harcs = std::max(harcs, sa[4 * B + sr]);
+        hnodes += cnt[4 * B + sr];
       }
       if (hnodes) reserve_heavy(harcs, hnodes);
       uint32_t run = 0;
@@ -1195,7 +1204,7 @@ struct LouvainRun {
       MKP_HIP(hipMemcpyAsync(offsets.p, stg_off, size_t(slots) * 4, hipMemcpyHostToDevice, s));
       lv_scatter_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, offsets.p, cursor.p, list.p);
       MKP_LAUNCH_CHECK();
-      if (hnodes) prepare_heavy(g, a, list.p + off[3 * B], hnodes);
+      if (hnodes) prepare_heavy(g, a, list.p + off[4 * B], hnodes);
       fill_async<uint32_t>(mcount.p, 0u, 2, s);
       const LaneChoice lanes = choose_lanes<View>(cnt, sa, B);
       for (int sr = 0; sr < B; ++sr) {
@@ -1205,7 +1214,8 @@ struct LouvainRun {
         const int lanes_now = lanes_for(lanes, sr, &ta, &tb);
         launch_move(g, a, list.p, off, cnt, sa, sb, sr, lanes_now, ta, tb);
         // at most one move per node of this sub-round
-        const uint32_t sub_nodes = cnt[sr] + cnt[B + sr] + cnt[2 * B + sr] + cnt[3 * B + sr];
+        uint32_t sub_nodes = 0;
+        for (int cl = 0; cl < kLvClasses; ++cl) sub_nodes += cnt[cl * B + sr];
         louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(
             mu.p, mto.p, a.move_count, nvol, C, cvol.p, csize.p, moved_total.p, mcount.p + ((sr + 1) & 1),
             nullptr, nullptr);
@@ -1511,7 +1521,7 @@ struct LouvainBatchRun : LouvainRun {
     const hipStream_t s = c.stream;
     const uint32_t n = L.n;
     const int B = std::max(1, prm.sub_rounds);
-    const int slots = 4 * B;
+    const int slots = kLvClasses * B;
     changed.assign(G, 0);
     cvol_.alloc(std::max<uint32_t>(n, 1));
     DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
@@ -1566,8 +1576,8 @@ struct LouvainBatchRun : LouvainRun {
       uint64_t harcs = 0;
       uint32_t hnodes = 0;
       for (int sr = 0; sr < B; ++sr) {
-        harcs = std::max(harcs, sa[3 * B + sr]);
-        hnodes += cnt[3 * B + sr];
+        harcs = std::max(harcs, sa[4 * B + sr]);
+        hnodes += cnt[4 * B + sr];
       }
       if (hnodes) reserve_heavy(harcs, hnodes);
       uint32_t run = 0;
@@ -1576,7 +1586,7 @@ struct LouvainBatchRun : LouvainRun {
       MKP_HIP(hipMemcpyAsync(offsets.p, stg_off, size_t(slots) * 4, hipMemcpyHostToDevice, s));
       lv_scatter_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, offsets.p, cursor.p, list.p);
       MKP_LAUNCH_CHECK();
-      if (hnodes) prepare_heavy(g, a, list.p + off[3 * B], hnodes);
+      if (hnodes) prepare_heavy(g, a, list.p + off[4 * B], hnodes);
       fill_async<uint32_t>(mcount.p, 0u, 2, s);
       const LaneChoice lanes = choose_lanes<View>(cnt, sa, B);
       for (int sr = 0; sr < B; ++sr) {
@@ -1585,7 +1595,8 @@ struct LouvainBatchRun : LouvainRun {
         hipEvent_t ta, tb;
         const int lanes_now = lanes_for(lanes, sr, &ta, &tb);
         launch_move(g, a, list.p, off, cnt, sa, sb, sr, lanes_now, ta, tb);
-        const uint32_t sub_nodes = cnt[sr] + cnt[B + sr] + cnt[2 * B + sr] + cnt[3 * B + sr];
+        uint32_t sub_nodes = 0;
+        for (int cl = 0; cl < kLvClasses; ++cl) sub_nodes += cnt[cl * B + sr];
         louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(
             mu.p, mto.p, a.move_count, nvol, C, cvol_.p, csize.p, moved_total.p, mcount.p + ((sr + 1) & 1),
             L.nmember.p, gmoved_.p);

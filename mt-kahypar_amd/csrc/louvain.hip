@@ -464,88 +464,109 @@ __device__ inline uint32_t hv_node_of(const uint64_t* aoff, uint32_t count, uint
 }
 
 // aoff / toff hold offsets of the whole round's heavy nodes; a sub-round's
-// arcs and table slots are counted from its first node's offsets.
+// arcs and table slots are counted from its first node's offsets.  A block
+// takes kHvArcs consecutive arcs: every heavy node has more, so the chunk
+// spans at most two nodes, found with one search per block.
+constexpr uint32_t kHvArcs = 1024;
+
+__device__ inline uint32_t hv_chunk_node(const HeavyArgs& h, uint64_t a0, uint64_t c0, uint32_t* s_node) {
+  if (threadIdx.x == 0) *s_node = hv_node_of(h.aoff, h.count, a0 + c0);
+  __syncthreads();
+  return *s_node;
+}
+
 template <class View>
 __global__ void __launch_bounds__(256) hv_insert_kernel(View g, LvArgs a, HeavyArgs h) {
+  __shared__ uint32_t s_node;
   const uint64_t a0 = h.aoff[0], t0 = h.toff[0];
   const uint64_t total = h.aoff[h.count] - a0;
   const uint64_t ep = uint64_t(h.epoch) << 32;
-  for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < total; j += uint64_t(gridDim.x) * blockDim.x) {
-    const uint32_t i = hv_node_of(h.aoff, h.count, a0 + j);
-    const uint32_t u = h.list[i];
-    uint32_t v;
-    double w;
-    g.arc(u, uint32_t(a0 + j - h.aoff[i]), v, w);
-    const uint32_t c = uint32_t(a.C[v]);
-    const uint64_t base = h.toff[i] - t0;
-    const uint32_t T = uint32_t(h.toff[i + 1] - h.toff[i]);
-    const int tbits = __builtin_ctz(T);
-    const uint64_t key = ep | c;
-    uint32_t s = slot_hash(int32_t(c), tbits);
-    // a key of the current epoch never changes, so a plain read that shows
-    // one is exact; anything else is settled by the CAS
-    uint64_t cur = h.keys[base + s];
-    while (true) {
-      if ((cur & 0xffffffff00000000ull) == ep) {
-        if (uint32_t(cur) == c) break;
-        s = (s + 1) & (T - 1);
-        cur = h.keys[base + s];
-        continue;
+  for (uint64_t c0 = blockIdx.x * uint64_t(kHvArcs); c0 < total; c0 += uint64_t(gridDim.x) * kHvArcs) {
+    const uint32_t i0 = hv_chunk_node(h, a0, c0, &s_node);
+    const uint64_t c1 = min(c0 + kHvArcs, total), split = h.aoff[i0 + 1] - a0;
+    for (uint64_t j = c0 + threadIdx.x; j < c1; j += blockDim.x) {
+      const uint32_t i = j < split ? i0 : i0 + 1;
+      const uint32_t u = h.list[i];
+      uint32_t v;
+      double w;
+      g.arc(u, uint32_t(a0 + j - h.aoff[i]), v, w);
+      const uint32_t c = uint32_t(a.C[v]);
+      const uint64_t base = h.toff[i] - t0;
+      const uint32_t T = uint32_t(h.toff[i + 1] - h.toff[i]);
+      const int tbits = __builtin_ctz(T);
+      const uint64_t key = ep | c;
+      uint32_t s = slot_hash(int32_t(c), tbits);
+      // a key of the current epoch never changes, so a plain read that shows
+      // one is exact; anything else is settled by the CAS
+      uint64_t cur = h.keys[base + s];
+      while (true) {
+        if ((cur & 0xffffffff00000000ull) == ep) {
+          if (uint32_t(cur) == c) break;
+          s = (s + 1) & (T - 1);
+          cur = h.keys[base + s];
+          continue;
+        }
+        const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(&h.keys[base + s]),
+                                       static_cast<unsigned long long>(cur), static_cast<unsigned long long>(key));
+        if (old == cur) {
+          h.wsum[base + s] = 0.0;
+          h.first[base + s] = 0xffffffffu;
+          break;
+        }
+        cur = old;
       }
-      const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(&h.keys[base + s]),
-                                     static_cast<unsigned long long>(cur), static_cast<unsigned long long>(key));
-      if (old == cur) {
-        h.wsum[base + s] = 0.0;
-        h.first[base + s] = 0xffffffffu;
-        break;
-      }
-      cur = old;
+      h.arc_slot[j] = uint32_t(base + s);
     }
-    h.arc_slot[j] = uint32_t(base + s);
+    __syncthreads();  // s_node is rewritten for the next chunk
   }
 }
 
 template <class View>
 __global__ void __launch_bounds__(256) hv_accum_kernel(View g, LvArgs a, HeavyArgs h) {
+  __shared__ uint32_t s_node;
   const uint64_t a0 = h.aoff[0];
   const uint64_t total = h.aoff[h.count] - a0;
   const int lane = threadIdx.x & 63;
-  // whole waves iterate together (the aggregation shuffles need every lane)
-  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-  for (uint64_t j0 = blockIdx.x * uint64_t(blockDim.x) + (threadIdx.x & ~63u); j0 < total; j0 += stride) {
-    const uint64_t j = j0 + lane;
-    bool pending = j < total;
-    uint32_t slot = 0xffffffffu, pos = 0;
-    double w = 0.0;
-    if (pending) {
-      const uint32_t i = hv_node_of(h.aoff, h.count, a0 + j);
-      pos = uint32_t(a0 + j - h.aoff[i]);
-      uint32_t v;
-      g.arc(h.list[i], pos, v, w);
-      slot = h.arc_slot[j];
-    }
-    for (int it = 0; it < 2; ++it) {  // combine lanes of the lowest pending slot
-      const uint64_t act = __ballot(pending);
-      if (!act) break;
-      const int leader = __ffsll(static_cast<long long>(act)) - 1;
-      const uint32_t ls = __shfl(slot, leader, 64);
-      const bool mine = pending && slot == ls;
-      double sum = mine ? w : 0.0;
-      uint32_t mn = mine ? pos : 0xffffffffu;
-      for (int o = 32; o > 0; o >>= 1) {
-        sum += __shfl_xor(sum, o, 64);
-        mn = min(mn, __shfl_xor(mn, o, 64));
+  for (uint64_t c0 = blockIdx.x * uint64_t(kHvArcs); c0 < total; c0 += uint64_t(gridDim.x) * kHvArcs) {
+    const uint32_t i0 = hv_chunk_node(h, a0, c0, &s_node);
+    const uint64_t c1 = min(c0 + kHvArcs, total), split = h.aoff[i0 + 1] - a0;
+    // whole waves iterate together (the aggregation shuffles need every lane)
+    for (uint64_t j0 = c0 + (threadIdx.x & ~63u); j0 < c1; j0 += blockDim.x) {
+      const uint64_t j = j0 + lane;
+      bool pending = j < c1;
+      uint32_t slot = 0xffffffffu, pos = 0;
+      double w = 0.0;
+      if (pending) {
+        const uint32_t i = j < split ? i0 : i0 + 1;
+        pos = uint32_t(a0 + j - h.aoff[i]);
+        uint32_t v;
+        g.arc(h.list[i], pos, v, w);
+        slot = h.arc_slot[j];
       }
-      if (lane == leader) {
-        atomicAdd(&h.wsum[ls], sum);
-        atomicMin(&h.first[ls], mn);
+      for (int it = 0; it < 2; ++it) {  // combine lanes of the lowest pending slot
+        const uint64_t act = __ballot(pending);
+        if (!act) break;
+        const int leader = __ffsll(static_cast<long long>(act)) - 1;
+        const uint32_t ls = __shfl(slot, leader, 64);
+        const bool mine = pending && slot == ls;
+        double sum = mine ? w : 0.0;
+        uint32_t mn = mine ? pos : 0xffffffffu;
+        for (int o = 32; o > 0; o >>= 1) {
+          sum += __shfl_xor(sum, o, 64);
+          mn = min(mn, __shfl_xor(mn, o, 64));
+        }
+        if (lane == leader) {
+          atomicAdd(&h.wsum[ls], sum);
+          atomicMin(&h.first[ls], mn);
+        }
+        if (mine) pending = false;
       }
-      if (mine) pending = false;
+      if (pending) {
+        atomicAdd(&h.wsum[slot], w);
+        atomicMin(&h.first[slot], pos);
+      }
     }
-    if (pending) {
-      atomicAdd(&h.wsum[slot], w);
-      atomicMin(&h.first[slot], pos);
-    }
+    __syncthreads();  // s_node is rewritten for the next chunk
   }
 }
 
@@ -991,7 +1012,7 @@ struct LouvainRun {
                     uint64_t arcs, double bytes) {
     const hipStream_t s = c.stream;
     HeavyArgs h{list, count, haoff_.p + first, htoff_.p + first, hkeys_.p, hw_.p, hf_.p, hslot_.p, ++epoch_};
-    const unsigned grid = grid_for(arcs, 256, unsigned(c.num_cus) * 16);
+    const unsigned grid = grid_for(arcs, kHvArcs, unsigned(c.num_cus) * 16);
     {
       // the compulsory bytes of the heavy moves (node data, head id and
       // cluster gather per arc) are attributed to the gather pass

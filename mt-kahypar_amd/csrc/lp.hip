@@ -128,6 +128,32 @@ __device__ inline void counter_add(int64_t* c, int64_t v) {
   if (v) atomicAdd(reinterpret_cast<unsigned long long*>(c), static_cast<unsigned long long>(v));
 }
 
+// Adds v to arr[idx] for every lane with on set.  Up to `passes` times, the
+// lanes that share the index of the first pending lane add their sum with one
+// atomic (consecutive moves and vertices mostly share a member, and a move
+// often a block); the rest add alone.  Every lane of the wave calls it.
+__device__ inline void wave_add_i64(int64_t* arr, bool on, uint64_t idx, int64_t v, int passes) {
+  const int lane = threadIdx.x & 63;
+  bool pending = on && v != 0;
+  for (int it = 0; it < passes; ++it) {
+    const uint64_t act = __ballot(pending);
+    if (!act) return;
+    const int leader = __ffsll(static_cast<long long>(act)) - 1;
+    const uint64_t li = __shfl(idx, leader, 64);
+    const bool mine = pending && idx == li;
+    int64_t sum = mine ? v : 0;
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    if (lane == leader) counter_add(arr + li, sum);
+    if (mine) pending = false;
+  }
+  if (pending) counter_add(arr + idx, v);
+}
+
+// counter `slot` of member g (kGcStride words per member)
+__device__ inline void member_add_wave(int64_t* counters, bool on, uint32_t g, int slot, int64_t v) {
+  wave_add_i64(counters, on, uint64_t(kGcStride) * g + uint64_t(slot), v, 1);
+}
+
 __device__ inline uint32_t pc_field(const uint64_t* pc, const PcLayout& L, uint32_t e, int wi, int sh) {
   return uint32_t((pc[uint64_t(e) * L.stride + wi] >> sh) & L.mask);
 }
@@ -172,11 +198,16 @@ lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
                       int32_t* __restrict__ cand_to, int32_t* __restrict__ cand_gain, uint32_t* __restrict__ heavy) {
   if (!a.vgraph && a.counters[6]) return;  // a rolled-back sub-round stopped the refinement
   int64_t visits = 0, swept = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    const uint32_t u = active[i];
+  const int lane = threadIdx.x & 63;
+  // whole waves step together: the per-member counts combine across lanes
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < na; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + uint32_t(lane);
+    const bool valid = i < na;
+    const uint32_t u = valid ? active[i] : 0u;
     const LpMember mb = member_of(a, u);
     uint32_t flag = 0;
-    if (in_bucket(a, u) && !(a.vgraph && member_counters(a, mb.g)[6])) {
+    int64_t member_visits = 0;
+    if (valid && in_bucket(a, u) && !(a.vgraph && member_counters(a, mb.g)[6])) {
       const uint32_t jb = a.voff[u], je = a.voff[u + 1];
       const uint32_t deg = je - jb;
       if (deg > kThreadDegreeLimit) {
@@ -221,8 +252,7 @@ lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
           }
         }
         if (a.vgraph) {
-          counter_add(member_counters(a, mb.g) + 1, deg);
-          counter_add(member_counters(a, mb.g) + 5, 1);
+          member_visits = deg;
         } else {
           visits += deg;
           swept += 1;
@@ -249,7 +279,11 @@ lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
         }
       }
     }
-    cand_flag[i] = flag;
+    if (a.vgraph) {
+      member_add_wave(a.counters, member_visits > 0, mb.g, 1, member_visits);
+      member_add_wave(a.counters, member_visits > 0, mb.g, 5, 1);
+    }
+    if (valid) cand_flag[i] = flag;
   }
   if (a.vgraph) return;  // batched: counted per member above
   visits = wave_sum(visits);
@@ -459,59 +493,63 @@ __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const
   // delta: the sub-round's delta was already set aside by the check)
   if (mode == 2 && !a.vgraph && !a.counters[6]) return;
   int64_t d = 0;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
-    if (admit && !admit[j]) continue;
-    if (mode == 1 && keep[j]) continue;
-    if (mode == 2 && !keep[j]) continue;
-    const uint32_t u = cu[j];
+  const int lane = threadIdx.x & 63;
+  // whole waves step together: block weights and member deltas combine
+  // across lanes
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < nc; base += gridDim.x * blockDim.x) {
+    const uint32_t j = base + uint32_t(lane);
+    bool on = j < nc && !(admit && !admit[j]) && !(mode == 1 && keep[j]) && !(mode == 2 && !keep[j]);
+    const uint32_t u = on ? cu[j] : 0u;
     const LpMember mb = member_of(a, u);
-    if (mode == 2 && a.vgraph && !member_counters(a, mb.g)[6]) continue;
-    const int from = mode == 0 ? cfrom[j] : cto[j];
-    const int to = mode == 0 ? cto[j] : cfrom[j];
-    const int64_t wu = a.vw[u];
-    a.part[u] = to;
-    atomicAdd(reinterpret_cast<unsigned long long*>(&a.part_w[mb.kb + to]), static_cast<unsigned long long>(wu));
-    atomicAdd(reinterpret_cast<unsigned long long*>(&a.part_w[mb.kb + from]), static_cast<unsigned long long>(-wu));
+    if (mode == 2 && a.vgraph && on && !member_counters(a, mb.g)[6]) on = false;
+    const int from = on ? (mode == 0 ? cfrom[j] : cto[j]) : 0;
+    const int to = on ? (mode == 0 ? cto[j] : cfrom[j]) : 0;
+    const int64_t wu = on ? int64_t(a.vw[u]) : 0;
+    wave_add_i64(a.part_w, on, uint64_t(mb.kb) + uint64_t(to), wu, 3);
+    wave_add_i64(a.part_w, on, uint64_t(mb.kb) + uint64_t(from), -wu, 3);
     int64_t du = 0;
-    const int fwi = from / a.L.epw, fsh = (from - fwi * a.L.epw) * a.L.b;
-    const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
-    const unsigned long long dec = static_cast<unsigned long long>(-(int64_t(1) << fsh));
-    const unsigned long long incv = 1ull << tsh;
-    // batches of 8 incident nets: both counter updates of all 8 nets are in
-    // flight together; the 0<->1 transitions then toggle connectivity
-    constexpr int NB = 8;
-    const uint32_t jb = a.voff[u], je = a.voff[u + 1];
-    for (uint32_t j0 = jb; j0 < je; j0 += NB) {
-      uint32_t e[NB];
-      uint64_t of[NB], ot[NB];
+    if (on) {
+      a.part[u] = to;
+      const int fwi = from / a.L.epw, fsh = (from - fwi * a.L.epw) * a.L.b;
+      const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
+      const unsigned long long dec = static_cast<unsigned long long>(-(int64_t(1) << fsh));
+      const unsigned long long incv = 1ull << tsh;
+      // batches of 8 incident nets: both counter updates of all 8 nets are in
+      // flight together; the 0<->1 transitions then toggle connectivity
+      constexpr int NB = 8;
+      const uint32_t jb = a.voff[u], je = a.voff[u + 1];
+      for (uint32_t j0 = jb; j0 < je; j0 += NB) {
+        uint32_t e[NB];
+        uint64_t of[NB], ot[NB];
 #pragma unroll
-      for (int t = 0; t < NB; ++t) e[t] = j0 + t < je ? a.inc[j0 + t] : 0u;
+        for (int t = 0; t < NB; ++t) e[t] = j0 + t < je ? a.inc[j0 + t] : 0u;
 #pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        if (j0 + t < je) {
-          unsigned long long* base = reinterpret_cast<unsigned long long*>(a.pc + uint64_t(e[t]) * a.L.stride);
-          of[t] = atomicAdd(base + fwi, dec);
-          ot[t] = atomicAdd(base + twi, incv);
+        for (int t = 0; t < NB; ++t) {
+          if (j0 + t < je) {
+            unsigned long long* rec = reinterpret_cast<unsigned long long*>(a.pc + uint64_t(e[t]) * a.L.stride);
+            of[t] = atomicAdd(rec + fwi, dec);
+            ot[t] = atomicAdd(rec + twi, incv);
+          }
         }
-      }
 #pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        if (j0 + t >= je) break;
-        const int32_t w = int32_t(a.pc[uint64_t(e[t]) * a.L.stride + a.L.woff]);
-        if (((of[t] >> fsh) & a.L.mask) == 1) {
-          atomicXor(reinterpret_cast<unsigned long long*>(&a.pc[uint64_t(e[t]) * a.L.stride + a.L.coff + from / 64]),
-                    1ull << (from % 64));
-          du -= w;
-        }
-        if (((ot[t] >> tsh) & a.L.mask) == 0) {
-          atomicXor(reinterpret_cast<unsigned long long*>(&a.pc[uint64_t(e[t]) * a.L.stride + a.L.coff + to / 64]),
-                    1ull << (to % 64));
-          du += w;
+        for (int t = 0; t < NB; ++t) {
+          if (j0 + t >= je) break;
+          const int32_t w = int32_t(a.pc[uint64_t(e[t]) * a.L.stride + a.L.woff]);
+          if (((of[t] >> fsh) & a.L.mask) == 1) {
+            atomicXor(reinterpret_cast<unsigned long long*>(&a.pc[uint64_t(e[t]) * a.L.stride + a.L.coff + from / 64]),
+                      1ull << (from % 64));
+            du -= w;
+          }
+          if (((ot[t] >> tsh) & a.L.mask) == 0) {
+            atomicXor(reinterpret_cast<unsigned long long*>(&a.pc[uint64_t(e[t]) * a.L.stride + a.L.coff + to / 64]),
+                      1ull << (to % 64));
+            du += w;
+          }
         }
       }
     }
     if (mode != 2) {
-      if (a.vgraph) counter_add(member_counters(a, mb.g), du);
+      if (a.vgraph) member_add_wave(a.counters, on, mb.g, 0, du);
       else d += du;
     }
   }

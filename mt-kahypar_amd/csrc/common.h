@@ -119,7 +119,10 @@ struct DevicePool {
     std::lock_guard<std::mutex> g(mu);
     return peak;
   }
-  static constexpr size_t kMaxCached = size_t(64) << 30;
+  // bytes the cache may hold; the context sets it from the device's memory
+  // (a step whose working set does not fit would free and re-allocate, and a
+  // free waits for the device)
+  size_t max_cached = size_t(64) << 30;
   static size_t size_class(size_t bytes) {
     size_t c = 256;
     while (c < bytes) c <<= 1;
@@ -153,7 +156,7 @@ struct DevicePool {
     {
       std::lock_guard<std::mutex> g(mu);
       in_use -= cls;
-      if (!closed && cached + cls <= kMaxCached) {
+      if (!closed && cached + cls <= max_cached) {
         blocks.emplace(cls, p);
         cached += cls;
         return;

@@ -122,6 +122,8 @@ int mkp_context_create(int device, mkp_context** out) {
     hipDeviceProp_t prop;
     MKP_HIP(hipGetDeviceProperties(&prop, device));
     ctx->c.num_cus = prop.multiProcessorCount;
+    // the allocation cache may hold up to 3/5 of the device memory
+    ctx->c.pool->max_cached = size_t(double(prop.totalGlobalMem) * 0.6);
     *out = ctx;
   });
 }

@@ -249,6 +249,12 @@ typedef struct {
  * bounded by (1 + epsilon) * ceil(W / k) (context.cpp:237-250); p is
  * initialised with the result. */
 MKP_API int mkp_initial_partition(mkp_partition* p, double epsilon, uint64_t seed, int32_t runs, int64_t* km1);
+/* Km1Rebalancer (rebalancer.cpp:36-262) on the host: empty blocks get a
+ * vertex, then border vertices of overloaded blocks move to the fitting block
+ * of best km1 gain (moves that do not worsen km1 first, the rest in
+ * increasing gain order while their block is over its bound); max_part_weights
+ * holds k bounds.  The partition state is rebuilt on the device. */
+MKP_API int mkp_rebalance(mkp_partition* p, const int64_t* max_part_weights, int64_t* km1_delta, int64_t* moves);
 /* Fast-preset defaults with the given k, epsilon and seed. */
 MKP_API int mkp_partition_params_default(mkp_partition_params* params, int32_t k, double epsilon, uint64_t seed);
 typedef struct {

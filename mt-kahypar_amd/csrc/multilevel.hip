@@ -9,6 +9,7 @@
 #include <memory>
 
 #include "../host/initial_partitioning.h"
+#include "../host/rebalancer.h"
 #include "internal.h"
 
 struct mkp_hierarchy {
@@ -85,6 +86,24 @@ mt_kahypar_amd::HostHypergraph download(mkp_hypergraph* hg) {
   return h;
 }
 
+// Km1Rebalancer on the host (rebalancer.cpp:36-179) for the partition p:
+// the hypergraph and block ids go to the host, the result comes back and the
+// partition state is rebuilt.
+mt_kahypar_amd::RebalanceResult rebalance_partition(mkp_partition* p, const std::vector<int64_t>& max_w) {
+  Context& c = p->hg->ctx->c;
+  const uint32_t n = p->hg->n;
+  std::vector<int32_t> part(n);
+  if (n) MKP_HIP(hipMemcpyAsync(part.data(), p->part.p, size_t(n) * 4, hipMemcpyDeviceToHost, c.stream));
+  mt_kahypar_amd::HostHypergraph h = download(p->hg);  // synchronises
+  mt_kahypar_amd::RebalanceResult r = mt_kahypar_amd::rebalance_km1(h, p->k, max_w, part);
+  if (r.moves) {
+    MKP_HIP(hipMemcpyAsync(p->part.p, part.data(), size_t(n) * 4, hipMemcpyHostToDevice, c.stream));
+    partition_initialize(p);
+    c.sync();
+  }
+  return r;
+}
+
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -154,6 +173,19 @@ int mkp_initial_partition(mkp_partition* p, double epsilon, uint64_t seed, int32
     partition_initialize(p);
     c.sync();
     if (km1) *km1 = partition_km1(p);
+  });
+}
+
+int mkp_rebalance(mkp_partition* p, const int64_t* max_part_weights, int64_t* km1_delta, int64_t* moves) {
+  return guarded([&] {
+    MKP_CHECK(p && max_part_weights, "null argument");
+    Context& c = p->hg->ctx->c;
+    Bind bind_(c);
+    std::vector<int64_t> mw(max_part_weights, max_part_weights + p->k);
+    mt_kahypar_amd::RebalanceResult r = rebalance_partition(p, mw);
+    c.sync();
+    if (km1_delta) *km1_delta = r.km1_delta;
+    if (moves) *moves = r.moves;
   });
 }
 
@@ -267,6 +299,17 @@ int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm
       }
     }
     mkp_partition* top = phg[0].get();
+    {
+      // uncoarsening.cpp:146-180: an imbalanced top-level partition is
+      // rebalanced (the initial partitioning repairs balance and LP keeps
+      // it, so this runs only when the bounds cannot all be met)
+      std::vector<int64_t> tw(k);
+      MKP_HIP(hipMemcpyAsync(tw.data(), top->part_w.p, size_t(k) * 8, hipMemcpyDeviceToHost, c.stream));
+      c.sync();
+      bool over = false;
+      for (int b = 0; b < k; ++b) over |= tw[b] > maxw;
+      if (over) rebalance_partition(top, maxv);
+    }
     std::unique_ptr<mkp_partition> full;
     if (red.hg) {
       // restore (partitioner.cpp:190-195): large nets return with their

@@ -148,6 +148,7 @@ SIGNATURES = {
     "mkp_partition_project": (C.c_int, [vp, vp, vp, C.c_int32]),
     "mkp_partition_hypergraph": (C.c_int, [vp, P(PartitionParams), vp, P(PartitionStats)]),
     "mkp_hypergraph_reduce": (C.c_int, [vp, C.c_uint32, C.c_uint32, P(vp), vp, P(C.c_uint32), P(C.c_uint32)]),
+    "mkp_rebalance": (C.c_int, [vp, vp, P(C.c_int64), P(C.c_int64)]),
     "mkp_large_net_threshold": (C.c_uint32, [C.c_uint32, C.c_double]),
     "mkp_partition_params_default": (C.c_int, [P(PartitionParams), C.c_int32, C.c_double, C.c_uint64]),
     "mkp_initial_partition": (C.c_int, [vp, C.c_double, C.c_uint64, C.c_int32, P(C.c_int64)]),
@@ -511,6 +512,13 @@ class Partition:
         v = C.c_int64()
         check(lib().mkp_partition_km1(self.h, C.byref(v)))
         return v.value
+
+    def rebalance(self, max_w):
+        """km1 rebalancer: (km1 change, moves)."""
+        mw = np.ascontiguousarray(max_w, dtype=np.int64)
+        d, mv = C.c_int64(), C.c_int64()
+        check(lib().mkp_rebalance(self.h, _p(mw), C.byref(d), C.byref(mv)))
+        return d.value, mv.value
 
     def lp_refine(self, max_w, perfect_w, seed=0, max_rounds=5, sub_rounds=4, activation_threshold=100,
                   rebalancing=True, nodes=None):

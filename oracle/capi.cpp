@@ -4,6 +4,7 @@
 // restated pieces.
 #include "oracle.h"
 #include "../mt-kahypar_amd/host/initial_partitioning.h"
+#include "../mt-kahypar_amd/host/rebalancer.h"
 
 #include <algorithm>
 #include <chrono>
@@ -209,6 +210,19 @@ void* orc_hg_contract(void* h, const uint32_t* cluster, uint32_t* cmap) {
 }
 // Runs the coarsening driver and returns the number of levels; the coarsest
 // hypergraph is returned as a new handle in *coarsest (or null).
+// km1 rebalancer (host code shared with the device pipeline): part is
+// rebalanced in place; returns the km1 change, moves in *moves
+int64_t orc_rebalance(void* h, int k, int32_t* part, const int64_t* max_w, int64_t* moves, int32_t* balanced) {
+  const auto* hg = static_cast<const Hypergraph*>(h);
+  std::vector<int32_t> p(part, part + hg->n);
+  mt_kahypar_amd::RebalanceResult r =
+      mt_kahypar_amd::rebalance_km1(to_host(*hg), k, std::vector<int64_t>(max_w, max_w + k), p);
+  std::copy(p.begin(), p.end(), part);
+  if (moves) *moves = r.moves;
+  if (balanced) *balanced = r.balanced ? 1 : 0;
+  return r.km1_delta;
+}
+
 // degree-zero vertex and large-net removal (sanitize.cpp)
 void* orc_reduce(void* h, uint32_t contraction_limit, uint32_t net_threshold, uint32_t* vmap, uint32_t* removed) {
   const auto* hg = static_cast<const Hypergraph*>(h);
@@ -308,6 +322,15 @@ int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int t
     for (uint32_t u = 0; u < fine.n; ++u) fp[u] = part[levels[i].cmap[u]];
     part.swap(fp);
     refine(fine, part, i);
+  }
+  {
+    // uncoarsening.cpp:146-180: an imbalanced top-level partition is
+    // rebalanced (host code shared with the device pipeline)
+    std::vector<int64_t> pw(k, 0);
+    for (uint32_t u = 0; u < top.n; ++u) pw[part[u]] += top.vw[u];
+    bool over = false;
+    for (int b = 0; b < k; ++b) over |= pw[b] > maxw;
+    if (over) mt_kahypar_amd::rebalance_km1(to_host(top), k, std::vector<int64_t>(k, maxw), part);
   }
   if (reduced) {
     // restore (partitioner.cpp:190-195): large nets return with their pins,

@@ -61,6 +61,7 @@ def lib():
                                       C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
             "orc_partition": (C.c_int64, [vp, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int, i32p, f64p, i64p]),
             "orc_reduce": (vp, [vp, C.c_uint32, C.c_uint32, u32p, u32p]),
+            "orc_rebalance": (C.c_int64, [vp, C.c_int, i32p, i64p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
             "orc_large_net_threshold": (C.c_uint32, [C.c_uint32, C.c_double]),
             "orc_restore_degree_zero": (None, [C.c_uint32, u32p, C.c_uint32, i32p, C.c_int, i64p, i32p]),
         }
@@ -182,6 +183,14 @@ class Hypergraph:
                                    ignore, sampling, seed, sub_rounds, int(disable_rand), int(prefer_unmatched),
                                    C.byref(out))
         return levels, (Hypergraph(0, None, None, handle=out.value) if out.value else None)
+
+    def rebalance(self, k, part, max_w):
+        """km1 rebalancer (shared host code): (new part, km1 change, moves,
+        balanced)."""
+        p = np.array(part, np.int32)
+        mv, bal = C.c_int64(), C.c_int32()
+        d = lib().orc_rebalance(self.h, k, p, np.ascontiguousarray(max_w, np.int64), C.byref(mv), C.byref(bal))
+        return p, int(d), mv.value, bool(bal.value)
 
     def reduce(self, contraction_limit, net_threshold):
         """Degree-zero vertex and large-net removal: (reduced hypergraph,

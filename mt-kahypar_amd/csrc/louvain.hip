@@ -800,8 +800,9 @@ lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uin
   const int slots = kLvClasses * a.buckets;
   for (int i = threadIdx.x; i < slots; i += blockDim.x) { h[i] = 0; ha[i] = 0; hb[i] = 0; }
   __syncthreads();
-  const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
-  const uint32_t lo = min(blockIdx.x * per, n), hi = min(lo + per, n);
+  const uint32_t per = n / gridDim.x + (n % gridDim.x != 0);
+  const uint32_t lo = uint32_t(min(uint64_t(blockIdx.x) * per, uint64_t(n)));
+  const uint32_t hi = uint32_t(min(uint64_t(lo) + per, uint64_t(n)));
   for (uint32_t u = lo + threadIdx.x; u < hi; u += blockDim.x) {
     const int slot = lv_slot(g, a, u);
     if (slot >= 0) {
@@ -831,8 +832,9 @@ lv_scatter_kernel(View g, LvArgs a, uint32_t n, const uint32_t* __restrict__ off
   const int slots = kLvClasses * a.buckets;
   for (int i = threadIdx.x; i < slots; i += blockDim.x) h[i] = 0;
   __syncthreads();
-  const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
-  const uint32_t lo = min(blockIdx.x * per, n), hi = min(lo + per, n);
+  const uint32_t per = n / gridDim.x + (n % gridDim.x != 0);
+  const uint32_t lo = uint32_t(min(uint64_t(blockIdx.x) * per, uint64_t(n)));
+  const uint32_t hi = uint32_t(min(uint64_t(lo) + per, uint64_t(n)));
   for (uint32_t u = lo + threadIdx.x; u < hi; u += blockDim.x) {
     const int slot = lv_slot(g, a, u);
     if (slot >= 0) atomicAdd(&h[slot], 1u);

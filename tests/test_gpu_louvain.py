@@ -109,7 +109,7 @@ def test_fractional_weights_modularity(ctx, weight):
 
 
 def test_group_lane_shapes_and_autotuner_agree(ctx):
-    """The lanes per node of the 17..128-arc class (8, 16, 32, 64, or chosen by
+    """The lanes per node of the 17..64-arc class (8, 16, 32, 64, or chosen by
     the autotuner) change only the speed: communities and statistics are
     identical, and equal to the oracle's."""
     n, noff, pins = powerlaw(8000, 12000, 200, 17, heavy_nets=1)

@@ -399,6 +399,8 @@ typedef struct {
   uint64_t hbm_budget_bytes;  /* admission budget; 0 = device memory less 4 GiB */
   int32_t spin_wait;          /* 0 (default): slot threads sleep while their stream runs; 1: they spin */
   int32_t ip_threads;         /* initial-partitioning threads per request; 0 = the request's own (4) */
+  int32_t host_threads;       /* with ip_threads 0: host threads shared by the requests in flight; a request
+                                 admitted beside r - 1 others gets max(its own, host_threads / r); 0 = off */
 } mkp_server_params;
 typedef struct {
   int64_t km1;

@@ -47,6 +47,7 @@ struct Request {
 struct mkp_server {
   uint64_t budget = 0;
   int ip_threads = 0;
+  int host_threads = 0;
   std::vector<mkp_context*> ctx;
   std::vector<uint8_t> busy;
   std::vector<std::thread> workers;
@@ -98,6 +99,13 @@ struct mkp_server {
       stats.max_reserved_bytes = std::max(stats.max_reserved_bytes, reserved);
       r->slot = s;
       r->t_start = Clock::now();
+      if (host_threads > 0 && ip_threads == 0) {
+        // a request admitted while few others run may spread its initial
+        // partitioning over more host threads (its result does not depend
+        // on the count)
+        const int own = r->prm.ip_threads > 0 ? r->prm.ip_threads : 4;
+        r->prm.ip_threads = std::max(own, host_threads / running);
+      }
       const bool more = !queue.empty();
       lk.unlock();
       if (more) cv_work.notify_one();  // the next request may fit as well
@@ -178,6 +186,8 @@ int mkp_server_create(const mkp_server_params* prm, mkp_server** out) {
       s->budget = prm->hbm_budget_bytes;
       MKP_CHECK(prm->ip_threads >= 0 && prm->ip_threads <= 256, "ip_threads must be in [0, 256]");
       s->ip_threads = prm->ip_threads;
+      MKP_CHECK(prm->host_threads >= 0 && prm->host_threads <= 1024, "host_threads must be in [0, 1024]");
+      s->host_threads = prm->host_threads;
       if (!s->budget) {
         size_t fr = 0, total = 0;
         MKP_HIP(hipSetDevice(prm->device));

@@ -75,7 +75,7 @@ class BatchLpParams(C.Structure):
 
 class ServerParams(C.Structure):
     _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("hbm_budget_bytes", C.c_uint64),
-                ("spin_wait", C.c_int32), ("ip_threads", C.c_int32)]
+                ("spin_wait", C.c_int32), ("ip_threads", C.c_int32), ("host_threads", C.c_int32)]
 
 
 class RequestResult(C.Structure):
@@ -645,8 +645,8 @@ class Server:
     """Request loop of the served form: submit() queues a partition request,
     wait() returns its block ids and timings."""
 
-    def __init__(self, device=0, slots=4, hbm_budget_bytes=0, spin_wait=False, ip_threads=0):
-        prm = ServerParams(device, slots, hbm_budget_bytes, 1 if spin_wait else 0, ip_threads)
+    def __init__(self, device=0, slots=4, hbm_budget_bytes=0, spin_wait=False, ip_threads=0, host_threads=0):
+        prm = ServerParams(device, slots, hbm_budget_bytes, 1 if spin_wait else 0, ip_threads, host_threads)
         self.h = vp()
         check(lib().mkp_server_create(C.byref(prm), C.byref(self.h)))
         self._hosts = {}

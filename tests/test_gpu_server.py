@@ -50,7 +50,9 @@ def check_balance(host, k, parts, eps=0.03):
 
 def test_results_identical_alone_and_in_the_mix(ctx):
     reqs = requests(12, 3)
-    srv = mtk_amd.Server(0, slots=4)
+    # host_threads: requests admitted beside few others use more initial
+    # partitioning threads than alone, which must not change their result
+    srv = mtk_amd.Server(0, slots=4, host_threads=16)
     tickets = [srv.submit(h, k, 0.03, sd) for h, k, sd in reqs]
     results = [srv.wait(t) for t in tickets]
     st = srv.stats()

@@ -886,7 +886,8 @@ __device__ inline uint32_t arc_source(const View& g, uint64_t a) {
 template <class View>
 __global__ void emit_arcs_kernel(View g, uint64_t num_arcs, const int32_t* __restrict__ C, int bits,
                                  uint64_t* __restrict__ keys, double* __restrict__ vals,
-                                 const uint32_t* __restrict__ nmember, const uint8_t* __restrict__ gkeep) {
+                                 const uint32_t* __restrict__ nmember, const uint8_t* __restrict__ gkeep,
+                                 uint32_t* __restrict__ keep) {
   const uint64_t sentinel = uint64_t(1) << (2 * bits);
   for (uint64_t a = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; a < num_arcs;
        a += uint64_t(gridDim.x) * blockDim.x) {
@@ -894,6 +895,7 @@ __global__ void emit_arcs_kernel(View g, uint64_t num_arcs, const int32_t* __res
     if (nmember && !gkeep[nmember[u]]) {  // batch: the member ends at this level
       keys[a] = sentinel;
       vals[a] = 0.0;
+      if (keep) keep[a] = 0;
       continue;
     }
     uint32_t v;
@@ -902,6 +904,19 @@ __global__ void emit_arcs_kernel(View g, uint64_t num_arcs, const int32_t* __res
     const uint64_t cu = uint32_t(C[u]), cv = uint32_t(C[v]);
     keys[a] = cu == cv ? sentinel : ((cu << bits) | cv);
     vals[a] = w;
+    if (keep) keep[a] = cu == cv ? 0u : 1u;
+  }
+}
+
+// stable compaction of the kept arcs (emission order preserved)
+__global__ void compact_arcs_kernel(const uint64_t* __restrict__ keys, const double* __restrict__ vals,
+                                    const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos, uint64_t n,
+                                    uint64_t* __restrict__ okeys, double* __restrict__ ovals) {
+  for (uint64_t a = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; a < n; a += uint64_t(gridDim.x) * blockDim.x) {
+    if (keep[a]) {
+      okeys[pos[a]] = keys[a];
+      ovals[pos[a]] = vals[a];
+    }
   }
 }
 
@@ -1320,20 +1335,43 @@ struct LouvainRun {
     }
     DevBuf<uint64_t> keys(num_arcs), skeys(num_arcs), ukeys(num_arcs);
     DevBuf<double> vals(num_arcs), svals(num_arcs), uvals(num_arcs);
+    // arcs inside one cluster (self loops of the coarse graph) are dropped
+    // before the sort by a stable compaction into ukeys / uvals, which are
+    // free until the reduction
+    const bool compact = num_arcs < 0xffffffffull;
+    DevBuf<uint32_t> keep(compact ? num_arcs : 1), pos(compact ? num_arcs + 1 : 1);
     emit_arcs_kernel<View><<<grid_for(num_arcs, 256, 65536), 256, 0, s>>>(g, num_arcs, C, bits, keys.p, vals.p,
-                                                                           nmember, gkeep);
+                                                                           nmember, gkeep,
+                                                                           compact ? keep.p : nullptr);
     MKP_LAUNCH_CHECK();
-    size_t bytes = 0;
-    MKP_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys.p, skeys.p, vals.p, svals.p, num_arcs, 0u,
-                                      unsigned(2 * bits + 1), s));
-    MKP_HIP(rocprim::radix_sort_pairs(c.temp(bytes), bytes, keys.p, skeys.p, vals.p, svals.p, num_arcs, 0u,
-                                      unsigned(2 * bits + 1), s));
+    uint64_t ns = num_arcs;
+    const uint64_t* sk = keys.p;
+    const double* sv = vals.p;
+    if (compact) {
+      exclusive_scan_u32(c, keep.p, pos.p, num_arcs);
+      compact_arcs_kernel<<<grid_for(num_arcs, 256, 65536), 256, 0, s>>>(keys.p, vals.p, keep.p, pos.p, num_arcs,
+                                                                          ukeys.p, uvals.p);
+      MKP_LAUNCH_CHECK();
+      uint32_t kept = 0;
+      MKP_HIP(hipMemcpyAsync(&kept, pos.p + num_arcs, 4, hipMemcpyDeviceToHost, s));
+      c.sync();
+      ns = kept;
+      sk = ukeys.p;
+      sv = uvals.p;
+    }
     DevBuf<uint64_t> ucount(1);
-    bytes = 0;
-    MKP_HIP(rocprim::reduce_by_key(nullptr, bytes, skeys.p, svals.p, num_arcs, ukeys.p, uvals.p, ucount.p,
-                                   rocprim::plus<double>(), rocprim::equal_to<uint64_t>(), s));
-    MKP_HIP(rocprim::reduce_by_key(c.temp(bytes), bytes, skeys.p, svals.p, num_arcs, ukeys.p, uvals.p, ucount.p,
-                                   rocprim::plus<double>(), rocprim::equal_to<uint64_t>(), s));
+    fill_async<uint64_t>(ucount.p, 0ull, 1, s);
+    if (ns) {
+      size_t bytes = 0;
+      MKP_HIP(rocprim::radix_sort_pairs(nullptr, bytes, sk, skeys.p, sv, svals.p, ns, 0u, unsigned(2 * bits + 1), s));
+      MKP_HIP(rocprim::radix_sort_pairs(c.temp(bytes), bytes, sk, skeys.p, sv, svals.p, ns, 0u,
+                                        unsigned(2 * bits + 1), s));
+      bytes = 0;
+      MKP_HIP(rocprim::reduce_by_key(nullptr, bytes, skeys.p, svals.p, ns, ukeys.p, uvals.p, ucount.p,
+                                     rocprim::plus<double>(), rocprim::equal_to<uint64_t>(), s));
+      MKP_HIP(rocprim::reduce_by_key(c.temp(bytes), bytes, skeys.p, svals.p, ns, ukeys.p, uvals.p, ucount.p,
+                                     rocprim::plus<double>(), rocprim::equal_to<uint64_t>(), s));
+    }
     uint64_t U = 0, last = 0;
     MKP_HIP(hipMemcpyAsync(&U, ucount.p, 8, hipMemcpyDeviceToHost, s));
     c.sync();

@@ -154,8 +154,10 @@ int mkp_request_estimate(uint32_t n, uint32_t m, uint64_t p, uint32_t max_net_si
     const double hyper = 28.0 * P + 12.0 * N + 20.0 * E;
     // community detection: node volumes, clustering and its volumes and
     // sizes, work lists, heavy-node tables, then the contraction (three key
-    // and three weight arrays per arc, sort storage, the coarse graph)
-    const double louvain = 40.0 * nodes + 20.0 * arcs + 48.0 * arcs + 16.0 * arcs + 16.0 * arcs + 24.0 * nodes;
+    // and three weight arrays per arc, the kept flags and their scan, sort
+    // storage, the coarse graph)
+    const double louvain = 40.0 * nodes + 20.0 * arcs + 48.0 * arcs + 8.0 * arcs + 16.0 * arcs + 16.0 * arcs +
+                           24.0 * nodes;
     // coarsening: the levels of the hierarchy (at most the input again in
     // total under the shrink bounds) and the rating and contraction work
     const double coarsen = hyper + 96.0 * P + 64.0 * N;

@@ -564,41 +564,51 @@ __global__ void lp_verify_kernel(LpArgs a, const uint32_t* __restrict__ cu, cons
                                  const int32_t* __restrict__ cto, const int32_t* __restrict__ cgain,
                                  const uint8_t* __restrict__ admit, uint32_t nc, uint8_t* __restrict__ keep) {
   int64_t reverts = 0;
-  for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
-    if (admit && !admit[j]) continue;
-    const uint32_t u = cu[j];
-    const int from = cfrom[j], to = cto[j];
-    const int fwi = from / a.L.epw, fsh = (from - fwi * a.L.epw) * a.L.b;
-    const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
-    int64_t real = 0;
-    constexpr int NB = 8;  // batches of 8 incident nets, loads issued together
-    const uint32_t jb = a.voff[u], je = a.voff[u + 1];
-    for (uint32_t j0 = jb; j0 < je; j0 += NB) {
-      uint32_t e[NB];
+  const int lane = threadIdx.x & 63;
+  // whole waves step together: the per-member counts combine across lanes
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < nc; base += gridDim.x * blockDim.x) {
+    const uint32_t j = base + uint32_t(lane);
+    const bool on = j < nc && !(admit && !admit[j]);
+    bool k = false;
+    uint32_t g = 0;
+    if (on) {
+      const uint32_t u = cu[j];
+      const int from = cfrom[j], to = cto[j];
+      const int fwi = from / a.L.epw, fsh = (from - fwi * a.L.epw) * a.L.b;
+      const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
+      int64_t real = 0;
+      constexpr int NB = 8;  // batches of 8 incident nets, loads issued together
+      const uint32_t jb = a.voff[u], je = a.voff[u + 1];
+      for (uint32_t j0 = jb; j0 < je; j0 += NB) {
+        uint32_t e[NB];
 #pragma unroll
-      for (int t = 0; t < NB; ++t) e[t] = a.inc[j0 + t < je ? j0 + t : jb];
-      int32_t w[NB];
-      uint32_t pt[NB], pf[NB];
+        for (int t = 0; t < NB; ++t) e[t] = a.inc[j0 + t < je ? j0 + t : jb];
+        int32_t w[NB];
+        uint32_t pt[NB], pf[NB];
 #pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        const uint64_t* r = a.pc + uint64_t(e[t]) * a.L.stride;  // one record per net
-        w[t] = int32_t(r[a.L.woff]);
-        pt[t] = uint32_t((r[twi] >> tsh) & a.L.mask);
-        pf[t] = uint32_t((r[fwi] >> fsh) & a.L.mask);
+        for (int t = 0; t < NB; ++t) {
+          const uint64_t* r = a.pc + uint64_t(e[t]) * a.L.stride;  // one record per net
+          w[t] = int32_t(r[a.L.woff]);
+          pt[t] = uint32_t((r[twi] >> tsh) & a.L.mask);
+          pf[t] = uint32_t((r[fwi] >> fsh) & a.L.mask);
+        }
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+          if (j0 + t >= je) break;
+          if (pt[t] == 1) real += w[t];
+          if (pf[t] == 0) real -= w[t];
+        }
       }
-#pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        if (j0 + t >= je) break;
-        if (pt[t] == 1) real += w[t];
-        if (pf[t] == 0) real -= w[t];
-      }
+      k = real == int64_t(cgain[j]) || real <= 0;
+      keep[j] = k ? 1 : 0;
+      if (a.vgraph) g = a.vgraph[u];
+      else reverts += k ? 0 : 1;
     }
-    const bool k = real == int64_t(cgain[j]) || real <= 0;
-    keep[j] = k ? 1 : 0;
     if (a.vgraph) {
-      if (!k) counter_add(member_counters(a, a.vgraph[u]) + 2, 1);
-    } else {
-      reverts += k ? 0 : 1;
+      // reverts, and the kept moves, which count once the member's
+      // sub-round is not rolled back (lp_check_kernel)
+      member_add_wave(a.counters, on && !k, g, 2, 1);
+      member_add_wave(a.counters, on && k, g, 10, 1);
     }
   }
   if (a.vgraph) return;
@@ -623,7 +633,7 @@ lp_activate_kernel(LpArgs a, const uint32_t* __restrict__ cu, const uint8_t* __r
     if (a.vgraph) {
       const uint32_t g = a.vgraph[u];
       if (member_counters(a, g)[6]) continue;  // the member's sub-round was rolled back
-      if (lane == 0) { flag[u] = 1; counter_add(member_counters(a, g) + 4, 1); }
+      if (lane == 0) flag[u] = 1;  // counted by the verify and check kernels
     } else if (lane == 0) {
       flag[u] = 1;
       ++moves;
@@ -670,8 +680,13 @@ __global__ void lp_check_kernel(int64_t* __restrict__ counters, uint32_t members
     int64_t* c = counters + kGcStride * g;
     const int64_t d = c[0];
     c[0] = 0;
-    if (d > 0) c[6] = 1;
-    else c[7] += d;
+    if (d > 0) {
+      c[6] = 1;
+    } else {
+      c[7] += d;
+      c[4] += c[10];  // the sub-round's kept moves (batched form)
+    }
+    c[10] = 0;
   }
 }
 

@@ -1590,9 +1590,17 @@ struct LouvainBatchRun : LouvainRun {
     changed.assign(G, 0);
     cvol_.alloc(std::max<uint32_t>(n, 1));
     DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
-    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), counts(slots), cursor(slots), offsets(slots), mcount(2),
-        mu(std::max<uint32_t>(n, 1)), moved_total(1);
-    DevBuf<uint64_t> sbytes(slots), sarcs(slots);
+    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), offsets(slots), mu(std::max<uint32_t>(n, 1));
+    // the round's counters in one buffer, laid out as in local_moving
+    const size_t cw = (size_t(slots) + 1) / 2;
+    const size_t rwords = 2 * size_t(slots) + 2 * cw + 2;
+    DevBuf<uint64_t> rb(rwords);
+    uint64_t* sarcs = rb.p;
+    uint64_t* sbytes = rb.p + slots;
+    uint32_t* counts = reinterpret_cast<uint32_t*>(rb.p + 2 * slots);
+    uint32_t* cursor = reinterpret_cast<uint32_t*>(rb.p + 2 * slots + cw);
+    uint32_t* moved_total = reinterpret_cast<uint32_t*>(rb.p + 2 * slots + 2 * cw);
+    uint32_t* mcount = moved_total + 1;
     DevBuf<int32_t> mto(std::max<uint32_t>(n, 1));
     std::vector<uint64_t> sb(slots, 0), sa(slots, 0);
     std::vector<uint32_t> cnt(slots), off(slots);
@@ -1602,7 +1610,7 @@ struct LouvainBatchRun : LouvainRun {
     LvArgs a;
     a.seed = 0; a.level = level; a.sampling = uint32_t(std::min<uint64_t>(prm.sampling, 0xffffffffu));
     a.recip = 0.0; a.C = C; a.cvol = cvol_.p; a.csize = csize.p; a.nvol = nvol;
-    a.move_u = mu.p; a.move_to = mto.p; a.move_count = mcount.p;
+    a.move_u = mu.p; a.move_to = mto.p; a.move_count = mcount;
     a.buckets = B; a.bucket_salt = 0;
     a.nmember = L.nmember.p; a.nlocal = L.nlocal.p; a.grecip = grecip_.p; a.gactive = gflag_.p; a.gseed = gseed_.p;
     std::vector<uint64_t> moved(L.nodes.begin(), L.nodes.end());
@@ -1618,22 +1626,16 @@ struct LouvainBatchRun : LouvainRun {
       if (!any) break;
       upload_flags(act);
       a.round = uint32_t(r);
-      fill_async<uint32_t>(counts.p, 0u, slots, s);
-      fill_async<uint32_t>(cursor.p, 0u, slots, s);
-      fill_async<uint32_t>(moved_total.p, 0u, 1, s);
+      fill_async<uint64_t>(rb.p, 0ull, rwords, s);
       fill_async<uint32_t>(gmoved_.p, 0u, G, s);
-      fill_async<uint64_t>(sarcs.p, 0ull, slots, s);
       const bool prof = c.prof.enabled;
-      if (prof) fill_async<uint64_t>(sbytes.p, 0ull, slots, s);
-      lv_count_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, counts.p, sarcs.p,
-                                                                           prof ? sbytes.p : nullptr);
+      lv_count_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, counts, sarcs,
+                                                                           prof ? sbytes : nullptr);
       MKP_LAUNCH_CHECK();
-      uint64_t* stg = c.staging<uint64_t>(4 * size_t(slots));
+      uint64_t* stg = c.staging<uint64_t>(2 * size_t(slots) + 2 * cw);
       uint32_t* stg_cnt = reinterpret_cast<uint32_t*>(stg + 2 * slots);
-      uint32_t* stg_off = reinterpret_cast<uint32_t*>(stg + 3 * slots);
-      MKP_HIP(hipMemcpyAsync(stg_cnt, counts.p, size_t(slots) * 4, hipMemcpyDeviceToHost, s));
-      MKP_HIP(hipMemcpyAsync(stg, sarcs.p, size_t(slots) * 8, hipMemcpyDeviceToHost, s));
-      if (prof) MKP_HIP(hipMemcpyAsync(stg + slots, sbytes.p, size_t(slots) * 8, hipMemcpyDeviceToHost, s));
+      uint32_t* stg_off = reinterpret_cast<uint32_t*>(stg + 2 * slots + cw);
+      MKP_HIP(hipMemcpyAsync(stg, rb.p, (2 * size_t(slots) + cw) * 8, hipMemcpyDeviceToHost, s));
       c.sync();
       std::copy(stg_cnt, stg_cnt + slots, cnt.begin());
       std::copy(stg, stg + slots, sa.begin());
@@ -1649,21 +1651,20 @@ struct LouvainBatchRun : LouvainRun {
       for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
       std::copy(off.begin(), off.end(), stg_off);
       MKP_HIP(hipMemcpyAsync(offsets.p, stg_off, size_t(slots) * 4, hipMemcpyHostToDevice, s));
-      lv_scatter_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, offsets.p, cursor.p, list.p);
+      lv_scatter_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, offsets.p, cursor, list.p);
       MKP_LAUNCH_CHECK();
       if (hnodes) prepare_heavy(g, a, list.p + off[4 * B], hnodes);
-      fill_async<uint32_t>(mcount.p, 0u, 2, s);
       const LaneChoice lanes = choose_lanes<View>(cnt, sa, B);
       for (int sr = 0; sr < B; ++sr) {
         a.bucket = sr;
-        a.move_count = mcount.p + (sr & 1);
+        a.move_count = mcount + (sr & 1);
         hipEvent_t ta, tb;
         const int lanes_now = lanes_for(lanes, sr, &ta, &tb);
         launch_move(g, a, list.p, off, cnt, sa, sb, sr, lanes_now, ta, tb);
         uint32_t sub_nodes = 0;
         for (int cl = 0; cl < kLvClasses; ++cl) sub_nodes += cnt[cl * B + sr];
         louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(
-            mu.p, mto.p, a.move_count, nvol, C, cvol_.p, csize.p, moved_total.p, mcount.p + ((sr + 1) & 1),
+            mu.p, mto.p, a.move_count, nvol, C, cvol_.p, csize.p, moved_total, mcount + ((sr + 1) & 1),
             L.nmember.p, gmoved_.p);
         MKP_LAUNCH_CHECK();
       }

@@ -1188,8 +1188,10 @@ struct LouvainRun {
     DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
     DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), offsets(slots), mu(std::max<uint32_t>(n, 1));
     // the round's counters in one buffer, cleared by one launch and read back
-    // by one copy: [sarcs u64][sbytes u64][counts u32][cursor u32][moved, two
-    // move counters u32]
+    // by one copy: [sarcs u64][sbytes u64][counts u32][cursor u32][two move
+    // counters u32][moved u32 of even rounds, of odd rounds].  The moves of
+    // round r are read back with the counts of round r + 1, which are
+    // computed before the loop condition is known and dropped if it fails.
     const size_t cw = (size_t(slots) + 1) / 2;  // words of slots u32 entries
     const size_t rwords = 2 * size_t(slots) + 2 * cw + 2;
     DevBuf<uint64_t> rb(rwords);
@@ -1197,8 +1199,8 @@ struct LouvainRun {
     uint64_t* sbytes = rb.p + slots;
     uint32_t* counts = reinterpret_cast<uint32_t*>(rb.p + 2 * slots);
     uint32_t* cursor = reinterpret_cast<uint32_t*>(rb.p + 2 * slots + cw);
-    uint32_t* moved_total = reinterpret_cast<uint32_t*>(rb.p + 2 * slots + 2 * cw);
-    uint32_t* mcount = moved_total + 1;
+    uint32_t* mcount = reinterpret_cast<uint32_t*>(rb.p + 2 * slots + 2 * cw);
+    uint32_t* moved_rounds = mcount + 2;
     std::vector<uint64_t> sb(slots, 0), sa(slots, 0);
     DevBuf<int32_t> mto(std::max<uint32_t>(n, 1));
     lv_init_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, nvol, C, cvol.p, csize.p);
@@ -1215,22 +1217,29 @@ struct LouvainRun {
     bool changed = false;
     uint64_t moved = n;
     std::vector<uint32_t> cnt(slots), off(slots);
-    for (int r = 0; double(moved) >= prm.min_move_fraction * double(n) && r < prm.max_passes; ++r) {
+    // pinned staging: the device buffer up to and including the moved
+    // counters, then the offsets (u32) for the way back
+    uint64_t* stg = c.staging<uint64_t>(rwords + cw);
+    uint32_t* stg_cnt = reinterpret_cast<uint32_t*>(stg + 2 * slots);
+    const uint32_t* stg_moved = reinterpret_cast<const uint32_t*>(stg + 2 * slots + 2 * cw) + 2;
+    uint32_t* stg_off = reinterpret_cast<uint32_t*>(stg + rwords);
+    // group round r's nodes by (degree class, sub-round bucket), clear its
+    // counters and read the buffer back (the moved counter of round r - 1
+    // with it); the moved counter of round r itself is cleared here too
+    auto count_round = [&](int r) {
       a.round = uint32_t(r);
       a.bucket_salt = bucket_salt(a.seed, kLvBucketTag | (uint64_t(level) << 8) | uint64_t(r));
-      // group this round's nodes by (degree class, sub-round bucket)
-      fill_async<uint64_t>(rb.p, 0ull, rwords, s);
-      const bool prof = c.prof.enabled;
+      fill_async<uint64_t>(rb.p, 0ull, rwords - 1, s);
+      fill_async<uint32_t>(moved_rounds + (r & 1), 0u, 1, s);
       lv_count_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, counts, sarcs,
-                                                                           prof ? sbytes : nullptr);
+                                                                           c.prof.enabled ? sbytes : nullptr);
       MKP_LAUNCH_CHECK();
-      // pinned staging: [sarcs u64][sbytes u64][counts u32] as on the device,
-      // then the offsets (u32) for the way back
-      uint64_t* stg = c.staging<uint64_t>(2 * size_t(slots) + 2 * cw);
-      uint32_t* stg_cnt = reinterpret_cast<uint32_t*>(stg + 2 * slots);
-      uint32_t* stg_off = reinterpret_cast<uint32_t*>(stg + 2 * slots + cw);
-      MKP_HIP(hipMemcpyAsync(stg, rb.p, (2 * size_t(slots) + cw) * 8, hipMemcpyDeviceToHost, s));
+      MKP_HIP(hipMemcpyAsync(stg, rb.p, rwords * 8, hipMemcpyDeviceToHost, s));
       c.sync();
+    };
+    count_round(0);
+    for (int r = 0; double(moved) >= prm.min_move_fraction * double(n) && r < prm.max_passes; ++r) {
+      const bool prof = c.prof.enabled;
       std::copy(stg_cnt, stg_cnt + slots, cnt.begin());
       std::copy(stg, stg + slots, sa.begin());
       if (prof) std::copy(stg + slots, stg + 2 * slots, sb.begin());
@@ -1259,14 +1268,13 @@ struct LouvainRun {
         uint32_t sub_nodes = 0;
         for (int cl = 0; cl < kLvClasses; ++cl) sub_nodes += cnt[cl * B + sr];
         louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(
-            mu.p, mto.p, a.move_count, nvol, C, cvol.p, csize.p, moved_total, mcount + ((sr + 1) & 1),
-            nullptr, nullptr);
+            mu.p, mto.p, a.move_count, nvol, C, cvol.p, csize.p, moved_rounds + (r & 1),
+            mcount + ((sr + 1) & 1), nullptr, nullptr);
         MKP_LAUNCH_CHECK();
       }
-      MKP_HIP(hipMemcpyAsync(c.pinned, moved_total, 4, hipMemcpyDeviceToHost, s));
-      c.sync();
+      count_round(r + 1);
       finish_lanes(lanes, sa, B);
-      moved = *reinterpret_cast<const uint32_t*>(c.pinned);
+      moved = stg_moved[r & 1];
       st.arcs_swept += int64_t(num_arcs);
       st.rounds += 1;
       st.moves += int64_t(moved);

@@ -1188,7 +1188,7 @@ int mkp_cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params* prm, int32_t 
                      uint32_t* nodes_after, int64_t* pin_visits) {
   return guarded([&] {
     MKP_CHECK(hg && prm && clusters, "null argument");
-    MKP_CHECK(prm->sub_rounds >= 1 && prm->max_shrink_factor > 0, "bad parameters");
+    MKP_CHECK(prm->sub_rounds >= 1 && prm->sub_rounds <= 255 && prm->max_shrink_factor > 0, "bad parameters");
     Context& c = hg->ctx->c;
     Bind bind_(c);
     DevBuf<uint32_t> cid(std::max<uint32_t>(hg->n, 1));

@@ -132,6 +132,10 @@ void louvain_batch(mkp_hypergraph* hg, const mkp_louvain_params& prm, int wt, co
 void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, const uint32_t* nodes = nullptr,
                uint32_t num_nodes = 0, const LpBatch* batch = nullptr);
 
+// Rejects partition parameters the pipeline cannot run (k, epsilon, preset
+// bounds, sub-round counts above 255) with an invalid-argument error.
+void validate_partition_params(const mkp_partition_params& p);
+
 // Degree-zero vertex and large-net removal (degree_zero_hn_remover.h:46-61,
 // large_he_remover.h:47-58) as one compaction: hg is null when nothing was
 // removed, vmap maps every vertex of the input to its reduced id

@@ -104,6 +104,22 @@ mt_kahypar_amd::RebalanceResult rebalance_partition(mkp_partition* p, const std:
   return r;
 }
 
+// Checks every parameter the pipeline forwards before any kernel runs.
+// Sub-round counts go into 8-bit fields of the round tags and index 256-entry
+// bucket tables, so each must lie in [1, 255] (0 selects the default).
+void validate_partition_params(const mkp_partition_params& p) {
+  MKP_CHECK(p.k >= 2 && p.epsilon >= 0, "bad k or epsilon");
+  MKP_CHECK(p.contraction_limit_multiplier >= 1 && p.max_shrink_factor > 1.0 &&
+                p.max_allowed_weight_multiplier > 0 && p.lp_max_rounds >= 0,
+            "bad preset parameters");
+  MKP_CHECK(p.lp_sub_rounds >= 0 && p.lp_sub_rounds <= 255, "lp_sub_rounds must be in [1, 255] (0: default)");
+  MKP_CHECK(p.louvain_sub_rounds >= 0 && p.louvain_sub_rounds <= 255,
+            "louvain_sub_rounds must be in [1, 255] (0: default)");
+  MKP_CHECK(p.coarsening_sub_rounds >= 0 && p.coarsening_sub_rounds <= 255,
+            "coarsening_sub_rounds must be in [1, 255] (0: default)");
+  MKP_CHECK(p.louvain_max_passes >= 0 && p.ip_threads >= 0 && p.ip_threads <= 256, "bad preset parameters");
+}
+
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -117,6 +133,7 @@ extern "C" {
 int mkp_coarsen(mkp_hypergraph* hg, const mkp_coarsen_params* prm, mkp_hierarchy** out) {
   return guarded([&] {
     MKP_CHECK(hg && prm && out, "null argument");
+    MKP_CHECK(prm->sub_rounds >= 1 && prm->sub_rounds <= 255 && prm->max_shrink_factor > 0, "bad parameters");
     Bind bind_(hg->ctx->c);
     *out = coarsen(hg, *prm);
   });
@@ -224,10 +241,7 @@ int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm
                              mkp_partition_stats* stats) {
   return guarded([&] {
     MKP_CHECK(hg && prm && parts, "null argument");
-    MKP_CHECK(prm->k >= 2 && prm->epsilon >= 0, "bad k or epsilon");
-    MKP_CHECK(prm->contraction_limit_multiplier >= 1 && prm->max_shrink_factor > 1.0 &&
-                  prm->max_allowed_weight_multiplier > 0 && prm->lp_max_rounds >= 0,
-              "bad preset parameters");
+    validate_partition_params(*prm);
     Context& c = hg->ctx->c;
     Bind bind_(c);
     const double t0 = now_ms();

@@ -226,7 +226,7 @@ int mkp_server_submit(mkp_server* s, const mkp_host_hypergraph* h, const mkp_par
                       uint64_t* ticket) {
   return guarded([&] {
     MKP_CHECK(s && h && prm && ticket, "null argument");
-    MKP_CHECK(prm->k >= 2 && prm->epsilon >= 0, "bad k or epsilon");
+    validate_partition_params(*prm);
     auto r = std::make_shared<Request>();
     r->t_submit = Clock::now();
     r->h = h;

@@ -69,3 +69,24 @@ def test_errors_are_reported_not_fatal():
     st = mtk_amd.lib().mkp_generate_powerlaw(ctypes.byref(prm), ctypes.byref(h))
     assert st == 1
     assert b"bad sizes" in mtk_amd.lib().mkp_last_error()
+
+
+def test_partition_params_out_of_range_are_refused_before_any_work():
+    """Sub-round counts above 255 would overflow the 8-bit sub-round field
+    of the round tags and the 256-entry bucket tables: the pipeline and the
+    server refuse them with an invalid-argument status before touching the
+    hypergraph (the handles below are never dereferenced)."""
+    L = mtk_amd.lib()
+    dummy = ctypes.create_string_buffer(64)
+    hg = ctypes.cast(dummy, ctypes.c_void_p)
+    parts = np.zeros(4, np.int32)
+    for field in ("lp_sub_rounds", "louvain_sub_rounds", "coarsening_sub_rounds"):
+        for bad in (256, 1000, -1):
+            prm = mtk_amd.partition_params(4, 0.03, 1, **{field: bad})
+            st = L.mkp_partition_hypergraph(hg, ctypes.byref(prm), parts.ctypes.data_as(ctypes.c_void_p), None)
+            assert st == 1, (field, bad)
+            assert field.encode() in L.mkp_last_error()
+            t = ctypes.c_uint64()
+            st = L.mkp_server_submit(hg, hg, ctypes.byref(prm), ctypes.byref(t))
+            assert st == 1, (field, bad)
+            assert field.encode() in L.mkp_last_error()

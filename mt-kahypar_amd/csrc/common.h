@@ -80,6 +80,25 @@ __host__ __device__ inline uint32_t bucket_of(uint32_t salt, uint32_t u, uint32_
   return fmix32(u * 0x9E3779B1u + salt) % buckets;
 }
 
+// Louvain sub-rounds take whole tiles of 2^kLvTileShift consecutive node ids
+// (the bucket of node u is the bucket of tile u >> kLvTileShift), so a
+// sub-round reads whole cache lines of the per-node and per-arc arrays
+// instead of one id in eight.
+constexpr int kLvTileShift = 6;
+
+// Block b of a grid of nb blocks (nb a multiple of 8) works on chunk
+// (b % 8) * (nb / 8) + b / 8 of a launch's work.  Workgroups are dealt to the
+// eight XCDs round-robin, so every XCD then takes one contiguous eighth of
+// the work and its L2 holds the data that neighbouring items gather.
+__host__ __device__ inline uint32_t xcd_chunk(uint32_t b, uint32_t nb) { return (b & 7u) * (nb >> 3) + (b >> 3); }
+// grid_for rounded up to a multiple of 8 blocks (see xcd_chunk)
+inline unsigned xcd_grid(uint64_t work, unsigned per_block, unsigned max_blocks = 1u << 20) {
+  uint64_t g = (work + per_block - 1) / per_block;
+  if (g == 0) g = 1;
+  if (g > max_blocks) g = max_blocks;
+  return unsigned((g + 7) & ~uint64_t(7));
+}
+
 // round tags (one per decision family)
 constexpr uint64_t kLpBucketTag = 0x4C50420000000000ull;
 constexpr uint64_t kLpCoinTag = 0x4C50430000000000ull;

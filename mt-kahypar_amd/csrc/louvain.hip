@@ -279,7 +279,11 @@ template <class View>
 __global__ void __launch_bounds__(256)
 louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count) {
   constexpr int D = 16;
-  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < count; idx += gridDim.x * blockDim.x) {
+  // one contiguous chunk of the list per block, chunks dealt per XCD
+  const uint32_t per = uint32_t((uint64_t(count) + gridDim.x - 1) / gridDim.x);
+  const uint64_t lo = uint64_t(xcd_chunk(blockIdx.x, gridDim.x)) * per;
+  const uint32_t hi = uint32_t(min(uint64_t(count), lo + per));
+  for (uint32_t idx = uint32_t(lo) + threadIdx.x; idx < hi; idx += blockDim.x) {
     const uint32_t u = list[idx];
     const uint32_t deg = min(g.degree(u), a.sampling);
     uint32_t head[D];
@@ -349,9 +353,13 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
   // touched by its own lanes only, so waves never wait for each other
   constexpr int GPW = 64 / G, WPB = BLOCK / 64;
   const uint32_t wid = threadIdx.x >> 6, wgrp = (threadIdx.x & 63) / G;
-  for (uint32_t base = (blockIdx.x * WPB + wid) * GPW; base < count; base += gridDim.x * WPB * GPW) {
+  // one contiguous chunk of the list per block, chunks dealt per XCD
+  const uint32_t per = uint32_t((uint64_t(count) + gridDim.x - 1) / gridDim.x);
+  const uint64_t lo = uint64_t(xcd_chunk(blockIdx.x, gridDim.x)) * per;
+  const uint32_t hi = uint32_t(min(uint64_t(count), lo + per));
+  for (uint32_t base = uint32_t(lo) + wid * GPW; base < hi; base += WPB * GPW) {
     const uint32_t idx = base + wgrp;
-    const bool in = idx < count;  // lists hold exactly the nodes of this sub-round
+    const bool in = idx < hi;  // lists hold exactly the nodes of this sub-round
     const uint32_t u = in ? list[idx] : 0;
     const uint32_t deg = in ? min(g.degree(u), a.sampling) : 0;
     // the node's own inputs of the decision, issued ahead of the sweep
@@ -775,25 +783,29 @@ __device__ inline int lv_slot(const View& g, const LvArgs& a, uint32_t u) {
     if (!a.gactive[gm]) return -1;
     if (a.buckets > 1) {
       const uint32_t salt = bucket_salt(a.gseed[gm], kLvBucketTag | (uint64_t(a.level) << 8) | uint64_t(a.round));
-      b = int(bucket_of(salt, a.nlocal[u], uint32_t(a.buckets)));
+      b = int(bucket_of(salt, a.nlocal[u] >> kLvTileShift, uint32_t(a.buckets)));
     }
   } else if (a.buckets > 1) {
-    b = int(bucket_of(a.bucket_salt, u, uint32_t(a.buckets)));
+    b = int(bucket_of(a.bucket_salt, u >> kLvTileShift, uint32_t(a.buckets)));
   }
   return c * a.buckets + b;
 }
 
 // Per-block LDS histograms over a contiguous range of nodes; a block flushes
 // once, so global atomics stay at one per (block, slot) for a grid of at most
-// kLvListBlocks blocks whatever the node count.  Contiguous ranges keep the
-// work lists in runs of neighbouring nodes, whose gathers share cache lines
-// in the move kernels.
+// kLvListBlocks blocks whatever the node count.  The block also stores its
+// count of every slot in bmat (slot-major), whose exclusive scan places the
+// blocks' ranges of a slot in block order: every work list is then in
+// ascending node order up to the order inside one block's range, so list
+// neighbours are id neighbours and the XCD-chunked move kernels gather from
+// one region of the cluster arrays per XCD.
 constexpr unsigned kLvListBlocks = 2048;
+inline unsigned lv_list_blocks(uint32_t n) { return grid_for(n, 256, kLvListBlocks); }
 
 template <class View>
 __global__ void __launch_bounds__(256)
 lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uint64_t* __restrict__ sarcs,
-                uint64_t* __restrict__ sbytes) {
+                uint64_t* __restrict__ sbytes, uint32_t* __restrict__ bmat) {
   __shared__ uint32_t h[kLvClasses * 256];
   __shared__ unsigned long long ha[kLvClasses * 256];
   __shared__ unsigned long long hb[kLvClasses * 256];
@@ -815,34 +827,25 @@ lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uin
   }
   __syncthreads();
   for (int i = threadIdx.x; i < slots; i += blockDim.x) {
+    bmat[size_t(i) * gridDim.x + blockIdx.x] = h[i];
     if (h[i]) atomicAdd(&counts[i], h[i]);
     if (ha[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&sarcs[i]), ha[i]);
     if (sbytes && hb[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&sbytes[i]), hb[i]);
   }
 }
 
-// Two passes over the block's nodes (the same contiguous range): count per
-// slot, reserve the block's range of every slot with one global atomic, then
-// place the nodes in it.
+// Places the nodes of the block's range (the range of the same block of
+// lv_count_kernel, same grid) at the block's scanned position of their slot.
 template <class View>
 __global__ void __launch_bounds__(256)
-lv_scatter_kernel(View g, LvArgs a, uint32_t n, const uint32_t* __restrict__ offsets, uint32_t* __restrict__ cursor,
-                  uint32_t* __restrict__ list) {
+lv_scatter_kernel(View g, LvArgs a, uint32_t n, const uint32_t* __restrict__ bpos, uint32_t* __restrict__ list) {
   __shared__ uint32_t h[kLvClasses * 256];
   const int slots = kLvClasses * a.buckets;
-  for (int i = threadIdx.x; i < slots; i += blockDim.x) h[i] = 0;
+  for (int i = threadIdx.x; i < slots; i += blockDim.x) h[i] = bpos[size_t(i) * gridDim.x + blockIdx.x];
   __syncthreads();
   const uint32_t per = n / gridDim.x + (n % gridDim.x != 0);
   const uint32_t lo = uint32_t(min(uint64_t(blockIdx.x) * per, uint64_t(n)));
   const uint32_t hi = uint32_t(min(uint64_t(lo) + per, uint64_t(n)));
-  for (uint32_t u = lo + threadIdx.x; u < hi; u += blockDim.x) {
-    const int slot = lv_slot(g, a, u);
-    if (slot >= 0) atomicAdd(&h[slot], 1u);
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < slots; i += blockDim.x)
-    if (h[i]) h[i] = offsets[i] + atomicAdd(&cursor[i], h[i]);
-  __syncthreads();
   for (uint32_t u = lo + threadIdx.x; u < hi; u += blockDim.x) {
     const int slot = lv_slot(g, a, u);
     if (slot >= 0) list[atomicAdd(&h[slot], 1u)] = u;
@@ -1138,7 +1141,7 @@ struct LouvainRun {
     if (cnt[s0]) {
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s0]),
                      std::string("louvain_move_small_kernel<") + View::kName + ">");
-      louvain_move_small_kernel<View><<<grid_for(cnt[s0], 256, 65536), 256, 0, s>>>(g, a, list + off[s0], cnt[s0]);
+      louvain_move_small_kernel<View><<<xcd_grid(cnt[s0], 256, 65536), 256, 0, s>>>(g, a, list + off[s0], cnt[s0]);
     }
     if (cnt[s1]) {  // 17..64 arcs: 128-slot maps, lanes per node autotuned
       const int G = lanes;
@@ -1147,29 +1150,29 @@ struct LouvainRun {
                      std::string("louvain_move_group_kernel<") + View::kName + ", " + std::to_string(G) + ", 7, " +
                          (G == 8 ? "128>" : "256>"));
       if (G == 8)
-        louvain_move_group_kernel<View, 8, 7, 128><<<grid_for(cnt[s1], 16, 65536), 128, 0, s>>>(g, a, list + off[s1],
+        louvain_move_group_kernel<View, 8, 7, 128><<<xcd_grid(cnt[s1], 16, 65536), 128, 0, s>>>(g, a, list + off[s1],
                                                                                                 cnt[s1]);
       else if (G == 16)
-        louvain_move_group_kernel<View, 16, 7, 256><<<grid_for(cnt[s1], 16, 65536), 256, 0, s>>>(g, a, list + off[s1],
+        louvain_move_group_kernel<View, 16, 7, 256><<<xcd_grid(cnt[s1], 16, 65536), 256, 0, s>>>(g, a, list + off[s1],
                                                                                                  cnt[s1]);
       else if (G == 64)
-        louvain_move_group_kernel<View, 64, 7, 256><<<grid_for(cnt[s1], 4, 65536), 256, 0, s>>>(g, a, list + off[s1],
+        louvain_move_group_kernel<View, 64, 7, 256><<<xcd_grid(cnt[s1], 4, 65536), 256, 0, s>>>(g, a, list + off[s1],
                                                                                                 cnt[s1]);
       else
-        louvain_move_group_kernel<View, 32, 7, 256><<<grid_for(cnt[s1], 8, 65536), 256, 0, s>>>(g, a, list + off[s1],
+        louvain_move_group_kernel<View, 32, 7, 256><<<xcd_grid(cnt[s1], 8, 65536), 256, 0, s>>>(g, a, list + off[s1],
                                                                                                 cnt[s1]);
       if (tb) MKP_HIP(hipEventRecord(tb, s));
     }
     if (cnt[s1w]) {  // 65..128 arcs: 256-slot maps
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s1w]),
                      std::string("louvain_move_group_kernel<") + View::kName + ", 32, 8, 256>");
-      louvain_move_group_kernel<View, 32, 8, 256><<<grid_for(cnt[s1w], 8, 65536), 256, 0, s>>>(
+      louvain_move_group_kernel<View, 32, 8, 256><<<xcd_grid(cnt[s1w], 8, 65536), 256, 0, s>>>(
           g, a, list + off[s1w], cnt[s1w]);
     }
     if (cnt[s2]) {
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s2]),
                      std::string("louvain_move_group_kernel<") + View::kName + ", 64, 11, 64>");
-      louvain_move_group_kernel<View, 64, 11, 64><<<grid_for(cnt[s2], 1, 65536), 64, 0, s>>>(g, a, list + off[s2],
+      louvain_move_group_kernel<View, 64, 11, 64><<<xcd_grid(cnt[s2], 1, 65536), 64, 0, s>>>(g, a, list + off[s2],
                                                                                              cnt[s2]);
     }
     if (cnt[s3]) launch_heavy(g, a, list + off[s3], off[s3] - off[4 * B], cnt[s3], sa[s3], double(sb[s3]));
@@ -1186,9 +1189,12 @@ struct LouvainRun {
     cvol_.alloc(std::max<uint32_t>(n, 1));
     DevBuf<double>& cvol = cvol_;
     DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
-    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), offsets(slots), mu(std::max<uint32_t>(n, 1));
+    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), mu(std::max<uint32_t>(n, 1));
+    // per-(slot, block) node counts of the work lists and their scan
+    const unsigned lblocks = lv_list_blocks(n);
+    DevBuf<uint32_t> bmat(size_t(slots) * lblocks), bpos(size_t(slots) * lblocks + 1);
     // the round's counters in one buffer, cleared by one launch and read back
-    // by one copy: [sarcs u64][sbytes u64][counts u32][cursor u32][two move
+    // by one copy: [sarcs u64][sbytes u64][counts u32][unused u32][two move
     // counters u32][moved u32 of even rounds, of odd rounds].  The moves of
     // round r are read back with the counts of round r + 1, which are
     // computed before the loop condition is known and dropped if it fails.
@@ -1198,7 +1204,6 @@ struct LouvainRun {
     uint64_t* sarcs = rb.p;
     uint64_t* sbytes = rb.p + slots;
     uint32_t* counts = reinterpret_cast<uint32_t*>(rb.p + 2 * slots);
-    uint32_t* cursor = reinterpret_cast<uint32_t*>(rb.p + 2 * slots + cw);
     uint32_t* mcount = reinterpret_cast<uint32_t*>(rb.p + 2 * slots + 2 * cw);
     uint32_t* moved_rounds = mcount + 2;
     std::vector<uint64_t> sb(slots, 0), sa(slots, 0);
@@ -1218,11 +1223,10 @@ struct LouvainRun {
     uint64_t moved = n;
     std::vector<uint32_t> cnt(slots), off(slots);
     // pinned staging: the device buffer up to and including the moved
-    // counters, then the offsets (u32) for the way back
-    uint64_t* stg = c.staging<uint64_t>(rwords + cw);
+    // counters
+    uint64_t* stg = c.staging<uint64_t>(rwords);
     uint32_t* stg_cnt = reinterpret_cast<uint32_t*>(stg + 2 * slots);
     const uint32_t* stg_moved = reinterpret_cast<const uint32_t*>(stg + 2 * slots + 2 * cw) + 2;
-    uint32_t* stg_off = reinterpret_cast<uint32_t*>(stg + rwords);
     // group round r's nodes by (degree class, sub-round bucket), clear its
     // counters and read the buffer back (the moved counter of round r - 1
     // with it); the moved counter of round r itself is cleared here too
@@ -1231,9 +1235,10 @@ struct LouvainRun {
       a.bucket_salt = bucket_salt(a.seed, kLvBucketTag | (uint64_t(level) << 8) | uint64_t(r));
       fill_async<uint64_t>(rb.p, 0ull, rwords - 1, s);
       fill_async<uint32_t>(moved_rounds + (r & 1), 0u, 1, s);
-      lv_count_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, counts, sarcs,
-                                                                           c.prof.enabled ? sbytes : nullptr);
+      lv_count_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, counts, sarcs, c.prof.enabled ? sbytes : nullptr,
+                                                     bmat.p);
       MKP_LAUNCH_CHECK();
+      exclusive_scan_u32(c, bmat.p, bpos.p, size_t(slots) * lblocks);
       MKP_HIP(hipMemcpyAsync(stg, rb.p, rwords * 8, hipMemcpyDeviceToHost, s));
       c.sync();
     };
@@ -1252,9 +1257,7 @@ struct LouvainRun {
       if (hnodes) reserve_heavy(harcs, hnodes);
       uint32_t run = 0;
       for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
-      std::copy(off.begin(), off.end(), stg_off);
-      MKP_HIP(hipMemcpyAsync(offsets.p, stg_off, size_t(slots) * 4, hipMemcpyHostToDevice, s));
-      lv_scatter_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, offsets.p, cursor, list.p);
+      lv_scatter_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, bpos.p, list.p);
       MKP_LAUNCH_CHECK();
       if (hnodes) prepare_heavy(g, a, list.p + off[4 * B], hnodes);
       const LaneChoice lanes = choose_lanes<View>(cnt, sa, B);
@@ -1598,7 +1601,9 @@ struct LouvainBatchRun : LouvainRun {
     changed.assign(G, 0);
     cvol_.alloc(std::max<uint32_t>(n, 1));
     DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
-    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), offsets(slots), mu(std::max<uint32_t>(n, 1));
+    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), mu(std::max<uint32_t>(n, 1));
+    const unsigned lblocks = lv_list_blocks(n);
+    DevBuf<uint32_t> bmat(size_t(slots) * lblocks), bpos(size_t(slots) * lblocks + 1);
     // the round's counters in one buffer, laid out as in local_moving
     const size_t cw = (size_t(slots) + 1) / 2;
     const size_t rwords = 2 * size_t(slots) + 2 * cw + 2;
@@ -1606,7 +1611,6 @@ struct LouvainBatchRun : LouvainRun {
     uint64_t* sarcs = rb.p;
     uint64_t* sbytes = rb.p + slots;
     uint32_t* counts = reinterpret_cast<uint32_t*>(rb.p + 2 * slots);
-    uint32_t* cursor = reinterpret_cast<uint32_t*>(rb.p + 2 * slots + cw);
     uint32_t* moved_total = reinterpret_cast<uint32_t*>(rb.p + 2 * slots + 2 * cw);
     uint32_t* mcount = moved_total + 1;
     DevBuf<int32_t> mto(std::max<uint32_t>(n, 1));
@@ -1637,12 +1641,11 @@ struct LouvainBatchRun : LouvainRun {
       fill_async<uint64_t>(rb.p, 0ull, rwords, s);
       fill_async<uint32_t>(gmoved_.p, 0u, G, s);
       const bool prof = c.prof.enabled;
-      lv_count_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, counts, sarcs,
-                                                                           prof ? sbytes : nullptr);
+      lv_count_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, counts, sarcs, prof ? sbytes : nullptr, bmat.p);
       MKP_LAUNCH_CHECK();
+      exclusive_scan_u32(c, bmat.p, bpos.p, size_t(slots) * lblocks);
       uint64_t* stg = c.staging<uint64_t>(2 * size_t(slots) + 2 * cw);
       uint32_t* stg_cnt = reinterpret_cast<uint32_t*>(stg + 2 * slots);
-      uint32_t* stg_off = reinterpret_cast<uint32_t*>(stg + 2 * slots + cw);
       MKP_HIP(hipMemcpyAsync(stg, rb.p, (2 * size_t(slots) + cw) * 8, hipMemcpyDeviceToHost, s));
       c.sync();
       std::copy(stg_cnt, stg_cnt + slots, cnt.begin());
@@ -1657,9 +1660,7 @@ struct LouvainBatchRun : LouvainRun {
       if (hnodes) reserve_heavy(harcs, hnodes);
       uint32_t run = 0;
       for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
-      std::copy(off.begin(), off.end(), stg_off);
-      MKP_HIP(hipMemcpyAsync(offsets.p, stg_off, size_t(slots) * 4, hipMemcpyHostToDevice, s));
-      lv_scatter_kernel<View><<<grid_for(n, 256, kLvListBlocks), 256, 0, s>>>(g, a, n, offsets.p, cursor, list.p);
+      lv_scatter_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, bpos.p, list.p);
       MKP_LAUNCH_CHECK();
       if (hnodes) prepare_heavy(g, a, list.p + off[4 * B], hnodes);
       const LaneChoice lanes = choose_lanes<View>(cnt, sa, B);

@@ -281,7 +281,8 @@ std::vector<int32_t> louvain_ref(Graph& g, const LouvainParams& prm, LouvainStat
 
 // ==================================================================== sync ==
 // Synchronous local moving: round r is split into sub-rounds by a hash of
-// (seed, level, r, u); within a sub-round every node of the bucket computes
+// (seed, level, r, tile of u), a tile being 64 consecutive node ids; within a
+// sub-round every node of the bucket computes
 // its best cluster (same gain, same first-occurrence tie rule) on the state at
 // the start of the sub-round, then all moves are applied.  A singleton cluster
 // moves into another singleton cluster only towards the smaller id, which
@@ -290,7 +291,7 @@ namespace {
 constexpr uint64_t kLvBucketTag = 0x4C56420000000000ull;
 
 bool local_moving_sync_level(const Graph& g, std::vector<int32_t>& C, const LouvainParams& prm,
-                             int level, LouvainStats* st) {
+                             int level, LouvainStats* st, LouvainTrace* tr = nullptr) {
   const double recip = 1.0 / g.tv;
   std::vector<double> cvol(g.n);
   std::vector<int32_t> csize(g.n, 1);
@@ -307,12 +308,18 @@ bool local_moving_sync_level(const Graph& g, std::vector<int32_t>& C, const Louv
     for (int s = 0; s < B; ++s) {
       moves.clear();
       for (uint32_t u = 0; u < g.n; ++u) {
-        if (B > 1 && bucket_of(salt, u, uint32_t(B)) != uint32_t(s)) continue;
+        if (B > 1 && bucket_of(salt, u >> kLvTileShift, uint32_t(B)) != uint32_t(s)) continue;
+        if (tr) tr->members.push_back(u);
         const int32_t from = C[u];
         const int32_t best = best_cluster_with(g, C.data(), cvol.data(), u, prm.sampling, recip, map, nullptr);
         if (best == from) continue;
         if (csize[from] == 1 && csize[best] == 1 && best > from) continue;
         moves.push_back({u, best});
+      }
+      if (tr) {
+        tr->member_off.push_back(tr->members.size());
+        for (const auto& mv : moves) { tr->mv_u.push_back(mv.first); tr->mv_to.push_back(mv.second); }
+        tr->move_off.push_back(tr->mv_u.size());
       }
       for (const auto& mv : moves) {
         const uint32_t u = mv.first;
@@ -345,9 +352,10 @@ std::vector<int32_t> recurse_sync(Graph& g, const LouvainParams& prm, int level,
 }  // namespace
 
 bool louvain_local_moving_sync(const Graph& g, std::vector<int32_t>& C, const LouvainParams& prm,
-                               LouvainStats* st) {
+                               LouvainStats* st, LouvainTrace* tr) {
   C.assign(g.n, 0);
-  return local_moving_sync_level(g, C, prm, 0, st);
+  if (tr) { tr->member_off.assign(1, 0); tr->move_off.assign(1, 0); }
+  return local_moving_sync_level(g, C, prm, 0, st, tr);
 }
 
 std::vector<int32_t> louvain_sync(Graph& g, const LouvainParams& prm, LouvainStats* st) {

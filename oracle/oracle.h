@@ -52,6 +52,9 @@ inline uint32_t bucket_salt(uint64_t seed, uint64_t tag) { return uint32_t(mix64
 inline uint32_t bucket_of(uint32_t salt, uint32_t u, uint32_t buckets) {
   return fmix32(u * 0x9E3779B1u + salt) % buckets;
 }
+// Louvain sub-round membership is per tile of 2^kLvTileShift consecutive
+// node ids: node u takes the bucket of tile u >> kLvTileShift.
+constexpr int kLvTileShift = 6;
 
 // ------------------------------------------------------------- hypergraph --
 // Static CSR hypergraph (static_hypergraph.h:44); incident nets ascending,
@@ -164,9 +167,18 @@ int32_t louvain_best_cluster(const Graph& g, const std::vector<int32_t>& C,
                              double* best_gain);
 std::vector<int32_t> louvain_ref(Graph& g, const LouvainParams& prm, LouvainStats* st);
 std::vector<int32_t> louvain_sync(Graph& g, const LouvainParams& prm, LouvainStats* st);
-// One synchronous local-moving level (no contraction); returns #moved per round.
+// Record of a synchronous local-moving level: the nodes of every sub-round
+// (members[member_off[i] .. member_off[i+1])) and its moves (mv_u / mv_to
+// over move_off[i] .. move_off[i+1]), sub-rounds in execution order.
+struct LouvainTrace {
+  std::vector<uint32_t> members, mv_u;
+  std::vector<int32_t> mv_to;
+  std::vector<uint64_t> member_off, move_off;
+};
+// One synchronous local-moving level (no contraction); returns whether a
+// node moved.  tr (optional) receives the level's sub-rounds.
 bool louvain_local_moving_sync(const Graph& g, std::vector<int32_t>& C, const LouvainParams& prm,
-                               LouvainStats* st);
+                               LouvainStats* st, LouvainTrace* tr = nullptr);
 
 // -------------------------------------------------------------- coarsening --
 struct CoarsenParams {
@@ -187,6 +199,13 @@ std::vector<uint32_t> cluster_pass_ref(const Hypergraph& hg, const CoarsenParams
                                        uint32_t* nodes_after);
 std::vector<uint32_t> cluster_pass_sync(const Hypergraph& hg, const CoarsenParams& prm, int pass,
                                         uint32_t* nodes_after, int64_t* pin_visits);
+// The rating step of one synchronous sub-round (pass, sub_round) for the given
+// proposers on the state (cid, cw, matched): target (UINT32_MAX: none) and its
+// 36.28 fixed-point rating per proposer.
+void rate_sync(const Hypergraph& hg, const CoarsenParams& prm, int pass, int sub_round,
+               const std::vector<uint32_t>& cid, const std::vector<int64_t>& cw,
+               const std::vector<uint8_t>& matched, const std::vector<uint32_t>& proposers,
+               std::vector<uint32_t>& target, std::vector<uint64_t>& rating);
 // StaticHypergraph::contract (static_hypergraph.cpp:58-513), canonical form:
 // pins ascending per net, incident nets ascending, survivor of parallel nets
 // is the smallest original net id.  cmap receives fine->coarse vertex ids.

@@ -178,6 +178,30 @@ int orc_louvain_level_sync(void* gp, uint64_t seed, int max_passes, double min_f
   return changed ? 1 : 0;
 }
 
+// Level-0 synchronous local moving with its sub-rounds recorded.  Returns the
+// number of sub-rounds, or -1 if a capacity (members / moves entries,
+// sub-rounds + 1 offsets) is too small.
+int orc_louvain_level_sync_trace(void* gp, uint64_t seed, int max_passes, double min_frac, uint64_t sampling,
+                                 int sub_rounds, int32_t* C_out, uint32_t* members, uint64_t members_cap,
+                                 uint64_t* member_off, uint32_t* mv_u, int32_t* mv_to, uint64_t moves_cap,
+                                 uint64_t* move_off, uint64_t offs_cap) {
+  auto* g = static_cast<Graph*>(gp);
+  LouvainParams prm;
+  prm.seed = seed; prm.max_passes = max_passes; prm.min_move_fraction = min_frac; prm.sampling = sampling;
+  prm.sub_rounds = sub_rounds;
+  LouvainTrace tr;
+  std::vector<int32_t> C;
+  louvain_local_moving_sync(*g, C, prm, nullptr, &tr);
+  if (tr.members.size() > members_cap || tr.mv_u.size() > moves_cap || tr.member_off.size() > offs_cap) return -1;
+  std::copy(C.begin(), C.end(), C_out);
+  std::copy(tr.members.begin(), tr.members.end(), members);
+  std::copy(tr.member_off.begin(), tr.member_off.end(), member_off);
+  std::copy(tr.mv_u.begin(), tr.mv_u.end(), mv_u);
+  std::copy(tr.mv_to.begin(), tr.mv_to.end(), mv_to);
+  std::copy(tr.move_off.begin(), tr.move_off.end(), move_off);
+  return int(tr.member_off.size()) - 1;
+}
+
 // --------------------------------------------------------- coarsening --
 static CoarsenParams coarsen_params(uint32_t contraction_limit, int32_t max_node_weight, double min_shrink,
                                     double max_shrink, uint32_t ignore, uint64_t sampling, uint64_t seed,
@@ -200,6 +224,21 @@ uint32_t orc_cluster_pass(void* h, int sync, int pass, uint32_t contraction_limi
                                  : cluster_pass_ref(*hg, p, pass, &after);
   std::copy(c.begin(), c.end(), cluster_out);
   return after;
+}
+// The rating step of one synchronous clustering sub-round on a given state.
+void orc_rate_sync(void* h, int pass, int sub_round, int32_t max_node_weight, uint32_t ignore, uint64_t sampling,
+                   uint64_t seed, int prefer_unmatched, const uint32_t* cid, const int64_t* cw,
+                   const uint8_t* matched, const uint32_t* proposers, uint32_t count, uint32_t* target_out,
+                   uint64_t* rating_out) {
+  auto* hg = static_cast<Hypergraph*>(h);
+  CoarsenParams p = coarsen_params(0, max_node_weight, 1.01, 2.5, ignore, sampling, seed, 1, 0, prefer_unmatched, 1);
+  std::vector<uint32_t> c(cid, cid + hg->n), pr(proposers, proposers + count), t;
+  std::vector<int64_t> w(cw, cw + hg->n);
+  std::vector<uint8_t> m(matched, matched + hg->n);
+  std::vector<uint64_t> r;
+  rate_sync(*hg, p, pass, sub_round, c, w, m, pr, t, r);
+  std::copy(t.begin(), t.end(), target_out);
+  std::copy(r.begin(), r.end(), rating_out);
 }
 void* orc_hg_contract(void* h, const uint32_t* cluster, uint32_t* cmap) {
   auto* hg = static_cast<Hypergraph*>(h);

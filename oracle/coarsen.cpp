@@ -187,7 +187,73 @@ constexpr uint64_t kCoBucketTag = 0x434F420000000000ull;
 constexpr uint64_t kCoTieTag = 0x434F540000000000ull;
 constexpr uint64_t kCoPrioTag = 0x434F500000000000ull;
 constexpr int kFixShift = 28;
+
+// Rating of one proposer (multilevel_vertex_pair_rater.h:109-199 with exact
+// distinct counts and fixed-point scores) and its acceptance
+// (rating_acceptance_policy.h:56-72 with the hash as the coin).
+struct SyncRater {
+  const Hypergraph& hg;
+  const CoarsenParams& prm;
+  std::vector<uint64_t> rval;
+  std::vector<int32_t> rpos;
+  std::vector<uint32_t> rkeys, distinct;
+  std::vector<uint8_t> seen;
+  int64_t visits = 0;
+  SyncRater(const Hypergraph& h, const CoarsenParams& p) : hg(h), prm(p), rval(h.n, 0), rpos(h.n, -1), seen(h.n, 0) {}
+  // returns the target (UINT32_MAX: none) and sets its rating
+  uint32_t rate(uint32_t u, uint64_t tag, const std::vector<uint32_t>& cid, const std::vector<int64_t>& cw,
+                const std::vector<uint8_t>& matched, uint64_t& rating) {
+    uint64_t accesses = 0;
+    for (uint64_t j = hg.voff[u]; j < hg.voff[u + 1]; ++j) {
+      const uint32_t e = hg.inc[j];
+      const uint32_t esz = hg.esize(e);
+      if (esz >= prm.ignore_he_size) continue;
+      distinct.clear();
+      for (uint64_t q = hg.noff[e]; q < hg.noff[e + 1]; ++q) {
+        const uint32_t rep = cid[hg.pins[q]];
+        if (!seen[rep]) { seen[rep] = 1; distinct.push_back(rep); }
+      }
+      for (uint32_t rep : distinct) seen[rep] = 0;
+      visits += 2 * int64_t(esz);
+      const uint32_t sz = std::max<uint32_t>(uint32_t(distinct.size()), 2);
+      if (accesses + sz > prm.sampling) break;
+      const uint64_t score = (uint64_t(uint32_t(hg.nw[e])) << kFixShift) / uint64_t(sz - 1);
+      for (uint32_t rep : distinct) {
+        if (rpos[rep] < 0) { rpos[rep] = int32_t(rkeys.size()); rkeys.push_back(rep); rval[rep] = 0; }
+        rval[rep] += score;
+        ++accesses;
+      }
+    }
+    uint64_t best_val = 0, best_h = 0;
+    int best_un = -1;
+    uint32_t best = UINT32_MAX;
+    for (uint32_t t : rkeys) {
+      if (t == u || cw[u] + cw[t] > prm.max_node_weight || hg.comm[t] != hg.comm[u]) continue;
+      const uint64_t v = rval[t];
+      const int un = prm.prefer_unmatched ? (matched[t] ? 0 : 1) : 0;
+      const uint64_t h = hash4(prm.seed, kCoTieTag | tag, u, t);
+      if (best == UINT32_MAX || v > best_val || (v == best_val && (un > best_un || (un == best_un && h > best_h)))) {
+        best = t; best_val = v; best_un = un; best_h = h;
+      }
+    }
+    for (uint32_t t : rkeys) rpos[t] = -1;
+    rkeys.clear();
+    rating = best_val;
+    return best;
+  }
+};
 }  // namespace
+
+void rate_sync(const Hypergraph& hg, const CoarsenParams& prm, int pass, int sub_round,
+               const std::vector<uint32_t>& cid, const std::vector<int64_t>& cw,
+               const std::vector<uint8_t>& matched, const std::vector<uint32_t>& proposers,
+               std::vector<uint32_t>& target, std::vector<uint64_t>& rating) {
+  SyncRater r(hg, prm);
+  const uint64_t tag = uint64_t(pass) << 8 | uint64_t(sub_round);
+  target.assign(proposers.size(), UINT32_MAX);
+  rating.assign(proposers.size(), 0);
+  for (size_t i = 0; i < proposers.size(); ++i) target[i] = r.rate(proposers[i], tag, cid, cw, matched, rating[i]);
+}
 
 std::vector<uint32_t> cluster_pass_sync(const Hypergraph& hg, const CoarsenParams& prm, int pass,
                                         uint32_t* nodes_after, int64_t* pin_visits) {
@@ -200,11 +266,7 @@ std::vector<uint32_t> cluster_pass_sync(const Hypergraph& hg, const CoarsenParam
   for (uint32_t u = 0; u < n; ++u) { cid[u] = u; cw[u] = hg.vw[u]; }
   const uint32_t limit = std::max(uint32_t(double(n) / prm.max_shrink), prm.contraction_limit);
   uint32_t current = n;
-  std::vector<uint64_t> rval(n, 0);
-  std::vector<int32_t> rpos(n, -1);
-  std::vector<uint32_t> rkeys, distinct;
-  std::vector<uint8_t> seen(n, 0);
-  int64_t visits = 0;
+  SyncRater rater(hg, prm);
   for (int s = 0; s < S; ++s) {
     if (current <= limit) break;
     const uint64_t tag = uint64_t(pass) << 8 | uint64_t(s);
@@ -212,46 +274,7 @@ std::vector<uint32_t> cluster_pass_sync(const Hypergraph& hg, const CoarsenParam
     const uint32_t salt = bucket_salt(prm.seed, kCoBucketTag | uint64_t(pass));
     for (uint32_t u = 0; u < n; ++u)
       if (!matched[u] && (S == 1 || bucket_of(salt, u, uint32_t(S)) == uint32_t(s))) proposers.push_back(u);
-    for (uint32_t u : proposers) {
-      target[u] = UINT32_MAX;
-      uint64_t accesses = 0;
-      for (uint64_t j = hg.voff[u]; j < hg.voff[u + 1]; ++j) {
-        const uint32_t e = hg.inc[j];
-        const uint32_t esz = hg.esize(e);
-        if (esz >= prm.ignore_he_size) continue;
-        distinct.clear();
-        for (uint64_t q = hg.noff[e]; q < hg.noff[e + 1]; ++q) {
-          const uint32_t rep = cid[hg.pins[q]];
-          if (!seen[rep]) { seen[rep] = 1; distinct.push_back(rep); }
-        }
-        for (uint32_t rep : distinct) seen[rep] = 0;
-        visits += 2 * int64_t(esz);
-        const uint32_t sz = std::max<uint32_t>(uint32_t(distinct.size()), 2);
-        if (accesses + sz > prm.sampling) break;
-        const uint64_t score = (uint64_t(uint32_t(hg.nw[e])) << kFixShift) / uint64_t(sz - 1);
-        for (uint32_t rep : distinct) {
-          if (rpos[rep] < 0) { rpos[rep] = int32_t(rkeys.size()); rkeys.push_back(rep); rval[rep] = 0; }
-          rval[rep] += score;
-          ++accesses;
-        }
-      }
-      uint64_t best_val = 0, best_h = 0;
-      int best_un = -1;
-      uint32_t best = UINT32_MAX;
-      for (uint32_t t : rkeys) {
-        if (t == u || cw[u] + cw[t] > prm.max_node_weight || hg.comm[t] != hg.comm[u]) continue;
-        const uint64_t v = rval[t];
-        const int un = prm.prefer_unmatched ? (matched[t] ? 0 : 1) : 0;
-        const uint64_t h = hash4(prm.seed, kCoTieTag | tag, u, t);
-        if (best == UINT32_MAX || v > best_val || (v == best_val && (un > best_un || (un == best_un && h > best_h)))) {
-          best = t; best_val = v; best_un = un; best_h = h;
-        }
-      }
-      for (uint32_t t : rkeys) rpos[t] = -1;
-      rkeys.clear();
-      target[u] = best;
-      rating[u] = best_val;
-    }
+    for (uint32_t u : proposers) target[u] = rater.rate(u, tag, cid, cw, matched, rating[u]);
     // resolution
     std::vector<uint8_t> moving(n, 0);
     for (uint32_t u : proposers) if (target[u] != UINT32_MAX) moving[u] = 1;
@@ -293,7 +316,7 @@ std::vector<uint32_t> cluster_pass_sync(const Hypergraph& hg, const CoarsenParam
       cw[t] = cum;
     }
   }
-  if (pin_visits) *pin_visits += visits;
+  if (pin_visits) *pin_visits += rater.visits;
   *nodes_after = current;
   return cid;
 }

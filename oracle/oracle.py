@@ -57,6 +57,11 @@ def lib():
             "orc_cluster_pass": (C.c_uint32, [vp, C.c_int, C.c_int, C.c_uint32, C.c_int32, C.c_double, C.c_uint32,
                                               C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, u32p, i64p]),
             "orc_hg_contract": (vp, [vp, u32p, u32p]),
+            "orc_rate_sync": (None, [vp, C.c_int, C.c_int, C.c_int32, C.c_uint32, C.c_uint64, C.c_uint64, C.c_int,
+                                     u32p, i64p, u8p, u32p, C.c_uint32, u32p, u64p]),
+            "orc_louvain_level_sync_trace": (C.c_int, [vp, C.c_uint64, C.c_int, C.c_double, C.c_uint64, C.c_int,
+                                                       i32p, u32p, C.c_uint64, u64p, u32p, i32p, C.c_uint64, u64p,
+                                                       C.c_uint64]),
             "orc_coarsen": (C.c_int, [vp, C.c_int, C.c_uint32, C.c_int32, C.c_double, C.c_double, C.c_uint32,
                                       C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
             "orc_partition": (C.c_int64, [vp, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int, i32p, f64p, i64p]),
@@ -159,6 +164,20 @@ class Hypergraph:
         return part, km1, dict(delta=int(st[0]), moves=int(st[1]), reverts=int(st[2]), visits=int(st[3]),
                                rounds=int(st[4]))
 
+    def rate_sync(self, pass_nr, sub_round, cid, cw, matched, proposers, max_node_weight, ignore=1000,
+                  sampling=200000, seed=0, prefer_unmatched=True):
+        """The rating step of one synchronous clustering sub-round on the given
+        state: (target per proposer, 0xffffffff for none; 36.28 fixed-point
+        rating per proposer)."""
+        pr = np.ascontiguousarray(proposers, dtype=np.uint32)
+        t = np.zeros(max(len(pr), 1), np.uint32)
+        r = np.zeros(max(len(pr), 1), np.uint64)
+        lib().orc_rate_sync(self.h, pass_nr, sub_round, max_node_weight, ignore, sampling, seed,
+                            int(prefer_unmatched), np.ascontiguousarray(cid, dtype=np.uint32),
+                            np.ascontiguousarray(cw, dtype=np.int64), np.ascontiguousarray(matched, dtype=np.uint8),
+                            pr, len(pr), t, r)
+        return t[:len(pr)], r[:len(pr)]
+
     def cluster_pass(self, sync, pass_nr, contraction_limit, max_node_weight, max_shrink=2.5, ignore=1000,
                      sampling=200000, seed=0, sub_rounds=16, disable_rand=False, prefer_unmatched=True,
                      with_visits=False):
@@ -260,6 +279,20 @@ class Graph:
         lib().orc_louvain(self.h, int(sync), seed, max_passes, min_frac, sampling, sub_rounds, threads,
                           int(disable_rand), out, st)
         return out, dict(arcs=int(st[0]), rounds=int(st[1]), levels=int(st[2]), moves=int(st[3]))
+
+    def louvain_level_sync_trace(self, seed=0, max_passes=5, min_frac=0.01, sampling=200000, sub_rounds=8):
+        """Level-0 synchronous local moving with every sub-round recorded:
+        (final clustering, [(members, moves u, moves to) per sub-round])."""
+        n = self.sizes()[0]
+        cap = n * max_passes + 1
+        out = np.zeros(max(n, 1), np.int32)
+        mem = np.zeros(cap, np.uint32); moff = np.zeros(max_passes * sub_rounds + 2, np.uint64)
+        mu = np.zeros(cap, np.uint32); mt = np.zeros(cap, np.int32); voff = np.zeros_like(moff)
+        cnt = lib().orc_louvain_level_sync_trace(self.h, seed, max_passes, min_frac, sampling, sub_rounds, out,
+                                                 mem, cap, moff, mu, mt, cap, voff, len(moff))
+        assert cnt >= 0
+        subs = [(mem[moff[i]:moff[i + 1]], mu[voff[i]:voff[i + 1]], mt[voff[i]:voff[i + 1]]) for i in range(cnt)]
+        return out[:n], subs
 
     def louvain_level_sync(self, seed=0, max_passes=5, min_frac=0.01, sampling=200000, sub_rounds=8):
         n = self.sizes()[0]

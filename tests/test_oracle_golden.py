@@ -363,3 +363,170 @@ def test_ranked_star_expansion_is_a_relabelling():
             keys.append((p[(len(p) - 1) // 2], e))
         order = [e for _, e in sorted(keys)]
         assert [perm[n + e] - n for e in order] == list(range(m))
+
+
+# ------------------------------------------------------------ border nodes --
+def test_border_node_vectors():
+    """isBorderNode and numIncidentCutHyperedges after the moves of
+    partitioned_hypergraph_test.cc:373-466 (fixture partition, k = 3)."""
+    bv = V["border_nodes"]
+    fx = V["fixture_hypergraph"]
+    hg = fixture_hg()
+    k = bv["k"]
+    for case in bv["cases"]:
+        part = list(bv["part"])
+        for u, frm, to in case["moves"]:
+            assert part[u] == frm
+            part[u] = to
+        assert hg.border(k, part).tolist() == case["border"], case["_line"]
+        st = hg.phg_init(k, part)
+        lam = [bin(int(st["conn"][e][0])).count("1") for e in range(len(fx["nets"]))]
+        cut_incident = [sum(1 for e, net in enumerate(fx["nets"]) if u in net and lam[e] > 1)
+                        for u in range(fx["n"])]
+        assert cut_incident == case["cut_incident"], case["_line"]
+
+
+def test_border_brute_force():
+    """A vertex is a border vertex iff some incident net spans more than one
+    block, unless its degree exceeds 100,000 (partitioned_hypergraph.h:67,
+    605-620): checked on random partitions and on a hub of degree 100,001."""
+    rng = np.random.default_rng(17)
+    for k in (2, 3, 8, 65):
+        hg, nets, _, _ = random_hg(rng, 120, 90, 8, weighted=True)
+        part = rng.integers(0, k, 120).astype(np.int32)
+        got = hg.border(k, part)
+        for u in range(120):
+            want = any(u in net and len({int(part[p]) for p in net}) > 1 for net in nets)
+            assert bool(got[u]) == want, (k, u)
+    # hub vertex 0 in 100,001 two-pin nets, all cut: never a border vertex;
+    # vertex 1 (degree 100,000 through the hub's nets) still is one
+    n = 100_002
+    nets = [[0, 1 + (i % (n - 1))] for i in range(100_001)]
+    noff, pins = csr(nets)
+    hub = orc.Hypergraph(n, noff, pins)
+    part = np.ones(n, np.int32)
+    part[0] = 0
+    b = hub.border(2, part)
+    assert b[0] == 0 and b[1] == 1 and b[2] == 1
+
+
+# ----------------------------------------------------- synchronous rating --
+def test_sync_rating_brute_force():
+    """The rating step of a synchronous clustering sub-round against a
+    brute-force restatement in exact rationals: for every proposer u the score
+    of a cluster is the sum over u's nets e (|e| < ignore) holding a member of
+    it of w(e) / (distinct clusters of e - 1), distinct counted at least 2
+    (multilevel_vertex_pair_rater.h:109-199, HeavyEdgeScore
+    rating_score_policy.h:29-35); candidates are other clusters of u's
+    community within the node weight limit; the target has the largest score,
+    an unmatched cluster among equal scores (rating_acceptance_policy.h:56-72),
+    and its 36.28 fixed-point rating is the score up to the truncation of each
+    net's term."""
+    from fractions import Fraction
+    rng = np.random.default_rng(23)
+    for trial in range(6):
+        n, m = 150, 170
+        nets = []
+        for _ in range(m):
+            s = int(rng.integers(2, 12))
+            c = int(rng.integers(0, n))
+            nets.append(sorted({(c + int(x)) % n for x in rng.integers(-10, 11, s)} | {c}))
+        nets = [e for e in nets if len(e) >= 2]
+        noff, pins = csr(nets)
+        vw = rng.integers(1, 4, n).astype(np.int32)
+        nw = rng.integers(1, 5, len(nets)).astype(np.int32)
+        hg = orc.Hypergraph(n, noff, pins, vw, nw)
+        comm = (np.arange(n) // 40).astype(np.int32)
+        hg.set_communities(comm)
+        # a state: some clusters formed, representatives point to themselves
+        cid = np.arange(n, dtype=np.uint32)
+        for u in rng.choice(n, 40, replace=False):
+            v = int(rng.integers(max(0, u - 5), min(n, u + 6)))
+            if comm[v] == comm[u] and cid[v] == v and cid[u] == u and u != v:
+                cid[u] = v
+        cw = np.zeros(n, np.int64)
+        for u in range(n):
+            cw[cid[u]] += vw[u]
+        size = np.bincount(cid, minlength=n)
+        matched = (size[cid] > 1).astype(np.uint8)
+        proposers = np.array([u for u in range(n) if not matched[u]], np.uint32)
+        maxw = int(rng.integers(4, 9))
+        ignore = 9 if trial % 2 else 1000
+        tgt, rat = hg.rate_sync(trial, 3, cid, cw, matched, proposers, maxw, ignore=ignore, seed=trial)
+        incident = [[e for e, net in enumerate(nets) if u in net] for u in range(n)]
+        for u, t, r in zip(proposers.tolist(), tgt.tolist(), rat.tolist()):
+            score, terms = {}, {}
+            for e in incident[u]:
+                if len(nets[e]) >= ignore:
+                    continue
+                reps = {int(cid[p]) for p in nets[e]}
+                d = max(len(reps), 2)
+                for c in reps:
+                    score[c] = score.get(c, Fraction(0)) + Fraction(int(nw[e]), d - 1)
+                    terms[c] = terms.get(c, 0) + 1
+            cand = {c: x for c, x in score.items()
+                    if c != u and cw[u] + cw[c] <= maxw and comm[c] == comm[u]}
+            if not cand:
+                assert t == 0xFFFFFFFF, u
+                continue
+            best = max(cand.values())
+            assert t in cand and cand[t] == best, (u, t)
+            exact = best * (1 << 28)
+            assert exact - terms[t] < r <= exact, (u, r, float(exact))
+            tied = [c for c, x in cand.items() if x == best]
+            if any(not matched[c] for c in tied):
+                assert not matched[t], (u, t)
+
+
+# ------------------------------------------ synchronous local moving rounds --
+def test_louvain_sync_sub_rounds_brute_force():
+    """Every sub-round of synchronous local moving on a star expansion,
+    replayed from the recorded trace: its nodes are whole tiles of 64
+    consecutive ids (each round covers every node once), and its moves are
+    exactly the nodes whose best cluster on the state at the sub-round start
+    (computeMaxGainCluster, local_moving_modularity.h:87-129, volumes
+    recomputed from that state) differs from their own, except a singleton
+    moving to a larger singleton; every sampled move alone raises the
+    brute-force modularity (local_moving_modularity.cpp:28-56)."""
+    h = orc.Hypergraph(*_powerlaw_arrays(1500, 1500, seed=3))
+    g = orc.Graph(h, orc.Graph.UNIFORM, ranked=True)
+    n = g.sizes()[0]
+    final, subs = g.louvain_level_sync_trace(seed=11, sub_rounds=8)
+    assert len(subs) >= 8
+    C = np.arange(n, dtype=np.int32)
+    rng = np.random.default_rng(4)
+    for i, (members, mu, mt) in enumerate(subs):
+        if i % 8 == 0:
+            round_nodes = []
+        round_nodes.extend(members.tolist())
+        tiles = set((members >> 6).tolist())
+        assert sum(min(64, n - (t << 6)) for t in tiles) == len(members)
+        csize = np.bincount(C, minlength=n)
+        expect = {}
+        for u in members.tolist():
+            to, _ = g.best_cluster(C, u)
+            frm = int(C[u])
+            if to != frm and not (csize[frm] == 1 and csize[to] == 1 and to > frm):
+                expect[u] = to
+        assert dict(zip(mu.tolist(), mt.tolist())) == expect, i
+        q0 = g.modularity(C)
+        for j in rng.choice(len(mu), min(5, len(mu)), replace=False) if len(mu) else []:
+            C2 = C.copy()
+            C2[mu[j]] = mt[j]
+            assert g.modularity(C2) > q0
+        C[mu] = mt
+        if i % 8 == 7:
+            assert sorted(round_nodes) == list(range(n))
+    assert np.array_equal(C, final)
+
+
+def _powerlaw_arrays(n, m, seed):
+    rng = np.random.default_rng(seed)
+    nets = []
+    for _ in range(m):
+        s = int(min(40, 2 + rng.zipf(2.3)))
+        c = int(rng.integers(0, n))
+        nets.append(sorted({(c + int(x)) % n for x in rng.integers(-60, 61, s)} | {c}))
+    nets = [e for e in nets if len(e) >= 2]
+    noff, pins = csr(nets)
+    return n, noff, pins

@@ -1,0 +1,269 @@
+"""Parity at the full sizes of the five benchmark workloads.
+
+* power-law 16,384 x 16,384, k = 2: the whole pipeline, Louvain and LP bit
+  for bit against the oracle's synchronous mode, and the pipeline's km1
+  median over seeds 42-46 within 1% of the median of the oracle's
+  reference-semantics mode, with the (1 + eps) bound held on every run;
+* power-law 1M x 2M, k = 8: partition state (pin counts, connectivity,
+  block weights, km1) on identical states, one LP refinement and Louvain,
+  bit for bit;
+* row-net 4M x 4M, k = 64, vertex weight = row nnz: partition state and LP;
+* batched, 4,096 members: every 128th member equal to its own run, two
+  members equal to the oracle;
+* served, requests of 10k to 20M pins: each request's result equal alone
+  and in the mix.
+
+The inputs are the seeded synthetic hypergraphs of the benchmark; each test
+prints its case before the device work starts."""
+import numpy as np
+import pytest
+
+import mtk_amd
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+EPS = 0.03
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = mtk_amd.Context(0)
+    yield c
+    c.close()
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def start_partition(n, k):
+    return (np.arange(n, dtype=np.int64) * k // n).astype(np.int32)
+
+
+def bounds(vw, k):
+    W = int(np.sum(vw))
+    perfect = -(-W // k)
+    return int((1.0 + EPS) * perfect), perfect
+
+
+def powerlaw_16k():
+    return mtk_amd.HostHypergraph.powerlaw(16_384, 16_384, 2.5, 2, 64, window=1024, local_fraction=0.8, seed=42)
+
+
+def powerlaw_1m():
+    return mtk_amd.HostHypergraph.powerlaw(1_000_000, 2_000_000, 2.2, 2, 10_000, window=4096, local_fraction=0.8,
+                                           seed=42)
+
+
+def rownet_4m():
+    return mtk_amd.HostHypergraph.rownet(4_000_000, 4_000_000, 8.0, 5000, 10_000, 0.7, seed=42)
+
+
+def check_state(dp, oh, k, part):
+    ref = oh.phg_init(k, part)
+    assert np.array_equal(dp.pin_counts(), ref["pin_counts"])
+    assert np.array_equal(dp.connectivity(), ref["conn"])
+    assert np.array_equal(dp.part_weights(), ref["part_w"])
+    assert dp.km1() == ref["km1"]
+    return ref
+
+
+def lp_pair(ctx, host, k, seed):
+    """Partition state on the contiguous starting partition, then one LP
+    refinement, both bit for bit against the synchronous oracle."""
+    a = host.arrays()
+    n = len(a["vw"])
+    part = start_partition(n, k)
+    maxw, perfect = bounds(a["vw"], k)
+    dh = mtk_amd.Hypergraph(ctx, host=host)
+    oh = orc.Hypergraph(n, a["noff"], a["pins"], a["vw"], a["nw"])
+    dp = mtk_amd.Partition(dh, k)
+    dp.set_parts(part)
+    check_state(dp, oh, k, part)
+    st = dp.lp_refine([maxw] * k, [perfect] * k, seed=seed)
+    rpart, rkm1, rst = oh.lp_refine(k, part, [maxw] * k, [perfect] * k, seed=seed, sync=True)
+    got = dp.parts()
+    assert np.array_equal(got, rpart)
+    assert dp.km1() == rkm1
+    for key in ("delta", "moves", "reverts", "visits", "rounds"):
+        assert st[key] == rst[key], key
+    assert st["moves"] > 0 and st["delta"] < 0
+    check_state(dp, oh, k, got)
+    assert dp.part_weights().max() <= maxw
+    dp.close()
+    return dh, oh
+
+
+def louvain_pair(dh, oh, seed):
+    comm, st = dh.louvain(seed=seed)
+    g = orc.Graph(oh, orc.Graph.UNIFORM, ranked=True)
+    C, ost = g.louvain(sync=True, seed=seed)
+    n = len(comm)
+    assert np.array_equal(comm, C[:n])
+    assert (st["arcs_swept"], st["rounds"], st["levels"], st["moves"]) == \
+        (ost["arcs"], ost["rounds"], ost["levels"], ost["moves"])
+    assert st["levels"] >= 2
+
+
+# ------------------------------------------------ power-law 16k x 16k, k=2
+def test_powerlaw_16k_pipeline_bit_exact(ctx):
+    say("case: power-law 16,384 x 16,384, k=2: pipeline, seed 42")
+    h = powerlaw_16k()
+    a = h.arrays()
+    n = len(a["vw"])
+    dh = mtk_amd.Hypergraph(ctx, host=h)
+    part, st = dh.partition(2, EPS, 42)
+    opart, okm1, ost = orc.Hypergraph(n, a["noff"], a["pins"], a["vw"], a["nw"]).partition(2, EPS, 42, sync=True)
+    assert np.array_equal(part, opart)
+    assert st["km1"] == okm1
+    assert (st["louvain_arcs"], st["rating_visits"], st["lp_visits"]) == \
+        (ost["louvain_arcs"], ost["rating_visits"], ost["lp_visits"])
+    maxw, _ = bounds(a["vw"], 2)
+    assert np.bincount(part, minlength=2).max() <= maxw
+    dh.close()
+    h.close()
+
+
+def test_powerlaw_16k_louvain_and_lp_bit_exact(ctx):
+    say("case: power-law 16,384 x 16,384, k=2: partition state, LP, Louvain")
+    h = powerlaw_16k()
+    dh, oh = lp_pair(ctx, h, 2, seed=42)
+    louvain_pair(dh, oh, seed=42)
+    dh.close()
+    h.close()
+
+
+def test_powerlaw_16k_quality_against_reference_semantics(ctx):
+    """End-to-end km1: median of the device pipeline over seeds 42-46 within
+    1% of the median of the oracle's reference-semantics pipeline
+    (asynchronous Louvain, CAS matching, optimistic LP; one thread, so the
+    reference median is deterministic), every run within (1 + eps)."""
+    say("case: power-law 16,384 x 16,384, k=2: quality over seeds 42-46")
+    h = powerlaw_16k()
+    a = h.arrays()
+    n = len(a["vw"])
+    dh = mtk_amd.Hypergraph(ctx, host=h)
+    oh = orc.Hypergraph(n, a["noff"], a["pins"], a["vw"], a["nw"])
+    maxw, _ = bounds(a["vw"], 2)
+    gpu, ref = [], []
+    for seed in range(42, 47):
+        part, st = dh.partition(2, EPS, seed)
+        assert np.bincount(part, minlength=2).max() <= maxw
+        gpu.append(st["km1"])
+        rpart, rkm1, _ = oh.partition(2, EPS, seed, sync=False, threads=1)
+        assert np.bincount(rpart, minlength=2).max() <= maxw
+        ref.append(rkm1)
+    say("km1 device %s (median %d), reference semantics %s (median %d)" % (
+        gpu, np.median(gpu), ref, np.median(ref)))
+    assert np.median(gpu) <= 1.01 * np.median(ref)
+    dh.close()
+    h.close()
+
+
+# -------------------------------------------------- power-law 1M x 2M, k=8
+def test_powerlaw_1m_partition_state_lp_louvain_bit_exact(ctx):
+    say("case: power-law 1,000,000 x 2,000,000, k=8: partition state, LP, Louvain")
+    h = powerlaw_1m()
+    dh, oh = lp_pair(ctx, h, 8, seed=42)
+    louvain_pair(dh, oh, seed=42)
+    dh.close()
+    h.close()
+
+
+# --------------------------------------------------- row-net 4M x 4M, k=64
+def test_rownet_4m_partition_state_and_lp_bit_exact(ctx):
+    say("case: row-net 4,000,000 x 4,000,000, k=64, weighted: partition state, LP")
+    h = rownet_4m()
+    dh, oh = lp_pair(ctx, h, 64, seed=42)
+    dh.close()
+    h.close()
+
+
+# ------------------------------------------------ batched, 4,096 members
+def test_batch_4096_members_equal_their_own_runs(ctx):
+    say("case: batch of 4,096 members (1k-64k vertices, k in {2,4,8,16}): Louvain + LP step")
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(42)
+    G = 4096
+    ns = np.exp(rng.uniform(np.log(1000), np.log(64000), G)).astype(np.int64)
+    ks = [int(k) for k in rng.choice([2, 4, 8, 16], G)]
+
+    def gen(g):
+        n = int(ns[g])
+        return mtk_amd.HostHypergraph.powerlaw(n, int(round(1.2 * n)), 2.5, 2, 256, window=512, local_fraction=0.8,
+                                               seed=42 + g)
+
+    mtk_amd.lib()
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        hosts = list(ex.map(gen, range(G)))
+    seeds = [42 + g for g in range(G)]
+    b = mtk_amd.Batch(ctx, hosts)
+    comm, lst = b.louvain(seeds=seeds, want_communities=True)
+    bp = mtk_amd.BatchPartition(b, ks)
+    bp.set_parts(np.concatenate([start_partition(int(n), k) for n, k in zip(ns, ks)]))
+    stats = bp.lp_refine(seeds, epsilon=EPS)
+    parts = bp.parts()
+    km1 = bp.km1()
+    voff = np.concatenate([[0], np.cumsum(ns)])
+    say("batch done; members alone")
+    for g in list(range(0, G, 128)) + [G - 1]:
+        n, k = int(ns[g]), ks[g]
+        dh = mtk_amd.Hypergraph(ctx, host=hosts[g])
+        c_alone, st_alone = dh.louvain(seed=seeds[g])
+        assert np.array_equal(comm[voff[g]:voff[g + 1]], c_alone), g
+        assert lst[g]["moves"] == st_alone["moves"] and lst[g]["levels"] == st_alone["levels"], g
+        maxw, perfect = bounds(np.ones(n), k)
+        dp = mtk_amd.Partition(dh, k)
+        dp.set_parts(start_partition(n, k))
+        st = dp.lp_refine([maxw] * k, [perfect] * k, seed=seeds[g])
+        assert np.array_equal(parts[voff[g]:voff[g + 1]], dp.parts()), g
+        assert km1[g] == dp.km1(), g
+        for key in ("delta", "moves", "reverts", "visits", "rounds"):
+            assert stats[g][key] == st[key], (g, key)
+        dp.close()
+        dh.close()
+    for g in (0, G // 2):
+        a = hosts[g].arrays()
+        n, k = int(ns[g]), ks[g]
+        maxw, perfect = bounds(a["vw"], k)
+        rpart, rkm1, _ = orc.Hypergraph(n, a["noff"], a["pins"], a["vw"], a["nw"]).lp_refine(
+            k, start_partition(n, k), [maxw] * k, [perfect] * k, seed=seeds[g], sync=True)
+        assert np.array_equal(parts[voff[g]:voff[g + 1]], rpart) and km1[g] == rkm1, g
+    bp.close()
+    b.close()
+    for x in hosts:
+        x.close()
+
+
+# --------------------------------------------- served, 10k to 20M pins
+def test_served_requests_10k_to_20m_pins_equal_alone(ctx):
+    say("case: served form, 10 requests of 10k-20M pins, k in {2,8,32,128}")
+    rng = np.random.default_rng(42)
+    mean = 4.6  # mean net size of the power law 2.3 on [2, 10000]
+    ps = [10_000, 20_000_000] + [float(x) for x in np.exp(rng.uniform(np.log(1e4), np.log(2e7), 8))]
+    ks = [int(k) for k in rng.choice([2, 8, 32, 128], len(ps))]
+    reqs = []
+    for i, p in enumerate(ps):
+        n = max(64, int(round(p / mean)))
+        h = mtk_amd.HostHypergraph.powerlaw(n, n, 2.3, 2, 10_000, window=4096, local_fraction=0.8,
+                                            vertex_weight_min=1, vertex_weight_max=8, seed=42 + i)
+        reqs.append((h, ks[i], 42 + i))
+    say("pins per request: %s" % [r[0].info().num_pins for r in reqs])
+    srv = mtk_amd.Server(0, slots=8)
+    tickets = [srv.submit(h, k, EPS, sd) for h, k, sd in reqs]
+    results = [srv.wait(t) for t in tickets]
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs) and st["failed"] == 0 and st["max_in_flight"] >= 2
+    for (h, k, sd), (parts, res) in zip(reqs, results):
+        dh = mtk_amd.Hypergraph(ctx, host=h)
+        apart, ast = dh.partition(k, EPS, sd)
+        dh.close()
+        assert np.array_equal(parts, apart)
+        assert res["km1"] == ast["km1"]
+        maxw, _ = bounds(h.arrays()["vw"], k)
+        assert np.bincount(parts, weights=h.arrays()["vw"], minlength=k).max() <= maxw
+        assert res["peak_bytes"] <= res["estimate_bytes"]
+    for h, _, _ in reqs:
+        h.close()

@@ -99,6 +99,33 @@ inline unsigned xcd_grid(uint64_t work, unsigned per_block, unsigned max_blocks 
   return unsigned((g + 7) & ~uint64_t(7));
 }
 
+// Groups the lanes of a wave by a small key ("slot", -1 = none).  Lanes with
+// the same slot are served together: up to 64 rounds (one per
+// distinct slot of the wave), each with the lanes of the lowest pending slot.
+// Returns, for the calling lane, the rank among its slot's lanes and the
+// slot's lane count and leader; every lane of the wave calls it.
+struct SlotGroup {
+  uint32_t rank, count;
+  bool leader;
+};
+__device__ inline SlotGroup slot_group(int slot) {
+  const int lane = threadIdx.x & 63;
+  uint64_t pending = __ballot(slot >= 0);
+  SlotGroup r{0, 0, false};
+  while (pending) {
+    const int first = __ffsll(static_cast<long long>(pending)) - 1;
+    const int s = __shfl(slot, first, 64);
+    const uint64_t same = __ballot(slot == s) & pending;
+    if (slot == s && ((same >> lane) & 1ull)) {
+      r.count = uint32_t(__popcll(same));
+      r.rank = uint32_t(__popcll(same & ((1ull << lane) - 1ull)));
+      r.leader = lane == first;
+    }
+    pending &= ~same;
+  }
+  return r;
+}
+
 // round tags (one per decision family)
 constexpr uint64_t kLpBucketTag = 0x4C50420000000000ull;
 constexpr uint64_t kLpCoinTag = 0x4C50430000000000ull;

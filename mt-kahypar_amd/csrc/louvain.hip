@@ -152,6 +152,8 @@ struct ExplicitGraph {
   ExplicitView view() const { return ExplicitView{n, idx.p, head.p, src.p, w.p}; }
 };
 
+constexpr int kLvClassesMax = 5;  // degree classes of local moving (see kLvClasses)
+
 struct LvArgs {
   uint64_t seed;
   uint32_t level, round;
@@ -163,9 +165,9 @@ struct LvArgs {
   const double* cvol;
   const int32_t* csize;
   const double* nvol;
-  uint32_t* move_u;
-  int32_t* move_to;
-  uint32_t* move_count;
+  // decision per position of the launch's list: the target cluster, or -1
+  // (written for every node of the sub-round, so no compaction is needed)
+  int32_t* dec;
   // batch of member hypergraphs as one disjoint union (nullptr: one
   // hypergraph): member and member-local id of every node, the member's
   // 1 / total volume, seed and whether it takes part in the round
@@ -236,17 +238,16 @@ __device__ inline bool better(double g, uint32_t f, double bg, uint32_t bf) {
   return g > bg || (g == bg && f < bf);
 }
 
-__device__ inline void lv_decide(const LvArgs& a, uint32_t u, double wfrom, double best_gain, int32_t best_key) {
+__device__ inline void lv_decide(const LvArgs& a, uint32_t pos, uint32_t u, double wfrom, double best_gain,
+                                 int32_t best_key) {
   const int32_t from = a.C[u];
   const double volU = a.nvol[u];
   const double mult = lv_recip(a, u) * volU;
   const double bg = wfrom - mult * (a.cvol[from] - volU);
-  if (best_key >= 0 && best_gain > bg) {
-    if (a.csize[from] == 1 && a.csize[best_key] == 1 && best_key > from) return;
-    const uint32_t slot = atomicAdd(a.move_count, 1u);
-    a.move_u[slot] = u;
-    a.move_to[slot] = best_key;
-  }
+  int32_t to = -1;
+  if (best_key >= 0 && best_gain > bg && !(a.csize[from] == 1 && a.csize[best_key] == 1 && best_key > from))
+    to = best_key;
+  a.dec[pos] = to;
 }
 
 // The node's own inputs of the decision, loaded ahead of the arc sweep so
@@ -256,15 +257,13 @@ struct LvNode {
   double volU, mult, cvol_from;
   int32_t csize_from;
 };
-__device__ inline void lv_decide(const LvArgs& a, uint32_t u, const LvNode& nd, double wfrom, double best_gain,
+__device__ inline void lv_decide(const LvArgs& a, uint32_t pos, const LvNode& nd, double wfrom, double best_gain,
                                  int32_t best_key) {
   const double bg = wfrom - nd.mult * (nd.cvol_from - nd.volU);
-  if (best_key >= 0 && best_gain > bg) {
-    if (nd.csize_from == 1 && a.csize[best_key] == 1 && best_key > nd.from) return;
-    const uint32_t slot = atomicAdd(a.move_count, 1u);
-    a.move_u[slot] = u;
-    a.move_to[slot] = best_key;
-  }
+  int32_t to = -1;
+  if (best_key >= 0 && best_gain > bg && !(nd.csize_from == 1 && a.csize[best_key] == 1 && best_key > nd.from))
+    to = best_key;
+  a.dec[pos] = to;
 }
 
 // ---------------------------------------------------------- move kernels --
@@ -332,7 +331,7 @@ louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
         }
       }
     }
-    lv_decide(a, u, nd, wfrom, bgain, bkey);
+    lv_decide(a, idx, nd, wfrom, bgain, bkey);
   }
 }
 
@@ -419,7 +418,7 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
         wfrom += __shfl_xor(wfrom, o, G);
         if (ok >= 0 && (bkey < 0 || better(og, of, bgain, bfirst))) { bgain = og; bfirst = of; bkey = ok; }
       }
-      if (lane == 0) lv_decide(a, u, nd, wfrom, bgain, bkey);
+      if (lane == 0) lv_decide(a, idx, nd, wfrom, bgain, bkey);
     }
     lds_wave_sync();
   }
@@ -551,7 +550,7 @@ __global__ void __launch_bounds__(256) hv_accum_kernel(View g, LvArgs a, HeavyAr
         g.arc(h.list[i], pos, v, w);
         slot = h.arc_slot[j];
       }
-      for (int it = 0; it < 2; ++it) {  // combine lanes of the lowest pending slot
+      for (int it = 0; it < 4; ++it) {  // combine lanes of the lowest pending slot
         const uint64_t act = __ballot(pending);
         if (!act) break;
         const int leader = __ffsll(static_cast<long long>(act)) - 1;
@@ -650,7 +649,7 @@ __global__ void hv_decide_kernel(LvArgs a, HeavyArgs h, const HvBest* __restrict
         r.gain = b.gain; r.first = b.first; r.key = b.key;
       }
     }
-    lv_decide(a, h.list[i], r.wfrom, r.gain, r.key);
+    lv_decide(a, i, h.list[i], r.wfrom, r.gain, r.key);
   }
 }
 
@@ -662,55 +661,73 @@ __global__ void hv_decide_kernel(LvArgs a, HeavyArgs h, const HvBest* __restrict
 __device__ inline void lv_volume_add(double* __restrict__ cvol, int32_t* __restrict__ csize, bool valid, int32_t key,
                                      double dv, int dn) {
   const int lane = threadIdx.x & 63;
-  const uint64_t act = __ballot(valid);
-  const int leader = act ? __ffsll(static_cast<long long>(act)) - 1 : 0;
-  const int32_t lk = __shfl(key, leader, 64);
-  const bool mine = valid && key == lk;
-  double sum = mine ? dv : 0.0;
-  for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
-  const int cnt = __popcll(__ballot(mine));
-  if (act && lane == leader) {
-    atomicAdd(&cvol[lk], sum);
-    atomicAdd(&csize[lk], dn * cnt);
+  bool pending = valid;
+  // up to four passes, each combining the lanes of the lowest pending key
+  for (int it = 0; it < 4; ++it) {
+    const uint64_t act = __ballot(pending);
+    if (!act) return;
+    const int leader = __ffsll(static_cast<long long>(act)) - 1;
+    const int32_t lk = __shfl(key, leader, 64);
+    const bool mine = pending && key == lk;
+    double sum = mine ? dv : 0.0;
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    const int cnt = __popcll(__ballot(mine));
+    if (lane == leader) {
+      atomicAdd(&cvol[lk], sum);
+      atomicAdd(&csize[lk], dn * cnt);
+    }
+    if (mine) pending = false;
   }
-  if (valid && !mine) {
+  if (pending) {
     atomicAdd(&cvol[key], dv);
     atomicAdd(&csize[key], dn);
   }
 }
 
-// Applies a sub-round's moves.  The move counter alternates between two
-// words: this launch clears the other one, which the next sub-round fills.
-// Whole waves step through the moves together (the volume and move-count
-// updates combine across lanes).
-__global__ void louvain_apply_kernel(const uint32_t* __restrict__ mu, const int32_t* __restrict__ mto,
-                                     const uint32_t* __restrict__ mcount, const double* __restrict__ nvol,
-                                     int32_t* __restrict__ C, double* __restrict__ cvol, int32_t* __restrict__ csize,
-                                     uint32_t* __restrict__ moved_total, uint32_t* __restrict__ next_count,
-                                     const uint32_t* __restrict__ nmember, uint32_t* __restrict__ gmoved) {
-  const uint32_t cnt = *mcount;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    atomicAdd(moved_total, cnt);
-    *next_count = 0;
-  }
+// The work-list ranges of one sub-round (one per degree class).
+struct LvRanges {
+  uint32_t off[kLvClassesMax], cnt[kLvClassesMax];
+  uint32_t total;
+};
+
+// Applies a sub-round's moves: every position of its list ranges whose
+// decision is a cluster moves its node.  Whole waves step together (the
+// volume and member updates combine across lanes); each block adds its move
+// count once.
+__global__ void __launch_bounds__(256)
+louvain_apply_kernel(const uint32_t* __restrict__ list, const int32_t* __restrict__ dec, LvRanges rg,
+                     const double* __restrict__ nvol, int32_t* __restrict__ C, double* __restrict__ cvol,
+                     int32_t* __restrict__ csize, uint32_t* __restrict__ moved_total,
+                     const uint32_t* __restrict__ nmember, uint32_t* __restrict__ gmoved) {
+  __shared__ uint32_t bmoves;
+  if (threadIdx.x == 0) bmoves = 0;
+  __syncthreads();
   const uint32_t stride = gridDim.x * blockDim.x;
   const int lane = threadIdx.x & 63;
-  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < cnt; base += stride) {
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < rg.total; base += stride) {
     const uint32_t j = base + uint32_t(lane);
-    const bool valid = j < cnt;
+    bool valid = false;
     uint32_t u = 0;
     int32_t to = 0, from = 0;
     double vu = 0.0;
-    if (valid) {
-      u = mu[j];
-      to = mto[j];
-      from = C[u];
-      C[u] = to;
-      vu = nvol[u];
+    if (j < rg.total) {
+      uint32_t r = 0, k = j;
+      while (k >= rg.cnt[r]) { k -= rg.cnt[r]; ++r; }
+      const uint32_t pos = rg.off[r] + k;
+      to = dec[pos];
+      if (to >= 0) {
+        valid = true;
+        u = list[pos];
+        from = C[u];
+        C[u] = to;
+        vu = nvol[u];
+      }
     }
+    const uint64_t movers = __ballot(valid);
+    if (lane == 0 && movers) atomicAdd(&bmoves, uint32_t(__popcll(movers)));
     if (nmember) {  // moves per member (batch): consecutive moves mostly share the member
       const uint32_t g = valid ? nmember[u] : 0u;
-      const uint64_t act = __ballot(valid);
+      const uint64_t act = movers;
       const int leader = act ? __ffsll(static_cast<long long>(act)) - 1 : 0;
       const uint32_t lg = __shfl(g, leader, 64);
       const bool mine = valid && g == lg;
@@ -721,12 +738,46 @@ __global__ void louvain_apply_kernel(const uint32_t* __restrict__ mu, const int3
     lv_volume_add(cvol, csize, valid, to, vu, 1);
     lv_volume_add(cvol, csize, valid, from, -vu, -1);
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && bmoves) atomicAdd(moved_total, bmoves);
+}
+
+// Maximum of v over the block, added to *out with one atomic per block
+// (atomics on one address run one after another in memory, so the grids
+// that use it are kept to a few hundred blocks).
+__device__ inline void block_max_to(uint32_t* out, uint32_t v) {
+  __shared__ uint32_t wmax[16];
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m = 0;
+    for (uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w) m = max(m, wmax[w]);
+    atomicMax(out, m);
+  }
 }
 
 // node volumes (graph.cpp:319-327), sequential sum in arc order per node
 template <class View>
 __device__ inline bool constant_arc_weights(const View&, uint32_t) { return false; }
 __device__ inline bool constant_arc_weights(const StarView& g, uint32_t u) { return g.wt == 0 && u >= g.N; }
+// Under the uniform weighting a hypernode's arcs weigh w(e) of its incident
+// nets: the sequential sum is the sum of the net weights (its degree for
+// unit weights), read without the arc heads.
+template <class View>
+__device__ inline bool net_weight_sum(const View&, uint32_t, double&) { return false; }
+__device__ inline bool net_weight_sum(const StarView& g, uint32_t u, double& s) {
+  if (g.wt != 0 || u >= g.N) return false;
+  const uint32_t jb = g.voff[u], je = g.voff[u + 1];
+  if (g.unit) {
+    s = double(je - jb);
+    return true;
+  }
+  double acc = 0.0;
+  for (uint32_t j = jb; j < je; ++j) acc += double(g.nw[g.inc[j]]);
+  s = acc;
+  return true;
+}
 
 template <class View>
 __global__ void node_volume_kernel(View g, uint32_t n, double* __restrict__ vol, uint32_t* __restrict__ deg_max) {
@@ -741,6 +792,7 @@ __global__ void node_volume_kernel(View g, uint32_t n, double* __restrict__ vol,
       double w;
       g.arc(u, 0, v, w);
       s = double(d) * w;
+    } else if (net_weight_sum(g, u, s)) {
     } else {
       for (uint32_t i = 0; i < d; ++i) {
         uint32_t v;
@@ -752,8 +804,7 @@ __global__ void node_volume_kernel(View g, uint32_t n, double* __restrict__ vol,
     vol[u] = s;
     dm = max(dm, d);
   }
-  for (int o = 32; o > 0; o >>= 1) dm = max(dm, __shfl_xor(dm, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(deg_max, dm);
+  block_max_to(deg_max, dm);
 }
 
 template <class View>
@@ -761,8 +812,7 @@ __global__ void max_degree_kernel(View g, uint32_t n, uint32_t* __restrict__ deg
   uint32_t dm = 0;
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x)
     dm = max(dm, g.degree(u));
-  for (int o = 32; o > 0; o >>= 1) dm = max(dm, __shfl_xor(dm, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(deg_max, dm);
+  block_max_to(deg_max, dm);
 }
 
 // degree classes: 0: <= 16 arcs, 1: <= 64, 2: <= 128, 3: <= 1024, 4: heavier;
@@ -771,6 +821,7 @@ __global__ void max_degree_kernel(View g, uint32_t n, uint32_t* __restrict__ deg
 // same group kernel with maps of 128 and 256 slots: the smaller maps of the
 // 17..64-arc nodes leave room in LDS for twice the waves.
 constexpr int kLvClasses = 5;
+static_assert(kLvClasses <= kLvClassesMax, "class ranges of a sub-round");
 // In a batch, nodes of members that sit the round out get no slot (-1), and
 // the bucket hash takes the member's seed and local id.
 template <class View>
@@ -799,38 +850,110 @@ __device__ inline int lv_slot(const View& g, const LvArgs& a, uint32_t u) {
 // ascending node order up to the order inside one block's range, so list
 // neighbours are id neighbours and the XCD-chunked move kernels gather from
 // one region of the cluster arrays per XCD.
-constexpr unsigned kLvListBlocks = 2048;
+constexpr unsigned kLvListBlocks = 1024;
 inline unsigned lv_list_blocks(uint32_t n) { return grid_for(n, 256, kLvListBlocks); }
+// the block's node range: tile-aligned chunks (a wave of 64 consecutive
+// nodes then holds one tile, whose nodes share their bucket)
+__device__ inline void lv_block_range(uint32_t n, uint32_t& lo, uint32_t& hi) {
+  uint64_t per = (uint64_t(n) + gridDim.x - 1) / gridDim.x;
+  per = (per + 63) & ~uint64_t(63);
+  lo = uint32_t(min(uint64_t(blockIdx.x) * per, uint64_t(n)));
+  hi = uint32_t(min(uint64_t(lo) + per, uint64_t(n)));
+}
 
 template <class View>
 __global__ void __launch_bounds__(256)
-lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uint64_t* __restrict__ sarcs,
-                uint64_t* __restrict__ sbytes, uint32_t* __restrict__ bmat) {
+lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ bmat, uint64_t* __restrict__ amat,
+                uint64_t* __restrict__ bymat) {
   __shared__ uint32_t h[kLvClasses * 256];
   __shared__ unsigned long long ha[kLvClasses * 256];
   __shared__ unsigned long long hb[kLvClasses * 256];
   const int slots = kLvClasses * a.buckets;
   for (int i = threadIdx.x; i < slots; i += blockDim.x) { h[i] = 0; ha[i] = 0; hb[i] = 0; }
   __syncthreads();
-  const uint32_t per = n / gridDim.x + (n % gridDim.x != 0);
-  const uint32_t lo = uint32_t(min(uint64_t(blockIdx.x) * per, uint64_t(n)));
-  const uint32_t hi = uint32_t(min(uint64_t(lo) + per, uint64_t(n)));
-  for (uint32_t u = lo + threadIdx.x; u < hi; u += blockDim.x) {
-    const int slot = lv_slot(g, a, u);
-    if (slot >= 0) {
-      const uint32_t d = min(g.degree(u), a.sampling);
-      atomicAdd(&h[slot], 1u);
-      atomicAdd(&ha[slot], static_cast<unsigned long long>(d));
-      // algorithmic bytes of the node's move: 28 per node + its (sampled) arcs
-      if (sbytes) atomicAdd(&hb[slot], 28ull + uint64_t(d) * g.arc_bytes(u));
+  uint32_t lo, hi;
+  lv_block_range(n, lo, hi);
+  // whole waves step together: lanes of one slot add with one LDS atomic
+  for (uint32_t base = lo + (threadIdx.x & ~63u); base < hi; base += blockDim.x) {
+    const uint32_t u = base + (threadIdx.x & 63u);
+    const int slot = u < hi ? lv_slot(g, a, u) : -1;
+    const uint32_t d = slot >= 0 ? min(g.degree(u), a.sampling) : 0u;
+    const uint64_t by = slot >= 0 && bymat ? 28ull + uint64_t(d) * g.arc_bytes(u) : 0ull;
+    const SlotGroup sg = slot_group(slot);
+    // per-slot sums of the arcs and bytes: one shuffle reduction per
+    // distinct slot of the wave, led by its lowest lane
+    uint64_t pending = __ballot(slot >= 0);
+    while (pending) {
+      const int first = __ffsll(static_cast<long long>(pending)) - 1;
+      const int s = __shfl(slot, first, 64);
+      const bool mine = slot == s && slot >= 0;
+      uint64_t sa = mine ? uint64_t(d) : 0ull, sb = mine ? by : 0ull;
+      for (int o = 32; o > 0; o >>= 1) {
+        sa += __shfl_xor(sa, o, 64);
+        sb += __shfl_xor(sb, o, 64);
+      }
+      if ((threadIdx.x & 63) == uint32_t(first)) {
+        atomicAdd(&ha[s], static_cast<unsigned long long>(sa));
+        if (bymat) atomicAdd(&hb[s], static_cast<unsigned long long>(sb));
+      }
+      pending &= ~__ballot(mine);
     }
+    if (sg.leader) atomicAdd(&h[slot], sg.count);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < slots; i += blockDim.x) {
-    bmat[size_t(i) * gridDim.x + blockIdx.x] = h[i];
-    if (h[i]) atomicAdd(&counts[i], h[i]);
-    if (ha[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&sarcs[i]), ha[i]);
-    if (sbytes && hb[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&sbytes[i]), hb[i]);
+    const size_t at = size_t(i) * gridDim.x + blockIdx.x;
+    bmat[at] = h[i];
+    amat[at] = ha[i];
+    if (bymat) bymat[at] = hb[i];
+  }
+}
+
+// One block per slot: the slot's node count, arc count and bytes over the
+// list blocks, and the exclusive scan of its per-block node counts in place
+// (offsets inside the slot's list range).  No global atomics: a grid of
+// counting blocks would otherwise queue on each slot's counter.
+__global__ void __launch_bounds__(256)
+lv_totals_kernel(uint32_t nb, uint32_t* __restrict__ bmat, const uint64_t* __restrict__ amat,
+                 const uint64_t* __restrict__ bymat, uint32_t* __restrict__ counts, uint64_t* __restrict__ sarcs,
+                 uint64_t* __restrict__ sbytes) {
+  __shared__ uint32_t part[256];
+  __shared__ unsigned long long ra[4], rb[4];
+  const uint32_t slot = blockIdx.x, t = threadIdx.x;
+  uint32_t* row = bmat + size_t(slot) * nb;
+  const uint32_t per = (nb + blockDim.x - 1) / blockDim.x;
+  const uint32_t b0 = min(nb, t * per), b1 = min(nb, b0 + per);
+  uint32_t sum = 0;
+  uint64_t sa = 0, sb = 0;
+  for (uint32_t b = b0; b < b1; ++b) {
+    sum += row[b];
+    sa += amat[size_t(slot) * nb + b];
+    if (bymat) sb += bymat[size_t(slot) * nb + b];
+  }
+  part[t] = sum;
+  for (int o = 32; o > 0; o >>= 1) {
+    sa += __shfl_xor(sa, o, 64);
+    sb += __shfl_xor(sb, o, 64);
+  }
+  if ((t & 63) == 0) { ra[t >> 6] = sa; rb[t >> 6] = sb; }
+  __syncthreads();
+  if (t == 0) {  // exclusive scan of the thread sums (256 values)
+    uint32_t run = 0;
+    for (uint32_t i = 0; i < blockDim.x; ++i) {
+      const uint32_t x = part[i];
+      part[i] = run;
+      run += x;
+    }
+    counts[slot] = run;
+    sarcs[slot] = ra[0] + ra[1] + ra[2] + ra[3];
+    if (sbytes) sbytes[slot] = rb[0] + rb[1] + rb[2] + rb[3];
+  }
+  __syncthreads();
+  uint32_t run = part[t];
+  for (uint32_t b = b0; b < b1; ++b) {
+    const uint32_t x = row[b];
+    row[b] = run;
+    run += x;
   }
 }
 
@@ -838,17 +961,40 @@ lv_count_kernel(View g, LvArgs a, uint32_t n, uint32_t* __restrict__ counts, uin
 // lv_count_kernel, same grid) at the block's scanned position of their slot.
 template <class View>
 __global__ void __launch_bounds__(256)
-lv_scatter_kernel(View g, LvArgs a, uint32_t n, const uint32_t* __restrict__ bpos, uint32_t* __restrict__ list) {
+lv_scatter_kernel(View g, LvArgs a, uint32_t n, const uint32_t* __restrict__ bpos, const uint32_t* __restrict__ counts,
+                  uint32_t* __restrict__ list) {
   __shared__ uint32_t h[kLvClasses * 256];
   const int slots = kLvClasses * a.buckets;
-  for (int i = threadIdx.x; i < slots; i += blockDim.x) h[i] = bpos[size_t(i) * gridDim.x + blockIdx.x];
+  if (threadIdx.x == 0) {  // start of every slot's range: the counts of the slots before it
+    uint32_t run = 0;
+    for (int i = 0; i < slots; ++i) {
+      h[i] = run;
+      run += counts[i];
+    }
+  }
   __syncthreads();
-  const uint32_t per = n / gridDim.x + (n % gridDim.x != 0);
-  const uint32_t lo = uint32_t(min(uint64_t(blockIdx.x) * per, uint64_t(n)));
-  const uint32_t hi = uint32_t(min(uint64_t(lo) + per, uint64_t(n)));
-  for (uint32_t u = lo + threadIdx.x; u < hi; u += blockDim.x) {
-    const int slot = lv_slot(g, a, u);
-    if (slot >= 0) list[atomicAdd(&h[slot], 1u)] = u;
+  for (int i = threadIdx.x; i < slots; i += blockDim.x) h[i] += bpos[size_t(i) * gridDim.x + blockIdx.x];
+  __syncthreads();
+  uint32_t lo, hi;
+  lv_block_range(n, lo, hi);
+  // whole waves step together: the lanes of one slot reserve their places
+  // with one LDS atomic and keep their lane order inside it
+  for (uint32_t base = lo + (threadIdx.x & ~63u); base < hi; base += blockDim.x) {
+    const uint32_t u = base + (threadIdx.x & 63u);
+    const int slot = u < hi ? lv_slot(g, a, u) : -1;
+    const SlotGroup sg = slot_group(slot);
+    uint32_t start = 0;
+    if (sg.leader) start = atomicAdd(&h[slot], sg.count);
+    // the slot's start travels from its leader to its lanes
+    uint64_t pending = __ballot(slot >= 0);
+    while (pending) {
+      const int first = __ffsll(static_cast<long long>(pending)) - 1;
+      const int s = __shfl(slot, first, 64);
+      const uint32_t st = __shfl(start, first, 64);
+      const bool mine = slot == s && slot >= 0;
+      if (mine) list[st + sg.rank] = u;
+      pending &= ~__ballot(mine);
+    }
   }
 }
 
@@ -1076,8 +1222,12 @@ struct LouvainRun {
     int trials = 0;  // sub-rounds that time a shape
     uint64_t key = 0;
   };
+  // Round 0 is not timed: every node is then a singleton and each arc a
+  // cluster of its own, while later rounds insert fewer, repeated keys, so
+  // the shapes are compared on the first round that follows (round 0 runs 8
+  // lanes per node).
   template <class View>
-  LaneChoice choose_lanes(const std::vector<uint32_t>& cnt, const std::vector<uint64_t>& sa, int B) {
+  LaneChoice choose_lanes(const std::vector<uint32_t>& cnt, const std::vector<uint64_t>& sa, int B, int round) {
     LaneChoice ch;
     if (c.tune.lv_group_lanes) {
       ch.lanes = c.tune.lv_group_lanes;
@@ -1092,6 +1242,10 @@ struct LouvainRun {
     auto it = c.tune.lv_group_choice.find(ch.key);
     if (it != c.tune.lv_group_choice.end()) {
       ch.lanes = it->second;
+      return ch;
+    }
+    if (round == 0 && prm.max_passes > 1) {
+      ch.lanes = 8;
       return ch;
     }
     ch.trials = B >= kTrials ? kTrials : (B >= 4 ? 4 : 0);
@@ -1130,6 +1284,22 @@ struct LouvainRun {
     return ch.lanes;
   }
 
+  // applies the decisions of sub-round sr (its list ranges of every class)
+  void launch_apply(const uint32_t* list, const int32_t* dec, const std::vector<uint32_t>& off,
+                    const std::vector<uint32_t>& cnt, int B, int sr, const double* nvol, int32_t* C, double* cvol,
+                    int32_t* csize, uint32_t* moved, const uint32_t* nmember, uint32_t* gmoved) {
+    LvRanges rg{};
+    for (int cl = 0; cl < kLvClasses; ++cl) {
+      rg.off[cl] = off[cl * B + sr];
+      rg.cnt[cl] = cnt[cl * B + sr];
+      rg.total += rg.cnt[cl];
+    }
+    if (!rg.total) return;
+    louvain_apply_kernel<<<grid_for(rg.total, 256, 256), 256, 0, c.stream>>>(list, dec, rg, nvol, C, cvol, csize,
+                                                                            moved, nmember, gmoved);
+    MKP_LAUNCH_CHECK();
+  }
+
   template <class View>
   void launch_move(const View& g, const LvArgs& a, const uint32_t* list, const std::vector<uint32_t>& off,
                    const std::vector<uint32_t>& cnt, const std::vector<uint64_t>& sa,
@@ -1138,10 +1308,16 @@ struct LouvainRun {
     const hipStream_t s = c.stream;
     const int B = a.buckets;
     const int s0 = 0 * B + sr, s1 = 1 * B + sr, s1w = 2 * B + sr, s2 = 3 * B + sr, s3 = 4 * B + sr;
+    // each launch writes the decisions of its own list range
+    auto at = [&](int slot) {
+      LvArgs x = a;
+      x.dec = a.dec + off[slot];
+      return x;
+    };
     if (cnt[s0]) {
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s0]),
                      std::string("louvain_move_small_kernel<") + View::kName + ">");
-      louvain_move_small_kernel<View><<<xcd_grid(cnt[s0], 256, 65536), 256, 0, s>>>(g, a, list + off[s0], cnt[s0]);
+      louvain_move_small_kernel<View><<<xcd_grid(cnt[s0], 256, 65536), 256, 0, s>>>(g, at(s0), list + off[s0], cnt[s0]);
     }
     if (cnt[s1]) {  // 17..64 arcs: 128-slot maps, lanes per node autotuned
       const int G = lanes;
@@ -1150,32 +1326,31 @@ struct LouvainRun {
                      std::string("louvain_move_group_kernel<") + View::kName + ", " + std::to_string(G) + ", 7, " +
                          (G == 8 ? "128>" : "256>"));
       if (G == 8)
-        louvain_move_group_kernel<View, 8, 7, 128><<<xcd_grid(cnt[s1], 16, 65536), 128, 0, s>>>(g, a, list + off[s1],
+        louvain_move_group_kernel<View, 8, 7, 128><<<xcd_grid(cnt[s1], 16, 65536), 128, 0, s>>>(g, at(s1), list + off[s1],
                                                                                                 cnt[s1]);
       else if (G == 16)
-        louvain_move_group_kernel<View, 16, 7, 256><<<xcd_grid(cnt[s1], 16, 65536), 256, 0, s>>>(g, a, list + off[s1],
+        louvain_move_group_kernel<View, 16, 7, 256><<<xcd_grid(cnt[s1], 16, 65536), 256, 0, s>>>(g, at(s1), list + off[s1],
                                                                                                  cnt[s1]);
       else if (G == 64)
-        louvain_move_group_kernel<View, 64, 7, 256><<<xcd_grid(cnt[s1], 4, 65536), 256, 0, s>>>(g, a, list + off[s1],
+        louvain_move_group_kernel<View, 64, 7, 256><<<xcd_grid(cnt[s1], 4, 65536), 256, 0, s>>>(g, at(s1), list + off[s1],
                                                                                                 cnt[s1]);
       else
-        louvain_move_group_kernel<View, 32, 7, 256><<<xcd_grid(cnt[s1], 8, 65536), 256, 0, s>>>(g, a, list + off[s1],
+        louvain_move_group_kernel<View, 32, 7, 256><<<xcd_grid(cnt[s1], 8, 65536), 256, 0, s>>>(g, at(s1), list + off[s1],
                                                                                                 cnt[s1]);
       if (tb) MKP_HIP(hipEventRecord(tb, s));
     }
     if (cnt[s1w]) {  // 65..128 arcs: 256-slot maps
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s1w]),
                      std::string("louvain_move_group_kernel<") + View::kName + ", 32, 8, 256>");
-      louvain_move_group_kernel<View, 32, 8, 256><<<xcd_grid(cnt[s1w], 8, 65536), 256, 0, s>>>(
-          g, a, list + off[s1w], cnt[s1w]);
+      louvain_move_group_kernel<View, 32, 8, 256><<<xcd_grid(cnt[s1w], 8, 65536), 256, 0, s>>>(g, at(s1w), list + off[s1w], cnt[s1w]);
     }
     if (cnt[s2]) {
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s2]),
                      std::string("louvain_move_group_kernel<") + View::kName + ", 64, 11, 64>");
-      louvain_move_group_kernel<View, 64, 11, 64><<<xcd_grid(cnt[s2], 1, 65536), 64, 0, s>>>(g, a, list + off[s2],
+      louvain_move_group_kernel<View, 64, 11, 64><<<xcd_grid(cnt[s2], 1, 65536), 64, 0, s>>>(g, at(s2), list + off[s2],
                                                                                              cnt[s2]);
     }
-    if (cnt[s3]) launch_heavy(g, a, list + off[s3], off[s3] - off[4 * B], cnt[s3], sa[s3], double(sb[s3]));
+    if (cnt[s3]) launch_heavy(g, at(s3), list + off[s3], off[s3] - off[4 * B], cnt[s3], sa[s3], double(sb[s3]));
     MKP_LAUNCH_CHECK();
   }
 
@@ -1189,13 +1364,15 @@ struct LouvainRun {
     cvol_.alloc(std::max<uint32_t>(n, 1));
     DevBuf<double>& cvol = cvol_;
     DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
-    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), mu(std::max<uint32_t>(n, 1));
+    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1));
+    DevBuf<int32_t> dec(std::max<uint32_t>(n, 1));  // decision per list position
     // per-(slot, block) node counts of the work lists and their scan
     const unsigned lblocks = lv_list_blocks(n);
-    DevBuf<uint32_t> bmat(size_t(slots) * lblocks), bpos(size_t(slots) * lblocks + 1);
+    DevBuf<uint32_t> bmat(size_t(slots) * lblocks);
+    DevBuf<uint64_t> amat(size_t(slots) * lblocks), bymat(c.prof.enabled ? size_t(slots) * lblocks : 1);
     // the round's counters in one buffer, cleared by one launch and read back
-    // by one copy: [sarcs u64][sbytes u64][counts u32][unused u32][two move
-    // counters u32][moved u32 of even rounds, of odd rounds].  The moves of
+    // by one copy: [sarcs u64][sbytes u64][counts u32][unused u32][two unused
+    // u32][moved u32 of even rounds, of odd rounds].  The moves of
     // round r are read back with the counts of round r + 1, which are
     // computed before the loop condition is known and dropped if it fails.
     const size_t cw = (size_t(slots) + 1) / 2;  // words of slots u32 entries
@@ -1204,10 +1381,8 @@ struct LouvainRun {
     uint64_t* sarcs = rb.p;
     uint64_t* sbytes = rb.p + slots;
     uint32_t* counts = reinterpret_cast<uint32_t*>(rb.p + 2 * slots);
-    uint32_t* mcount = reinterpret_cast<uint32_t*>(rb.p + 2 * slots + 2 * cw);
-    uint32_t* moved_rounds = mcount + 2;
+    uint32_t* moved_rounds = reinterpret_cast<uint32_t*>(rb.p + 2 * slots + 2 * cw) + 2;
     std::vector<uint64_t> sb(slots, 0), sa(slots, 0);
-    DevBuf<int32_t> mto(std::max<uint32_t>(n, 1));
     lv_init_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, nvol, C, cvol.p, csize.p);
     MKP_LAUNCH_CHECK();
     if (num_arcs == 0) {
@@ -1217,7 +1392,7 @@ struct LouvainRun {
     LvArgs a;
     a.seed = prm.seed; a.level = level; a.sampling = uint32_t(std::min<uint64_t>(prm.sampling, 0xffffffffu));
     a.recip = 1.0 / tv; a.C = C; a.cvol = cvol.p; a.csize = csize.p; a.nvol = nvol;
-    a.move_u = mu.p; a.move_to = mto.p; a.move_count = mcount;
+    a.dec = dec.p;
     a.buckets = B;
     bool changed = false;
     uint64_t moved = n;
@@ -1235,10 +1410,10 @@ struct LouvainRun {
       a.bucket_salt = bucket_salt(a.seed, kLvBucketTag | (uint64_t(level) << 8) | uint64_t(r));
       fill_async<uint64_t>(rb.p, 0ull, rwords - 1, s);
       fill_async<uint32_t>(moved_rounds + (r & 1), 0u, 1, s);
-      lv_count_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, counts, sarcs, c.prof.enabled ? sbytes : nullptr,
-                                                     bmat.p);
+      uint64_t* by = c.prof.enabled ? bymat.p : nullptr;
+      lv_count_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, bmat.p, amat.p, by);
+      lv_totals_kernel<<<slots, 256, 0, s>>>(lblocks, bmat.p, amat.p, by, counts, sarcs, by ? sbytes : nullptr);
       MKP_LAUNCH_CHECK();
-      exclusive_scan_u32(c, bmat.p, bpos.p, size_t(slots) * lblocks);
       MKP_HIP(hipMemcpyAsync(stg, rb.p, rwords * 8, hipMemcpyDeviceToHost, s));
       c.sync();
     };
@@ -1257,23 +1432,17 @@ struct LouvainRun {
       if (hnodes) reserve_heavy(harcs, hnodes);
       uint32_t run = 0;
       for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
-      lv_scatter_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, bpos.p, list.p);
+      lv_scatter_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, bmat.p, counts, list.p);
       MKP_LAUNCH_CHECK();
       if (hnodes) prepare_heavy(g, a, list.p + off[4 * B], hnodes);
-      const LaneChoice lanes = choose_lanes<View>(cnt, sa, B);
+      const LaneChoice lanes = choose_lanes<View>(cnt, sa, B, r);
       for (int sr = 0; sr < B; ++sr) {
         a.bucket = sr;
-        a.move_count = mcount + (sr & 1);
         hipEvent_t ta, tb;
         const int lanes_now = lanes_for(lanes, sr, &ta, &tb);
         launch_move(g, a, list.p, off, cnt, sa, sb, sr, lanes_now, ta, tb);
-        // at most one move per node of this sub-round
-        uint32_t sub_nodes = 0;
-        for (int cl = 0; cl < kLvClasses; ++cl) sub_nodes += cnt[cl * B + sr];
-        louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(
-            mu.p, mto.p, a.move_count, nvol, C, cvol.p, csize.p, moved_rounds + (r & 1),
-            mcount + ((sr + 1) & 1), nullptr, nullptr);
-        MKP_LAUNCH_CHECK();
+        launch_apply(list.p, dec.p, off, cnt, B, sr, nvol, C, cvol.p, csize.p, moved_rounds + (r & 1), nullptr,
+                     nullptr);
       }
       count_round(r + 1);
       finish_lanes(lanes, sa, B);
@@ -1295,9 +1464,9 @@ struct LouvainRun {
     DevBuf<uint32_t> dmax(1);
     fill_async<uint32_t>(dmax.p, 0u, 1, c.stream);
     if (nvol)
-      node_volume_kernel<View><<<grid_for(n, 256, 65536), 256, 0, c.stream>>>(g, n, nvol, dmax.p);
+      node_volume_kernel<View><<<grid_for(n, 256, 256), 256, 0, c.stream>>>(g, n, nvol, dmax.p);
     else
-      max_degree_kernel<View><<<grid_for(n, 256, 65536), 256, 0, c.stream>>>(g, n, dmax.p);
+      max_degree_kernel<View><<<grid_for(n, 256, 256), 256, 0, c.stream>>>(g, n, dmax.p);
     MKP_LAUNCH_CHECK();
     MKP_HIP(hipMemcpyAsync(&max_degree_, dmax.p, 4, hipMemcpyDeviceToHost, c.stream));
     c.sync();
@@ -1601,9 +1770,11 @@ struct LouvainBatchRun : LouvainRun {
     changed.assign(G, 0);
     cvol_.alloc(std::max<uint32_t>(n, 1));
     DevBuf<int32_t> csize(std::max<uint32_t>(n, 1));
-    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), mu(std::max<uint32_t>(n, 1));
+    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1));
+    DevBuf<int32_t> dec(std::max<uint32_t>(n, 1));  // decision per list position
     const unsigned lblocks = lv_list_blocks(n);
-    DevBuf<uint32_t> bmat(size_t(slots) * lblocks), bpos(size_t(slots) * lblocks + 1);
+    DevBuf<uint32_t> bmat(size_t(slots) * lblocks);
+    DevBuf<uint64_t> amat(size_t(slots) * lblocks), bymat(c.prof.enabled ? size_t(slots) * lblocks : 1);
     // the round's counters in one buffer, laid out as in local_moving
     const size_t cw = (size_t(slots) + 1) / 2;
     const size_t rwords = 2 * size_t(slots) + 2 * cw + 2;
@@ -1612,8 +1783,6 @@ struct LouvainBatchRun : LouvainRun {
     uint64_t* sbytes = rb.p + slots;
     uint32_t* counts = reinterpret_cast<uint32_t*>(rb.p + 2 * slots);
     uint32_t* moved_total = reinterpret_cast<uint32_t*>(rb.p + 2 * slots + 2 * cw);
-    uint32_t* mcount = moved_total + 1;
-    DevBuf<int32_t> mto(std::max<uint32_t>(n, 1));
     std::vector<uint64_t> sb(slots, 0), sa(slots, 0);
     std::vector<uint32_t> cnt(slots), off(slots);
     if (n == 0) return;
@@ -1622,7 +1791,7 @@ struct LouvainBatchRun : LouvainRun {
     LvArgs a;
     a.seed = 0; a.level = level; a.sampling = uint32_t(std::min<uint64_t>(prm.sampling, 0xffffffffu));
     a.recip = 0.0; a.C = C; a.cvol = cvol_.p; a.csize = csize.p; a.nvol = nvol;
-    a.move_u = mu.p; a.move_to = mto.p; a.move_count = mcount;
+    a.dec = dec.p;
     a.buckets = B; a.bucket_salt = 0;
     a.nmember = L.nmember.p; a.nlocal = L.nlocal.p; a.grecip = grecip_.p; a.gactive = gflag_.p; a.gseed = gseed_.p;
     std::vector<uint64_t> moved(L.nodes.begin(), L.nodes.end());
@@ -1641,9 +1810,10 @@ struct LouvainBatchRun : LouvainRun {
       fill_async<uint64_t>(rb.p, 0ull, rwords, s);
       fill_async<uint32_t>(gmoved_.p, 0u, G, s);
       const bool prof = c.prof.enabled;
-      lv_count_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, counts, sarcs, prof ? sbytes : nullptr, bmat.p);
+      uint64_t* by = prof ? bymat.p : nullptr;
+      lv_count_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, bmat.p, amat.p, by);
+      lv_totals_kernel<<<slots, 256, 0, s>>>(lblocks, bmat.p, amat.p, by, counts, sarcs, by ? sbytes : nullptr);
       MKP_LAUNCH_CHECK();
-      exclusive_scan_u32(c, bmat.p, bpos.p, size_t(slots) * lblocks);
       uint64_t* stg = c.staging<uint64_t>(2 * size_t(slots) + 2 * cw);
       uint32_t* stg_cnt = reinterpret_cast<uint32_t*>(stg + 2 * slots);
       MKP_HIP(hipMemcpyAsync(stg, rb.p, (2 * size_t(slots) + cw) * 8, hipMemcpyDeviceToHost, s));
@@ -1660,22 +1830,17 @@ struct LouvainBatchRun : LouvainRun {
       if (hnodes) reserve_heavy(harcs, hnodes);
       uint32_t run = 0;
       for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
-      lv_scatter_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, bpos.p, list.p);
+      lv_scatter_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, bmat.p, counts, list.p);
       MKP_LAUNCH_CHECK();
       if (hnodes) prepare_heavy(g, a, list.p + off[4 * B], hnodes);
-      const LaneChoice lanes = choose_lanes<View>(cnt, sa, B);
+      const LaneChoice lanes = choose_lanes<View>(cnt, sa, B, r);
       for (int sr = 0; sr < B; ++sr) {
         a.bucket = sr;
-        a.move_count = mcount + (sr & 1);
         hipEvent_t ta, tb;
         const int lanes_now = lanes_for(lanes, sr, &ta, &tb);
         launch_move(g, a, list.p, off, cnt, sa, sb, sr, lanes_now, ta, tb);
-        uint32_t sub_nodes = 0;
-        for (int cl = 0; cl < kLvClasses; ++cl) sub_nodes += cnt[cl * B + sr];
-        louvain_apply_kernel<<<grid_for(sub_nodes, 256, 4096), 256, 0, s>>>(
-            mu.p, mto.p, a.move_count, nvol, C, cvol_.p, csize.p, moved_total, mcount + ((sr + 1) & 1),
-            L.nmember.p, gmoved_.p);
-        MKP_LAUNCH_CHECK();
+        launch_apply(list.p, dec.p, off, cnt, B, sr, nvol, C, cvol_.p, csize.p, moved_total, L.nmember.p,
+                     gmoved_.p);
       }
       uint32_t* stg_moved = c.staging<uint32_t>(G);
       MKP_HIP(hipMemcpyAsync(stg_moved, gmoved_.p, size_t(G) * 4, hipMemcpyDeviceToHost, s));

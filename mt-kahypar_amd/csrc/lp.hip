@@ -28,6 +28,7 @@ namespace mkp {
 constexpr uint32_t kHighDegree = 100000;   // partitioned_hypergraph.h:67
 constexpr uint32_t kThreadDegreeLimit = 512;
 constexpr int kGainThreads = 128;
+constexpr int kLpStripes = 32, kLpStripeWords = 16;  // striped counters (see stripe_add)
 
 struct LpWork {
   uint32_t n = 0;
@@ -42,6 +43,7 @@ struct LpWork {
   DevBuf<uint32_t> net_stamp;  // last round stamp that expanded each net
   DevBuf<uint64_t> bkeys, bkeys_sorted;
   DevBuf<int64_t> bcount;
+  DevBuf<int64_t> stripes;  // striped counter copies (single hypergraph)
   uint32_t m = 0, stamp = 0;
   void ensure_nets(uint32_t mm, hipStream_t s) {
     if (mm <= m && net_stamp.p) return;
@@ -57,6 +59,7 @@ struct LpWork {
       members = mm;
       counters.alloc(size_t(kGcStrideWords) * members);
     }
+    if (!stripes.p) stripes.alloc(size_t(kLpStripes) * kLpStripeWords);
     if (nn <= n && kk <= k) return;
     n = std::max(nn, n);
     k = std::max(kk, k);
@@ -66,7 +69,7 @@ struct LpWork {
     cand_to_a.alloc(N); cand_gain_a.alloc(N); cand_from.alloc(N); cand_to.alloc(N); cand_gain.alloc(N);
     admit.alloc(N); keep.alloc(N);
     keys.alloc(N); keys_sorted.alloc(N); vals.alloc(N); vals_sorted.alloc(N);
-    wsort.alloc(N); wscan.alloc(N); bkeys.alloc(N); bkeys_sorted.alloc(N); bcount.alloc(256);
+    wsort.alloc(N); wscan.alloc(N); bkeys.alloc(N); bkeys_sorted.alloc(N); bcount.alloc(4 * 256);
     seg_base.alloc(k); in_w.alloc(k); max_w.alloc(k); perfect_w.alloc(k); overflow.alloc(k);
   }
   static constexpr int kGcStrideWords = 16;
@@ -76,7 +79,11 @@ struct LpWork {
 //   0 = delta of the current sub-round, 1 = visits, 2 = reverts,
 //   3 = heavy-list length (member 0's word, shared), 4 = moves,
 //   5 = vertices swept by the gain kernels, 6 = stopped (rolled back),
-//   7 = total delta, 8 = rounds, 9 = active in the current round
+//   7 = total delta, 8 = rounds, 9 = active in the current round,
+//   10 = kept moves of the current sub-round (batched form: counted per
+//        member by the verify kernel, added into word 4 by the check kernel),
+//   11 = incidences of the current sub-round's candidates (member 0's word,
+//        shared; profiling only)
 constexpr int kGcStride = LpWork::kGcStrideWords;
 
 struct LpArgs {
@@ -102,6 +109,10 @@ struct LpArgs {
   int bucket;
   uint32_t bucket_salt;
   int64_t* counters;
+  // single hypergraph: words 0, 1, 2, 4 and 5 of the counters are added into
+  // striped copies and folded into the counters by lp_check_kernel (see
+  // stripe_add)
+  int64_t* stripes;
   // batch of independent hypergraphs stored as one disjoint union: block ids
   // stay local to a member, its block weights live at gkbase[g] + b, and every
   // hash uses the member's seed and local vertex id, so each member evolves
@@ -124,6 +135,28 @@ __device__ inline LpMember member_of(const LpArgs& a, uint32_t u) {
   return LpMember{g, u - a.gvbase[g], a.gkbase[g], a.gk[g], a.gseed[g]};
 }
 __device__ inline int64_t* member_counters(const LpArgs& a, uint32_t g) { return a.counters + kGcStride * g; }
+
+// Device atomics execute in memory, and atomics on one address one after
+// another (about 11 ns each on MI355X), so a counter that every wave of a
+// launch updates is kept in kLpStripes copies 128 bytes apart, one per
+// residue of the block index, and summed once per sub-round.
+constexpr int kLpFoldWords[5] = {0, 1, 2, 4, 5};
+__device__ inline void stripe_add(int64_t* stripes, int word, int64_t v) {
+  if (v)
+    atomicAdd(reinterpret_cast<unsigned long long*>(&stripes[(blockIdx.x % kLpStripes) * kLpStripeWords + word]),
+              static_cast<unsigned long long>(v));
+}
+// adds the stripes into the counters and clears them (one thread)
+__device__ inline void stripe_fold(int64_t* counters, int64_t* stripes) {
+  for (int w : kLpFoldWords) {
+    int64_t sum = 0;
+    for (int t = 0; t < kLpStripes; ++t) {
+      sum += stripes[t * kLpStripeWords + w];
+      stripes[t * kLpStripeWords + w] = 0;
+    }
+    counters[w] += sum;
+  }
+}
 __device__ inline void counter_add(int64_t* c, int64_t v) {
   if (v) atomicAdd(reinterpret_cast<unsigned long long*>(c), static_cast<unsigned long long>(v));
 }
@@ -289,8 +322,8 @@ lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
   visits = wave_sum(visits);
   swept = wave_sum(swept);
   if ((threadIdx.x & 63) == 0 && visits) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[1]), static_cast<unsigned long long>(visits));
-    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[5]), static_cast<unsigned long long>(swept));
+    stripe_add(a.stripes, 1, visits);
+    stripe_add(a.stripes, 5, swept);
   }
 }
 
@@ -399,8 +432,8 @@ lp_gain_wave_kernel(LpArgs a, const uint32_t* __restrict__ active, const uint32_
     __syncthreads();
   }
   if (lane == 0 && visits) {
-    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[1]), static_cast<unsigned long long>(visits));
-    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[5]), static_cast<unsigned long long>(swept));
+    stripe_add(a.stripes, 1, visits);
+    stripe_add(a.stripes, 5, swept);
   }
 }
 
@@ -412,17 +445,35 @@ __global__ void lp_scatter_candidates_kernel(const uint32_t* __restrict__ active
                                              uint32_t* __restrict__ cu, int32_t* __restrict__ cfrom,
                                              int32_t* __restrict__ cto, int32_t* __restrict__ cgain,
                                              int64_t* __restrict__ in_w, const uint32_t* __restrict__ vgraph,
-                                             const uint32_t* __restrict__ gkbase) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    if (!cand_flag[i]) continue;
-    const uint32_t j = pos[i];
-    const uint32_t u = active[i];
-    cu[j] = u;
-    cfrom[j] = part[u];
-    cto[j] = cto_a[i];
-    cgain[j] = cgain_a[i];
-    const uint32_t kb = vgraph ? gkbase[vgraph[u]] : 0u;  // arrivals per (member, block)
-    atomicAdd(reinterpret_cast<unsigned long long*>(&in_w[kb + cto_a[i]]), static_cast<unsigned long long>(vw[u]));
+                                             const uint32_t* __restrict__ gkbase, const uint32_t* __restrict__ voff,
+                                             int64_t* __restrict__ inc_sum) {
+  // whole waves step together (the incidence count combines across lanes,
+  // then across the block: one atomic per block)
+  __shared__ unsigned long long binc;
+  if (threadIdx.x == 0) binc = 0;
+  __syncthreads();
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < na; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + (threadIdx.x & 63u);
+    int64_t d = 0;
+    if (i < na && cand_flag[i]) {
+      const uint32_t j = pos[i];
+      const uint32_t u = active[i];
+      cu[j] = u;
+      cfrom[j] = part[u];
+      cto[j] = cto_a[i];
+      cgain[j] = cgain_a[i];
+      const uint32_t kb = vgraph ? gkbase[vgraph[u]] : 0u;  // arrivals per (member, block)
+      atomicAdd(reinterpret_cast<unsigned long long*>(&in_w[kb + cto_a[i]]), static_cast<unsigned long long>(vw[u]));
+      if (inc_sum) d = int64_t(voff[u + 1] - voff[u]);
+    }
+    if (inc_sum) {
+      d = wave_sum(d);
+      if ((threadIdx.x & 63) == 0 && d) atomicAdd(&binc, static_cast<unsigned long long>(d));
+    }
+  }
+  if (inc_sum) {
+    __syncthreads();
+    if (threadIdx.x == 0 && binc) counter_add(inc_sum, int64_t(binc));
   }
 }
 
@@ -555,8 +606,7 @@ __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const
   }
   if (a.vgraph) return;
   d = wave_sum(d);
-  if ((threadIdx.x & 63) == 0 && d)
-    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[0]), static_cast<unsigned long long>(d));
+  if ((threadIdx.x & 63) == 0) stripe_add(a.stripes, 0, d);
 }
 
 // real km1 delta of each applied move on the post-move state
@@ -613,8 +663,7 @@ __global__ void lp_verify_kernel(LpArgs a, const uint32_t* __restrict__ cu, cons
   }
   if (a.vgraph) return;
   reverts = wave_sum(reverts);
-  if ((threadIdx.x & 63) == 0 && reverts)
-    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[2]), static_cast<unsigned long long>(reverts));
+  if ((threadIdx.x & 63) == 0) stripe_add(a.stripes, 2, reverts);
 }
 
 // One wave per kept move; lanes take the incident nets.  A net of at most
@@ -646,8 +695,7 @@ lp_activate_kernel(LpArgs a, const uint32_t* __restrict__ cu, const uint8_t* __r
         for (uint32_t q = pb; q < pe; ++q) flag[a.pins[q]] = 1;
     }
   }
-  if (lane == 0 && moves)
-    atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[4]), static_cast<unsigned long long>(moves));
+  if (lane == 0) stripe_add(a.stripes, 4, moves);
 }
 
 __global__ void lp_compact_active_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
@@ -675,7 +723,11 @@ __global__ void lp_border_flag_kernel(LpArgs a, uint32_t n, uint32_t* __restrict
 // After a sub-round's moves and reverts: if its total real delta is positive
 // the sub-round is rolled back and the refinement stops (flag counters[6]),
 // else the delta joins the total (counters[7]).
-__global__ void lp_check_kernel(int64_t* __restrict__ counters, uint32_t members) {
+__global__ void lp_check_kernel(int64_t* __restrict__ counters, uint32_t members, int64_t* __restrict__ stripes) {
+  if (stripes) {  // single hypergraph: one thread folds the stripes first
+    if (blockIdx.x == 0 && threadIdx.x == 0) stripe_fold(counters, stripes);
+    else return;
+  }
   for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < members; g += gridDim.x * blockDim.x) {
     int64_t* c = counters + kGcStride * g;
     const int64_t d = c[0];
@@ -706,6 +758,10 @@ __global__ void lp_round_count_kernel(int64_t* __restrict__ counters, uint32_t m
   }
 }
 
+__global__ void lp_fold_kernel(int64_t* __restrict__ counters, int64_t* __restrict__ stripes) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) stripe_fold(counters, stripes);
+}
+
 __global__ void iota_kernel(uint32_t* out, uint32_t n) {
   for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) out[u] = u;
 }
@@ -713,26 +769,72 @@ __global__ void iota_kernel(uint32_t* out, uint32_t n) {
 // --------------------------------------------------------------- driver --
 // bucket key per active entry; per-block LDS histograms, one global atomic per
 // (block, bucket)
+// With `stats` set (profiling), count[B + b], count[2B + b] and count[3B + b]
+// also receive the incidences of bucket b's vertices that the thread-per-vertex
+// gain kernel takes, and the vertices and incidences of those the
+// wave-per-vertex kernel takes: the gain launches' algorithmic bytes.
 __global__ void __launch_bounds__(256)
 lp_bucket_keys_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na, uint64_t* __restrict__ keys,
-                      int64_t* __restrict__ count) {
+                      int64_t* __restrict__ count, int stats) {
   __shared__ uint32_t h[256];
+  __shared__ unsigned long long hs[3 * 256];
   for (int i = threadIdx.x; i < a.buckets; i += blockDim.x) h[i] = 0;
+  if (stats)
+    for (int i = threadIdx.x; i < 3 * a.buckets; i += blockDim.x) hs[i] = 0;
   __syncthreads();
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < na; i += gridDim.x * blockDim.x) {
-    uint32_t b;
-    if (a.vgraph) {  // the member's own (seed, round) salt and local id
-      const LpMember mb = member_of(a, active[i]);
-      b = bucket_of(bucket_salt(mb.seed, kLpBucketTag | uint64_t(a.round)), mb.lu, uint32_t(a.buckets));
-    } else {
-      b = bucket_of(a.bucket_salt, active[i], uint32_t(a.buckets));
+  // whole waves step together: the lanes of one bucket count with one LDS
+  // atomic per wave
+  for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < na; base += gridDim.x * blockDim.x) {
+    const uint32_t i = base + (threadIdx.x & 63u);
+    int b = -1;
+    if (i < na) {
+      if (a.vgraph) {  // the member's own (seed, round) salt and local id
+        const LpMember mb = member_of(a, active[i]);
+        b = int(bucket_of(bucket_salt(mb.seed, kLpBucketTag | uint64_t(a.round)), mb.lu, uint32_t(a.buckets)));
+      } else {
+        b = int(bucket_of(a.bucket_salt, active[i], uint32_t(a.buckets)));
+      }
+      if (keys) keys[i] = uint64_t(b);
     }
-    keys[i] = b;
-    atomicAdd(&h[b], 1u);
+    const SlotGroup sg = slot_group(b);
+    if (sg.leader) atomicAdd(&h[b], sg.count);
+    if (stats) {
+      // per bucket of the wave: incidences of thread-kernel vertices,
+      // wave-kernel vertices and their incidences
+      uint64_t li = 0, wv = 0, wi = 0;
+      if (i < na) {
+        const uint32_t u = active[i];
+        const uint32_t d = a.voff[u + 1] - a.voff[u];
+        if (d <= kHighDegree) {
+          if (a.k > 32 || d > kThreadDegreeLimit) { wv = 1; wi = d; } else { li = d; }
+        }
+      }
+      uint64_t pending = __ballot(b >= 0);
+      while (pending) {
+        const int first = __ffsll(static_cast<long long>(pending)) - 1;
+        const int bb = __shfl(b, first, 64);
+        const bool mine = b == bb && b >= 0;
+        uint64_t x = mine ? li : 0, y = mine ? wv : 0, z = mine ? wi : 0;
+        for (int o = 32; o > 0; o >>= 1) {
+          x += __shfl_xor(x, o, 64);
+          y += __shfl_xor(y, o, 64);
+          z += __shfl_xor(z, o, 64);
+        }
+        if ((threadIdx.x & 63) == uint32_t(first)) {
+          if (x) atomicAdd(&hs[bb], static_cast<unsigned long long>(x));
+          if (y) atomicAdd(&hs[a.buckets + bb], static_cast<unsigned long long>(y));
+          if (z) atomicAdd(&hs[2 * a.buckets + bb], static_cast<unsigned long long>(z));
+        }
+        pending &= ~__ballot(mine);
+      }
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < a.buckets; i += blockDim.x)
     if (h[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&count[i]), static_cast<unsigned long long>(h[i]));
+  if (stats)
+    for (int i = threadIdx.x; i < 3 * a.buckets; i += blockDim.x)
+      if (hs[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&count[a.buckets + i]), hs[i]);
 }
 
 // --------------------------------------------------------------- driver --
@@ -742,19 +844,35 @@ static void launch_thread_gain(const LpArgs& a, LpWork& w, const uint32_t* act, 
       a, act, na, w.cand_flag.p, w.cand_to_a.p, w.cand_gain_a.p, w.heavy.p);
 }
 
-static void launch_gain(const LpArgs& a, LpWork& w, const uint32_t* act, uint32_t na, hipStream_t s, int num_cus) {
+// Algorithmic bytes of the two gain launches of a sub-round (Km1Policy
+// sweep): per vertex its two offsets, block id and weight (16 B); per
+// incidence the net id, net weight, the pin-count word of the source block
+// and the connectivity words.
+struct GainBytes {
+  double thread = 0, wave = 0;
+};
+
+static void launch_gain(Context& c, const LpArgs& a, LpWork& w, const uint32_t* act, uint32_t na, hipStream_t s,
+                        const GainBytes& gb) {
+  const int num_cus = c.num_cus;
   if (a.k <= 32) {
-    if (a.k <= 2) launch_thread_gain<2>(a, w, act, na, s);
-    else if (a.k <= 4) launch_thread_gain<4>(a, w, act, na, s);
-    else if (a.k <= 8) launch_thread_gain<8>(a, w, act, na, s);
-    else if (a.k <= 16) launch_thread_gain<16>(a, w, act, na, s);
-    else launch_thread_gain<32>(a, w, act, na, s);
-    MKP_LAUNCH_CHECK();
+    {
+      const int K = a.k <= 2 ? 2 : a.k <= 4 ? 4 : a.k <= 8 ? 8 : a.k <= 16 ? 16 : 32;
+      TimedRegion tr(c.prof, s, kFamLpGain, gb.thread, "lp_gain_thread_kernel<" + std::to_string(K) + ">");
+      if (K == 2) launch_thread_gain<2>(a, w, act, na, s);
+      else if (K == 4) launch_thread_gain<4>(a, w, act, na, s);
+      else if (K == 8) launch_thread_gain<8>(a, w, act, na, s);
+      else if (K == 16) launch_thread_gain<16>(a, w, act, na, s);
+      else launch_thread_gain<32>(a, w, act, na, s);
+      MKP_LAUNCH_CHECK();
+    }
     // heavy vertices: persistent grid over the device-side list
+    TimedRegion tr(c.prof, s, kFamLpGain, gb.wave, "lp_gain_wave_kernel");
     lp_gain_wave_kernel<<<unsigned(num_cus) * 8, 64, a.k * sizeof(int32_t), s>>>(
         a, act, w.heavy.p, a.counters + 3, 0, w.cand_flag.p, w.cand_to_a.p, w.cand_gain_a.p);
     MKP_LAUNCH_CHECK();
   } else {
+    TimedRegion tr(c.prof, s, kFamLpGain, gb.wave, "lp_gain_wave_kernel");
     lp_gain_wave_kernel<<<grid_for(na, 1, unsigned(num_cus) * 64), 64, a.k * sizeof(int32_t), s>>>(
         a, act, nullptr, nullptr, na, w.cand_flag.p, w.cand_to_a.p, w.cand_gain_a.p);
     MKP_LAUNCH_CHECK();
@@ -782,6 +900,7 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
   MKP_HIP(hipMemcpyAsync(w.max_w.p, prm.max_part_weights, size_t(K) * 8, hipMemcpyHostToDevice, s));
   MKP_HIP(hipMemcpyAsync(w.perfect_w.p, prm.perfect_part_weights, size_t(K) * 8, hipMemcpyHostToDevice, s));
   fill_async<int64_t>(w.counters.p, 0, size_t(kGcStride) * G, s);
+  fill_async<int64_t>(w.stripes.p, 0, size_t(kLpStripes) * kLpStripeWords, s);
   MKP_HIP(hipEventRecord(c.ev_a, s));
 
   LpArgs a;
@@ -791,6 +910,7 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
   a.L = p->L; a.vcs = p->vcs; a.k = batch ? batch->kmax : p->k; a.rebalancing = prm.rebalancing;
   a.seed = prm.seed;
   a.counters = w.counters.p;
+  a.stripes = w.stripes.p;
   a.vgraph = batch ? batch->vgraph : nullptr;
   a.gvbase = batch ? batch->gvbase : nullptr;
   a.gkbase = batch ? batch->gkbase : nullptr;
@@ -824,6 +944,7 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
   while ((1 << kbits) < K) ++kbits;
   const bool split = kbits > 10;
   int64_t* host = c.pinned;
+  const bool prof = c.prof.enabled;
   int rounds = 0;
   bool stop = false;
   for (int r = 0; r < prm.max_rounds && na > 0 && !stop; ++r) {
@@ -846,18 +967,27 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
     boff[B] = na;
     a.buckets = B;
     a.bucket_salt = bucket_salt(a.seed, kLpBucketTag | uint64_t(r));
-    if (B > 1) {
-      fill_async<int64_t>(w.bcount.p, 0, B, s);
-      lp_bucket_keys_kernel<<<grid_for(na, 256, 65536), 256, 0, s>>>(a, w.active.p, na, w.bkeys.p, w.bcount.p);
+    // per bucket: vertices; with profiling also the incidences of the thread
+    // kernel's vertices, the wave kernel's vertices and their incidences
+    std::vector<int64_t> cnt(4 * size_t(B), 0);
+    if (B > 1 || prof) {
+      fill_async<int64_t>(w.bcount.p, 0, 4 * size_t(B), s);
+      lp_bucket_keys_kernel<<<grid_for(na, 256, 65536), 256, 0, s>>>(a, w.active.p, na, B > 1 ? w.bkeys.p : nullptr,
+                                                                      w.bcount.p, prof ? 1 : 0);
       MKP_LAUNCH_CHECK();
-      int bb = 1;
-      while ((1 << bb) < B) ++bb;
-      sort_pairs_u64_u32(c, w.bkeys.p, w.bkeys_sorted.p, w.active.p, w.active_next.p, na, 0, bb);
-      std::vector<int64_t> cnt(B);
-      MKP_HIP(hipMemcpyAsync(cnt.data(), w.bcount.p, size_t(B) * 8, hipMemcpyDeviceToHost, s));
+      if (B > 1) {
+        int bb = 1;
+        while ((1 << bb) < B) ++bb;
+        sort_pairs_u64_u32(c, w.bkeys.p, w.bkeys_sorted.p, w.active.p, w.active_next.p, na, 0, bb);
+      }
+      int64_t* stg_b = c.staging<int64_t>(4 * size_t(B));
+      MKP_HIP(hipMemcpyAsync(stg_b, w.bcount.p, 4 * size_t(B) * 8, hipMemcpyDeviceToHost, s));
       c.sync();
-      for (int b = 0; b < B; ++b) boff[b + 1] = boff[b] + uint32_t(cnt[b]);
-      base = w.active_next.p;
+      std::copy(stg_b, stg_b + 4 * B, cnt.begin());
+      if (B > 1) {
+        for (int b = 0; b < B; ++b) boff[b + 1] = boff[b] + uint32_t(cnt[b]);
+        base = w.active_next.p;
+      }
     }
     a.buckets = 1;  // slices are pre-partitioned
     for (int sr = 0; sr < B && !stop; ++sr) {
@@ -867,24 +997,36 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
       const uint32_t ns = boff[sr + 1] - boff[sr];
       if (ns == 0) continue;
       fill_async<int64_t>(w.counters.p + 3, 0, 1, s);
+      if (prof) fill_async<int64_t>(w.counters.p + 11, 0, 1, s);
       {
-        TimedRegion tr(c.prof, s, kFamLpGain, 0.0);
-        launch_gain(a, w, act, ns, s, c.num_cus);
+        GainBytes gb;
+        if (prof) {
+          const double per_inc = 16.0 + 8.0 * p->vcs;
+          const int sb = B > 1 ? sr : 0;
+          gb.thread = 16.0 * double(int64_t(ns) - cnt[2 * B + sb]) + per_inc * double(cnt[B + sb]);
+          gb.wave = 16.0 * double(cnt[2 * B + sb]) + per_inc * double(cnt[3 * B + sb]);
+        }
+        launch_gain(c, a, w, act, ns, s, gb);
       }
       exclusive_scan_u32(c, w.cand_flag.p, w.pos.p, ns);
       fill_async<int64_t>(w.in_w.p, 0, K, s);
-      lp_scatter_candidates_kernel<<<grid_for(ns, 256, 65536), 256, 0, s>>>(
+      lp_scatter_candidates_kernel<<<grid_for(ns, 256, prof ? 512 : 65536), 256, 0, s>>>(
           act, ns, w.cand_flag.p, w.pos.p, w.cand_to_a.p, w.cand_gain_a.p, p->part.p, hg->vw.p, w.cand_u.p,
-          w.cand_from.p, w.cand_to.p, w.cand_gain.p, w.in_w.p, a.vgraph, a.gkbase);
+          w.cand_from.p, w.cand_to.p, w.cand_gain.p, w.in_w.p, a.vgraph, a.gkbase, hg->voff.p,
+          prof ? w.counters.p + 11 : nullptr);
       MKP_LAUNCH_CHECK();
       // read back (pinned staging): stop flag of the previous sub-round,
-      // number of candidates, arrivals per block, block weights
-      int64_t* stg = c.staging<int64_t>(2 * size_t(K) + 2 + (size_t(K) + 7) / 8);
+      // number of candidates, arrivals per block, block weights (and, with
+      // profiling, the candidates' incidences)
+      const size_t over_words = (size_t(K) + 7) / 8;
+      int64_t* stg = c.staging<int64_t>(2 * size_t(K) + 3 + over_words);
       uint8_t* stg_over = reinterpret_cast<uint8_t*>(stg + 2 * K + 2);
+      int64_t* stg_inc = stg + 2 * K + 2 + over_words;
       MKP_HIP(hipMemcpyAsync(stg, w.counters.p + 6, 8, hipMemcpyDeviceToHost, s));
       MKP_HIP(hipMemcpyAsync(stg + 1, w.pos.p + ns, 4, hipMemcpyDeviceToHost, s));
       MKP_HIP(hipMemcpyAsync(stg + 2, w.in_w.p, size_t(K) * 8, hipMemcpyDeviceToHost, s));
       MKP_HIP(hipMemcpyAsync(stg + 2 + K, p->part_w.p, size_t(K) * 8, hipMemcpyDeviceToHost, s));
+      if (prof) MKP_HIP(hipMemcpyAsync(stg_inc, w.counters.p + 11, 8, hipMemcpyDeviceToHost, s));
       c.sync();
       if (!batch && stg[0]) {  // the previous sub-round was rolled back on the device
         stop = true;
@@ -933,18 +1075,29 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
       }
       const unsigned gc = grid_for(nc, 256, 65536);
       {
-        TimedRegion tr_apply(c.prof, s, kFamLpApply, 0.0);
-        lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, nullptr, nc, 0);
-        MKP_LAUNCH_CHECK();
-        lp_verify_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, w.cand_gain.p, admit, nc,
-                                            w.keep.p);
-        MKP_LAUNCH_CHECK();
+        // changeNodePart per candidate: its id, blocks and admission flag
+        // (16 B), per incidence the net id and a read and a write of the
+        // packed pin-count word (20 B); the verification reads the net id
+        // and the word again (12 B per incidence)
+        const double cand_inc = prof ? double(*stg_inc) : 0.0;
+        {
+          TimedRegion tr(c.prof, s, kFamLpApply, 16.0 * nc + 20.0 * cand_inc, "lp_apply_kernel");
+          lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, nullptr, nc, 0);
+          MKP_LAUNCH_CHECK();
+        }
+        {
+          TimedRegion tr(c.prof, s, kFamLpApply, 16.0 * nc + 12.0 * cand_inc, "lp_verify_kernel");
+          lp_verify_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, w.cand_gain.p, admit, nc,
+                                              w.keep.p);
+          MKP_LAUNCH_CHECK();
+        }
+        TimedRegion tr(c.prof, s, kFamLpApply, 0.0, "lp_revert_kernel");
         lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, w.keep.p, nc, 1);
         MKP_LAUNCH_CHECK();
       }
       // decided on the device, per member: keep the sub-round (delta into the
       // total) or roll it back and stop that member
-      lp_check_kernel<<<grid_for(G, 256, 4096), 256, 0, s>>>(w.counters.p, G);
+      lp_check_kernel<<<grid_for(G, 256, 4096), 256, 0, s>>>(w.counters.p, G, batch ? nullptr : w.stripes.p);
       lp_apply_kernel<<<gc, 256, 0, s>>>(a, w.cand_u.p, w.cand_from.p, w.cand_to.p, admit, w.keep.p, nc, 2);
       MKP_LAUNCH_CHECK();
       lp_activate_kernel<<<grid_for(nc, 1, unsigned(c.num_cus) * 16), 64, 0, s>>>(
@@ -962,6 +1115,10 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
     na = *reinterpret_cast<const uint32_t*>(host);
     if (!batch && host[1]) stop = true;  // the round's last sub-round was rolled back
   }
+  if (!batch) {  // the moves of the last sub-round are still striped
+    lp_fold_kernel<<<1, 64, 0, s>>>(w.counters.p, w.stripes.p);
+    MKP_LAUNCH_CHECK();
+  }
   MKP_HIP(hipEventRecord(c.ev_b, s));
   std::vector<int64_t> cnt(size_t(kGcStride) * G);
   MKP_HIP(hipMemcpyAsync(cnt.data(), w.counters.p, cnt.size() * 8, hipMemcpyDeviceToHost, s));
@@ -970,12 +1127,6 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
   for (uint32_t g = 0; g < G; ++g) {
     all_visits += cnt[kGcStride * g + 1];
     all_swept += cnt[kGcStride * g + 5];
-  }
-  if (c.prof.enabled) {
-    // gain sweep: 16 B per swept vertex (offsets, block id, weight) and per
-    // incidence the net id, net weight, one packed counter word and the
-    // connectivity words
-    c.prof.bytes[kFamLpGain] += 16.0 * double(all_swept) + (16.0 + 8.0 * p->vcs) * double(all_visits);
   }
   if (stats) {
     float ms = 0;

@@ -69,7 +69,7 @@ pin_count_init_kernel(const uint32_t* __restrict__ noff, const uint32_t* __restr
   for (int o = 32; o > 0; o >>= 1) local += __shfl_down(local, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = local;
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && km1_acc) {
     int64_t s = 0;
     for (int w = 0; w < int(blockDim.x >> 6); ++w) s += red[w];
     if (s) atomicAdd(reinterpret_cast<unsigned long long*>(km1_acc), static_cast<unsigned long long>(s));
@@ -88,8 +88,13 @@ __global__ void block_weight_kernel(const int32_t* __restrict__ part, const int3
     if (bins[i]) atomicAdd(reinterpret_cast<unsigned long long*>(&part_w[i]), static_cast<unsigned long long>(bins[i]));
 }
 
-__global__ void km1_kernel(const uint64_t* __restrict__ rec, const int32_t* __restrict__ nw, uint32_t m, PcLayout L,
-                           int64_t* __restrict__ acc) {
+// km1 per block into acc[blockIdx.x] (no atomics: the host sums the
+// partials of the at most kKm1Blocks blocks)
+constexpr unsigned kKm1Blocks = 512;
+__global__ void __launch_bounds__(256)
+km1_kernel(const uint64_t* __restrict__ rec, const int32_t* __restrict__ nw, uint32_t m, PcLayout L,
+           int64_t* __restrict__ acc) {
+  __shared__ int64_t red[4];
   int64_t local = 0;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
     int lambda = 0;
@@ -97,8 +102,9 @@ __global__ void km1_kernel(const uint64_t* __restrict__ rec, const int32_t* __re
     local += int64_t(max(lambda - 1, 0)) * nw[e];
   }
   for (int o = 32; o > 0; o >>= 1) local += __shfl_down(local, o, 64);
-  if ((threadIdx.x & 63) == 0 && local)
-    atomicAdd(reinterpret_cast<unsigned long long*>(acc), static_cast<unsigned long long>(local));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) acc[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 __global__ void unpack_pc_kernel(const uint64_t* __restrict__ pc, uint32_t m, int k, PcLayout L,
@@ -149,13 +155,12 @@ void partition_initialize(mkp_partition* p) {
   mkp_hypergraph* hg = p->hg;
   Context& c = hg->ctx->c;
   fill_async<int64_t>(p->part_w.p, 0, p->k, c.stream);
-  fill_async<int64_t>(p->km1_acc.p, 0, 1, c.stream);
   // algorithmic bytes: block id + weight per vertex and k weights (block
   // weights); net offsets, pin id + block id per pin, packed counters and
   // connectivity written once (pin counts)
   if (hg->n) {
     TimedRegion tr(c.prof, c.stream, kFamPinCountInit, 8.0 * hg->n + 4.0 * p->k, "block_weight_kernel");
-    block_weight_kernel<<<grid_for(hg->n, 256, 2048), 256, p->k * sizeof(int64_t), c.stream>>>(
+    block_weight_kernel<<<grid_for(hg->n, 256, 256), 256, p->k * sizeof(int64_t), c.stream>>>(
         p->part.p, hg->vw.p, hg->n, p->k, p->part_w.p);
     MKP_LAUNCH_CHECK();
   }
@@ -166,7 +171,7 @@ void partition_initialize(mkp_partition* p) {
     const int npb = nets_per_block(p->L);
     const size_t lds = size_t(npb) * p->L.vpc * 8 + size_t(npb + 1) * 4;
     pin_count_init_kernel<<<(hg->m + npb - 1) / npb, kInitThreads, lds, c.stream>>>(
-        hg->noff.p, hg->pins.p, p->part.p, hg->nw.p, hg->m, p->k, p->L, p->vcs, npb, p->pc.p, p->km1_acc.p);
+        hg->noff.p, hg->pins.p, p->part.p, hg->nw.p, hg->m, p->k, p->L, p->vcs, npb, p->pc.p, nullptr);
     MKP_LAUNCH_CHECK();
   }
 }
@@ -237,7 +242,6 @@ void partition_initialize_batch(mkp_partition* p, const uint32_t* vgraph, const 
   mkp_hypergraph* hg = p->hg;
   Context& c = hg->ctx->c;
   fill_async<int64_t>(p->part_w.p, 0, ktot, c.stream);
-  fill_async<int64_t>(p->km1_acc.p, 0, 1, c.stream);
   if (hg->n) {
     block_weight_batch_kernel<<<grid_for(hg->n, kBwChunk, 65536), 256, 0, c.stream>>>(p->part.p, hg->vw.p, hg->n,
                                                                                      vgraph, gkbase, p->part_w.p);
@@ -247,7 +251,7 @@ void partition_initialize_batch(mkp_partition* p, const uint32_t* vgraph, const 
     const int npb = nets_per_block(p->L);
     const size_t lds = size_t(npb) * p->L.vpc * 8 + size_t(npb + 1) * 4;
     pin_count_init_kernel<<<(hg->m + npb - 1) / npb, kInitThreads, lds, c.stream>>>(
-        hg->noff.p, hg->pins.p, p->part.p, hg->nw.p, hg->m, p->k, p->L, p->vcs, npb, p->pc.p, p->km1_acc.p);
+        hg->noff.p, hg->pins.p, p->part.p, hg->nw.p, hg->m, p->k, p->L, p->vcs, npb, p->pc.p, nullptr);
     MKP_LAUNCH_CHECK();
   }
 }
@@ -267,15 +271,15 @@ void partition_km1_batch(mkp_partition* p, const uint32_t* ngraph, uint32_t memb
 
 int64_t partition_km1(mkp_partition* p) {
   Context& c = p->hg->ctx->c;
-  fill_async<int64_t>(p->km1_acc.p, 0, 1, c.stream);
-  if (p->hg->m) {
-    km1_kernel<<<grid_for(p->hg->m, 256, 4096), 256, 0, c.stream>>>(p->pc.p, p->hg->nw.p, p->hg->m, p->L,
-                                                                    p->km1_acc.p);
-    MKP_LAUNCH_CHECK();
-  }
-  int64_t v = 0;
-  MKP_HIP(hipMemcpyAsync(&v, p->km1_acc.p, 8, hipMemcpyDeviceToHost, c.stream));
+  if (!p->hg->m) return 0;
+  const unsigned blocks = grid_for(p->hg->m, 256, kKm1Blocks);
+  km1_kernel<<<blocks, 256, 0, c.stream>>>(p->pc.p, p->hg->nw.p, p->hg->m, p->L, p->km1_acc.p);
+  MKP_LAUNCH_CHECK();
+  int64_t* stg = c.staging<int64_t>(blocks);
+  MKP_HIP(hipMemcpyAsync(stg, p->km1_acc.p, size_t(blocks) * 8, hipMemcpyDeviceToHost, c.stream));
   c.sync();
+  int64_t v = 0;
+  for (unsigned b = 0; b < blocks; ++b) v += stg[b];
   return v;
 }
 
@@ -299,7 +303,7 @@ int mkp_partition_create(mkp_hypergraph* hg, int32_t k, mkp_partition** out) {
       p->part.alloc(std::max<uint32_t>(hg->n, 1));
       p->part_w.alloc(k);
       p->pc.alloc(std::max<uint64_t>(uint64_t(hg->m) * p->L.stride, 1));
-      p->km1_acc.alloc(1);
+      p->km1_acc.alloc(kKm1Blocks);
       fill_async<int32_t>(p->part.p, 0, hg->n, hg->ctx->c.stream);
       partition_initialize(p);
       hg->ctx->c.sync();

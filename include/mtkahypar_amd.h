@@ -124,6 +124,14 @@ MKP_API int mkp_partition_get_connectivity(const mkp_partition* p, uint64_t* wor
 MKP_API int mkp_partition_get_border(const mkp_partition* p, uint8_t* border);
 /* metrics::km1 (partition/metrics.cpp:49-65). */
 MKP_API int mkp_partition_km1(mkp_partition* p, int64_t* km1);
+/* mkp_partition_get_border without any device allocation.  It has no
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under stream capture, tight serving loops, harnesses that forbid
+ * allocation).  The per-vertex flags are staged in a fixed device array of
+ * 16 MiB per GPU (held until the process ends), so partitions of more than
+ * 16,777,216 vertices are refused with status 1.  Calls that overlap on one
+ * GPU share that array and race; the call waits for its stream. */
+MKP_API int mkp_partition_get_border_noalloc(const mkp_partition* p, uint8_t* border);
 
 /* ---------------------------------------------------------------- LP --- */
 typedef struct {

@@ -423,6 +423,54 @@ int mkp_partition_get_border(const mkp_partition* p, uint8_t* border) {
   });
 }
 
+// ------------------------------------------------ allocation-free border --
+// The border flags of the allocation-free entry point are staged in this
+// array (one byte per vertex, at most kBorderScratchBytes vertices).  Every
+// GPU the library runs on holds its own copy until the process ends.  Host
+// code never passes the array's name to a kernel or copy: it uses the
+// device address that hipGetSymbolAddress returns on that device.
+}  // extern "C"
+namespace mkp {
+constexpr size_t kBorderScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) uint8_t scratch[kBorderScratchBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64] = {};  // per device: address of `scratch`
+
+// Address of this device's `scratch`, looked up once per device with the
+// device current.
+static uint8_t* border_scratch(int device) {
+  MKP_CHECK(device >= 0 && device < 64, "device index out of range");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    MKP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    MKP_CHECK(ptr != nullptr, "hipGetSymbolAddress returned no address for the border scratch");
+    g_scratch_ptr[device] = ptr;
+  }
+  return static_cast<uint8_t*>(g_scratch_ptr[device]);
+}
+}  // namespace mkp
+extern "C" {
+
+int mkp_partition_get_border_noalloc(const mkp_partition* p, uint8_t* border) {
+  return guarded([&] {
+    MKP_CHECK(p && border, "null argument");
+    const uint32_t n = p->hg->n;
+    MKP_CHECK(n <= kBorderScratchBytes, "mkp_partition_get_border_noalloc: " + std::to_string(n) +
+                                            " vertices exceed the scratch limit of " +
+                                            std::to_string(kBorderScratchBytes) + " vertices");
+    Context& c = p->hg->ctx->c;
+    MKP_HIP(hipSetDevice(c.device));
+    if (!n) return;
+    uint8_t* ptr = border_scratch(c.device);
+    border_kernel<<<grid_for(n, 256, 65536), 256, 0, c.stream>>>(p->hg->voff.p, p->hg->inc.p, p->pc.p, n, p->L,
+                                                                  ptr);
+    MKP_LAUNCH_CHECK();
+    MKP_HIP(hipMemcpyAsync(border, ptr, n, hipMemcpyDeviceToHost, c.stream));
+    MKP_HIP(hipStreamSynchronize(c.stream));
+  });
+}
+
 int mkp_partition_km1(mkp_partition* p, int64_t* km1) {
   return guarded([&] {
     MKP_CHECK(p && km1, "null argument");

@@ -132,6 +132,7 @@ SIGNATURES = {
     "mkp_partition_get_pin_counts": (C.c_int, [vp, vp]),
     "mkp_partition_get_connectivity": (C.c_int, [vp, vp]),
     "mkp_partition_get_border": (C.c_int, [vp, vp]),
+    "mkp_partition_get_border_noalloc": (C.c_int, [vp, vp]),
     "mkp_partition_km1": (C.c_int, [vp, P(C.c_int64)]),
     "mkp_lp_refine": (C.c_int, [vp, P(LpParams), P(LpStats)]),
     "mkp_lp_refine_nodes": (C.c_int, [vp, P(LpParams), vp, C.c_uint32, P(LpStats)]),
@@ -506,6 +507,12 @@ class Partition:
     def border(self):
         out = np.zeros(max(self.hg.info().num_vertices, 1), np.uint8)
         check(lib().mkp_partition_get_border(self.h, _p(out)))
+        return out[:self.hg.info().num_vertices]
+
+    def border_noalloc(self):
+        """border() through the entry point that allocates no device memory."""
+        out = np.zeros(max(self.hg.info().num_vertices, 1), np.uint8)
+        check(lib().mkp_partition_get_border_noalloc(self.h, _p(out)))
         return out[:self.hg.info().num_vertices]
 
     def km1(self):

@@ -82,34 +82,128 @@ def start_partition(n, k):
     return (np.arange(n, dtype=np.int64) * k // n).astype(np.int32)
 
 
-def cpu_baseline(w, arrays, part, maxw, perfect, threads):
+def host_info():
+    """Host cores this process may use and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    # the worker pool follows the process's CPU share (OMP_NUM_THREADS where
+    # the environment sets it), never more than the cores it may run on
+    share = int(os.environ.get("OMP_NUM_THREADS", affinity) or affinity)
+    return dict(nproc=os.cpu_count(), affinity=affinity, threads=max(1, min(affinity, share)), cpu_model=model)
+
+
+def cpu_step(oh, g, w, part, maxw, perfect, seed, threads):
     """One step of the oracle's reference-semantics path (asynchronous Louvain
     and LP, OpenMP) on the same hypergraph and starting partition."""
-    import oracle as orc
     k = w["k"]
-    oh = orc.Hypergraph(arrays["n"], arrays["noff"], arrays["pins"], arrays["vw"], arrays["nw"])
-    g = orc.Graph(oh, orc.Graph.UNIFORM, ranked=True)
     t0 = time.perf_counter()
-    _, lst = g.louvain(sync=False, seed=w["seed"], threads=threads)
+    _, lst = g.louvain(sync=False, seed=seed, threads=threads)
     t1 = time.perf_counter()
-    _, km1, st = oh.lp_refine(k, part, [maxw] * k, [perfect] * k, seed=w["seed"], sync=False, threads=threads)
+    _, km1, st = oh.lp_refine(k, part, [maxw] * k, [perfect] * k, seed=seed, sync=False, threads=threads)
     t2 = time.perf_counter()
     return dict(louvain_s=t1 - t0, lp_s=t2 - t1, pins=lst["arcs"] // 2 + st["visits"], km1=km1)
 
 
-def traffic_per_launch(workload, kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary
-    (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled by the factor the
-    same counter shows for the calibration copy), or None without one."""
+def cpu_baseline(w, arrays, part, maxw, perfect, threads, seeds=(42, 43, 44, 45, 46)):
+    """The step of the oracle's reference-semantics path over partitioner
+    seeds 42-46 on `threads` host threads (median reported), and seed 42 on
+    one thread for context."""
+    import oracle as orc
+    oh = orc.Hypergraph(arrays["n"], arrays["noff"], arrays["pins"], arrays["vw"], arrays["nw"])
+    g = orc.Graph(oh, orc.Graph.UNIFORM, ranked=True)
+    runs = [cpu_step(oh, g, w, part, maxw, perfect, sd, threads) for sd in seeds]
+    one = cpu_step(oh, g, w, part, maxw, perfect, seeds[0], 1)
+    rates = [r["pins"] / (r["louvain_s"] + r["lp_s"]) for r in runs]
+    med = sorted(range(len(runs)), key=lambda i: rates[i])[len(runs) // 2]
+    return dict(runs=runs, median=runs[med], median_rate=rates[med], one_thread=one,
+                one_thread_rate=one["pins"] / (one["louvain_s"] + one["lp_s"]))
+
+
+def lp_parity(arrays, part, k, maxw, perfect, seed, gpu_parts, gpu_km1):
+    """The device LP's block ids and km1 against the oracle's synchronous mode
+    from the same starting partition (bit for bit)."""
+    import oracle as orc
+    oh = orc.Hypergraph(arrays["n"], arrays["noff"], arrays["pins"], arrays["vw"], arrays["nw"])
+    t0 = time.perf_counter()
+    rpart, rkm1, _ = oh.lp_refine(k, part, [maxw] * k, [perfect] * k, seed=seed, sync=True)
+    ok = bool(np.array_equal(rpart, gpu_parts)) and int(rkm1) == int(gpu_km1)
+    return dict(lp_vs_sync_oracle="bit-exact" if ok else "MISMATCH", km1_gpu=int(gpu_km1), km1_oracle=int(rkm1),
+                oracle_s=round(time.perf_counter() - t0, 2))
+
+
+def phases_of(ctx, dh, w, arrays, cpu, threads):
+    """One full pipeline run (Louvain, coarsening, initial partitioning, LP
+    on every level) of the same hypergraph with every kernel family timed:
+    device time per phase, the rating roofline, and the CPU restatement's
+    time per phase (Louvain and LP: the baseline step's median; coarsening:
+    the reference-semantics rater's throughput on a sample of level-0
+    vertices, scaled to the pipeline's rating work; initial partitioning:
+    host code shared by both paths)."""
+    import oracle as orc
+    ctx.set_profiling(True)
+    ctx.reset_profile()
+    part, st = dh.partition(w["k"], w["eps"], w["seed"])
+    fam = ctx.profile()
+    ctx.set_profiling(False)
+    ctx.reset_profile()
+    n = arrays["n"]
+    oh = orc.Hypergraph(n, arrays["noff"], arrays["pins"], arrays["vw"], arrays["nw"])
+    sample = np.arange(0, n, 50, dtype=np.uint32)
+    t0 = time.perf_counter()
+    vis = oh.rating_sample_ref(sample, threads=threads)
+    rate = vis / max(time.perf_counter() - t0, 1e-9)
+    rating = fam["rating"]
+    contraction = fam["contraction"]
+    out = {
+        "workload_note": "one multilevel pipeline run of the same hypergraph (k=%d, eps=%g, seed %d): km1 %d, "
+                         "imbalance %.4f, %d levels" % (w["k"], w["eps"], w["seed"], st["km1"], st["imbalance"],
+                                                       st["levels"]),
+        "louvain": {"gpu_ms": round(st["louvain_ms"], 2),
+                    "cpu_ms": round(1e3 * cpu["median"]["louvain_s"], 1) if cpu else None},
+        "coarsening": {"gpu_ms": round(st["coarsening_ms"], 2),
+                       "rating_gpu_ms": round(rating["ms"], 2), "contraction_gpu_ms": round(contraction["ms"], 2),
+                       "rating_visits": st["rating_visits"],
+                       "rating_roofline": {"bound": "hbm", "achieved": round(rating["bytes"] / (rating["ms"] * 1e6), 1)
+                                           if rating["ms"] else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                           "frac": round(rating["bytes"] / (rating["ms"] * 1e6) / HBM_PEAK_GBS, 4)
+                                           if rating["ms"] else None,
+                                           "bytes": "8 per rating pin visit + 12 per pin + 12 per vertex, per pass"},
+                       "cpu_ms_estimate": round(1e3 * st["rating_visits"] / rate, 1),
+                       "cpu_sample": "reference-semantics rating of every 50th level-0 vertex on %d threads: %d pin "
+                                     "visits at %.3g visits/s, scaled to the pipeline's %d rating visits" % (
+                                         threads, vis, rate, st["rating_visits"])},
+        "initial_partitioning": {"host_ms": round(st["initial_ms"], 2),
+                                 "note": "host code shared by the device pipeline and the CPU restatement"},
+        "lp": {"gpu_ms_all_levels": round(st["refinement_ms"], 2),
+               "cpu_ms_finest_level": round(1e3 * cpu["median"]["lp_s"], 1) if cpu else None},
+        "total_gpu_pipeline_ms": round(st["total_ms"], 2),
+    }
+    return out
+
+
+def counter_summary(workload, kernel):
+    """From the committed counter summary of the workload: HBM bytes per
+    launch of `kernel` (rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled
+    by the factor the same counter shows for the calibration copy) and its
+    device atomics per launch (TCC_ATOMIC_sum), each None when absent."""
     path = os.path.join(PROFILES, "hbm_traffic_%s.json" % workload)
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
         t = json.load(f)
     k = t.get("kernels", {}).get(kernel)
     if not k:
-        return None
-    return round(k["fetch_bytes"] / t["fetch_factor"] + k["write_bytes"] / t["write_factor"])
+        return None, None
+    traffic = round(k["fetch_bytes"] / t["fetch_factor"] + k["write_bytes"] / t["write_factor"])
+    return traffic, k.get("atomics")
 
 
 def normalize_kernel_name(name):
@@ -129,10 +223,11 @@ def normalize_kernel_name(name):
     return name[:cut].replace("mkp::", "").strip()
 
 
-def pmc_summary(fetch_dir, write_dir, workload):
+def pmc_summary(fetch_dir, write_dir, workload, atomic_dir=None):
     """Per-kernel mean FETCH_SIZE / WRITE_SIZE per dispatch (bytes) from two
-    rocprofv3 --pmc runs of this script, and the factors the calibration copy
-    (COPY_BYTES read and written) gives; written to profiles/."""
+    rocprofv3 --pmc runs of this script, the factors the calibration copy
+    (COPY_BYTES read and written) gives and, from a third run, TCC_ATOMIC_sum
+    per dispatch; written to profiles/."""
     import csv
     import glob
 
@@ -150,10 +245,12 @@ def pmc_summary(fetch_dir, write_dir, workload):
         return {k: v[0] / v[1] for k, v in acc.items() if v[1]}
 
     fetch, write = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    atom = {k: v / 1024.0 for k, v in load(atomic_dir, "TCC_ATOMIC_sum").items()} if atomic_dir else {}
     copy = "stream_copy_kernel"
     out = dict(workload=workload, fetch_factor=fetch[copy] / COPY_BYTES, write_factor=write[copy] / COPY_BYTES,
                calibration="stream_copy_kernel: %d bytes read and %d written per dispatch" % (COPY_BYTES, COPY_BYTES),
-               kernels={k: dict(fetch_bytes=fetch.get(k, 0.0), write_bytes=write.get(k, 0.0))
+               kernels={k: dict(fetch_bytes=fetch.get(k, 0.0), write_bytes=write.get(k, 0.0),
+                                **({"atomics": atom[k]} if k in atom else {}))
                         for k in sorted(set(fetch) | set(write))})
     path = os.path.join(PROFILES, "hbm_traffic_%s.json" % workload)
     with open(path, "w") as f:
@@ -183,10 +280,12 @@ def roofline_of(ctx, prof, workload, dom=None):
     avg_ms = d["ms"] / d["launches"]
     bytes_per_launch = d["bytes"] / d["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = traffic_per_launch(workload, dom)
+    traffic, atomics = counter_summary(workload, dom)
     copy_ms = ctx.stream_copy(COPY_BYTES, 20)
     return dict(bound="hbm", kernel=dom, achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                atomics_per_launch=round(atomics) if atomics is not None else None,
+                atomic_rate_per_s=round(atomics / (avg_ms * 1e-3), 1) if atomics is not None else None,
                 avg_launch_us=round(avg_ms * 1e3, 2), launches=d["launches"],
                 algorithmic_bytes_per_launch=round(bytes_per_launch),
                 share_of_step_device_time=(round(d["ms"] / sum(v["ms"] for v in prof.values()), 3)
@@ -338,7 +437,7 @@ def batch_main(args):
 
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = host_info()["threads"]
         csample = list(range(0, G, w["cpu_stride"]))
         log("cpu baseline: oracle reference-semantics Louvain + LP on %d members, %d threads" % (len(csample),
                                                                                                threads))
@@ -532,11 +631,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-loop", action="store_true",
                     help="batched workloads: skip the per-hypergraph comparison leg (for per-kernel profiles)")
-    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
-                    help="summarise two rocprofv3 --pmc output folders of this script into profiles/ and exit")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="DIR",
+                    help="summarise rocprofv3 --pmc output folders of this script (FETCH_SIZE, WRITE_SIZE and "
+                         "optionally TCC_ATOMIC_sum runs) into profiles/ and exit")
+    ap.add_argument("--no-phases", action="store_true", help="skip the per-phase pipeline run")
     args = ap.parse_args()
     if args.pmc_summary:
-        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.workload)
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.workload,
+                    args.pmc_summary[2] if len(args.pmc_summary) > 2 else None)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (requested --gpus %d)"
@@ -611,16 +713,33 @@ def main():
         roofline = roofline_of(ctx, None, args.workload, dom)
         roofline["share_of_step_device_time"] = round(share, 3)
     ctx.set_profiling(False)
+    hi = host_info()
+    threads = hi["threads"]
 
-    cpu = None
+    # the LP of the last timed step against the oracle's synchronous mode
+    parity = lp_parity(arrays, part0, k, maxw, perfect, w["seed"], work.parts(), km1_end)
+    log("parity: LP %s vs the synchronous oracle (km1 %d / %d)" % (parity["lp_vs_sync_oracle"], parity["km1_gpu"],
+                                                                   parity["km1_oracle"]))
+
+    cpu = cpu_run = None
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        log("cpu baseline: oracle reference-semantics Louvain + LP, %d threads" % threads)
-        c = cpu_baseline(w, arrays, part0, maxw, perfect, threads)
-        cpu = dict(value=round(c["pins"] / (c["louvain_s"] + c["lp_s"]), 1), unit="pins/s", cores=threads,
-                   kind="port",
-                   sample="one step on the same hypergraph and starting partition: Louvain %.2f s + LP %.2f s "
-                          "(asynchronous restatement, OpenMP), LP km1 %d" % (c["louvain_s"], c["lp_s"], c["km1"]))
+        log("cpu baseline: oracle reference-semantics Louvain + LP, %d threads, seeds 42-46, and one thread" % threads)
+        cpu_run = cpu_baseline(w, arrays, part0, maxw, perfect, threads)
+        m = cpu_run["median"]
+        cpu = dict(value=round(cpu_run["median_rate"], 1), unit="pins/s", cores=threads, kind="port",
+                   sample="one step (Louvain + LP, asynchronous restatement, OpenMP) on the same hypergraph and "
+                          "starting partition per partitioner seed 42-46, median reported: Louvain %.2f s + LP %.2f s, "
+                          "LP km1 %d; seed 42 on one thread: %.3g pins/s (%.2f s)" % (
+                              m["louvain_s"], m["lp_s"], m["km1"], cpu_run["one_thread_rate"],
+                              cpu_run["one_thread"]["louvain_s"] + cpu_run["one_thread"]["lp_s"]),
+                   seeds_pins_per_s=[round(r["pins"] / (r["louvain_s"] + r["lp_s"]), 1) for r in cpu_run["runs"]],
+                   one_thread_pins_per_s=round(cpu_run["one_thread_rate"], 1),
+                   host={"nproc": hi["nproc"], "affinity_cores": hi["affinity"], "cpu_model": hi["cpu_model"]})
+
+    phases = None
+    if not args.no_phases:
+        log("phases: one pipeline run with every kernel family timed")
+        phases = phases_of(ctx, dh, w, arrays, cpu_run, threads)
 
     value = work_units / wall
     out = {
@@ -638,11 +757,17 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "vertices": n, "nets": len(arrays["noff"]) - 1, "pins": P,
                    "k": k, "epsilon": w["eps"], "km1_start": km1_start, "km1_end": km1_end,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU",
+                   "outside_timed_region": "hypergraph upload; net locality ranks (built once per hypergraph at "
+                                           "its first Louvain call, in the warm-up)"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity": parity,
+        "phases": phases,
     }
     print(json.dumps(out), flush=True)
+    if parity["lp_vs_sync_oracle"] != "bit-exact":
+        sys.exit(3)
 
 
 if __name__ == "__main__":

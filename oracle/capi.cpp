@@ -240,6 +240,10 @@ void orc_rate_sync(void* h, int pass, int sub_round, int32_t max_node_weight, ui
   std::copy(t.begin(), t.end(), target_out);
   std::copy(r.begin(), r.end(), rating_out);
 }
+int64_t orc_rating_sample_ref(void* h, const uint32_t* vertices, uint32_t count, uint32_t ignore, int threads) {
+  auto* hg = static_cast<Hypergraph*>(h);
+  return rating_sample_ref(*hg, std::vector<uint32_t>(vertices, vertices + count), ignore, threads);
+}
 void* orc_hg_contract(void* h, const uint32_t* cluster, uint32_t* cmap) {
   auto* hg = static_cast<Hypergraph*>(h);
   std::vector<uint32_t> cl(cluster, cluster + hg->n), cm;

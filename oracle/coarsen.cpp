@@ -389,3 +389,63 @@ std::vector<Level> coarsen(const Hypergraph& hg, const CoarsenParams& prm, bool 
 }
 
 }  // namespace orc
+
+namespace orc {
+// Rating throughput sample of the reference-semantics rater: fillRatingMap
+// (bloom-filter adaptive edge size, heavy-edge score, multilevel_vertex_pair_
+// rater.h:109-279) and the acceptance scan for the given vertices on the
+// level-0 state (every vertex its own cluster), on `threads` OpenMP threads.
+// No matching: the vertices are independent.  Returns the pin visits (two
+// per pin of a rated net, as the rating sweeps do).
+int64_t rating_sample_ref(const Hypergraph& hg, const std::vector<uint32_t>& vertices, uint32_t ignore,
+                          int threads) {
+  const uint32_t n = hg.n;
+  const uint64_t mask = next_pow2(std::min<uint64_t>(10ull * hg.max_esize, n)) - 1;
+  int64_t visits = 0;
+  int64_t sink = 0;
+#pragma omp parallel num_threads(std::max(1, threads)) reduction(+ : visits, sink)
+  {
+    std::vector<uint8_t> bloom(mask + 1, 0);
+    std::vector<double> val(n, 0.0);
+    std::vector<int32_t> pos(n, -1);
+    std::vector<uint32_t> keys;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t i = 0; i < int64_t(vertices.size()); ++i) {
+      const uint32_t u = vertices[size_t(i)];
+      for (uint64_t j = hg.voff[u]; j < hg.voff[u + 1]; ++j) {
+        const uint32_t e = hg.inc[j];
+        const uint32_t esz = hg.esize(e);
+        if (esz >= ignore) continue;
+        uint32_t s = 0;
+        for (uint64_t q = hg.noff[e]; q < hg.noff[e + 1]; ++q) {
+          const uint64_t b = hg.pins[q] & mask;
+          if (!bloom[b]) { ++s; bloom[b] = 1; }
+        }
+        for (uint64_t q = hg.noff[e]; q < hg.noff[e + 1]; ++q) bloom[hg.pins[q] & mask] = 0;
+        const double score = double(hg.nw[e]) / double(std::max<uint32_t>(s, 2) - 1);
+        for (uint64_t q = hg.noff[e]; q < hg.noff[e + 1]; ++q) {
+          const uint32_t rep = hg.pins[q];
+          const uint64_t b = rep & mask;
+          if (!bloom[b]) {
+            if (pos[rep] < 0) { pos[rep] = int32_t(keys.size()); keys.push_back(rep); val[rep] = 0.0; }
+            val[rep] += score;
+            bloom[b] = 1;
+          }
+        }
+        for (uint64_t q = hg.noff[e]; q < hg.noff[e + 1]; ++q) bloom[hg.pins[q] & mask] = 0;
+        visits += 2 * int64_t(esz);
+      }
+      double best = 0.0;
+      uint32_t target = UINT32_MAX;
+      for (size_t q = keys.size(); q-- > 0;) {
+        const uint32_t t = keys[q];
+        if (t != u && val[t] > best) { best = val[t]; target = t; }
+        pos[t] = -1;
+      }
+      keys.clear();
+      sink += target != UINT32_MAX;
+    }
+  }
+  return visits + 0 * sink;
+}
+}  // namespace orc

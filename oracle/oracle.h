@@ -206,6 +206,10 @@ void rate_sync(const Hypergraph& hg, const CoarsenParams& prm, int pass, int sub
                const std::vector<uint32_t>& cid, const std::vector<int64_t>& cw,
                const std::vector<uint8_t>& matched, const std::vector<uint32_t>& proposers,
                std::vector<uint32_t>& target, std::vector<uint64_t>& rating);
+// Reference-semantics rating of the given vertices on the level-0 state with
+// OpenMP threads (a throughput sample, no matching); returns pin visits.
+int64_t rating_sample_ref(const Hypergraph& hg, const std::vector<uint32_t>& vertices, uint32_t ignore,
+                          int threads);
 // StaticHypergraph::contract (static_hypergraph.cpp:58-513), canonical form:
 // pins ascending per net, incident nets ascending, survivor of parallel nets
 // is the smallest original net id.  cmap receives fine->coarse vertex ids.

@@ -57,6 +57,7 @@ def lib():
             "orc_cluster_pass": (C.c_uint32, [vp, C.c_int, C.c_int, C.c_uint32, C.c_int32, C.c_double, C.c_uint32,
                                               C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, u32p, i64p]),
             "orc_hg_contract": (vp, [vp, u32p, u32p]),
+            "orc_rating_sample_ref": (C.c_int64, [vp, u32p, C.c_uint32, C.c_uint32, C.c_int]),
             "orc_rate_sync": (None, [vp, C.c_int, C.c_int, C.c_int32, C.c_uint32, C.c_uint64, C.c_uint64, C.c_int,
                                      u32p, i64p, u8p, u32p, C.c_uint32, u32p, u64p]),
             "orc_louvain_level_sync_trace": (C.c_int, [vp, C.c_uint64, C.c_int, C.c_double, C.c_uint64, C.c_int,
@@ -177,6 +178,12 @@ class Hypergraph:
                             np.ascontiguousarray(cw, dtype=np.int64), np.ascontiguousarray(matched, dtype=np.uint8),
                             pr, len(pr), t, r)
         return t[:len(pr)], r[:len(pr)]
+
+    def rating_sample_ref(self, vertices, ignore=1000, threads=1):
+        """Reference-semantics rating of the given vertices on the level-0
+        state (OpenMP threads, no matching): pin visits."""
+        v = np.ascontiguousarray(vertices, dtype=np.uint32)
+        return int(lib().orc_rating_sample_ref(self.h, v, len(v), ignore, threads))
 
     def cluster_pass(self, sync, pass_nr, contraction_limit, max_node_weight, max_shrink=2.5, ignore=1000,
                      sampling=200000, seed=0, sub_rounds=16, disable_rand=False, prefer_unmatched=True,

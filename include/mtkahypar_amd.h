@@ -185,6 +185,18 @@ typedef struct {
  * parallel_louvain.cpp:62-66.  communities (host, num_vertices) may be NULL. */
 MKP_API int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* params, int32_t* communities,
                         mkp_louvain_stats* stats);
+/* One evaluation of computeMaxGainCluster (local_moving_modularity.h:87-129)
+ * for every node of the level-0 graph (the star expansion with net node
+ * N + rank(e), or the graph itself when every net has two pins) on the
+ * clustering node_communities (host, one id per node), by the move kernels,
+ * and metrics::modularity of it (local_moving_modularity.cpp:28-56).
+ * targets (may be NULL): the cluster the node would move to, -1 to stay
+ * (including a singleton's move to a larger singleton); gains (may be NULL):
+ * max(gain of staying, best candidate's gain) in FP64; modularity may be
+ * NULL.  Nothing moves. */
+MKP_API int mkp_louvain_evaluate(mkp_hypergraph* hg, const mkp_louvain_params* params,
+                                 const int32_t* node_communities, int32_t* targets, double* gains,
+                                 double* modularity);
 
 /* -------------------------------------------------------- coarsening --- */
 typedef struct mkp_hierarchy mkp_hierarchy;

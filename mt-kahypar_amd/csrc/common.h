@@ -126,6 +126,14 @@ __device__ inline SlotGroup slot_group(int slot) {
   return r;
 }
 
+// orders the LDS accesses of the lanes of one wave (in place of a block
+// barrier when every LDS region is private to one wave)
+__device__ inline void lds_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // round tags (one per decision family)
 constexpr uint64_t kLpBucketTag = 0x4C50420000000000ull;
 constexpr uint64_t kLpCoinTag = 0x4C50430000000000ull;

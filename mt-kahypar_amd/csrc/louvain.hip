@@ -168,6 +168,9 @@ struct LvArgs {
   // decision per position of the launch's list: the target cluster, or -1
   // (written for every node of the sub-round, so no compaction is needed)
   int32_t* dec;
+  // evaluation only (nullptr otherwise): the best gain per list position,
+  // max(gain of staying, best candidate's gain) in FP64
+  double* gain_out = nullptr;
   // batch of member hypergraphs as one disjoint union (nullptr: one
   // hypergraph): member and member-local id of every node, the member's
   // 1 / total volume, seed and whether it takes part in the round
@@ -225,13 +228,6 @@ __device__ inline void map_insert_aggregated(int32_t* K, double* Wt, uint32_t* F
   if (pending) insert(key, w, pos);
 }
 
-// orders the LDS accesses of the lanes of one wave (in place of a block
-// barrier when every LDS region is private to one wave)
-__device__ inline void lds_wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // best (gain, first) pair: larger gain, then smaller first occurrence
 __device__ inline bool better(double g, uint32_t f, double bg, uint32_t bf) {
@@ -248,6 +244,7 @@ __device__ inline void lv_decide(const LvArgs& a, uint32_t pos, uint32_t u, doub
   if (best_key >= 0 && best_gain > bg && !(a.csize[from] == 1 && a.csize[best_key] == 1 && best_key > from))
     to = best_key;
   a.dec[pos] = to;
+  if (a.gain_out) a.gain_out[pos] = best_key >= 0 && best_gain > bg ? best_gain : bg;
 }
 
 // The node's own inputs of the decision, loaded ahead of the arc sweep so
@@ -264,6 +261,7 @@ __device__ inline void lv_decide(const LvArgs& a, uint32_t pos, const LvNode& nd
   if (best_key >= 0 && best_gain > bg && !(nd.csize_from == 1 && a.csize[best_key] == 1 && best_key > nd.from))
     to = best_key;
   a.dec[pos] = to;
+  if (a.gain_out) a.gain_out[pos] = best_key >= 0 && best_gain > bg ? best_gain : bg;
 }
 
 // ---------------------------------------------------------- move kernels --
@@ -1111,6 +1109,56 @@ __global__ void sum_volume_kernel(const double* __restrict__ vol, uint32_t n, do
   if (threadIdx.x == 0) out[blockIdx.x] = part[0];
 }
 
+// ------------------------------------------------------------ evaluation --
+// Cluster volumes and sizes of a given clustering (sums of node volumes;
+// exact under integer arc weights).
+__global__ void cluster_state_kernel(const int32_t* __restrict__ C, const double* __restrict__ nvol, uint32_t n,
+                                     double* __restrict__ cvol, int32_t* __restrict__ csize) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    atomicAdd(&cvol[C[u]], nvol[u]);
+    atomicAdd(&csize[C[u]], 1);
+  }
+}
+
+// metrics::modularity (local_moving_modularity.cpp:28-56), first pass: per
+// cluster its total volume and its internal volume (node volumes minus the
+// weights of arcs leaving the cluster).
+template <class View>
+__global__ void modularity_volume_kernel(View g, const int32_t* __restrict__ C, const double* __restrict__ nvol,
+                                         uint32_t n, double* __restrict__ internal, double* __restrict__ total) {
+  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    const int32_t cu = C[u];
+    double cut = 0.0;
+    const uint32_t d = g.degree(u);
+    for (uint32_t i = 0; i < d; ++i) {
+      uint32_t v;
+      double w;
+      g.arc(u, i, v, w);
+      if (C[v] != cu) cut += w;
+    }
+    atomicAdd(&total[cu], nvol[u]);
+    atomicAdd(&internal[cu], nvol[u] - cut);
+  }
+}
+
+// second pass: sum over clusters with volume of internal - total^2 / TV in a
+// fixed shape (64 blocks of 256 threads, the partials summed on the host)
+__global__ void __launch_bounds__(256)
+modularity_sum_kernel(const double* __restrict__ internal, const double* __restrict__ total, uint32_t n, double tv,
+                      double* __restrict__ out) {
+  __shared__ double part[256];
+  double s = 0.0;
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x)
+    if (total[c] > 0.0) s += internal[c] - (total[c] * total[c]) / tv;
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = part[0];
+}
+
 // --------------------------------------------------------------- driver --
 struct LouvainRun {
   Context& c;
@@ -1312,6 +1360,7 @@ struct LouvainRun {
     auto at = [&](int slot) {
       LvArgs x = a;
       x.dec = a.dec + off[slot];
+      if (a.gain_out) x.gain_out = a.gain_out + off[slot];
       return x;
     };
     if (cnt[s0]) {
@@ -1453,6 +1502,78 @@ struct LouvainRun {
       changed |= moved > 0;
     }
     return changed;
+  }
+
+  // One evaluation of computeMaxGainCluster for every node of the level-0
+  // graph g on the clustering C (host, n entries): per node the decision of
+  // the move kernels (target or -1) and the best gain in FP64, and
+  // metrics::modularity of C.  Nothing moves.
+  template <class View>
+  void evaluate(const View& g, uint32_t n, const int32_t* C_host, int32_t* target, double* gain, double* q) {
+    const hipStream_t s = c.stream;
+    DevBuf<double> vol(std::max<uint32_t>(n, 1)), cvol(std::max<uint32_t>(n, 1));
+    DevBuf<int32_t> C(std::max<uint32_t>(n, 1)), csize(std::max<uint32_t>(n, 1));
+    prepare(g, n, vol.p);
+    const double tv = total_volume(vol.p, n);
+    MKP_HIP(hipMemcpyAsync(C.p, C_host, size_t(n) * 4, hipMemcpyHostToDevice, s));
+    fill_async<double>(cvol.p, 0.0, n, s);
+    fill_async<int32_t>(csize.p, 0, n, s);
+    cluster_state_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(C.p, vol.p, n, cvol.p, csize.p);
+    MKP_LAUNCH_CHECK();
+    if (q) {
+      DevBuf<double> internal(std::max<uint32_t>(n, 1)), total(std::max<uint32_t>(n, 1)), part(64);
+      fill_async<double>(internal.p, 0.0, n, s);
+      fill_async<double>(total.p, 0.0, n, s);
+      modularity_volume_kernel<View><<<grid_for(n, 256, 65536), 256, 0, s>>>(g, C.p, vol.p, n, internal.p, total.p);
+      modularity_sum_kernel<<<64, 256, 0, s>>>(internal.p, total.p, n, tv, part.p);
+      MKP_LAUNCH_CHECK();
+      double* h = c.staging<double>(64);
+      MKP_HIP(hipMemcpyAsync(h, part.p, 64 * 8, hipMemcpyDeviceToHost, s));
+      c.sync();
+      double sum = 0.0;
+      for (int i = 0; i < 64; ++i) sum += h[i];
+      *q = sum / tv;
+    }
+    if (!target && !gain) return;
+    // every node in one sub-round
+    const int B = 1, slots = kLvClasses;
+    const unsigned lblocks = lv_list_blocks(n);
+    DevBuf<uint32_t> list(std::max<uint32_t>(n, 1)), bmat(size_t(slots) * lblocks), counts(slots);
+    DevBuf<uint64_t> amat(size_t(slots) * lblocks), sarcs(slots);
+    DevBuf<int32_t> dec(std::max<uint32_t>(n, 1));
+    DevBuf<double> dg(std::max<uint32_t>(n, 1));
+    LvArgs a;
+    a.seed = prm.seed; a.level = 0; a.round = 0; a.buckets = B; a.bucket = 0; a.bucket_salt = 0;
+    a.sampling = uint32_t(std::min<uint64_t>(prm.sampling, 0xffffffffu));
+    a.recip = 1.0 / tv; a.C = C.p; a.cvol = cvol.p; a.csize = csize.p; a.nvol = vol.p;
+    a.dec = dec.p; a.gain_out = dg.p;
+    lv_count_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, bmat.p, amat.p, nullptr);
+    lv_totals_kernel<<<slots, 256, 0, s>>>(lblocks, bmat.p, amat.p, nullptr, counts.p, sarcs.p, nullptr);
+    lv_scatter_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, bmat.p, counts.p, list.p);
+    MKP_LAUNCH_CHECK();
+    std::vector<uint32_t> cnt(slots), off(slots);
+    std::vector<uint64_t> sa(slots), sb(slots, 0);
+    MKP_HIP(hipMemcpyAsync(cnt.data(), counts.p, slots * 4, hipMemcpyDeviceToHost, s));
+    MKP_HIP(hipMemcpyAsync(sa.data(), sarcs.p, slots * 8, hipMemcpyDeviceToHost, s));
+    c.sync();
+    uint32_t run = 0;
+    for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
+    if (cnt[4 * B]) {
+      reserve_heavy(sa[4 * B], cnt[4 * B]);
+      prepare_heavy(g, a, list.p + off[4 * B], cnt[4 * B]);
+    }
+    launch_move(g, a, list.p, off, cnt, sa, sb, 0, c.tune.lv_group_lanes ? c.tune.lv_group_lanes : 8);
+    std::vector<uint32_t> hl(n);
+    std::vector<int32_t> hd(n);
+    std::vector<double> hg(n);
+    MKP_HIP(hipMemcpyAsync(hl.data(), list.p, size_t(n) * 4, hipMemcpyDeviceToHost, s));
+    MKP_HIP(hipMemcpyAsync(hd.data(), dec.p, size_t(n) * 4, hipMemcpyDeviceToHost, s));
+    MKP_HIP(hipMemcpyAsync(hg.data(), dg.p, size_t(n) * 8, hipMemcpyDeviceToHost, s));
+    c.sync();
+    for (uint32_t i = 0; i < run; ++i) {
+      if (target) target[hl[i]] = hd[i];
+      if (gain) gain[hl[i]] = hg[i];
+    }
   }
 
   uint32_t max_degree_ = 0;
@@ -2176,6 +2297,40 @@ void ensure_net_ranks(mkp_hypergraph* hg) {
 }  // namespace mkp
 
 using namespace mkp;
+
+extern "C" int mkp_louvain_evaluate(mkp_hypergraph* hg, const mkp_louvain_params* prm,
+                                    const int32_t* node_communities, int32_t* targets, double* gains,
+                                    double* modularity) {
+  return guarded([&] {
+    MKP_CHECK(hg && prm && node_communities, "null argument");
+    Context& c = hg->ctx->c;
+    Bind bind_(c);
+    int wt = prm->edge_weight;
+    if (wt < 0) wt = (double(hg->m) / double(std::max<uint32_t>(hg->n, 1)) < 0.75) ? 2 : 0;
+    LouvainRun run(c, *prm);
+    bool is_graph = hg->m > 0;
+    {
+      std::vector<uint32_t> noff(size_t(hg->m) + 1);
+      MKP_HIP(hipMemcpyAsync(noff.data(), hg->noff.p, noff.size() * 4, hipMemcpyDeviceToHost, c.stream));
+      c.sync();
+      for (uint32_t e = 0; e < hg->m && is_graph; ++e) is_graph = noff[e + 1] - noff[e] == 2;
+    }
+    const uint32_t nodes = is_graph ? hg->n : hg->n + hg->m;
+    for (uint32_t u = 0; u < nodes; ++u)
+      MKP_CHECK(node_communities[u] >= 0 && uint32_t(node_communities[u]) < nodes, "community id out of range");
+    if (is_graph) {
+      PlainView g{hg->n, hg->voff.p, hg->inc.p, hg->noff.p, hg->pins.p, hg->inc_vtx.p, hg->nw.p, wt,
+                  hg->unit_net_weights};
+      run.evaluate(g, nodes, node_communities, targets, gains, modularity);
+    } else {
+      ensure_net_ranks(hg);
+      StarView g{hg->n,         hg->m,         hg->p,    hg->voff.p, hg->inc.p,          hg->noff.p,
+                 hg->pins.p,    hg->inc_vtx.p, hg->pin_net.p, hg->nw.p, wt, hg->unit_net_weights,
+                 hg->net_order.p, hg->inc_rank.p, hg->rpoff.p, hg->rpins.p, hg->rpin_rank.p};
+      run.evaluate(g, nodes, node_communities, targets, gains, modularity);
+    }
+  });
+}
 
 extern "C" int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* prm, int32_t* communities,
                            mkp_louvain_stats* stats) {

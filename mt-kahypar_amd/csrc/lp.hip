@@ -327,6 +327,148 @@ lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
   }
 }
 
+// ------------------------------------------------------------ gain (G) --
+// G lanes per vertex for 32 < k <= KMAX and degree <= kThreadDegreeLimit:
+// the lanes take the incident nets and subtract w(e) from the LDS score of
+// every block in the net's connectivity set; the Km1Policy scan with its coin
+// flips is evaluated as group reductions exactly as in lp_gain_wave_kernel.
+// Heavier vertices go to the heavy list for the wave kernel.  Each group's
+// scores are private to its wave, so waves never wait for each other.
+template <int G, int KMAX>
+__global__ void __launch_bounds__(256)
+lp_gain_group_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na, uint32_t* __restrict__ cand_flag,
+                     int32_t* __restrict__ cand_to, int32_t* __restrict__ cand_gain, uint32_t* __restrict__ heavy) {
+  if (!a.vgraph && a.counters[6]) return;  // a rolled-back sub-round stopped the refinement
+  constexpr int GPB = 256 / G, GPW = 64 / G;
+  __shared__ int32_t scores[GPB * KMAX];
+  // one hypergraph: its block weights and bounds, read by every score scan,
+  // are staged in LDS once per block (the weights do not change during the
+  // gain sweep)
+  __shared__ int64_t s_pw[KMAX], s_mw[KMAX];
+  if (!a.vgraph) {
+    for (int b = threadIdx.x; b < a.k; b += blockDim.x) { s_pw[b] = a.part_w[b]; s_mw[b] = a.max_w[b]; }
+    __syncthreads();
+  }
+  const int grp = threadIdx.x / G, gl = threadIdx.x % G;
+  int32_t* sc = scores + grp * KMAX;
+  const uint32_t wid = threadIdx.x >> 6, wgrp = (threadIdx.x & 63) / G;
+  int64_t visits = 0, swept = 0;
+  for (uint32_t base = (blockIdx.x * 4 + wid) * GPW; base < na; base += gridDim.x * 4 * GPW) {
+    const uint32_t i = base + wgrp;
+    bool live = i < na;
+    const uint32_t u = live ? active[i] : 0u;
+    const LpMember mb = member_of(a, u);
+    if (live && a.vgraph && member_counters(a, mb.g)[6]) {
+      if (gl == 0) cand_flag[i] = 0;
+      live = false;
+    }
+    const uint32_t jb = live ? a.voff[u] : 0u, je = live ? a.voff[u + 1] : 0u;
+    if (live && je - jb > kThreadDegreeLimit) {
+      if (gl == 0) {
+        if (je - jb <= kHighDegree) {
+          const uint32_t h = atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[3]), 1ull);
+          heavy[h] = i;
+        } else {
+          cand_flag[i] = 0;
+        }
+      }
+      live = false;
+    }
+    const int k = mb.k;
+    for (int b = gl; b < k; b += G) sc[b] = 0;
+    lds_wave_sync();
+    const int from = live ? a.part[u] : 0;
+    const int fwi = from / a.L.epw;
+    const int fsh = (from - fwi * a.L.epw) * a.L.b;
+    int32_t internal = 0;
+    int border = 0;
+    if (live) {
+      for (uint32_t j = jb + gl; j < je; j += G) {
+        const uint32_t e = a.inc[j];
+        const uint64_t* r = a.pc + uint64_t(e) * a.L.stride;
+        const int32_t w = int32_t(r[a.L.woff]);
+        if (uint32_t((r[fwi] >> fsh) & a.L.mask) > 1) internal += w;
+        int lambda = 0;
+        for (int ci = 0; ci < a.vcs; ++ci) {
+          uint64_t bits = r[a.L.coff + ci];
+          lambda += __popcll(bits);
+          while (bits) {
+            const int b = ci * 64 + __ffsll(static_cast<long long>(bits)) - 1;
+            bits &= bits - 1;
+            atomicAdd(&sc[b], -w);
+          }
+        }
+        border |= lambda > 1;
+      }
+    }
+    for (int o = G / 2; o > 0; o >>= 1) {
+      internal += __shfl_xor(internal, o, G);
+      border |= __shfl_xor(border, o, G);
+    }
+    lds_wave_sync();
+    int flag = 0, to = from;
+    int32_t gain = 0;
+    if (live) {
+      if (a.vgraph) {
+        if (gl == 0) {
+          counter_add(member_counters(a, mb.g) + 1, int64_t(je - jb));
+          counter_add(member_counters(a, mb.g) + 5, 1);
+        }
+      } else if (gl == 0) {
+        visits += int64_t(je - jb);
+        swept += 1;
+      }
+    }
+    // the group reductions run on every lane of the wave
+    const int64_t wu = live ? a.vw[u] : 0;
+    const int64_t* pw = a.vgraph ? a.part_w + mb.kb : s_pw;
+    const int64_t* mw = a.vgraph ? a.max_w + mb.kb : s_mw;
+    const bool scan = live && border;
+    const int INF = 0x7fffffff;
+    int m = INF;
+    if (scan)
+      for (int b = gl; b < k; b += G)
+        if (b != from && pw[b] + wu <= mw[b]) m = min(m, sc[b] + internal);
+    for (int o = G / 2; o > 0; o >>= 1) m = min(m, __shfl_xor(m, o, G));
+    int i0 = INF;
+    if (scan && m != INF && m < 0)
+      for (int b = gl; b < k; b += G)
+        if (b != from && pw[b] + wu <= mw[b] && sc[b] + internal == m) i0 = min(i0, b);
+    for (int o = G / 2; o > 0; o >>= 1) i0 = min(i0, __shfl_xor(i0, o, G));
+    int i1 = -1;
+    if (scan && m != INF && m <= 0) {
+      // m < 0: last coin-winning block with score m after the first one;
+      // m == 0: last coin-winning block with score 0
+      const int lo = m < 0 ? i0 : -1;
+      for (int b = gl; b < k; b += G)
+        if (b > lo && b != from && pw[b] + wu <= mw[b] && sc[b] + internal == m && lp_coin(a, mb, b)) i1 = max(i1, b);
+    }
+    for (int o = G / 2; o > 0; o >>= 1) i1 = max(i1, __shfl_xor(i1, o, G));
+    if (scan && m != INF) {
+      if (m < 0) {
+        to = i1 >= 0 ? i1 : i0;
+        gain = m;
+      } else if (m == 0) {
+        if (i1 >= 0) to = i1;
+        gain = 0;
+      }
+      flag = lp_perform(a, mb, from, to, gain) ? 1 : 0;
+    }
+    if (live && gl == 0) {
+      cand_flag[i] = uint32_t(flag);
+      if (flag) { cand_to[i] = to; cand_gain[i] = gain; }
+    }
+    lds_wave_sync();
+  }
+  if (a.vgraph) return;
+  visits = wave_sum(visits);
+  swept = wave_sum(swept);
+  if ((threadIdx.x & 63) == 0 && visits) {
+    stripe_add(a.stripes, 1, visits);
+    stripe_add(a.stripes, 5, swept);
+  }
+}
+
 // ------------------------------------------------------------ gain (B) --
 // One wave per vertex, any k: lanes split the incident nets and accumulate
 // negative scores into a per-wave LDS array with LDS atomics; the sequential
@@ -537,6 +679,88 @@ __global__ void lp_admit_kernel(LpArgs a, const uint64_t* __restrict__ keys, con
 // --------------------------------------------------------------- apply --
 // mode 0: apply admitted moves; mode 1: undo admitted && !keep;
 // mode 2: undo admitted && keep (sub-round rollback)
+//
+// Vertices of more than kLpWaveMove incident nets are moved by their whole
+// wave (lanes over the nets) after the lanes' own light moves; a single
+// thread would otherwise walk thousands of nets alone while the rest of the
+// launch waits for it.
+constexpr uint32_t kLpWaveMove = 32;
+
+// changeNodePart's pin-count updates for nets j = jb + t0, jb + t0 + step, ...
+// of u moving from -> to: 64-bit atomic adds on the packed counters (eight
+// nets in flight at a time), connectivity toggled on 0<->1 transitions;
+// returns the km1 delta of these nets.
+__device__ inline int64_t lp_move_nets(const LpArgs& a, uint32_t u, int from, int to, uint32_t t0, uint32_t step) {
+  const int fwi = from / a.L.epw, fsh = (from - fwi * a.L.epw) * a.L.b;
+  const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
+  const unsigned long long dec = static_cast<unsigned long long>(-(int64_t(1) << fsh));
+  const unsigned long long incv = 1ull << tsh;
+  constexpr int NB = 8;
+  const uint32_t jb = a.voff[u] + t0, je = a.voff[u + 1];
+  int64_t du = 0;
+  for (uint32_t j0 = jb; j0 < je; j0 += NB * step) {
+    uint32_t e[NB];
+    uint64_t of[NB], ot[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) e[t] = j0 + t * step < je ? a.inc[j0 + t * step] : 0u;
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      if (j0 + t * step < je) {
+        unsigned long long* rec = reinterpret_cast<unsigned long long*>(a.pc + uint64_t(e[t]) * a.L.stride);
+        of[t] = atomicAdd(rec + fwi, dec);
+        ot[t] = atomicAdd(rec + twi, incv);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      if (j0 + t * step >= je) break;
+      const int32_t w = int32_t(a.pc[uint64_t(e[t]) * a.L.stride + a.L.woff]);
+      if (((of[t] >> fsh) & a.L.mask) == 1) {
+        atomicXor(reinterpret_cast<unsigned long long*>(&a.pc[uint64_t(e[t]) * a.L.stride + a.L.coff + from / 64]),
+                  1ull << (from % 64));
+        du -= w;
+      }
+      if (((ot[t] >> tsh) & a.L.mask) == 0) {
+        atomicXor(reinterpret_cast<unsigned long long*>(&a.pc[uint64_t(e[t]) * a.L.stride + a.L.coff + to / 64]),
+                  1ull << (to % 64));
+        du += w;
+      }
+    }
+  }
+  return du;
+}
+
+// real km1 delta of the move u: from -> to on the post-move state over nets
+// j = jb + t0, jb + t0 + step, ...
+__device__ inline int64_t lp_real_delta(const LpArgs& a, uint32_t u, int from, int to, uint32_t t0, uint32_t step) {
+  const int fwi = from / a.L.epw, fsh = (from - fwi * a.L.epw) * a.L.b;
+  const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
+  int64_t real = 0;
+  constexpr int NB = 8;  // batches of 8 incident nets, loads issued together
+  const uint32_t jb = a.voff[u] + t0, je = a.voff[u + 1];
+  for (uint32_t j0 = jb; j0 < je; j0 += NB * step) {
+    uint32_t e[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) e[t] = a.inc[j0 + t * step < je ? j0 + t * step : jb];
+    int32_t w[NB];
+    uint32_t pt[NB], pf[NB];
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      const uint64_t* r = a.pc + uint64_t(e[t]) * a.L.stride;  // one record per net
+      w[t] = int32_t(r[a.L.woff]);
+      pt[t] = uint32_t((r[twi] >> tsh) & a.L.mask);
+      pf[t] = uint32_t((r[fwi] >> fsh) & a.L.mask);
+    }
+#pragma unroll
+    for (int t = 0; t < NB; ++t) {
+      if (j0 + t * step >= je) break;
+      if (pt[t] == 1) real += w[t];
+      if (pf[t] == 0) real -= w[t];
+    }
+  }
+  return real;
+}
+
 __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const int32_t* __restrict__ cfrom,
                                 const int32_t* __restrict__ cto, const uint8_t* __restrict__ admit,
                                 const uint8_t* __restrict__ keep, uint32_t nc, int mode) {
@@ -559,45 +783,20 @@ __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const
     wave_add_i64(a.part_w, on, uint64_t(mb.kb) + uint64_t(to), wu, 3);
     wave_add_i64(a.part_w, on, uint64_t(mb.kb) + uint64_t(from), -wu, 3);
     int64_t du = 0;
+    const bool wide = on && a.voff[u + 1] - a.voff[u] > kLpWaveMove;
     if (on) {
       a.part[u] = to;
-      const int fwi = from / a.L.epw, fsh = (from - fwi * a.L.epw) * a.L.b;
-      const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
-      const unsigned long long dec = static_cast<unsigned long long>(-(int64_t(1) << fsh));
-      const unsigned long long incv = 1ull << tsh;
-      // batches of 8 incident nets: both counter updates of all 8 nets are in
-      // flight together; the 0<->1 transitions then toggle connectivity
-      constexpr int NB = 8;
-      const uint32_t jb = a.voff[u], je = a.voff[u + 1];
-      for (uint32_t j0 = jb; j0 < je; j0 += NB) {
-        uint32_t e[NB];
-        uint64_t of[NB], ot[NB];
-#pragma unroll
-        for (int t = 0; t < NB; ++t) e[t] = j0 + t < je ? a.inc[j0 + t] : 0u;
-#pragma unroll
-        for (int t = 0; t < NB; ++t) {
-          if (j0 + t < je) {
-            unsigned long long* rec = reinterpret_cast<unsigned long long*>(a.pc + uint64_t(e[t]) * a.L.stride);
-            of[t] = atomicAdd(rec + fwi, dec);
-            ot[t] = atomicAdd(rec + twi, incv);
-          }
-        }
-#pragma unroll
-        for (int t = 0; t < NB; ++t) {
-          if (j0 + t >= je) break;
-          const int32_t w = int32_t(a.pc[uint64_t(e[t]) * a.L.stride + a.L.woff]);
-          if (((of[t] >> fsh) & a.L.mask) == 1) {
-            atomicXor(reinterpret_cast<unsigned long long*>(&a.pc[uint64_t(e[t]) * a.L.stride + a.L.coff + from / 64]),
-                      1ull << (from % 64));
-            du -= w;
-          }
-          if (((ot[t] >> tsh) & a.L.mask) == 0) {
-            atomicXor(reinterpret_cast<unsigned long long*>(&a.pc[uint64_t(e[t]) * a.L.stride + a.L.coff + to / 64]),
-                      1ull << (to % 64));
-            du += w;
-          }
-        }
-      }
+      if (!wide) du = lp_move_nets(a, u, from, to, 0, 1);
+    }
+    // the wide moves of the wave, one after another, all lanes on its nets
+    uint64_t wides = __ballot(wide);
+    while (wides) {
+      const int owner = __ffsll(static_cast<long long>(wides)) - 1;
+      wides &= wides - 1;
+      const uint32_t wu_ = __shfl(u, owner, 64);
+      const int wf = __shfl(from, owner, 64), wt = __shfl(to, owner, 64);
+      const int64_t part = wave_sum(lp_move_nets(a, wu_, wf, wt, uint32_t(lane), 64u));
+      if (lane == owner) du = part;
     }
     if (mode != 2) {
       if (a.vgraph) member_add_wave(a.counters, on, mb.g, 0, du);
@@ -619,36 +818,23 @@ __global__ void lp_verify_kernel(LpArgs a, const uint32_t* __restrict__ cu, cons
   for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < nc; base += gridDim.x * blockDim.x) {
     const uint32_t j = base + uint32_t(lane);
     const bool on = j < nc && !(admit && !admit[j]);
+    const uint32_t u = on ? cu[j] : 0u;
+    const int from = on ? cfrom[j] : 0, to = on ? cto[j] : 0;
+    const bool wide = on && a.voff[u + 1] - a.voff[u] > kLpWaveMove;
+    int64_t real = 0;
+    if (on && !wide) real = lp_real_delta(a, u, from, to, 0, 1);
+    uint64_t wides = __ballot(wide);
+    while (wides) {
+      const int owner = __ffsll(static_cast<long long>(wides)) - 1;
+      wides &= wides - 1;
+      const uint32_t wu_ = __shfl(u, owner, 64);
+      const int wf = __shfl(from, owner, 64), wt = __shfl(to, owner, 64);
+      const int64_t part = wave_sum(lp_real_delta(a, wu_, wf, wt, uint32_t(lane), 64u));
+      if (lane == owner) real = part;
+    }
     bool k = false;
     uint32_t g = 0;
     if (on) {
-      const uint32_t u = cu[j];
-      const int from = cfrom[j], to = cto[j];
-      const int fwi = from / a.L.epw, fsh = (from - fwi * a.L.epw) * a.L.b;
-      const int twi = to / a.L.epw, tsh = (to - twi * a.L.epw) * a.L.b;
-      int64_t real = 0;
-      constexpr int NB = 8;  // batches of 8 incident nets, loads issued together
-      const uint32_t jb = a.voff[u], je = a.voff[u + 1];
-      for (uint32_t j0 = jb; j0 < je; j0 += NB) {
-        uint32_t e[NB];
-#pragma unroll
-        for (int t = 0; t < NB; ++t) e[t] = a.inc[j0 + t < je ? j0 + t : jb];
-        int32_t w[NB];
-        uint32_t pt[NB], pf[NB];
-#pragma unroll
-        for (int t = 0; t < NB; ++t) {
-          const uint64_t* r = a.pc + uint64_t(e[t]) * a.L.stride;  // one record per net
-          w[t] = int32_t(r[a.L.woff]);
-          pt[t] = uint32_t((r[twi] >> tsh) & a.L.mask);
-          pf[t] = uint32_t((r[fwi] >> fsh) & a.L.mask);
-        }
-#pragma unroll
-        for (int t = 0; t < NB; ++t) {
-          if (j0 + t >= je) break;
-          if (pt[t] == 1) real += w[t];
-          if (pf[t] == 0) real -= w[t];
-        }
-      }
       k = real == int64_t(cgain[j]) || real <= 0;
       keep[j] = k ? 1 : 0;
       if (a.vgraph) g = a.vgraph[u];
@@ -806,7 +992,7 @@ lp_bucket_keys_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
         const uint32_t u = active[i];
         const uint32_t d = a.voff[u + 1] - a.voff[u];
         if (d <= kHighDegree) {
-          if (a.k > 32 || d > kThreadDegreeLimit) { wv = 1; wi = d; } else { li = d; }
+          if (a.k > 128 || d > kThreadDegreeLimit) { wv = 1; wi = d; } else { li = d; }
         }
       }
       uint64_t pending = __ballot(b >= 0);
@@ -867,6 +1053,19 @@ static void launch_gain(Context& c, const LpArgs& a, LpWork& w, const uint32_t* 
       MKP_LAUNCH_CHECK();
     }
     // heavy vertices: persistent grid over the device-side list
+    TimedRegion tr(c.prof, s, kFamLpGain, gb.wave, "lp_gain_wave_kernel");
+    lp_gain_wave_kernel<<<unsigned(num_cus) * 8, 64, a.k * sizeof(int32_t), s>>>(
+        a, act, w.heavy.p, a.counters + 3, 0, w.cand_flag.p, w.cand_to_a.p, w.cand_gain_a.p);
+    MKP_LAUNCH_CHECK();
+  } else if (a.k <= 128) {
+    {
+      // 8 lanes per vertex: mean degrees are small next to the k = 64..128
+      // scores each group scans
+      TimedRegion tr(c.prof, s, kFamLpGain, gb.thread, "lp_gain_group_kernel<8, 128>");
+      lp_gain_group_kernel<8, 128><<<grid_for(na, 32, 65536), 256, 0, s>>>(a, act, na, w.cand_flag.p, w.cand_to_a.p,
+                                                                        w.cand_gain_a.p, w.heavy.p);
+      MKP_LAUNCH_CHECK();
+    }
     TimedRegion tr(c.prof, s, kFamLpGain, gb.wave, "lp_gain_wave_kernel");
     lp_gain_wave_kernel<<<unsigned(num_cus) * 8, 64, a.k * sizeof(int32_t), s>>>(
         a, act, w.heavy.p, a.counters + 3, 0, w.cand_flag.p, w.cand_to_a.p, w.cand_gain_a.p);

@@ -140,6 +140,7 @@ SIGNATURES = {
     "mkp_write_hgr": (C.c_int, [vp, C.c_char_p]),
     "mkp_write_partition_file": (C.c_int, [vp, C.c_uint32, C.c_char_p]),
     "mkp_louvain": (C.c_int, [vp, P(LouvainParams), vp, P(LouvainStats)]),
+    "mkp_louvain_evaluate": (C.c_int, [vp, P(LouvainParams), vp, vp, vp, P(C.c_double)]),
     "mkp_cluster_pass": (C.c_int, [vp, P(CoarsenParams), C.c_int32, vp, P(C.c_uint32), P(C.c_int64)]),
     "mkp_hypergraph_contract": (C.c_int, [vp, vp, P(vp), vp]),
     "mkp_coarsen": (C.c_int, [vp, P(CoarsenParams), P(vp)]),
@@ -364,6 +365,17 @@ class Hypergraph:
         stats = dict(arcs_swept=st.arcs_swept, rounds=st.rounds, levels=st.levels, moves=st.moves,
                      device_ms=st.device_ms)
         return (out[:self.info().num_vertices] if out is not None else None), stats
+
+    def louvain_evaluate(self, node_communities, edge_weight=-1, sampling=200000):
+        """Best-cluster decision, FP64 best gain per node of the level-0 graph
+        on the given clustering, and its modularity (nothing moves)."""
+        prm = LouvainParams(edge_weight, 5, 0.01, sampling, 0, 1)
+        cm = np.ascontiguousarray(node_communities, dtype=np.int32)
+        tg = np.zeros(max(len(cm), 1), np.int32)
+        gn = np.zeros(max(len(cm), 1), np.float64)
+        q = C.c_double()
+        check(lib().mkp_louvain_evaluate(self.h, C.byref(prm), _p(cm), _p(tg), _p(gn), C.byref(q)))
+        return tg[:len(cm)], gn[:len(cm)], q.value
 
     @staticmethod
     def coarsen_params(contraction_limit, max_node_weight, min_shrink=1.01, max_shrink=2.5, ignore=1000,

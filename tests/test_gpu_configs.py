@@ -104,6 +104,12 @@ def louvain_pair(dh, oh, seed):
     assert (st["arcs_swept"], st["rounds"], st["levels"], st["moves"]) == \
         (ost["arcs"], ost["rounds"], ost["levels"], ost["moves"])
     assert st["levels"] >= 2
+    # modularity of the final clustering of all nodes (bit-identical on both
+    # sides), summed on the device and by the oracle: within 1e-12
+    _, _, q = dh.louvain_evaluate(C)
+    qo = g.modularity(C)
+    say("modularity device %.17g oracle %.17g" % (q, qo))
+    assert abs(q - qo) <= 1e-12
 
 
 # ------------------------------------------------ power-law 16k x 16k, k=2

@@ -132,3 +132,37 @@ def test_group_lane_shapes_and_autotuner_agree(ctx):
             c2.set_tuning("louvain_group_lanes", 24)
     finally:
         c2.close()
+
+
+@pytest.mark.parametrize("state", ["random", "louvain"])
+def test_fp64_gains_targets_and_modularity(ctx, state):
+    """computeMaxGainCluster on the device for every node of the star
+    expansion against the oracle (local_moving_modularity.h:87-129): the best
+    gain in FP64 within 1e-12, the same target (or staying, including the
+    singleton rule of the synchronous rounds), and metrics::modularity
+    (local_moving_modularity.cpp:28-56) within 1e-12."""
+    print("case: FP64 gains and modularity, %s clustering" % state, flush=True)
+    n, noff, pins = powerlaw(5000, 7000, 300, 21, heavy_nets=2)
+    oh = orc.Hypergraph(n, noff, pins)
+    g = orc.Graph(oh, orc.Graph.UNIFORM, ranked=True)
+    nodes = g.sizes()[0]
+    if state == "random":
+        rng = np.random.default_rng(3)
+        C = rng.integers(0, nodes // 20, nodes).astype(np.int32)
+    else:
+        C, _ = g.louvain(sync=True, seed=5)
+    dh = mtk_amd.Hypergraph(ctx, n, noff, pins)
+    tg, gain, q = dh.louvain_evaluate(C)
+    size = np.bincount(C, minlength=nodes)
+    worst = 0.0
+    for u in range(nodes):
+        to, og = g.best_cluster(C, u)
+        worst = max(worst, abs(og - gain[u]))
+        frm = int(C[u])
+        if to == frm or (size[frm] == 1 and size[to] == 1 and to > frm):
+            assert tg[u] == -1, u
+        else:
+            assert tg[u] == to, u
+    assert worst <= 1e-12, worst
+    assert abs(q - g.modularity(C)) <= 1e-12
+    dh.close()

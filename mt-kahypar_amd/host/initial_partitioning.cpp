@@ -95,6 +95,32 @@ struct ScoreHeap {
       up(size_t(pos[v]));
     }
   }
+  // Deferred form of add: key[v] += d now, the heap position is restored by
+  // settle() once per touched vertex (a grown vertex's nets may add to the
+  // same vertex thousands of times).  Keys are summed in the same order as
+  // by add(), so the heap holds the same keys and pops the same vertices.
+  std::vector<uint32_t> touched;
+  std::vector<uint8_t> is_touched;
+  void add_deferred(uint32_t v, double d) {
+    if (is_touched.empty()) is_touched.assign(pos.size(), 0);
+    key[v] += d;
+    if (!is_touched[v]) {
+      is_touched[v] = 1;
+      touched.push_back(v);
+    }
+  }
+  void settle() {
+    for (uint32_t v : touched) {
+      is_touched[v] = 0;
+      if (pos[v] < 0) {
+        heap.push_back(v);
+        up(heap.size() - 1);
+      } else {
+        up(size_t(pos[v]));
+      }
+    }
+    touched.clear();
+  }
   bool empty() const { return heap.empty(); }
   uint32_t pop() {
     const uint32_t top = heap[0];
@@ -182,16 +208,18 @@ struct Bisector {
         const double add = double(h.nw[e]) / double(std::max<uint64_t>(sz - 1, 1));
         for (uint64_t j = h.noff[e]; j < h.noff[e + 1]; ++j) {
           const uint32_t x = h.pins[j];
-          if (side[x] == 1 && !skip[x]) pq.add(x, add);
+          if (side[x] == 1 && !skip[x]) pq.add_deferred(x, add);
         }
       }
+      pq.settle();
     }
   }
 
-  // breadth-first growing of side 0 from a random seed
+  // breadth-first growing of side 0 from a random seed (a net's pins are
+  // all seen after its first scan, so each net is scanned once)
   void grow_bfs(std::mt19937_64& rng) {
     side.assign(h.n, 1);
-    std::vector<uint8_t> seen(h.n, 0);
+    std::vector<uint8_t> seen(h.n, 0), scanned(h.m, 0);
     std::deque<uint32_t> q;
     int64_t w0 = 0;
     const uint32_t start = std::uniform_int_distribution<uint32_t>(0, h.n - 1)(rng);
@@ -211,7 +239,8 @@ struct Bisector {
       w0 += h.vw[v];
       for (uint64_t i = h.voff[v]; i < h.voff[v + 1]; ++i) {
         const uint32_t e = h.inc[i];
-        if (h.noff[e + 1] - h.noff[e] > kGrowNetLimit) continue;
+        if (scanned[e] || h.noff[e + 1] - h.noff[e] > kGrowNetLimit) continue;
+        scanned[e] = 1;
         for (uint64_t j = h.noff[e]; j < h.noff[e + 1]; ++j) {
           const uint32_t x = h.pins[j];
           if (!seen[x]) { seen[x] = 1; q.push_back(x); }
@@ -234,19 +263,48 @@ struct Bisector {
     }
   }
 
+  // Gains of all vertices kept current across moves (integers, so always
+  // equal to gain(u) recomputed): moving v changes the gain of another pin
+  // of a net only when the net's count on v's side is 1 or 2 or on the
+  // other side 0 or 1 before the move.  v's own gain becomes the negation
+  // of its gain before the move (per net, [b == 0] - [a == 1] with a, b the
+  // counts on v's old and new side before the move).
+  std::vector<int64_t> gc;
+  void move_cached(uint32_t v) {
+    const int f = side[v], t = 1 - f;
+    const int64_t gv = gc[v];
+    for (uint64_t i = h.voff[v]; i < h.voff[v + 1]; ++i) {
+      const uint32_t e = h.inc[i];
+      const uint32_t a = pc[f][e], b = pc[t][e];
+      const int64_t w = h.nw[e];
+      // gain change of a pin on v's side and of a pin on the other side
+      const int64_t df = w * ((a == 2) - (a == 1) + (b == 0));
+      const int64_t dt = w * ((b == 0) - (a == 1) - (b == 1));
+      if (df || dt)
+        for (uint64_t j = h.noff[e]; j < h.noff[e + 1]; ++j) {
+          const uint32_t x = h.pins[j];
+          if (x != v) gc[x] += side[x] == f ? df : dt;
+        }
+    }
+    move(v);
+    gc[v] = -gv;
+  }
+
   // restore balance, then passes of positive-gain moves (zero-gain moves
   // only towards the lighter side) in random order
   void refine(int passes, std::mt19937_64& rng) {
     counts();
+    gc.resize(h.n);
+    for (uint32_t u = 0; u < h.n; ++u) gc[u] = gain(u);
     for (int f = 0; f < 2; ++f) {
       if (w[f] <= max_w[f]) continue;
       std::vector<std::pair<int64_t, uint32_t>> cand;
       for (uint32_t u = 0; u < h.n; ++u)
-        if (side[u] == f) cand.push_back({-gain(u), u});
+        if (side[u] == f) cand.push_back({-gc[u], u});
       std::sort(cand.begin(), cand.end());
       for (const auto& c : cand) {
         if (w[f] <= max_w[f]) break;
-        if (w[1 - f] + h.vw[c.second] <= max_w[1 - f]) move(c.second);
+        if (w[1 - f] + h.vw[c.second] <= max_w[1 - f]) move_cached(c.second);
       }
     }
     std::vector<uint32_t> order(h.n);
@@ -257,9 +315,9 @@ struct Bisector {
       for (uint32_t u : order) {
         const int f = side[u], t = 1 - f;
         if (w[t] + h.vw[u] > max_w[t]) continue;
-        const int64_t g = gain(u);
+        const int64_t g = gc[u];
         if (g > 0 || (g == 0 && w[f] - h.vw[u] > w[t] + h.vw[u] && w[f] > w[t] + 2 * int64_t(h.vw[u]))) {
-          move(u);
+          move_cached(u);
           improved |= g > 0;
         }
       }

@@ -119,12 +119,31 @@ co_mark_nets_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t*
   const uint32_t cnt = *count;
   const uint32_t lane = threadIdx.x % G;
   const uint32_t groups = gridDim.x * (blockDim.x / G);
-  for (uint32_t i = blockIdx.x * (blockDim.x / G) + threadIdx.x / G; i < cnt; i += groups) {
-    const uint32_t u = list[i];
-    for (uint32_t j = a.voff[u] + lane; j < a.voff[u + 1]; j += G) {
-      const uint32_t e = a.inc[j];
-      if (a.noff[e + 1] - a.noff[e] >= a.ignore || nmark[e] == stamp) continue;
-      if (atomicExch(&nmark[e], stamp) != stamp) mlist[atomicAdd(mcount, 1u)] = e;
+  // the loops run in whole waves (trip counts are wave-uniform), so the
+  // appends of a wave go to mlist through one atomic
+  const uint32_t wlane = threadIdx.x & 63;
+  const uint32_t first = blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+  const uint32_t wfirst = first - (wlane / G);  // group of the wave's lane 0
+  for (uint32_t i0 = wfirst; i0 < cnt; i0 += groups) {
+    const uint32_t i = i0 + wlane / G;
+    const uint32_t u = i < cnt ? list[i] : 0u;
+    const uint32_t jb = i < cnt ? a.voff[u] : 0u, je = i < cnt ? a.voff[u + 1] : 0u;
+    uint32_t len = je - jb;
+    for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o, 64));
+    for (uint32_t k0 = 0; k0 < len; k0 += G) {
+      const uint32_t k = k0 + lane;
+      bool add = false;
+      uint32_t e = 0;
+      if (jb + k < je) {
+        e = a.inc[jb + k];
+        if (a.noff[e + 1] - a.noff[e] < a.ignore && nmark[e] != stamp) add = atomicExch(&nmark[e], stamp) != stamp;
+      }
+      const uint64_t bal = __ballot(add);
+      if (!bal) continue;
+      uint32_t base = 0;
+      if (wlane == 0) base = atomicAdd(mcount, uint32_t(__popcll(bal)));
+      base = __shfl(base, 0, 64);
+      if (add) mlist[base + __popcll(bal & ((1ull << wlane) - 1))] = e;
     }
   }
 }
@@ -522,6 +541,366 @@ hr_insert_kernel(CoArgs a, HrArgs h, uint64_t total_items, uint64_t total_inc, c
   }
 }
 
+// Heavy proposers rated by one block each, grid-strided, with the rating map
+// in LDS.  The items of up to BLOCK nets at a time are spread over the whole
+// block: a block scan of the distinct counts, and each item finds its net by
+// a binary search over the scan in LDS.  A proposer with at most BLOCK swept
+// nets loads them once for all its passes.
+//
+// hr_hash_kernel: a hash map of 2^tb slots (2^tb >= 2 min(L, n) when that
+// fits in T / 2, L the proposer's (net, cluster) items and n the clusters of
+// the level, else T).  With P passes the keys are split by a hash into P
+// classes and pass p inserts only class p, so every key collects its whole
+// rating in its own pass and the best target is the best over the passes.
+// A pass reserves a count per new key and stops at 3/4 of the slots; the
+// proposer then starts over with twice the passes, and with more than
+// `max_passes` it goes to `over` (the arena path, which counts its visits
+// itself).
+//
+// hr_dense_kernel (levels with few clusters): the map is an array indexed by
+// cluster id, in passes over id ranges of T, with no probing and no
+// overflow.  Scores are positive except for nets of weight 0, whose keys are
+// kept in a presence bit set.
+constexpr uint32_t kHbDense = 8192;  // dense slots per pass
+
+__device__ inline uint32_t hb_class(uint32_t key, uint32_t P) { return fmix32(key ^ 0x5bd1e995u) % P; }
+
+// adds score at key unless the key is new and the map already holds `cap`
+// keys (then *over is set)
+__device__ inline void hb_map_add(uint32_t* K, uint64_t* V, int bits, uint32_t rep, uint64_t score,
+                                  uint32_t* nkeys, uint32_t cap, int* over) {
+  const uint32_t T = 1u << bits;
+  uint32_t h = co_slot(rep, bits);
+  bool reserved = false;
+  while (true) {
+    uint32_t cur = __atomic_load_n(&K[h], __ATOMIC_RELAXED);
+    if (cur == kNone) {
+      if (!reserved) {
+        if (atomicAdd(nkeys, 1u) >= cap) {
+          *over = 1;
+          return;
+        }
+        reserved = true;
+      }
+      cur = atomicCAS(&K[h], kNone, rep);
+    }
+    if (cur == kNone || cur == rep) break;
+    h = (h + 1) & (T - 1);
+  }
+  atomicAdd(reinterpret_cast<unsigned long long*>(&V[h]), static_cast<unsigned long long>(score));
+}
+
+// LDS state of the per-block item walk
+template <int BLOCK>
+struct HbWalk {
+  uint32_t incl[BLOCK];  // running item count per net of the chunk
+  uint32_t pb[BLOCK];    // pin offset of the net
+  uint64_t sc[BLOCK];    // score of the net
+  uint32_t w[BLOCK / 64];
+  uint64_t l[BLOCK / 64];
+  uint32_t end;
+};
+
+// loads nets [j0, min(j0 + BLOCK, jend)) of the proposer; returns their items
+template <int BLOCK>
+__device__ inline uint32_t hb_load(const CoArgs& a, const uint32_t* __restrict__ dcount, uint32_t j0, uint32_t jend,
+                                   HbWalk<BLOCK>& sh) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const uint32_t j = j0 + tid;
+  uint32_t d = 0, pb = 0;
+  uint64_t score = 0;
+  if (j < jend) {
+    const uint32_t e = a.inc[j];
+    pb = a.noff[e];
+    if (a.noff[e + 1] - pb < a.ignore) {
+      d = dcount[e];
+      score = (uint64_t(uint32_t(a.nw[e])) << kFixShift) / uint64_t(max(d, 2u) - 1);
+    }
+  }
+  uint32_t incl = d;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, o, 64);
+    if (int(lane) >= o) incl += t;
+  }
+  if (lane == 63) sh.w[wid] = incl;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+  for (int w = 0; w < BLOCK / 64; ++w) {
+    const uint32_t x = sh.w[w];
+    if (w < int(wid)) before += x;
+    total += x;
+  }
+  sh.incl[tid] = before + incl;
+  sh.pb[tid] = pb;
+  sh.sc[tid] = score;
+  __syncthreads();
+  return total;
+}
+
+// calls f(key, score) for the items [0, total) of the loaded chunk that
+// this thread takes (tid, tid + BLOCK, ...), eight at a time so that their
+// net searches and key loads are issued together; stops when *stop is set
+// (checked per eight); returns the items taken
+template <int BLOCK, class F>
+__device__ inline uint32_t hb_walk(const uint32_t* __restrict__ dclus, const HbWalk<BLOCK>& sh, uint32_t total,
+                                   const int* stop, F&& f) {
+  constexpr int Q = 8;
+  uint32_t done = 0;
+  for (uint32_t base = threadIdx.x; base < total; base += Q * BLOCK) {
+    if (stop && __atomic_load_n(stop, __ATOMIC_RELAXED)) break;
+    uint32_t key[Q];
+    uint64_t sc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const uint32_t it = min(base + uint32_t(q) * BLOCK, total - 1);
+      uint32_t lo = 0, hi = BLOCK - 1;  // first net whose running count exceeds it
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sh.incl[mid] > it) hi = mid; else lo = mid + 1;
+      }
+      const uint32_t start = lo ? sh.incl[lo - 1] : 0u;
+      key[q] = dclus[sh.pb[lo] + (it - start)];
+      sc[q] = sh.sc[lo];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+      if (base + uint32_t(q) * BLOCK < total) {
+        f(key[q], sc[q]);
+        ++done;
+      }
+  }
+  return done;
+}
+
+// sweep end and item count L of proposer u; with at most BLOCK swept nets
+// they stay loaded in sh (*single set, *total their items)
+template <int BLOCK>
+__device__ inline uint64_t hb_prologue(const CoArgs& a, const uint32_t* __restrict__ dcount, uint32_t u,
+                                       HbWalk<BLOCK>& sh, int64_t& vu, uint32_t& jend, bool& single,
+                                       uint32_t& total) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (wid == 0) {
+    const uint32_t je = co_sweep_end_wave(a, dcount, u, lane, vu);
+    if (lane == 0) sh.end = je;
+  }
+  __syncthreads();
+  const uint32_t jb = a.voff[u];
+  jend = sh.end;
+  single = jend - jb <= uint32_t(BLOCK);
+  if (single) {
+    total = hb_load<BLOCK>(a, dcount, jb, jend, sh);
+    return total;
+  }
+  uint64_t L = 0;
+  for (uint32_t j = jb + tid; j < jend; j += BLOCK) {
+    const uint32_t e = a.inc[j];
+    if (a.noff[e + 1] - a.noff[e] < a.ignore) L += dcount[e];
+  }
+  for (int o = 32; o > 0; o >>= 1) L += __shfl_xor(L, o, 64);
+  if (lane == 0) sh.l[wid] = L;
+  __syncthreads();
+  L = 0;
+  for (int w = 0; w < BLOCK / 64; ++w) L += sh.l[w];
+  return L;
+}
+
+// best of the block's per-thread choices; written by thread 0
+template <int BLOCK>
+__device__ inline void hb_finish(const CoArgs& a, uint32_t u, Choice best, uint64_t* s_val, int* s_un,
+                                 uint32_t* s_t) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  for (int o = 32; o > 0; o >>= 1) {
+    Choice other;
+    other.val = __shfl_xor(best.val, o, 64);
+    other.un = __shfl_xor(best.un, o, 64);
+    other.t = __shfl_xor(best.t, o, 64);
+    if (choice_better(a, u, other, best)) best = other;
+  }
+  if (lane == 0) {
+    s_val[wid] = best.val;
+    s_un[wid] = best.un;
+    s_t[wid] = best.t;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < BLOCK / 64; ++w) {
+      Choice other{s_val[w], s_un[w], s_t[w]};
+      if (choice_better(a, u, other, best)) best = other;
+    }
+    a.target[u] = best.t;
+    a.rating[u] = best.val;
+  }
+  __syncthreads();
+}
+
+template <int TBITS, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+hr_hash_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+               const uint32_t* __restrict__ dclus, const uint32_t* __restrict__ dcount, uint32_t max_passes,
+               uint32_t* __restrict__ over, uint32_t* __restrict__ over_count) {
+  constexpr int T = 1 << TBITS;
+  constexpr int NW = BLOCK / 64;
+  __shared__ uint32_t K[T];
+  __shared__ uint64_t V[T];
+  __shared__ HbWalk<BLOCK> sh;
+  __shared__ uint64_t s_val[NW];
+  __shared__ int s_un[NW];
+  __shared__ uint32_t s_t[NW];
+  __shared__ uint32_t s_nkeys, s_done;
+  __shared__ int s_over;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t cnt = *count;
+  int64_t visits = 0;
+  for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+    const uint32_t u = list[i];
+    int64_t vu = 0;
+    uint32_t jend, total0;
+    bool single;
+    const uint64_t L = hb_prologue<BLOCK>(a, dcount, u, sh, vu, jend, single, total0);
+    const uint32_t jb = a.voff[u];
+    const uint64_t D = min<uint64_t>(L, a.n);
+    int tb = TBITS;
+    if (D <= uint64_t(T / 2)) {
+      tb = 6;
+      while ((uint64_t(1) << tb) < 2 * D) ++tb;
+    }
+    uint32_t P = 1;
+    bool ok = max_passes > 1 || D <= uint64_t(T / 2);  // one-pass tier: only maps that cannot overflow
+    Choice best{0, 0, kNone};
+    while (ok) {
+      const uint32_t Tn = 1u << tb;
+      const uint32_t cap = 3 * Tn / 4;
+      best = Choice{0, 0, kNone};
+      auto add = [&](uint32_t pass) {
+        return [&, pass](uint32_t key, uint64_t score) {
+          if (P == 1 || hb_class(key, P) == pass) hb_map_add(K, V, tb, key, score, &s_nkeys, cap, &s_over);
+        };
+      };
+      for (uint32_t pass = 0; pass < P; ++pass) {
+        uint32_t done = 0;  // items this thread walked in the pass
+        for (uint32_t t = tid; t < Tn; t += BLOCK) {
+          K[t] = kNone;
+          V[t] = 0;
+        }
+        if (tid == 0) {
+          s_nkeys = 0;
+          s_over = 0;
+          s_done = 0;
+        }
+        __syncthreads();
+        if (single) {
+          done += hb_walk<BLOCK>(dclus, sh, total0, &s_over, add(pass));
+        } else {
+          for (uint32_t j0 = jb; j0 < jend; j0 += BLOCK) {
+            const uint32_t total = hb_load<BLOCK>(a, dcount, j0, jend, sh);
+            done += hb_walk<BLOCK>(dclus, sh, total, &s_over, add(pass));
+            __syncthreads();
+            if (s_over) break;
+          }
+        }
+        __syncthreads();
+        const bool failed = s_over != 0;
+        if (!failed) {
+          const Choice c = co_scan_map(a, u, K, V, Tn, tid, BLOCK);
+          if (choice_better(a, u, c, best)) best = c;
+        } else if (done) {
+          atomicAdd(&s_done, done);
+        }
+        __syncthreads();
+        if (failed) {
+          ok = false;
+          break;
+        }
+      }
+      if (ok) break;
+      // the failed pass saw `cap` keys of its class within s_done of the L
+      // items: size the classes for about half a map each
+      const uint64_t seen = max<uint64_t>(s_done, 1);
+      __syncthreads();  // s_done is cleared by the next pass
+      const uint64_t est = uint64_t(cap) * P * L / seen;
+      const uint64_t per = 3 * uint64_t(T) / 8;
+      const uint64_t next = max<uint64_t>(2 * uint64_t(P), (est + per - 1) / per);
+      if (next > max_passes) break;
+      P = uint32_t(next);
+      tb = TBITS;
+      ok = true;
+    }
+    if (!ok) {
+      if (tid == 0) over[atomicAdd(over_count, 1u)] = u;
+      continue;
+    }
+    visits += vu;
+    hb_finish<BLOCK>(a, u, best, s_val, s_un, s_t);
+  }
+  if ((tid & 63) == 0 && visits)
+    atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(visits));
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+hr_dense_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                const uint32_t* __restrict__ dclus, const uint32_t* __restrict__ dcount) {
+  constexpr uint32_t T = kHbDense;
+  constexpr int NW = BLOCK / 64;
+  __shared__ uint64_t V[T];
+  __shared__ uint32_t bits[T / 32];
+  __shared__ HbWalk<BLOCK> sh;
+  __shared__ uint64_t s_val[NW];
+  __shared__ int s_un[NW];
+  __shared__ uint32_t s_t[NW];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t cnt = *count;
+  int64_t visits = 0;
+  for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+    const uint32_t u = list[i];
+    int64_t vu = 0;
+    uint32_t jend, total0;
+    bool single;
+    hb_prologue<BLOCK>(a, dcount, u, sh, vu, jend, single, total0);
+    const uint32_t jb = a.voff[u];
+    const int64_t cwu = a.cw[u];
+    const int32_t cu = a.comm[u];
+    Choice best{0, 0, kNone};
+    for (uint32_t lo = 0; lo < a.n; lo += T) {
+      const uint32_t span = min(T, a.n - lo);
+      for (uint32_t t = tid; t < span; t += BLOCK) V[t] = 0;
+      for (uint32_t t = tid; t < (span + 31) / 32; t += BLOCK) bits[t] = 0;
+      __syncthreads();
+      auto add = [&](uint32_t key, uint64_t score) {
+        const uint32_t r = key - lo;
+        if (r < span) {
+          if (score) atomicAdd(reinterpret_cast<unsigned long long*>(&V[r]), static_cast<unsigned long long>(score));
+          else atomicOr(&bits[r >> 5], 1u << (r & 31));
+        }
+      };
+      if (single) {
+        hb_walk<BLOCK>(dclus, sh, total0, nullptr, add);
+      } else {
+        for (uint32_t j0 = jb; j0 < jend; j0 += BLOCK) {
+          const uint32_t total = hb_load<BLOCK>(a, dcount, j0, jend, sh);
+          hb_walk<BLOCK>(dclus, sh, total, nullptr, add);
+          __syncthreads();
+        }
+      }
+      __syncthreads();
+      for (uint32_t t = tid; t < span; t += BLOCK) {
+        const uint64_t v = V[t];
+        if (v == 0 && !((bits[t >> 5] >> (t & 31)) & 1u)) continue;
+        const uint32_t rep = lo + t;
+        if (rep == u) continue;
+        if (cwu + a.cw[rep] > a.max_w || a.comm[rep] != cu) continue;
+        Choice c{v, a.prefer_unmatched ? (a.matched[rep] ? 0 : 1) : 0, rep};
+        if (choice_better(a, u, c, best)) best = c;
+      }
+      __syncthreads();
+    }
+    visits += vu;
+    hb_finish<BLOCK>(a, u, best, s_val, s_un, s_t);
+  }
+  if ((tid & 63) == 0 && visits)
+    atomicAdd(reinterpret_cast<unsigned long long*>(a.visits), static_cast<unsigned long long>(visits));
+}
+
 __global__ void __launch_bounds__(256) hr_scan_kernel(CoArgs a, HrArgs h, uint64_t chunks, uint64_t* __restrict__ cval,
                                                       int* __restrict__ cun, uint32_t* __restrict__ ct) {
   __shared__ uint64_t s_val[4];
@@ -572,20 +951,29 @@ __global__ void hr_decide_kernel(CoArgs a, HrArgs h, const uint64_t* __restrict_
   }
 }
 
-// rating upper bound: sum of the sizes of the nets the rating sweeps
+// rating upper bound: sum of the sizes of the nets the rating sweeps; G
+// lanes per vertex (G divides 64)
+template <int G>
 __global__ void co_ub_kernel(const uint32_t* __restrict__ voff, const uint32_t* __restrict__ inc,
                              const uint32_t* __restrict__ noff, uint32_t n, uint32_t ignore,
                              uint64_t* __restrict__ ub, uint64_t* __restrict__ ub_max) {
   uint64_t mx = 0;
-  for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+  const uint32_t lane = threadIdx.x % G;
+  const uint32_t groups = gridDim.x * (blockDim.x / G);
+  // whole waves iterate together: the group reduction shuffles over G lanes
+  const uint32_t g0 = blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
+  for (uint32_t b = g0 - (threadIdx.x & 63) / G; b < n; b += groups) {
+    const uint32_t u = b + (threadIdx.x & 63) / G;
     uint64_t s = 0;
-    for (uint32_t j = voff[u]; j < voff[u + 1]; ++j) {
-      const uint32_t e = inc[j];
-      const uint32_t sz = noff[e + 1] - noff[e];
-      s += sz < ignore ? sz : 0;
-    }
-    ub[u] = s;
-    mx = max(mx, s);
+    if (u < n)
+      for (uint32_t j = voff[u] + lane; j < voff[u + 1]; j += G) {
+        const uint32_t e = inc[j];
+        const uint32_t sz = noff[e + 1] - noff[e];
+        s += sz < ignore ? sz : 0;
+      }
+    for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+    if (u < n && lane == 0) ub[u] = s;
+    if (u < n) mx = max(mx, s);
   }
   for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
   if ((threadIdx.x & 63) == 0 && mx) atomicMax(reinterpret_cast<unsigned long long*>(ub_max), static_cast<unsigned long long>(mx));
@@ -833,15 +1221,46 @@ namespace mkp {
 // work arrays of the heavy-proposer rating, kept across sub-rounds
 struct HeavyRating {
   DevBuf<uint64_t> tsize, isize, ioff, toff, inc_cnt, inc_off, cval;
-  DevBuf<uint32_t> inc_net, inc_own, keys, ct;
+  DevBuf<uint32_t> inc_net, inc_own, keys, ct, over1, over2, ocount;
   DevBuf<uint64_t> vals;
   DevBuf<int> cun;
 };
 
-// rating of the heavy proposers (list, *count on the device)
+// rating of the heavy proposers (list, *count on the device), one block per
+// proposer with the map in LDS: on levels of at most 2 * kHbDense clusters
+// by cluster id (hr_dense_kernel); otherwise by hash, first in one pass with
+// maps of 4096 slots (blocks of 256, three per CU), then in up to
+// `rating_block_passes` passes with maps of 8192 slots (blocks of 1024); the
+// rest in the global arena (rating_block_passes = 0: every one there)
 void rate_heavy(Context& c, const CoArgs& a, const uint32_t* list, const uint32_t* count_dev, const uint32_t* dclus,
                 const uint32_t* dcount, HeavyRating& w) {
   const hipStream_t s = c.stream;
+  const uint32_t max_passes = uint32_t(c.tune.rating_block_passes);
+  if (max_passes > 0 && a.n <= 2 * kHbDense) {
+    TimedRegion tr(c.prof, s, kFamRating, 0.0, "hr_dense_kernel<1024>");
+    hr_dense_kernel<1024><<<unsigned(c.num_cus), 1024, 0, s>>>(a, list, count_dev, dclus, dcount);
+    MKP_LAUNCH_CHECK();
+    return;
+  }
+  if (max_passes > 0) {
+    w.over1.ensure(a.n);
+    w.over2.ensure(a.n);
+    w.ocount.ensure(2);
+    fill_async<uint32_t>(w.ocount.p, 0u, 2, s);
+    {
+      TimedRegion tr(c.prof, s, kFamRating, 0.0, "hr_hash_kernel<12, 256>");
+      hr_hash_kernel<12, 256><<<unsigned(c.num_cus) * 3, 256, 0, s>>>(a, list, count_dev, dclus, dcount, 1u,
+                                                                      w.over1.p, w.ocount.p);
+    }
+    {
+      TimedRegion tr(c.prof, s, kFamRating, 0.0, "hr_hash_kernel<13, 1024>");
+      hr_hash_kernel<13, 1024><<<unsigned(c.num_cus), 1024, 0, s>>>(a, w.over1.p, w.ocount.p, dclus, dcount,
+                                                                    max_passes, w.over2.p, w.ocount.p + 1);
+    }
+    MKP_LAUNCH_CHECK();
+    list = w.over2.p;
+    count_dev = w.ocount.p + 1;
+  }
   uint32_t* pin = reinterpret_cast<uint32_t*>(c.pinned);
   MKP_HIP(hipMemcpyAsync(pin, count_dev, 4, hipMemcpyDeviceToHost, s));
   c.sync();
@@ -929,8 +1348,15 @@ ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, in
   co_init_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(n, hg->vw.p, cid_out, cw.p, matched.p);
   DevBuf<uint64_t> ub_max(1);
   fill_async<uint64_t>(ub_max.p, 0ull, 1, s);
-  co_ub_kernel<<<grid_for(n, 256, 65536), 256, 0, s>>>(hg->voff.p, hg->inc.p, hg->noff.p, n, prm.ignore_net_size,
-                                                       ub.p, ub_max.p);
+  if (hg->p > 64ull * n)  // long incidence lists: a wave per vertex
+    co_ub_kernel<64><<<grid_for(n, 4, 65536), 256, 0, s>>>(hg->voff.p, hg->inc.p, hg->noff.p, n, prm.ignore_net_size,
+                                                           ub.p, ub_max.p);
+  else if (hg->p > 16ull * n)
+    co_ub_kernel<8><<<grid_for(n, 32, 65536), 256, 0, s>>>(hg->voff.p, hg->inc.p, hg->noff.p, n, prm.ignore_net_size,
+                                                           ub.p, ub_max.p);
+  else
+    co_ub_kernel<1><<<grid_for(n, 256, 65536), 256, 0, s>>>(hg->voff.p, hg->inc.p, hg->noff.p, n, prm.ignore_net_size,
+                                                            ub.p, ub_max.p);
   fill_async<int64_t>(visits.p, 0, 1, s);
   MKP_LAUNCH_CHECK();
   // the heavy tier runs only when some proposer has more than 1024 rating keys

@@ -387,6 +387,9 @@ struct Tuning {
   // the autotuner choose per (graph kind, class size, mean degree)
   int lv_group_lanes = 0;
   std::map<uint64_t, int> lv_group_choice;  // autotuner cache
+  // heavy rating proposers are rated by one block each with LDS maps when
+  // their keys fit in at most this many passes (0: every one in the arena)
+  int rating_block_passes = 4;
 };
 
 struct Context {

@@ -176,6 +176,9 @@ int mkp_context_set_tuning(mkp_context* ctx, const char* key, int64_t value) {
       MKP_CHECK(value == 0 || value == 8 || value == 16 || value == 32 || value == 64,
                 "louvain_group_lanes must be 0 (autotune), 8, 16, 32 or 64");
       ctx->c.tune.lv_group_lanes = int(value);
+    } else if (k == "rating_block_passes") {
+      MKP_CHECK(value >= 0 && value <= 1024, "rating_block_passes must be in [0, 1024]");
+      ctx->c.tune.rating_block_passes = int(value);
     } else if (k == "sleep_wait") {
       MKP_CHECK(value == 0 || value == 1, "sleep_wait must be 0 (spin) or 1 (poll, then sleep)");
       ctx->c.sleep_wait = value != 0;

@@ -127,6 +127,42 @@ def test_cluster_pass_bit_exact(ctx, seed, sub_rounds, pass_nr):
     dh.close()
 
 
+@pytest.mark.parametrize("n", [8000, 20000])
+@pytest.mark.parametrize("passes", [16, 1, 0])
+def test_cluster_pass_heavy_tiers(ctx, passes, n):
+    """Proposers with more than 1024 rating keys are rated by one block each
+    with LDS maps: on levels of at most 16,384 clusters by cluster id (n =
+    8000), otherwise by hash (n = 20000), in one pass when their keys fit and
+    else in up to `rating_block_passes` passes over hash classes of the keys;
+    the others in the global arena.  Fewer passes (1: only the one-pass
+    proposers in LDS; 0: everything in the arena) change no decision."""
+    rng = np.random.default_rng(11)
+    n, nets, noff, pins = powerlaw(n, n * 5 // 4, 300, 11)
+    # 24 nets of 700 pins inside 3,000 vertices: their pins sweep up to ~14
+    # of them, so proposers need one to four passes
+    for _ in range(24):
+        nets.append(sorted((1000 + rng.choice(3000, size=700, replace=False)).tolist()))
+    noff, pins = csr(nets)
+    vw = rng.integers(1, 3, n).astype(np.int32)
+    comm = (np.arange(n) // 4000).astype(np.int32)
+    dh, oh = both(ctx, n, noff, pins, vw, None, comm)
+    sizes = np.diff(noff).astype(np.int64)
+    ub = np.bincount(pins, weights=np.repeat(np.where(sizes < 1000, sizes, 0), sizes), minlength=n)
+    assert (ub > 6144).sum() > 100 and ((ub > 4096) & (ub <= 6144)).sum() > 100
+    assert ((ub > 1024) & (ub <= 4096)).sum() > 100
+    ctx.set_tuning("rating_block_passes", passes)
+    try:
+        for sub_rounds, pass_nr in ((1, 0), (8, 1)):
+            kw = dict(contraction_limit=200, max_node_weight=40, seed=3, sub_rounds=sub_rounds)
+            cl, after, visits = dh.cluster_pass(pass_nr, **kw)
+            ocl, oafter, ovisits = oh.cluster_pass(True, pass_nr, with_visits=True, **kw)
+            assert np.array_equal(cl, ocl), (sub_rounds, pass_nr)
+            assert (after, visits) == (oafter, ovisits)
+    finally:
+        ctx.set_tuning("rating_block_passes", 4)
+    dh.close()
+
+
 def test_cluster_pass_weight_bound_and_sampling(ctx):
     rng = np.random.default_rng(5)
     n, nets, noff, pins = powerlaw(4000, 6000, 200, 5)

@@ -162,6 +162,33 @@ MKP_API int mkp_lp_refine(mkp_partition* p, const mkp_lp_params* params, mkp_lp_
 MKP_API int mkp_lp_refine_nodes(mkp_partition* p, const mkp_lp_params* params, const uint32_t* nodes,
                                 uint32_t num_nodes, mkp_lp_stats* stats);
 
+/* ---------------------------------------------------------------- FM --- */
+typedef struct {
+  const int64_t* max_part_weights; /* k entries (context.cpp:238-247) */
+  int32_t rounds;                  /* r-fm-multitry-rounds (fast preset: 10) */
+  int32_t seeds;                   /* r-fm-seed-nodes (25), at most 64 */
+  uint32_t ignore_net_size;        /* nets this large are not expanded (1000) */
+  uint64_t seed;
+  int32_t sub_rounds;              /* groups of searches per round, each applied before the next runs
+                                      (0: 8) */
+} mkp_fm_params;
+typedef struct {
+  int64_t improvement; /* km1 decrease */
+  int64_t moves;       /* applied moves */
+  int64_t rounds;
+  int64_t searches;
+  double device_ms;    /* wall time of the refinement */
+} mkp_fm_stats;
+/* MultiTryKWayFM<GainCacheStrategy>::refineImpl
+ * (refinement/fm/multitry_kway_fm.cpp:30-150) with LocalizedKWayFM searches
+ * (localized_kway_fm_core.cpp:26-190) and GlobalRollback
+ * (global_rollback.cpp:146-283), in the synchronous form: the searches of a
+ * round run in sub_rounds groups; a group's searches run on the state the
+ * earlier groups left, their best prefixes are merged in search order and the
+ * best balanced prefix of the merged sequence, with recalculated gains, is
+ * applied.  k <= 128. */
+MKP_API int mkp_fm_refine(mkp_partition* p, const mkp_fm_params* params, mkp_fm_stats* stats);
+
 /* ----------------------------------------------------------- Louvain --- */
 typedef struct {
   int32_t edge_weight;       /* -1 hybrid (density rule of partitioner.cpp:62-72),
@@ -261,6 +288,9 @@ typedef struct {
   int32_t ip_threads;              /* host threads of initial partitioning (4); the result does not depend on it */
   double large_net_remove_factor;  /* maxnet-remove-factor (0.01): nets larger than max(N * it, 100, 50000) are
                                       set aside during the multilevel phase */
+  int32_t fm_rounds;               /* r-fm-multitry-rounds (10 with r-fm-type=fm_gain_cache); default 0: FM
+                                      not run (it takes one to two orders of magnitude longer than LP) */
+  int32_t fm_seeds;                /* r-fm-seed-nodes (25), at most 64 */
 } mkp_partition_params;
 /* Initial partitioning of p's hypergraph (the coarsest level) on the host:
  * a pool of greedy-growing and random starts, each improved by k-way km1
@@ -284,12 +314,16 @@ typedef struct {
   double louvain_ms, coarsening_ms, initial_ms, refinement_ms, total_ms;
   int64_t louvain_arcs, rating_visits, lp_visits;
   int64_t removed_vertices, removed_nets;  /* degree-zero vertices and large nets set aside */
+  double fm_ms;                            /* FM part of refinement_ms */
+  int64_t fm_improvement;                  /* km1 decrease by FM over all levels */
 } mkp_partition_stats;
 /* partition() of partitioner.cpp:163-204 with the fast preset (community
  * detection, removal of degree-zero vertices and large nets, multilevel
- * coarsening, initial partitioning of the coarsest hypergraph, LP at every
- * level, restore of the removed vertices by bin packing; FM and rebalancing
- * not run).  parts (host, num_vertices) receives the block ids. */
+ * coarsening, initial partitioning of the coarsest hypergraph, LP and then
+ * (with fm_rounds > 0) FM at every level (uncoarsening.cpp:94-188), the km1
+ * rebalancer when the top
+ * level is over its bounds, restore of the removed vertices by bin packing).
+ * parts (host, num_vertices) receives the block ids. */
 MKP_API int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* params, int32_t* parts,
                                      mkp_partition_stats* stats);
 /* The removals of partition()'s sanitize step (partitioner.cpp:74-86):

@@ -222,6 +222,41 @@ class LabelPropagationRefiner final : public IRefiner {
   mkp_lp_stats last_{};
 };
 
+// MultiTryKWayFM<GainCacheStrategy> (multitry_kway_fm.h:40-92) on the
+// device, synchronous form; refinement nodes are not used (every border
+// vertex seeds a search, as in the log(n)-level case of
+// multitry_kway_fm.cpp:160-175).
+class MultiTryKWayFM final : public IRefiner {
+ public:
+  explicit MultiTryKWayFM(const PartitionContext& ctx) : ctx_(ctx) {}
+  mkp_fm_stats lastStats() const { return last_; }
+
+ private:
+  void initializeImpl(PartitionedHypergraph&) override {}
+
+  bool refineImpl(PartitionedHypergraph& phg, const std::vector<uint32_t>&, Metrics& best,
+                  const double) override {
+    const mkp_partition_params& p = ctx_.p;
+    if (p.fm_rounds <= 0) return false;
+    int64_t W = 0;
+    for (int64_t w : phg.partWeights()) W += w;
+    const int64_t perfect = (W + phg.k() - 1) / phg.k();
+    const int64_t maxw = int64_t(int32_t((1.0 + p.epsilon) * double(perfect)));
+    std::vector<int64_t> maxv(size_t(phg.k()), maxw);
+    mkp_fm_params fp{maxv.data(), p.fm_rounds, p.fm_seeds, p.ignore_net_size,
+                     p.seed + uint64_t(phg.level()) * 1000003ull};
+    mkp_fm_stats st{};
+    check(mkp_fm_refine(phg.handle(), &fp, &st));
+    last_ = st;
+    best.km1 -= st.improvement;  // refineImpl: metrics.km1 -= overall_improvement (multitry_kway_fm.cpp:139)
+    best.imbalance = imbalance(phg);
+    return st.improvement > 0;
+  }
+
+  PartitionContext ctx_;
+  mkp_fm_stats last_{};
+};
+
 // run_parallel_louvain + restrictClusteringToHypernodes: community id per
 // hypernode, also stored as the hypergraph's community ids.
 inline std::vector<int32_t> run_parallel_louvain(Hypergraph& hg, const PartitionContext& ctx,

@@ -298,6 +298,7 @@ enum Family : int {
   kFamLouvainContract,
   kFamRating,
   kFamContraction,
+  kFamFm,
   kNumFamilies
 };
 

@@ -129,6 +129,7 @@ void louvain_batch(mkp_hypergraph* hg, const mkp_louvain_params& prm, int wt, co
 // nodes (host, distinct, ascending) replaces the initial active set if given.
 // With a batch, prm's weight arrays hold ktot entries, stats holds one entry
 // per member and every member refines exactly as it would alone.
+void fm_refine(mkp_partition* p, const mkp_fm_params& prm, mkp_fm_stats* stats);
 void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, const uint32_t* nodes = nullptr,
                uint32_t num_nodes = 0, const LpBatch* batch = nullptr);
 

@@ -118,6 +118,9 @@ void validate_partition_params(const mkp_partition_params& p) {
   MKP_CHECK(p.coarsening_sub_rounds >= 0 && p.coarsening_sub_rounds <= 255,
             "coarsening_sub_rounds must be in [1, 255] (0: default)");
   MKP_CHECK(p.louvain_max_passes >= 0 && p.ip_threads >= 0 && p.ip_threads <= 256, "bad preset parameters");
+  MKP_CHECK(p.fm_rounds >= 0 && p.fm_rounds <= 1000, "fm_rounds must be in [0, 1000]");
+  MKP_CHECK(p.fm_rounds == 0 || (p.fm_seeds >= 1 && p.fm_seeds <= 64), "fm_seeds must be in [1, 64]");
+  MKP_CHECK(p.fm_rounds == 0 || p.k <= 128, "FM supports at most 128 blocks (set fm_rounds = 0)");
 }
 
 double now_ms() {
@@ -234,6 +237,8 @@ int mkp_partition_params_default(mkp_partition_params* p, int32_t k, double epsi
     p->lp_rebalancing = 1;
     p->ip_threads = 4;
     p->large_net_remove_factor = 0.01;
+    p->fm_rounds = 0;  // FM is switched on by r-fm-type=fm_gain_cache (10 rounds) or fm_rounds
+    p->fm_seeds = 25;
   });
 }
 
@@ -310,6 +315,15 @@ int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm
         mkp_lp_stats ls{};
         lp_refine(p, lpp, &ls);
         st.lp_visits += ls.visits;
+      }
+      if (prm->fm_rounds > 0) {
+        const double f0 = now_ms();
+        mkp_fm_params fp{maxv.data(), prm->fm_rounds, prm->fm_seeds, prm->ignore_net_size,
+                         prm->seed + uint64_t(i) * 1000003ull};
+        mkp_fm_stats fs{};
+        fm_refine(p, fp, &fs);
+        st.fm_improvement += fs.improvement;
+        st.fm_ms += now_ms() - f0;
       }
     }
     mkp_partition* top = phg[0].get();

@@ -90,6 +90,12 @@ void apply_option(mkp_partition_params& p, const std::string& key, const std::st
   } else if (key == "r-lp-maximum-iterations") p.lp_max_rounds = std::stoi(v);
   else if (key == "r-lp-rebalancing") p.lp_rebalancing = parse_bool(v);
   else if (key == "r-lp-he-size-activation-threshold") p.lp_activation_threshold = uint32_t(std::stoul(v));
+  else if (key == "r-fm-type") {
+    if (v == "do_nothing") p.fm_rounds = 0;
+    else if (v == "fm_gain_cache") p.fm_rounds = p.fm_rounds > 0 ? p.fm_rounds : 10;
+    else throw std::runtime_error("unknown r-fm-type '" + v + "'");
+  } else if (key == "r-fm-multitry-rounds") p.fm_rounds = std::stoi(v);
+  else if (key == "r-fm-seed-nodes") p.fm_seeds = std::stoi(v);
 }
 
 }  // namespace

@@ -32,6 +32,16 @@ class LpStats(C.Structure):
                 ("rounds", C.c_int64), ("device_ms", C.c_double)]
 
 
+class FmParams(C.Structure):
+    _fields_ = [("max_part_weights", P(C.c_int64)), ("rounds", C.c_int32), ("seeds", C.c_int32),
+                ("ignore_net_size", C.c_uint32), ("seed", C.c_uint64), ("sub_rounds", C.c_int32)]
+
+
+class FmStats(C.Structure):
+    _fields_ = [("improvement", C.c_int64), ("moves", C.c_int64), ("rounds", C.c_int64), ("searches", C.c_int64),
+                ("device_ms", C.c_double)]
+
+
 class LouvainParams(C.Structure):
     _fields_ = [("edge_weight", C.c_int32), ("max_passes", C.c_int32), ("min_move_fraction", C.c_double),
                 ("sampling", C.c_uint64), ("seed", C.c_uint64), ("sub_rounds", C.c_int32)]
@@ -58,14 +68,16 @@ class PartitionParams(C.Structure):
                 ("max_allowed_weight_multiplier", C.c_double), ("contraction_limit_multiplier", C.c_uint32),
                 ("ignore_net_size", C.c_uint32), ("rating_sampling", C.c_uint64), ("lp_enabled", C.c_int32),
                 ("lp_max_rounds", C.c_int32), ("lp_activation_threshold", C.c_uint32), ("lp_rebalancing", C.c_int32),
-                ("ip_threads", C.c_int32), ("large_net_remove_factor", C.c_double)]
+                ("ip_threads", C.c_int32), ("large_net_remove_factor", C.c_double), ("fm_rounds", C.c_int32),
+                ("fm_seeds", C.c_int32)]
 
 
 class PartitionStats(C.Structure):
     _fields_ = [("km1", C.c_int64), ("imbalance", C.c_double), ("levels", C.c_int32), ("louvain_ms", C.c_double),
                 ("coarsening_ms", C.c_double), ("initial_ms", C.c_double), ("refinement_ms", C.c_double),
                 ("total_ms", C.c_double), ("louvain_arcs", C.c_int64), ("rating_visits", C.c_int64),
-                ("lp_visits", C.c_int64), ("removed_vertices", C.c_int64), ("removed_nets", C.c_int64)]
+                ("lp_visits", C.c_int64), ("removed_vertices", C.c_int64), ("removed_nets", C.c_int64),
+                ("fm_ms", C.c_double), ("fm_improvement", C.c_int64)]
 
 
 class BatchLpParams(C.Structure):
@@ -135,6 +147,7 @@ SIGNATURES = {
     "mkp_partition_get_border_noalloc": (C.c_int, [vp, vp]),
     "mkp_partition_km1": (C.c_int, [vp, P(C.c_int64)]),
     "mkp_lp_refine": (C.c_int, [vp, P(LpParams), P(LpStats)]),
+    "mkp_fm_refine": (C.c_int, [vp, P(FmParams), P(FmStats)]),
     "mkp_lp_refine_nodes": (C.c_int, [vp, P(LpParams), vp, C.c_uint32, P(LpStats)]),
     "mkp_read_hgr": (C.c_int, [C.c_char_p, P(vp), P(C.c_uint64)]),
     "mkp_write_hgr": (C.c_int, [vp, C.c_char_p]),
@@ -225,7 +238,7 @@ class Context:
     def synchronize(self):
         check(lib().mkp_context_synchronize(self.h))
 
-    FAMILIES = 7
+    FAMILIES = 8
 
     def set_profiling(self, on):
         check(lib().mkp_context_set_profiling(self.h, int(on)))
@@ -552,6 +565,15 @@ class Partition:
             nd = np.ascontiguousarray(nodes, dtype=np.uint32)
             check(lib().mkp_lp_refine_nodes(self.h, C.byref(prm), _p(nd), len(nd), C.byref(st)))
         return dict(delta=st.delta, moves=st.moves, reverts=st.reverts, visits=st.visits, rounds=st.rounds,
+                    device_ms=st.device_ms)
+
+    def fm_refine(self, max_w, seed=0, rounds=10, seeds=25, ignore=1000, sub_rounds=8):
+        """Multitry localized k-way FM (synchronous form)."""
+        mw = np.ascontiguousarray(max_w, dtype=np.int64)
+        prm = FmParams(mw.ctypes.data_as(P(C.c_int64)), rounds, seeds, ignore, seed, sub_rounds)
+        st = FmStats()
+        check(lib().mkp_fm_refine(self.h, C.byref(prm), C.byref(st)))
+        return dict(improvement=st.improvement, moves=st.moves, rounds=st.rounds, searches=st.searches,
                     device_ms=st.device_ms)
 
     def close(self):

@@ -114,6 +114,51 @@ int64_t orc_lp_refine(void* h, int k, int32_t* part, const int64_t* max_w, const
   return km1(p);
 }
 
+// ---------------------------------------------------------------- FM --
+// stats: improvement, kept moves, rounds, searches; returns km1 after
+int64_t orc_fm_refine(void* h, int k, int32_t* part, const int64_t* max_w, uint64_t seed, int rounds, int seeds,
+                      int pq, int max_moves, uint32_t ignore, uint64_t delta_nets, uint64_t expand_cap, int sub_rounds,
+                      int64_t* stats) {
+  auto* hg = static_cast<Hypergraph*>(h);
+  Partition p;
+  phg_init(p, *hg, k, part);
+  FmParams prm;
+  prm.max_w.assign(max_w, max_w + k);
+  prm.seed = seed;
+  prm.rounds = rounds;
+  prm.seeds = seeds;
+  prm.pq = pq;
+  prm.max_moves = max_moves;
+  prm.ignore = ignore;
+  prm.delta_nets = size_t(delta_nets);
+  prm.expand_cap = expand_cap;
+  prm.sub_rounds = sub_rounds;
+  FmStats st = fm_refine_sync(p, prm);
+  std::copy(p.part.begin(), p.part.end(), part);
+  if (stats) { stats[0] = st.improvement; stats[1] = st.moves; stats[2] = st.rounds; stats[3] = st.searches; }
+  return km1(p);
+}
+// recalculated gains of a move sequence applied in order to (hg, part)
+void orc_fm_recalculate(void* h, int k, const int32_t* part, uint32_t nm, const uint32_t* v, const int32_t* from,
+                        const int32_t* to, int64_t* gain) {
+  Partition p;
+  phg_init(p, *static_cast<Hypergraph*>(h), k, part);
+  std::vector<FmMove> seq(nm);
+  for (uint32_t i = 0; i < nm; ++i) seq[i] = FmMove{v[i], from[i], to[i], 0};
+  const std::vector<int64_t> g = fm_recalculate(p, seq);
+  std::copy(g.begin(), g.end(), gain);
+}
+// best balanced prefix of a sequence with the given gains
+uint64_t orc_fm_best_prefix(void* h, int k, const int32_t* part, uint32_t nm, const uint32_t* v, const int32_t* from,
+                            const int32_t* to, const int64_t* gain, const int64_t* max_w, int64_t* best_gain) {
+  Partition p;
+  phg_init(p, *static_cast<Hypergraph*>(h), k, part);
+  std::vector<FmMove> seq(nm);
+  for (uint32_t i = 0; i < nm; ++i) seq[i] = FmMove{v[i], from[i], to[i], 0};
+  return fm_best_prefix(p, seq, std::vector<int64_t>(gain, gain + nm), std::vector<int64_t>(max_w, max_w + k),
+                        best_gain);
+}
+
 // ------------------------------------------------------------- graph --
 void* orc_graph_new(void* h, int wt, int ranked) {
   return new Graph(build_graph(*static_cast<Hypergraph*>(h), EdgeWeight(wt), ranked != 0));
@@ -305,9 +350,9 @@ int orc_coarsen(void* h, int sync, uint32_t contraction_limit, int32_t max_node_
 // with LP at every level (FM disabled), restore of the removed vertices.
 // times: louvain, coarsening, initial partitioning, refinement, total
 // counters: louvain arcs swept, lp incidences visited, levels, rating pin
-// visits, removed degree-zero vertices, removed large nets
-int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int threads, int32_t* part_out,
-                      double* times, int64_t* counters) {
+// visits, removed degree-zero vertices, removed large nets, FM improvement
+int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int threads, int fm_rounds,
+                      int32_t* part_out, double* times, int64_t* counters) {
   const double t0 = now();
   Hypergraph hg = *static_cast<Hypergraph*>(h);
   // setupPartWeights / setupContractionLimit / setupMaximumAllowedNodeWeight
@@ -350,12 +395,22 @@ int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int t
   LpParams prm;
   prm.k = k; prm.max_w.assign(k, maxw); prm.perfect_w.assign(k, perfect); prm.seed = seed; prm.threads = threads;
   int64_t lp_visits = 0;
+  // LP and then FM at every level (uncoarsening.cpp:94-188); FM has one
+  // form, the synchronous one the device computes, in both modes
+  FmParams fprm;
+  fprm.max_w.assign(k, maxw);
+  fprm.rounds = fm_rounds;
+  int64_t fm_improvement = 0;
   auto refine = [&](const Hypergraph& g, std::vector<int32_t>& pt, int level) {
     Partition p;
     phg_init(p, g, k, pt.data());
     prm.seed = seed + uint64_t(level) * 1000003ull;
     LpStats st = sync ? lp_refine_sync(p, prm) : lp_refine_ref(p, prm);
     lp_visits += st.visits;
+    if (fm_rounds > 0) {
+      fprm.seed = prm.seed;
+      fm_improvement += fm_refine_sync(p, fprm).improvement;
+    }
     pt = p.part;
   };
   refine(coarsest, part, int(levels.size()));
@@ -398,6 +453,7 @@ int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int t
   if (counters) {
     counters[0] = lst.arcs_swept; counters[1] = lp_visits; counters[2] = int64_t(levels.size());
     counters[3] = rating_visits; counters[4] = removed_vertices; counters[5] = removed_nets;
+    counters[6] = fm_improvement;
   }
   return km1(fin);
 }

@@ -128,6 +128,39 @@ void km1_max_gain_move(const Partition& p, uint32_t u, const std::vector<int64_t
 LpStats lp_refine_ref(Partition& p, const LpParams& prm);
 LpStats lp_refine_sync(Partition& p, const LpParams& prm);
 
+// --------------------------------------------------------------------- FM --
+struct FmParams {
+  std::vector<int64_t> max_w;
+  int rounds = 10;               // r-fm-multitry-rounds (fast_preset.ini:63)
+  int seeds = 25;                // r-fm-seed-nodes (fast_preset.ini:67)
+  int pq = 64;                   // queue entries per search
+  int max_moves = 64;            // moves per search
+  uint32_t ignore = 1000;        // nets this large are not expanded
+  size_t delta_nets = 256;       // nets a search may change
+  uint64_t expand_cap = 1024;    // pins one expansion may see
+  int sub_rounds = 8;            // groups of searches per round, applied one after the other
+  uint64_t seed = 0;
+};
+struct FmMove {
+  uint32_t v;
+  int32_t from, to;
+  int64_t gain;
+};
+struct FmRound {
+  std::vector<uint32_t> search_len;     // best local prefix of every search
+  std::vector<FmMove> search_moves;     // those prefixes, concatenated
+  std::vector<FmMove> sequence;         // merged sequence
+  std::vector<int64_t> gains;           // recalculated gains of the sequence
+  size_t kept = 0;
+  int64_t improvement = 0;
+};
+struct FmStats { int rounds = 0; int64_t improvement = 0; int64_t moves = 0; int64_t searches = 0; };
+FmStats fm_refine_sync(Partition& p, const FmParams& prm, std::vector<FmRound>* trace = nullptr);
+std::vector<int64_t> fm_recalculate(const Partition& p, const std::vector<FmMove>& seq);
+size_t fm_best_prefix(const Partition& p, const std::vector<FmMove>& seq, const std::vector<int64_t>& gain,
+                      const std::vector<int64_t>& max_w, int64_t* best_gain);
+void apply_move(Partition& p, const FmMove& m);
+
 // ------------------------------------------------------------------ graph --
 // Graph (graph.h:41): star expansion of a hypergraph, or the hypergraph
 // itself when every net has two pins.

@@ -42,6 +42,11 @@ def lib():
                                          C.POINTER(C.c_int32)]),
             "orc_lp_refine": (C.c_int64, [vp, C.c_int, i32p, i64p, i64p, C.c_uint64, C.c_int, C.c_int, C.c_int,
                                           C.c_int, C.c_int, C.c_int, u32p, C.c_uint32, i64p]),
+            "orc_fm_refine": (C.c_int64, [vp, C.c_int, i32p, i64p, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.c_uint32, C.c_uint64, C.c_uint64, C.c_int, i64p]),
+            "orc_fm_recalculate": (None, [vp, C.c_int, i32p, C.c_uint32, u32p, i32p, i32p, i64p]),
+            "orc_fm_best_prefix": (C.c_uint64, [vp, C.c_int, i32p, C.c_uint32, u32p, i32p, i32p, i64p, i64p,
+                                                C.POINTER(C.c_int64)]),
             "orc_graph_new": (vp, [vp, C.c_int, C.c_int]),
             "orc_graph_free": (None, [vp]),
             "orc_graph_sizes": (None, [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_double),
@@ -65,7 +70,8 @@ def lib():
                                                        C.c_uint64]),
             "orc_coarsen": (C.c_int, [vp, C.c_int, C.c_uint32, C.c_int32, C.c_double, C.c_double, C.c_uint32,
                                       C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
-            "orc_partition": (C.c_int64, [vp, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int, i32p, f64p, i64p]),
+            "orc_partition": (C.c_int64, [vp, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int, C.c_int, i32p, f64p,
+                                          i64p]),
             "orc_reduce": (vp, [vp, C.c_uint32, C.c_uint32, u32p, u32p]),
             "orc_rebalance": (C.c_int64, [vp, C.c_int, i32p, i64p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
             "orc_large_net_threshold": (C.c_uint32, [C.c_uint32, C.c_double]),
@@ -165,6 +171,33 @@ class Hypergraph:
         return part, km1, dict(delta=int(st[0]), moves=int(st[1]), reverts=int(st[2]), visits=int(st[3]),
                                rounds=int(st[4]))
 
+    def fm_refine(self, k, part, max_w, seed=0, rounds=10, seeds=25, pq=64, max_moves=64, ignore=1000,
+                  delta_nets=256, expand_cap=1024, sub_rounds=8):
+        """Synchronous multitry localized k-way FM; returns (parts, km1, stats)."""
+        part = np.array(part, dtype=np.int32)
+        st = np.zeros(4, np.int64)
+        km1 = lib().orc_fm_refine(self.h, k, part, np.ascontiguousarray(max_w, dtype=np.int64), seed, rounds, seeds,
+                                  pq, max_moves, ignore, delta_nets, expand_cap, sub_rounds, st)
+        return part, km1, dict(improvement=int(st[0]), moves=int(st[1]), rounds=int(st[2]), searches=int(st[3]))
+
+    def fm_recalculate(self, k, part, v, frm, to):
+        """recalculateGains: exact gain of every move of the sequence applied in order."""
+        v = np.ascontiguousarray(v, dtype=np.uint32)
+        g = np.zeros(max(len(v), 1), np.int64)
+        lib().orc_fm_recalculate(self.h, k, np.ascontiguousarray(part, dtype=np.int32), len(v), v,
+                                 np.ascontiguousarray(frm, dtype=np.int32), np.ascontiguousarray(to, dtype=np.int32), g)
+        return g[:len(v)]
+
+    def fm_best_prefix(self, k, part, v, frm, to, gain, max_w):
+        """Best balanced prefix of a move sequence: (length, gain)."""
+        v = np.ascontiguousarray(v, dtype=np.uint32)
+        bg = C.c_int64()
+        n = lib().orc_fm_best_prefix(self.h, k, np.ascontiguousarray(part, dtype=np.int32), len(v), v,
+                                     np.ascontiguousarray(frm, dtype=np.int32), np.ascontiguousarray(to, dtype=np.int32),
+                                     np.ascontiguousarray(gain, dtype=np.int64),
+                                     np.ascontiguousarray(max_w, dtype=np.int64), C.byref(bg))
+        return int(n), bg.value
+
     def rate_sync(self, pass_nr, sub_round, cid, cw, matched, proposers, max_node_weight, ignore=1000,
                   sampling=200000, seed=0, prefer_unmatched=True):
         """The rating step of one synchronous clustering sub-round on the given
@@ -228,16 +261,18 @@ class Hypergraph:
         h = lib().orc_reduce(self.h, contraction_limit, net_threshold, vmap, rem)
         return Hypergraph(0, None, None, handle=h), vmap, int(rem[0]), int(rem[1])
 
-    def partition(self, k, eps, seed, sync, threads=1):
+    def partition(self, k, eps, seed, sync, threads=1, fm_rounds=0):
+        """The fast-preset pipeline (LP at every level, then FM when
+        fm_rounds > 0), with the device pipeline's defaults."""
         n = self.sizes()[0]
         part = np.zeros(n, np.int32)
         times = np.zeros(5, np.float64)
-        cnt = np.zeros(6, np.int64)
-        km1 = lib().orc_partition(self.h, k, eps, seed, int(sync), threads, part, times, cnt)
+        cnt = np.zeros(7, np.int64)
+        km1 = lib().orc_partition(self.h, k, eps, seed, int(sync), threads, fm_rounds, part, times, cnt)
         return part, km1, dict(louvain_s=times[0], coarsen_s=times[1], ip_s=times[2], refine_s=times[3],
                                total_s=times[4], louvain_arcs=int(cnt[0]), lp_visits=int(cnt[1]),
                                levels=int(cnt[2]), rating_visits=int(cnt[3]), removed_vertices=int(cnt[4]),
-                               removed_nets=int(cnt[5]))
+                               removed_nets=int(cnt[5]), fm_improvement=int(cnt[6]))
 
 
 class Graph:

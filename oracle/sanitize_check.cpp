@@ -23,8 +23,8 @@ extern "C" {
 void* orc_hg_new(uint32_t n, uint32_t m, const uint64_t* noff, const uint32_t* pins, const int32_t* vw,
                  const int32_t* nw);
 void orc_hg_free(void* h);
-int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int threads, int32_t* part_out,
-                      double* times, int64_t* counters);
+int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int threads, int fm_rounds,
+                      int32_t* part_out, double* times, int64_t* counters);
 }
 
 namespace {
@@ -99,9 +99,9 @@ int main(int argc, char** argv) {
     void* oh = orc_hg_new(h.n, h.m, h.noff.data(), h.pins.data(), h.vw.data(), h.nw.data());
     std::vector<int32_t> part(h.n);
     std::vector<double> times(5);
-    std::vector<int64_t> cnt(6);
+    std::vector<int64_t> cnt(7);
     const int k = sync ? 4 : 8;
-    orc_partition(oh, k, 0.03, 42, sync, 8, part.data(), times.data(), cnt.data());
+    orc_partition(oh, k, 0.03, 42, sync, 8, sync ? 10 : 0, part.data(), times.data(), cnt.data());
     int64_t W = 0;
     for (int32_t w : h.vw) W += w;
     const int64_t perfect = (W + k - 1) / k;

@@ -1,9 +1,11 @@
 // Runs the multilevel pipeline through the C++ plugin interfaces
 // (run_parallel_louvain, MultilevelCoarsener / ICoarsener, initial
-// partitioning, LabelPropagationRefiner / IRefiner) and checks that it gives
-// the same block ids and km1 as the one-call pipeline mkp_partition_hypergraph.
+// partitioning, LabelPropagationRefiner and MultiTryKWayFM / IRefiner) and
+// checks that it gives the same block ids and km1 as the one-call pipeline
+// mkp_partition_hypergraph.
 //
-// usage: plugin_pipeline <file.hgr> <k> <seed>
+// usage: plugin_pipeline <file.hgr> <k> <seed> [fm]
+// (with "fm", both paths run FM after LP at every level, 10 rounds)
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
@@ -14,8 +16,8 @@
 using namespace mt_kahypar_amd;
 
 int main(int argc, char** argv) {
-  if (argc != 4) {
-    std::fprintf(stderr, "usage: %s <file.hgr> <k> <seed>\n", argv[0]);
+  if (argc != 4 && argc != 5) {
+    std::fprintf(stderr, "usage: %s <file.hgr> <k> <seed> [fm]\n", argv[0]);
     return 2;
   }
   const int k = std::atoi(argv[2]);
@@ -33,6 +35,7 @@ int main(int argc, char** argv) {
 
     Context ctx(0);
     PartitionContext pc(k, 0.03, seed);
+    if (argc == 5) pc.p.fm_rounds = 10;
 
     // plugin path
     Hypergraph hg(ctx, info.num_vertices, info.num_nets, noff, pins, vw.data(), nw.data());
@@ -41,7 +44,7 @@ int main(int argc, char** argv) {
     coarsener.coarsen();
     initial_partition(coarsener.coarsestPartitionedHypergraph(), pc);
     std::unique_ptr<IRefiner> lp(new LabelPropagationRefiner(pc));
-    std::unique_ptr<IRefiner> fm;
+    std::unique_ptr<IRefiner> fm(new MultiTryKWayFM(pc));
     PartitionedHypergraph phg = coarsener.uncoarsen(lp, fm);
     const std::vector<int32_t> parts = phg.partIDs();
     const int64_t km1 = phg.km1();
@@ -53,9 +56,9 @@ int main(int argc, char** argv) {
     check(mkp_partition_hypergraph(hg2.handle(), &pc.p, parts2.data(), &st));
 
     const bool same = parts == parts2 && km1 == st.km1;
-    std::printf("levels=%d km1_plugins=%lld km1_pipeline=%lld imbalance=%.4f %s\n", coarsener.numLevels(),
-                static_cast<long long>(km1), static_cast<long long>(st.km1), st.imbalance,
-                same ? "SAME" : "DIFFERENT");
+    std::printf("levels=%d km1_plugins=%lld km1_pipeline=%lld imbalance=%.4f fm_improvement=%lld %s\n",
+                coarsener.numLevels(), static_cast<long long>(km1), static_cast<long long>(st.km1), st.imbalance,
+                static_cast<long long>(st.fm_improvement), same ? "SAME" : "DIFFERENT");
     return same ? 0 : 1;
   } catch (const std::exception& e) {
     std::fprintf(stderr, "error: %s\n", e.what());

@@ -106,12 +106,36 @@ def test_capi_program_in_c(tmp_path):
     assert ("objective=%d" % okm1) in r.stdout
 
 
-@pytest.mark.parametrize("k,seed", [(2, 1), (8, 5)])
-def test_cxx_plugin_interfaces_match_pipeline(k, seed):
+@pytest.mark.parametrize("k,seed,fm", [(2, 1, False), (8, 5, False), (4, 3, True)])
+def test_cxx_plugin_interfaces_match_pipeline(k, seed, fm):
+    """LabelPropagationRefiner and MultiTryKWayFM through IRefiner, level by
+    level, against the one-call pipeline."""
     exe = os.path.join(PROGRAMS, "plugin_pipeline")
-    r = subprocess.run([exe, IBM01, str(k), str(seed)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([exe, IBM01, str(k), str(seed)] + (["fm"] if fm else []), capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "SAME" in r.stdout
+    if fm:
+        assert "fm_improvement=0 " not in r.stdout
+
+
+def test_capi_ini_switches_fm_on(ctx, tmp_path):
+    L = capi()
+    L.mt_kahypar_amd_set_error_mode(1)
+    n, noff, pins, vw, nw = read_hgr(IBM01)
+    ini = tmp_path / "fm.ini"
+    ini.write_text("r-fm-type=fm_gain_cache\nr-fm-multitry-rounds=10\nr-fm-seed-nodes=25\n")
+    c = L.mt_kahypar_context_new()
+    L.mt_kahypar_configure_context_from_file(c, str(ini).encode())
+    part, obj = capi_partition(L, c, n, noff, pins, vw, nw, 4, 0.03, 7)
+    L.mt_kahypar_context_free(c)
+    dh = mtk_amd.Hypergraph(ctx, n, noff, pins, vw, nw)
+    ref, st = dh.partition(4, 0.03, 7, fm_rounds=10)
+    dh.close()
+    assert np.array_equal(part.astype(np.int32), ref) and obj == st["km1"]
+    assert st["fm_improvement"] > 0
+    opart, okm1, ost = orc.Hypergraph(n, noff, pins, vw, nw).partition(4, 0.03, 7, sync=True, fm_rounds=10)
+    assert np.array_equal(ref, opart) and okm1 == st["km1"] and ost["fm_improvement"] == st["fm_improvement"]
 
 
 def random_state(n, noff, pins, k, seed):

@@ -146,8 +146,12 @@ def test_capi_context_ini(tmp_path):
     L.mt_kahypar_configure_context_from_file(ctx, str(p).encode())
     got = mtk_amd.PartitionParams()
     L.mt_kahypar_amd_get_params(ctx, C.byref(got))
-    for f, _ in mtk_amd.PartitionParams._fields_:  # the preset equals the defaults
-        assert getattr(got, f) == getattr(default, f), f
+    # the preset equals the defaults, except that it switches FM on (the
+    # defaults leave it off): r-fm-type=fm_gain_cache, 10 rounds, 25 seeds
+    for f, _ in mtk_amd.PartitionParams._fields_:
+        if f != "fm_rounds":
+            assert getattr(got, f) == getattr(default, f), f
+    assert (default.fm_rounds, got.fm_rounds, got.fm_seeds) == (0, 10, 25)
     p.write_text("c-t=80\np-louvain-edge-weight-function=degree\nr-lp-type=do_nothing\ni-runs=3\n"
                  "p-enable-community-detection=false\nc-max-shrink-factor=3.0\n")
     L.mt_kahypar_configure_context_from_file(ctx, str(p).encode())

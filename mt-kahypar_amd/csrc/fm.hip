@@ -210,9 +210,20 @@ __device__ inline bool fm_key(const FmArgs& a, const FmLds& s, uint32_t y, FmEnt
 #pragma unroll
   for (int i = 0; i < kFmCorr; ++i) { cb[i] = -1; cwt[i] = 0; }
   bool over = false;
-  for (uint32_t j = a.voff[y]; j < a.voff[y + 1]; ++j) {
-    const uint32_t e = a.inc[j];
-    const int slot = fm_dfind(s, e);
+  // the incident nets four at a time (their loads issued together); only
+  // nets in the delta contribute
+  const uint32_t jb = a.voff[y], je = a.voff[y + 1];
+  for (uint32_t j0 = jb; j0 < je; j0 += 4) {
+    uint32_t ee[4];
+    int sl[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) ee[t] = j0 + t < je ? a.inc[j0 + t] : kFmEmpty;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) sl[t] = ee[t] != kFmEmpty ? fm_dfind(s, ee[t]) : -1;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+    const uint32_t e = ee[t];
+    const int slot = sl[t];
     if (slot < 0) continue;
     const int64_t w = a.nw[e];
     const int pf = int(fm_pc(a, e, f));
@@ -238,6 +249,7 @@ __device__ inline bool fm_key(const FmArgs& a, const FmLds& s, uint32_t y, FmEnt
           if (c == nc) { cb[c] = int(b); cwt[c] = w * ch; }
         ++nc;
       }
+    }
     }
   }
   if (over) return false;
@@ -600,52 +612,73 @@ __global__ void fm_nets_kernel(const uint32_t* __restrict__ voff, const uint32_t
 }
 
 // recalculateGains (global_rollback.cpp:207-283) per net, one wave per net:
-// for every moved pin i (id, from f, to t) of net e,
-//   benefit: no later move leaves f, no earlier move enters f, no unmoved pin in f
-//   penalty: no earlier move enters t, no later move leaves t, no unmoved pin in t
+// a pass over the pins collects, per block, the first move into it and the
+// last move out of it and the moved pins leaving it (LDS); the unmoved pins of
+// block b are its pin count at the group start minus the pins leaving it.  A
+// second pass gives every moved pin (id, from f, to t)
+//   benefit if it is the last to leave f, none enters f before it and no
+//   unmoved pin is in f;
+//   penalty if it is the first to enter t, every pin leaving t leaves before
+//   it and no unmoved pin is in t.
 __global__ void __launch_bounds__(256)
 fm_recalc_kernel(FmArgs a, const uint32_t* __restrict__ nlist, const uint32_t* __restrict__ ncount,
                  const uint32_t* __restrict__ move_of, const int32_t* __restrict__ sf, const int32_t* __restrict__ st,
                  int64_t* __restrict__ gain) {
-  const int lane = threadIdx.x & 63;
+  __shared__ uint32_t s_in[4][kFmMaxK], s_out[4][kFmMaxK], s_left[4][kFmMaxK];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t* first_in = s_in[wid];
+  uint32_t* last_out = s_out[wid];
+  uint32_t* left = s_left[wid];
+  for (int b = lane; b < a.k; b += 64) {
+    first_in[b] = 0xffffffffu;
+    last_out[b] = 0;
+    left[b] = 0;
+  }
+  lds_wave_sync();
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
   const uint32_t cnt = *ncount;
-  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < cnt; i += waves) {
+  for (uint32_t i = blockIdx.x * (blockDim.x >> 6) + wid; i < cnt; i += waves) {
     const uint32_t e = nlist[i];
     const uint32_t pb = a.noff[e], pe = a.noff[e + 1];
     const int64_t w = a.nw[e];
     for (uint32_t q0 = pb; q0 < pe; q0 += 64) {
-      const uint32_t y = q0 + lane < pe ? a.pins[q0 + lane] : 0u;
-      const uint32_t mid = q0 + lane < pe ? move_of[y] : 0u;
-      uint64_t movers = __ballot(mid != 0);
-      while (movers) {
-        const int src = __ffsll(static_cast<long long>(movers)) - 1;
-        movers &= movers - 1;
-        const uint32_t id = __shfl(mid, src, 64);
-        const int f = sf[id - 1], t = st[id - 1];
-        bool ben_bad = false, pen_bad = false;
-        for (uint32_t r0 = pb; r0 < pe; r0 += 64) {
-          if (r0 + lane < pe) {
-            const uint32_t z = a.pins[r0 + lane];
-            const uint32_t zid = move_of[z];
-            if (zid == 0) {
-              const int pz = a.part[z];
-              ben_bad |= pz == f;
-              pen_bad |= pz == t;
-            } else if (zid != id) {
-              const int zf = sf[zid - 1], zt = st[zid - 1];
-              ben_bad |= (zf == f && zid > id) || (zt == f && zid < id);
-              pen_bad |= (zt == t && zid < id) || (zf == t && zid > id);
-            }
-          }
-        }
-        const bool ben = !__ballot(ben_bad), pen = !__ballot(pen_bad);
-        if (lane == 0) {
-          if (ben && !pen) atomicAdd(reinterpret_cast<unsigned long long*>(&gain[id - 1]), static_cast<unsigned long long>(w));
-          if (!ben && pen) atomicAdd(reinterpret_cast<unsigned long long*>(&gain[id - 1]), static_cast<unsigned long long>(-w));
+      if (q0 + lane < pe) {
+        const uint32_t id = move_of[a.pins[q0 + lane]];
+        if (id) {
+          atomicMin(&first_in[st[id - 1]], id);
+          atomicMax(&last_out[sf[id - 1]], id);
+          atomicAdd(&left[sf[id - 1]], 1u);
         }
       }
     }
+    lds_wave_sync();
+    for (uint32_t q0 = pb; q0 < pe; q0 += 64) {
+      if (q0 + lane < pe) {
+        const uint32_t id = move_of[a.pins[q0 + lane]];
+        if (id) {
+          const int f = sf[id - 1], t = st[id - 1];
+          const bool ben = last_out[f] == id && first_in[f] > id && fm_pc(a, e, f) == left[f];
+          const bool pen = first_in[t] == id && last_out[t] < id && fm_pc(a, e, t) == left[t];
+          if (ben && !pen)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&gain[id - 1]), static_cast<unsigned long long>(w));
+          if (!ben && pen)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&gain[id - 1]), static_cast<unsigned long long>(-w));
+        }
+      }
+    }
+    lds_wave_sync();
+    // reset the blocks this net touched
+    for (uint32_t q0 = pb; q0 < pe; q0 += 64) {
+      if (q0 + lane < pe) {
+        const uint32_t id = move_of[a.pins[q0 + lane]];
+        if (id) {
+          first_in[st[id - 1]] = 0xffffffffu;
+          last_out[sf[id - 1]] = 0;
+          left[sf[id - 1]] = 0;
+        }
+      }
+    }
+    lds_wave_sync();
   }
 }
 

@@ -186,6 +186,29 @@ def phases_of(ctx, dh, w, arrays, cpu, threads):
                "cpu_ms_finest_level": round(1e3 * cpu["median"]["lp_s"], 1) if cpu else None},
         "total_gpu_pipeline_ms": round(st["total_ms"], 2),
     }
+    # FM (10 multitry rounds, 25 seeds per search) on the top level of the
+    # pipeline's LP-refined partition: km1 with LP + FM
+    maxw = int((1.0 + w["eps"]) * -(-int(arrays["vw"].sum()) // w["k"]))
+    dp = mtk_amd.Partition(dh, w["k"])
+    dp.set_parts(part)
+    km1_lp = dp.km1()
+    ctx.set_profiling(True)
+    ctx.reset_profile()
+    fst = dp.fm_refine([maxw] * w["k"], seed=w["seed"], rounds=10)
+    kp = ctx.kernel_profile()
+    ctx.set_profiling(False)
+    ctx.reset_profile()
+    km1_fm = dp.km1()
+    pw = np.bincount(dp.parts(), weights=arrays["vw"], minlength=w["k"])
+    dp.close()
+    out["fm_top_level"] = {
+        "km1_lp": km1_lp, "km1_lp_fm": km1_fm, "improvement": fst["improvement"],
+        "balanced": bool(pw.max() <= maxw), "wall_ms": round(fst["device_ms"], 1),
+        "rounds": fst["rounds"], "searches": fst["searches"], "moves": fst["moves"],
+        "kernel_ms": {name: round(v["ms"], 2) for name, v in kp.items() if name.startswith("fm_")},
+        "note": "FM is off in the default pipeline (fm_rounds = 0) and on with r-fm-type=fm_gain_cache; "
+                "every level with FM: fm_rounds = 10",
+    }
     return out
 
 

@@ -180,6 +180,7 @@ struct FmLds {
 
 __device__ inline uint32_t fm_dslot0(uint32_t e) { return (e * 0x9E3779B1u) >> (32 - 9); }
 static_assert(kFmDeltaSlots == 512, "fm_dslot0 hashes into 2^9 slots");
+static_assert(kFmMaxK <= 256, "block pairs are staged as two bytes");
 
 // slot of net e in the delta or -1
 __device__ inline int fm_dfind(const FmLds& s, uint32_t e) {
@@ -682,48 +683,69 @@ fm_recalc_kernel(FmArgs a, const uint32_t* __restrict__ nlist, const uint32_t* _
   }
 }
 
-// BalanceAndBestIndexScan (global_rollback.cpp:29-141) in sequential form
-// (one thread; the heaviest block weight is kept up to date and rescanned
-// only when the heaviest block loses weight): the prefix with the largest
-// gain sum and no block above its bound, ties to the lighter heaviest block.
-// res[0] = length, res[1] = gain.
-__global__ void fm_prefix_kernel(FmArgs a, const uint32_t* __restrict__ sv, const int32_t* __restrict__ sf,
-                                 const int32_t* __restrict__ st, const int64_t* __restrict__ gain, uint32_t nm,
-                                 int64_t* __restrict__ res) {
-  __shared__ int64_t w[kFmMaxK];
-  if (threadIdx.x != 0) return;
+// BalanceAndBestIndexScan (global_rollback.cpp:29-141) in sequential form:
+// one wave stages 1024 moves at a time in LDS (block pair, vertex weight,
+// gain), then one lane walks them, keeping the heaviest block weight up to
+// date (rescanned only when the heaviest block loses weight): the prefix with
+// the largest gain sum and no block above its bound, ties to the lighter
+// heaviest block.  res[0] = length, res[1] = gain.
+constexpr uint32_t kFmPrefixChunk = 1024;
+__global__ void __launch_bounds__(64)
+fm_prefix_kernel(FmArgs a, const uint32_t* __restrict__ sv, const int32_t* __restrict__ sf,
+                 const int32_t* __restrict__ st, const int64_t* __restrict__ gain, uint32_t nm,
+                 int64_t* __restrict__ res) {
+  __shared__ int64_t w[kFmMaxK], mw[kFmMaxK];
+  __shared__ int64_t c_gain[kFmPrefixChunk], c_w[kFmPrefixChunk];
+  __shared__ uint16_t c_ft[kFmPrefixChunk];
+  const int lane = threadIdx.x;
   int over = 0;
-  int64_t hv = INT64_MIN;
-  for (int b = 0; b < a.k; ++b) {
-    w[b] = a.part_w[b];
-    over += w[b] > a.max_w[b] ? 1 : 0;
-    hv = max(hv, w[b]);
-  }
-  int64_t bg = 0, bh = hv, sum = 0;
+  int64_t hv = INT64_MIN, bg = 0, bh = 0, sum = 0;
   uint32_t bi = 0;
-  for (uint32_t i = 0; i < nm; ++i) {
-    const int f = sf[i], t = st[i];
-    const int64_t wv = a.vw[sv[i]];
-    sum += gain[i];
-    const int64_t wf = w[f], wt = w[t];
-    if (wf > a.max_w[f] && wf - wv <= a.max_w[f]) --over;
-    if (wt <= a.max_w[t] && wt + wv > a.max_w[t]) ++over;
-    w[f] = wf - wv;
-    w[t] = wt + wv;
-    if (wt + wv > hv) {
-      hv = wt + wv;
-    } else if (wf == hv && wv > 0) {
-      hv = INT64_MIN;
-      for (int b = 0; b < a.k; ++b) hv = max(hv, w[b]);
+  if (lane == 0) {
+    for (int b = 0; b < a.k; ++b) {
+      w[b] = a.part_w[b];
+      mw[b] = a.max_w[b];
+      over += w[b] > mw[b] ? 1 : 0;
+      hv = max(hv, w[b]);
     }
-    if (over == 0 && sum >= bg && (sum > bg || hv < bh)) {
-      bg = sum;
-      bh = hv;
-      bi = i + 1;
+    bh = hv;
+  }
+  for (uint32_t c0 = 0; c0 < nm; c0 += kFmPrefixChunk) {
+    const uint32_t cn = min(kFmPrefixChunk, nm - c0);
+    __syncthreads();
+    for (uint32_t i = lane; i < cn; i += 64) {
+      c_gain[i] = gain[c0 + i];
+      c_w[i] = a.vw[sv[c0 + i]];
+      c_ft[i] = uint16_t((uint32_t(sf[c0 + i]) << 8) | uint32_t(st[c0 + i]));
+    }
+    __syncthreads();
+    if (lane != 0) continue;
+    for (uint32_t i = 0; i < cn; ++i) {
+      const int f = c_ft[i] >> 8, t = c_ft[i] & 0xff;
+      const int64_t wv = c_w[i];
+      sum += c_gain[i];
+      const int64_t wf = w[f], wt = w[t];
+      if (wf > mw[f] && wf - wv <= mw[f]) --over;
+      if (wt <= mw[t] && wt + wv > mw[t]) ++over;
+      w[f] = wf - wv;
+      w[t] = wt + wv;
+      if (wt + wv > hv) {
+        hv = wt + wv;
+      } else if (wf == hv && wv > 0) {
+        hv = INT64_MIN;
+        for (int b = 0; b < a.k; ++b) hv = max(hv, w[b]);
+      }
+      if (over == 0 && sum >= bg && (sum > bg || hv < bh)) {
+        bg = sum;
+        bh = hv;
+        bi = c0 + i + 1;
+      }
     }
   }
-  res[0] = bi;
-  res[1] = bg;
+  if (lane == 0) {
+    res[0] = bi;
+    res[1] = bg;
+  }
 }
 
 // applies moves [0, count) of the sequence: pin counts, connectivity,

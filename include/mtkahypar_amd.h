@@ -285,20 +285,30 @@ typedef struct {
   int32_t lp_max_rounds;           /* r-lp-maximum-iterations (5) */
   uint32_t lp_activation_threshold;/* r-lp-he-size-activation-threshold (100) */
   int32_t lp_rebalancing;          /* r-lp-rebalancing (1) */
-  int32_t ip_threads;              /* host threads of initial partitioning (4); the result does not depend on it */
+  int32_t ip_threads;              /* unused (initial partitioning runs on the device); kept for the layout */
   double large_net_remove_factor;  /* maxnet-remove-factor (0.01): nets larger than max(N * it, 100, 50000) are
                                       set aside during the multilevel phase */
   int32_t fm_rounds;               /* r-fm-multitry-rounds (10 with r-fm-type=fm_gain_cache); default 0: FM
                                       not run (it takes one to two orders of magnitude longer than LP) */
   int32_t fm_seeds;                /* r-fm-seed-nodes (25), at most 64 */
 } mkp_partition_params;
-/* Initial partitioning of p's hypergraph (the coarsest level) on the host:
- * a pool of greedy-growing and random starts, each improved by k-way km1
- * hill climbing, best balanced result (simplified stand-in for
- * recursive_bisection_initial_partitioner.cpp:475-492).  Block weights are
- * bounded by (1 + epsilon) * ceil(W / k) (context.cpp:237-250); p is
- * initialised with the result. */
+/* Initial partitioning of p's hypergraph (the coarsest level) on the device:
+ * recursive bisection (recursive_bisection_initial_partitioner.cpp:475-492)
+ * in which every bisection is the best balanced of `runs` runs made at once,
+ * one workgroup each (greedy growing, BFS or random fill by run index mod 3,
+ * then 2-way gain-move passes), in place of the flat pool of
+ * flat/pool_initial_partitioner.h:36-75.  Block weights are bounded by
+ * (1 + epsilon) * ceil(W / k) (context.cpp:237-250); p is initialised with
+ * the result. */
 MKP_API int mkp_initial_partition(mkp_partition* p, double epsilon, uint64_t seed, int32_t runs, int64_t* km1);
+/* The runs of one bisection of hg as mkp_initial_partition makes them, for
+ * inspection and tests: side limits max0 / max1, side-0 target weight
+ * target0, run stream key `stream`; sides receives runs x n side ids (host
+ * memory), results runs x {cut, balanced, side-0 weight, side-1 weight}.
+ * Diagnostic entry point with no reference counterpart (one flat run of the
+ * pool, flat/pool_initial_partitioner.h:36-75, is the closest). */
+MKP_API int mkp_ip_bisect_runs(mkp_hypergraph* hg, int64_t max0, int64_t max1, int64_t target0, uint64_t stream,
+                               int32_t runs, int32_t passes, uint8_t* sides, int64_t* results);
 /* Km1Rebalancer (rebalancer.cpp:36-262) on the host: empty blocks get a
  * vertex, then border vertices of overloaded blocks move to the fitting block
  * of best km1 gain (moves that do not worsen km1 first, the rest in
@@ -452,9 +462,8 @@ typedef struct {
   int32_t slots;              /* requests in flight at once (1..64) */
   uint64_t hbm_budget_bytes;  /* admission budget; 0 = device memory less 4 GiB */
   int32_t spin_wait;          /* 0 (default): slot threads sleep while their stream runs; 1: they spin */
-  int32_t ip_threads;         /* initial-partitioning threads per request; 0 = the request's own (4) */
-  int32_t host_threads;       /* with ip_threads 0: host threads shared by the requests in flight; a request
-                                 admitted beside r - 1 others gets max(its own, host_threads / r); 0 = off */
+  int32_t ip_threads;         /* unused (initial partitioning runs on the device); kept for the layout */
+  int32_t host_threads;       /* unused, as ip_threads */
 } mkp_server_params;
 typedef struct {
   int64_t km1;

@@ -299,6 +299,7 @@ enum Family : int {
   kFamRating,
   kFamContraction,
   kFamFm,
+  kFamInitial,
   kNumFamilies
 };
 

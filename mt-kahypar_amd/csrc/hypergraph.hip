@@ -254,7 +254,7 @@ int mkp_stream_copy(mkp_context* ctx, uint64_t bytes, int32_t reps, double* ms_p
 
 const char* mkp_profile_family_name(int family) {
   static const char* names[kNumFamilies] = {"pin_count_init", "lp_gain", "lp_apply", "louvain_move",
-                                            "louvain_contract", "rating", "contraction", "fm"};
+                                            "louvain_contract", "rating", "contraction", "fm", "initial"};
   return (family >= 0 && family < kNumFamilies) ? names[family] : "";
 }
 

@@ -133,6 +133,17 @@ void fm_refine(mkp_partition* p, const mkp_fm_params& prm, mkp_fm_stats* stats);
 void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, const uint32_t* nodes = nullptr,
                uint32_t num_nodes = 0, const LpBatch* batch = nullptr);
 
+// Initial partitioning on the device (recursive bisection, a portfolio of
+// `runs` runs per bisection): block ids of hg's vertices into part (device).
+void initial_partition(mkp_hypergraph* hg, int k, const std::vector<int64_t>& max_w, uint64_t seed, int runs,
+                       int passes, int32_t* part);
+// The runs of one bisection of hg (sides: runs x n on the host, results:
+// runs x {cut, balanced, side-0 weight, side-1 weight}).
+void ip_bisect_runs(mkp_hypergraph* hg, int64_t max0, int64_t max1, int64_t target0, uint64_t stream, int runs,
+                    int passes, uint8_t* sides, int64_t* results);
+void ip_side_limits(int64_t W, int k, double eps_level, int64_t* max0, int64_t* max1, int64_t* target0);
+uint64_t ip_stream(uint64_t seed, int first, int k);
+
 // Rejects partition parameters the pipeline cannot run (k, epsilon, preset
 // bounds, sub-round counts above 255) with an invalid-argument error.
 void validate_partition_params(const mkp_partition_params& p);

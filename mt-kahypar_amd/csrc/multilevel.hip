@@ -8,7 +8,6 @@
 #include <cmath>
 #include <memory>
 
-#include "../host/initial_partitioning.h"
 #include "../host/rebalancer.h"
 #include "internal.h"
 
@@ -182,17 +181,21 @@ int mkp_initial_partition(mkp_partition* p, double epsilon, uint64_t seed, int32
     const int k = p->k;
     const int64_t W = p->hg->total_w;
     const int64_t perfect = int64_t(std::ceil(double(W) / double(k)));
-    mt_kahypar_amd::InitialPartitioningParams ip;
-    ip.k = k;
-    ip.max_w.assign(k, int64_t(int32_t((1.0 + epsilon) * double(perfect))));
-    ip.seed = seed;
-    ip.runs = runs;
-    std::vector<int32_t> part = mt_kahypar_amd::initial_partition(download(p->hg), ip, nullptr);
-    if (!part.empty())
-      MKP_HIP(hipMemcpyAsync(p->part.p, part.data(), part.size() * 4, hipMemcpyHostToDevice, c.stream));
+    const std::vector<int64_t> max_w(k, int64_t(int32_t((1.0 + epsilon) * double(perfect))));
+    initial_partition(p->hg, k, max_w, seed, runs, 4, p->part.p);
     partition_initialize(p);
     c.sync();
     if (km1) *km1 = partition_km1(p);
+  });
+}
+
+int mkp_ip_bisect_runs(mkp_hypergraph* hg, int64_t max0, int64_t max1, int64_t target0, uint64_t stream,
+                       int32_t runs, int32_t passes, uint8_t* sides, int64_t* results) {
+  return guarded([&] {
+    MKP_CHECK(hg && sides && results, "null argument");
+    MKP_CHECK(runs >= 1 && runs <= 1024 && passes >= 0 && passes <= 64, "runs must be in [1, 1024], passes in [0, 64]");
+    Bind bind_(hg->ctx->c);
+    ip_bisect_runs(hg, max0, max1, target0, stream, runs, passes, sides, results);
   });
 }
 
@@ -287,26 +290,25 @@ int mkp_partition_hypergraph(mkp_hypergraph* hg, const mkp_partition_params* prm
     const double t2 = now_ms();
     const int L = h->num_levels();
     mkp_hypergraph* coarsest = h->level(L - 1);
-    mt_kahypar_amd::InitialPartitioningParams ip;
-    ip.k = k;
-    ip.max_w.assign(k, maxw);
-    ip.seed = prm->seed;
-    if (prm->ip_runs > 0) ip.runs = prm->ip_runs;
-    if (prm->ip_threads > 0) ip.threads = prm->ip_threads;
-    std::vector<int32_t> part = mt_kahypar_amd::initial_partition(download(coarsest), ip, nullptr);
-    const double t3 = now_ms();
     std::vector<int64_t> maxv(k, maxw), perv(k, perfect);
+    std::vector<std::unique_ptr<mkp_partition>> phg(L);
+    {
+      mkp_partition* p = nullptr;
+      MKP_CHECK(mkp_partition_create(coarsest, k, &p) == 0, mkp_last_error());
+      phg[L - 1].reset(p);
+      initial_partition(coarsest, k, maxv, prm->seed, prm->ip_runs > 0 ? prm->ip_runs : 20, 4, p->part.p);
+      c.sync();
+    }
+    const double t3 = now_ms();
     mkp_lp_params lpp{prm->lp_max_rounds, prm->lp_sub_rounds > 0 ? prm->lp_sub_rounds : 4,
                       prm->lp_activation_threshold, prm->lp_rebalancing, 0, maxv.data(), perv.data()};
-    std::vector<std::unique_ptr<mkp_partition>> phg(L);
     for (int i = L - 1; i >= 0; --i) {
-      mkp_partition* p = nullptr;
-      MKP_CHECK(mkp_partition_create(h->level(i), k, &p) == 0, mkp_last_error());
-      phg[i].reset(p);
+      mkp_partition* p = phg[i].get();
       if (i == L - 1) {
-        MKP_HIP(hipMemcpyAsync(p->part.p, part.data(), part.size() * 4, hipMemcpyHostToDevice, c.stream));
         partition_initialize(p);
       } else {
+        MKP_CHECK(mkp_partition_create(h->level(i), k, &p) == 0, mkp_last_error());
+        phg[i].reset(p);
         project(p, phg[i + 1].get(), h, i);
         phg[i + 1].reset();
       }

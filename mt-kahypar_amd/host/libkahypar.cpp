@@ -24,7 +24,6 @@ struct mt_kahypar_context_s {
 namespace {
 
 std::mutex g_lock;                 // partition() is serialised
-size_t g_host_threads = 0;         // mt_kahypar_initialize_thread_pool
 std::map<int, mkp_context*> g_ctx;  // one device context per GPU, kept until exit
 int g_return_on_error = 0;
 
@@ -135,11 +134,8 @@ void mt_kahypar_configure_context_from_file(mt_kahypar_context_t* c, const char*
   c->prm = p;
 }
 
-void mt_kahypar_initialize_thread_pool(const size_t num_threads, const bool) {
-  // The device path has no thread pool; the count bounds the host threads of
-  // initial partitioning.
-  std::lock_guard<std::mutex> guard(g_lock);
-  g_host_threads = num_threads;
+void mt_kahypar_initialize_thread_pool(const size_t, const bool) {
+  // Every phase runs on the device: there is no host thread pool to size.
 }
 
 void mt_kahypar_read_hypergraph_from_file(const char* file_name, mt_kahypar_hypernode_id_t* num_vertices,
@@ -206,7 +202,6 @@ void mt_kahypar_partition(const mt_kahypar_hypernode_id_t num_vertices, const mt
   prm.k = int32_t(num_blocks);
   prm.epsilon = epsilon;
   prm.seed = uint64_t(int64_t(seed));
-  if (g_host_threads > 0) prm.ip_threads = int32_t(std::min<size_t>(g_host_threads, 256));
   std::vector<int32_t> parts(std::max<uint32_t>(n, 1));
   mkp_partition_stats st{};
   mkp_hypergraph* hg = nullptr;

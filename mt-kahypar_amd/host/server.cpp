@@ -99,13 +99,6 @@ struct mkp_server {
       stats.max_reserved_bytes = std::max(stats.max_reserved_bytes, reserved);
       r->slot = s;
       r->t_start = Clock::now();
-      if (host_threads > 0 && ip_threads == 0) {
-        // a request admitted while few others run may spread its initial
-        // partitioning over more host threads (its result does not depend
-        // on the count)
-        const int own = r->prm.ip_threads > 0 ? r->prm.ip_threads : 4;
-        r->prm.ip_threads = std::max(own, host_threads / running);
-      }
       const bool more = !queue.empty();
       lk.unlock();
       if (more) cv_work.notify_one();  // the next request may fit as well
@@ -181,8 +174,8 @@ int mkp_server_create(const mkp_server_params* prm, mkp_server** out) {
         mkp_context* c = nullptr;
         MKP_CHECK(mkp_context_create(prm->device, &c) == 0, mkp_last_error());
         s->ctx.push_back(c);
-        // the slots share the host cores with the initial partitioning of
-        // other requests: their waits sleep unless asked to spin
+        // the slot threads share the host cores with each other and with
+        // the submitting threads: their waits sleep unless asked to spin
         c->c.sleep_wait = prm->spin_wait == 0;
       }
       s->budget = prm->hbm_budget_bytes;
@@ -231,7 +224,6 @@ int mkp_server_submit(mkp_server* s, const mkp_host_hypergraph* h, const mkp_par
     r->t_submit = Clock::now();
     r->h = h;
     r->prm = *prm;
-    if (s->ip_threads > 0) r->prm.ip_threads = s->ip_threads;
     uint32_t me = 0;
     for (uint32_t e = 0; e < h->h.m; ++e) me = std::max<uint32_t>(me, uint32_t(h->h.noff[e + 1] - h->h.noff[e]));
     MKP_CHECK(mkp_request_estimate(h->h.n, h->h.m, h->h.pins.size(), me, prm->k, &r->estimate) == 0,

@@ -167,6 +167,7 @@ SIGNATURES = {
     "mkp_large_net_threshold": (C.c_uint32, [C.c_uint32, C.c_double]),
     "mkp_partition_params_default": (C.c_int, [P(PartitionParams), C.c_int32, C.c_double, C.c_uint64]),
     "mkp_initial_partition": (C.c_int, [vp, C.c_double, C.c_uint64, C.c_int32, P(C.c_int64)]),
+    "mkp_ip_bisect_runs": (C.c_int, [vp, C.c_int64, C.c_int64, C.c_int64, C.c_uint64, C.c_int32, C.c_int32, vp, vp]),
     "mkp_generate_powerlaw": (C.c_int, [P(PowerlawParams), P(vp)]),
     "mkp_generate_rownet": (C.c_int, [P(RownetParams), P(vp)]),
     "mkp_host_hypergraph_info": (C.c_int, [vp, P(HypergraphInfo)]),
@@ -238,7 +239,7 @@ class Context:
     def synchronize(self):
         check(lib().mkp_context_synchronize(self.h))
 
-    FAMILIES = 8
+    FAMILIES = 9
 
     def set_profiling(self, on):
         check(lib().mkp_context_set_profiling(self.h, int(on)))
@@ -366,6 +367,15 @@ class Hypergraph:
     def set_communities(self, comm):
         c = np.ascontiguousarray(comm, dtype=np.int32)
         check(lib().mkp_hypergraph_set_communities(self.h, _p(c)))
+
+    def ip_bisect_runs(self, max0, max1, target0, stream, runs=20, passes=4):
+        """The runs of one initial-partitioning bisection: (sides runs x n,
+        results runs x [cut, balanced, side-0 weight, side-1 weight])."""
+        n = self.info().num_vertices
+        sides = np.zeros((runs, max(n, 1)), np.uint8)
+        res = np.zeros((runs, 4), np.int64)
+        check(lib().mkp_ip_bisect_runs(self.h, max0, max1, target0, stream, runs, passes, _p(sides), _p(res)))
+        return sides[:, :n], res
 
     UNIFORM, NON_UNIFORM, DEGREE, HYBRID = 0, 1, 2, -1
 
@@ -543,6 +553,12 @@ class Partition:
     def km1(self):
         v = C.c_int64()
         check(lib().mkp_partition_km1(self.h, C.byref(v)))
+        return v.value
+
+    def initial_partition(self, epsilon, seed, runs=20):
+        """Recursive-bisection initial partitioning on the device; returns km1."""
+        v = C.c_int64()
+        check(lib().mkp_initial_partition(self.h, epsilon, seed, runs, C.byref(v)))
         return v.value
 
     def rebalance(self, max_w):

@@ -3,7 +3,6 @@
 // plus the multilevel pipeline of partitioner.cpp:163-204 built from the
 // restated pieces.
 #include "oracle.h"
-#include "../mt-kahypar_amd/host/initial_partitioning.h"
 #include "../mt-kahypar_amd/host/rebalancer.h"
 
 #include <algorithm>
@@ -388,9 +387,9 @@ int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int t
   std::vector<Level> levels = coarsen(top, cp, sync != 0, &rating_visits);
   const double t2 = now();
   const Hypergraph& coarsest = levels.empty() ? top : levels.back().hg;
-  mt_kahypar_amd::InitialPartitioningParams ip;
-  ip.k = k; ip.max_w.assign(k, maxw); ip.seed = seed;
-  std::vector<int32_t> part = mt_kahypar_amd::initial_partition(to_host(coarsest), ip, nullptr);
+  IpParams ip;
+  ip.k = k; ip.max_w.assign(k, maxw); ip.seed = seed; ip.threads = std::max(1, threads);
+  std::vector<int32_t> part = initial_partition(coarsest, ip, nullptr);
   const double t3 = now();
   LpParams prm;
   prm.k = k; prm.max_w.assign(k, maxw); prm.perfect_w.assign(k, perfect); prm.seed = seed; prm.threads = threads;
@@ -456,6 +455,36 @@ int64_t orc_partition(void* h, int k, double eps, uint64_t seed, int sync, int t
     counters[6] = fm_improvement;
   }
   return km1(fin);
+}
+
+// ---------------------------------------------------- initial partitioning --
+int64_t orc_initial_partition(void* h, int k, const int64_t* max_w, uint64_t seed, int runs, int threads,
+                              int32_t* part_out) {
+  const Hypergraph& hg = *static_cast<Hypergraph*>(h);
+  IpParams ip;
+  ip.k = k; ip.max_w.assign(max_w, max_w + k); ip.seed = seed; ip.runs = runs; ip.threads = std::max(1, threads);
+  int64_t km1v = 0;
+  std::vector<int32_t> part = initial_partition(hg, ip, &km1v);
+  std::copy(part.begin(), part.end(), part_out);
+  return km1v;
+}
+// one run of a bisection: side per vertex, out = {cut, balanced, side-0 weight}
+void orc_ip_bisect_run(void* h, int64_t max0, int64_t max1, int64_t target0, uint64_t stream, int r, int passes,
+                       uint8_t* side_out, int64_t* out) {
+  const Hypergraph& hg = *static_cast<Hypergraph*>(h);
+  IpRun run = ip_bisect_run(hg, max0, max1, target0, stream, r, passes);
+  std::copy(run.side.begin(), run.side.end(), side_out);
+  out[0] = run.cut; out[1] = run.balanced; out[2] = run.w0;
+}
+void orc_ip_perm(uint64_t key, uint32_t n, uint32_t* out) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = ip_perm(key, n, i);
+}
+uint64_t orc_ip_stream(uint64_t seed, int first, int k) { return ip_stream(seed, first, k); }
+uint64_t orc_ip_run_key(uint64_t stream, int r) { return ip_run_key(stream, r); }
+uint32_t orc_ip_start(uint64_t rk, uint32_t n) { return ip_start(rk, n); }
+int64_t orc_ip_move_check(void* h, const uint8_t* side, const uint32_t* moves, uint32_t count) {
+  const Hypergraph& hg = *static_cast<Hypergraph*>(h);
+  return ip_move_check(hg, std::vector<uint8_t>(side, side + hg.n), std::vector<uint32_t>(moves, moves + count));
 }
 
 }  // extern "C"

@@ -1,0 +1,461 @@
+// initial.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// Restatement of the device initial partitioner (mt-kahypar_amd/csrc/
+// initial.hip): recursive bisection of the coarsest hypergraph, every
+// bisection the best of a portfolio of runs.  It takes the place of the
+// reference's RecursiveBisectionInitialPartitioner
+// (recursive_bisection_initial_partitioner.cpp:475-492) with its flat pool
+// (flat/pool_initial_partitioner.h:36-75: BFS, greedy growing, random, LP,
+// each improved by 2-way FM); the pool is not reproduced, the recursion is:
+//   * k is split into ceil(k/2) and floor(k/2) blocks; side 0 targets the
+//     weight share of its blocks, both sides allow (1 + eps') times their
+//     share with the adaptive eps' = (1 + eps)^(1 / ceil(log2 k)) - 1
+//     (the reference's adaptive imbalance for recursive bisection);
+//   * cut nets are split between the halves (the km1 recursion), nets left
+//     with fewer than two pins are dropped;
+//   * a final pass moves the lightest vertices of an overloaded block into
+//     the lightest block they fit.
+// Every rule is stated in integers and stateless hashes so that one workgroup
+// per run on the device computes exactly the same sides:
+//   * run r of a bisection uses variant r % 3: greedy growing, BFS, random
+//     fill; all `runs` runs are made and the balanced run with the smallest
+//     cut wins (ties: the smaller run index);
+//   * greedy growing scores a vertex by the sum over its nets of at most 100
+//     pins of floor(w(e) * 2^16 / (|e| - 1)) over the nets shared with side 0
+//     (64-bit wrap-around sums, so the order of the additions is free) and
+//     takes the highest score, ties to the smaller id;
+//   * BFS is level-synchronous: a level's vertices join side 0 in ascending
+//     id order, the next level is every unseen pin of an unscanned net of at
+//     most 100 pins incident to a vertex that joined;
+//   * random fill and the refinement passes visit vertices in the order of a
+//     keyed Feistel permutation of [0, n) (four rounds, cycle walking);
+//   * refinement: balance repair (overloaded side's vertices in descending
+//     gain order at the time of the repair, ties to the smaller id), then
+//     passes of moves with positive gain, or zero gain towards the lighter
+//     side, until a pass makes no positive-gain move.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <numeric>
+#include <thread>
+
+#include "oracle.h"
+
+namespace orc {
+
+namespace {
+
+constexpr uint64_t kGrowNetLimit = 100;  // larger nets do not steer growing
+constexpr int kScoreFrac = 16;           // fractional bits of a growing score
+
+struct Sub {
+  Hypergraph hg;
+  std::vector<uint32_t> orig;  // id in the coarsest hypergraph
+  int64_t W = 0;
+};
+
+struct Bisection {
+  const Hypergraph& h;
+  int64_t max_w[2];
+  int64_t target0;
+  std::vector<uint8_t> side;
+  std::vector<uint32_t> pc[2];
+  int64_t w[2] = {0, 0};
+  std::vector<int64_t> gc;
+
+  Bisection(const Hypergraph& hg, int64_t m0, int64_t m1, int64_t t0) : h(hg), max_w{m0, m1}, target0(t0) {}
+
+  void grow_greedy(uint64_t rk) {
+    const uint32_t n = h.n;
+    side.assign(n, 1);
+    std::vector<uint8_t> skip(n, 0), in(n, 0);
+    std::vector<uint64_t> key(n, 0);
+    const uint32_t start = ip_start(rk, n);
+    uint32_t scan = 0;
+    int64_t w0 = 0;
+    while (w0 < target0) {
+      int64_t v = -1;
+      for (uint32_t x = 0; x < n; ++x)  // highest score, ties to the smaller id
+        if (in[x] && (v < 0 || key[x] > key[v])) v = x;
+      while (v < 0 && scan < n) {
+        const uint32_t x = (start + scan++) % n;
+        if (side[x] == 1 && !skip[x]) v = x;
+      }
+      if (v < 0) break;
+      in[v] = 0;
+      if (w0 + h.vw[v] > max_w[0]) {
+        skip[v] = 1;
+        continue;
+      }
+      side[v] = 0;
+      w0 += h.vw[v];
+      for (uint64_t i = h.voff[v]; i < h.voff[v + 1]; ++i) {
+        const uint32_t e = h.inc[i];
+        const uint64_t sz = h.noff[e + 1] - h.noff[e];
+        if (sz > kGrowNetLimit) continue;
+        const uint64_t add = (uint64_t(uint32_t(h.nw[e])) << kScoreFrac) / std::max<uint64_t>(sz - 1, 1);
+        for (uint64_t j = h.noff[e]; j < h.noff[e + 1]; ++j) {
+          const uint32_t x = h.pins[j];
+          if (side[x] == 1 && !skip[x]) {
+            key[x] += add;
+            in[x] = 1;
+          }
+        }
+      }
+    }
+  }
+
+  void grow_bfs(uint64_t rk) {
+    const uint32_t n = h.n;
+    side.assign(n, 1);
+    std::vector<uint8_t> seen(n, 0), scanned(h.m, 0);
+    const uint32_t start = ip_start(rk, n);
+    uint32_t scan = 0;
+    int64_t w0 = 0;
+    std::vector<uint32_t> frontier, added;
+    while (w0 < target0) {
+      if (frontier.empty()) {
+        while (scan < n && seen[(start + scan) % n]) ++scan;
+        if (scan >= n) break;
+        const uint32_t s = (start + scan) % n;
+        seen[s] = 1;
+        frontier.push_back(s);
+      }
+      added.clear();
+      for (uint32_t v : frontier) {  // ascending ids
+        if (w0 >= target0) break;
+        if (w0 + h.vw[v] > max_w[0]) continue;
+        side[v] = 0;
+        w0 += h.vw[v];
+        added.push_back(v);
+      }
+      if (w0 >= target0) break;
+      std::vector<uint8_t> next(n, 0);
+      for (uint32_t v : added)
+        for (uint64_t i = h.voff[v]; i < h.voff[v + 1]; ++i) {
+          const uint32_t e = h.inc[i];
+          if (scanned[e] || h.noff[e + 1] - h.noff[e] > kGrowNetLimit) continue;
+          scanned[e] = 1;
+          for (uint64_t j = h.noff[e]; j < h.noff[e + 1]; ++j)
+            if (!seen[h.pins[j]]) {
+              seen[h.pins[j]] = 1;
+              next[h.pins[j]] = 1;
+            }
+        }
+      frontier.clear();
+      for (uint32_t x = 0; x < n; ++x)
+        if (next[x]) frontier.push_back(x);
+    }
+  }
+
+  void random_fill(uint64_t rk) {
+    const uint32_t n = h.n;
+    side.assign(n, 1);
+    const uint64_t key = mix64(rk ^ kIpFillTag);
+    int64_t w0 = 0;
+    for (uint32_t i = 0; i < n && w0 < target0; ++i) {
+      const uint32_t u = ip_perm(key, n, i);
+      if (w0 + h.vw[u] > max_w[0]) continue;
+      side[u] = 0;
+      w0 += h.vw[u];
+    }
+  }
+
+  void counts() {
+    pc[0].assign(h.m, 0);
+    pc[1].assign(h.m, 0);
+    w[0] = w[1] = 0;
+    for (uint32_t e = 0; e < h.m; ++e)
+      for (uint64_t i = h.noff[e]; i < h.noff[e + 1]; ++i) pc[side[h.pins[i]]][e]++;
+    for (uint32_t u = 0; u < h.n; ++u) w[side[u]] += h.vw[u];
+  }
+  // cut reduction of moving u to the other side
+  int64_t gain(uint32_t u) const {
+    const int f = side[u], t = 1 - f;
+    int64_t g = 0;
+    for (uint64_t i = h.voff[u]; i < h.voff[u + 1]; ++i) {
+      const uint32_t e = h.inc[i];
+      if (pc[f][e] == 1) g += h.nw[e];
+      if (pc[t][e] == 0) g -= h.nw[e];
+    }
+    return g;
+  }
+  // moves v and keeps every gain current: with a, b the pin counts of a net
+  // on v's old and new side before the move, a pin on the old side gains
+  // w * ([a == 2] - [a == 1] + [b == 0]), a pin on the new side
+  // w * ([b == 0] - [a == 1] - [b == 1]), and v's own gain is negated
+  void move(uint32_t v) {
+    const int f = side[v], t = 1 - f;
+    const int64_t gv = gc[v];
+    for (uint64_t i = h.voff[v]; i < h.voff[v + 1]; ++i) {
+      const uint32_t e = h.inc[i];
+      const uint32_t a = pc[f][e], b = pc[t][e];
+      const int64_t we = h.nw[e];
+      const int64_t df = we * ((a == 2) - (a == 1) + (b == 0));
+      const int64_t dt = we * ((b == 0) - (a == 1) - (b == 1));
+      if (df || dt)
+        for (uint64_t j = h.noff[e]; j < h.noff[e + 1]; ++j) {
+          const uint32_t x = h.pins[j];
+          if (x != v) gc[x] += side[x] == f ? df : dt;
+        }
+      pc[f][e] = a - 1;
+      pc[t][e] = b + 1;
+    }
+    side[v] = uint8_t(t);
+    w[f] -= h.vw[v];
+    w[t] += h.vw[v];
+    gc[v] = -gv;
+  }
+
+  void refine(int passes, uint64_t rk) {
+    counts();
+    gc.assign(h.n, 0);
+    for (uint32_t u = 0; u < h.n; ++u) gc[u] = gain(u);
+    for (int f = 0; f < 2; ++f) {
+      if (w[f] <= max_w[f]) continue;
+      std::vector<std::pair<int64_t, uint32_t>> cand;
+      for (uint32_t u = 0; u < h.n; ++u)
+        if (side[u] == f) cand.push_back({-gc[u], u});
+      std::sort(cand.begin(), cand.end());
+      for (const auto& c : cand) {
+        if (w[f] <= max_w[f]) break;
+        if (w[1 - f] + h.vw[c.second] <= max_w[1 - f]) move(c.second);
+      }
+    }
+    for (int p = 0; p < passes; ++p) {
+      const uint64_t key = ip_pass_key(rk, p);
+      bool improved = false;
+      for (uint32_t i = 0; i < h.n; ++i) {
+        const uint32_t u = ip_perm(key, h.n, i);
+        const int f = side[u], t = 1 - f;
+        const int64_t vu = h.vw[u];
+        if (w[t] + vu > max_w[t]) continue;
+        const int64_t g = gc[u];
+        if (g > 0 || (g == 0 && w[f] - vu > w[t] + vu)) {
+          move(u);
+          improved |= g > 0;
+        }
+      }
+      if (!improved) break;
+    }
+  }
+
+  int64_t cut() const {
+    int64_t c = 0;
+    for (uint32_t e = 0; e < h.m; ++e) c += (pc[0][e] && pc[1][e]) ? h.nw[e] : 0;
+    return c;
+  }
+};
+
+// the vertices of side s with the pins of every net on that side (nets left
+// with fewer than two pins are dropped); ids keep their relative order
+Sub extract(const Sub& s, const std::vector<uint8_t>& side, int which) {
+  const Hypergraph& h = s.hg;
+  Sub o;
+  std::vector<uint32_t> id(h.n, UINT32_MAX);
+  std::vector<int32_t> vw, nw;
+  uint32_t n = 0;
+  for (uint32_t u = 0; u < h.n; ++u)
+    if (side[u] == which) {
+      id[u] = n++;
+      vw.push_back(h.vw[u]);
+      o.orig.push_back(s.orig[u]);
+    }
+  std::vector<uint64_t> noff(1, 0);
+  std::vector<uint32_t> pins;
+  for (uint32_t e = 0; e < h.m; ++e) {
+    const size_t before = pins.size();
+    for (uint64_t i = h.noff[e]; i < h.noff[e + 1]; ++i)
+      if (side[h.pins[i]] == which) pins.push_back(id[h.pins[i]]);
+    if (pins.size() - before < 2) {
+      pins.resize(before);
+      continue;
+    }
+    noff.push_back(pins.size());
+    nw.push_back(h.nw[e]);
+  }
+  o.hg = build_hypergraph(n, uint32_t(nw.size()), noff.data(), pins.data(), vw.data(), nw.data());
+  o.W = 0;
+  for (int32_t x : vw) o.W += x;
+  return o;
+}
+
+struct Recursion {
+  const IpParams& prm;
+  double eps_level;
+  std::vector<int32_t>& part;
+
+  void run(const Sub& s, int k, int first) {
+    const Hypergraph& h = s.hg;
+    if (h.n == 0) return;
+    if (k == 1) {
+      for (uint32_t u = 0; u < h.n; ++u) part[s.orig[u]] = first;
+      return;
+    }
+    const int k0 = (k + 1) / 2, k1 = k - k0;
+    int64_t max0, max1, target0;
+    ip_side_limits(s.W, k, eps_level, &max0, &max1, &target0);
+    const uint64_t stream = ip_stream(prm.seed, first, k);
+    const int runs = std::max(1, prm.runs);
+    std::vector<IpRun> res(runs);
+    const int t = std::max(1, std::min(prm.threads, runs));
+    std::atomic<int> next{0};
+    auto work = [&] {
+      for (int r = next++; r < runs; r = next++)
+        res[r] = ip_bisect_run(h, max0, max1, target0, stream, r, prm.passes);
+    };
+    if (t == 1 || h.P() < 20000) {
+      work();
+    } else {
+      std::vector<std::thread> pool;
+      for (int i = 0; i < t; ++i) pool.emplace_back(work);
+      for (auto& th : pool) th.join();
+    }
+    const int best = ip_best_run(res);
+    Sub a = extract(s, res[best].side, 0), b = extract(s, res[best].side, 1);
+    res.clear();
+    run(a, k0, first);
+    run(b, k1, first + k0);
+  }
+};
+
+}  // namespace
+
+// Feistel permutation of [0, 2^(2h)) with 2^(2h) >= n, h-bit halves and four
+// rounds of mix64 keyed by (key, round); values >= n walk the cycle on.
+uint32_t ip_perm(uint64_t key, uint32_t n, uint32_t i) {
+  int b = 1;
+  while ((uint64_t(1) << b) < n) ++b;
+  const int hb = (b + 1) / 2;
+  const uint32_t mask = (1u << hb) - 1u;
+  uint32_t x = i;
+  do {
+    uint32_t L = x >> hb, R = x & mask;
+    for (uint32_t r = 0; r < 4; ++r) {
+      const uint32_t t = L ^ (uint32_t(mix64(key ^ (uint64_t(r) << 32) ^ R)) & mask);
+      L = R;
+      R = t;
+    }
+    x = (L << hb) | R;
+  } while (x >= n);
+  return x;
+}
+
+uint64_t ip_stream(uint64_t seed, int first, int k) {
+  return mix64(seed ^ mix64((uint64_t(uint32_t(first)) << 32) | uint32_t(k)));
+}
+uint64_t ip_run_key(uint64_t stream, int r) { return mix64(stream ^ mix64(uint64_t(r) + 1)); }
+uint32_t ip_start(uint64_t rk, uint32_t n) { return uint32_t(mix64(rk ^ kIpStartTag) % n); }
+uint64_t ip_pass_key(uint64_t rk, int p) { return mix64(rk ^ mix64(kIpPassTag + uint64_t(p))); }
+
+void ip_side_limits(int64_t W, int k, double eps_level, int64_t* max0, int64_t* max1, int64_t* target0) {
+  const int k0 = (k + 1) / 2, k1 = k - k0;
+  const double share = double(W) / double(k);
+  *max0 = int64_t((1.0 + eps_level) * std::ceil(share * k0));
+  *max1 = int64_t((1.0 + eps_level) * std::ceil(share * k1));
+  *target0 = int64_t(std::ceil(share * k0));
+}
+
+int ip_best_run(const std::vector<IpRun>& res) {
+  int best = 0;
+  for (int r = 1; r < int(res.size()); ++r) {
+    const IpRun &a = res[r], &b = res[best];
+    if ((a.balanced && !b.balanced) || (a.balanced == b.balanced && a.cut < b.cut)) best = r;
+  }
+  return best;
+}
+
+IpRun ip_bisect_run(const Hypergraph& h, int64_t max0, int64_t max1, int64_t target0, uint64_t stream, int r,
+                    int passes) {
+  Bisection b(h, max0, max1, target0);
+  const uint64_t rk = ip_run_key(stream, r);
+  if (r % 3 == 0) b.grow_greedy(rk);
+  else if (r % 3 == 1) b.grow_bfs(rk);
+  else b.random_fill(rk);
+  b.refine(passes, rk);
+  IpRun out;
+  out.cut = b.cut();
+  out.balanced = b.w[0] <= max0 && b.w[1] <= max1;
+  out.w0 = b.w[0];
+  out.side = std::move(b.side);
+  return out;
+}
+
+int64_t ip_move_check(const Hypergraph& h, const std::vector<uint8_t>& side, const std::vector<uint32_t>& moves) {
+  Bisection b(h, 0, 0, 0);
+  b.side = side;
+  b.counts();
+  b.gc.assign(h.n, 0);
+  for (uint32_t u = 0; u < h.n; ++u) b.gc[u] = b.gain(u);
+  int64_t bad = 0;
+  for (uint32_t v : moves) {
+    b.move(v);
+    Bisection fresh(h, 0, 0, 0);
+    fresh.side = b.side;
+    fresh.counts();
+    for (uint32_t e = 0; e < h.m; ++e) bad += fresh.pc[0][e] != b.pc[0][e] || fresh.pc[1][e] != b.pc[1][e];
+    for (uint32_t u = 0; u < h.n; ++u) bad += fresh.gain(u) != b.gc[u];
+    bad += fresh.w[0] != b.w[0] || fresh.w[1] != b.w[1];
+  }
+  return bad;
+}
+
+double ip_eps_level(int k, double eps) {
+  int levels = 0;
+  while ((1 << levels) < k) ++levels;
+  return levels > 0 ? std::pow(1.0 + std::max(eps, 0.0), 1.0 / levels) - 1.0 : eps;
+}
+
+void ip_repair_balance(const Hypergraph& hg, int k, const std::vector<int64_t>& max_w, std::vector<int32_t>& part) {
+  std::vector<int64_t> pw(k, 0);
+  for (uint32_t u = 0; u < hg.n; ++u) pw[part[u]] += hg.vw[u];
+  for (int b = 0; b < k; ++b) {
+    if (pw[b] <= max_w[b]) continue;
+    std::vector<std::pair<int32_t, uint32_t>> vs;
+    for (uint32_t u = 0; u < hg.n; ++u)
+      if (part[u] == b) vs.push_back({hg.vw[u], u});
+    std::sort(vs.begin(), vs.end());
+    for (const auto& x : vs) {
+      if (pw[b] <= max_w[b]) break;
+      int t = -1;
+      for (int c = 0; c < k; ++c)
+        if (c != b && pw[c] + x.first <= max_w[c] && (t < 0 || pw[c] < pw[t])) t = c;
+      if (t < 0) continue;
+      part[x.second] = t;
+      pw[b] -= x.first;
+      pw[t] += x.first;
+    }
+  }
+}
+
+std::vector<int32_t> initial_partition(const Hypergraph& hg, const IpParams& prm, int64_t* km1_out) {
+  const int k = prm.k;
+  std::vector<int32_t> part(hg.n, 0);
+  Sub root;
+  root.hg = hg;
+  root.orig.resize(hg.n);
+  std::iota(root.orig.begin(), root.orig.end(), 0u);
+  for (int32_t x : hg.vw) root.W += x;
+  const int64_t perfect = (root.W + k - 1) / std::max(k, 1);
+  double eps = prm.epsilon;
+  if (eps < 0) eps = perfect > 0 ? double(prm.max_w[0]) / double(perfect) - 1.0 : 0.0;
+  Recursion rec{prm, ip_eps_level(k, eps), part};
+  rec.run(root, k, 0);
+  ip_repair_balance(hg, k, prm.max_w, part);
+  if (km1_out) {
+    std::vector<uint32_t> stamp(k, UINT32_MAX);
+    int64_t s = 0;
+    for (uint32_t e = 0; e < hg.m; ++e) {
+      int l = 0;
+      for (uint64_t i = hg.noff[e]; i < hg.noff[e + 1]; ++i) {
+        const int b = part[hg.pins[i]];
+        if (stamp[b] != e) { stamp[b] = e; ++l; }
+      }
+      s += int64_t(std::max(l - 1, 0)) * hg.nw[e];
+    }
+    *km1_out = s;
+  }
+  return part;
+}
+
+}  // namespace orc

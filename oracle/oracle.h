@@ -264,4 +264,43 @@ Hypergraph reduce_hypergraph(const Hypergraph& hg, uint32_t contraction_limit, u
 void restore_degree_zero(const std::vector<uint32_t>& removed, const std::vector<int32_t>& weight, int k,
                          std::vector<int64_t>& part_w, std::vector<int32_t>& part);
 
+// ---------------------------------------------------- initial partitioning --
+// initial.cpp: recursive bisection with a portfolio of runs per bisection
+// (greedy growing, level-synchronous BFS, random fill, each refined by 2-way
+// gain moves), the form the device computes.
+struct IpParams {
+  int k = 2;
+  std::vector<int64_t> max_w;  // per block
+  uint64_t seed = 0;
+  double epsilon = -1;         // < 0: max_w[0] / ceil(W / k) - 1
+  int runs = 20;               // fast_preset.ini:29 (i-runs), per bisection
+  int passes = 4;              // refinement passes per run
+  int threads = 4;             // host threads for the runs (result independent of it)
+};
+struct IpRun {
+  std::vector<uint8_t> side;
+  int64_t cut = 0;
+  bool balanced = false;
+  int64_t w0 = 0;
+};
+constexpr uint64_t kIpStartTag = 0x243F6A8885A308D3ull;
+constexpr uint64_t kIpFillTag = 0x13198A2E03707344ull;
+constexpr uint64_t kIpPassTag = 0xA4093822299F31D0ull;
+uint32_t ip_perm(uint64_t key, uint32_t n, uint32_t i);
+uint64_t ip_stream(uint64_t seed, int first, int k);
+uint64_t ip_run_key(uint64_t stream, int r);
+uint32_t ip_start(uint64_t rk, uint32_t n);
+uint64_t ip_pass_key(uint64_t rk, int p);
+double ip_eps_level(int k, double eps);
+void ip_side_limits(int64_t W, int k, double eps_level, int64_t* max0, int64_t* max1, int64_t* target0);
+int ip_best_run(const std::vector<IpRun>& res);
+IpRun ip_bisect_run(const Hypergraph& h, int64_t max0, int64_t max1, int64_t target0, uint64_t stream, int r,
+                    int passes);
+// applies moves to the bisection `side` with incremental gain upkeep and
+// counts, after every move, the pin counts and gains that differ from a
+// recount
+int64_t ip_move_check(const Hypergraph& h, const std::vector<uint8_t>& side, const std::vector<uint32_t>& moves);
+void ip_repair_balance(const Hypergraph& hg, int k, const std::vector<int64_t>& max_w, std::vector<int32_t>& part);
+std::vector<int32_t> initial_partition(const Hypergraph& hg, const IpParams& prm, int64_t* km1_out);
+
 }  // namespace orc

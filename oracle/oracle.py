@@ -76,6 +76,14 @@ def lib():
             "orc_rebalance": (C.c_int64, [vp, C.c_int, i32p, i64p, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]),
             "orc_large_net_threshold": (C.c_uint32, [C.c_uint32, C.c_double]),
             "orc_restore_degree_zero": (None, [C.c_uint32, u32p, C.c_uint32, i32p, C.c_int, i64p, i32p]),
+            "orc_initial_partition": (C.c_int64, [vp, C.c_int, i64p, C.c_uint64, C.c_int, C.c_int, i32p]),
+            "orc_ip_bisect_run": (None, [vp, C.c_int64, C.c_int64, C.c_int64, C.c_uint64, C.c_int, C.c_int, u8p,
+                                         i64p]),
+            "orc_ip_perm": (None, [C.c_uint64, C.c_uint32, u32p]),
+            "orc_ip_stream": (C.c_uint64, [C.c_uint64, C.c_int, C.c_int]),
+            "orc_ip_run_key": (C.c_uint64, [C.c_uint64, C.c_int]),
+            "orc_ip_start": (C.c_uint32, [C.c_uint64, C.c_uint32]),
+            "orc_ip_move_check": (C.c_int64, [vp, u8p, u32p, C.c_uint32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -261,6 +269,30 @@ class Hypergraph:
         h = lib().orc_reduce(self.h, contraction_limit, net_threshold, vmap, rem)
         return Hypergraph(0, None, None, handle=h), vmap, int(rem[0]), int(rem[1])
 
+    def initial_partition(self, k, max_w, seed, runs=20, threads=4):
+        """Recursive-bisection initial partitioning (the device's form):
+        (block per vertex, km1)."""
+        n = self.sizes()[0]
+        part = np.zeros(max(n, 1), np.int32)
+        km1 = lib().orc_initial_partition(self.h, k, np.ascontiguousarray(max_w, np.int64), seed, runs, threads,
+                                          part)
+        return part[:n], int(km1)
+
+    def ip_bisect_run(self, max0, max1, target0, stream, r, passes=4):
+        """One run of one bisection: (side per vertex, cut, balanced, side-0 weight)."""
+        n = self.sizes()[0]
+        side = np.zeros(max(n, 1), np.uint8)
+        out = np.zeros(3, np.int64)
+        lib().orc_ip_bisect_run(self.h, max0, max1, target0, stream, r, passes, side, out)
+        return side[:n], int(out[0]), bool(out[1]), int(out[2])
+
+    def ip_move_check(self, side, moves):
+        """Pin counts and gains that differ from a recount after each of the
+        moves, applied with the initial partitioner's incremental upkeep."""
+        sd = np.ascontiguousarray(side, np.uint8)
+        mv = np.ascontiguousarray(moves, np.uint32)
+        return int(lib().orc_ip_move_check(self.h, sd, mv, len(mv)))
+
     def partition(self, k, eps, seed, sync, threads=1, fm_rounds=0):
         """The fast-preset pipeline (LP at every level, then FM when
         fm_rounds > 0), with the device pipeline's defaults."""
@@ -342,3 +374,23 @@ class Graph:
         st = np.zeros(4, np.int64)
         changed = lib().orc_louvain_level_sync(self.h, seed, max_passes, min_frac, sampling, sub_rounds, out, st)
         return out, bool(changed), dict(arcs=int(st[0]), rounds=int(st[1]), moves=int(st[3]))
+
+
+def ip_perm(key, n):
+    """The keyed Feistel permutation of [0, n) the initial partitioner visits
+    vertices in."""
+    out = np.zeros(max(n, 1), np.uint32)
+    lib().orc_ip_perm(key, n, out)
+    return out[:n]
+
+
+def ip_stream(seed, first, k):
+    return int(lib().orc_ip_stream(seed, first, k))
+
+
+def ip_run_key(stream, r):
+    return int(lib().orc_ip_run_key(stream, r))
+
+
+def ip_start(run_key, n):
+    return int(lib().orc_ip_start(run_key, n))

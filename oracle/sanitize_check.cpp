@@ -1,6 +1,6 @@
 // Sanitizer driver (test infrastructure): runs the host code that the product
-// shares with the oracle -- initial partitioning (with its worker threads),
-// the km1 rebalancer -- and the oracle's pipelines in both modes (OpenMP
+// shares with the oracle -- the km1 rebalancer -- the oracle's initial
+// partitioning with its worker threads, and the oracle's pipelines in both modes (OpenMP
 // threads in reference mode) on seeded random hypergraphs, so that a build
 // with -fsanitize=address,undefined or -fsanitize=thread reports memory
 // errors, undefined behaviour and data races of that code.  Prints "ok" and
@@ -16,7 +16,7 @@
 #include <string>
 #include <vector>
 
-#include "../mt-kahypar_amd/host/initial_partitioning.h"
+#include "oracle.h"
 #include "../mt-kahypar_amd/host/rebalancer.h"
 
 extern "C" {
@@ -71,22 +71,23 @@ bool balanced(const mt_kahypar_amd::HostHypergraph& h, int k, const std::vector<
 int main(int argc, char** argv) {
   const bool threads_only = argc > 1 && std::string(argv[1]) == "threads";
   int failures = 0;
-  // initial partitioning with worker threads: bisections of more than
-  // 100,000 pins run their starts on threads, halves of more than 50,000
-  // pins recurse on helper threads
+  // initial partitioning with worker threads: bisections of at least
+  // 20,000 pins run their runs on threads
   for (int t = 0; t < 3; ++t) {
     const int k = t == 0 ? 2 : (t == 1 ? 8 : 32);
     auto h = random_hypergraph(6000, 30000, 60, t == 2, 11 + t);
     int64_t W = 0;
     for (int32_t w : h.vw) W += w;
     const int64_t perfect = (W + k - 1) / k, maxw = int64_t(1.03 * double(perfect));
-    mt_kahypar_amd::InitialPartitioningParams ip;
+    orc::IpParams ip;
     ip.k = k;
     ip.max_w.assign(k, maxw);
     ip.seed = 5 + t;
     ip.threads = 8;
     int64_t km1 = 0;
-    std::vector<int32_t> part = mt_kahypar_amd::initial_partition(h, ip, &km1);
+    const orc::Hypergraph oh = orc::build_hypergraph(h.n, h.m, h.noff.data(), h.pins.data(), h.vw.data(),
+                                                     h.nw.data());
+    std::vector<int32_t> part = orc::initial_partition(oh, ip, &km1);
     if (!balanced(h, k, part, maxw)) { std::printf("initial partition over its bound (k=%d)\n", k); ++failures; }
     // rebalancer on a partition pushed out of balance
     for (uint32_t u = 0; u < h.n / 4; ++u) part[u] = 0;

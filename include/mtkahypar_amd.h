@@ -484,8 +484,9 @@ typedef struct {
   uint64_t trims;              /* allocation caches released for admission */
 } mkp_server_stats;
 /* Device bytes a request needs at most (hypergraph, community detection,
- * coarsening hierarchy, partition state and refinement work, rounded to the
- * allocation cache's power-of-two size classes). */
+ * coarsening hierarchy, initial partitioning with 20 runs per bisection,
+ * partition state and refinement work, rounded to the allocation cache's
+ * power-of-two size classes). */
 MKP_API int mkp_request_estimate(uint32_t num_vertices, uint32_t num_nets, uint64_t num_pins, uint32_t max_net_size,
                                  int32_t k, uint64_t* bytes);
 MKP_API int mkp_server_create(const mkp_server_params* params, mkp_server** out);

@@ -6,8 +6,8 @@
 // Every bisection makes all of its runs at once, one workgroup of 1024
 // threads per run, and keeps the balanced run with the smallest cut (ties to
 // the smaller run index).  Run r grows side 0 by variant r % 3 (greedy
-// growing, level-synchronous BFS, random fill) and then improves the
-// bisection with passes of 2-way gain moves.
+// growing, level-synchronous BFS, random fill; r % 2 above 2048 vertices)
+// and then improves the bisection with passes of 2-way gain moves.
 //
 // The steps of a run are sequential -- growing one vertex at a time and
 // moving one vertex at a time is what makes them effective -- and the work
@@ -15,13 +15,16 @@
 // vertices, the next positive-gain vertex of a pass) is spread over the
 // workgroup.  Growing scores and gains are integers updated by atomics, so a
 // run does not depend on the order of the updates.  For up to kIpLdsN
-// vertices the per-vertex state (side, flags, score or gain) lives in LDS.
+// vertices the per-vertex state (side, flags, score or gain) lives in LDS;
+// for up to kIpAdjN vertices the scores a joining vertex adds are one
+// precomputed row of clique scores.
 //
 // The rules, each stated so that the result is one function of the input:
 //   * greedy growing: score of a side-1 vertex = sum over its nets of at most
 //     100 pins shared with side 0 of floor(w(e) * 2^16 / (|e| - 1)) (64-bit
-//     wrap-around); the highest score joins, ties to the smaller id; with no
-//     scored vertex the next free vertex after a hashed start vertex seeds;
+//     wrap-around); the min(256, max(1, n / 2048)) highest scores join in
+//     order, ties to the smaller id; with no scored vertex the next free
+//     vertex after a hashed start vertex seeds;
 //   * BFS: a level's vertices join in ascending id order; the next level is
 //     every unseen pin of an unscanned net of at most 100 pins incident to a
 //     vertex that joined;
@@ -47,9 +50,14 @@ constexpr int kIpThreads = 1024;
 constexpr int kIpWaves = kIpThreads / 64;
 constexpr uint32_t kIpGrowNetLimit = 100;  // larger nets do not steer growing
 constexpr int kIpScoreFrac = 16;           // fractional bits of a growing score
-constexpr uint32_t kIpLdsN = 8192;         // per-vertex state in LDS up to this many vertices
+constexpr uint32_t kIpLdsN = 8192;         // all per-vertex state in LDS up to this many vertices
+constexpr uint32_t kIpSideN = 32768;       // sides and flags in LDS up to this many vertices
+constexpr uint32_t kIpAdjN = 4096;         // clique scores precomputed up to this many vertices (128 MB)
+constexpr uint64_t kIpAdjBytes = uint64_t(256) << 20;  // clique-score bytes of one recursion level
 constexpr uint32_t kIpBigNet = 64;         // nets of a move with more pins are updated by a whole wave
 constexpr uint32_t kIpQueue = 512;
+constexpr int kIpMoveBatch = 4;
+constexpr int kIpScanBatch = 8;            // vertices whose scores one thread loads at once in an arg-max            // incident nets of a move whose records one thread loads at once
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr uint64_t kIpStartTag = 0x243F6A8885A308D3ull;
 constexpr uint64_t kIpFillTag = 0x13198A2E03707344ull;
@@ -57,10 +65,34 @@ constexpr uint64_t kIpPassTag = 0xA4093822299F31D0ull;
 
 // per-vertex flag bits
 constexpr uint8_t kSkip = 1;  // growing: too heavy for side 0
-constexpr uint8_t kHeap = 2;  // growing: scored and not taken
 constexpr uint8_t kSeen = 4;  // BFS: reached
 constexpr uint8_t kNext = 8;  // BFS: in the next level
 constexpr uint8_t kCand = 16; // repair: candidate not yet visited
+constexpr uint8_t kPick = 32; // growing: taken in the current step
+constexpr uint32_t kIpRandomMaxN = 2048;  // random-fill runs up to this many vertices
+constexpr uint32_t kIpMaxBatch = 256;     // growing takes min(256, max(1, n / 2048)) vertices per step
+
+// Pin counts of a net on side 0 / 1 in one 32-bit word (16 bits each) when
+// every net has fewer than 2^16 pins ("narrow"), else in two words: a run's
+// counts of all nets then stay small enough for the device's last-level
+// cache while the nets' pins and weights are shared by all runs.
+template <bool kWide>
+__device__ inline void pc_get(const uint32_t* pc, uint32_t e, uint32_t& c0, uint32_t& c1) {
+  if (kWide) {
+    const uint2 v = reinterpret_cast<const uint2*>(pc)[e];
+    c0 = v.x;
+    c1 = v.y;
+  } else {
+    const uint32_t v = pc[e];
+    c0 = v & 0xFFFFu;
+    c1 = v >> 16;
+  }
+}
+template <bool kWide>
+__device__ inline void pc_set(uint32_t* pc, uint32_t e, uint32_t c0, uint32_t c1) {
+  if (kWide) reinterpret_cast<uint2*>(pc)[e] = make_uint2(c0, c1);
+  else pc[e] = c0 | (c1 << 16);
+}
 
 struct IpArgs {
   const uint32_t* noff;
@@ -74,14 +106,18 @@ struct IpArgs {
   int64_t max0, max1, target0;
   uint64_t stream;
   int passes;
-  // per-run buffers, run r at r * n (r * 2m for pc, r * 4 for out)
+  // per-run buffers, run r at r * n (r * m or r * 2m words for pc, r * 4
+  // for out)
   uint8_t* side;   // the run's bisection
-  uint8_t* flag;   // beyond kIpLdsN vertices only
-  int64_t* val;    // beyond kIpLdsN vertices only
+  uint8_t* flag;   // beyond kIpSideN vertices only
+  int64_t* val;    // gains from the gain kernel; growing scores beyond kIpLdsN vertices
   int64_t* snap;
   uint32_t* list;
-  uint32_t* pc;    // pins of net e on side 0 / 1 at 2e / 2e+1
+  uint32_t* pc;    // per net: pins on side 0 / 1
   int64_t* out;    // cut, balanced, side-0 weight, side-1 weight
+  // clique scores of growing, n x n (row v: what v's joining adds to every
+  // vertex), for up to kIpAdjN vertices; null: growing walks the nets
+  const uint64_t* adj;
 };
 
 struct IpPerm {
@@ -130,6 +166,8 @@ struct IpShared {
   uint32_t qe[kIpQueue];
   int64_t qdf[kIpQueue], qdt[kIpQueue];
   uint32_t buf[kIpThreads];
+  uint32_t pick[kIpMaxBatch];
+  uint32_t pre[kIpMaxBatch + 1];  // growing: incident-net offsets of the step's joined vertices
 };
 
 __device__ inline unsigned long long shfl_xor_u64(unsigned long long x, int off) { return __shfl_xor(x, off, 64); }
@@ -210,35 +248,72 @@ __device__ inline uint32_t compact_next(IpShared& s, uint8_t* flag, uint32_t n, 
 // pin counts of a net on v's old and new side before the move, a pin on the
 // old side gains w * ([a == 2] - [a == 1] + [b == 0]), a pin on the new side
 // w * ([b == 0] - [a == 1] - [b == 1]); v's own gain is negated.
+// val[x] += d for the pins x of pins[jb, je) other than `skip` for which
+// take(x) holds; up to 8 pins are loaded before any is used
+template <typename F>
+__device__ inline void ip_add_pins(const uint32_t* pins, uint32_t jb, uint32_t je, uint32_t skip, int64_t* val,
+                                   F&& delta) {
+  for (uint32_t j0 = jb; j0 < je; j0 += 8) {
+    uint32_t xs[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) xs[q] = j0 + q < je ? pins[j0 + q] : kNone;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (xs[q] != kNone && xs[q] != skip) {
+        const int64_t d = delta(xs[q]);
+        if (d) atomicAdd(reinterpret_cast<unsigned long long*>(&val[xs[q]]), (unsigned long long)d);
+      }
+  }
+}
+
+template <bool kWide>
 __device__ inline void ip_move(const IpArgs& a, IpShared& s, uint8_t* side, int64_t* val, uint32_t* pc, uint32_t v) {
   const int f = side[v], t = 1 - f;
   const int64_t gv = val[v];
   const uint32_t vb = a.voff[v], ve = a.voff[v + 1];
-  for (uint32_t i = vb + threadIdx.x; i < ve; i += kIpThreads) {
-    const uint32_t e = a.inc[i];
-    const uint32_t ca = pc[2 * e + f], cb = pc[2 * e + t];
-    const int64_t we = a.nw[e];
+  for (uint32_t i0 = vb + threadIdx.x; i0 < ve; i0 += kIpMoveBatch * kIpThreads) {
+    // the records of up to kIpMoveBatch nets are loaded before any is used
+    uint32_t eb[kIpMoveBatch], c0b[kIpMoveBatch], c1b[kIpMoveBatch];
+    int32_t wb[kIpMoveBatch];
+#pragma unroll
+    for (int q = 0; q < kIpMoveBatch; ++q) {
+      const uint32_t i = i0 + q * kIpThreads;
+      eb[q] = i < ve ? a.inc[i] : kNone;
+    }
+#pragma unroll
+    for (int q = 0; q < kIpMoveBatch; ++q) {
+      c0b[q] = c1b[q] = 0;
+      wb[q] = 0;
+      if (eb[q] != kNone) {
+        pc_get<kWide>(pc, eb[q], c0b[q], c1b[q]);
+        wb[q] = a.nw[eb[q]];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kIpMoveBatch; ++q) {
+    if (eb[q] == kNone) continue;
+    const uint32_t e = eb[q];
+    const uint32_t ca = f ? c1b[q] : c0b[q], cb = f ? c0b[q] : c1b[q];
+    const int64_t we = wb[q];
     const int64_t df = we * (int64_t(ca == 2) - int64_t(ca == 1) + int64_t(cb == 0));
     const int64_t dt = we * (int64_t(cb == 0) - int64_t(ca == 1) - int64_t(cb == 1));
     if (df || dt) {
       const uint32_t jb = a.noff[e], je = a.noff[e + 1];
-      uint32_t q = kNone;
+      uint32_t slot = kNone;
       if (je - jb > kIpBigNet) {
-        q = atomicAdd(&s.qn, 1u);
-        if (q < kIpQueue) {
-          s.qe[q] = e;
-          s.qdf[q] = df;
-          s.qdt[q] = dt;
+        slot = atomicAdd(&s.qn, 1u);
+        if (slot < kIpQueue) {
+          s.qe[slot] = e;
+          s.qdf[slot] = df;
+          s.qdt[slot] = dt;
         }
       }
-      if (q == kNone || q >= kIpQueue)
-        for (uint32_t j = jb; j < je; ++j) {
-          const uint32_t x = a.pins[j];
-          if (x != v) atomicAdd(reinterpret_cast<unsigned long long*>(&val[x]), (unsigned long long)(side[x] == f ? df : dt));
-        }
+      if (slot == kNone || slot >= kIpQueue)
+        ip_add_pins(a.pins, jb, je, v, val, [&](uint32_t x) { return side[x] == f ? df : dt; });
     }
-    pc[2 * e + f] = ca - 1;
-    pc[2 * e + t] = cb + 1;
+    if (f == 0) pc_set<kWide>(pc, e, ca - 1, cb + 1);
+    else pc_set<kWide>(pc, e, cb + 1, ca - 1);
+    }
   }
   __syncthreads();
   const uint32_t qn = min(s.qn, kIpQueue);
@@ -262,24 +337,37 @@ __device__ inline void ip_move(const IpArgs& a, IpShared& s, uint8_t* side, int6
   __syncthreads();
 }
 
-template <bool kLds>
-__global__ __launch_bounds__(kIpThreads) void ip_run_kernel(IpArgs a) {
+// Where a run keeps its per-vertex state: kMode 0 all in LDS (up to kIpLdsN
+// vertices), 1 sides and flags in LDS and scores / gains in global memory (up
+// to kIpSideN), 2 all in global memory.
+template <int kMode>
+constexpr uint32_t ip_side_slots() { return kMode == 0 ? kIpLdsN : (kMode == 1 ? kIpSideN : 1); }
+template <int kMode>
+constexpr uint32_t ip_val_slots() { return kMode == 0 ? kIpLdsN : 1; }
+
+// The run kernels take the bisections of one recursion level at once:
+// workgroup b works on bisection b / runs (its arguments args[b / runs]).
+template <int kMode, bool kWide>
+__global__ __launch_bounds__(kIpThreads) void ip_grow_kernel(const IpArgs* args, uint32_t runs) {
+  const IpArgs a = args[blockIdx.x / runs];
   __shared__ IpShared s;
-  __shared__ uint8_t l_side[kLds ? kIpLdsN : 1];
-  __shared__ uint8_t l_flag[kLds ? kIpLdsN : 1];
-  __shared__ int64_t l_val[kLds ? kIpLdsN : 1];
-  const int r = blockIdx.x;
+  __shared__ uint8_t l_side[ip_side_slots<kMode>()];
+  __shared__ uint8_t l_flag[ip_side_slots<kMode>()];
+  __shared__ int64_t l_val[ip_val_slots<kMode>()];
+  const int r = int(blockIdx.x % runs);
   const uint32_t n = a.n, m = a.m;
   const uint32_t tid = threadIdx.x;
   uint8_t* g_side = a.side + size_t(r) * n;
-  uint8_t* side = kLds ? l_side : g_side;
-  uint8_t* flag = kLds ? l_flag : a.flag + size_t(r) * n;
-  int64_t* val = kLds ? l_val : a.val + size_t(r) * n;
-  int64_t* snap = a.snap + size_t(r) * n;
+  uint8_t* side = kMode < 2 ? l_side : g_side;
+  uint8_t* flag = kMode < 2 ? l_flag : a.flag + size_t(r) * n;
+  int64_t* val = kMode == 0 ? l_val : a.val + size_t(r) * n;
   uint32_t* list = a.list + size_t(r) * n;
-  uint32_t* pc = a.pc + size_t(r) * 2 * m;
+  uint32_t* pc = a.pc + size_t(r) * m * (kWide ? 2 : 1);
   const uint64_t rk = ip_run_key(a.stream, r);
-  const int variant = r % 3;
+  // no random fill beyond kIpRandomMaxN vertices: a random start needs
+  // about n / 4 refinement moves there and never wins
+  const int variant = n > kIpRandomMaxN ? r % 2 : r % 3;
+  const uint32_t batch = min(kIpMaxBatch, max(1u, n / 2048u));
   const uint32_t start = uint32_t(mix64(rk ^ kIpStartTag) % n);
   for (uint32_t x = tid; x < n; x += kIpThreads) {
     side[x] = 1;
@@ -287,7 +375,7 @@ __global__ __launch_bounds__(kIpThreads) void ip_run_kernel(IpArgs a) {
     val[x] = 0;
   }
   if (variant == 1)
-    for (uint32_t e = tid; e < m; e += kIpThreads) pc[2 * e] = 0;  // scanned marks
+    for (uint32_t e = tid; e < m; e += kIpThreads) pc[kWide ? 2 * e : e] = 0;  // scanned marks
   if (tid == 0) {
     s.w[0] = 0;
     s.scan = 0;
@@ -300,15 +388,38 @@ __global__ __launch_bounds__(kIpThreads) void ip_run_kernel(IpArgs a) {
   if (variant == 0) {
     for (;;) {
       if (s.w[0] >= a.target0) break;
-      uint64_t key = 0;
-      uint32_t v = kNone;
-      for (uint32_t x = tid; x < n; x += kIpThreads)
-        if (flag[x] & kHeap) {
-          const uint64_t k = uint64_t(val[x]);
-          if (v == kNone || k > key) { key = k; v = x; }
+      // the `batch` highest non-zero scores, ties to the smaller id
+      uint32_t np = 0;
+      while (np < batch) {
+        uint64_t key = 0;
+        uint32_t v = kNone;
+        for (uint32_t x0 = tid; x0 < n; x0 += kIpScanBatch * kIpThreads) {
+          // scores of kIpScanBatch vertices are loaded before any is used
+          uint64_t kb[kIpScanBatch];
+#pragma unroll
+          for (int q = 0; q < kIpScanBatch; ++q) {
+            const uint32_t x = x0 + q * kIpThreads;
+            kb[q] = x < n ? uint64_t(val[x]) : 0;
+          }
+#pragma unroll
+          for (int q = 0; q < kIpScanBatch; ++q) {
+            const uint32_t x = x0 + q * kIpThreads;
+            if (kb[q] != 0 && side[x] == 1 && !(flag[x] & (kSkip | kPick)) && (v == kNone || kb[q] > key)) {
+              key = kb[q];
+              v = x;
+            }
+          }
         }
-      block_argmax(s, key, v);
-      if (v == kNone) {  // a new seed: the next free vertex after `start`
+        block_argmax(s, key, v);
+        if (v == kNone) break;
+        if (tid == 0) {
+          flag[v] = uint8_t(flag[v] | kPick);
+          s.pick[np] = v;
+        }
+        ++np;
+        __syncthreads();
+      }
+      if (np == 0) {  // a seed: the next free vertex after `start`
         const uint32_t sc = s.scan;
         uint32_t first = kNone;
         for (uint32_t i = sc + tid; i < n; i += kIpThreads) {
@@ -317,32 +428,61 @@ __global__ __launch_bounds__(kIpThreads) void ip_run_kernel(IpArgs a) {
         }
         first = block_min(s, first);
         if (first == kNone) break;
-        v = (start + first) % n;
-        if (tid == 0) s.scan = first + 1;
-      }
-      const bool fits = s.w[0] + a.vw[v] <= a.max0;
-      __syncthreads();
-      if (tid == 0) {
-        flag[v] = uint8_t((flag[v] & ~kHeap) | (fits ? 0 : kSkip));
-        if (fits) {
-          side[v] = 0;
-          s.w[0] += a.vw[v];
+        if (tid == 0) {
+          s.pick[0] = (start + first) % n;
+          s.scan = first + 1;
         }
+        np = 1;
+        __syncthreads();
+      }
+      // the picks join in order while side 0 is below its target
+      if (tid == 0) {
+        uint32_t na = 0, deg = 0;
+        int64_t w0 = s.w[0];
+        for (uint32_t i = 0; i < np; ++i) {
+          const uint32_t v = s.pick[i];
+          flag[v] = uint8_t(flag[v] & ~kPick);
+          if (w0 >= a.target0) continue;
+          if (w0 + a.vw[v] > a.max0) {
+            flag[v] = uint8_t(flag[v] | kSkip);
+            continue;
+          }
+          side[v] = 0;
+          w0 += a.vw[v];
+          s.pre[na] = deg;
+          s.pick[na++] = v;
+          deg += a.voff[v + 1] - a.voff[v];
+        }
+        s.pre[na] = deg;
+        s.w[0] = w0;
+        s.count = na;
       }
       __syncthreads();
-      if (!fits) continue;
-      for (uint32_t i = a.voff[v] + tid; i < a.voff[v + 1]; i += kIpThreads) {
-        const uint32_t e = a.inc[i];
-        const uint32_t jb = a.noff[e], je = a.noff[e + 1];
-        if (je - jb > kIpGrowNetLimit) continue;
-        const uint64_t add = (uint64_t(uint32_t(a.nw[e])) << kIpScoreFrac) / uint64_t(max(je - jb - 1, 1u));
-        for (uint32_t j = jb; j < je; ++j) {
-          const uint32_t x = a.pins[j];
-          const uint8_t fl = flag[x];
-          if (side[x] == 1 && !(fl & kSkip)) {
-            atomicAdd(reinterpret_cast<unsigned long long*>(&val[x]), (unsigned long long)add);
-            if (!(fl & kHeap)) flag[x] = uint8_t(fl | kHeap);
+      const uint32_t na = s.count;
+      if (a.adj) {
+        for (uint32_t i = 0; i < na; ++i) {
+          const uint64_t* row = a.adj + size_t(s.pick[i]) * n;
+          for (uint32_t x = tid; x < n; x += kIpThreads)
+            if (side[x] == 1 && !(flag[x] & kSkip)) val[x] = int64_t(uint64_t(val[x]) + row[x]);
+        }
+      } else {
+        // the incident nets of all joined vertices in one flat range, so
+        // that every thread's loads are independent of each other
+        const uint32_t total = s.pre[na];
+        for (uint32_t k = tid; k < total; k += kIpThreads) {
+          uint32_t lo = 0, hi = na - 1;  // the joined vertex whose range holds k
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (s.pre[mid] <= k) lo = mid;
+            else hi = mid - 1;
           }
+          const uint32_t e = a.inc[a.voff[s.pick[lo]] + (k - s.pre[lo])];
+          const uint32_t jb = a.noff[e], je = a.noff[e + 1];
+          if (je - jb > kIpGrowNetLimit) continue;
+          const uint64_t add = (uint64_t(uint32_t(a.nw[e])) << kIpScoreFrac) / uint64_t(max(je - jb - 1, 1u));
+          ip_add_pins(a.pins, jb, je, kNone, val, [&](uint32_t x) {
+            return (side[x] == 1 && !(flag[x] & kSkip)) ? int64_t(add) : int64_t(0);
+          });
         }
       }
       __syncthreads();
@@ -390,8 +530,8 @@ __global__ __launch_bounds__(kIpThreads) void ip_run_kernel(IpArgs a) {
         for (uint32_t q = a.voff[v] + tid; q < a.voff[v + 1]; q += kIpThreads) {
           const uint32_t e = a.inc[q];
           const uint32_t jb = a.noff[e], je = a.noff[e + 1];
-          if (pc[2 * e] || je - jb > kIpGrowNetLimit) continue;
-          pc[2 * e] = 1;
+          if (pc[kWide ? 2 * e : e] || je - jb > kIpGrowNetLimit) continue;
+          pc[kWide ? 2 * e : e] = 1;
           for (uint32_t j = jb; j < je; ++j) {
             const uint32_t x = a.pins[j];
             const uint8_t fl = flag[x];
@@ -425,29 +565,126 @@ __global__ __launch_bounds__(kIpThreads) void ip_run_kernel(IpArgs a) {
     }
   }
 
-  // ------------------------------------------------------- refinement --
-  for (uint32_t e = tid; e < m; e += kIpThreads) {
+  // the bisection and side-0 weight go to the refinement; its gains are
+  // summed into the zeroed global array by the gain kernel
+  __syncthreads();
+  int64_t* g_val = a.val + size_t(r) * n;
+  for (uint32_t x = tid; x < n; x += kIpThreads) {
+    if (kMode < 2) g_side[x] = side[x];
+    g_val[x] = 0;
+  }
+  if (tid == 0) a.out[4 * size_t(r) + 2] = s.w[0];
+}
+
+// Pin counts, net records and initial gains of every run's bisection, with
+// `parts` workgroups per run, each over one range of nets: a pin on side s
+// of net e gains w(e) if it is e's only pin on s and loses w(e) if e has no
+// pin on the other side.  Nets of more than kIpBigNet pins are done by whole
+// waves.
+template <int kMode, bool kWide>
+__global__ __launch_bounds__(kIpThreads) void ip_gain_kernel(const IpArgs* args, uint32_t runs, uint32_t parts) {
+  const IpArgs a = args[blockIdx.x / (runs * parts)];
+  __shared__ uint8_t l_side[ip_side_slots<kMode>()];
+  __shared__ int64_t l_val[ip_val_slots<kMode>()];
+  __shared__ uint32_t qe[kIpQueue];
+  __shared__ uint32_t qn;
+  const uint32_t r = (blockIdx.x / parts) % runs, g = blockIdx.x % parts;
+  const uint32_t n = a.n, m = a.m, tid = threadIdx.x;
+  const uint8_t* g_side = a.side + size_t(r) * n;
+  int64_t* g_val = a.val + size_t(r) * n;
+  uint32_t* pc = a.pc + size_t(r) * m * (kWide ? 2 : 1);
+  const uint8_t* side = kMode < 2 ? l_side : g_side;
+  int64_t* val = kMode == 0 ? l_val : g_val;
+  for (uint32_t x = tid; x < n; x += kIpThreads) {
+    if (kMode < 2) l_side[x] = g_side[x];
+    if (kMode == 0) l_val[x] = 0;
+  }
+  if (tid == 0) qn = 0;
+  __syncthreads();
+  const uint32_t chunk = (m + parts - 1) / parts;
+  const uint32_t e0 = min(m, g * chunk), e1 = min(m, e0 + chunk);
+  for (uint32_t e = e0 + tid; e < e1; e += kIpThreads) {
     const uint32_t jb = a.noff[e], je = a.noff[e + 1];
+    if (je - jb > kIpBigNet) {
+      const uint32_t q = atomicAdd(&qn, 1u);
+      if (q < kIpQueue) {
+        qe[q] = e;
+        continue;
+      }
+    }
     uint32_t c1 = 0;
     for (uint32_t j = jb; j < je; ++j) c1 += side[a.pins[j]];
-    pc[2 * e] = (je - jb) - c1;
-    pc[2 * e + 1] = c1;
-  }
-  if (tid == 0) s.w[1] = a.W - s.w[0];
-  __syncthreads();
-  {
-    const int w = tid >> 6, lane = tid & 63;
-    for (uint32_t u = w; u < n; u += kIpWaves) {
-      const int f = side[u], t = 1 - f;
-      int64_t g = 0;
-      for (uint32_t i = a.voff[u] + lane; i < a.voff[u + 1]; i += 64) {
-        const uint32_t e = a.inc[i];
-        const int64_t we = a.nw[e];
-        g += (pc[2 * e + f] == 1 ? we : 0) - (pc[2 * e + t] == 0 ? we : 0);
+    const uint32_t c0 = (je - jb) - c1;
+    const int32_t w = a.nw[e];
+    pc_set<kWide>(pc, e, c0, c1);
+    if (w != 0 && (c0 <= 1 || c1 <= 1)) {
+      const int64_t d0 = (c0 == 1 ? int64_t(w) : 0) - (c1 == 0 ? int64_t(w) : 0);
+      const int64_t d1 = (c1 == 1 ? int64_t(w) : 0) - (c0 == 0 ? int64_t(w) : 0);
+      for (uint32_t j = jb; j < je; ++j) {
+        const uint32_t x = a.pins[j];
+        const int64_t d = side[x] ? d1 : d0;
+        if (d) atomicAdd(reinterpret_cast<unsigned long long*>(&val[x]), (unsigned long long)d);
       }
-      for (int off = 32; off > 0; off >>= 1) g += int64_t(shfl_xor_u64((unsigned long long)g, off));
-      if (lane == 0) val[u] = g;
     }
+  }
+  __syncthreads();
+  const uint32_t nq = min(qn, kIpQueue);
+  const int wv = tid >> 6, lane = tid & 63;
+  for (uint32_t q = wv; q < nq; q += kIpWaves) {
+    const uint32_t e = qe[q];
+    const uint32_t jb = a.noff[e], je = a.noff[e + 1];
+    uint32_t c1 = 0;
+    for (uint32_t j = jb + lane; j < je; j += 64) c1 += side[a.pins[j]];
+    for (int off = 32; off > 0; off >>= 1) c1 += __shfl_xor(c1, off, 64);
+    const uint32_t c0 = (je - jb) - c1;
+    const int32_t w = a.nw[e];
+    if (lane == 0) pc_set<kWide>(pc, e, c0, c1);
+    if (w != 0 && (c0 <= 1 || c1 <= 1)) {
+      const int64_t d0 = (c0 == 1 ? int64_t(w) : 0) - (c1 == 0 ? int64_t(w) : 0);
+      const int64_t d1 = (c1 == 1 ? int64_t(w) : 0) - (c0 == 0 ? int64_t(w) : 0);
+      for (uint32_t j = jb + lane; j < je; j += 64) {
+        const uint32_t x = a.pins[j];
+        const int64_t d = side[x] ? d1 : d0;
+        if (d) atomicAdd(reinterpret_cast<unsigned long long*>(&val[x]), (unsigned long long)d);
+      }
+    }
+  }
+  if (kMode == 0) {
+    __syncthreads();
+    for (uint32_t x = tid; x < n; x += kIpThreads)
+      if (l_val[x]) atomicAdd(reinterpret_cast<unsigned long long*>(&g_val[x]), (unsigned long long)l_val[x]);
+  }
+}
+
+// Balance repair, gain-move passes and the cut of every run (one workgroup
+// per run, gains from the gain kernel).
+template <int kMode, bool kWide>
+__global__ __launch_bounds__(kIpThreads) void ip_refine_kernel(const IpArgs* args, uint32_t runs) {
+  const IpArgs a = args[blockIdx.x / runs];
+  __shared__ IpShared s;
+  __shared__ uint8_t l_side[ip_side_slots<kMode>()];
+  __shared__ uint8_t l_flag[ip_side_slots<kMode>()];
+  __shared__ int64_t l_val[ip_val_slots<kMode>()];
+  const int r = int(blockIdx.x % runs);
+  const uint32_t n = a.n, m = a.m;
+  const uint32_t tid = threadIdx.x;
+  uint8_t* g_side = a.side + size_t(r) * n;
+  int64_t* g_val = a.val + size_t(r) * n;
+  uint8_t* side = kMode < 2 ? l_side : g_side;
+  uint8_t* flag = kMode < 2 ? l_flag : a.flag + size_t(r) * n;
+  int64_t* val = kMode == 0 ? l_val : g_val;
+  int64_t* snap = a.snap + size_t(r) * n;
+  uint32_t* pc = a.pc + size_t(r) * m * (kWide ? 2 : 1);
+  const uint64_t rk = ip_run_key(a.stream, r);
+  for (uint32_t x = tid; x < n; x += kIpThreads) {
+    if (kMode < 2) l_side[x] = g_side[x];
+    if (kMode == 0) l_val[x] = g_val[x];
+    flag[x] = 0;
+  }
+  if (tid == 0) {
+    s.w[0] = a.out[4 * size_t(r) + 2];
+    s.w[1] = a.W - s.w[0];
+    s.qn = 0;
   }
   __syncthreads();
   for (int f = 0; f < 2; ++f) {
@@ -465,18 +702,29 @@ __global__ __launch_bounds__(kIpThreads) void ip_run_kernel(IpArgs a) {
       // the signed gain
       uint64_t key = 0;
       uint32_t c = kNone;
-      for (uint32_t x = tid; x < n; x += kIpThreads)
-        if (flag[x] & kCand) {
-          const uint64_t k = uint64_t(snap[x]) ^ (1ull << 63);
-          if (c == kNone || k > key) { key = k; c = x; }
+      for (uint32_t x0 = tid; x0 < n; x0 += kIpScanBatch * kIpThreads) {
+        uint64_t kb[kIpScanBatch];
+#pragma unroll
+        for (int q = 0; q < kIpScanBatch; ++q) {
+          const uint32_t x = x0 + q * kIpThreads;
+          kb[q] = x < n ? uint64_t(snap[x]) ^ (1ull << 63) : 0;
         }
+#pragma unroll
+        for (int q = 0; q < kIpScanBatch; ++q) {
+          const uint32_t x = x0 + q * kIpThreads;
+          if (x < n && (flag[x] & kCand) && (c == kNone || kb[q] > key)) {
+            key = kb[q];
+            c = x;
+          }
+        }
+      }
       block_argmax(s, key, c);
       if (c == kNone) break;
       const bool fits = s.w[1 - f] + a.vw[c] <= mt;
       __syncthreads();
       if (tid == 0) flag[c] = uint8_t(flag[c] & ~kCand);
       __syncthreads();
-      if (fits) ip_move(a, s, side, val, pc, c);
+      if (fits) ip_move<kWide>(a, s, side, val, pc, c);
     }
     for (uint32_t x = tid; x < n; x += kIpThreads) flag[x] = uint8_t(flag[x] & ~kCand);
     __syncthreads();
@@ -509,17 +757,20 @@ __global__ __launch_bounds__(kIpThreads) void ip_run_kernel(IpArgs a) {
       const uint32_t u = s.buf[hit - pos];
       improved |= val[u] > 0;
       __syncthreads();
-      ip_move(a, s, side, val, pc, u);
+      ip_move<kWide>(a, s, side, val, pc, u);
       pos = hit + 1;
     }
     if (!improved) break;
   }
 
   int64_t cut = 0;
-  for (uint32_t e = tid; e < m; e += kIpThreads)
-    if (pc[2 * e] && pc[2 * e + 1]) cut += a.nw[e];
+  for (uint32_t e = tid; e < m; e += kIpThreads) {
+    uint32_t c0, c1;
+    pc_get<kWide>(pc, e, c0, c1);
+    if (c0 && c1) cut += a.nw[e];
+  }
   cut = block_sum(s, cut);
-  if (kLds)
+  if (kMode < 2)
     for (uint32_t x = tid; x < n; x += kIpThreads) g_side[x] = side[x];
   if (tid == 0) {
     int64_t* o = a.out + 4 * size_t(r);
@@ -530,9 +781,39 @@ __global__ __launch_bounds__(kIpThreads) void ip_run_kernel(IpArgs a) {
   }
 }
 
-// index of the balanced run with the smallest cut (ties: the smaller index)
-__global__ void ip_best_kernel(const int64_t* out, int runs, int32_t* best) {
+// Row v of the clique scores: for every other vertex x, the sum over the
+// nets of at most 100 pins holding both of floor(w(e) * 2^16 / (|e| - 1)),
+// i.e. what v's joining side 0 adds to x's growing score (built in LDS with
+// atomics, written out in one coalesced pass).
+__global__ __launch_bounds__(kIpThreads) void ip_adjacency_kernel(const uint32_t* noff, const uint32_t* pins,
+                                                                  const uint32_t* voff, const uint32_t* inc,
+                                                                  const int32_t* nw, uint32_t n, uint64_t* adj) {
+  __shared__ uint64_t row[kIpAdjN];
+  for (uint32_t v = blockIdx.x; v < n; v += gridDim.x) {
+    for (uint32_t x = threadIdx.x; x < n; x += kIpThreads) row[x] = 0;
+    __syncthreads();
+    for (uint32_t i = voff[v] + threadIdx.x; i < voff[v + 1]; i += kIpThreads) {
+      const uint32_t e = inc[i];
+      const uint32_t jb = noff[e], je = noff[e + 1];
+      if (je - jb > kIpGrowNetLimit) continue;
+      const uint64_t add = (uint64_t(uint32_t(nw[e])) << kIpScoreFrac) / uint64_t(max(je - jb - 1, 1u));
+      for (uint32_t j = jb; j < je; ++j) {
+        const uint32_t x = pins[j];
+        if (x != v) atomicAdd(reinterpret_cast<unsigned long long*>(&row[x]), (unsigned long long)add);
+      }
+    }
+    __syncthreads();
+    for (uint32_t x = threadIdx.x; x < n; x += kIpThreads) adj[size_t(v) * n + x] = row[x];
+    __syncthreads();
+  }
+}
+
+// index of the balanced run with the smallest cut (ties: the smaller index),
+// one workgroup per bisection
+__global__ void ip_best_kernel(const IpArgs* args, int runs, int32_t* bests) {
   if (threadIdx.x != 0) return;
+  const int64_t* out = args[blockIdx.x].out;
+  int32_t* best = bests + blockIdx.x;
   int b = 0;
   for (int r = 1; r < runs; ++r) {
     const bool ba = out[4 * r + 1] != 0, bb = out[4 * b + 1] != 0;
@@ -650,6 +931,7 @@ __global__ void ip_assign_kernel(const uint32_t* orig, uint32_t n, int32_t block
 // hypergraph's arrays, the others own theirs.
 struct IpSub {
   uint32_t n = 0, m = 0;
+  uint32_t max_esize = 0;  // bound on the net sizes
   uint64_t P = 0;
   int64_t W = 0;
   const uint32_t *noff = nullptr, *pins = nullptr, *voff = nullptr, *inc = nullptr, *orig = nullptr;
@@ -658,117 +940,225 @@ struct IpSub {
   DevBuf<int32_t> b_vw, b_nw;
 };
 
-// Scratch of one bisection's runs.
+// Scratch of the runs of one bisection.
 struct IpRuns {
   DevBuf<uint8_t> side, flag;
   DevBuf<int64_t> val, snap, out;
   DevBuf<uint32_t> list, pc;
-  DevBuf<int32_t> best;
+  DevBuf<uint64_t> adj;
 };
 
-void launch_runs(Context& c, const IpSub& h, int64_t max0, int64_t max1, int64_t target0, uint64_t stream, int runs,
-                 int passes, IpRuns& rb) {
+// Temporaries of splitting one sub-hypergraph into its halves.
+struct IpSplit {
+  DevBuf<uint32_t> newid, cnt0, cnt1, keep0, keep1, off0, off1, eid0, eid1;
+  DevBuf<int64_t> sizes;
+};
+
+// One bisection of a recursion level.
+struct IpTask {
+  std::unique_ptr<IpSub> h;
+  int k = 0, first = 0;
+  int mode = 0;
+  bool wide = false;
+  IpArgs args{};
+  IpRuns rb;
+  IpSplit sp;
+  std::unique_ptr<IpSub> half[2];
+};
+
+// Buffers and arguments of the runs of task t (and its clique scores).
+void prepare(Context& c, IpTask& t, int64_t max0, int64_t max1, int64_t target0, uint64_t stream, int runs,
+             int passes, bool use_adj) {
+  const IpSub& h = *t.h;
+  IpRuns& rb = t.rb;
   const size_t n = std::max<uint32_t>(h.n, 1), m = h.m;
-  const bool lds = h.n <= kIpLdsN;
-  rb.side.ensure(size_t(runs) * n);
-  rb.snap.ensure(size_t(runs) * n);
-  rb.list.ensure(size_t(runs) * n);
-  rb.pc.ensure(std::max<size_t>(size_t(runs) * 2 * m, 1));
-  rb.out.ensure(size_t(runs) * 4);
-  rb.best.ensure(1);
-  if (!lds) {
-    rb.flag.ensure(size_t(runs) * n);
-    rb.val.ensure(size_t(runs) * n);
-  }
-  IpArgs a{h.noff, h.pins, h.voff, h.inc, h.vw, h.nw, h.n, h.m, h.W, max0, max1, target0, stream, passes,
-           rb.side.p, lds ? nullptr : rb.flag.p, lds ? nullptr : rb.val.p, rb.snap.p, rb.list.p, rb.pc.p, rb.out.p};
-  {
-    TimedRegion tr(c.prof, c.stream, kFamInitial, 0.0, lds ? "ip_run_kernel<true>" : "ip_run_kernel<false>");
-    if (lds) ip_run_kernel<true><<<runs, kIpThreads, 0, c.stream>>>(a);
-    else ip_run_kernel<false><<<runs, kIpThreads, 0, c.stream>>>(a);
+  t.mode = h.n <= kIpLdsN ? 0 : (h.n <= kIpSideN ? 1 : 2);
+  t.wide = h.max_esize > 0xFFFFu;
+  rb.side.alloc(size_t(runs) * n);
+  rb.snap.alloc(size_t(runs) * n);
+  rb.list.alloc(size_t(runs) * n);
+  rb.pc.alloc(std::max<size_t>(size_t(runs) * m * (t.wide ? 2 : 1), 1));
+  rb.out.alloc(size_t(runs) * 4);
+  rb.val.alloc(size_t(runs) * n);
+  if (t.mode == 2) rb.flag.alloc(size_t(runs) * n);
+  if (use_adj) {
+    // clique scores, shared by the growing runs of this bisection
+    rb.adj.alloc(n * n);
+    TimedRegion tr(c.prof, c.stream, kFamInitial, 0.0, "ip_adjacency_kernel");
+    ip_adjacency_kernel<<<unsigned(std::min<size_t>(n, 2048)), kIpThreads, 0, c.stream>>>(h.noff, h.pins, h.voff,
+                                                                                       h.inc, h.nw, h.n, rb.adj.p);
     MKP_LAUNCH_CHECK();
   }
-  ip_best_kernel<<<1, 64, 0, c.stream>>>(rb.out.p, runs, rb.best.p);
+  t.args = IpArgs{h.noff, h.pins, h.voff, h.inc, h.vw, h.nw, h.n, h.m, h.W, max0, max1, target0, stream, passes,
+                  rb.side.p, t.mode == 2 ? rb.flag.p : nullptr, rb.val.p, rb.snap.p, rb.list.p, rb.pc.p, rb.out.p,
+                  use_adj ? rb.adj.p : nullptr};
+}
+
+template <int kMode, bool kWide>
+void run_group(Context& c, const IpArgs* args, uint32_t count, uint32_t runs, uint32_t parts) {
+  const std::string tag = "<" + std::to_string(kMode) + (kWide ? ", true>" : ", false>");
+  {
+    TimedRegion tr(c.prof, c.stream, kFamInitial, 0.0, std::string("ip_grow_kernel") + tag);
+    ip_grow_kernel<kMode, kWide><<<count * runs, kIpThreads, 0, c.stream>>>(args, runs);
+    MKP_LAUNCH_CHECK();
+  }
+  {
+    TimedRegion tr(c.prof, c.stream, kFamInitial, 0.0, std::string("ip_gain_kernel") + tag);
+    ip_gain_kernel<kMode, kWide><<<count * runs * parts, kIpThreads, 0, c.stream>>>(args, runs, parts);
+    MKP_LAUNCH_CHECK();
+  }
+  {
+    TimedRegion tr(c.prof, c.stream, kFamInitial, 0.0, std::string("ip_refine_kernel") + tag);
+    ip_refine_kernel<kMode, kWide><<<count * runs, kIpThreads, 0, c.stream>>>(args, runs);
+    MKP_LAUNCH_CHECK();
+  }
+}
+
+// Runs every prepared bisection of `tasks` (reordered by state placement,
+// which each launch shares) and leaves the index of each one's best run in
+// best[i] for tasks[i].
+void run_level(Context& c, std::vector<IpTask*>& tasks, int runs, DevBuf<IpArgs>& dargs, DevBuf<int32_t>& best) {
+  std::stable_sort(tasks.begin(), tasks.end(), [](const IpTask* a, const IpTask* b) {
+    return a->mode * 2 + int(a->wide) < b->mode * 2 + int(b->wide);
+  });
+  const size_t T = tasks.size();
+  dargs.ensure(T);
+  best.ensure(T);
+  IpArgs* ha = c.staging<IpArgs>(T);
+  uint64_t max_m = 0;
+  for (size_t i = 0; i < T; ++i) {
+    ha[i] = tasks[i]->args;
+    max_m = std::max<uint64_t>(max_m, tasks[i]->h->m);
+  }
+  MKP_HIP(hipMemcpyAsync(dargs.p, ha, T * sizeof(IpArgs), hipMemcpyHostToDevice, c.stream));
+  // enough workgroups per run for the pass over the nets to fill the chip
+  const uint32_t R = uint32_t(runs);
+  uint32_t parts = std::max<uint32_t>(1, uint32_t(2 * c.num_cus) / uint32_t(T * R));
+  parts = std::min<uint32_t>(parts, uint32_t(std::max<uint64_t>(1, (max_m + 4095) / 4096)));
+  for (size_t lo = 0; lo < T;) {
+    size_t hi = lo;
+    while (hi < T && tasks[hi]->mode == tasks[lo]->mode && tasks[hi]->wide == tasks[lo]->wide) ++hi;
+    const IpArgs* ga = dargs.p + lo;
+    const uint32_t cnt = uint32_t(hi - lo);
+    const int mode = tasks[lo]->mode;
+    const bool wide = tasks[lo]->wide;
+    if (mode == 0) wide ? run_group<0, true>(c, ga, cnt, R, parts) : run_group<0, false>(c, ga, cnt, R, parts);
+    else if (mode == 1) wide ? run_group<1, true>(c, ga, cnt, R, parts) : run_group<1, false>(c, ga, cnt, R, parts);
+    else wide ? run_group<2, true>(c, ga, cnt, R, parts) : run_group<2, false>(c, ga, cnt, R, parts);
+    lo = hi;
+  }
+  ip_best_kernel<<<unsigned(T), 64, 0, c.stream>>>(dargs.p, runs, best.p);
   MKP_LAUNCH_CHECK();
 }
 
-// the two halves of h under the best run's bisection
-void split(Context& c, const IpSub& h, const IpRuns& rb, IpSub& a, IpSub& b) {
+// The halves of every task's sub-hypergraph under its best run (one wait
+// for all sizes).
+void split_level(Context& c, std::vector<IpTask*>& tasks, const int32_t* best) {
   const hipStream_t st = c.stream;
-  const uint32_t n = h.n, m = h.m;
-  DevBuf<uint32_t> newid(std::max<uint32_t>(n, 1));
-  a.b_vw.alloc(std::max<uint32_t>(n, 1));
-  b.b_vw.alloc(std::max<uint32_t>(n, 1));
-  a.b_orig.alloc(std::max<uint32_t>(n, 1));
-  b.b_orig.alloc(std::max<uint32_t>(n, 1));
-  DevBuf<int64_t> sizes(4);
-  ip_split_vertices_kernel<<<1, kIpThreads, 0, st>>>(rb.side.p, rb.best.p, n, h.vw, h.orig, newid.p, a.b_vw.p,
-                                                     b.b_vw.p, a.b_orig.p, b.b_orig.p, sizes.p);
-  MKP_LAUNCH_CHECK();
-  const size_t ms = std::max<uint32_t>(m, 1);
-  DevBuf<uint32_t> cnt0(ms), cnt1(ms), keep0(ms), keep1(ms), off0(size_t(m) + 1), off1(size_t(m) + 1),
-      eid0(size_t(m) + 1), eid1(size_t(m) + 1);
-  if (m) {
-    ip_net_count_kernel<<<grid_for(m, 256, 65536), 256, 0, st>>>(rb.side.p, rb.best.p, n, h.noff, h.pins, m, cnt0.p,
-                                                                cnt1.p, keep0.p, keep1.p);
+  const size_t T = tasks.size();
+  // sizes of task i: n0, W0, W1 then off0[m], off1[m], eid0[m], eid1[m]
+  int64_t* hs = c.staging<int64_t>(8 * T);
+  for (size_t i = 0; i < T; ++i) {
+    IpTask& t = *tasks[i];
+    const IpSub& h = *t.h;
+    IpSplit& sp = t.sp;
+    const uint32_t n = h.n, m = h.m;
+    const size_t ms = std::max<uint32_t>(m, 1);
+    t.half[0] = std::make_unique<IpSub>();
+    t.half[1] = std::make_unique<IpSub>();
+    IpSub &a = *t.half[0], &b = *t.half[1];
+    sp.newid.alloc(std::max<uint32_t>(n, 1));
+    a.b_vw.alloc(std::max<uint32_t>(n, 1));
+    b.b_vw.alloc(std::max<uint32_t>(n, 1));
+    a.b_orig.alloc(std::max<uint32_t>(n, 1));
+    b.b_orig.alloc(std::max<uint32_t>(n, 1));
+    sp.sizes.alloc(4);
+    ip_split_vertices_kernel<<<1, kIpThreads, 0, st>>>(t.rb.side.p, best + i, n, h.vw, h.orig, sp.newid.p, a.b_vw.p,
+                                                       b.b_vw.p, a.b_orig.p, b.b_orig.p, sp.sizes.p);
     MKP_LAUNCH_CHECK();
+    sp.cnt0.alloc(ms); sp.cnt1.alloc(ms); sp.keep0.alloc(ms); sp.keep1.alloc(ms);
+    sp.off0.alloc(size_t(m) + 1); sp.off1.alloc(size_t(m) + 1);
+    sp.eid0.alloc(size_t(m) + 1); sp.eid1.alloc(size_t(m) + 1);
+    if (m) {
+      ip_net_count_kernel<<<grid_for(m, 256, 65536), 256, 0, st>>>(t.rb.side.p, best + i, n, h.noff, h.pins, m,
+                                                                  sp.cnt0.p, sp.cnt1.p, sp.keep0.p, sp.keep1.p);
+      MKP_LAUNCH_CHECK();
+    }
+    exclusive_scan_u32(c, sp.cnt0.p, sp.off0.p, m);
+    exclusive_scan_u32(c, sp.cnt1.p, sp.off1.p, m);
+    exclusive_scan_u32(c, sp.keep0.p, sp.eid0.p, m);
+    exclusive_scan_u32(c, sp.keep1.p, sp.eid1.p, m);
+    int64_t* hi = hs + 8 * i;
+    uint32_t* hi32 = reinterpret_cast<uint32_t*>(hi + 4);
+    MKP_HIP(hipMemcpyAsync(hi, sp.sizes.p, 3 * 8, hipMemcpyDeviceToHost, st));
+    MKP_HIP(hipMemcpyAsync(hi32 + 0, sp.off0.p + m, 4, hipMemcpyDeviceToHost, st));
+    MKP_HIP(hipMemcpyAsync(hi32 + 1, sp.off1.p + m, 4, hipMemcpyDeviceToHost, st));
+    MKP_HIP(hipMemcpyAsync(hi32 + 2, sp.eid0.p + m, 4, hipMemcpyDeviceToHost, st));
+    MKP_HIP(hipMemcpyAsync(hi32 + 3, sp.eid1.p + m, 4, hipMemcpyDeviceToHost, st));
   }
-  exclusive_scan_u32(c, cnt0.p, off0.p, m);
-  exclusive_scan_u32(c, cnt1.p, off1.p, m);
-  exclusive_scan_u32(c, keep0.p, eid0.p, m);
-  exclusive_scan_u32(c, keep1.p, eid1.p, m);
-  int64_t* hs = c.staging<int64_t>(8);
-  uint32_t* hs32 = reinterpret_cast<uint32_t*>(hs + 4);
-  MKP_HIP(hipMemcpyAsync(hs, sizes.p, 3 * 8, hipMemcpyDeviceToHost, st));
-  MKP_HIP(hipMemcpyAsync(hs32 + 0, off0.p + m, 4, hipMemcpyDeviceToHost, st));
-  MKP_HIP(hipMemcpyAsync(hs32 + 1, off1.p + m, 4, hipMemcpyDeviceToHost, st));
-  MKP_HIP(hipMemcpyAsync(hs32 + 2, eid0.p + m, 4, hipMemcpyDeviceToHost, st));
-  MKP_HIP(hipMemcpyAsync(hs32 + 3, eid1.p + m, 4, hipMemcpyDeviceToHost, st));
   c.sync();
-  a.n = uint32_t(hs[0]);
-  b.n = n - a.n;
-  a.W = hs[1];
-  b.W = hs[2];
-  a.P = hs32[0];
-  b.P = hs32[1];
-  a.m = hs32[2];
-  b.m = hs32[3];
-  for (IpSub* s : {&a, &b}) {
-    s->b_noff.alloc(size_t(s->m) + 1);
-    s->b_pins.alloc(std::max<uint64_t>(s->P, 1));
-    s->b_nw.alloc(std::max<uint32_t>(s->m, 1));
-    fill_async<uint32_t>(s->b_noff.p + s->m, uint32_t(s->P), 1, st);
-  }
-  if (m) {
-    ip_net_scatter_kernel<<<grid_for(m, 256, 65536), 256, 0, st>>>(
-        rb.side.p, rb.best.p, n, h.noff, h.pins, h.nw, m, newid.p, off0.p, off1.p, eid0.p, eid1.p, a.b_noff.p,
-        b.b_noff.p, a.b_pins.p, b.b_pins.p, a.b_nw.p, b.b_nw.p);
-    MKP_LAUNCH_CHECK();
-  }
-  for (IpSub* s : {&a, &b}) {
-    s->b_voff.alloc(size_t(s->n) + 1);
-    s->b_inc.alloc(std::max<uint64_t>(s->P, 1));
-    DevBuf<uint32_t> deg(size_t(s->n) + 1);
-    fill_async<uint32_t>(deg.p, 0u, size_t(s->n) + 1, st);
-    if (s->P) {
-      ip_degree_kernel<<<grid_for(s->P, 256, 65536), 256, 0, st>>>(s->b_pins.p, s->P, deg.p);
+  for (size_t i = 0; i < T; ++i) {
+    IpTask& t = *tasks[i];
+    const IpSub& h = *t.h;
+    IpSplit& sp = t.sp;
+    const uint32_t n = h.n, m = h.m;
+    IpSub &a = *t.half[0], &b = *t.half[1];
+    const int64_t* hi = hs + 8 * i;
+    const uint32_t* hi32 = reinterpret_cast<const uint32_t*>(hi + 4);
+    a.max_esize = b.max_esize = h.max_esize;
+    a.n = uint32_t(hi[0]);
+    b.n = n - a.n;
+    a.W = hi[1];
+    b.W = hi[2];
+    a.P = hi32[0];
+    b.P = hi32[1];
+    a.m = hi32[2];
+    b.m = hi32[3];
+    for (IpSub* s : {&a, &b}) {
+      s->b_noff.alloc(size_t(s->m) + 1);
+      s->b_pins.alloc(std::max<uint64_t>(s->P, 1));
+      s->b_nw.alloc(std::max<uint32_t>(s->m, 1));
+      fill_async<uint32_t>(s->b_noff.p + s->m, uint32_t(s->P), 1, st);
+    }
+    if (m) {
+      ip_net_scatter_kernel<<<grid_for(m, 256, 65536), 256, 0, st>>>(
+          t.rb.side.p, best + i, n, h.noff, h.pins, h.nw, m, sp.newid.p, sp.off0.p, sp.off1.p, sp.eid0.p,
+          sp.eid1.p, a.b_noff.p, b.b_noff.p, a.b_pins.p, b.b_pins.p, a.b_nw.p, b.b_nw.p);
       MKP_LAUNCH_CHECK();
     }
-    exclusive_scan_u32(c, deg.p, s->b_voff.p, s->n);
-    fill_async<uint32_t>(deg.p, 0u, size_t(s->n) + 1, st);
-    if (s->m) {
-      ip_incidence_kernel<<<grid_for(s->m, 256, 65536), 256, 0, st>>>(s->b_noff.p, s->b_pins.p, s->m, s->b_voff.p,
-                                                                       deg.p, s->b_inc.p);
-      MKP_LAUNCH_CHECK();
+    for (IpSub* s : {&a, &b}) {
+      s->b_voff.alloc(size_t(s->n) + 1);
+      s->b_inc.alloc(std::max<uint64_t>(s->P, 1));
+      DevBuf<uint32_t> deg(size_t(s->n) + 1);
+      fill_async<uint32_t>(deg.p, 0u, size_t(s->n) + 1, st);
+      if (s->P) {
+        ip_degree_kernel<<<grid_for(s->P, 256, 65536), 256, 0, st>>>(s->b_pins.p, s->P, deg.p);
+        MKP_LAUNCH_CHECK();
+      }
+      exclusive_scan_u32(c, deg.p, s->b_voff.p, s->n);
+      fill_async<uint32_t>(deg.p, 0u, size_t(s->n) + 1, st);
+      if (s->m) {
+        ip_incidence_kernel<<<grid_for(s->m, 256, 65536), 256, 0, st>>>(s->b_noff.p, s->b_pins.p, s->m,
+                                                                         s->b_voff.p, deg.p, s->b_inc.p);
+        MKP_LAUNCH_CHECK();
+      }
+      s->noff = s->b_noff.p;
+      s->pins = s->b_pins.p;
+      s->voff = s->b_voff.p;
+      s->inc = s->b_inc.p;
+      s->vw = s->b_vw.p;
+      s->nw = s->b_nw.p;
+      s->orig = s->b_orig.p;
     }
-    s->noff = s->b_noff.p;
-    s->pins = s->b_pins.p;
-    s->voff = s->b_voff.p;
-    s->inc = s->b_inc.p;
-    s->vw = s->b_vw.p;
-    s->nw = s->b_nw.p;
-    s->orig = s->b_orig.p;
   }
+}
+
+std::unique_ptr<IpSub> root_of(mkp_hypergraph* hg) {
+  auto r = std::make_unique<IpSub>();
+  r->n = hg->n; r->m = hg->m; r->P = hg->p; r->W = hg->total_w; r->max_esize = hg->max_esize;
+  r->noff = hg->noff.p; r->pins = hg->pins.p; r->voff = hg->voff.p; r->inc = hg->inc.p;
+  r->vw = hg->vw.p; r->nw = hg->nw.p;
+  return r;
 }
 
 }  // namespace
@@ -789,15 +1179,16 @@ uint64_t ip_stream(uint64_t seed, int first, int k) {
 void ip_bisect_runs(mkp_hypergraph* hg, int64_t max0, int64_t max1, int64_t target0, uint64_t stream, int runs,
                     int passes, uint8_t* sides, int64_t* results) {
   Context& c = hg->ctx->c;
-  IpSub h;
-  h.n = hg->n; h.m = hg->m; h.P = hg->p; h.W = hg->total_w;
-  h.noff = hg->noff.p; h.pins = hg->pins.p; h.voff = hg->voff.p; h.inc = hg->inc.p;
-  h.vw = hg->vw.p; h.nw = hg->nw.p;
-  if (h.n == 0) return;
-  IpRuns rb;
-  launch_runs(c, h, max0, max1, target0, stream, runs, passes, rb);
-  MKP_HIP(hipMemcpyAsync(sides, rb.side.p, size_t(runs) * h.n, hipMemcpyDeviceToHost, c.stream));
-  MKP_HIP(hipMemcpyAsync(results, rb.out.p, size_t(runs) * 4 * 8, hipMemcpyDeviceToHost, c.stream));
+  if (hg->n == 0) return;
+  IpTask t;
+  t.h = root_of(hg);
+  prepare(c, t, max0, max1, target0, stream, runs, passes, hg->n <= kIpAdjN);
+  std::vector<IpTask*> tasks{&t};
+  DevBuf<IpArgs> dargs;
+  DevBuf<int32_t> best;
+  run_level(c, tasks, runs, dargs, best);
+  MKP_HIP(hipMemcpyAsync(sides, t.rb.side.p, size_t(runs) * hg->n, hipMemcpyDeviceToHost, c.stream));
+  MKP_HIP(hipMemcpyAsync(results, t.rb.out.p, size_t(runs) * 4 * 8, hipMemcpyDeviceToHost, c.stream));
   c.sync();
 }
 
@@ -810,37 +1201,56 @@ void initial_partition(mkp_hypergraph* hg, int k, const std::vector<int64_t>& ma
   int levels = 0;
   while ((1 << levels) < k) ++levels;
   const double eps_level = levels > 0 ? std::pow(1.0 + std::max(eps, 0.0), 1.0 / levels) - 1.0 : eps;
-  struct Node {
-    std::unique_ptr<IpSub> h;
-    int k, first;
-  };
-  std::vector<Node> stack;
+  runs = std::max(1, runs);
+  // the recursion one level at a time: every bisection of a level runs in
+  // the same launches
+  std::vector<std::unique_ptr<IpTask>> level;
   {
-    auto root = std::make_unique<IpSub>();
-    root->n = hg->n; root->m = hg->m; root->P = hg->p; root->W = W;
-    root->noff = hg->noff.p; root->pins = hg->pins.p; root->voff = hg->voff.p; root->inc = hg->inc.p;
-    root->vw = hg->vw.p; root->nw = hg->nw.p;
-    stack.push_back(Node{std::move(root), k, 0});
+    auto t = std::make_unique<IpTask>();
+    t->h = root_of(hg);
+    t->k = k;
+    t->first = 0;
+    level.push_back(std::move(t));
   }
-  IpRuns rb;
-  while (!stack.empty()) {
-    Node nd = std::move(stack.back());
-    stack.pop_back();
-    const IpSub& h = *nd.h;
-    if (h.n == 0) continue;
-    if (nd.k == 1) {
-      ip_assign_kernel<<<grid_for(h.n, 256, 65536), 256, 0, c.stream>>>(h.orig, h.n, nd.first, part);
-      MKP_LAUNCH_CHECK();
-      continue;
+  DevBuf<IpArgs> dargs;
+  DevBuf<int32_t> best;
+  while (!level.empty()) {
+    std::vector<IpTask*> active;
+    // clique scores for the bisections of up to kIpAdjN vertices while the
+    // level's matrices fit kIpAdjBytes (the others grow by walking the nets,
+    // with the same result)
+    uint64_t adj_bytes = 0;
+    for (auto& t : level) {
+      const IpSub& h = *t->h;
+      if (h.n == 0) continue;
+      if (t->k == 1) {
+        ip_assign_kernel<<<grid_for(h.n, 256, 65536), 256, 0, c.stream>>>(h.orig, h.n, t->first, part);
+        MKP_LAUNCH_CHECK();
+        continue;
+      }
+      int64_t max0, max1, target0;
+      ip_side_limits(h.W, t->k, eps_level, &max0, &max1, &target0);
+      const uint64_t ab = uint64_t(h.n) * h.n * 8;
+      const bool use_adj = h.n <= kIpAdjN && adj_bytes + ab <= kIpAdjBytes;
+      if (use_adj) adj_bytes += ab;
+      prepare(c, *t, max0, max1, target0, ip_stream(seed, t->first, t->k), runs, passes, use_adj);
+      active.push_back(t.get());
     }
-    const int k0 = (nd.k + 1) / 2, k1 = nd.k - k0;
-    int64_t max0, max1, target0;
-    ip_side_limits(h.W, nd.k, eps_level, &max0, &max1, &target0);
-    launch_runs(c, h, max0, max1, target0, ip_stream(seed, nd.first, nd.k), std::max(1, runs), passes, rb);
-    auto a = std::make_unique<IpSub>(), b = std::make_unique<IpSub>();
-    split(c, h, rb, *a, *b);
-    stack.push_back(Node{std::move(b), k1, nd.first + k0});
-    stack.push_back(Node{std::move(a), k0, nd.first});
+    if (active.empty()) break;
+    run_level(c, active, runs, dargs, best);
+    split_level(c, active, best.p);
+    std::vector<std::unique_ptr<IpTask>> next;
+    for (IpTask* t : active) {
+      const int k0 = (t->k + 1) / 2, k1 = t->k - k0;
+      for (int s2 = 0; s2 < 2; ++s2) {
+        auto ch = std::make_unique<IpTask>();
+        ch->h = std::move(t->half[s2]);
+        ch->k = s2 == 0 ? k0 : k1;
+        ch->first = s2 == 0 ? t->first : t->first + k0;
+        next.push_back(std::move(ch));
+      }
+    }
+    level.swap(next);  // the finished level's buffers go back to the allocation cache
   }
   // a block still over its bound gives its lightest vertices to the
   // lightest blocks they fit (host: the coarsest hypergraph is small)

@@ -136,6 +136,14 @@ struct mkp_server {
 
 extern "C" {
 
+namespace {
+// device bytes of one initial-partitioning run on a coarsest level of at
+// most n vertices and m nets
+double ip_run_bytes(uint32_t n, uint32_t m, uint32_t max_net_size) {
+  return (max_net_size > 0xFFFFu ? 8.0 : 4.0) * double(m) + 22.0 * double(n);
+}
+}  // namespace
+
 int mkp_request_estimate(uint32_t n, uint32_t m, uint64_t p, uint32_t max_net_size, int32_t k, uint64_t* bytes) {
   return guarded([&] {
     MKP_CHECK(bytes, "null argument");
@@ -156,9 +164,15 @@ int mkp_request_estimate(uint32_t n, uint32_t m, uint64_t p, uint32_t max_net_si
     const double coarsen = hyper + 96.0 * P + 64.0 * N;
     // partition state and label propagation work at the finest level
     const double partition = 8.0 * E * double(L.stride) + 4.0 * N + 80.0 * N + 24.0 * P;
+    // initial partitioning on the coarsest level (at most the input), one
+    // recursion level at a time: per run pin counts of every net and
+    // per-vertex state, the level's clique scores (at most 256 MB), the
+    // halves of the level's sub-hypergraphs and their split temporaries
+    const double ip = ip_run_bytes(n, m, max_net_size) * 20.0 + std::min(8.0 * N * N, 268435456.0) + 16.0 * P +
+                      48.0 * E;
     // with degree-zero vertices or large nets set aside, the multilevel phase
     // runs on a reduced copy of the hypergraph
-    const double need = hyper + std::max(louvain, hyper + coarsen + partition);
+    const double need = hyper + std::max(louvain, hyper + coarsen + std::max(partition, ip));
     // every buffer rounds up to a power of two in the allocation cache
     *bytes = uint64_t(2.0 * need) + (uint64_t(64) << 20);
   });
@@ -228,6 +242,8 @@ int mkp_server_submit(mkp_server* s, const mkp_host_hypergraph* h, const mkp_par
     for (uint32_t e = 0; e < h->h.m; ++e) me = std::max<uint32_t>(me, uint32_t(h->h.noff[e + 1] - h->h.noff[e]));
     MKP_CHECK(mkp_request_estimate(h->h.n, h->h.m, h->h.pins.size(), me, prm->k, &r->estimate) == 0,
               mkp_last_error());
+    if (prm->ip_runs > 20)  // the estimate assumes the default 20 runs per bisection
+      r->estimate += uint64_t(2.0 * ip_run_bytes(h->h.n, h->h.m, me) * double(prm->ip_runs - 20));
     std::lock_guard<std::mutex> g(s->mu);
     MKP_CHECK(r->estimate <= s->budget, "request needs more device memory than the whole budget");
     MKP_CHECK(!s->stop, "server is stopping");

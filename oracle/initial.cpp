@@ -18,12 +18,16 @@
 // Every rule is stated in integers and stateless hashes so that one workgroup
 // per run on the device computes exactly the same sides:
 //   * run r of a bisection uses variant r % 3: greedy growing, BFS, random
-//     fill; all `runs` runs are made and the balanced run with the smallest
-//     cut wins (ties: the smaller run index);
+//     fill (r % 2 above 2048 vertices: a random start needs about n / 4
+//     refinement moves there and never wins); all `runs` runs are made and
+//     the balanced run with the smallest cut wins (ties: the smaller run
+//     index);
 //   * greedy growing scores a vertex by the sum over its nets of at most 100
-//     pins of floor(w(e) * 2^16 / (|e| - 1)) over the nets shared with side 0
-//     (64-bit wrap-around sums, so the order of the additions is free) and
-//     takes the highest score, ties to the smaller id;
+//     pins of floor(w(e) * 2^16 / (|e| - 1)), once per side-0 pin of the net
+//     (64-bit wrap-around sums, so the order of the additions is free), and
+//     takes the min(256, max(1, n / 2048)) highest non-zero scores at a time
+//     (ties to the smaller id), in that order while side 0 is below its
+//     target;
 //   * BFS is level-synchronous: a level's vertices join side 0 in ascending
 //     id order, the next level is every unseen pin of an unscanned net of at
 //     most 100 pins incident to a vertex that joined;
@@ -68,37 +72,51 @@ struct Bisection {
   void grow_greedy(uint64_t rk) {
     const uint32_t n = h.n;
     side.assign(n, 1);
-    std::vector<uint8_t> skip(n, 0), in(n, 0);
+    std::vector<uint8_t> skip(n, 0);
     std::vector<uint64_t> key(n, 0);
     const uint32_t start = ip_start(rk, n);
     uint32_t scan = 0;
     int64_t w0 = 0;
+    const uint32_t batch = ip_grow_batch(n);
+    std::vector<uint8_t> picked(n, 0);
+    std::vector<uint32_t> picks, added;
     while (w0 < target0) {
-      int64_t v = -1;
-      for (uint32_t x = 0; x < n; ++x)  // highest score, ties to the smaller id
-        if (in[x] && (v < 0 || key[x] > key[v])) v = x;
-      while (v < 0 && scan < n) {
+      // the `batch` highest non-zero scores, ties to the smaller id
+      picks.clear();
+      while (picks.size() < batch) {
+        int64_t v = -1;
+        for (uint32_t x = 0; x < n; ++x)
+          if (side[x] == 1 && !skip[x] && !picked[x] && key[x] != 0 && (v < 0 || key[x] > key[v])) v = x;
+        if (v < 0) break;
+        picked[v] = 1;
+        picks.push_back(uint32_t(v));
+      }
+      for (uint32_t v : picks) picked[v] = 0;
+      while (picks.empty() && scan < n) {  // a seed: the next free vertex after `start`
         const uint32_t x = (start + scan++) % n;
-        if (side[x] == 1 && !skip[x]) v = x;
+        if (side[x] == 1 && !skip[x]) picks.push_back(x);
       }
-      if (v < 0) break;
-      in[v] = 0;
-      if (w0 + h.vw[v] > max_w[0]) {
-        skip[v] = 1;
-        continue;
+      if (picks.empty()) break;
+      added.clear();
+      for (uint32_t v : picks) {
+        if (w0 >= target0) break;
+        if (w0 + h.vw[v] > max_w[0]) {
+          skip[v] = 1;
+          continue;
+        }
+        side[v] = 0;
+        w0 += h.vw[v];
+        added.push_back(v);
       }
-      side[v] = 0;
-      w0 += h.vw[v];
-      for (uint64_t i = h.voff[v]; i < h.voff[v + 1]; ++i) {
-        const uint32_t e = h.inc[i];
-        const uint64_t sz = h.noff[e + 1] - h.noff[e];
-        if (sz > kGrowNetLimit) continue;
-        const uint64_t add = (uint64_t(uint32_t(h.nw[e])) << kScoreFrac) / std::max<uint64_t>(sz - 1, 1);
-        for (uint64_t j = h.noff[e]; j < h.noff[e + 1]; ++j) {
-          const uint32_t x = h.pins[j];
-          if (side[x] == 1 && !skip[x]) {
-            key[x] += add;
-            in[x] = 1;
+      for (uint32_t v : added) {
+        for (uint64_t i = h.voff[v]; i < h.voff[v + 1]; ++i) {
+          const uint32_t e = h.inc[i];
+          const uint64_t sz = h.noff[e + 1] - h.noff[e];
+          if (sz > kGrowNetLimit) continue;
+          const uint64_t add = (uint64_t(uint32_t(h.nw[e])) << kScoreFrac) / std::max<uint64_t>(sz - 1, 1);
+          for (uint64_t j = h.noff[e]; j < h.noff[e + 1]; ++j) {
+            const uint32_t x = h.pins[j];
+            if (side[x] == 1 && !skip[x]) key[x] += add;
           }
         }
       }
@@ -341,6 +359,9 @@ uint32_t ip_perm(uint64_t key, uint32_t n, uint32_t i) {
   return x;
 }
 
+int ip_variant(int r, uint32_t n) { return n > kIpRandomMaxN ? r % 2 : r % 3; }
+uint32_t ip_grow_batch(uint32_t n) { return std::min<uint32_t>(256, std::max<uint32_t>(1, n / 2048)); }
+
 uint64_t ip_stream(uint64_t seed, int first, int k) {
   return mix64(seed ^ mix64((uint64_t(uint32_t(first)) << 32) | uint32_t(k)));
 }
@@ -369,8 +390,9 @@ IpRun ip_bisect_run(const Hypergraph& h, int64_t max0, int64_t max1, int64_t tar
                     int passes) {
   Bisection b(h, max0, max1, target0);
   const uint64_t rk = ip_run_key(stream, r);
-  if (r % 3 == 0) b.grow_greedy(rk);
-  else if (r % 3 == 1) b.grow_bfs(rk);
+  const int variant = ip_variant(r, h.n);
+  if (variant == 0) b.grow_greedy(rk);
+  else if (variant == 1) b.grow_bfs(rk);
   else b.random_fill(rk);
   b.refine(passes, rk);
   IpRun out;

@@ -1,8 +1,10 @@
 """Device initial partitioning (csrc/initial.hip) against the oracle's
 restatement (oracle/initial.cpp): every run of a bisection (side of every
 vertex, cut, balance, side weights) and the whole recursive bisection
-(block of every vertex, km1) are bit-exact, with per-vertex state in LDS and,
-past 8,192 vertices, in global memory."""
+(block of every vertex, km1) are bit-exact: growing from precomputed clique
+scores (up to 4,096 vertices) and by walking the nets, per-vertex state in
+LDS (up to 8,192 vertices), sides in LDS with scores in global memory (up to
+32,768) and all in global memory, 16-bit and 32-bit pin counts."""
 import math
 
 import numpy as np
@@ -33,7 +35,9 @@ CASES = {
     # above 100 pins do not steer growing
     "small": (500, 800, 150, False, 1),
     "weighted": (3000, 3600, 300, True, 2),
-    "global": (9500, 9000, 60, True, 3),  # per-vertex state beyond LDS
+    "mid": (6000, 7000, 200, True, 6),  # growing walks the nets (no clique-score matrix)
+    "global": (9500, 9000, 60, True, 3),  # scores and gains in global memory
+    "huge": (40000, 36000, 60, True, 8),  # all per-vertex state in global memory
     "dense": (300, 6000, 40, False, 4),
 }
 
@@ -61,6 +65,30 @@ def test_bisection_runs_equal_oracle(ctx, case, limits):
         assert np.array_equal(sides[r], side), "run %d" % r
         assert list(res[r]) == [cut, int(bal), w0, W - w0], "run %d" % r
     dh.close(); h.close()
+
+
+def test_bisection_runs_with_a_net_of_more_than_65535_pins(ctx):
+    # pin counts no longer fit 16 bits: the two-word count layout
+    n = 70000
+    rng = np.random.default_rng(5)
+    nets = [rng.choice(n, size=int(rng.integers(2, 9)), replace=False) for _ in range(30000)]
+    nets.append(np.arange(0, n, 1)[:66000])
+    noff = np.zeros(len(nets) + 1, np.uint64)
+    noff[1:] = np.cumsum([len(e) for e in nets])
+    pins = np.concatenate(nets).astype(np.uint32)
+    vw = rng.integers(1, 4, n).astype(np.int32)
+    nw = np.ones(len(nets), np.int32)
+    W = int(vw.sum())
+    max0, max1, target0 = W // 40, W, W // 50
+    stream = orc.ip_stream(9, 0, 2)
+    dh = mtk_amd.Hypergraph(ctx, n=n, noff=noff, pins=pins, vw=vw, nw=nw)
+    sides, res = dh.ip_bisect_runs(max0, max1, target0, stream, runs=6, passes=2)
+    oh = orc.Hypergraph(n, noff, pins, vw, nw)
+    for r in range(6):
+        side, cut, bal, w0 = oh.ip_bisect_run(max0, max1, target0, stream, r, passes=2)
+        assert np.array_equal(sides[r], side), "run %d" % r
+        assert list(res[r]) == [cut, int(bal), w0, W - w0], "run %d" % r
+    dh.close()
 
 
 def test_bisection_single_vertex_and_no_nets(ctx):

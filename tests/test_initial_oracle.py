@@ -208,3 +208,32 @@ def test_recursion_balanced_deterministic_and_thread_independent(k):
     assert ref["km1"] == km1
     p2, _ = oh.initial_partition(k, [maxw] * k, 8, threads=4)
     assert not np.array_equal(p1, p2)
+
+
+def test_greedy_growing_takes_n_over_2048_vertices_per_step():
+    # 4096 vertices: two picks per step.  From 0, the scores are 1: 3 * 2^16,
+    # 2: 2 * 2^16, 3: 2^16; one pick per step would take 1 and then 5 (its
+    # net of weight 100 makes 5 the best), two picks take 1 and 2 together
+    n = 4096
+    nets = [[0, 1], [0, 2], [0, 3], [1, 5]]
+    h = hg(n, nets, nw=[3, 2, 1, 100])
+    s = stream_with_start(0, n, 0)
+    side, _, _, w0 = h.ip_bisect_run(3, n, 3, s, 0, passes=0)
+    assert set(np.flatnonzero(side == 0).tolist()) == {0, 1, 2} and w0 == 3
+    h1 = hg(2000, nets, nw=[3, 2, 1, 100])
+    s1 = stream_with_start(0, 2000, 0)
+    side, _, _, _ = h1.ip_bisect_run(3, 2000, 3, s1, 0, passes=0)
+    assert set(np.flatnonzero(side == 0).tolist()) == {0, 1, 5}
+
+
+def test_no_random_fill_above_2048_vertices():
+    # run 2 fills at random up to 2048 vertices and grows greedily beyond
+    for n, random in ((2048, True), (2049, False)):
+        vw = np.ones(n, np.int32)
+        h = hg(n, [[i, i + 1] for i in range(n - 1)], vw=vw)
+        target = n // 2
+        s = 4242
+        side, _, _, _ = h.ip_bisect_run(target, n, target, s, 2, passes=0)
+        order = orc.ip_perm(mix64(orc.ip_run_key(s, 2) ^ 0x13198A2E03707344), n)
+        fill = set(order[:target].tolist())
+        assert (set(np.flatnonzero(side == 0).tolist()) == fill) == random

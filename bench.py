@@ -146,12 +146,14 @@ def phases_of(ctx, dh, w, arrays, cpu, threads):
     time per phase (Louvain and LP: the baseline step's median; coarsening:
     the reference-semantics rater's throughput on a sample of level-0
     vertices, scaled to the pipeline's rating work; initial partitioning:
-    host code shared by both paths)."""
+    the CPU restatement of the device partitioner on the coarsest level of a
+    coarsening of the same hypergraph)."""
     import oracle as orc
     ctx.set_profiling(True)
     ctx.reset_profile()
     part, st = dh.partition(w["k"], w["eps"], w["seed"])
     fam = ctx.profile()
+    kern = ctx.kernel_profile()
     ctx.set_profiling(False)
     ctx.reset_profile()
     n = arrays["n"]
@@ -162,6 +164,25 @@ def phases_of(ctx, dh, w, arrays, cpu, threads):
     rate = vis / max(time.perf_counter() - t0, 1e-9)
     rating = fam["rating"]
     contraction = fam["contraction"]
+    ip_cpu_ms, ip_sample = None, None
+    if cpu:
+        # the oracle's initial partitioning (same rules, host threads) on the
+        # coarsest level of the device coarsening of the same hypergraph
+        comm, _ = dh.louvain(seed=w["seed"])
+        dh.set_communities(comm)
+        W = int(arrays["vw"].sum())
+        climit = 160 * w["k"]
+        hi = dh.coarsen(contraction_limit=climit, max_node_weight=int(-(-W // climit)), seed=w["seed"])
+        ca = hi.level(hi.num_levels() - 1).arrays()
+        cn = len(ca["vw"])
+        maxw = int((1.0 + w["eps"]) * -(-W // w["k"]))
+        coh = orc.Hypergraph(cn, ca["noff"], ca["pins"], ca["vw"], ca["nw"])
+        t0 = time.perf_counter()
+        coh.initial_partition(w["k"], [maxw] * w["k"], w["seed"], threads=threads)
+        ip_cpu_ms = round(1e3 * (time.perf_counter() - t0), 1)
+        ip_sample = "oracle initial partitioning (20 runs per bisection, %d threads) of a %d-vertex, %d-net " \
+                    "coarsest level" % (threads, cn, len(ca["nw"]))
+        hi.close()
     out = {
         "workload_note": "one multilevel pipeline run of the same hypergraph (k=%d, eps=%g, seed %d): km1 %d, "
                          "imbalance %.4f, %d levels" % (w["k"], w["eps"], w["seed"], st["km1"], st["imbalance"],
@@ -180,8 +201,10 @@ def phases_of(ctx, dh, w, arrays, cpu, threads):
                        "cpu_sample": "reference-semantics rating of every 50th level-0 vertex on %d threads: %d pin "
                                      "visits at %.3g visits/s, scaled to the pipeline's %d rating visits" % (
                                          threads, vis, rate, st["rating_visits"])},
-        "initial_partitioning": {"host_ms": round(st["initial_ms"], 2),
-                                 "note": "host code shared by the device pipeline and the CPU restatement"},
+        "initial_partitioning": {"gpu_ms": round(st["initial_ms"], 2),
+                                 "kernel_ms": {k: round(v["ms"], 2) for k, v in kern.items()
+                                               if k.startswith("ip_") and v["launches"]},
+                                 "cpu_ms": ip_cpu_ms, "cpu_sample": ip_sample},
         "lp": {"gpu_ms_all_levels": round(st["refinement_ms"], 2),
                "cpu_ms_finest_level": round(1e3 * cpu["median"]["lp_s"], 1) if cpu else None},
         "total_gpu_pipeline_ms": round(st["total_ms"], 2),

@@ -6,7 +6,7 @@
 // Every bisection makes all of its runs at once, one workgroup of 1024
 // threads per run, and keeps the balanced run with the smallest cut (ties to
 // the smaller run index).  Run r grows side 0 by variant r % 3 (greedy
-// growing, level-synchronous BFS, random fill; r % 2 above 2048 vertices)
+// growing, level-synchronous BFS, random fill; r % 2 above 1024 vertices)
 // and then improves the bisection with passes of 2-way gain moves.
 //
 // The steps of a run are sequential -- growing one vertex at a time and
@@ -69,7 +69,7 @@ constexpr uint8_t kSeen = 4;  // BFS: reached
 constexpr uint8_t kNext = 8;  // BFS: in the next level
 constexpr uint8_t kCand = 16; // repair: candidate not yet visited
 constexpr uint8_t kPick = 32; // growing: taken in the current step
-constexpr uint32_t kIpRandomMaxN = 2048;  // random-fill runs up to this many vertices
+constexpr uint32_t kIpRandomMaxN = 1024;  // random-fill runs up to this many vertices
 constexpr uint32_t kIpMaxBatch = 256;     // growing takes min(256, max(1, n / 2048)) vertices per step
 
 // Pin counts of a net on side 0 / 1 in one 32-bit word (16 bits each) when
@@ -922,6 +922,40 @@ __global__ void ip_incidence_kernel(const uint32_t* noff, const uint32_t* pins, 
       inc[voff[x] + atomicAdd(&cursor[x], 1u)] = e;
     }
 }
+// The same two steps for up to kIpLdsN vertices, counting in LDS: every
+// workgroup takes one range of nets, counts its pins per vertex, reserves one
+// range per vertex with a single atomic and places its nets from LDS
+// cursors (a coarse level has few vertices, so per-pin atomics on the global
+// counters would all contend).
+__global__ __launch_bounds__(kIpThreads) void ip_degree_lds_kernel(const uint32_t* pins, uint64_t P, uint32_t n,
+                                                                   uint32_t* deg) {
+  __shared__ uint32_t h[kIpLdsN];
+  for (uint32_t x = threadIdx.x; x < n; x += kIpThreads) h[x] = 0;
+  __syncthreads();
+  for (uint64_t j = blockIdx.x * uint64_t(kIpThreads) + threadIdx.x; j < P; j += uint64_t(gridDim.x) * kIpThreads)
+    atomicAdd(&h[pins[j]], 1u);
+  __syncthreads();
+  for (uint32_t x = threadIdx.x; x < n; x += kIpThreads)
+    if (h[x]) atomicAdd(&deg[x], h[x]);
+}
+__global__ __launch_bounds__(kIpThreads) void ip_incidence_lds_kernel(const uint32_t* noff, const uint32_t* pins,
+                                                                      uint32_t m, uint32_t n, const uint32_t* voff,
+                                                                      uint32_t* cursor, uint32_t* inc) {
+  __shared__ uint32_t h[kIpLdsN];
+  const uint32_t chunk = (m + gridDim.x - 1) / gridDim.x;
+  const uint32_t e0 = min(m, blockIdx.x * chunk), e1 = min(m, e0 + chunk);
+  for (uint32_t x = threadIdx.x; x < n; x += kIpThreads) h[x] = 0;
+  __syncthreads();
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += kIpThreads)
+    for (uint32_t j = noff[e]; j < noff[e + 1]; ++j) atomicAdd(&h[pins[j]], 1u);
+  __syncthreads();
+  for (uint32_t x = threadIdx.x; x < n; x += kIpThreads)
+    if (h[x]) h[x] = voff[x] + atomicAdd(&cursor[x], h[x]);
+  __syncthreads();
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += kIpThreads)
+    for (uint32_t j = noff[e]; j < noff[e + 1]; ++j) inc[atomicAdd(&h[pins[j]], 1u)] = e;
+}
+
 __global__ void ip_assign_kernel(const uint32_t* orig, uint32_t n, int32_t block, int32_t* part) {
   for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n; x += gridDim.x * blockDim.x)
     part[orig ? orig[x] : x] = block;
@@ -1131,15 +1165,22 @@ void split_level(Context& c, std::vector<IpTask*>& tasks, const int32_t* best) {
       s->b_inc.alloc(std::max<uint64_t>(s->P, 1));
       DevBuf<uint32_t> deg(size_t(s->n) + 1);
       fill_async<uint32_t>(deg.p, 0u, size_t(s->n) + 1, st);
+      const bool lds = s->n <= kIpLdsN;
+      const unsigned lds_blocks = unsigned(std::min<uint64_t>(2 * c.num_cus, (s->P + 8191) / 8192 + 1));
       if (s->P) {
-        ip_degree_kernel<<<grid_for(s->P, 256, 65536), 256, 0, st>>>(s->b_pins.p, s->P, deg.p);
+        if (lds) ip_degree_lds_kernel<<<lds_blocks, kIpThreads, 0, st>>>(s->b_pins.p, s->P, s->n, deg.p);
+        else ip_degree_kernel<<<grid_for(s->P, 256, 65536), 256, 0, st>>>(s->b_pins.p, s->P, deg.p);
         MKP_LAUNCH_CHECK();
       }
       exclusive_scan_u32(c, deg.p, s->b_voff.p, s->n);
       fill_async<uint32_t>(deg.p, 0u, size_t(s->n) + 1, st);
       if (s->m) {
-        ip_incidence_kernel<<<grid_for(s->m, 256, 65536), 256, 0, st>>>(s->b_noff.p, s->b_pins.p, s->m,
-                                                                         s->b_voff.p, deg.p, s->b_inc.p);
+        if (lds)
+          ip_incidence_lds_kernel<<<lds_blocks, kIpThreads, 0, st>>>(s->b_noff.p, s->b_pins.p, s->m, s->n,
+                                                                     s->b_voff.p, deg.p, s->b_inc.p);
+        else
+          ip_incidence_kernel<<<grid_for(s->m, 256, 65536), 256, 0, st>>>(s->b_noff.p, s->b_pins.p, s->m,
+                                                                           s->b_voff.p, deg.p, s->b_inc.p);
         MKP_LAUNCH_CHECK();
       }
       s->noff = s->b_noff.p;

@@ -18,7 +18,7 @@
 // Every rule is stated in integers and stateless hashes so that one workgroup
 // per run on the device computes exactly the same sides:
 //   * run r of a bisection uses variant r % 3: greedy growing, BFS, random
-//     fill (r % 2 above 2048 vertices: a random start needs about n / 4
+//     fill (r % 2 above 1024 vertices: a random start needs about n / 4
 //     refinement moves there and never wins); all `runs` runs are made and
 //     the balanced run with the smallest cut wins (ties: the smaller run
 //     index);

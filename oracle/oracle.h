@@ -286,7 +286,7 @@ struct IpRun {
 constexpr uint64_t kIpStartTag = 0x243F6A8885A308D3ull;
 constexpr uint64_t kIpFillTag = 0x13198A2E03707344ull;
 constexpr uint64_t kIpPassTag = 0xA4093822299F31D0ull;
-constexpr uint32_t kIpRandomMaxN = 2048;  // random-fill runs up to this many vertices
+constexpr uint32_t kIpRandomMaxN = 1024;  // random-fill runs up to this many vertices
 uint32_t ip_perm(uint64_t key, uint32_t n, uint32_t i);
 int ip_variant(int r, uint32_t n);        // 0 greedy growing, 1 BFS, 2 random fill
 uint32_t ip_grow_batch(uint32_t n);       // vertices a growing step takes

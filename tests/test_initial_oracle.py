@@ -226,9 +226,9 @@ def test_greedy_growing_takes_n_over_2048_vertices_per_step():
     assert set(np.flatnonzero(side == 0).tolist()) == {0, 1, 5}
 
 
-def test_no_random_fill_above_2048_vertices():
-    # run 2 fills at random up to 2048 vertices and grows greedily beyond
-    for n, random in ((2048, True), (2049, False)):
+def test_no_random_fill_above_1024_vertices():
+    # run 2 fills at random up to 1024 vertices and grows greedily beyond
+    for n, random in ((1024, True), (1025, False)):
         vw = np.ones(n, np.int32)
         h = hg(n, [[i, i + 1] for i in range(n - 1)], vw=vw)
         target = n // 2

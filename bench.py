@@ -56,8 +56,8 @@ BATCH_WORKLOADS = {
 SERVE_WORKLOADS = {
     "serve_poisson_10k-20M": dict(requests=160, p_min=10_000, p_max=20_000_000, exponent=2.3, min_size=2,
                                   max_size=10_000, window=4096, local_fraction=0.8, vertex_weight_max=8,
-                                  ks=(2, 8, 32, 128), eps=0.03, seed=42, slots=8, loads=(0.5, 1.0, 2.0),
-                                  plain_stride=4),
+                                  ks=(2, 8, 32, 128), eps=0.03, seed=42, slots=16, hw_queues=16,
+                                  loads=(0.5, 1.0, 2.0), plain_stride=4),
 }
 
 
@@ -652,7 +652,8 @@ def serve_main(args):
         "data": "synthetic",
         "config": {"workload": args.workload, "requests": R, "pins_total": P, "pins_per_request":
                    "log-uniform [%d, %d]" % (w["p_min"], w["p_max"]), "k": "per request from %s" % (w["ks"],),
-                   "epsilon": w["eps"], "slots": w["slots"], "parallelism": "single GPU"},
+                   "epsilon": w["eps"], "slots": w["slots"],
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), "parallelism": "single GPU"},
         "served": {"saturation_req_per_s": round(sat, 2), "saturation_pins_per_s": round(P / makespan, 1),
                    "plain_req_per_s": round(plain_rate, 2), "plain_pins_per_s": round(plain_pins, 1),
                    "speedup_vs_plain": round(sat / plain_rate, 2),
@@ -694,6 +695,18 @@ def main():
         batch_main(args)
         return
     if args.workload in SERVE_WORKLOADS:
+        # every slot owns a stream; HIP maps the streams of a process onto
+        # GPU_MAX_HW_QUEUES hardware queues (4 by default), and streams that
+        # share a queue run one after the other.  Raised to the workload's
+        # count before the first HIP call of the process (the runtime reads
+        # it when it starts).
+        want = SERVE_WORKLOADS[args.workload]["hw_queues"]
+        try:
+            have = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+        except ValueError:
+            have = 4
+        if have < want:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(want)
         serve_main(args)
         return
 

@@ -26,7 +26,12 @@
 namespace mkp {
 
 constexpr uint32_t kHighDegree = 100000;   // partitioned_hypergraph.h:67
-constexpr uint32_t kThreadDegreeLimit = 512;
+// thread-per-vertex gains (k <= 32) up to this degree: a wave runs as long
+// as its heaviest vertex, and at coarse levels degrees spread from tens to
+// thousands, so heavier vertices take a wave each
+constexpr uint32_t kThreadDegreeLimit = 64;
+// group-per-vertex gains (32 < k <= 128) up to this degree
+constexpr uint32_t kGroupDegreeLimit = 512;
 constexpr int kGainThreads = 128;
 constexpr int kLpStripes = 32, kLpStripeWords = 16;  // striped counters (see stripe_add)
 
@@ -328,7 +333,7 @@ lp_gain_thread_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
 }
 
 // ------------------------------------------------------------ gain (G) --
-// G lanes per vertex for 32 < k <= KMAX and degree <= kThreadDegreeLimit:
+// G lanes per vertex for 32 < k <= KMAX and degree <= kGroupDegreeLimit:
 // the lanes take the incident nets and subtract w(e) from the LDS score of
 // every block in the net's connectivity set; the Km1Policy scan with its coin
 // flips is evaluated as group reductions exactly as in lp_gain_wave_kernel.
@@ -363,7 +368,7 @@ lp_gain_group_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na,
       live = false;
     }
     const uint32_t jb = live ? a.voff[u] : 0u, je = live ? a.voff[u + 1] : 0u;
-    if (live && je - jb > kThreadDegreeLimit) {
+    if (live && je - jb > kGroupDegreeLimit) {
       if (gl == 0) {
         if (je - jb <= kHighDegree) {
           const uint32_t h = atomicAdd(reinterpret_cast<unsigned long long*>(&a.counters[3]), 1ull);
@@ -992,7 +997,7 @@ lp_bucket_keys_kernel(LpArgs a, const uint32_t* __restrict__ active, uint32_t na
         const uint32_t u = active[i];
         const uint32_t d = a.voff[u + 1] - a.voff[u];
         if (d <= kHighDegree) {
-          if (a.k > 128 || d > kThreadDegreeLimit) { wv = 1; wi = d; } else { li = d; }
+          if (a.k > 128 || d > (a.k > 32 ? kGroupDegreeLimit : kThreadDegreeLimit)) { wv = 1; wi = d; } else { li = d; }
         }
       }
       uint64_t pending = __ballot(b >= 0);

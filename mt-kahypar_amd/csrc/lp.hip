@@ -883,7 +883,14 @@ lp_activate_kernel(LpArgs a, const uint32_t* __restrict__ cu, const uint8_t* __r
       const uint32_t e = a.inc[jj];
       const uint32_t pb = a.noff[e], pe = a.noff[e + 1];
       if (pe - pb <= threshold && atomicExch(&net_stamp[e], stamp) != stamp)
-        for (uint32_t q = pb; q < pe; ++q) flag[a.pins[q]] = 1;
+        for (uint32_t q0 = pb; q0 < pe; q0 += 8) {  // eight pin loads in flight at once
+          uint32_t xs[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) xs[t] = q0 + t < pe ? a.pins[q0 + t] : 0xffffffffu;
+#pragma unroll
+          for (int t = 0; t < 8; ++t)
+            if (xs[t] != 0xffffffffu) flag[xs[t]] = 1;
+        }
     }
   }
   if (lane == 0) stripe_add(a.stripes, 4, moves);

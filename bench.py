@@ -76,6 +76,12 @@ def make_workload(name):
     return w, h
 
 
+def apply_tuning(ctx, args):
+    for kv in args.tuning:
+        key, value = kv.split("=", 1)
+        ctx.set_tuning(key, int(value))
+
+
 def start_partition(n, k):
     # contiguous vertex-id blocks: with id-local pins this is a projected-like
     # starting partition whose block borders carry the LP work
@@ -396,6 +402,7 @@ def batch_main(args):
     log("workload %s: %d members, n=%d m=%d pins=%d (generated in %.1f s)" % (
         args.workload, G, sum(ns), sum(i.num_nets for i in infos), P, time.perf_counter() - t_gen))
     ctx = mtk_amd.Context(0)
+    apply_tuning(ctx, args)
     b = mtk_amd.Batch(ctx, hosts)
     seeds = [w["seed"] + g for g in range(G)]
     part0 = np.concatenate([start_partition(n, k) for n, k in zip(ns, ks)])
@@ -593,6 +600,7 @@ def serve_main(args):
 
     # plain path
     ctx = mtk_amd.Context(0)
+    apply_tuning(ctx, args)
     sample = list(range(0, R, w["plain_stride"]))
     plain_km1 = {}
     for i in sample[:2]:  # warm-up of the plain path
@@ -682,6 +690,9 @@ def main():
                     help="summarise rocprofv3 --pmc output folders of this script (FETCH_SIZE, WRITE_SIZE and "
                          "optionally TCC_ATOMIC_sum runs) into profiles/ and exit")
     ap.add_argument("--no-phases", action="store_true", help="skip the per-phase pipeline run")
+    ap.add_argument("--tuning", action="append", default=[], metavar="KEY=VALUE",
+                    help="kernel shape choices of the context (mkp_context_set_tuning), e.g. "
+                         "louvain_fused_classes=0 gives every degree class of a Louvain sub-round its own launch")
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.workload,
@@ -716,6 +727,7 @@ def main():
     P = len(arrays["pins"])
     log("workload %s: n=%d m=%d pins=%d" % (args.workload, n, len(arrays["noff"]) - 1, P))
     ctx = mtk_amd.Context(0)
+    apply_tuning(ctx, args)
     dh = mtk_amd.Hypergraph(ctx, host=host)
     W = int(arrays["vw"].sum())
     perfect = -(-W // k)

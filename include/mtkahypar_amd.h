@@ -55,11 +55,15 @@ MKP_API const char* mkp_profile_family_name(int family);
  * kernel inside a timed region adds one event pair per launch of it only. */
 MKP_API int mkp_context_set_profile_filter(mkp_context* ctx, const char* kernel_name);
 /* Choices that never change results: "louvain_group_lanes" (lanes per node
- * of the 17..128-arc Louvain class: 8, 16, 32 or 64; 0, the default, lets the
+ * of the 17..64-arc Louvain class: 8, 16, 32 or 64; 0, the default, lets the
  * autotuner time the four on the first sub-rounds of a level and keep the
- * fastest per graph kind, class size and mean degree) and "sleep_wait" (0,
- * the default: host waits on the context's stream spin; 1: they poll for
- * 20 us, then sleep between polls, leaving the core to other threads). */
+ * fastest per graph kind, class size and mean degree),
+ * "louvain_fused_classes" (the Louvain degree classes up to 1024 arcs of a
+ * sub-round in one launch: 1, the default, with the 17..64-arc class when it
+ * has at most 2^21 arcs; 2 never with it; 3 always with it; 0 one launch per
+ * class), "rating_block_passes" and "sleep_wait" (0, the default: host waits
+ * on the context's stream spin; 1: they poll for 20 us, then sleep between
+ * polls, leaving the core to other threads). */
 MKP_API int mkp_context_set_tuning(mkp_context* ctx, const char* key, int64_t value);
 /* Per-kernel records: launches timed one at a time (an event pair around a
  * single kernel), keyed by kernel name and template variant, with their

@@ -392,6 +392,10 @@ struct Tuning {
   // heavy rating proposers are rated by one block each with LDS maps when
   // their keys fit in at most this many passes (0: every one in the arena)
   int rating_block_passes = 4;
+  // the Louvain degree classes up to 1024 arcs of a sub-round share one
+  // launch: 1 with the 17..64-arc class when it is small, 2 never with it,
+  // 3 always with it, 0 one launch per class
+  int lv_fused_classes = 1;
 };
 
 struct Context {

@@ -179,6 +179,9 @@ int mkp_context_set_tuning(mkp_context* ctx, const char* key, int64_t value) {
     } else if (k == "rating_block_passes") {
       MKP_CHECK(value >= 0 && value <= 1024, "rating_block_passes must be in [0, 1024]");
       ctx->c.tune.rating_block_passes = int(value);
+    } else if (k == "louvain_fused_classes") {
+      MKP_CHECK(value >= 0 && value <= 3, "louvain_fused_classes must be 0, 1, 2 or 3");
+      ctx->c.tune.lv_fused_classes = int(value);
     } else if (k == "sleep_wait") {
       MKP_CHECK(value == 0 || value == 1, "sleep_wait must be 0 (spin) or 1 (poll, then sleep)");
       ctx->c.sleep_wait = value != 0;

@@ -272,13 +272,15 @@ __device__ inline void lv_decide(const LvArgs& a, uint32_t pos, const LvNode& nd
 // computeMaxGainCluster, summation order included.  The loads are issued in
 // three independent batches (arcs, clusters of the heads, cluster volumes) so
 // a node waits for three memory round trips, not one per arc.
+// Every move body runs as block `bid` of `nb` blocks (a multiple of 8): on its
+// own in a kernel of its class, or as one segment of a fused launch.
 template <class View>
-__global__ void __launch_bounds__(256)
-louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count) {
+__device__ __forceinline__ void lv_small_body(const View& g, const LvArgs& a, const uint32_t* __restrict__ list,
+                                              uint32_t count, uint32_t bid, uint32_t nb, uint32_t dbase) {
   constexpr int D = 16;
   // one contiguous chunk of the list per block, chunks dealt per XCD
-  const uint32_t per = uint32_t((uint64_t(count) + gridDim.x - 1) / gridDim.x);
-  const uint64_t lo = uint64_t(xcd_chunk(blockIdx.x, gridDim.x)) * per;
+  const uint32_t per = uint32_t((uint64_t(count) + nb - 1) / nb);
+  const uint64_t lo = uint64_t(xcd_chunk(bid, nb)) * per;
   const uint32_t hi = uint32_t(min(uint64_t(count), lo + per));
   for (uint32_t idx = uint32_t(lo) + threadIdx.x; idx < hi; idx += blockDim.x) {
     const uint32_t u = list[idx];
@@ -329,19 +331,30 @@ louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
         }
       }
     }
-    lv_decide(a, idx, nd, wfrom, bgain, bkey);
+    lv_decide(a, dbase + idx, nd, wfrom, bgain, bkey);
   }
 }
 
-// G lanes per node, T-slot open-addressing map per group in LDS.
+template <class View>
+__global__ void __launch_bounds__(256)
+louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count) {
+  lv_small_body(g, a, list, count, blockIdx.x, gridDim.x, 0);
+}
+
+// G lanes per node, T-slot open-addressing map per group in LDS (16 bytes a
+// slot: weight, key, first position).
+template <int G, int TBITS, int BLOCK>
+constexpr int lv_group_lds() { return (BLOCK / G) * (1 << TBITS) * 16; }
+
 template <class View, int G, int TBITS, int BLOCK>
-__global__ void __launch_bounds__(BLOCK)
-louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count) {
+__device__ __forceinline__ void lv_group_body(const View& g, const LvArgs& a, const uint32_t* __restrict__ list,
+                                              uint32_t count, uint32_t bid, uint32_t nb, uint32_t dbase,
+                                              unsigned char* lds) {
   constexpr int T = 1 << TBITS;
   constexpr int GPB = BLOCK / G;
-  __shared__ int32_t keys[GPB * T];
-  __shared__ double wsum[GPB * T];
-  __shared__ uint32_t first[GPB * T];
+  double* wsum = reinterpret_cast<double*>(lds);
+  int32_t* keys = reinterpret_cast<int32_t*>(lds + GPB * T * 8);
+  uint32_t* first = reinterpret_cast<uint32_t*>(lds + GPB * T * 12);
   const int grp = threadIdx.x / G, lane = threadIdx.x % G;
   int32_t* K = keys + grp * T;
   double* Wt = wsum + grp * T;
@@ -351,8 +364,8 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
   constexpr int GPW = 64 / G, WPB = BLOCK / 64;
   const uint32_t wid = threadIdx.x >> 6, wgrp = (threadIdx.x & 63) / G;
   // one contiguous chunk of the list per block, chunks dealt per XCD
-  const uint32_t per = uint32_t((uint64_t(count) + gridDim.x - 1) / gridDim.x);
-  const uint64_t lo = uint64_t(xcd_chunk(blockIdx.x, gridDim.x)) * per;
+  const uint32_t per = uint32_t((uint64_t(count) + nb - 1) / nb);
+  const uint64_t lo = uint64_t(xcd_chunk(bid, nb)) * per;
   const uint32_t hi = uint32_t(min(uint64_t(count), lo + per));
   for (uint32_t base = uint32_t(lo) + wid * GPW; base < hi; base += WPB * GPW) {
     const uint32_t idx = base + wgrp;
@@ -416,34 +429,181 @@ louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, u
         wfrom += __shfl_xor(wfrom, o, G);
         if (ok >= 0 && (bkey < 0 || better(og, of, bgain, bfirst))) { bgain = og; bfirst = of; bkey = ok; }
       }
-      if (lane == 0) lv_decide(a, idx, nd, wfrom, bgain, bkey);
+      if (lane == 0) lv_decide(a, dbase + idx, nd, wfrom, bgain, bkey);
     }
     lds_wave_sync();
   }
 }
 
+template <class View, int G, int TBITS, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+louvain_move_group_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count) {
+  __shared__ __align__(16) unsigned char lds[lv_group_lds<G, TBITS, BLOCK>()];
+  lv_group_body<View, G, TBITS, BLOCK>(g, a, list, count, blockIdx.x, gridDim.x, 0, lds);
+}
+
+// 129..1024 arcs: one block per node; its waves insert into one map of
+// T >= 2 deg slots in LDS (up to 2^TBITS), so a node takes deg / BLOCK trips
+// instead of deg / 64.  Several waves adding to one slot sum in no fixed
+// order: exact under the integer arc weights (the fractional weightings are
+// compared by quality only, as in the heavy path).  The best slot is the same
+// whatever the order of the scan: larger gain, then smaller first position.
+template <int TBITS>
+constexpr int lv_block_lds() { return (1 << TBITS) * 16; }
+
+template <class View, int TBITS, int BLOCK>
+__device__ __forceinline__ void lv_block_body(const View& g, const LvArgs& a, const uint32_t* __restrict__ list,
+                                              uint32_t count, uint32_t bid, uint32_t nb, uint32_t dbase,
+                                              unsigned char* lds) {
+  constexpr int T = 1 << TBITS;
+  constexpr int WPB = BLOCK / 64;
+  double* Wt = reinterpret_cast<double*>(lds);
+  int32_t* K = reinterpret_cast<int32_t*>(lds + T * 8);
+  uint32_t* F = reinterpret_cast<uint32_t*>(lds + T * 12);
+  // the per-wave results reuse the start of the map once it has been read
+  double* r_gain = reinterpret_cast<double*>(lds);
+  double* r_wfrom = r_gain + WPB;
+  uint32_t* r_first = reinterpret_cast<uint32_t*>(r_wfrom + WPB);
+  int32_t* r_key = reinterpret_cast<int32_t*>(r_first + WPB);
+  static_assert(WPB * 24 <= T * 16, "per-wave results fit in the map");
+  const uint32_t wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t per = uint32_t((uint64_t(count) + nb - 1) / nb);
+  const uint64_t lo = uint64_t(xcd_chunk(bid, nb)) * per;
+  const uint32_t hi = uint32_t(min(uint64_t(count), lo + per));
+  for (uint32_t idx = uint32_t(lo); idx < hi; ++idx) {
+    const uint32_t u = list[idx];
+    const uint32_t deg = min(g.degree(u), a.sampling);
+    LvNode nd{0, 0.0, 0.0, 0.0, 0};
+    nd.from = a.C[u];
+    nd.volU = a.nvol[u];
+    const double rc = lv_recip(a, u);
+    int tb = 1;
+    while (tb < TBITS && (1u << tb) < 2 * deg) ++tb;
+    const int Tn = 1 << tb;
+    for (int t = threadIdx.x; t < Tn; t += BLOCK) { K[t] = -1; Wt[t] = 0.0; F[t] = 0xffffffffu; }
+    __syncthreads();
+    // the trip count is uniform over the block
+    for (uint32_t i0 = 0; i0 < deg; i0 += BLOCK) {
+      const uint32_t i = i0 + threadIdx.x;
+      const bool valid = i < deg;
+      int32_t c = -1;
+      double w = 0.0;
+      if (valid) {
+        uint32_t v;
+        g.arc(u, i, v, w);
+        c = a.C[v];
+      }
+      map_insert_aggregated<64>(K, Wt, F, tb, valid, c, w, i, 2);
+    }
+    if (threadIdx.x == 0) {
+      nd.cvol_from = a.cvol[nd.from];
+      nd.csize_from = a.csize[nd.from];
+    }
+    __syncthreads();
+    nd.mult = rc * nd.volU;
+    double wfrom = 0.0, bgain = -DBL_MAX;
+    uint32_t bfirst = 0xffffffffu;
+    int32_t bkey = -1;
+    for (int t = threadIdx.x; t < Tn; t += BLOCK) {
+      const int32_t k = K[t];
+      if (k < 0) continue;
+      if (k == nd.from) { wfrom = Wt[t]; continue; }
+      const double gain = Wt[t] - nd.mult * a.cvol[k];
+      if (better(gain, F[t], bgain, bfirst)) { bgain = gain; bfirst = F[t]; bkey = k; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      const double og = __shfl_xor(bgain, o, 64);
+      const uint32_t of = __shfl_xor(bfirst, o, 64);
+      const int32_t ok = __shfl_xor(bkey, o, 64);
+      wfrom += __shfl_xor(wfrom, o, 64);
+      if (ok >= 0 && (bkey < 0 || better(og, of, bgain, bfirst))) { bgain = og; bfirst = of; bkey = ok; }
+    }
+    __syncthreads();  // every wave has read the map
+    if (lane == 0) { r_gain[wid] = bgain; r_first[wid] = bfirst; r_key[wid] = bkey; r_wfrom[wid] = wfrom; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < WPB; ++w) {
+        wfrom += r_wfrom[w];
+        if (r_key[w] >= 0 && (bkey < 0 || better(r_gain[w], r_first[w], bgain, bfirst))) {
+          bgain = r_gain[w]; bfirst = r_first[w]; bkey = r_key[w];
+        }
+      }
+      lv_decide(a, dbase + idx, nd, wfrom, bgain, bkey);
+    }
+    __syncthreads();  // the map is cleared for the next node
+  }
+}
+
+template <class View, int TBITS, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+louvain_move_block_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count) {
+  __shared__ __align__(16) unsigned char lds[lv_block_lds<TBITS>()];
+  lv_block_body<View, TBITS, BLOCK>(g, a, list, count, blockIdx.x, gridDim.x, 0, lds);
+}
+
+// One launch for the classes up to 1024 arcs of a sub-round: the grid is cut
+// into segments, in this order 129..1024 arcs (a block per node), 65..128
+// (32 lanes, 256-slot maps), 17..64 (G1 lanes, 128-slot maps; G1 = 0: that
+// class has its own launch) and <= 16 arcs (a thread per node).  Every
+// segment has a multiple of 8 blocks, so each keeps its XCD-aware chunking,
+// and the long-latency nodes are dispatched first.
+struct LvFused {
+  uint32_t off[4];     // first list position (and decision slot) per segment
+  uint32_t cnt[4];
+  uint32_t blocks[4];
+};
+constexpr int kLvFusedLds = 32768;
+constexpr uint64_t kLvFuse1Arcs = 1ull << 21;
+static_assert(lv_block_lds<11>() <= kLvFusedLds && lv_group_lds<32, 8, 256>() <= kLvFusedLds &&
+                  lv_group_lds<16, 7, 256>() <= kLvFusedLds, "fused move maps fit");
+
+template <class View, int G1>
+__global__ void __launch_bounds__(256)
+louvain_move_fused_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, LvFused f) {
+  __shared__ __align__(16) unsigned char lds[kLvFusedLds];
+  uint32_t b = blockIdx.x;
+  if (b < f.blocks[0]) {
+    lv_block_body<View, 11, 256>(g, a, list + f.off[0], f.cnt[0], b, f.blocks[0], f.off[0], lds);
+    return;
+  }
+  b -= f.blocks[0];
+  if (b < f.blocks[1]) {
+    lv_group_body<View, 32, 8, 256>(g, a, list + f.off[1], f.cnt[1], b, f.blocks[1], f.off[1], lds);
+    return;
+  }
+  b -= f.blocks[1];
+  if constexpr (G1 != 0) {
+    if (b < f.blocks[2]) {
+      lv_group_body<View, G1, 7, 256>(g, a, list + f.off[2], f.cnt[2], b, f.blocks[2], f.off[2], lds);
+      return;
+    }
+    b -= f.blocks[2];
+  }
+  lv_small_body<View>(g, a, list + f.off[3], f.cnt[3], b, f.blocks[3], f.off[3]);
+}
+
 // ------------------------------------------------------- heavy nodes --
 // Nodes with more than 1024 (sampled) arcs.  Their arcs are spread over the
 // whole grid instead of one block per node.  Every heavy node of a sub-round
-// owns a table of T = pow2 >= 2d slots in one arena; a slot key carries the
-// sub-round epoch in its high word, so the tables are never cleared: a slot
-// of an older epoch is empty.
+// owns a table of T = pow2 >= 2d slots in one arena.  The arena starts empty
+// (key kHvEmpty, weight 0, first position ~0), and the scan that reads a
+// sub-round's tables leaves every slot it read empty again, so a pass over
+// the arcs can claim a slot and add to it at once.
 //   hv_prep     d and T per node            (then exclusive scans: arc and
-//   hv_insert   per arc: find or claim the    table offsets)
-//               slot of its cluster; a claimer resets weight and first
-//               position; the slot is remembered per arc
-//   hv_accum    per arc: weight sum, minimum arc position (wave-aggregated)
-//   hv_decide   per node: best cluster over its table, then the move rule
+//                                             table offsets)
+//   hv_insert   per arc: find or claim the slot of its cluster, then add the
+//               weight and the arc position (wave-aggregated per slot)
+//   hv_scan     per chunk of slots: best candidate; the slots are emptied
+//   hv_decide   per node: best cluster over its chunks, then the move rule
+constexpr uint64_t kHvEmpty = ~0ull;
 struct HeavyArgs {
   const uint32_t* list;  // heavy nodes of the sub-round
   uint32_t count;
   const uint64_t* aoff;  // count + 1 arc offsets
   const uint64_t* toff;  // count + 1 table offsets
-  uint64_t* keys;        // (epoch << 32) | cluster
+  uint64_t* keys;        // cluster id, or kHvEmpty
   double* wsum;
   uint32_t* first;
-  uint32_t* arc_slot;    // per arc: absolute table slot
-  uint32_t epoch;
 };
 
 template <class View>
@@ -485,52 +645,6 @@ __global__ void __launch_bounds__(256) hv_insert_kernel(View g, LvArgs a, HeavyA
   __shared__ uint32_t s_node;
   const uint64_t a0 = h.aoff[0], t0 = h.toff[0];
   const uint64_t total = h.aoff[h.count] - a0;
-  const uint64_t ep = uint64_t(h.epoch) << 32;
-  for (uint64_t c0 = blockIdx.x * uint64_t(kHvArcs); c0 < total; c0 += uint64_t(gridDim.x) * kHvArcs) {
-    const uint32_t i0 = hv_chunk_node(h, a0, c0, &s_node);
-    const uint64_t c1 = min(c0 + kHvArcs, total), split = h.aoff[i0 + 1] - a0;
-    for (uint64_t j = c0 + threadIdx.x; j < c1; j += blockDim.x) {
-      const uint32_t i = j < split ? i0 : i0 + 1;
-      const uint32_t u = h.list[i];
-      uint32_t v;
-      double w;
-      g.arc(u, uint32_t(a0 + j - h.aoff[i]), v, w);
-      const uint32_t c = uint32_t(a.C[v]);
-      const uint64_t base = h.toff[i] - t0;
-      const uint32_t T = uint32_t(h.toff[i + 1] - h.toff[i]);
-      const int tbits = __builtin_ctz(T);
-      const uint64_t key = ep | c;
-      uint32_t s = slot_hash(int32_t(c), tbits);
-      // a key of the current epoch never changes, so a plain read that shows
-      // one is exact; anything else is settled by the CAS
-      uint64_t cur = h.keys[base + s];
-      while (true) {
-        if ((cur & 0xffffffff00000000ull) == ep) {
-          if (uint32_t(cur) == c) break;
-          s = (s + 1) & (T - 1);
-          cur = h.keys[base + s];
-          continue;
-        }
-        const uint64_t old = atomicCAS(reinterpret_cast<unsigned long long*>(&h.keys[base + s]),
-                                       static_cast<unsigned long long>(cur), static_cast<unsigned long long>(key));
-        if (old == cur) {
-          h.wsum[base + s] = 0.0;
-          h.first[base + s] = 0xffffffffu;
-          break;
-        }
-        cur = old;
-      }
-      h.arc_slot[j] = uint32_t(base + s);
-    }
-    __syncthreads();  // s_node is rewritten for the next chunk
-  }
-}
-
-template <class View>
-__global__ void __launch_bounds__(256) hv_accum_kernel(View g, LvArgs a, HeavyArgs h) {
-  __shared__ uint32_t s_node;
-  const uint64_t a0 = h.aoff[0];
-  const uint64_t total = h.aoff[h.count] - a0;
   const int lane = threadIdx.x & 63;
   for (uint64_t c0 = blockIdx.x * uint64_t(kHvArcs); c0 < total; c0 += uint64_t(gridDim.x) * kHvArcs) {
     const uint32_t i0 = hv_chunk_node(h, a0, c0, &s_node);
@@ -546,7 +660,25 @@ __global__ void __launch_bounds__(256) hv_accum_kernel(View g, LvArgs a, HeavyAr
         pos = uint32_t(a0 + j - h.aoff[i]);
         uint32_t v;
         g.arc(h.list[i], pos, v, w);
-        slot = h.arc_slot[j];
+        const uint64_t c = uint32_t(a.C[v]);
+        const uint64_t base = h.toff[i] - t0;
+        const uint32_t T = uint32_t(h.toff[i + 1] - h.toff[i]);
+        uint32_t t = slot_hash(int32_t(c), __builtin_ctz(T));
+        // a slot's key goes from empty to a cluster once per sub-round, so a
+        // plain read that shows a cluster is exact; an empty one is settled
+        // by the CAS
+        uint64_t cur = h.keys[base + t];
+        while (cur != c) {
+          if (cur != kHvEmpty) {
+            t = (t + 1) & (T - 1);
+            cur = h.keys[base + t];
+            continue;
+          }
+          cur = atomicCAS(reinterpret_cast<unsigned long long*>(&h.keys[base + t]),
+                          static_cast<unsigned long long>(kHvEmpty), static_cast<unsigned long long>(c));
+          if (cur == kHvEmpty) break;  // claimed
+        }
+        slot = uint32_t(base + t);
       }
       for (int it = 0; it < 4; ++it) {  // combine lanes of the lowest pending slot
         const uint64_t act = __ballot(pending);
@@ -590,7 +722,6 @@ template <class View>
 __global__ void __launch_bounds__(256) hv_scan_kernel(View g, LvArgs a, HeavyArgs h, HvBest* __restrict__ out) {
   __shared__ HvBest s_b[4];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const uint64_t ep = uint64_t(h.epoch) << 32;
   const uint64_t t0 = h.toff[0];
   const uint64_t chunks = (h.toff[h.count] - t0) / kHvChunk;
   for (uint64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
@@ -604,11 +735,15 @@ __global__ void __launch_bounds__(256) hv_scan_kernel(View g, LvArgs a, HeavyArg
     int32_t bkey = -1;
     for (uint32_t t = threadIdx.x; t < kHvChunk; t += blockDim.x) {
       const uint64_t kk = h.keys[s0 + t];
-      if ((kk & 0xffffffff00000000ull) != ep) continue;
+      if (kk == kHvEmpty) continue;
       const int32_t k = int32_t(uint32_t(kk));
-      if (k == from) { wfrom = h.wsum[s0 + t]; continue; }
-      const double gain = h.wsum[s0 + t] - mult * a.cvol[k];
+      const double ws = h.wsum[s0 + t];
       const uint32_t f = h.first[s0 + t];
+      h.keys[s0 + t] = kHvEmpty;  // empty again for the next sub-round
+      h.wsum[s0 + t] = 0.0;
+      h.first[s0 + t] = 0xffffffffu;
+      if (k == from) { wfrom = ws; continue; }
+      const double gain = ws - mult * a.cvol[k];
       if (better(gain, f, bgain, bfirst)) { bgain = gain; bfirst = f; bkey = k; }
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -1185,12 +1320,11 @@ struct LouvainRun {
   // heavy-node arena and per-sub-round arrays (see hv_* kernels)
   DevBuf<uint64_t> hkeys_, hdsz_, htsz_, haoff_, htoff_;
   DevBuf<double> hw_;
-  DevBuf<uint32_t> hf_, hslot_;
+  DevBuf<uint32_t> hf_;
   DevBuf<HvBest> hbest_;
-  uint32_t epoch_ = 0;
 
-  // reserves the arena for tables of up to 4 * arcs slots (T < 4d) and the
-  // per-arc slot array; new key memory starts at epoch 0, which is never used
+  // reserves the arena for tables of up to 4 * arcs slots (T < 4d); new
+  // memory is filled empty, and every scan leaves its slots empty
   void reserve_heavy(uint64_t arcs, uint32_t nodes) {
     const hipStream_t s = c.stream;
     const uint64_t slots = std::max<uint64_t>(4 * arcs, 2);
@@ -1199,9 +1333,10 @@ struct LouvainRun {
       hkeys_.alloc(slots);
       hw_.alloc(slots);
       hf_.alloc(slots);
-      fill_async<uint64_t>(hkeys_.p, 0ull, slots, s);
+      fill_async<uint64_t>(hkeys_.p, kHvEmpty, slots, s);
+      fill_async<double>(hw_.p, 0.0, slots, s);
+      fill_async<uint32_t>(hf_.p, 0xffffffffu, slots, s);
     }
-    hslot_.ensure(std::max<uint64_t>(arcs, 1));
     hbest_.ensure(slots / kHvChunk + 1);
     hdsz_.ensure(nodes + 1);
     htsz_.ensure(nodes + 1);
@@ -1223,20 +1358,14 @@ struct LouvainRun {
   // first: index of the sub-round's first heavy node in the round's list
   template <class View>
   void launch_heavy(const View& g, const LvArgs& a, const uint32_t* list, uint32_t first, uint32_t count,
-                    uint64_t arcs, double bytes) {
-    const hipStream_t s = c.stream;
-    HeavyArgs h{list, count, haoff_.p + first, htoff_.p + first, hkeys_.p, hw_.p, hf_.p, hslot_.p, ++epoch_};
+                    uint64_t arcs, double bytes, hipStream_t s) {
+    HeavyArgs h{list, count, haoff_.p + first, htoff_.p + first, hkeys_.p, hw_.p, hf_.p};
     const unsigned grid = grid_for(arcs, kHvArcs, unsigned(c.num_cus) * 16);
     {
       // the compulsory bytes of the heavy moves (node data, head id and
       // cluster gather per arc) are attributed to the gather pass
       TimedRegion t1(c.prof, s, kFamLouvainMove, bytes, std::string("hv_insert_kernel<") + View::kName + ">");
       hv_insert_kernel<View><<<grid, 256, 0, s>>>(g, a, h);
-      MKP_LAUNCH_CHECK();
-    }
-    {
-      TimedRegion t2(c.prof, s, kFamLouvainMove, 0.0, std::string("hv_accum_kernel<") + View::kName + ">");
-      hv_accum_kernel<View><<<grid, 256, 0, s>>>(g, a, h);
       MKP_LAUNCH_CHECK();
     }
     {
@@ -1353,7 +1482,6 @@ struct LouvainRun {
                    const std::vector<uint32_t>& cnt, const std::vector<uint64_t>& sa,
                    const std::vector<uint64_t>& sb, int sr, int lanes = 32, hipEvent_t ta = nullptr,
                    hipEvent_t tb = nullptr) {
-    const hipStream_t s = c.stream;
     const int B = a.buckets;
     const int s0 = 0 * B + sr, s1 = 1 * B + sr, s1w = 2 * B + sr, s2 = 3 * B + sr, s3 = 4 * B + sr;
     // each launch writes the decisions of its own list range
@@ -1363,13 +1491,54 @@ struct LouvainRun {
       if (a.gain_out) x.gain_out = a.gain_out + off[slot];
       return x;
     };
-    if (cnt[s0]) {
+    // The classes decide on the same state and write disjoint decision
+    // ranges.  With fusion on, the classes up to 1024 arcs share one launch
+    // (one launch of a class rarely fills the chip, and their tails
+    // overlap).  The 17..64-arc class keeps its own launch while the
+    // autotuner times its shapes, with 8 lanes per node (128-thread blocks),
+    // and when it has more than kLvFuse1Arcs arcs: the fused kernel runs at
+    // the occupancy of its heaviest body (the registers of the <= 16-arc
+    // nodes), which a large class of group maps would pay for.
+    const hipStream_t s = c.stream;
+    const int G = lanes;
+    const int mode = c.tune.lv_fused_classes;
+    const bool fused = mode != 0;
+    const bool join1 = mode == 3 || (mode == 1 && sa[s1] <= kLvFuse1Arcs);
+    const bool own1 = cnt[s1] && (!fused || ta || G == 8 || !join1);
+    if (fused && (cnt[s0] || cnt[s1w] || cnt[s2] || (cnt[s1] && !own1))) {
+      LvFused f{};
+      const int slot[4] = {s2, s1w, s1, s0};
+      const unsigned per1 = G == 16 ? 16 : (G == 64 ? 4 : 8);
+      const unsigned per[4] = {1, 8, per1, 256};
+      double bytes = 0.0;
+      unsigned blocks = 0;
+      for (int k = 0; k < 4; ++k) {
+        if (k == 2 && own1) continue;
+        f.off[k] = off[slot[k]];
+        f.cnt[k] = cnt[slot[k]];
+        f.blocks[k] = f.cnt[k] ? xcd_grid(f.cnt[k], per[k], 65536) : 0;
+        blocks += f.blocks[k];
+        bytes += double(sb[slot[k]]);
+      }
+      const int g1 = (cnt[s1] && !own1) ? G : 0;
+      TimedRegion tr(c.prof, s, kFamLouvainMove, bytes,
+                     std::string("louvain_move_fused_kernel<") + View::kName + ", " + std::to_string(g1) + ">");
+      if (g1 == 16)
+        louvain_move_fused_kernel<View, 16><<<blocks, 256, 0, s>>>(g, a, list, f);
+      else if (g1 == 32)
+        louvain_move_fused_kernel<View, 32><<<blocks, 256, 0, s>>>(g, a, list, f);
+      else if (g1 == 64)
+        louvain_move_fused_kernel<View, 64><<<blocks, 256, 0, s>>>(g, a, list, f);
+      else
+        louvain_move_fused_kernel<View, 0><<<blocks, 256, 0, s>>>(g, a, list, f);
+      MKP_LAUNCH_CHECK();
+    }
+    if (!fused && cnt[s0]) {
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s0]),
                      std::string("louvain_move_small_kernel<") + View::kName + ">");
       louvain_move_small_kernel<View><<<xcd_grid(cnt[s0], 256, 65536), 256, 0, s>>>(g, at(s0), list + off[s0], cnt[s0]);
     }
-    if (cnt[s1]) {  // 17..64 arcs: 128-slot maps, lanes per node autotuned
-      const int G = lanes;
+    if (own1) {  // 17..64 arcs: 128-slot maps, lanes per node autotuned
       if (ta) MKP_HIP(hipEventRecord(ta, s));
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s1]),
                      std::string("louvain_move_group_kernel<") + View::kName + ", " + std::to_string(G) + ", 7, " +
@@ -1388,18 +1557,18 @@ struct LouvainRun {
                                                                                                 cnt[s1]);
       if (tb) MKP_HIP(hipEventRecord(tb, s));
     }
-    if (cnt[s1w]) {  // 65..128 arcs: 256-slot maps
+    if (!fused && cnt[s1w]) {  // 65..128 arcs: 256-slot maps
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s1w]),
                      std::string("louvain_move_group_kernel<") + View::kName + ", 32, 8, 256>");
       louvain_move_group_kernel<View, 32, 8, 256><<<xcd_grid(cnt[s1w], 8, 65536), 256, 0, s>>>(g, at(s1w), list + off[s1w], cnt[s1w]);
     }
-    if (cnt[s2]) {
+    if (!fused && cnt[s2]) {  // 129..1024 arcs: a block per node, 2048-slot map
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s2]),
-                     std::string("louvain_move_group_kernel<") + View::kName + ", 64, 11, 64>");
-      louvain_move_group_kernel<View, 64, 11, 64><<<xcd_grid(cnt[s2], 1, 65536), 64, 0, s>>>(g, at(s2), list + off[s2],
-                                                                                             cnt[s2]);
+                     std::string("louvain_move_block_kernel<") + View::kName + ", 11, 256>");
+      louvain_move_block_kernel<View, 11, 256><<<xcd_grid(cnt[s2], 1, 65536), 256, 0, s>>>(g, at(s2), list + off[s2],
+                                                                                           cnt[s2]);
     }
-    if (cnt[s3]) launch_heavy(g, at(s3), list + off[s3], off[s3] - off[4 * B], cnt[s3], sa[s3], double(sb[s3]));
+    if (cnt[s3]) launch_heavy(g, at(s3), list + off[s3], off[s3] - off[4 * B], cnt[s3], sa[s3], double(sb[s3]), s);
     MKP_LAUNCH_CHECK();
   }
 

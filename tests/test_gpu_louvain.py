@@ -134,6 +134,38 @@ def test_group_lane_shapes_and_autotuner_agree(ctx):
         c2.close()
 
 
+def test_fused_class_launch_changes_only_the_speed(ctx):
+    """The degree classes up to 1024 arcs of a sub-round in one fused launch
+    (the 17..64-arc class in it when small, never, or always) or one launch
+    each, with every lane shape of the 17..64-arc class (8 lanes keep their
+    own launch): communities and statistics are identical
+    and equal to the oracle's.  The hypergraph has nodes in every class,
+    129..1024-arc net nodes (a block each) and net nodes above 1024 arcs (the
+    arc-parallel path) included."""
+    n, noff, pins = powerlaw(20000, 30000, 900, 19, heavy_nets=2)
+    c2 = mtk_amd.Context(0)
+    try:
+        out = []
+        for fused, lanes in ((1, 0), (0, 0), (2, 0), (3, 8), (3, 16), (3, 32), (3, 64), (0, 64)):
+            print("case: fused", fused, "lanes", lanes, flush=True)
+            c2.set_tuning("louvain_fused_classes", fused)
+            c2.set_tuning("louvain_group_lanes", lanes)
+            dh = mtk_amd.Hypergraph(c2, n, noff, pins)
+            out.append(dh.louvain(seed=7))
+            dh.close()
+        for comm, st in out[1:]:
+            assert np.array_equal(comm, out[0][0])
+            assert (st["arcs_swept"], st["rounds"], st["levels"], st["moves"]) == \
+                (out[0][1]["arcs_swept"], out[0][1]["rounds"], out[0][1]["levels"], out[0][1]["moves"])
+        g = orc.Graph(orc.Hypergraph(n, noff, pins), 0, ranked=True)
+        C, _ = g.louvain(sync=True, seed=7)
+        assert out[0][0].tolist() == C[:n].tolist()
+        with pytest.raises(RuntimeError):
+            c2.set_tuning("louvain_fused_classes", 4)
+    finally:
+        c2.close()
+
+
 @pytest.mark.parametrize("state", ["random", "louvain"])
 def test_fp64_gains_targets_and_modularity(ctx, state):
     """computeMaxGainCluster on the device for every node of the star

@@ -1422,7 +1422,8 @@ struct LouvainRun {
       return ch;
     }
     if (round == 0 && prm.max_passes > 1) {
-      ch.lanes = 8;
+      // 8 lanes would take the class out of the fused launch
+      ch.lanes = c.tune.lv_fused_classes ? 32 : 8;
       return ch;
     }
     ch.trials = B >= kTrials ? kTrials : (B >= 4 ? 4 : 0);
@@ -1443,11 +1444,15 @@ struct LouvainRun {
       ms_sum[trial_shape(i)] += double(ms);
       arcs_sum[trial_shape(i)] += double(sa[B + i]);
     }
-    int best = 1;
-    double best_t = 0;
-    for (int j = 0; j < 4; ++j) {
+    // 32 lanes unless another shape is faster by more than 5%: shapes within
+    // the timing noise of each other would otherwise be chosen at random
+    // from run to run.  With fused launches 8 lanes are not chosen: the
+    // class would leave the fused launch, which its own timing does not see.
+    int best = 2;
+    double best_t = ms_sum[2] / std::max(arcs_sum[2], 1.0);
+    for (int j = c.tune.lv_fused_classes ? 1 : 0; j < 4; ++j) {
       const double t = ms_sum[j] / std::max(arcs_sum[j], 1.0);
-      if (j == 0 || t < best_t) { best = j; best_t = t; }
+      if (j != 2 && t < 0.95 * best_t) { best = j; best_t = t; }
     }
     c.tune.lv_group_choice[ch.key] = kLaneTrials[best];
   }
@@ -1849,7 +1854,7 @@ struct LouvainRun {
     uint64_t U = 0, last = 0;
     MKP_HIP(hipMemcpyAsync(&U, ucount.p, 8, hipMemcpyDeviceToHost, s));
     c.sync();
-    if (U) {
+    if (U && !compact) {  // the compaction already dropped the self loops
       MKP_HIP(hipMemcpyAsync(&last, ukeys.p + (U - 1), 8, hipMemcpyDeviceToHost, s));
       c.sync();
       if (last == (uint64_t(1) << (2 * bits))) --U;  // self-loop sentinel segment
@@ -1869,7 +1874,7 @@ struct LouvainRun {
     }
     fill_async<uint64_t>(cg.idx.p, 0, 1, s);
     inclusive_max_scan_u64(c, ends.p, cg.idx.p + 1, nc);
-    c.sync();
+    // the temporaries go back to the context's cache in stream order: no wait
   }
 
   // local_moving_contract_recurse (parallel_louvain.cpp:27-60), iterative

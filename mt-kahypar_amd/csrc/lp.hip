@@ -49,6 +49,7 @@ struct LpWork {
   DevBuf<uint64_t> bkeys, bkeys_sorted;
   DevBuf<int64_t> bcount;
   DevBuf<int64_t> stripes;  // striped counter copies (single hypergraph)
+  DevBuf<int64_t> rb;       // read-back words of a sub-round (2k + 3)
   uint32_t m = 0, stamp = 0;
   void ensure_nets(uint32_t mm, hipStream_t s) {
     if (mm <= m && net_stamp.p) return;
@@ -76,6 +77,7 @@ struct LpWork {
     keys.alloc(N); keys_sorted.alloc(N); vals.alloc(N); vals_sorted.alloc(N);
     wsort.alloc(N); wscan.alloc(N); bkeys.alloc(N); bkeys_sorted.alloc(N); bcount.alloc(4 * 256);
     seg_base.alloc(k); in_w.alloc(k); max_w.alloc(k); perfect_w.alloc(k); overflow.alloc(k);
+    rb.alloc(2 * size_t(k) + 3);
   }
   static constexpr int kGcStrideWords = 16;
 };
@@ -813,6 +815,25 @@ __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const
   if ((threadIdx.x & 63) == 0) stripe_add(a.stripes, 0, d);
 }
 
+// The read-back words of a sub-round gathered into one buffer, so one copy
+// brings them to the host: the rollback flag of the previous sub-round, the
+// candidate count, arrivals and weights per block and, with profiling, the
+// candidates' incidences.
+__global__ void lp_readback_kernel(const int64_t* __restrict__ counters, const uint32_t* __restrict__ pos,
+                                   uint32_t ns, const int64_t* __restrict__ in_w,
+                                   const int64_t* __restrict__ part_w, int K, int prof, int64_t* __restrict__ rb) {
+  const int t = int(blockIdx.x * blockDim.x + threadIdx.x), stride = int(gridDim.x * blockDim.x);
+  if (t == 0) {
+    rb[0] = counters[6];
+    rb[1] = int64_t(pos[ns]);
+    rb[2 + 2 * K] = prof ? counters[11] : 0;
+  }
+  for (int b = t; b < K; b += stride) {
+    rb[2 + b] = in_w[b];
+    rb[2 + K + b] = part_w[b];
+  }
+}
+
 // real km1 delta of each applied move on the post-move state
 __global__ void lp_verify_kernel(LpArgs a, const uint32_t* __restrict__ cu, const int32_t* __restrict__ cfrom,
                                  const int32_t* __restrict__ cto, const int32_t* __restrict__ cgain,
@@ -1231,19 +1252,18 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
       // profiling, the candidates' incidences)
       const size_t over_words = (size_t(K) + 7) / 8;
       int64_t* stg = c.staging<int64_t>(2 * size_t(K) + 3 + over_words);
-      uint8_t* stg_over = reinterpret_cast<uint8_t*>(stg + 2 * K + 2);
-      int64_t* stg_inc = stg + 2 * K + 2 + over_words;
-      MKP_HIP(hipMemcpyAsync(stg, w.counters.p + 6, 8, hipMemcpyDeviceToHost, s));
-      MKP_HIP(hipMemcpyAsync(stg + 1, w.pos.p + ns, 4, hipMemcpyDeviceToHost, s));
-      MKP_HIP(hipMemcpyAsync(stg + 2, w.in_w.p, size_t(K) * 8, hipMemcpyDeviceToHost, s));
-      MKP_HIP(hipMemcpyAsync(stg + 2 + K, p->part_w.p, size_t(K) * 8, hipMemcpyDeviceToHost, s));
-      if (prof) MKP_HIP(hipMemcpyAsync(stg_inc, w.counters.p + 11, 8, hipMemcpyDeviceToHost, s));
+      int64_t* stg_inc = stg + 2 * K + 2;
+      uint8_t* stg_over = reinterpret_cast<uint8_t*>(stg + 2 * K + 3);
+      lp_readback_kernel<<<unsigned(std::min<int>((K + 255) / 256, 256)), 256, 0, s>>>(
+          w.counters.p, w.pos.p, ns, w.in_w.p, p->part_w.p, K, prof ? 1 : 0, w.rb.p);
+      MKP_LAUNCH_CHECK();
+      MKP_HIP(hipMemcpyAsync(stg, w.rb.p, (2 * size_t(K) + 3) * 8, hipMemcpyDeviceToHost, s));
       c.sync();
       if (!batch && stg[0]) {  // the previous sub-round was rolled back on the device
         stop = true;
         break;
       }
-      const uint32_t nc = *reinterpret_cast<const uint32_t*>(stg + 1);
+      const uint32_t nc = uint32_t(stg[1]);
       if (nc == 0) continue;
       const int64_t* inw = stg + 2;
       const int64_t* pw = stg + 2 + K;

@@ -216,6 +216,18 @@ typedef struct {
  * parallel_louvain.cpp:62-66.  communities (host, num_vertices) may be NULL. */
 MKP_API int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* params, int32_t* communities,
                         mkp_louvain_stats* stats);
+/* Nodes of the graph Louvain runs on (Graph(hypergraph, edge weight),
+ * graph.h:66-72): N when every net has two pins (the graph itself), N + E
+ * otherwise (the star expansion). */
+MKP_API int mkp_louvain_graph_nodes(const mkp_hypergraph* hg, int64_t* num_nodes);
+/* run_parallel_louvain(graph, context) (parallel_louvain.h:27) before
+ * restrictClusteringToHypernodes (graph.h:123): the clustering of every node
+ * of that graph, net e of the star expansion at node N + e as in the
+ * reference's Graph.  clustering (host) holds mkp_louvain_graph_nodes()
+ * entries; its first N are mkp_louvain's communities, which are stored on the
+ * hypergraph as well. */
+MKP_API int mkp_louvain_graph(mkp_hypergraph* hg, const mkp_louvain_params* params, int32_t* clustering,
+                              mkp_louvain_stats* stats);
 /* One evaluation of computeMaxGainCluster (local_moving_modularity.h:87-129)
  * for every node of the level-0 graph (the star expansion with net node
  * N + rank(e), or the graph itself when every net has two pins) on the

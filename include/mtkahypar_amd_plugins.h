@@ -257,6 +257,44 @@ class MultiTryKWayFM final : public IRefiner {
   mkp_fm_stats last_{};
 };
 
+// Graph(hypergraph, edge weight) (graph.h:66-72): the graph Louvain runs on,
+// the star expansion (N + E nodes, net e at node N + e) or the hypergraph
+// itself when every net has two pins.  The device builds its arcs when
+// Louvain runs; this object names the graph and its node count.
+class Graph {
+ public:
+  explicit Graph(Hypergraph& hg) : hg_(hg) {
+    int64_t n = 0;
+    check(mkp_louvain_graph_nodes(hg.handle(), &n));
+    nodes_ = size_t(n);
+  }
+  size_t numNodes() const { return nodes_; }
+  Hypergraph& hypergraph() { return hg_; }
+  // graph.h:123-125: the first N entries are the hypernodes' clusters
+  void restrictClusteringToHypernodes(const Hypergraph& hg, std::vector<int32_t>& C) const {
+    C.resize(hg.initialNumNodes());
+  }
+
+ private:
+  Hypergraph& hg_;
+  size_t nodes_ = 0;
+};
+
+// run_parallel_louvain(Graph&, const Context&, bool) (parallel_louvain.h:27):
+// the clustering of every node of the graph; the hypernodes' part is also
+// stored as the hypergraph's community ids.
+inline std::vector<int32_t> run_parallel_louvain(Graph& graph, const PartitionContext& ctx,
+                                                 bool disable_randomization = false) {
+  (void)disable_randomization;  // the device path is deterministic for a seed
+  const mkp_partition_params& p = ctx.p;
+  mkp_louvain_params lp{p.louvain_edge_weight, p.louvain_max_passes, p.louvain_min_move_fraction,
+                        p.louvain_sampling, p.seed, p.louvain_sub_rounds};
+  std::vector<int32_t> C(graph.numNodes());
+  mkp_louvain_stats st{};
+  check(mkp_louvain_graph(graph.hypergraph().handle(), &lp, C.data(), &st));
+  return C;
+}
+
 // run_parallel_louvain + restrictClusteringToHypernodes: community id per
 // hypernode, also stored as the hypergraph's community ids.
 inline std::vector<int32_t> run_parallel_louvain(Hypergraph& hg, const PartitionContext& ctx,

@@ -2506,9 +2506,61 @@ extern "C" int mkp_louvain_evaluate(mkp_hypergraph* hg, const mkp_louvain_params
   });
 }
 
+// Clustering of net node N + rank(e) in the numbering of the reference's
+// Graph, net e at node N + e (graph.cpp: the star expansion's net nodes
+// follow the hypernodes in net order).
+__global__ void star_net_order_kernel(const int32_t* __restrict__ C, uint32_t N, uint32_t M,
+                                      const uint32_t* __restrict__ order, int32_t* __restrict__ out) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < M; r += gridDim.x * blockDim.x)
+    out[N + order[r]] = C[N + r];
+}
+
+// all_size_two over the nets: the Louvain graph is the hypergraph itself
+static bool louvain_is_graph(mkp_hypergraph* hg) {
+  Context& c = hg->ctx->c;
+  if (!hg->m) return false;
+  DevBuf<uint32_t> flag(1);
+  fill_async<uint32_t>(flag.p, 1u, 1, c.stream);
+  all_size_two_kernel<<<grid_for(hg->m, 256, 65536), 256, 0, c.stream>>>(hg->noff.p, hg->m, flag.p);
+  MKP_LAUNCH_CHECK();
+  uint32_t is_graph = 0;
+  MKP_HIP(hipMemcpyAsync(&is_graph, flag.p, 4, hipMemcpyDeviceToHost, c.stream));
+  c.sync();
+  return is_graph != 0;
+}
+
+// graph_clustering (host, one id per node of the Louvain graph) may be NULL
+static void louvain_run(mkp_hypergraph* hg, const mkp_louvain_params* prm, int32_t* communities,
+                        int32_t* graph_clustering, mkp_louvain_stats* stats);
+
+extern "C" int mkp_louvain_graph_nodes(const mkp_hypergraph* hg, int64_t* num_nodes) {
+  return guarded([&] {
+    MKP_CHECK(hg && num_nodes, "null argument");
+    mkp_hypergraph* h = const_cast<mkp_hypergraph*>(hg);
+    Bind bind_(h->ctx->c);
+    *num_nodes = louvain_is_graph(h) ? int64_t(hg->n) : int64_t(hg->n) + int64_t(hg->m);
+  });
+}
+
+extern "C" int mkp_louvain_graph(mkp_hypergraph* hg, const mkp_louvain_params* prm, int32_t* clustering,
+                                 mkp_louvain_stats* stats) {
+  return guarded([&] {
+    MKP_CHECK(hg && prm && clustering, "null argument");
+    louvain_run(hg, prm, nullptr, clustering, stats);
+  });
+}
+
 extern "C" int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* prm, int32_t* communities,
                            mkp_louvain_stats* stats) {
   return guarded([&] {
+    MKP_CHECK(hg && prm, "null argument");
+    louvain_run(hg, prm, communities, nullptr, stats);
+  });
+}
+
+static void louvain_run(mkp_hypergraph* hg, const mkp_louvain_params* prm, int32_t* communities,
+                        int32_t* graph_clustering, mkp_louvain_stats* stats) {
+  {
     MKP_CHECK(hg && prm, "null argument");
     MKP_CHECK(prm->max_passes >= 0 && prm->sub_rounds >= 1 && prm->sub_rounds <= 255, "bad round parameters");
     Context& c = hg->ctx->c;
@@ -2519,16 +2571,7 @@ extern "C" int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* prm, in
     int wt = prm->edge_weight;
     if (wt < 0) wt = (double(hg->m) / double(std::max<uint32_t>(hg->n, 1)) < 0.75) ? 2 : 0;
     LouvainRun run(c, *prm);
-    DevBuf<uint32_t> flag(1);
-    fill_async<uint32_t>(flag.p, 1u, 1, c.stream);
-    if (hg->m) {
-      all_size_two_kernel<<<grid_for(hg->m, 256, 65536), 256, 0, c.stream>>>(hg->noff.p, hg->m, flag.p);
-      MKP_LAUNCH_CHECK();
-    }
-    uint32_t is_graph = 0;
-    MKP_HIP(hipMemcpyAsync(&is_graph, flag.p, 4, hipMemcpyDeviceToHost, c.stream));
-    c.sync();
-    is_graph = is_graph && hg->m > 0;
+    const bool is_graph = louvain_is_graph(hg);
     const uint32_t nodes = is_graph ? hg->n : hg->n + hg->m;
     DevBuf<int32_t> C(std::max<uint32_t>(nodes, 1));
     if (is_graph) {
@@ -2547,6 +2590,17 @@ extern "C" int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* prm, in
     MKP_HIP(hipMemcpyAsync(hg->comm.p, C.p, size_t(hg->n) * 4, hipMemcpyDeviceToDevice, c.stream));
     if (communities)
       MKP_HIP(hipMemcpyAsync(communities, C.p, size_t(hg->n) * 4, hipMemcpyDeviceToHost, c.stream));
+    DevBuf<int32_t> G;
+    if (graph_clustering && !is_graph && hg->m) {  // net nodes back to net order
+      G.alloc(nodes);
+      MKP_HIP(hipMemcpyAsync(G.p, C.p, size_t(hg->n) * 4, hipMemcpyDeviceToDevice, c.stream));
+      star_net_order_kernel<<<grid_for(hg->m, 256, 65536), 256, 0, c.stream>>>(C.p, hg->n, hg->m,
+                                                                               hg->net_order.p, G.p);
+      MKP_LAUNCH_CHECK();
+    }
+    if (graph_clustering)
+      MKP_HIP(hipMemcpyAsync(graph_clustering, G.p ? G.p : C.p, size_t(nodes) * 4, hipMemcpyDeviceToHost,
+                             c.stream));
     MKP_HIP(hipEventRecord(c.ev_b, c.stream));
     c.sync();
     if (stats) {
@@ -2555,5 +2609,5 @@ extern "C" int mkp_louvain(mkp_hypergraph* hg, const mkp_louvain_params* prm, in
       MKP_HIP(hipEventElapsedTime(&ms, c.ev_a, c.ev_b));
       stats->device_ms = ms;
     }
-  });
+  }
 }

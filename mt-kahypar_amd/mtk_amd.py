@@ -153,6 +153,8 @@ SIGNATURES = {
     "mkp_write_hgr": (C.c_int, [vp, C.c_char_p]),
     "mkp_write_partition_file": (C.c_int, [vp, C.c_uint32, C.c_char_p]),
     "mkp_louvain": (C.c_int, [vp, P(LouvainParams), vp, P(LouvainStats)]),
+    "mkp_louvain_graph_nodes": (C.c_int, [vp, P(C.c_int64)]),
+    "mkp_louvain_graph": (C.c_int, [vp, P(LouvainParams), vp, P(LouvainStats)]),
     "mkp_louvain_evaluate": (C.c_int, [vp, P(LouvainParams), vp, vp, vp, P(C.c_double)]),
     "mkp_cluster_pass": (C.c_int, [vp, P(CoarsenParams), C.c_int32, vp, P(C.c_uint32), P(C.c_int64)]),
     "mkp_hypergraph_contract": (C.c_int, [vp, vp, P(vp), vp]),
@@ -388,6 +390,20 @@ class Hypergraph:
         stats = dict(arcs_swept=st.arcs_swept, rounds=st.rounds, levels=st.levels, moves=st.moves,
                      device_ms=st.device_ms)
         return (out[:self.info().num_vertices] if out is not None else None), stats
+
+    def louvain_graph(self, edge_weight=-1, max_passes=5, min_move_fraction=0.01, sampling=200000, seed=0,
+                      sub_rounds=8):
+        """Clustering of every node of the Louvain graph (N + E for the star
+        expansion, net e at node N + e), as run_parallel_louvain returns it."""
+        nodes = C.c_int64()
+        check(lib().mkp_louvain_graph_nodes(self.h, C.byref(nodes)))
+        prm = LouvainParams(edge_weight, max_passes, min_move_fraction, sampling, seed, sub_rounds)
+        st = LouvainStats()
+        out = np.zeros(max(nodes.value, 1), np.int32)
+        check(lib().mkp_louvain_graph(self.h, C.byref(prm), _p(out), C.byref(st)))
+        stats = dict(arcs_swept=st.arcs_swept, rounds=st.rounds, levels=st.levels, moves=st.moves,
+                     device_ms=st.device_ms)
+        return out[:nodes.value], stats
 
     def louvain_evaluate(self, node_communities, edge_weight=-1, sampling=200000):
         """Best-cluster decision, FP64 best gain per node of the level-0 graph

@@ -39,7 +39,15 @@ int main(int argc, char** argv) {
 
     // plugin path
     Hypergraph hg(ctx, info.num_vertices, info.num_nets, noff, pins, vw.data(), nw.data());
-    run_parallel_louvain(hg, pc);
+    // preprocess() (partitioner.cpp:111-117): Louvain on the graph, then its
+    // clustering restricted to the hypernodes (stored on the hypergraph)
+    Graph graph(hg);
+    std::vector<int32_t> communities = run_parallel_louvain(graph, pc);
+    if (communities.size() != graph.numNodes()) {
+      std::printf("FAIL clustering of %zu entries for %zu graph nodes\n", communities.size(), graph.numNodes());
+      return 1;
+    }
+    graph.restrictClusteringToHypernodes(hg, communities);
     MultilevelCoarsener coarsener(hg, pc, true);
     coarsener.coarsen();
     initial_partition(coarsener.coarsestPartitionedHypergraph(), pc);

@@ -64,6 +64,42 @@ def test_karate_club_plain_graph_path(ctx):
         assert st["levels"] == ost["levels"] and st["moves"] == ost["moves"]
 
 
+def test_graph_clustering_of_every_node(ctx):
+    """mkp_louvain_graph: the clustering of every node of the Louvain graph as
+    run_parallel_louvain returns it (parallel_louvain.h:27), before the
+    restriction to the hypernodes: N + E nodes of the star expansion with net
+    e at node N + e.  The hypernodes' part equals mkp_louvain's communities,
+    and net e carries the oracle's label of its net node (the oracle numbers
+    net nodes by locality rank; the rank of a net is read from the arcs of
+    its pins, whose incident nets are listed in ascending order)."""
+    n, noff, pins = powerlaw(6000, 8000, 300, 23, heavy_nets=1)
+    m = len(noff) - 1
+    dh = mtk_amd.Hypergraph(ctx, n, noff, pins)
+    full, st = dh.louvain_graph(seed=4)
+    comm, _ = dh.louvain(seed=4)
+    dh.close()
+    assert len(full) == n + m
+    assert np.array_equal(full[:n], comm)
+    g = orc.Graph(orc.Hypergraph(n, noff, pins), 0, ranked=True)
+    C, _ = g.louvain(sync=True, seed=4)
+    arr = g.arrays()
+    net_of_pin = np.repeat(np.arange(m), np.diff(noff.astype(np.int64)))
+    order = np.lexsort((net_of_pin, pins))  # by vertex, then by net
+    heads = arr["head"][:int(arr["idx"][n])].astype(np.int64)
+    rank = np.full(m, -1, np.int64)
+    rank[net_of_pin[order]] = heads - n
+    assert (rank >= 0).all() and sorted(rank.tolist()) == list(range(m))
+    assert np.array_equal(full[n:], C[n + rank])
+    assert np.array_equal(full[:n], C[:n])
+    # a hypergraph whose nets all have two pins is its own Louvain graph
+    kn, knoff, kpins, kvw, knw = read_hgr(os.path.join(GOLDEN, vectors()["karate_club"]["instance"]))
+    kh = mtk_amd.Hypergraph(ctx, kn, knoff, kpins, None, knw)
+    kfull, _ = kh.louvain_graph(seed=2, sub_rounds=4)
+    kcomm, _ = kh.louvain(seed=2, sub_rounds=4)
+    kh.close()
+    assert len(kfull) == kn and np.array_equal(kfull, kcomm)
+
+
 @pytest.mark.parametrize("seed,sub_rounds", [(1, 1), (2, 4), (3, 8), (4, 16)])
 def test_powerlaw_bit_exact(ctx, seed, sub_rounds):
     n, noff, pins = powerlaw(6000, 9000, 400, seed, heavy_nets=3)

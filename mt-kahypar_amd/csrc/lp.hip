@@ -768,12 +768,25 @@ __device__ inline int64_t lp_real_delta(const LpArgs& a, uint32_t u, int from, i
   return real;
 }
 
-__global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const int32_t* __restrict__ cfrom,
-                                const int32_t* __restrict__ cto, const uint8_t* __restrict__ admit,
-                                const uint8_t* __restrict__ keep, uint32_t nc, int mode) {
+// Block weights of a single hypergraph with up to kLpApplyLdsK blocks are
+// summed per workgroup in LDS and added once per (workgroup, block): every
+// move changes two of only k words, and device atomics on one address run
+// one after another.
+constexpr int kLpApplyLdsK = 256;
+
+__global__ void __launch_bounds__(256)
+lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const int32_t* __restrict__ cfrom,
+                const int32_t* __restrict__ cto, const uint8_t* __restrict__ admit,
+                const uint8_t* __restrict__ keep, uint32_t nc, int mode) {
   // roll back only a sub-round that worsened km1 (mode 2 adds nothing to the
   // delta: the sub-round's delta was already set aside by the check)
   if (mode == 2 && !a.vgraph && !a.counters[6]) return;
+  __shared__ unsigned long long s_pw[kLpApplyLdsK];
+  const bool lds_pw = !a.vgraph && a.k <= kLpApplyLdsK;
+  if (lds_pw) {
+    for (int t = threadIdx.x; t < a.k; t += blockDim.x) s_pw[t] = 0ull;
+    __syncthreads();
+  }
   int64_t d = 0;
   const int lane = threadIdx.x & 63;
   // whole waves step together: block weights and member deltas combine
@@ -787,8 +800,15 @@ __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const
     const int from = on ? (mode == 0 ? cfrom[j] : cto[j]) : 0;
     const int to = on ? (mode == 0 ? cto[j] : cfrom[j]) : 0;
     const int64_t wu = on ? int64_t(a.vw[u]) : 0;
-    wave_add_i64(a.part_w, on, uint64_t(mb.kb) + uint64_t(to), wu, 3);
-    wave_add_i64(a.part_w, on, uint64_t(mb.kb) + uint64_t(from), -wu, 3);
+    if (lds_pw) {
+      if (on && wu) {
+        atomicAdd(&s_pw[to], static_cast<unsigned long long>(wu));
+        atomicAdd(&s_pw[from], static_cast<unsigned long long>(-wu));
+      }
+    } else {
+      wave_add_i64(a.part_w, on, uint64_t(mb.kb) + uint64_t(to), wu, 3);
+      wave_add_i64(a.part_w, on, uint64_t(mb.kb) + uint64_t(from), -wu, 3);
+    }
     int64_t du = 0;
     const bool wide = on && a.voff[u + 1] - a.voff[u] > kLpWaveMove;
     if (on) {
@@ -809,6 +829,11 @@ __global__ void lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const
       if (a.vgraph) member_add_wave(a.counters, on, mb.g, 0, du);
       else d += du;
     }
+  }
+  if (lds_pw) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < a.k; t += blockDim.x)
+      if (s_pw[t]) counter_add(a.part_w + t, static_cast<int64_t>(s_pw[t]));
   }
   if (a.vgraph) return;
   d = wave_sum(d);

@@ -587,6 +587,12 @@ lp_gain_wave_kernel(LpArgs a, const uint32_t* __restrict__ active, const uint32_
 }
 
 // ------------------------------------------------------------- compact --
+// Block weights and arrivals of a single hypergraph with up to kLpApplyLdsK
+// blocks are summed per workgroup in LDS and added once per (workgroup,
+// block): every move changes two of only k words, and device atomics on one
+// address run one after another.
+constexpr int kLpApplyLdsK = 256;
+
 __global__ void lp_scatter_candidates_kernel(const uint32_t* __restrict__ active, uint32_t na,
                                              const uint32_t* __restrict__ cand_flag, const uint32_t* __restrict__ pos,
                                              const int32_t* __restrict__ cto_a, const int32_t* __restrict__ cgain_a,
@@ -595,11 +601,15 @@ __global__ void lp_scatter_candidates_kernel(const uint32_t* __restrict__ active
                                              int32_t* __restrict__ cto, int32_t* __restrict__ cgain,
                                              int64_t* __restrict__ in_w, const uint32_t* __restrict__ vgraph,
                                              const uint32_t* __restrict__ gkbase, const uint32_t* __restrict__ voff,
-                                             int64_t* __restrict__ inc_sum) {
+                                             int64_t* __restrict__ inc_sum, int lds_k) {
   // whole waves step together (the incidence count combines across lanes,
-  // then across the block: one atomic per block)
+  // then across the block: one atomic per block).  With lds_k > 0 (a single
+  // hypergraph of lds_k <= kLpApplyLdsK blocks) the arrivals per block are
+  // summed in LDS and added once per (workgroup, block).
   __shared__ unsigned long long binc;
+  __shared__ unsigned long long s_in[kLpApplyLdsK];
   if (threadIdx.x == 0) binc = 0;
+  for (int t = threadIdx.x; t < lds_k; t += blockDim.x) s_in[t] = 0ull;
   __syncthreads();
   for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < na; base += gridDim.x * blockDim.x) {
     const uint32_t i = base + (threadIdx.x & 63u);
@@ -611,8 +621,13 @@ __global__ void lp_scatter_candidates_kernel(const uint32_t* __restrict__ active
       cfrom[j] = part[u];
       cto[j] = cto_a[i];
       cgain[j] = cgain_a[i];
-      const uint32_t kb = vgraph ? gkbase[vgraph[u]] : 0u;  // arrivals per (member, block)
-      atomicAdd(reinterpret_cast<unsigned long long*>(&in_w[kb + cto_a[i]]), static_cast<unsigned long long>(vw[u]));
+      if (lds_k) {
+        atomicAdd(&s_in[cto_a[i]], static_cast<unsigned long long>(vw[u]));
+      } else {
+        const uint32_t kb = vgraph ? gkbase[vgraph[u]] : 0u;  // arrivals per (member, block)
+        atomicAdd(reinterpret_cast<unsigned long long*>(&in_w[kb + cto_a[i]]),
+                  static_cast<unsigned long long>(vw[u]));
+      }
       if (inc_sum) d = int64_t(voff[u + 1] - voff[u]);
     }
     if (inc_sum) {
@@ -620,10 +635,10 @@ __global__ void lp_scatter_candidates_kernel(const uint32_t* __restrict__ active
       if ((threadIdx.x & 63) == 0 && d) atomicAdd(&binc, static_cast<unsigned long long>(d));
     }
   }
-  if (inc_sum) {
-    __syncthreads();
-    if (threadIdx.x == 0 && binc) counter_add(inc_sum, int64_t(binc));
-  }
+  if (inc_sum || lds_k) __syncthreads();
+  if (inc_sum && threadIdx.x == 0 && binc) counter_add(inc_sum, int64_t(binc));
+  for (int t = threadIdx.x; t < lds_k; t += blockDim.x)
+    if (s_in[t]) counter_add(in_w + t, int64_t(s_in[t]));
 }
 
 // admission sort key: target block, gain clamped to 24 bits, 30 hash bits.
@@ -768,11 +783,6 @@ __device__ inline int64_t lp_real_delta(const LpArgs& a, uint32_t u, int from, i
   return real;
 }
 
-// Block weights of a single hypergraph with up to kLpApplyLdsK blocks are
-// summed per workgroup in LDS and added once per (workgroup, block): every
-// move changes two of only k words, and device atomics on one address run
-// one after another.
-constexpr int kLpApplyLdsK = 256;
 
 __global__ void __launch_bounds__(256)
 lp_apply_kernel(LpArgs a, const uint32_t* __restrict__ cu, const int32_t* __restrict__ cfrom,
@@ -1270,7 +1280,7 @@ void lp_refine(mkp_partition* p, const mkp_lp_params& prm, mkp_lp_stats* stats, 
       lp_scatter_candidates_kernel<<<grid_for(ns, 256, prof ? 512 : 65536), 256, 0, s>>>(
           act, ns, w.cand_flag.p, w.pos.p, w.cand_to_a.p, w.cand_gain_a.p, p->part.p, hg->vw.p, w.cand_u.p,
           w.cand_from.p, w.cand_to.p, w.cand_gain.p, w.in_w.p, a.vgraph, a.gkbase, hg->voff.p,
-          prof ? w.counters.p + 11 : nullptr);
+          prof ? w.counters.p + 11 : nullptr, (!a.vgraph && a.k <= kLpApplyLdsK) ? a.k : 0);
       MKP_LAUNCH_CHECK();
       // read back (pinned staging): stop flag of the previous sub-round,
       // number of candidates, arrivals per block, block weights (and, with

@@ -152,7 +152,8 @@ struct ExplicitGraph {
   ExplicitView view() const { return ExplicitView{n, idx.p, head.p, src.p, w.p}; }
 };
 
-constexpr int kLvClassesMax = 5;  // degree classes of local moving (see kLvClasses)
+constexpr int kLvClassesMax = 6;  // degree classes of local moving (see kLvClasses)
+constexpr uint32_t kLvBigArcs = 6144;  // at most 3/4 of the 8192-slot map of a block
 
 struct LvArgs {
   uint64_t seed;
@@ -179,6 +180,10 @@ struct LvArgs {
   const double* grecip = nullptr;
   const uint8_t* gactive = nullptr;
   const uint64_t* gseed = nullptr;
+  // nodes of more than 1024 and at most big_arcs arcs take a block each
+  // (class 4), heavier ones the arena path; big_arcs = 1024: every node above
+  // 1024 arcs takes the arena path
+  uint32_t big_arcs = kLvBigArcs;
 };
 
 __device__ inline double lv_recip(const LvArgs& a, uint32_t u) { return a.nmember ? a.grecip[a.nmember[u]] : a.recip; }
@@ -948,19 +953,22 @@ __global__ void max_degree_kernel(View g, uint32_t n, uint32_t* __restrict__ deg
   block_max_to(deg_max, dm);
 }
 
-// degree classes: 0: <= 16 arcs, 1: <= 64, 2: <= 128, 3: <= 1024, 4: heavier;
-// the work list of a round is grouped by (class, bucket) so that every
-// sub-round launch covers exactly its own nodes.  Classes 1 and 2 run the
-// same group kernel with maps of 128 and 256 slots: the smaller maps of the
-// 17..64-arc nodes leave room in LDS for twice the waves.
-constexpr int kLvClasses = 5;
+// degree classes: 0: <= 16 arcs, 1: <= 64, 2: <= 128, 3: <= 1024, 4: <=
+// kLvBigArcs (a 1024-thread block per node, 8192-slot LDS map), 5: heavier
+// (the arc-parallel arena path); the work list of a round is grouped by
+// (class, bucket) so that every sub-round launch covers exactly its own
+// nodes.  Classes 1 and 2 run the same group kernel with maps of 128 and 256
+// slots: the smaller maps of the 17..64-arc nodes leave room in LDS for twice
+// the waves.
+constexpr int kLvClasses = 6;
+constexpr int kLvHeavy = 5;  // class of the arena path
 static_assert(kLvClasses <= kLvClassesMax, "class ranges of a sub-round");
 // In a batch, nodes of members that sit the round out get no slot (-1), and
 // the bucket hash takes the member's seed and local id.
 template <class View>
 __device__ inline int lv_slot(const View& g, const LvArgs& a, uint32_t u) {
   const uint32_t d = min(g.degree(u), a.sampling);
-  const int c = d <= 16 ? 0 : (d <= 64 ? 1 : (d <= 128 ? 2 : (d <= 1024 ? 3 : 4)));
+  const int c = d <= 16 ? 0 : (d <= 64 ? 1 : (d <= 128 ? 2 : (d <= 1024 ? 3 : (d <= a.big_arcs ? 4 : kLvHeavy))));
   int b = 0;
   if (a.nmember) {
     const uint32_t gm = a.nmember[u];
@@ -1488,7 +1496,8 @@ struct LouvainRun {
                    const std::vector<uint64_t>& sb, int sr, int lanes = 32, hipEvent_t ta = nullptr,
                    hipEvent_t tb = nullptr) {
     const int B = a.buckets;
-    const int s0 = 0 * B + sr, s1 = 1 * B + sr, s1w = 2 * B + sr, s2 = 3 * B + sr, s3 = 4 * B + sr;
+    const int s0 = 0 * B + sr, s1 = 1 * B + sr, s1w = 2 * B + sr, s2 = 3 * B + sr, s2b = 4 * B + sr,
+              s3 = kLvHeavy * B + sr;
     // each launch writes the decisions of its own list range
     auto at = [&](int slot) {
       LvArgs x = a;
@@ -1573,7 +1582,14 @@ struct LouvainRun {
       louvain_move_block_kernel<View, 11, 256><<<xcd_grid(cnt[s2], 1, 65536), 256, 0, s>>>(g, at(s2), list + off[s2],
                                                                                            cnt[s2]);
     }
-    if (cnt[s3]) launch_heavy(g, at(s3), list + off[s3], off[s3] - off[4 * B], cnt[s3], sa[s3], double(sb[s3]), s);
+    if (cnt[s2b]) {  // 1025..kLvBigArcs arcs: a 1024-thread block per node, 8192-slot map
+      TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s2b]),
+                     std::string("louvain_move_block_kernel<") + View::kName + ", 13, 1024>");
+      louvain_move_block_kernel<View, 13, 1024><<<xcd_grid(cnt[s2b], 1, 65536), 1024, 0, s>>>(
+          g, at(s2b), list + off[s2b], cnt[s2b]);
+    }
+    if (cnt[s3])
+      launch_heavy(g, at(s3), list + off[s3], off[s3] - off[kLvHeavy * B], cnt[s3], sa[s3], double(sb[s3]), s);
     MKP_LAUNCH_CHECK();
   }
 
@@ -1614,6 +1630,9 @@ struct LouvainRun {
     }
     LvArgs a;
     a.seed = prm.seed; a.level = level; a.sampling = uint32_t(std::min<uint64_t>(prm.sampling, 0xffffffffu));
+    // a level with nodes above kLvBigArcs arcs sends every node above 1024
+    // arcs to the arena path: one set of launches per sub-round, not two
+    a.big_arcs = std::min<uint32_t>(max_degree_, a.sampling) > kLvBigArcs ? 1024u : kLvBigArcs;
     a.recip = 1.0 / tv; a.C = C; a.cvol = cvol.p; a.csize = csize.p; a.nvol = nvol;
     a.dec = dec.p;
     a.buckets = B;
@@ -1649,15 +1668,15 @@ struct LouvainRun {
       uint64_t harcs = 0;
       uint32_t hnodes = 0;
       for (int sr = 0; sr < B; ++sr) {
-        harcs = std::max(harcs, sa[4 * B + sr]);
-        hnodes += cnt[4 * B + sr];
+        harcs = std::max(harcs, sa[kLvHeavy * B + sr]);
+        hnodes += cnt[kLvHeavy * B + sr];
       }
       if (hnodes) reserve_heavy(harcs, hnodes);
       uint32_t run = 0;
       for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
       lv_scatter_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, bmat.p, counts, list.p);
       MKP_LAUNCH_CHECK();
-      if (hnodes) prepare_heavy(g, a, list.p + off[4 * B], hnodes);
+      if (hnodes) prepare_heavy(g, a, list.p + off[kLvHeavy * B], hnodes);
       const LaneChoice lanes = choose_lanes<View>(cnt, sa, B, r);
       for (int sr = 0; sr < B; ++sr) {
         a.bucket = sr;
@@ -1732,9 +1751,9 @@ struct LouvainRun {
     c.sync();
     uint32_t run = 0;
     for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
-    if (cnt[4 * B]) {
-      reserve_heavy(sa[4 * B], cnt[4 * B]);
-      prepare_heavy(g, a, list.p + off[4 * B], cnt[4 * B]);
+    if (cnt[kLvHeavy * B]) {
+      reserve_heavy(sa[kLvHeavy * B], cnt[kLvHeavy * B]);
+      prepare_heavy(g, a, list.p + off[kLvHeavy * B], cnt[kLvHeavy * B]);
     }
     launch_move(g, a, list.p, off, cnt, sa, sb, 0, c.tune.lv_group_lanes ? c.tune.lv_group_lanes : 8);
     std::vector<uint32_t> hl(n);
@@ -2119,15 +2138,15 @@ struct LouvainBatchRun : LouvainRun {
       uint64_t harcs = 0;
       uint32_t hnodes = 0;
       for (int sr = 0; sr < B; ++sr) {
-        harcs = std::max(harcs, sa[4 * B + sr]);
-        hnodes += cnt[4 * B + sr];
+        harcs = std::max(harcs, sa[kLvHeavy * B + sr]);
+        hnodes += cnt[kLvHeavy * B + sr];
       }
       if (hnodes) reserve_heavy(harcs, hnodes);
       uint32_t run = 0;
       for (int i = 0; i < slots; ++i) { off[i] = run; run += cnt[i]; }
       lv_scatter_kernel<View><<<lblocks, 256, 0, s>>>(g, a, n, bmat.p, counts, list.p);
       MKP_LAUNCH_CHECK();
-      if (hnodes) prepare_heavy(g, a, list.p + off[4 * B], hnodes);
+      if (hnodes) prepare_heavy(g, a, list.p + off[kLvHeavy * B], hnodes);
       const LaneChoice lanes = choose_lanes<View>(cnt, sa, B, r);
       for (int sr = 0; sr < B; ++sr) {
         a.bucket = sr;

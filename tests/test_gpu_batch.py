@@ -90,14 +90,16 @@ def test_batch_member_matches_oracle(ctx):
         h.close()
 
 
-def arrays_member(n, m, max_size, seed, heavy_nets=0, exponent=2.3):
+def arrays_member(n, m, max_size, seed, heavy_nets=0, exponent=2.3, heavy_size=1300):
     h = mtk_amd.HostHypergraph.powerlaw(n, m, exponent, 2, max_size, window=256, local_fraction=0.8, seed=seed)
     a = h.arrays()
     h.close()
     nets = [a["pins"][a["noff"][e]:a["noff"][e + 1]] for e in range(len(a["noff"]) - 1)]
     rng = np.random.default_rng(seed)
-    for _ in range(heavy_nets):  # a net node of degree > 1024: the heavy-node kernels
-        nets.append(np.sort(rng.choice(n, size=1300, replace=False)).astype(np.uint32))
+    # a net node of degree > 1024 (a 1024-thread block each), > 6144 the
+    # arc-parallel arena path
+    for _ in range(heavy_nets):
+        nets.append(np.sort(rng.choice(n, size=heavy_size, replace=False)).astype(np.uint32))
     noff = np.zeros(len(nets) + 1, np.uint64)
     noff[1:] = np.cumsum([len(x) for x in nets])
     return mtk_amd.HostHypergraph.from_arrays(n, noff, np.concatenate(nets))
@@ -138,15 +140,20 @@ def test_batch_louvain_equals_members_alone(ctx, sub_rounds, count):
 
 
 def test_batch_louvain_member_matches_oracle(ctx):
-    hosts = [arrays_member(3000, 3600, 300, 31, heavy_nets=1), arrays_member(1000, 1200, 40, 32)]
+    hosts = [arrays_member(3000, 3600, 300, 31, heavy_nets=1), arrays_member(1000, 1200, 40, 32),
+             arrays_member(9000, 10000, 300, 33, heavy_nets=2, heavy_size=7000)]
     b = mtk_amd.Batch(ctx, hosts)
-    comm, stats = b.louvain(seeds=[9, 10])
-    a = hosts[0].arrays()
-    g = orc.Graph(orc.Hypergraph(3000, a["noff"], a["pins"]), 0, ranked=True)
-    C, ost = g.louvain(sync=True, seed=9, sub_rounds=8)
-    assert comm[:3000].tolist() == C[:3000].tolist()
-    assert (stats[0]["arcs_swept"], stats[0]["rounds"], stats[0]["levels"], stats[0]["moves"]) == \
-        (ost["arcs"], ost["rounds"], ost["levels"], ost["moves"])
+    comm, stats = b.louvain(seeds=[9, 10, 11])
+    off = 0
+    for i, (h, seed) in enumerate(zip(hosts, [9, 10, 11])):
+        a = h.arrays()
+        n = a["n"]
+        g = orc.Graph(orc.Hypergraph(n, a["noff"], a["pins"]), 0, ranked=True)
+        C, ost = g.louvain(sync=True, seed=seed, sub_rounds=8)
+        assert comm[off:off + n].tolist() == C[:n].tolist(), i
+        assert (stats[i]["arcs_swept"], stats[i]["rounds"], stats[i]["levels"], stats[i]["moves"]) == \
+            (ost["arcs"], ost["rounds"], ost["levels"], ost["moves"]), i
+        off += n
     b.close()
     for h in hosts:
         h.close()

@@ -20,14 +20,16 @@ def ctx():
     c.close()
 
 
-def powerlaw(n, m, max_size, seed, window=512, exponent=2.3, heavy_nets=0):
+def powerlaw(n, m, max_size, seed, window=512, exponent=2.3, heavy_nets=0, huge_nets=0):
     h = mtk_amd.HostHypergraph.powerlaw(n, m, exponent, 2, max_size, window=window, local_fraction=0.8, seed=seed)
     a = h.arrays()
     h.close()
     nets = [a["pins"][a["noff"][e]:a["noff"][e + 1]].tolist() for e in range(len(a["noff"]) - 1)]
     rng = np.random.default_rng(seed)
-    for _ in range(heavy_nets):  # net nodes of degree > 1024 take the global-map kernel
+    for _ in range(heavy_nets):  # net nodes of degree > 1024: a 1024-thread block each
         nets.append(sorted(rng.choice(n, size=1500, replace=False).tolist()))
+    for _ in range(huge_nets):  # degree > 6144: the arc-parallel arena path
+        nets.append(sorted(rng.choice(n, size=7000, replace=False).tolist()))
     noff, pins = csr(nets)
     return n, noff, pins
 
@@ -110,6 +112,19 @@ def test_powerlaw_bit_exact(ctx, seed, sub_rounds):
     assert st["levels"] >= 2
 
 
+@pytest.mark.parametrize("seed,sub_rounds", [(5, 1), (6, 8)])
+def test_heavy_classes_bit_exact(ctx, seed, sub_rounds):
+    """Net nodes of 1,500 and of 7,000 pins beside the small classes: a level
+    with a node above 6,144 arcs sends every node above 1,024 arcs to the
+    arc-parallel arena path (without one, as in test_powerlaw_bit_exact, they
+    take a 1024-thread block and an 8192-slot LDS map each)."""
+    n, noff, pins = powerlaw(12000, 15000, 600, seed, heavy_nets=3, huge_nets=2)
+    comm, ref, st, ost = compare(ctx, n, noff, pins, seed=seed, sub_rounds=sub_rounds)
+    assert comm.tolist() == ref.tolist()
+    assert (st["arcs_swept"], st["rounds"], st["levels"], st["moves"]) == \
+        (ost["arcs"], ost["rounds"], ost["levels"], ost["moves"])
+
+
 def test_sampling_threshold(ctx):
     n, noff, pins = powerlaw(3000, 4000, 300, 7, heavy_nets=2)
     comm, ref, st, ost = compare(ctx, n, noff, pins, sampling=12)
@@ -178,7 +193,7 @@ def test_fused_class_launch_changes_only_the_speed(ctx):
     and equal to the oracle's.  The hypergraph has nodes in every class,
     129..1024-arc net nodes (a block each) and net nodes above 1024 arcs (the
     arc-parallel path) included."""
-    n, noff, pins = powerlaw(20000, 30000, 900, 19, heavy_nets=2)
+    n, noff, pins = powerlaw(20000, 30000, 900, 19, heavy_nets=2, huge_nets=1)
     c2 = mtk_amd.Context(0)
     try:
         out = []

@@ -66,6 +66,13 @@ struct StarView {  // star expansion of a hypergraph (graph.cpp:331-398)
     if (u < N) return 8u + (wt == 0 ? (unit ? 0u : 8u) : 16u);
     return 8u + (wt == 0 ? 0u : 8u);
   }
+  // unit arc weights (uniform weighting, all net weights one): the move
+  // kernels then need the heads only
+  static constexpr bool kHasUnit = true;
+  __host__ __device__ bool unit_arcs() const { return wt == 0 && unit; }
+  __device__ uint32_t head_of(uint32_t u, uint32_t i) const {
+    return u < N ? N + inc_rank[voff[u] + i] : rpins[rpoff[u - N] + i];
+  }
   __device__ double weight(int32_t w, uint32_t size, uint32_t deg) const {
     if (wt == 0) return double(w);
     if (wt == 1) return double(w) / double(size);
@@ -112,6 +119,13 @@ struct PlainView {  // all nets have two pins (graph.cpp:401-446)
   __device__ uint32_t source(uint64_t a) const { return inc_vtx[a]; }
   // net id, net offset, both pins, cluster gather (+ net weight)
   __device__ uint32_t arc_bytes(uint32_t) const { return 20u + (unit ? 0u : 4u); }
+  static constexpr bool kHasUnit = true;
+  __host__ __device__ bool unit_arcs() const { return wt == 0 && unit; }
+  __device__ uint32_t head_of(uint32_t u, uint32_t i) const {
+    const uint32_t e = inc[voff[u] + i];
+    const uint32_t p0 = pins[noff[e]], p1 = pins[noff[e] + 1];
+    return p0 != u ? p0 : p1;
+  }
   __device__ void arc(uint32_t u, uint32_t i, uint32_t& head, double& w) const {
     const uint32_t e = inc[voff[u] + i];
     const uint32_t p0 = pins[noff[e]], p1 = pins[noff[e] + 1];
@@ -135,6 +149,9 @@ struct ExplicitView {  // contracted levels
   __device__ uint32_t source(uint64_t a) const { return src[a]; }
   // head id, f64 weight, cluster gather
   __device__ uint32_t arc_bytes(uint32_t) const { return 16u; }
+  static constexpr bool kHasUnit = false;
+  __host__ __device__ bool unit_arcs() const { return false; }
+  __device__ uint32_t head_of(uint32_t u, uint32_t i) const { return head[idx[u] + i]; }
   __device__ void arc(uint32_t u, uint32_t i, uint32_t& h, double& ww) const {
     const uint64_t a = idx[u] + i;
     h = head[a];
@@ -279,7 +296,7 @@ __device__ inline void lv_decide(const LvArgs& a, uint32_t pos, const LvNode& nd
 // a node waits for three memory round trips, not one per arc.
 // Every move body runs as block `bid` of `nb` blocks (a multiple of 8): on its
 // own in a kernel of its class, or as one segment of a fused launch.
-template <class View>
+template <class View, bool UNIT>
 __device__ __forceinline__ void lv_small_body(const View& g, const LvArgs& a, const uint32_t* __restrict__ list,
                                               uint32_t count, uint32_t bid, uint32_t nb, uint32_t dbase) {
   constexpr int D = 16;
@@ -291,12 +308,18 @@ __device__ __forceinline__ void lv_small_body(const View& g, const LvArgs& a, co
     const uint32_t u = list[idx];
     const uint32_t deg = min(g.degree(u), a.sampling);
     uint32_t head[D];
-    double w[D];
+    // with unit arc weights every weight is 1.0 and none is held (the
+    // registers of the weights would cost a wave per SIMD of occupancy)
+    double w[UNIT ? 1 : D];
 #pragma unroll
     for (int i = 0; i < D; ++i) {
       head[i] = u;  // a valid index for the unconditional gathers below
-      w[i] = 0.0;
-      if (uint32_t(i) < deg) g.arc(u, uint32_t(i), head[i], w[i]);
+      if constexpr (UNIT) {
+        if (uint32_t(i) < deg) head[i] = g.head_of(u, uint32_t(i));
+      } else {
+        w[i] = 0.0;
+        if (uint32_t(i) < deg) g.arc(u, uint32_t(i), head[i], w[i]);
+      }
     }
     LvNode nd;
     nd.from = a.C[u];
@@ -324,10 +347,10 @@ __device__ __forceinline__ void lv_small_body(const View& g, const LvArgs& a, co
 #pragma unroll
       for (int j = 0; j < i; ++j) first = first && key[j] != key[i];
       if (first) {
-        double sum = w[i];
+        double sum = UNIT ? 1.0 : w[UNIT ? 0 : i];
 #pragma unroll
         for (int j = i + 1; j < D; ++j)
-          if (key[j] == key[i]) sum += w[j];
+          if (key[j] == key[i]) sum += UNIT ? 1.0 : w[UNIT ? 0 : j];
         if (key[i] == from) {
           wfrom = sum;
         } else {
@@ -340,10 +363,10 @@ __device__ __forceinline__ void lv_small_body(const View& g, const LvArgs& a, co
   }
 }
 
-template <class View>
+template <class View, bool UNIT>
 __global__ void __launch_bounds__(256)
 louvain_move_small_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, uint32_t count) {
-  lv_small_body(g, a, list, count, blockIdx.x, gridDim.x, 0);
+  lv_small_body<View, UNIT>(g, a, list, count, blockIdx.x, gridDim.x, 0);
 }
 
 // G lanes per node, T-slot open-addressing map per group in LDS (16 bytes a
@@ -562,7 +585,7 @@ constexpr uint64_t kLvFuse1Arcs = 1ull << 21;
 static_assert(lv_block_lds<11>() <= kLvFusedLds && lv_group_lds<32, 8, 256>() <= kLvFusedLds &&
                   lv_group_lds<16, 7, 256>() <= kLvFusedLds, "fused move maps fit");
 
-template <class View, int G1>
+template <class View, int G1, bool UNIT>
 __global__ void __launch_bounds__(256)
 louvain_move_fused_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, LvFused f) {
   __shared__ __align__(16) unsigned char lds[kLvFusedLds];
@@ -584,7 +607,7 @@ louvain_move_fused_kernel(View g, LvArgs a, const uint32_t* __restrict__ list, L
     }
     b -= f.blocks[2];
   }
-  lv_small_body<View>(g, a, list + f.off[3], f.cnt[3], b, f.blocks[3], f.off[3]);
+  lv_small_body<View, UNIT>(g, a, list + f.off[3], f.cnt[3], b, f.blocks[3], f.off[3]);
 }
 
 // ------------------------------------------------------- heavy nodes --
@@ -1490,6 +1513,19 @@ struct LouvainRun {
     MKP_LAUNCH_CHECK();
   }
 
+  template <class View, bool UNIT>
+  static void launch_fused(const View& g, const LvArgs& a, const uint32_t* list, const LvFused& f, unsigned blocks,
+                           int g1, hipStream_t s) {
+    if (g1 == 16)
+      louvain_move_fused_kernel<View, 16, UNIT><<<blocks, 256, 0, s>>>(g, a, list, f);
+    else if (g1 == 32)
+      louvain_move_fused_kernel<View, 32, UNIT><<<blocks, 256, 0, s>>>(g, a, list, f);
+    else if (g1 == 64)
+      louvain_move_fused_kernel<View, 64, UNIT><<<blocks, 256, 0, s>>>(g, a, list, f);
+    else
+      louvain_move_fused_kernel<View, 0, UNIT><<<blocks, 256, 0, s>>>(g, a, list, f);
+  }
+
   template <class View>
   void launch_move(const View& g, const LvArgs& a, const uint32_t* list, const std::vector<uint32_t>& off,
                    const std::vector<uint32_t>& cnt, const std::vector<uint64_t>& sa,
@@ -1535,22 +1571,26 @@ struct LouvainRun {
         bytes += double(sb[slot[k]]);
       }
       const int g1 = (cnt[s1] && !own1) ? G : 0;
+      const bool unit = g.unit_arcs();
       TimedRegion tr(c.prof, s, kFamLouvainMove, bytes,
-                     std::string("louvain_move_fused_kernel<") + View::kName + ", " + std::to_string(g1) + ">");
-      if (g1 == 16)
-        louvain_move_fused_kernel<View, 16><<<blocks, 256, 0, s>>>(g, a, list, f);
-      else if (g1 == 32)
-        louvain_move_fused_kernel<View, 32><<<blocks, 256, 0, s>>>(g, a, list, f);
-      else if (g1 == 64)
-        louvain_move_fused_kernel<View, 64><<<blocks, 256, 0, s>>>(g, a, list, f);
+                     std::string("louvain_move_fused_kernel<") + View::kName + ", " + std::to_string(g1) +
+                         (unit ? ", true>" : ", false>"));
+      if (unit)
+        launch_fused<View, View::kHasUnit>(g, a, list, f, blocks, g1, s);
       else
-        louvain_move_fused_kernel<View, 0><<<blocks, 256, 0, s>>>(g, a, list, f);
+        launch_fused<View, false>(g, a, list, f, blocks, g1, s);
       MKP_LAUNCH_CHECK();
     }
     if (!fused && cnt[s0]) {
+      const bool unit = g.unit_arcs();
       TimedRegion tr(c.prof, s, kFamLouvainMove, double(sb[s0]),
-                     std::string("louvain_move_small_kernel<") + View::kName + ">");
-      louvain_move_small_kernel<View><<<xcd_grid(cnt[s0], 256, 65536), 256, 0, s>>>(g, at(s0), list + off[s0], cnt[s0]);
+                     std::string("louvain_move_small_kernel<") + View::kName + (unit ? ", true>" : ", false>"));
+      if (unit)
+        louvain_move_small_kernel<View, View::kHasUnit><<<xcd_grid(cnt[s0], 256, 65536), 256, 0, s>>>(
+            g, at(s0), list + off[s0], cnt[s0]);
+      else
+        louvain_move_small_kernel<View, false><<<xcd_grid(cnt[s0], 256, 65536), 256, 0, s>>>(
+            g, at(s0), list + off[s0], cnt[s0]);
     }
     if (own1) {  // 17..64 arcs: 128-slot maps, lanes per node autotuned
       if (ta) MKP_HIP(hipEventRecord(ta, s));

@@ -119,9 +119,24 @@ co_mark_nets_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t*
   const uint32_t cnt = *count;
   const uint32_t lane = threadIdx.x % G;
   const uint32_t groups = gridDim.x * (blockDim.x / G);
-  // the loops run in whole waves (trip counts are wave-uniform), so the
-  // appends of a wave go to mlist through one atomic
+  // the loops run in whole waves (trip counts are wave-uniform); a wave
+  // gathers its appends in LDS and moves them to mlist with one atomic per
+  // kMarkStage entries (an atomic on one address per step of every wave
+  // would queue in memory)
+  constexpr uint32_t kMarkStage = 256;
+  __shared__ uint32_t stage[4][kMarkStage];
   const uint32_t wlane = threadIdx.x & 63;
+  uint32_t* st = stage[threadIdx.x >> 6];
+  uint32_t fill = 0;  // wave-uniform
+  auto flush = [&]() {
+    lds_wave_sync();
+    uint32_t base = 0;
+    if (wlane == 0) base = atomicAdd(mcount, fill);
+    base = __shfl(base, 0, 64);
+    for (uint32_t t = wlane; t < fill; t += 64) mlist[base + t] = st[t];
+    lds_wave_sync();
+    fill = 0;
+  };
   const uint32_t first = blockIdx.x * (blockDim.x / G) + threadIdx.x / G;
   const uint32_t wfirst = first - (wlane / G);  // group of the wave's lane 0
   for (uint32_t i0 = wfirst; i0 < cnt; i0 += groups) {
@@ -140,12 +155,13 @@ co_mark_nets_kernel(CoArgs a, const uint32_t* __restrict__ list, const uint32_t*
       }
       const uint64_t bal = __ballot(add);
       if (!bal) continue;
-      uint32_t base = 0;
-      if (wlane == 0) base = atomicAdd(mcount, uint32_t(__popcll(bal)));
-      base = __shfl(base, 0, 64);
-      if (add) mlist[base + __popcll(bal & ((1ull << wlane) - 1))] = e;
+      const uint32_t nadd = uint32_t(__popcll(bal));
+      if (fill + nadd > kMarkStage) flush();
+      if (add) st[fill + __popcll(bal & ((1ull << wlane) - 1))] = e;
+      fill += nadd;
     }
   }
+  if (fill) flush();
 }
 
 // distinct clusters of the marked nets with at most 16 pins: one net per

@@ -1403,7 +1403,7 @@ ClusterResult cluster_pass(mkp_hypergraph* hg, const mkp_coarsen_params& prm, in
       {
         TimedRegion tr(c.prof, s, kFamRating, 0.0, "co_mark_nets_kernel");
         co_mark_nets_kernel<8><<<mg, 256, 0, s>>>(a, list_c.p, counts.p + 6, stamp, nmark.p, mlist.p, counts.p + 7);
-        co_mark_nets_kernel<64><<<mg, 256, 0, s>>>(a, list_a.p, counts.p + 0, stamp, nmark.p, mlist.p, counts.p + 7);
+        co_mark_nets_kernel<16><<<mg, 256, 0, s>>>(a, list_a.p, counts.p + 0, stamp, nmark.p, mlist.p, counts.p + 7);
         co_mark_nets_kernel<64><<<mg, 256, 0, s>>>(a, list_b.p, counts.p + 1, stamp, nmark.p, mlist.p, counts.p + 7);
       }
       {

@@ -1508,7 +1508,7 @@ struct LouvainRun {
       rg.total += rg.cnt[cl];
     }
     if (!rg.total) return;
-    louvain_apply_kernel<<<grid_for(rg.total, 256, 256), 256, 0, c.stream>>>(list, dec, rg, nvol, C, cvol, csize,
+    louvain_apply_kernel<<<grid_for(rg.total, 256, 1024), 256, 0, c.stream>>>(list, dec, rg, nvol, C, cvol, csize,
                                                                             moved, nmember, gmoved);
     MKP_LAUNCH_CHECK();
   }
